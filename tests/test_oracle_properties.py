@@ -1,0 +1,201 @@
+"""Pin the CPU restatement where the reference's KATs leave it open.
+
+Checks use independent formulations (explicit superset sums, explicit bucket
+sweeps, numpy reductions, closed forms) so that a plausible slip in a restated
+loop -- a dropped term, a wrong sign, a wrong index -- fails one of them. The
+last tests plant such slips in Python copies of the restated formulas and show
+the checks catch them.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+from bayesiangrouptesting_amd import workload
+from oracle import pyoracle as orc
+
+
+def random_posterior(seed, n, k, dilution=False, steps=3):
+    pi = workload.priors(seed, n * k, 0.02, 0.4)
+    post = orc.prior(pi)
+    dil = orc.dilution_table(n, 0.05, 0.9) if dilution else None
+    g = workload.SplitMix64(seed + 99)
+    for _ in range(steps):
+        e = g.below(1 << n) | 1
+        r = g.below(1 << k)
+        orc.update_in_place(post, dilution, n, k, e, r, dil)
+    return post
+
+
+def superset_sums(x, atoms):
+    out = np.zeros_like(x)
+    for e in range(1 << atoms):
+        out[e] = sum(x[s] for s in range(1 << atoms) if s & e == e)
+    return out
+
+
+@pytest.mark.parametrize("n", [1, 3, 6, 8])
+def test_zeta_equals_explicit_superset_sums(n):
+    x = random_posterior(n, n, 1)
+    np.testing.assert_allclose(orc.zeta(x, n), superset_sums(x, n), rtol=1e-13, atol=0)
+
+
+def test_prior_closed_form_and_normalisation():
+    pi = workload.priors(5, 10, 0.005, 0.3)
+    post = orc.prior(pi)
+    assert abs(post.sum() - 1.0) < 1e-13
+    s = 0b1011001110
+    want = np.prod([(1 - p) if s >> a & 1 else p for a, p in enumerate(pi)])
+    assert abs(post[s] - want) <= 1e-15 * want
+
+
+@pytest.mark.parametrize("dilution", [False, True])
+@pytest.mark.parametrize("n,k", [(5, 1), (3, 2), (2, 3)])
+def test_update_denominator_and_normalisation(dilution, n, k):
+    post = random_posterior(11, n, k, dilution, steps=1)
+    dil = orc.dilution_table(n, 0.05, 0.9) if dilution else None
+    e, r = (1 << n) - 2 | 1, (1 << k) - 1
+    lik = np.array([orc.response_prob(dilution, n, k, e, r, s, dil) for s in range(1 << (n * k))])
+    x = post.copy()
+    den = orc.update_in_place(x, dilution, n, k, e, r, dil)
+    assert abs(den - float(np.sum(post * lik))) < 1e-14
+    assert abs(x.sum() - 1.0) < 1e-13
+    np.testing.assert_allclose(x, post * lik / den, rtol=1e-14)
+
+
+def test_dilution_likelihood_closed_form():
+    n, alpha, h = 6, 0.05, 0.9
+    dil = orc.dilution_table(n, alpha, h)
+    for s_pool in range(1, n + 1):
+        for m in range(0, s_pool + 1):
+            want = alpha if m == 0 else 1 - alpha * m / ((s_pool - m) * h + m)
+            assert abs(dil[(s_pool - 1) * (n + 1) + m] - want) < 1e-15
+    # response probability picks row |E|-1 and column (|E| - |E & S|)
+    e, s = 0b101101, 0b100111
+    p1 = orc.response_prob(True, n, 1, e, 1, s, dil)
+    size, missing = 4, 4 - bin(e & s).count("1")
+    assert p1 == dil[(size - 1) * (n + 1) + missing]
+
+
+def test_non_dilution_likelihood_multivariant():
+    n, k = 3, 2
+    for e, r, s in itertools.product(range(8), range(4), range(64)):
+        want = 1.0
+        for v in range(k):
+            vs = (s >> (v * n)) & 7
+            inside = (e & vs) == e
+            rb = r >> v & 1
+            lo = 1.0 - 0.99  # replicated_lattice.cpp:12 uses (1.0 - negative_response)
+            want *= (0.99 if rb else lo) if inside else (lo if rb else 0.99)
+        assert orc.response_prob(False, n, k, e, r, s) == want
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_op_bha_matches_explicit_argmin(seed):
+    n = 7
+    x = random_posterior(seed, n, 1)
+    e, upset = orc.op_bha(x, n)
+    m = superset_sums(x, n)
+    np.testing.assert_allclose(upset, m, rtol=1e-13)
+    dist = np.abs(m - 0.5)
+    order = np.argsort(dist, kind="stable")
+    if dist[order[1]] - dist[order[0]] > 1e-12:
+        assert e == order[0]
+
+
+@pytest.mark.parametrize("n,k", [(6, 1), (4, 2), (3, 3), (2, 4)])
+@pytest.mark.parametrize("seed", range(4))
+def test_bbpa_matches_brute_force(n, k, seed):
+    x = random_posterior(seed + 10 * k, n, k, dilution=bool(seed % 2))
+    e1, s1 = orc.bbpa(x, n, k)
+    e2, s2 = orc.brute_force(x, n, k)
+    assert abs(s1 - s2) < 1e-12
+    # score of the brute-force pick recomputed independently
+    t = 1.0 / (1 << k)
+    mass = np.zeros(1 << k)
+    for s in range(1 << (n * k)):
+        rid = sum(((e2 & ~((s >> (v * n)) & ((1 << n) - 1))) != 0) << v for v in range(k))
+        mass[rid] += x[s]
+    assert abs(np.abs(mass - t).sum() - s2) < 1e-12
+    if e1 != e2:
+        pytest.fail(f"BBPA picked {e1}, brute force {e2} (scores {s1} vs {s2})")
+
+
+@pytest.mark.parametrize("n,k", [(7, 1), (3, 2), (2, 3)])
+def test_atom_masses_and_upset_mass(n, k):
+    a = n * k
+    x = random_posterior(3, n, k)
+    m = orc.atom_masses(x, a)
+    for atom in range(a):
+        want = x[[s for s in range(1 << a) if s >> atom & 1]].sum()
+        assert abs(m[atom] - want) < 1e-14
+        assert abs(orc.upset_mass(x, a, 1 << atom) - want) < 1e-14
+    z = orc.zeta(x, a)
+    for e in (0, 3, 5, (1 << a) - 1):
+        assert abs(orc.upset_mass(x, a, e) - z[e]) < 1e-14
+
+
+def test_classification_thresholds_and_stickiness():
+    masses = np.array([0.005, 0.5, 0.995, 0.01, 0.99])
+    pos, neg = orc.classify(masses, 0.01, 0.01)
+    assert pos == 0b00001 and neg == 0b00100
+    # sticky: an already classified atom is skipped
+    pos2, neg2 = orc.classify(np.array([0.5, 0.5, 0.5, 0.5, 0.5]), 0.01, 0.01, pos, neg)
+    assert (pos2, neg2) == (pos, neg)
+
+
+def test_tree_invariants_random_configs():
+    # correct + incorrect + unclassified covers all mass when nothing is pruned;
+    # E[tests] never exceeds the depth
+    for seed, (n, k, dil) in enumerate([(4, 1, False), (4, 1, True), (2, 2, False), (5, 1, True)]):
+        pi = workload.priors(seed, n * k, 0.02, 0.2)
+        st = orc.run_simulation(dil, n, k, pi, 3, 0.05, 0.05, 0.0, alpha=0.05, h=0.9)
+        total = st.correct_probability + st.incorrect_probability + st.unclassified_probability
+        assert abs(total - 1.0) < 1e-12
+        assert st.expected_tests <= 3 + 1e-12
+        ns = orc.run_simulation_noshrink(dil, n, k, pi, 3, 0.05, 0.05, 0.0, alpha=0.05, h=0.9)
+        assert ns.total_leaves == st.total_leaves
+        for f in ("correct_probability", "incorrect_probability", "unclassified_probability",
+                  "false_positive_probability", "false_negative_probability", "expected_tests"):
+            assert abs(getattr(st, f) - getattr(ns, f)) < 1e-12, f
+
+
+# ---- planted mistakes: the checks above must reject plausible slips -------------------------
+
+def _k2_scores(fm0, fm1, fmb, total, t, sign=+1):
+    r0 = fmb
+    r1 = fm1 - fmb
+    r2 = fm0 - fmb
+    r3 = total - fm0 - fm1 + sign * fmb
+    return np.abs(r0 - t) + np.abs(r1 - t) + np.abs(r2 - t) + np.abs(r3 - t)
+
+
+def test_planted_sign_slip_in_k2_scoring_is_caught():
+    n, k = 3, 2
+    x = random_posterior(4, n, k)
+    mask = (1 << n) - 1
+    tabs = np.zeros((3, 1 << n))
+    for s in range(1 << (n * k)):
+        m0, m1 = s & mask, (s >> n) & mask
+        tabs[0, m0] += x[s]
+        tabs[1, m1] += x[s]
+        tabs[2, m0 & m1] += x[s]
+    z = np.array([orc.zeta(t, n) for t in tabs])
+    good = _k2_scores(z[0], z[1], z[2], x.sum(), 0.25)
+    bad = _k2_scores(z[0], z[1], z[2], x.sum(), 0.25, sign=-1)
+    _, s_bf = orc.brute_force(x, n, k)
+    assert abs(good.min() - s_bf) < 1e-12
+    assert abs(bad.min() - s_bf) > 1e-6
+
+
+def test_planted_wrong_bit_order_in_zeta_is_caught():
+    # a zeta that adds subsets instead of supersets (a[e|b] += a[e]) gives different masses
+    n = 5
+    x = random_posterior(8, n, 1)
+    wrong = x.copy()
+    for b in range(n):
+        for e in range(1 << n):
+            if not e >> b & 1:
+                wrong[e | (1 << b)] += wrong[e]
+    assert np.max(np.abs(wrong - superset_sums(x, n))) > 1e-3
+    np.testing.assert_allclose(orc.zeta(x, n), superset_sums(x, n), rtol=1e-13)
