@@ -1,0 +1,37 @@
+"""MI355X (gfx950) HIP provider for the BayesianGroupTesting lattice hot path.
+
+The numerical work runs in hand-written HIP kernels behind the C ABI in
+include/bgt_hip.h (libbgt_hip.so, built in-tree by bayesiangrouptesting_amd.build).
+This package mirrors the reference's public API for that path.
+"""
+from .api import (  # noqa: F401
+    DilutionParameters,
+    DilutionMode,
+    DilutionTable,
+    Error,
+    InvalidArgument,
+    KernelSpec,
+    Lattice,
+    LatticeType,
+    Precision,
+    Provider,
+    Runtime,
+    Selection,
+    SelectorType,
+    SimulationConfig,
+    SimulationOptions,
+    SimulationResult,
+    TreeStats,
+    Unsupported,
+    available_providers,
+    hip_available,
+    reverse_bits,
+    run_simulation,
+)
+
+__all__ = [
+    "DilutionParameters", "DilutionMode", "DilutionTable", "Error", "InvalidArgument", "KernelSpec", "Lattice",
+    "LatticeType", "Precision", "Provider", "Runtime", "Selection", "SelectorType", "SimulationConfig",
+    "SimulationOptions", "SimulationResult", "TreeStats", "Unsupported", "available_providers", "hip_available",
+    "reverse_bits", "run_simulation",
+]

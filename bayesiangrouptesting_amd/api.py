@@ -1,0 +1,445 @@
+"""Host-side mirror of the reference's public API for the hot path, over the HIP provider.
+
+Names, argument meanings and error behaviour follow include/bgt/lattice.hpp,
+include/bgt/types.hpp, include/bgt/simulation.hpp and include/bgt/runtime.hpp. Every
+numerical operation runs on the GPU through the C ABI (include/bgt_hip.h); this module
+only validates arguments, applies the public bit flip of binary lattices
+(src/core/lattice_api.cpp:17-26) and converts types.
+"""
+from __future__ import annotations
+
+import ctypes
+import enum
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _native as nat
+from ._native import Error, InvalidArgument, Unsupported  # noqa: F401
+
+
+class LatticeType(enum.IntEnum):
+    # include/bgt/types.hpp:81-87
+    distributed_non_dilution = 0
+    distributed_dilution = 1
+    replicated_non_dilution = 2
+    replicated_dilution = 3
+
+
+class SelectorType(enum.IntEnum):
+    # include/bgt/types.hpp:89-94, plus brute_force (advertised in README.md:87-89)
+    auto_select = 0
+    op_bha = 1
+    op_bbpa = 2
+    brute_force = 3
+
+
+class Provider(enum.IntEnum):
+    # include/bgt/types.hpp:96-101, plus the HIP provider
+    auto_select = 0
+    cpu = 1
+    cuda = 2
+    hip = 3
+
+
+class Precision(enum.IntEnum):
+    # include/bgt/compile_options.hpp:19-23 plus the compensated FP32 storage form
+    float32 = 0
+    float64 = 1
+    float32_compensated = 2
+
+
+class DilutionMode(enum.IntEnum):
+    non_dilution = 0
+    dilution = 1
+
+
+def _storage(precision: Precision) -> int:
+    if precision == Precision.float64:
+        return nat.FP64
+    if precision in (Precision.float32_compensated, Precision.float32):
+        return nat.FP32C
+    raise InvalidArgument(nat.BAD_INPUT, "unknown precision")
+
+
+def reverse_bits(value: int, width: int) -> int:
+    """lattice_api.cpp:17-26"""
+    ret = 0
+    for i in range(width):
+        ret = (ret << 1) | ((value >> i) & 1)
+    return ret
+
+
+def _uses_dilution(t: LatticeType) -> bool:
+    return t in (LatticeType.replicated_dilution, LatticeType.distributed_dilution)
+
+
+class DilutionTable:
+    """include/bgt/lattice.hpp:17-32; entries as dilution.cpp:44-67 (row s-1: pool size s)."""
+
+    def __init__(self, subjects: int = 0, alpha: float = 0.99, h: float = 0.005):
+        if subjects <= 0:
+            raise InvalidArgument(nat.BAD_INPUT, "subjects must be positive.")
+        self._subjects = subjects
+        vals = np.zeros(subjects * (subjects + 1), dtype=np.float64)
+        for rk in range(1, subjects + 1):
+            base = (rk - 1) * (subjects + 1)
+            vals[base] = alpha
+            for r in range(1, rk + 1):
+                k = rk - r
+                vals[base + r] = 1 - alpha * r / (k * h + r)
+        self.values = vals
+
+    def subjects(self) -> int:
+        return self._subjects
+
+    def at(self, group_size: int, positives: int) -> float:
+        if group_size <= 0 or group_size > self._subjects or positives < 0 or positives > group_size:
+            raise IndexError("dilution table index is out of range.")
+        return float(self.values[(group_size - 1) * (self._subjects + 1) + positives])
+
+    def pointer(self):
+        return self.values.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+@dataclass
+class Selection:
+    experiment: int
+    runner_up: int
+    objective: float
+    runner_up_objective: float
+    gap_bound: float
+    near_tie: bool
+    redecided: bool
+
+    @classmethod
+    def from_c(cls, s: nat.SelectionC, flip=None) -> "Selection":
+        f = flip or (lambda v: v)
+        return cls(f(s.experiment), f(s.runner_up) if s.runner_up != 0xFFFFFFFF else s.runner_up,
+                   s.objective, s.runner_up_objective, s.gap_bound, bool(s.near_tie), bool(s.redecided))
+
+
+class Lattice:
+    """bgt::Lattice (include/bgt/lattice.hpp:34-67) backed by a device-resident posterior."""
+
+    def __init__(self, type: LatticeType, subjects: int, prior: Sequence[float], variants: int = 1,
+                 precision: Precision = Precision.float64):
+        # lattice_api.cpp:66-77
+        if subjects <= 0 or variants <= 0:
+            raise InvalidArgument(nat.BAD_INPUT, "subjects and variants must be positive.")
+        prior = np.ascontiguousarray(np.asarray(prior, dtype=np.float64))
+        if prior.size != subjects * variants:
+            raise InvalidArgument(nat.BAD_INPUT, "prior length must equal subjects * variants.")
+        if type in (LatticeType.distributed_non_dilution, LatticeType.distributed_dilution):
+            raise Unsupported(nat.UNSUPPORTED, "distributed lattices are multi-rank; this provider is single GPU")
+        self._type = LatticeType(type)
+        self._n = subjects
+        self._k = variants
+        self._prior = prior
+        self._h = nat._H()
+        nat.check(nat.lib().bgt_hip_lattice_create(int(_uses_dilution(self._type)), subjects, variants,
+                                                   prior.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                                   _storage(precision), ctypes.byref(self._h)))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                nat.lib().bgt_hip_lattice_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def close(self) -> None:
+        self.__del__()
+
+    # -- shape
+    def subjects(self) -> int:
+        return self._n
+
+    def variants(self) -> int:
+        return self._k
+
+    def type(self) -> LatticeType:
+        return self._type
+
+    def _flip(self, v: int) -> int:
+        return reverse_bits(v, self._n) if self._k == 1 else v
+
+    @property
+    def handle(self):
+        return self._h
+
+    # -- queries
+    def posterior_probability(self, state: int) -> float:
+        out = ctypes.c_double()
+        nat.check(nat.lib().bgt_hip_lattice_posterior(self._h, self._flip(state), ctypes.byref(out)))
+        return out.value
+
+    def posterior(self) -> np.ndarray:
+        """All 2^(n*k) posterior entries in INTERNAL state order."""
+        out = np.empty(1 << (self._n * self._k), dtype=np.float64)
+        nat.check(nat.lib().bgt_hip_lattice_read_posterior(
+            self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), out.size))
+        return out
+
+    def upset_probability_mass(self, experiment: int) -> float:
+        out = ctypes.c_double()
+        nat.check(nat.lib().bgt_hip_lattice_upset_mass(self._h, self._flip(experiment), ctypes.byref(out)))
+        return out.value
+
+    def atom_masses(self) -> np.ndarray:
+        """Atom marginals in INTERNAL atom order."""
+        out = np.empty(self._n * self._k, dtype=np.float64)
+        nat.check(nat.lib().bgt_hip_lattice_atom_masses(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return out
+
+    def response_probability(self, experiment: int, response: int, true_state: int,
+                             dilution: Optional[DilutionTable] = None) -> float:
+        # lattice_api.cpp:101-106 -> replicated_lattice.cpp:8-28 (host-side scalar query)
+        e = self._flip(experiment)
+        s = self._flip(true_state)
+        ret = 1.0
+        if not _uses_dilution(self._type):
+            for v in range(self._k):
+                inside = (e & (s >> (v * self._n))) == e
+                rb = (response >> v) & 1
+                ret *= (0.99 if rb else (1.0 - 0.99)) if inside else ((1.0 - 0.99) if rb else 0.99)
+            return ret
+        if dilution is None:
+            raise InvalidArgument(nat.BAD_INPUT, "dilution table is required for dilution lattices")
+        size = bin(e).count("1")
+        for v in range(self._k):
+            p1 = dilution.values[(size - 1) * (dilution.subjects() + 1) + size - bin(e & (s >> (v * self._n))).count("1")]
+            ret *= p1 if (response >> v) & 1 else 1.0 - p1
+        return ret
+
+    # -- selection
+    def select(self, selector: SelectorType = SelectorType.auto_select) -> Selection:
+        """Full selection record (experiment in PUBLIC coordinates)."""
+        s = nat.SelectionC()
+        nat.check(nat.lib().bgt_hip_lattice_select(self._h, int(selector), ctypes.byref(s)))
+        return Selection.from_c(s, self._flip)
+
+    def select_experiment(self, selector: SelectorType = SelectorType.auto_select) -> int:
+        # lattice_api.cpp:108-114
+        return self.select(selector).experiment
+
+    def select_experiments(self, k: int, selector: SelectorType = SelectorType.op_bha) -> list[int]:
+        # lattice_api.cpp:116-168
+        if k < 0:
+            raise InvalidArgument(nat.BAD_INPUT, "k must be non-negative.")
+        if k == 0:
+            return []
+        resolved = SelectorType.op_bha if (selector == SelectorType.auto_select and self._k == 1) else selector
+        if selector == SelectorType.auto_select and self._k != 1:
+            resolved = SelectorType.op_bbpa
+        if resolved == SelectorType.op_bbpa:
+            if k != 1:
+                raise InvalidArgument(nat.BAD_INPUT, "Op-BBPA currently selects one experiment at a time.")
+            return [self.select_experiment(selector)]
+        if resolved != SelectorType.op_bha or self._k != 1:
+            raise InvalidArgument(nat.BAD_INPUT,
+                                  "Op-BHA multi-experiment selection is only supported for binary lattices.")
+        out = (ctypes.c_uint32 * k)()
+        nat.check(nat.lib().bgt_hip_lattice_select_experiments(self._h, k, out))
+        return [int(v) for v in out]
+
+    # -- update / classification
+    def update(self, experiment: int, response: int, dilution: Optional[DilutionTable] = None) -> None:
+        # lattice_api.cpp:170-174
+        ptr = dilution.pointer() if dilution is not None else None
+        if _uses_dilution(self._type) and dilution is None:
+            raise InvalidArgument(nat.BAD_INPUT, "dilution table is required for dilution lattices")
+        nat.check(nat.lib().bgt_hip_lattice_update(self._h, self._flip(experiment), response, ptr))
+
+    def round(self, experiment: int, response: int, threshold_up: float, threshold_lo: float,
+              dilution: Optional[DilutionTable] = None, masses: Optional[np.ndarray] = None) -> Selection:
+        """Fused update + classification + next Op-BHA selection (binary lattices)."""
+        s = nat.SelectionC()
+        ptr = dilution.pointer() if dilution is not None else None
+        mp = masses.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if masses is not None else None
+        nat.check(nat.lib().bgt_hip_lattice_round(self._h, self._flip(experiment), response, ptr, threshold_up,
+                                                  threshold_lo, ctypes.byref(s), None, None, mp))
+        return Selection.from_c(s, self._flip)
+
+    def update_classification(self, threshold_up: float, threshold_lo: float) -> None:
+        nat.check(nat.lib().bgt_hip_lattice_classify(self._h, threshold_up, threshold_lo, None, None))
+
+    def _masks(self):
+        p = ctypes.c_uint32()
+        q = ctypes.c_uint32()
+        nat.check(nat.lib().bgt_hip_lattice_masks(self._h, ctypes.byref(p), ctypes.byref(q)))
+        return p.value, q.value
+
+    def positive_classification_mask(self) -> int:
+        return self._flip(self._masks()[0])
+
+    def negative_classification_mask(self) -> int:
+        return self._flip(self._masks()[1])
+
+    def is_classified(self) -> bool:
+        p, q = self._masks()
+        return bin(p | q).count("1") == self._n * self._k
+
+    def error_bound(self) -> tuple[float, float]:
+        a = ctypes.c_double()
+        b = ctypes.c_double()
+        nat.check(nat.lib().bgt_hip_lattice_error_bound(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+
+# ------------------------------------------------------------------------------ simulation
+
+@dataclass
+class TreeStats:
+    # include/bgt/types.hpp:109-119
+    total_leaves: int = 0
+    correct_probability: float = 0.0
+    incorrect_probability: float = 0.0
+    false_positive_probability: float = 0.0
+    false_negative_probability: float = 0.0
+    unclassified_probability: float = 0.0
+    expected_stages: float = 0.0
+    expected_tests: float = 0.0
+
+    @classmethod
+    def from_c(cls, s: nat.TreeStatsC) -> "TreeStats":
+        return cls(*(getattr(s, f) for f, _ in nat.TreeStatsC._fields_))
+
+
+@dataclass
+class SimulationOptions:
+    # include/bgt/types.hpp:121-130
+    search_depth: int = 1
+    threshold_up: float = 0.01
+    threshold_lo: float = 0.01
+    branch_threshold: float = 0.0
+    selector: SelectorType = SelectorType.auto_select
+    provider: Provider = Provider.auto_select
+    precision: Precision = Precision.float64
+
+
+@dataclass
+class DilutionParameters:
+    # include/bgt/simulation.hpp:44-48
+    alpha: float = 0.99
+    h: float = 0.005
+
+
+@dataclass
+class SimulationConfig:
+    # include/bgt/simulation.hpp:87-103 (hot-path fields)
+    lattice_type: LatticeType = LatticeType.replicated_non_dilution
+    subjects: int = 0
+    variants: int = 1
+    prior: list = field(default_factory=list)
+    options: SimulationOptions = field(default_factory=SimulationOptions)
+    dilution: DilutionParameters = field(default_factory=DilutionParameters)
+
+
+@dataclass
+class KernelSpec:
+    # include/bgt/types.hpp:132-181; the arch string names the GPU target
+    provider: Provider = Provider.auto_select
+    subjects: int = 0
+    variants: int = 1
+    state_bits: int = 32
+    dilution: DilutionMode = DilutionMode.non_dilution
+    selector: SelectorType = SelectorType.auto_select
+    precision: Precision = Precision.float64
+    accumulator_precision: Precision = Precision.float64
+    gpu_arch: str = "gfx950"
+
+    def key(self) -> tuple:
+        return (int(self.provider), self.subjects, self.variants, self.state_bits, int(self.dilution),
+                int(self.selector), int(self.precision), int(self.accumulator_precision), self.gpu_arch)
+
+
+@dataclass
+class SimulationResult:
+    stats: TreeStats
+    kernel: KernelSpec
+    provider: Provider
+    total_states: int
+    evaluated_states: int
+
+
+def _resolve_selector(selector: SelectorType, variants: int) -> SelectorType:
+    # simulation_helpers.cpp:40-45
+    if selector == SelectorType.auto_select:
+        return SelectorType.op_bha if variants == 1 else SelectorType.op_bbpa
+    return selector
+
+
+def _validate(config: SimulationConfig) -> None:
+    # simulation_runner.cpp:30-68 (local-tree subset)
+    if config.subjects <= 0 or config.variants <= 0 or config.options.search_depth < 0:
+        raise InvalidArgument(nat.BAD_INPUT, "invalid simulation dimensions.")
+    if len(config.prior) != config.subjects * config.variants:
+        raise InvalidArgument(nat.BAD_INPUT, "prior length must equal subjects * variants.")
+    if _resolve_selector(config.options.selector, config.variants) == SelectorType.op_bha and config.variants != 1:
+        raise InvalidArgument(nat.BAD_INPUT, "Op-BHA tree selection is only supported for binary lattices.")
+    if _uses_dilution(config.lattice_type) and (config.dilution.alpha <= 0.0 or config.dilution.h <= 0.0):
+        raise InvalidArgument(nat.BAD_INPUT, "dilution parameters must be positive.")
+    if config.lattice_type in (LatticeType.distributed_dilution, LatticeType.distributed_non_dilution):
+        raise InvalidArgument(nat.BAD_INPUT, "local_tree requires a replicated lattice.")
+
+
+class Runtime:
+    """include/bgt/runtime.hpp:58-78: provider registry keyed by KernelSpec."""
+
+    def __init__(self):
+        self._registry: set = set()
+
+    def available_providers(self) -> list[Provider]:
+        out = [Provider.cpu]
+        if nat.available():
+            out.append(Provider.hip)
+        return out
+
+    def make_kernel_spec(self, config: SimulationConfig) -> KernelSpec:
+        # simulation_runner.cpp:166-179
+        return KernelSpec(provider=config.options.provider, subjects=config.subjects, variants=config.variants,
+                          dilution=DilutionMode.dilution if _uses_dilution(config.lattice_type)
+                          else DilutionMode.non_dilution,
+                          selector=_resolve_selector(config.options.selector, config.variants),
+                          precision=config.options.precision,
+                          accumulator_precision=Precision.float64)
+
+    def registry_size(self) -> int:
+        return len(self._registry)
+
+    def run_simulation(self, config: SimulationConfig) -> SimulationResult:
+        _validate(config)
+        spec = self.make_kernel_spec(config)
+        self._registry.add(spec.key())
+        if config.options.provider in (Provider.cpu, Provider.cuda):
+            raise Unsupported(nat.UNSUPPORTED, f"{config.options.provider.name} provider is not part of this build")
+        prior = np.ascontiguousarray(np.asarray(config.prior, dtype=np.float64))
+        dil = None
+        if _uses_dilution(config.lattice_type):
+            dil = DilutionTable(config.subjects, config.dilution.alpha, config.dilution.h)
+        st = nat.TreeStatsC()
+        o = config.options
+        nat.check(nat.lib().bgt_hip_provider_run_precision(
+            int(_uses_dilution(config.lattice_type)), config.subjects, config.variants,
+            prior.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), o.search_depth, o.threshold_up, o.threshold_lo,
+            o.branch_threshold, int(o.selector), dil.pointer() if dil is not None else None,
+            _storage(o.precision), ctypes.byref(st)))
+        spec.provider = Provider.hip
+        total = 1 << (config.subjects * config.variants)
+        return SimulationResult(TreeStats.from_c(st), spec, Provider.hip, total, total)
+
+
+def run_simulation(config: SimulationConfig) -> SimulationResult:
+    return Runtime().run_simulation(config)
+
+
+def available_providers() -> list[Provider]:
+    return Runtime().available_providers()
+
+
+def hip_available() -> bool:
+    return nat.available()

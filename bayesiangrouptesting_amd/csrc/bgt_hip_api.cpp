@@ -1,0 +1,356 @@
+// extern "C" boundary of the provider (include/bgt_hip.h). No exception crosses it.
+#include "bgt_hip.h"
+#include "bgt_hip_lattice.hpp"
+#include "bgt_hip_tree.hpp"
+
+#include <hip/hip_runtime_api.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+struct bgt_hip_lattice {
+	std::unique_ptr<bgt_hip::DeviceLattice> impl;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <typename F>
+int guarded(F &&f)
+{
+	try {
+		f();
+		g_last_error.clear();
+		return BGT_HIP_OK;
+	} catch (const bgt_hip::Error &e) {
+		g_last_error = e.what();
+		return e.code;
+	} catch (const std::bad_alloc &) {
+		g_last_error = "host allocation failed";
+		return BGT_HIP_INTERNAL;
+	} catch (const std::exception &e) {
+		g_last_error = e.what();
+		return BGT_HIP_INTERNAL;
+	}
+}
+
+bgt_hip::DeviceLattice &L(bgt_hip_lattice *h)
+{
+	if (h == nullptr || !h->impl)
+		throw bgt_hip::Error(BGT_HIP_BAD_INPUT, "null lattice handle");
+	return *h->impl;
+}
+
+bgt_hip::Storage storage_of(int precision)
+{
+	if (precision == BGT_HIP_FP64)
+		return bgt_hip::Storage::fp64;
+	if (precision == BGT_HIP_FP32C)
+		return bgt_hip::Storage::fp32c;
+	throw bgt_hip::Error(BGT_HIP_BAD_INPUT, "unknown precision");
+}
+
+} // namespace
+
+extern "C" {
+
+const char *bgt_hip_last_error(void) { return g_last_error.c_str(); }
+
+int bgt_hip_provider_available(void)
+{
+	try {
+		bgt_hip::require_device();
+		return 1;
+	} catch (...) {
+		return 0;
+	}
+}
+
+int bgt_hip_device_synchronize(void)
+{
+	return guarded([&] { BGT_HIP_CHECK(hipDeviceSynchronize()); });
+}
+
+int bgt_hip_device_name(char *out, int len)
+{
+	return guarded([&] {
+		if (out == nullptr || len <= 0)
+			throw bgt_hip::Error(BGT_HIP_BAD_INPUT, "null buffer");
+		if (bgt_hip::device_count() <= 0)
+			throw bgt_hip::Error(BGT_HIP_UNSUPPORTED, "no HIP device");
+		hipDeviceProp_t p;
+		BGT_HIP_CHECK(hipGetDeviceProperties(&p, 0));
+		std::snprintf(out, (size_t)len, "%s (%s, %d CUs)", p.name, p.gcnArchName, p.multiProcessorCount);
+	});
+}
+
+int bgt_hip_provider_run_precision(int use_dilution, int subjects, int variants, const double *prior, int search_depth,
+								   double threshold_up, double threshold_lo, double branch_threshold, int selector,
+								   const double *dilution, int precision, bgt_hip_tree_stats *stats)
+{
+	return guarded([&] {
+		if (stats == nullptr || prior == nullptr)
+			throw bgt_hip::Error(BGT_HIP_BAD_INPUT, "null argument");
+		*stats = bgt_hip::run_tree(use_dilution != 0, subjects, variants, prior, search_depth, threshold_up,
+								   threshold_lo, branch_threshold, selector, dilution, storage_of(precision));
+	});
+}
+
+int bgt_hip_provider_run(int use_dilution, int subjects, int variants, const double *prior, int search_depth,
+						 double threshold_up, double threshold_lo, double branch_threshold, int selector,
+						 const double *dilution, bgt_hip_tree_stats *stats)
+{
+	return bgt_hip_provider_run_precision(use_dilution, subjects, variants, prior, search_depth, threshold_up,
+										  threshold_lo, branch_threshold, selector, dilution, BGT_HIP_FP64, stats);
+}
+
+int bgt_hip_provider_run_distributed(int, int, int, const double *, int, double, double, double, int, int,
+									 const double *, bgt_hip_tree_stats *)
+{
+	g_last_error = "the HIP provider runs on one GPU; multi-rank evaluation is not provided";
+	return BGT_HIP_NO_BACKEND;
+}
+
+int bgt_hip_provider_benchmark_select(int subjects, int variants, const double *prior, int selector, int iterations,
+									  int warmup, uint32_t *candidate, double *init_seconds, double *mean_seconds,
+									  double *min_seconds, double *max_seconds, double *stddev_seconds)
+{
+	// timing shape of bgt_cuda_provider_benchmark_select (cuda_provider.cu:2449-2551)
+	return guarded([&] {
+		if (iterations <= 0 || warmup < 0 || prior == nullptr)
+			throw bgt_hip::Error(BGT_HIP_BAD_INPUT, "invalid benchmark arguments");
+		using clk = std::chrono::steady_clock;
+		const auto t0 = clk::now();
+		bgt_hip::DeviceLattice lat(false, subjects, variants, prior, bgt_hip::Storage::fp64);
+		const double init = std::chrono::duration<double>(clk::now() - t0).count();
+		bgt_hip_selection sel{};
+		for (int i = 0; i < warmup; ++i)
+			sel = lat.select(selector);
+		std::vector<double> t(iterations);
+		for (int i = 0; i < iterations; ++i) {
+			const auto a = clk::now();
+			sel = lat.select(selector);
+			t[i] = std::chrono::duration<double>(clk::now() - a).count();
+		}
+		double mean = 0, mn = t[0], mx = t[0];
+		for (double x : t) {
+			mean += x;
+			mn = std::min(mn, x);
+			mx = std::max(mx, x);
+		}
+		mean /= iterations;
+		double var = 0;
+		for (double x : t)
+			var += (x - mean) * (x - mean);
+		var /= iterations;
+		if (candidate)
+			*candidate = sel.experiment;
+		if (init_seconds)
+			*init_seconds = init;
+		if (mean_seconds)
+			*mean_seconds = mean;
+		if (min_seconds)
+			*min_seconds = mn;
+		if (max_seconds)
+			*max_seconds = mx;
+		if (stddev_seconds)
+			*stddev_seconds = std::sqrt(var);
+	});
+}
+
+int bgt_hip_lattice_create(int use_dilution, int subjects, int variants, const double *prior, int precision,
+						   bgt_hip_lattice **out)
+{
+	return guarded([&] {
+		if (out == nullptr)
+			throw bgt_hip::Error(BGT_HIP_BAD_INPUT, "null output handle");
+		*out = nullptr;
+		auto h = std::make_unique<bgt_hip_lattice>();
+		h->impl = std::make_unique<bgt_hip::DeviceLattice>(use_dilution != 0, subjects, variants, prior,
+														   storage_of(precision));
+		*out = h.release();
+	});
+}
+
+int bgt_hip_lattice_destroy(bgt_hip_lattice *lattice)
+{
+	return guarded([&] { delete lattice; });
+}
+
+int bgt_hip_lattice_update(bgt_hip_lattice *h, uint32_t experiment, uint32_t response, const double *dilution)
+{
+	return guarded([&] { L(h).update(experiment, response, dilution); });
+}
+
+int bgt_hip_lattice_select(bgt_hip_lattice *h, int selector, bgt_hip_selection *out)
+{
+	return guarded([&] {
+		if (out == nullptr)
+			throw bgt_hip::Error(BGT_HIP_BAD_INPUT, "null output");
+		*out = L(h).select(selector);
+	});
+}
+
+int bgt_hip_lattice_posterior(bgt_hip_lattice *h, uint32_t state, double *out)
+{
+	return guarded([&] {
+		if (out == nullptr)
+			throw bgt_hip::Error(BGT_HIP_BAD_INPUT, "null output");
+		*out = L(h).posterior(state);
+	});
+}
+
+int bgt_hip_lattice_read_posterior(bgt_hip_lattice *h, double *out, uint64_t count)
+{
+	return guarded([&] {
+		auto &l = L(h);
+		if (out == nullptr || count != l.states())
+			throw bgt_hip::Error(BGT_HIP_BAD_INPUT, "output must hold 2^(subjects*variants) doubles");
+		l.read_posterior(out);
+	});
+}
+
+int bgt_hip_lattice_upset_mass(bgt_hip_lattice *h, uint32_t experiment, double *out)
+{
+	return guarded([&] {
+		if (out == nullptr)
+			throw bgt_hip::Error(BGT_HIP_BAD_INPUT, "null output");
+		*out = L(h).upset_mass(experiment);
+	});
+}
+
+int bgt_hip_lattice_atom_masses(bgt_hip_lattice *h, double *out)
+{
+	return guarded([&] {
+		if (out == nullptr)
+			throw bgt_hip::Error(BGT_HIP_BAD_INPUT, "null output");
+		L(h).atom_masses(out);
+	});
+}
+
+int bgt_hip_lattice_classify(bgt_hip_lattice *h, double up, double lo, uint32_t *pos, uint32_t *neg)
+{
+	return guarded([&] {
+		auto &l = L(h);
+		l.classify(up, lo);
+		if (pos)
+			*pos = l.positive_mask();
+		if (neg)
+			*neg = l.negative_mask();
+	});
+}
+
+int bgt_hip_lattice_masks(bgt_hip_lattice *h, uint32_t *pos, uint32_t *neg)
+{
+	return guarded([&] {
+		auto &l = L(h);
+		if (pos)
+			*pos = l.positive_mask();
+		if (neg)
+			*neg = l.negative_mask();
+	});
+}
+
+int bgt_hip_lattice_select_experiments(bgt_hip_lattice *h, int steps, uint32_t *out)
+{
+	return guarded([&] {
+		if (out == nullptr && steps > 0)
+			throw bgt_hip::Error(BGT_HIP_BAD_INPUT, "null output");
+		auto v = L(h).select_experiments(steps);
+		for (size_t i = 0; i < v.size(); ++i)
+			out[i] = v[i];
+	});
+}
+
+int bgt_hip_lattice_round(bgt_hip_lattice *h, uint32_t experiment, uint32_t response, const double *dilution,
+						  double up, double lo, bgt_hip_selection *next, uint32_t *pos, uint32_t *neg,
+						  double *masses_out)
+{
+	return guarded([&] {
+		auto &l = L(h);
+		const bgt_hip_selection s = l.round(experiment, response, dilution, up, lo, masses_out);
+		if (next)
+			*next = s;
+		if (pos)
+			*pos = l.positive_mask();
+		if (neg)
+			*neg = l.negative_mask();
+	});
+}
+
+int bgt_hip_lattice_error_bound(bgt_hip_lattice *h, double *posterior_rel, double *mass_rel)
+{
+	return guarded([&] {
+		double a = 0, b = 0;
+		L(h).error_bound(&a, &b);
+		if (posterior_rel)
+			*posterior_rel = a;
+		if (mass_rel)
+			*mass_rel = b;
+	});
+}
+
+int bgt_hip_lattice_kernel_times(bgt_hip_lattice *h, int enable, double *ms_out, int *launches_out, int capacity,
+								 char *names, int names_len)
+{
+	return guarded([&] {
+		auto &clk = L(h).clock();
+		if (enable >= 0) {
+			clk.reset();
+			clk.enabled = enable != 0;
+			return;
+		}
+		clk.resolve();
+		std::string joined;
+		for (int i = 0; i < (int)clk.entries.size() && i < capacity; ++i) {
+			const auto &e = clk.entries[i];
+			if (ms_out)
+				ms_out[i] = e.launches ? e.total_ms / e.launches : 0.0;
+			if (launches_out)
+				launches_out[i] = e.launches;
+			if (i)
+				joined += ";";
+			joined += e.name;
+		}
+		if (names && names_len > 0) {
+			std::strncpy(names, joined.c_str(), (size_t)names_len - 1);
+			names[names_len - 1] = 0;
+		}
+	});
+}
+
+int bgt_hip_lattice_info(bgt_hip_lattice *h, int *subjects, int *variants, int *precision, uint64_t *device_bytes)
+{
+	return guarded([&] {
+		auto &l = L(h);
+		if (subjects)
+			*subjects = l.subjects();
+		if (variants)
+			*variants = l.variants();
+		if (precision)
+			*precision = l.storage() == bgt_hip::Storage::fp64 ? BGT_HIP_FP64 : BGT_HIP_FP32C;
+		if (device_bytes)
+			*device_bytes = l.device_bytes();
+	});
+}
+
+int bgt_hip_batch_step(bgt_hip_lattice **lattices, const uint32_t *experiments, const uint32_t *responses,
+					   const double *const *dilutions, int count, int selector, bgt_hip_selection *out)
+{
+	return guarded([&] {
+		if (count < 0 || (count > 0 && (lattices == nullptr || experiments == nullptr || responses == nullptr ||
+										 out == nullptr)))
+			throw bgt_hip::Error(BGT_HIP_BAD_INPUT, "null argument");
+		for (int i = 0; i < count; ++i)
+			L(lattices[i]).update(experiments[i], responses[i], dilutions ? dilutions[i] : nullptr);
+		for (int i = 0; i < count; ++i)
+			out[i] = L(lattices[i]).select(selector);
+	});
+}
+
+} // extern "C"

@@ -1,0 +1,129 @@
+// Internal declarations shared by the host orchestration and the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+namespace bgt_hip {
+
+// Largest lattice: 2^30 states (the reference's int state_count limit, state_encoding.hpp:26-33).
+constexpr int kMaxAtoms = 30;
+constexpr int kMaxVariants = 5;
+constexpr int kMaxSubjects = 30;
+
+// Register tile of the transform: 1024 threads x 32 values = 2^15 states per workgroup.
+constexpr int kTileBits = 15;
+constexpr int kTileThreads = 1024;
+constexpr int kTileRegs = 32;
+// Lattices with at most this many atoms run the transform in one LDS-resident workgroup.
+constexpr int kSmallBits = 14;
+
+enum class Storage : int { fp64 = 0, fp32c = 1 };
+
+enum Objective : int {
+	kObjHalving = 0,   // Op-BHA: |M(E) - 1/2|
+	kObjBinaryBbpa = 1, // BBPA k=1: |M - t| + |(T - M) - t|
+	kObjGather = 2,    // BBPA k>=2: scatter F_B(E) = M(spread_B(E)) into feature tables
+	kObjNone = 3,      // transform only (upset masses written out)
+};
+
+// Per-lattice scalar slots in device memory (double scal[kScalars]).
+enum Scalar : int {
+	kScalInv0 = 0, // 1/D, double-buffered with kScalInv1 (pending normalisation of y)
+	kScalInv1 = 1,
+	kScalDen = 2,   // last denominator D
+	kScalTotal = 3, // total mass of the last transform input
+	kScalUpset = 4, // last upset-mass query
+	kScalars = 8,
+};
+
+struct SelResult {
+	double d1, d2;
+	uint32_t e1, e2;
+};
+
+// Everything a kernel needs to know about one lattice. Arrays of these live in device
+// memory so that one launch can serve a batch of lattices (blockIdx.y selects).
+struct LatDesc {
+	void *post;        // unnormalised posterior y (double or float); x = y * inv
+	void *z;           // spilled transform intermediate (double or float)
+	double *scal;      // kScalars slots
+	double *part;      // per-block partial sums
+	SelResult *tops;   // per-block top-2 candidates
+	SelResult *result; // final selection
+	unsigned *counter; // last-block arrival counter
+	double *lik;       // likelihood factors f[v * (n+1) + idx]
+	double *masses;    // 32 atom masses (singletons of the transform)
+	double *ftab;      // (2^k) x 2^n feature tables F_B(E) for BBPA k >= 2
+	int n, k, atoms;
+	int len;           // |E| of the experiment being applied
+	uint32_t e, r;     // experiment / response being applied
+	uint32_t active;   // selectable subjects (experiments must be subsets)
+	int dilution;
+	int inv_in, inv_out; // scalar slot read / written for 1/D
+};
+
+constexpr int kLikCapacity = kMaxVariants * (kMaxSubjects + 1);
+
+// Kernel argument: one lattice by value (single calls need no host-to-device copy) or a
+// device array of `count` lattices (batched launches, blockIdx.y selects the lattice).
+struct Descs {
+	LatDesc one;
+	double one_lik[kLikCapacity];
+	const LatDesc *many;
+};
+
+// Host-side error type carrying a provider status code.
+struct Error : std::runtime_error {
+	int code;
+	Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+void check_hip(hipError_t e, const char *what);
+#define BGT_HIP_CHECK(expr) ::bgt_hip::check_hip((expr), #expr)
+
+// ---- kernel launchers (bgt_hip_kernels.hip). `descs` is a device array of `count` LatDesc,
+// all of the same (n, k). Every launcher is asynchronous on `stream`. ----
+void launch_prior(const Descs &descs, int count, int atoms, Storage st, const double *pi_pairs,
+				  uint64_t states, hipStream_t stream);
+void launch_update(const Descs &descs, int count, uint64_t states, int k, int dilution,
+				   Storage st, hipStream_t stream);
+// exact path: transform of x = y * inv (n >= kTileBits + 1): pass 1 over the low 15 bits
+void launch_zeta_pass1(const Descs &descs, int count, int n, Storage st, bool fused_update,
+					   int k, int dilution, hipStream_t stream);
+// pass 2 over bits 15..n-1 (or the whole transform when n == 15), with an objective
+void launch_zeta_final(const Descs &descs, int count, int n, int k, Storage st, int objective,
+					   bool from_posterior, bool scaled, double target, hipStream_t stream);
+// n <= kSmallBits: whole transform in one workgroup per lattice
+void launch_zeta_small(const Descs &descs, int count, int n, int k, Storage st, int objective,
+					   double target, hipStream_t stream);
+void launch_bbpa_score(const Descs &descs, int count, int n, int k, double target,
+					   hipStream_t stream);
+void launch_brute_force(const Descs &descs, int count, int n, int k, Storage st,
+						hipStream_t stream);
+void launch_atom_masses(const Descs &descs, int count, int atoms, Storage st, hipStream_t stream);
+void launch_upset_mass(const Descs &descs, int count, int atoms, uint32_t experiment,
+					   Storage st, hipStream_t stream);
+void launch_read_posterior(const Descs &descs, int atoms, Storage st, double *out,
+						   hipStream_t stream);
+void launch_lookahead(const Descs &descs, int n, Storage st, const uint32_t *selected,
+					  int count_selected, double *scores, hipStream_t stream);
+
+// tree statistics: one thread per true state walks the flattened tree
+struct FlatNode {
+	uint32_t ex, res;
+	uint32_t pos, neg;
+	int stage;
+	int nchild;
+	int first_child;
+	int subtree_end; // one past the last node of this node's subtree (preorder)
+};
+void launch_tree_stats(const FlatNode *nodes, int node_count, int n, int k, int dilution,
+					   const double *dil, const double *pi, int depth, double branch,
+					   double *partials, unsigned *counter, double *out, hipStream_t stream);
+
+} // namespace bgt_hip

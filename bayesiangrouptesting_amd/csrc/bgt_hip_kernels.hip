@@ -1,0 +1,1108 @@
+// gfx950 kernels of the BayesianGroupTesting HIP provider.
+//
+// Layout in HBM: one lattice = 2^A posterior entries (A = subjects * variants), entry S at
+// offset S (the reference's state index, variant-major atoms). The posterior is kept
+// unnormalised as y with a pending 1/D in a device scalar slot; every reader forms
+// x = y * (1/D), which is exactly the value the reference stores after
+// update_probs_in_place's `x * (1/D)` (lattice_base.cpp:230-244 under -ffast-math).
+//
+// The subset zeta transform (bbpa.cpp:497-512) runs on register tiles of 2^15 entries
+// per 1024-thread workgroup: tile bits 0..5 across the lanes of a wave, bits 6..9 and
+// 14 across registers, bits 10..13 across the 16 waves (one LDS transpose). Bits are
+// always swept in increasing order, so every output is the same chain of two-operand
+// adds as the reference's serial loop and the transform of identical inputs is
+// bit-identical. Lattices with more than 15 atoms take two HBM passes: pass 1 sweeps
+// bits 0..14 of contiguous tiles, pass 2 sweeps bits 15..A-1 on tiles of
+// 2^(30-A) contiguous columns x 2^(A-15) strided rows and fuses the selection objective,
+// the arg-min and the atom singletons, so the transform's output is never written.
+#include <hip/hip_runtime.h>
+
+#include "bgt_hip_internal.hpp"
+
+#include <cmath>
+
+namespace bgt_hip {
+
+namespace {
+
+constexpr unsigned kNone = 0xFFFFFFFFu;
+
+// Lattice buffers arrive through a descriptor struct, so the compiler cannot infer their
+// address space; cast to global so loads and stores are global_* rather than flat_*.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T *;
+template <typename T>
+__device__ __forceinline__ gptr<T> G(T *p)
+{
+	return (gptr<T>)p;
+}
+
+struct Top2 {
+	double d1, d2;
+	uint32_t e1, e2;
+};
+
+__device__ __forceinline__ bool better(double a, uint32_t ea, double b, uint32_t eb)
+{
+	return a < b || (a == b && ea < eb);
+}
+
+__device__ __forceinline__ void top2_init(Top2 &t)
+{
+	t.d1 = INFINITY;
+	t.d2 = INFINITY;
+	t.e1 = kNone;
+	t.e2 = kNone;
+}
+
+__device__ __forceinline__ void top2_insert(Top2 &t, double d, uint32_t e)
+{
+	if (better(d, e, t.d1, t.e1)) {
+		t.d2 = t.d1;
+		t.e2 = t.e1;
+		t.d1 = d;
+		t.e1 = e;
+	} else if (better(d, e, t.d2, t.e2)) {
+		t.d2 = d;
+		t.e2 = e;
+	}
+}
+
+__device__ __forceinline__ void top2_wave(Top2 &t)
+{
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1) {
+		const double d1 = __shfl_xor(t.d1, o);
+		const double d2 = __shfl_xor(t.d2, o);
+		const uint32_t e1 = __shfl_xor(t.e1, o);
+		const uint32_t e2 = __shfl_xor(t.e2, o);
+		top2_insert(t, d1, e1);
+		top2_insert(t, d2, e2);
+	}
+}
+
+// Block-wide top-2; the result is valid in thread 0.
+__device__ Top2 top2_block(Top2 t, Top2 *red)
+{
+	top2_wave(t);
+	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+	if (lane == 0)
+		red[w] = t;
+	__syncthreads();
+	Top2 r;
+	top2_init(r);
+	if (threadIdx.x == 0) {
+		const int nw = (blockDim.x + 63) >> 6;
+		for (int i = 0; i < nw; ++i) {
+			top2_insert(r, red[i].d1, red[i].e1);
+			top2_insert(r, red[i].d2, red[i].e2);
+		}
+	}
+	__syncthreads();
+	return r;
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1)
+		v += __shfl_xor(v, o);
+	return v;
+}
+
+// Block-wide sum in a fixed order (lanes by xor tree, waves in index order); valid in thread 0.
+__device__ double block_sum(double v, double *red)
+{
+	v = wave_sum(v);
+	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+	if (lane == 0)
+		red[w] = v;
+	__syncthreads();
+	double s = 0.0;
+	if (threadIdx.x == 0) {
+		const int nw = (blockDim.x + 63) >> 6;
+		for (int i = 0; i < nw; ++i)
+			s += red[i];
+	}
+	__syncthreads();
+	return s;
+}
+
+// Returns true in every thread of the last block of the lattice's grid to arrive.
+__device__ bool arrive_last(unsigned *counter, unsigned total)
+{
+	__shared__ unsigned s_last;
+	__threadfence();
+	__syncthreads();
+	if (threadIdx.x == 0)
+		s_last = (atomicAdd(counter, 1u) == total - 1) ? 1u : 0u;
+	__syncthreads();
+	if (s_last)
+		__threadfence();
+	return s_last != 0;
+}
+
+// Tile order: the 8 XCDs receive workgroups round-robin, so give XCD x the contiguous
+// tile range [x*G/8, (x+1)*G/8): neighbouring tiles (and the 64 B runs of pass 2) are
+// then read by CUs that share one L2.
+__device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned g)
+{
+	if (g < 8 || (g & 7))
+		return b;
+	return (b & 7) * (g >> 3) + (b >> 3);
+}
+
+// Likelihood L(E, r, S) (replicated_lattice.cpp:8-28): product over variants, in variant
+// order, of a per-variant factor chosen by the number of pooled subjects missing from
+// that variant's block. tab[v*(n+1)+idx] holds the factor; idx is the missing count for
+// dilution and [missing > 0] otherwise.
+__device__ __forceinline__ double likelihood(uint32_t s, uint32_t e, int n, int k, int len,
+											 int dilution, const double *tab)
+{
+	double ret = 1.0;
+	for (int v = 0; v < k; ++v) {
+		const uint32_t sv = s >> (v * n);
+		const int m = len - __popc(e & sv);
+		const int idx = dilution ? m : (m != 0);
+		ret *= tab[v * (n + 1) + idx];
+	}
+	return ret;
+}
+
+__device__ __forceinline__ void load_lik_table(const Descs &descs, const LatDesc &d, double *s_lik)
+{
+	const double *src = descs.many ? d.lik : descs.one_lik;
+	const int cnt = d.k * (d.n + 1);
+	for (int i = threadIdx.x; i < cnt; i += blockDim.x)
+		s_lik[i] = src[i];
+	__syncthreads();
+}
+
+// ------------------------------------------------------------------ register tile zeta
+
+template <int Q>
+__device__ __forceinline__ void bfly_lane(double (&v)[kTileRegs], int lane)
+{
+	const bool recv = ((lane >> Q) & 1) == 0;
+#pragma unroll
+	for (int j = 0; j < kTileRegs; ++j) {
+		const double o = __shfl_xor(v[j], 1 << Q);
+		v[j] = recv ? v[j] + o : v[j];
+	}
+}
+
+template <int B>
+__device__ __forceinline__ void bfly_reg(double (&v)[kTileRegs])
+{
+#pragma unroll
+	for (int j = 0; j < kTileRegs; ++j)
+		if (!(j & (1 << B)))
+			v[j] += v[j | (1 << B)];
+}
+
+// Layout A: tile index i = lane | jlo<<6 | wave<<10 | jhi<<14 (j = jlo + 16*jhi)
+// Layout B: tile index i = lane | wave<<6 | jlo<<10 | jhi<<14
+__device__ __forceinline__ uint32_t tile_index_a(int lane, int wave, int j)
+{
+	return (uint32_t)lane | ((uint32_t)(j & 15) << 6) | ((uint32_t)wave << 10) | ((uint32_t)(j >> 4) << 14);
+}
+__device__ __forceinline__ uint32_t tile_index_b(int lane, int wave, int j)
+{
+	return (uint32_t)lane | ((uint32_t)wave << 6) | ((uint32_t)(j & 15) << 10) | ((uint32_t)(j >> 4) << 14);
+}
+
+__device__ __forceinline__ void transpose_ab(double (&v)[kTileRegs], double *lds, int lane, int wave)
+{
+#pragma unroll
+	for (int h = 0; h < 2; ++h) {
+#pragma unroll
+		for (int j = 0; j < 16; ++j)
+			lds[((wave * 16 + j) << 6) + lane] = v[j + 16 * h];
+		__syncthreads();
+#pragma unroll
+		for (int j = 0; j < 16; ++j)
+			v[j + 16 * h] = lds[((j * 16 + wave) << 6) + lane];
+		__syncthreads();
+	}
+}
+
+// Sweep tile bits q0..14 in increasing order. Input in layout A, output in layout B.
+__device__ __forceinline__ void zeta_tile(double (&v)[kTileRegs], int q0, int lane, int wave, double *lds)
+{
+	if (q0 <= 0) bfly_lane<0>(v, lane);
+	if (q0 <= 1) bfly_lane<1>(v, lane);
+	if (q0 <= 2) bfly_lane<2>(v, lane);
+	if (q0 <= 3) bfly_lane<3>(v, lane);
+	if (q0 <= 4) bfly_lane<4>(v, lane);
+	if (q0 <= 5) bfly_lane<5>(v, lane);
+	if (q0 <= 6) bfly_reg<0>(v);
+	if (q0 <= 7) bfly_reg<1>(v);
+	if (q0 <= 8) bfly_reg<2>(v);
+	if (q0 <= 9) bfly_reg<3>(v);
+	transpose_ab(v, lds, lane, wave);
+	if (q0 <= 10) bfly_reg<0>(v);
+	if (q0 <= 11) bfly_reg<1>(v);
+	if (q0 <= 12) bfly_reg<2>(v);
+	if (q0 <= 13) bfly_reg<3>(v);
+	bfly_reg<4>(v);
+}
+
+// --------------------------------------------------------------- selection objective
+
+struct ObjCtx {
+	int objective;
+	uint32_t active;
+	double target, total;
+	int ns, k; // subjects, variants (gather)
+	double *ftab;
+	double *masses;
+};
+
+__device__ __forceinline__ void consume(const ObjCtx &c, uint32_t g, double m, Top2 &t)
+{
+	if (__popc(g) == 1)
+		c.masses[__ffs(g) - 1] = m;
+	if (c.objective == kObjHalving) {
+		if ((g & ~c.active) == 0)
+			top2_insert(t, fabs(m - 0.5), g);
+	} else if (c.objective == kObjBinaryBbpa) {
+		if ((g & ~c.active) == 0) {
+			const double positive = m;
+			const double negative = c.total - positive;
+			top2_insert(t, fabs(positive - c.target) + fabs(negative - c.target), g);
+		}
+	} else if (c.objective == kObjGather) {
+		// F_B(E) = M(spread_B(E)): g's non-zero variant blocks all equal E, B = those blocks
+		const uint32_t mask = (1u << c.ns) - 1u;
+		const uint32_t ec = 1u << c.ns;
+		if (g == 0) {
+			for (int b = 1; b < (1 << c.k); ++b)
+				c.ftab[(size_t)b * ec] = m;
+			return;
+		}
+		uint32_t e = 0, bset = 0;
+		bool ok = true;
+		for (int v = 0; v < c.k; ++v) {
+			const uint32_t blk = (g >> (v * c.ns)) & mask;
+			if (blk) {
+				if (e == 0)
+					e = blk;
+				else if (blk != e)
+					ok = false;
+				bset |= 1u << v;
+			}
+		}
+		if (ok)
+			c.ftab[(size_t)bset * ec + e] = m;
+	}
+}
+
+__device__ __forceinline__ void finish_selection(const LatDesc &d, unsigned slot, unsigned total,
+												 Top2 t, Top2 *red)
+{
+	Top2 b = top2_block(t, red);
+	if (threadIdx.x == 0)
+		reinterpret_cast<Top2 *>(d.tops)[slot] = b;
+	if (arrive_last(d.counter, total)) {
+		Top2 r;
+		top2_init(r);
+		for (unsigned i = threadIdx.x; i < total; i += blockDim.x) {
+			const Top2 p = reinterpret_cast<const Top2 *>(d.tops)[i];
+			top2_insert(r, p.d1, p.e1);
+			top2_insert(r, p.d2, p.e2);
+		}
+		r = top2_block(r, red);
+		if (threadIdx.x == 0) {
+			d.result->d1 = r.d1;
+			d.result->d2 = r.d2;
+			d.result->e1 = r.e1;
+			d.result->e2 = r.e2;
+			*d.counter = 0u;
+		}
+	}
+}
+
+// ----------------------------------------------------------------------------- kernels
+
+template <typename TP>
+__global__ __launch_bounds__(256) void k_prior(Descs descs, int atoms, const double *pairs, uint64_t states)
+{
+	// lattice_base.cpp:184-208: P(S) = prod over atoms a = 0..A-1, in order, from 1.0
+	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
+	gptr<TP> post = G(static_cast<TP *>(d.post));
+	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+	for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < states; s += stride) {
+		double prob = 1.0;
+		for (int a = 0; a < atoms; ++a)
+			prob *= ((s >> a) & 1u) ? pairs[2 * a + 1] : pairs[2 * a];
+		post[s] = static_cast<TP>(prob);
+	}
+}
+
+template <typename TP>
+__global__ __launch_bounds__(256) void k_update(Descs descs, uint64_t states)
+{
+	// lattice_base.cpp:230-244: y' = x * L(E, r, S); D = sum y'; the pending 1/D is applied
+	// by the next reader (x' = y' * (1/D))
+	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
+	__shared__ double s_lik[kMaxVariants * (kMaxSubjects + 1)];
+	__shared__ double red[16];
+	load_lik_table(descs, d, s_lik);
+	gptr<TP> post = G(static_cast<TP *>(d.post));
+	const double inv = d.scal[d.inv_in];
+	double acc = 0.0;
+	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+	for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < states; s += stride) {
+		const double x = static_cast<double>(post[s]) * inv;
+		const TP y = static_cast<TP>(x * likelihood((uint32_t)s, d.e, d.n, d.k, d.len, d.dilution, s_lik));
+		post[s] = y;
+		acc += static_cast<double>(y);
+	}
+	const double bs = block_sum(acc, red);
+	if (threadIdx.x == 0)
+		d.part[blockIdx.x] = bs;
+	if (arrive_last(d.counter, gridDim.x)) {
+		double t = 0.0;
+		for (unsigned i = threadIdx.x; i < gridDim.x; i += blockDim.x)
+			t += d.part[i];
+		t = block_sum(t, red);
+		if (threadIdx.x == 0) {
+			d.scal[kScalDen] = t;
+			d.scal[d.inv_out] = 1.0 / t;
+			*d.counter = 0u;
+		}
+	}
+}
+
+// Pass 1: sweep bits 0..14 of contiguous 2^15 tiles. Input is x = y * inv; with FUSED the
+// update y' = x * L is applied first (stored back in place) and the transform runs on y'.
+template <typename TP, typename TZ, bool FUSED>
+__global__ __launch_bounds__(kTileThreads) void k_zeta_pass1(Descs descs)
+{
+	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
+	__shared__ double lds[16 * 16 * 64];
+	__shared__ double red[16];
+	__shared__ double s_lik[kMaxVariants * (kMaxSubjects + 1)];
+	if (FUSED)
+		load_lik_table(descs, d, s_lik);
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const unsigned tiles = gridDim.x;
+	const unsigned tile = xcd_tile(blockIdx.x, tiles);
+	const uint64_t base = (uint64_t)tile << kTileBits;
+	// per-thread base pointers; the register index only adds compile-time offsets
+	gptr<TP> pa = G(static_cast<TP *>(d.post)) + base + lane + (wave << 10);
+	const double inv = d.scal[d.inv_in];
+	double v[kTileRegs];
+	double acc = 0.0;
+#pragma unroll
+	for (int j = 0; j < kTileRegs; ++j) {
+		const uint32_t off = ((uint32_t)(j & 15) << 6) | ((uint32_t)(j >> 4) << 14);
+		double x = static_cast<double>(pa[off]) * inv;
+		if (FUSED) {
+			const uint32_t g = (uint32_t)(base + tile_index_a(lane, wave, j));
+			const TP y = static_cast<TP>(x * likelihood(g, d.e, d.n, d.k, d.len, d.dilution, s_lik));
+			pa[off] = y;
+			x = static_cast<double>(y);
+		}
+		v[j] = x;
+		acc += x;
+	}
+	zeta_tile(v, 0, lane, wave, lds);
+	gptr<TZ> zb = G(static_cast<TZ *>(d.z)) + base + lane + (wave << 6);
+#pragma unroll
+	for (int j = 0; j < kTileRegs; ++j)
+		zb[((uint32_t)(j & 15) << 10) | ((uint32_t)(j >> 4) << 14)] = static_cast<TZ>(v[j]);
+	const double bs = block_sum(acc, red);
+	if (threadIdx.x == 0)
+		d.part[tile] = bs;
+	if (arrive_last(d.counter, tiles)) {
+		double t = 0.0;
+		for (unsigned i = threadIdx.x; i < tiles; i += blockDim.x)
+			t += d.part[i];
+		t = block_sum(t, red);
+		if (threadIdx.x == 0) {
+			if (FUSED) {
+				d.scal[kScalDen] = t;
+				d.scal[d.inv_out] = 1.0 / t;
+			}
+			d.scal[kScalTotal] = t;
+			*d.counter = 0u;
+		}
+	}
+}
+
+// Final pass: sweep tile bits q0..14 and evaluate the objective on every output entry.
+// Tile index i maps to state g = tile*c + (i & (c-1)) + ((i >> logc) << 15), c = 2^logc.
+// FROM_POST reads x = y * inv from the posterior (single-pass lattices, A == 15);
+// otherwise it reads the pass-1 intermediate, scaled by inv_out when `scaled`.
+template <typename TIN, bool FROM_POST>
+__global__ __launch_bounds__(kTileThreads) void k_zeta_final(Descs descs, int logc, int q0,
+															 int objective, int scaled, double target)
+{
+	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
+	__shared__ double lds[16 * 16 * 64];
+	__shared__ Top2 red2[16];
+	__shared__ double red[16];
+	__shared__ double s_total;
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const unsigned tiles = gridDim.x;
+	const unsigned tile = xcd_tile(blockIdx.x, tiles);
+	const uint32_t cmask = (1u << logc) - 1u;
+	const uint64_t cbase = (uint64_t)tile << logc;
+	gptr<const TIN> in = G(FROM_POST ? static_cast<const TIN *>(d.post) : static_cast<const TIN *>(d.z));
+	const double inv = FROM_POST ? d.scal[d.inv_in] : 1.0;
+	double v[kTileRegs];
+	double acc = 0.0;
+#pragma unroll
+	for (int j = 0; j < kTileRegs; ++j) {
+		const uint32_t i = tile_index_a(lane, wave, j);
+		const uint64_t g = cbase + (i & cmask) + ((uint64_t)(i >> logc) << kTileBits);
+		v[j] = static_cast<double>(in[g]) * inv;
+		acc += v[j];
+	}
+	ObjCtx c;
+	c.objective = objective;
+	c.active = d.active;
+	c.target = target;
+	c.ns = d.n;
+	c.k = d.k;
+	c.ftab = d.ftab;
+	c.masses = d.masses;
+	if (FROM_POST) {
+		// the whole lattice is this one tile: its total is known here
+		const double t = block_sum(acc, red);
+		if (threadIdx.x == 0) {
+			s_total = t;
+			d.scal[kScalTotal] = t;
+		}
+		__syncthreads();
+		c.total = s_total;
+	} else {
+		c.total = d.scal[kScalTotal];
+	}
+	zeta_tile(v, q0, lane, wave, lds);
+	const double scale = scaled ? d.scal[d.inv_out] : 1.0;
+	if (scaled)
+		c.total = c.total * scale;
+	Top2 t;
+	top2_init(t);
+#pragma unroll
+	for (int j = 0; j < kTileRegs; ++j) {
+		const uint32_t i = tile_index_b(lane, wave, j);
+		const uint32_t g = (uint32_t)(cbase + (i & cmask) + ((uint64_t)(i >> logc) << kTileBits));
+		const double m = scaled ? v[j] * scale : v[j];
+		consume(c, g, m, t);
+	}
+	finish_selection(d, tile, tiles, t, red2);
+}
+
+// Lattices of at most 2^14 states: the whole transform in the LDS of one workgroup.
+template <typename TP>
+__global__ __launch_bounds__(1024) void k_zeta_small(Descs descs, int atoms, int objective, double target)
+{
+	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
+	__shared__ double a[1 << kSmallBits];
+	__shared__ Top2 red2[16];
+	__shared__ double red[16];
+	__shared__ double s_total;
+	const uint32_t states = 1u << atoms;
+	gptr<const TP> post = G(static_cast<const TP *>(d.post));
+	const double inv = d.scal[d.inv_in];
+	double acc = 0.0;
+	for (uint32_t s = threadIdx.x; s < states; s += blockDim.x) {
+		const double x = static_cast<double>(post[s]) * inv;
+		a[s] = x;
+		acc += x;
+	}
+	const double t = block_sum(acc, red);
+	if (threadIdx.x == 0) {
+		s_total = t;
+		d.scal[kScalTotal] = t;
+	}
+	__syncthreads();
+	for (int b = 0; b < atoms; ++b) {
+		const uint32_t bit = 1u << b;
+		for (uint32_t p = threadIdx.x; p < (states >> 1); p += blockDim.x) {
+			const uint32_t lo = p & (bit - 1u);
+			const uint32_t e = ((p >> b) << (b + 1)) | lo;
+			a[e] += a[e | bit];
+		}
+		__syncthreads();
+	}
+	ObjCtx c;
+	c.objective = objective;
+	c.active = d.active;
+	c.target = target;
+	c.total = s_total;
+	c.ns = d.n;
+	c.k = d.k;
+	c.ftab = d.ftab;
+	c.masses = d.masses;
+	Top2 tt;
+	top2_init(tt);
+	for (uint32_t s = threadIdx.x; s < states; s += blockDim.x)
+		consume(c, s, a[s], tt);
+	Top2 r = top2_block(tt, red2);
+	if (threadIdx.x == 0) {
+		d.result->d1 = r.d1;
+		d.result->d2 = r.d2;
+		d.result->e1 = r.e1;
+		d.result->e2 = r.e2;
+	}
+}
+
+// BBPA k >= 2 scoring from the feature tables (bbpa.cpp:332-365 for k = 2 and 416-460
+// otherwise); same operation order per experiment.
+__global__ __launch_bounds__(256) void k_bbpa_score(Descs descs, int ns, int k, double target)
+{
+	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
+	__shared__ Top2 red2[16];
+	const uint32_t ec = 1u << ns;
+	const double total = d.scal[kScalTotal];
+	Top2 t;
+	top2_init(t);
+	const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+	if (e < ec && (e & ~d.active) == 0) {
+		const double *fm = d.ftab;
+		double score;
+		if (k == 2) {
+			const double p0 = fm[ec + e], p1 = fm[2 * ec + e], pb = fm[3 * ec + e];
+			const double r0 = pb;
+			const double r1 = p1 - pb;
+			const double r2 = p0 - pb;
+			const double r3 = total - p0 - p1 + pb;
+			score = fabs(r0 - target) + fabs(r1 - target) + fabs(r2 - target) + fabs(r3 - target);
+		} else {
+			const uint32_t rc = 1u << k, all = rc - 1u;
+			score = 0.0;
+			for (uint32_t resp = 0; resp < rc; ++resp) {
+				const uint32_t positive = (~resp) & all;
+				const uint32_t missing = resp;
+				double exact = 0.0;
+				for (uint32_t extra = missing;; extra = (extra - 1u) & missing) {
+					const uint32_t fs = positive | extra;
+					const double f = fs == 0 ? total : fm[(size_t)fs * ec + e];
+					if ((__popc(extra) & 1) == 0)
+						exact += f;
+					else
+						exact -= f;
+					if (extra == 0)
+						break;
+				}
+				score += fabs(exact - target);
+			}
+		}
+		top2_insert(t, score, e);
+	}
+	finish_selection(d, blockIdx.x, gridDim.x, t, red2);
+}
+
+// Explicit (experiment, state) sweep (benchmarks/experiment_selection_benchmark.cpp:214-247):
+// one workgroup per experiment, buckets per thread, fixed-order reductions.
+template <typename TP, int K>
+__global__ __launch_bounds__(256) void k_brute_force(Descs descs, int ns)
+{
+	constexpr int RC = 1 << K;
+	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
+	__shared__ double red[16];
+	__shared__ Top2 red2[16];
+	const uint32_t e = blockIdx.x;
+	const uint32_t states = 1u << (ns * K);
+	const uint32_t mask = (1u << ns) - 1u;
+	gptr<const TP> post = G(static_cast<const TP *>(d.post));
+	const double inv = d.scal[d.inv_in];
+	double mass[RC];
+#pragma unroll
+	for (int b = 0; b < RC; ++b)
+		mass[b] = 0.0;
+	for (uint32_t s = threadIdx.x; s < states; s += blockDim.x) {
+		const double x = static_cast<double>(post[s]) * inv;
+		int id = 0;
+#pragma unroll
+		for (int v = 0; v < K; ++v)
+			id |= ((e & ~((s >> (v * ns)) & mask)) != 0) << v;
+#pragma unroll
+		for (int b = 0; b < RC; ++b)
+			mass[b] += (id == b) ? x : 0.0;
+	}
+	const double target = 1.0 / (double)RC;
+	double score = 0.0;
+	for (int b = 0; b < RC; ++b) {
+		const double mb = block_sum(mass[b], red);
+		score += fabs(mb - target); // meaningful in thread 0
+	}
+	Top2 t;
+	top2_init(t);
+	if (threadIdx.x == 0 && (e & ~d.active) == 0)
+		top2_insert(t, score, e);
+	finish_selection(d, blockIdx.x, gridDim.x, t, red2);
+}
+
+// Atom marginals m_a = sum_{S : bit a} x_S (lattice_base.cpp:25-41). A workgroup covers
+// 2^14 contiguous states: thread bits are state bits 0..7, register index bits are 8..13,
+// and higher bits are uniform over the chunk.
+template <typename TP>
+__global__ __launch_bounds__(256) void k_atom_masses(Descs descs, int atoms)
+{
+	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
+	__shared__ double red[16];
+	const int cb = atoms < 14 ? atoms : 14; // chunk bits
+	const int jb = cb > 8 ? cb - 8 : 0;     // register bits
+	const uint32_t chunk = blockIdx.x;
+	const uint64_t base = (uint64_t)chunk << cb;
+	gptr<const TP> post = G(static_cast<const TP *>(d.post));
+	const double inv = d.scal[d.inv_in];
+	const uint32_t tid = threadIdx.x;
+	double total = 0.0;
+	double sq[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+	const bool live = (cb >= 8) || tid < (1u << cb);
+	if (live) {
+		for (uint32_t j = 0; j < (1u << jb); ++j) {
+			const double x = static_cast<double>(post[base + tid + (j << 8)]) * inv;
+			total += x;
+#pragma unroll
+			for (int q = 0; q < 6; ++q)
+				if (q < jb && ((j >> q) & 1u))
+					sq[q] += x;
+		}
+	}
+	for (int a = 0; a < atoms; ++a) {
+		double contrib;
+		if (a < 8) {
+			contrib = ((tid >> a) & 1u) ? total : 0.0;
+		} else if (a < 8 + jb) {
+			contrib = 0.0;
+#pragma unroll
+			for (int q = 0; q < 6; ++q)
+				if (a - 8 == q)
+					contrib = sq[q];
+		} else {
+			contrib = ((chunk >> (a - cb)) & 1u) ? total : 0.0;
+		}
+		const double s = block_sum(contrib, red);
+		if (threadIdx.x == 0)
+			d.part[(size_t)chunk * kMaxAtoms + a] = s;
+	}
+	if (arrive_last(d.counter, gridDim.x)) {
+		for (int a = 0; a < atoms; ++a) {
+			double s = 0.0;
+			for (unsigned i = threadIdx.x; i < gridDim.x; i += blockDim.x)
+				s += d.part[(size_t)i * kMaxAtoms + a];
+			s = block_sum(s, red);
+			if (threadIdx.x == 0)
+				d.masses[a] = s;
+		}
+		if (threadIdx.x == 0)
+			*d.counter = 0u;
+	}
+}
+
+// Upset mass M(E) = sum over supersets of E (lattice_base.cpp:318-351), enumerating the
+// free bits of E.
+template <typename TP>
+__global__ __launch_bounds__(256) void k_upset_mass(Descs descs, int atoms, uint32_t e)
+{
+	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
+	__shared__ double red[16];
+	__shared__ int pos[kMaxAtoms];
+	const uint32_t full = atoms == 32 ? 0xFFFFFFFFu : ((1u << atoms) - 1u);
+	const uint32_t freeb = full & ~e;
+	const int m = __popc(freeb);
+	if (threadIdx.x == 0) {
+		int c = 0;
+		for (int b = 0; b < atoms; ++b)
+			if ((freeb >> b) & 1u)
+				pos[c++] = b;
+	}
+	__syncthreads();
+	gptr<const TP> post = G(static_cast<const TP *>(d.post));
+	const double inv = d.scal[d.inv_in];
+	const uint64_t count = 1ull << m;
+	double acc = 0.0;
+	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+	for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+		uint32_t s = e;
+		for (int b = 0; b < m; ++b)
+			if ((i >> b) & 1u)
+				s |= 1u << pos[b];
+		acc += static_cast<double>(post[s]) * inv;
+	}
+	const double bs = block_sum(acc, red);
+	if (threadIdx.x == 0)
+		d.part[blockIdx.x] = bs;
+	if (arrive_last(d.counter, gridDim.x)) {
+		double t = 0.0;
+		for (unsigned i = threadIdx.x; i < gridDim.x; i += blockDim.x)
+			t += d.part[i];
+		t = block_sum(t, red);
+		if (threadIdx.x == 0) {
+			d.scal[kScalUpset] = t;
+			*d.counter = 0u;
+		}
+	}
+}
+
+template <typename TP>
+__global__ __launch_bounds__(256) void k_read_posterior(Descs descs, uint64_t states, double *out)
+{
+	const LatDesc &d = descs.many ? descs.many[0] : descs.one;
+	gptr<const TP> post = G(static_cast<const TP *>(d.post));
+	const double inv = d.scal[d.inv_in];
+	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+	for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < states; s += stride)
+		out[s] = static_cast<double>(post[s]) * inv;
+}
+
+// k-step lookahead (lattice_api.cpp:116-168), PUBLIC coordinates: for candidate c, the
+// mass of the 2^(m+1) groups formed by the m selected experiments and c.
+template <typename TP>
+__global__ __launch_bounds__(256) void k_lookahead(Descs descs, int n, const uint32_t *selected,
+												   int m, double *scores)
+{
+	const LatDesc &d = descs.many ? descs.many[0] : descs.one;
+	__shared__ double red[16];
+	__shared__ uint32_t sel[16];
+	if (threadIdx.x < (unsigned)m)
+		sel[threadIdx.x] = selected[threadIdx.x];
+	__syncthreads();
+	const uint32_t c = blockIdx.x;
+	const uint32_t states = 1u << n;
+	gptr<const TP> post = G(static_cast<const TP *>(d.post));
+	const double inv = d.scal[d.inv_in];
+	const int groups = 1 << (m + 1);
+	double mass[32];
+#pragma unroll
+	for (int g = 0; g < 32; ++g)
+		mass[g] = 0.0;
+	for (uint32_t s = threadIdx.x; s < states; s += blockDim.x) {
+		// public state s reads internal state reverse_bits(s, n)
+		const uint32_t internal = __brev(s) >> (32 - n);
+		const double x = static_cast<double>(post[internal]) * inv;
+		int gi = 0;
+		for (int i = 0; i < m; ++i)
+			if ((sel[i] & s) == s)
+				gi |= 1 << i;
+		if ((c & s) == s)
+			gi |= 1 << m;
+#pragma unroll
+		for (int g = 0; g < 32; ++g)
+			if (g == gi)
+				mass[g] += x;
+	}
+	const double target = 1.0 / (double)groups;
+	double score = 0.0;
+#pragma unroll
+	for (int g = 0; g < 32; ++g) {
+		if (g < groups) {
+			const double mg = block_sum(mass[g], red);
+			score += fabs(mg - target);
+		}
+	}
+	if (threadIdx.x == 0)
+		scores[c] = score;
+}
+
+// ------------------------------------------------------------------ tree statistics
+
+__device__ __forceinline__ double orig_likelihood(uint32_t e, uint32_t r, uint32_t s, int n, int k,
+												  int dilution, const double *dil)
+{
+	// replicated_lattice.cpp:8-28 on the original (unshrunk) lattice
+	double ret = 1.0;
+	const int len = __popc(e);
+	for (int v = 0; v < k; ++v) {
+		const uint32_t sv = s >> (v * n);
+		const bool rb = (r >> v) & 1u;
+		if (!dilution) {
+			const bool contained = (e & sv) == e;
+			ret *= contained ? (rb ? 0.99 : (1.0 - 0.99)) : (rb ? (1.0 - 0.99) : 0.99);
+		} else {
+			double p1 = 0.0;
+			if (len > 0)
+				p1 = dil[(len - 1) * (n + 1) + (len - __popc(e & sv))];
+			ret *= rb ? p1 : 1.0 - p1;
+		}
+	}
+	return ret;
+}
+
+// One thread per true state: apply_true_state + parse (tree.cpp:70-199) on a tree
+// flattened in preorder. Statistics are accumulated per state and reduced in a fixed order.
+__global__ __launch_bounds__(256) void k_tree_stats(const FlatNode *nodes, int node_count, int n, int k,
+													int dilution, const double *dil, const double *pi,
+													int depth, double branch, double *partials,
+													unsigned *counter, double *out)
+{
+	__shared__ double red[16];
+	const int atoms = n * k;
+	const uint32_t states = 1u << atoms;
+	double st[7] = {0, 0, 0, 0, 0, 0, 0};
+	const uint32_t stride = gridDim.x * blockDim.x;
+	for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < states; t += stride) {
+		double coef = 1.0;
+		for (int a = 0; a < atoms; ++a)
+			coef *= ((t >> a) & 1u) ? (1.0 - pi[a]) : pi[a];
+		coef = coef * 1.0;
+		double pathb[32];
+		double correct = 0.0, incorrect = 0.0, fp = 0.0, fn = 0.0, unclas = 0.0, es = 0.0, et = 0.0;
+		int i = 0;
+		pathb[0] = 1.0;
+		while (i < node_count) {
+			const FlatNode nd = nodes[i];
+			double b;
+			if (i == 0) {
+				b = 1.0;
+			} else {
+				const double cp = pathb[nd.stage - 1] * orig_likelihood(nd.ex, nd.res, t, n, k, dilution, dil);
+				if (!(cp > branch)) {
+					i = nd.subtree_end;
+					continue;
+				}
+				b = cp;
+			}
+			pathb[nd.stage] = b;
+			if (b < branch) {
+				i = nd.subtree_end;
+				continue;
+			}
+			if (nd.nchild == 0) {
+				const int index = nd.stage;
+				const bool classified = __popc(nd.pos | nd.neg) == atoms;
+				if (classified && nd.neg == t) {
+					correct += b * coef;
+				} else if (classified) {
+					incorrect += b * coef;
+					const double tp = (double)__popc(nd.pos), tn = (double)__popc(nd.neg);
+					const double fpv = tp == 0.0 ? 0.0 : __popc((nd.neg ^ t) & t) / tp;
+					const double fnv = tn == 0.0 ? 0.0 : __popc((nd.neg ^ t) & ~t) / tn;
+					fp += fpv * coef * b;
+					fn += fnv * coef * b;
+				} else {
+					unclas = unclas + b;
+				}
+				es = es + ceil((double)index / 1.0) * b;
+				et = et + index * b;
+			}
+			++i;
+		}
+		st[0] += correct;
+		st[1] += incorrect;
+		st[2] += fp;
+		st[3] += fn;
+		st[4] += unclas * coef;
+		st[5] += es * coef;
+		st[6] += et * coef;
+	}
+	double bs[7];
+	for (int q = 0; q < 7; ++q)
+		bs[q] = block_sum(st[q], red);
+	if (threadIdx.x == 0)
+		for (int q = 0; q < 7; ++q)
+			partials[(size_t)blockIdx.x * 7 + q] = bs[q];
+	if (arrive_last(counter, gridDim.x)) {
+		for (int q = 0; q < 7; ++q) {
+			double s = 0.0;
+			for (unsigned i2 = threadIdx.x; i2 < gridDim.x; i2 += blockDim.x)
+				s += partials[(size_t)i2 * 7 + q];
+			s = block_sum(s, red);
+			if (threadIdx.x == 0)
+				out[q] = s;
+		}
+		if (threadIdx.x == 0)
+			*counter = 0u;
+	}
+}
+
+template <typename F>
+void launch_checked(const char *name, F f)
+{
+	f();
+	check_hip(hipGetLastError(), name);
+}
+
+unsigned grid_for(uint64_t work, unsigned per_block, unsigned cap)
+{
+	uint64_t g = (work + per_block - 1) / per_block;
+	if (g < 1)
+		g = 1;
+	if (g > cap)
+		g = cap;
+	return (unsigned)g;
+}
+
+} // namespace
+
+// ------------------------------------------------------------------------- launchers
+
+void launch_prior(const Descs &descs, int count, int atoms, Storage st, const double *pairs,
+				  uint64_t states, hipStream_t stream)
+{
+	const dim3 grid(grid_for(states, 256 * 8, 4096), count);
+	launch_checked("k_prior", [&] {
+		if (st == Storage::fp64)
+			hipLaunchKernelGGL(k_prior<double>, grid, dim3(256), 0, stream, descs, atoms, pairs, states);
+		else
+			hipLaunchKernelGGL(k_prior<float>, grid, dim3(256), 0, stream, descs, atoms, pairs, states);
+	});
+}
+
+void launch_update(const Descs &descs, int count, uint64_t states, int, int, Storage st,
+				   hipStream_t stream)
+{
+	const dim3 grid(grid_for(states, 256 * 16, 2048), count);
+	launch_checked("k_update", [&] {
+		if (st == Storage::fp64)
+			hipLaunchKernelGGL(k_update<double>, grid, dim3(256), 0, stream, descs, states);
+		else
+			hipLaunchKernelGGL(k_update<float>, grid, dim3(256), 0, stream, descs, states);
+	});
+}
+
+void launch_zeta_pass1(const Descs &descs, int count, int n, Storage st, bool fused, int, int,
+					   hipStream_t stream)
+{
+	const dim3 grid(1u << (n - kTileBits), count);
+	launch_checked("k_zeta_pass1", [&] {
+		if (st == Storage::fp64) {
+			if (fused)
+				hipLaunchKernelGGL((k_zeta_pass1<double, double, true>), grid, dim3(kTileThreads), 0, stream, descs);
+			else
+				hipLaunchKernelGGL((k_zeta_pass1<double, double, false>), grid, dim3(kTileThreads), 0, stream, descs);
+		} else {
+			if (fused)
+				hipLaunchKernelGGL((k_zeta_pass1<float, float, true>), grid, dim3(kTileThreads), 0, stream, descs);
+			else
+				hipLaunchKernelGGL((k_zeta_pass1<float, float, false>), grid, dim3(kTileThreads), 0, stream, descs);
+		}
+	});
+}
+
+void launch_zeta_final(const Descs &descs, int count, int n, int, Storage st, int objective,
+					   bool from_posterior, bool scaled, double target, hipStream_t stream)
+{
+	// n == 15: one tile, contiguous, read from the posterior.
+	// n > 15: 2^(n-15) tiles of 2^(30-n) columns by 2^(n-15) rows, read from z.
+	const int logc = from_posterior ? kTileBits : (2 * kTileBits - n);
+	const int q0 = from_posterior ? 0 : logc;
+	const dim3 grid(from_posterior ? 1u : (1u << (n - kTileBits)), count);
+	launch_checked("k_zeta_final", [&] {
+		if (from_posterior) {
+			if (st == Storage::fp64)
+				hipLaunchKernelGGL((k_zeta_final<double, true>), grid, dim3(kTileThreads), 0, stream, descs, logc, q0,
+								   objective, 0, target);
+			else
+				hipLaunchKernelGGL((k_zeta_final<float, true>), grid, dim3(kTileThreads), 0, stream, descs, logc, q0,
+								   objective, 0, target);
+		} else {
+			if (st == Storage::fp64)
+				hipLaunchKernelGGL((k_zeta_final<double, false>), grid, dim3(kTileThreads), 0, stream, descs, logc,
+								   q0, objective, scaled ? 1 : 0, target);
+			else
+				hipLaunchKernelGGL((k_zeta_final<float, false>), grid, dim3(kTileThreads), 0, stream, descs, logc,
+								   q0, objective, scaled ? 1 : 0, target);
+		}
+	});
+}
+
+void launch_zeta_small(const Descs &descs, int count, int n, int, Storage st, int objective, double target,
+					   hipStream_t stream)
+{
+	const dim3 grid(1, count);
+	launch_checked("k_zeta_small", [&] {
+		if (st == Storage::fp64)
+			hipLaunchKernelGGL(k_zeta_small<double>, grid, dim3(1024), 0, stream, descs, n, objective, target);
+		else
+			hipLaunchKernelGGL(k_zeta_small<float>, grid, dim3(1024), 0, stream, descs, n, objective, target);
+	});
+}
+
+void launch_bbpa_score(const Descs &descs, int count, int n, int k, double target, hipStream_t stream)
+{
+	const dim3 grid(grid_for(1ull << n, 256, 1u << 20), count);
+	launch_checked("k_bbpa_score", [&] {
+		hipLaunchKernelGGL(k_bbpa_score, grid, dim3(256), 0, stream, descs, n, k, target);
+	});
+}
+
+void launch_brute_force(const Descs &descs, int count, int n, int k, Storage st, hipStream_t stream)
+{
+	const dim3 grid(1u << n, count);
+	launch_checked("k_brute_force", [&] {
+#define BGT_BF(KK)                                                                                  \
+	if (k == KK) {                                                                                  \
+		if (st == Storage::fp64)                                                                    \
+			hipLaunchKernelGGL((k_brute_force<double, KK>), grid, dim3(256), 0, stream, descs, n); \
+		else                                                                                        \
+			hipLaunchKernelGGL((k_brute_force<float, KK>), grid, dim3(256), 0, stream, descs, n);  \
+		return;                                                                                     \
+	}
+		BGT_BF(1)
+		BGT_BF(2)
+		BGT_BF(3)
+		BGT_BF(4)
+#undef BGT_BF
+		throw Error(2, "brute force supports at most 4 variants");
+	});
+}
+
+void launch_atom_masses(const Descs &descs, int count, int atoms, Storage st, hipStream_t stream)
+{
+	const int cb = atoms < 14 ? atoms : 14;
+	const dim3 grid(1u << (atoms - cb), count);
+	launch_checked("k_atom_masses", [&] {
+		if (st == Storage::fp64)
+			hipLaunchKernelGGL(k_atom_masses<double>, grid, dim3(256), 0, stream, descs, atoms);
+		else
+			hipLaunchKernelGGL(k_atom_masses<float>, grid, dim3(256), 0, stream, descs, atoms);
+	});
+}
+
+void launch_upset_mass(const Descs &descs, int count, int atoms, uint32_t e, Storage st, hipStream_t stream)
+{
+	const int freebits = atoms - __builtin_popcount(e & ((atoms >= 32) ? 0xFFFFFFFFu : ((1u << atoms) - 1u)));
+	const dim3 grid(grid_for(1ull << freebits, 256 * 8, 2048), count);
+	launch_checked("k_upset_mass", [&] {
+		if (st == Storage::fp64)
+			hipLaunchKernelGGL(k_upset_mass<double>, grid, dim3(256), 0, stream, descs, atoms, e);
+		else
+			hipLaunchKernelGGL(k_upset_mass<float>, grid, dim3(256), 0, stream, descs, atoms, e);
+	});
+}
+
+void launch_read_posterior(const Descs &descs, int atoms, Storage st, double *out, hipStream_t stream)
+{
+	const uint64_t states = 1ull << atoms;
+	const dim3 grid(grid_for(states, 256 * 4, 8192));
+	launch_checked("k_read_posterior", [&] {
+		if (st == Storage::fp64)
+			hipLaunchKernelGGL(k_read_posterior<double>, grid, dim3(256), 0, stream, descs, states, out);
+		else
+			hipLaunchKernelGGL(k_read_posterior<float>, grid, dim3(256), 0, stream, descs, states, out);
+	});
+}
+
+void launch_lookahead(const Descs &descs, int n, Storage st, const uint32_t *selected, int m, double *scores,
+					  hipStream_t stream)
+{
+	const dim3 grid(1u << n);
+	launch_checked("k_lookahead", [&] {
+		if (st == Storage::fp64)
+			hipLaunchKernelGGL(k_lookahead<double>, grid, dim3(256), 0, stream, descs, n, selected, m, scores);
+		else
+			hipLaunchKernelGGL(k_lookahead<float>, grid, dim3(256), 0, stream, descs, n, selected, m, scores);
+	});
+}
+
+void launch_tree_stats(const FlatNode *nodes, int node_count, int n, int k, int dilution, const double *dil,
+					   const double *pi, int depth, double branch, double *partials, unsigned *counter, double *out,
+					   hipStream_t stream)
+{
+	const uint64_t states = 1ull << (n * k);
+	const dim3 grid(grid_for(states, 256, 4096));
+	launch_checked("k_tree_stats", [&] {
+		hipLaunchKernelGGL(k_tree_stats, grid, dim3(256), 0, stream, nodes, node_count, n, k, dilution, dil, pi,
+						   depth, branch, partials, counter, out);
+	});
+}
+
+} // namespace bgt_hip

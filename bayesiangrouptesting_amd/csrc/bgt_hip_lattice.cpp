@@ -1,0 +1,648 @@
+// Host orchestration of one device-resident lattice.
+#include "bgt_hip_lattice.hpp"
+
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+
+namespace bgt_hip {
+
+void check_hip(hipError_t e, const char *what)
+{
+	if (e != hipSuccess)
+		throw Error(BGT_HIP_INTERNAL, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+int device_count()
+{
+	int n = 0;
+	if (hipGetDeviceCount(&n) != hipSuccess)
+		return 0;
+	return n;
+}
+
+void require_device()
+{
+	static thread_local int checked = -1;
+	if (checked == 1)
+		return;
+	if (device_count() <= 0)
+		throw Error(BGT_HIP_UNSUPPORTED, "no HIP device is visible");
+	hipDeviceProp_t prop;
+	BGT_HIP_CHECK(hipGetDeviceProperties(&prop, 0));
+	if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+		throw Error(BGT_HIP_UNSUPPORTED, std::string("this provider is built for gfx950, found ") + prop.gcnArchName);
+	checked = 1;
+}
+
+// ------------------------------------------------------------------------ kernel clock
+
+KernelClock::~KernelClock()
+{
+	for (auto &p : pending_) {
+		(void)hipEventDestroy(p.a);
+		(void)hipEventDestroy(p.b);
+	}
+	for (auto e : pool_)
+		(void)hipEventDestroy(e);
+	if (open_)
+		(void)hipEventDestroy(open_);
+}
+
+hipEvent_t KernelClock::take()
+{
+	if (!pool_.empty()) {
+		hipEvent_t e = pool_.back();
+		pool_.pop_back();
+		return e;
+	}
+	hipEvent_t e;
+	BGT_HIP_CHECK(hipEventCreate(&e));
+	return e;
+}
+
+void KernelClock::begin(hipStream_t s)
+{
+	if (!enabled)
+		return;
+	if (pending_.size() >= 4096)
+		resolve();
+	open_ = take();
+	BGT_HIP_CHECK(hipEventRecord(open_, s));
+}
+
+void KernelClock::end(hipStream_t s, const char *name)
+{
+	if (!enabled || !open_)
+		return;
+	hipEvent_t b = take();
+	BGT_HIP_CHECK(hipEventRecord(b, s));
+	pending_.push_back({name, open_, b});
+	open_ = nullptr;
+}
+
+void KernelClock::resolve()
+{
+	if (pending_.empty())
+		return;
+	BGT_HIP_CHECK(hipEventSynchronize(pending_.back().b));
+	for (auto &p : pending_) {
+		float ms = 0.0f;
+		BGT_HIP_CHECK(hipEventElapsedTime(&ms, p.a, p.b));
+		bool found = false;
+		for (auto &e : entries)
+			if (e.name == p.name) {
+				e.total_ms += ms;
+				e.launches += 1;
+				found = true;
+				break;
+			}
+		if (!found)
+			entries.push_back({p.name, ms, 1});
+		pool_.push_back(p.a);
+		pool_.push_back(p.b);
+	}
+	pending_.clear();
+}
+
+void KernelClock::reset()
+{
+	resolve();
+	entries.clear();
+}
+
+namespace {
+
+constexpr double kNegativeResponse = 0.99; // replicated_lattice.cpp:6
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+struct HostBox {
+	SelResult result;
+	double masses[32];
+	double scal[kScalars];
+	double value;
+	double raw;
+};
+
+} // namespace
+
+// ---------------------------------------------------------------------- construction
+
+DeviceLattice::DeviceLattice(bool dilution, int subjects, int variants, const double *prior, Storage storage)
+	: dilution_(dilution), n_(subjects), k_(variants), atoms_(subjects * variants), storage_(storage)
+{
+	if (subjects <= 0 || variants <= 0)
+		throw Error(BGT_HIP_BAD_INPUT, "subjects and variants must be positive");
+	if (variants > kMaxVariants || subjects > kMaxSubjects || atoms_ > kMaxAtoms)
+		throw Error(BGT_HIP_UNSUPPORTED, "lattice larger than 2^30 states or more than 5 variants");
+	if (prior == nullptr)
+		throw Error(BGT_HIP_BAD_INPUT, "prior is required");
+	require_device();
+	states_ = 1ull << atoms_;
+	active_ = (subjects >= 32) ? 0xFFFFFFFFu : ((1u << subjects) - 1u);
+	BGT_HIP_CHECK(hipStreamCreate(&stream_));
+	allocate();
+	std::vector<double> pairs(2 * kMaxAtoms, 0.0);
+	for (int a = 0; a < atoms_; ++a) {
+		pairs[2 * a] = prior[a];
+		pairs[2 * a + 1] = 1.0 - prior[a];
+	}
+	BGT_HIP_CHECK(hipMemcpyAsync(pairs_, pairs.data(), sizeof(double) * pairs.size(), hipMemcpyHostToDevice, stream_));
+	launch_prior(single(), 1, atoms_, storage_, pairs_, states_, stream_);
+	BGT_HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+DeviceLattice::DeviceLattice(const DeviceLattice &parent, bool copy_posterior)
+	: dilution_(parent.dilution_), n_(parent.n_), k_(parent.k_), atoms_(parent.atoms_), states_(parent.states_),
+	  storage_(parent.storage_)
+{
+	BGT_HIP_CHECK(hipStreamCreate(&stream_));
+	allocate();
+	pos_ = parent.pos_;
+	neg_ = parent.neg_;
+	active_ = parent.active_;
+	updates_ = parent.updates_;
+	inv_cur_ = parent.inv_cur_;
+	if (copy_posterior) {
+		// order the copy after the parent's pending work
+		BGT_HIP_CHECK(hipStreamSynchronize(parent.stream_));
+		const size_t eb = storage_ == Storage::fp64 ? 8 : 4;
+		BGT_HIP_CHECK(hipMemcpyAsync(post_, parent.post_, eb * states_, hipMemcpyDeviceToDevice, stream_));
+		BGT_HIP_CHECK(hipMemcpyAsync(scal_, parent.scal_, sizeof(double) * kScalars, hipMemcpyDeviceToDevice, stream_));
+	}
+}
+
+void DeviceLattice::allocate()
+{
+	const size_t eb = storage_ == Storage::fp64 ? 8 : 4;
+	BGT_HIP_CHECK(hipMalloc(&post_, eb * states_));
+	bytes_ += eb * states_;
+	const uint64_t tiles = atoms_ > kTileBits ? (1ull << (atoms_ - kTileBits)) : 1ull;
+	const uint64_t chunks = atoms_ > 14 ? (1ull << (atoms_ - 14)) : 1ull;
+	const uint64_t ec = 1ull << n_;
+	size_t part = std::max<uint64_t>({2048, tiles, (chunks + 1) * kMaxAtoms});
+	size_t tops = std::max<uint64_t>({tiles, ec / 256 + 1, 1});
+	if (n_ <= 16)
+		tops = std::max<uint64_t>(tops, ec);
+	const size_t ftab = k_ >= 2 ? (size_t(1) << k_) * ec : 1;
+	const size_t scores = (k_ == 1 && n_ <= 20) ? ec : 1;
+	size_t off = 0;
+	auto take = [&](size_t bytes) {
+		const size_t o = off;
+		off += align256(bytes);
+		return o;
+	};
+	const size_t o_scal = take(sizeof(double) * kScalars);
+	const size_t o_res = take(sizeof(SelResult));
+	const size_t o_cnt = take(sizeof(unsigned) * 4);
+	const size_t o_mass = take(sizeof(double) * 32);
+	const size_t o_pairs = take(sizeof(double) * 2 * kMaxAtoms);
+	const size_t o_part = take(sizeof(double) * part);
+	const size_t o_tops = take(sizeof(SelResult) * tops);
+	const size_t o_ftab = take(sizeof(double) * ftab);
+	const size_t o_scores = take(sizeof(double) * scores);
+	const size_t o_sel = take(sizeof(uint32_t) * 16);
+	BGT_HIP_CHECK(hipMalloc(reinterpret_cast<void **>(&arena_), off));
+	bytes_ += off;
+	BGT_HIP_CHECK(hipMemsetAsync(arena_, 0, off, stream_));
+	scal_ = reinterpret_cast<double *>(arena_ + o_scal);
+	result_ = reinterpret_cast<SelResult *>(arena_ + o_res);
+	counter_ = reinterpret_cast<unsigned *>(arena_ + o_cnt);
+	masses_ = reinterpret_cast<double *>(arena_ + o_mass);
+	pairs_ = reinterpret_cast<double *>(arena_ + o_pairs);
+	part_ = reinterpret_cast<double *>(arena_ + o_part);
+	tops_ = reinterpret_cast<SelResult *>(arena_ + o_tops);
+	ftab_ = reinterpret_cast<double *>(arena_ + o_ftab);
+	scores_ = reinterpret_cast<double *>(arena_ + o_scores);
+	selected_ = reinterpret_cast<uint32_t *>(arena_ + o_sel);
+	double init[kScalars] = {1.0, 1.0, 1.0, 1.0, 0.0, 0.0, 0.0, 0.0};
+	BGT_HIP_CHECK(hipMemcpyAsync(scal_, init, sizeof(init), hipMemcpyHostToDevice, stream_));
+	BGT_HIP_CHECK(hipHostMalloc(&host_box_, sizeof(HostBox), hipHostMallocDefault));
+	BGT_HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+void DeviceLattice::ensure_z()
+{
+	if (z_ || atoms_ <= kTileBits)
+		return;
+	const size_t eb = storage_ == Storage::fp64 ? 8 : 4;
+	BGT_HIP_CHECK(hipMalloc(&z_, eb * states_));
+	bytes_ += eb * states_;
+}
+
+DeviceLattice::~DeviceLattice()
+{
+	if (stream_)
+		(void)hipStreamSynchronize(stream_);
+	if (post_)
+		(void)hipFree(post_);
+	if (z_)
+		(void)hipFree(z_);
+	if (arena_)
+		(void)hipFree(arena_);
+	if (host_box_)
+		(void)hipHostFree(host_box_);
+	if (stream_)
+		(void)hipStreamDestroy(stream_);
+}
+
+LatDesc DeviceLattice::desc() const
+{
+	LatDesc d;
+	std::memset(&d, 0, sizeof(d));
+	d.post = post_;
+	d.z = z_;
+	d.scal = scal_;
+	d.part = part_;
+	d.tops = tops_;
+	d.result = result_;
+	d.counter = counter_;
+	d.lik = nullptr;
+	d.masses = masses_;
+	d.ftab = ftab_;
+	d.n = n_;
+	d.k = k_;
+	d.atoms = atoms_;
+	d.len = len_;
+	d.e = e_;
+	d.r = r_;
+	d.active = active_;
+	d.dilution = dilution_ ? 1 : 0;
+	d.inv_in = inv_cur_;
+	d.inv_out = inv_cur_ == kScalInv0 ? kScalInv1 : kScalInv0;
+	return d;
+}
+
+Descs DeviceLattice::single() const
+{
+	Descs ds;
+	std::memset(&ds, 0, sizeof(ds));
+	ds.one = desc();
+	std::memcpy(ds.one_lik, lik_, sizeof(lik_));
+	ds.many = nullptr;
+	return ds;
+}
+
+void DeviceLattice::sync() { BGT_HIP_CHECK(hipStreamSynchronize(stream_)); }
+
+// ----------------------------------------------------------------------- likelihoods
+
+void DeviceLattice::fill_lik(uint32_t experiment, uint32_t response, const double *dil)
+{
+	// factor tables for likelihood() in the kernels (replicated_lattice.cpp:8-28)
+	std::fill(std::begin(lik_), std::end(lik_), 0.0);
+	const int len = __builtin_popcount(experiment);
+	for (int v = 0; v < k_; ++v) {
+		const bool rb = (response >> v) & 1u;
+		double *f = lik_ + v * (n_ + 1);
+		if (!dilution_) {
+			f[0] = rb ? kNegativeResponse : (1.0 - kNegativeResponse);
+			f[1] = rb ? (1.0 - kNegativeResponse) : kNegativeResponse;
+		} else {
+			for (int m = 0; m <= len; ++m) {
+				// a pool of size 0 has no row (dilution.cpp indexes row -1); P(r=1 | E=0) := 0
+				const double p1 = len > 0 ? dil[(size_t)(len - 1) * (n_ + 1) + m] : 0.0;
+				f[m] = rb ? p1 : 1.0 - p1;
+			}
+		}
+	}
+	e_ = experiment;
+	r_ = response;
+	len_ = len;
+}
+
+void DeviceLattice::prepare_update(uint32_t experiment, uint32_t response, const double *dil)
+{
+	const uint32_t emask = (n_ >= 32) ? 0xFFFFFFFFu : ((1u << n_) - 1u);
+	if (experiment & ~emask)
+		throw Error(BGT_HIP_BAD_INPUT, "experiment has bits beyond the subject count");
+	if (response >> k_)
+		throw Error(BGT_HIP_BAD_INPUT, "response has bits beyond the variant count");
+	if (dilution_ && dil == nullptr)
+		throw Error(BGT_HIP_BAD_INPUT, "dilution table is required for dilution lattices");
+	fill_lik(experiment, response, dil);
+}
+
+void DeviceLattice::commit_update()
+{
+	inv_cur_ = inv_cur_ == kScalInv0 ? kScalInv1 : kScalInv0;
+	++updates_;
+}
+
+void DeviceLattice::update(uint32_t experiment, uint32_t response, const double *dil)
+{
+	prepare_update(experiment, response, dil);
+	clock_.begin(stream_);
+	launch_update(single(), 1, states_, k_, dilution_, storage_, stream_);
+	clock_.end(stream_, "update");
+	commit_update();
+}
+
+// ------------------------------------------------------------------------- selection
+
+int DeviceLattice::resolve_selector(int selector, int variants)
+{
+	// simulation_helpers.cpp:40-45
+	if (selector == kSelAuto)
+		return variants == 1 ? kSelBha : kSelBbpa;
+	return selector;
+}
+
+void DeviceLattice::check_selector(int selector) const
+{
+	if (selector < kSelAuto || selector > kSelBrute)
+		throw Error(BGT_HIP_BAD_INPUT, "unsupported selector");
+	if (resolve_selector(selector, k_) == kSelBha && k_ != 1)
+		throw Error(BGT_HIP_BAD_INPUT, "Op-BHA experiment selection is only supported for binary lattices");
+	if (selector == kSelBrute && (n_ > 16 || k_ > 4 || n_ + atoms_ > 40))
+		throw Error(BGT_HIP_UNSUPPORTED, "brute-force selection is limited to 16 subjects, 4 variants and 2^40 work");
+}
+
+SelResult DeviceLattice::fetch_result()
+{
+	HostBox *box = static_cast<HostBox *>(host_box_);
+	BGT_HIP_CHECK(hipMemcpyAsync(&box->result, result_, sizeof(SelResult), hipMemcpyDeviceToHost, stream_));
+	BGT_HIP_CHECK(hipStreamSynchronize(stream_));
+	return box->result;
+}
+
+void DeviceLattice::fetch_masses(double *out)
+{
+	HostBox *box = static_cast<HostBox *>(host_box_);
+	BGT_HIP_CHECK(hipMemcpyAsync(box->masses, masses_, sizeof(double) * 32, hipMemcpyDeviceToHost, stream_));
+	BGT_HIP_CHECK(hipStreamSynchronize(stream_));
+	std::memcpy(out, box->masses, sizeof(double) * atoms_);
+}
+
+bgt_hip_selection DeviceLattice::finish_selection(int selector, const SelResult &r, bool)
+{
+	const int sel = resolve_selector(selector, k_);
+	bgt_hip_selection out;
+	std::memset(&out, 0, sizeof(out));
+	const bool found = r.e1 != 0xFFFFFFFFu && r.d1 < (sel == kSelBrute ? INFINITY : 2.0);
+	// reference defaults when nothing beats the initial best: Op-BHA returns 0
+	// (bbpa.cpp:539), BBPA returns invalid_state (lattice.hpp:250), brute force 0.
+	out.experiment = found ? r.e1 : (sel == kSelBbpa ? 0xFFFFFFFFu : 0u);
+	out.objective = r.d1;
+	out.runner_up = r.e2;
+	out.runner_up_objective = r.d2;
+	double post_rel = 0.0, mass_rel = 0.0;
+	error_bound(&post_rel, &mass_rel);
+	const double scale = sel == kSelBha ? 1.0 : double(1 << k_);
+	out.gap_bound = 2.0 * scale * mass_rel;
+	out.near_tie = (r.d2 - r.d1) <= out.gap_bound ? 1 : 0;
+	out.redecided = 0;
+	if (storage_ == Storage::fp32c && out.near_tie && found && r.e2 != 0xFFFFFFFFu && k_ == 1)
+		redecide(out, sel);
+	return out;
+}
+
+double DeviceLattice::exact_objective(uint32_t e, int sel)
+{
+	// FP64 re-evaluation of one candidate's objective from the stored posterior
+	const double m = upset_mass(e);
+	if (sel == kSelBha)
+		return std::fabs(m - 0.5);
+	HostBox *box = static_cast<HostBox *>(host_box_);
+	BGT_HIP_CHECK(hipMemcpyAsync(box->scal, scal_, sizeof(double) * kScalars, hipMemcpyDeviceToHost, stream_));
+	BGT_HIP_CHECK(hipStreamSynchronize(stream_));
+	const double total = box->scal[kScalTotal];
+	return std::fabs(m - 0.5) + std::fabs((total - m) - 0.5);
+}
+
+void DeviceLattice::redecide(bgt_hip_selection &sel, int selector)
+{
+	const double a = exact_objective(sel.experiment, selector);
+	const double b = exact_objective(sel.runner_up, selector);
+	const bool swap = b < a || (b == a && sel.runner_up < sel.experiment);
+	if (swap) {
+		std::swap(sel.experiment, sel.runner_up);
+		sel.objective = b;
+		sel.runner_up_objective = a;
+	} else {
+		sel.objective = a;
+		sel.runner_up_objective = b;
+	}
+	sel.redecided = 1;
+}
+
+bgt_hip_selection DeviceLattice::select(int selector)
+{
+	check_selector(selector);
+	const int sel = resolve_selector(selector, k_);
+	const Descs ds = single();
+	if (sel == kSelBrute) {
+		clock_.begin(stream_);
+		launch_brute_force(ds, 1, n_, k_, storage_, stream_);
+		clock_.end(stream_, "brute_force");
+		return finish_selection(selector, fetch_result(), false);
+	}
+	const int objective = sel == kSelBha ? kObjHalving : (k_ == 1 ? kObjBinaryBbpa : kObjGather);
+	const double target = sel == kSelBha ? 0.5 : 1.0 / double(1 << k_);
+	if (atoms_ <= kSmallBits) {
+		clock_.begin(stream_);
+		launch_zeta_small(ds, 1, atoms_, k_, storage_, objective, target, stream_);
+		clock_.end(stream_, "zeta_small");
+	} else if (atoms_ == kTileBits) {
+		clock_.begin(stream_);
+		launch_zeta_final(ds, 1, atoms_, k_, storage_, objective, true, false, target, stream_);
+		clock_.end(stream_, "zeta_final");
+	} else {
+		ensure_z();
+		const Descs dz = single();
+		clock_.begin(stream_);
+		launch_zeta_pass1(dz, 1, atoms_, storage_, false, k_, dilution_, stream_);
+		clock_.end(stream_, "zeta_pass1");
+		clock_.begin(stream_);
+		launch_zeta_final(dz, 1, atoms_, k_, storage_, objective, false, false, target, stream_);
+		clock_.end(stream_, "zeta_pass2");
+	}
+	if (objective == kObjGather) {
+		clock_.begin(stream_);
+		launch_bbpa_score(ds, 1, n_, k_, target, stream_);
+		clock_.end(stream_, "bbpa_score");
+	}
+	return finish_selection(selector, fetch_result(), false);
+}
+
+// ------------------------------------------------------------------- mass queries
+
+double DeviceLattice::posterior(uint32_t state)
+{
+	if (state >= states_)
+		throw Error(BGT_HIP_BAD_INPUT, "state out of range");
+	HostBox *box = static_cast<HostBox *>(host_box_);
+	const size_t eb = storage_ == Storage::fp64 ? 8 : 4;
+	BGT_HIP_CHECK(hipMemcpyAsync(&box->raw, static_cast<char *>(post_) + eb * state, eb, hipMemcpyDeviceToHost, stream_));
+	BGT_HIP_CHECK(hipMemcpyAsync(box->scal, scal_, sizeof(double) * kScalars, hipMemcpyDeviceToHost, stream_));
+	BGT_HIP_CHECK(hipStreamSynchronize(stream_));
+	double y;
+	if (storage_ == Storage::fp64) {
+		y = box->raw;
+	} else {
+		float f;
+		std::memcpy(&f, &box->raw, sizeof(f));
+		y = f;
+	}
+	return y * box->scal[inv_cur_];
+}
+
+void DeviceLattice::read_posterior(double *out)
+{
+	HostBox *box = static_cast<HostBox *>(host_box_);
+	BGT_HIP_CHECK(hipMemcpyAsync(box->scal, scal_, sizeof(double) * kScalars, hipMemcpyDeviceToHost, stream_));
+	if (storage_ == Storage::fp64) {
+		BGT_HIP_CHECK(hipMemcpyAsync(out, post_, 8 * states_, hipMemcpyDeviceToHost, stream_));
+		BGT_HIP_CHECK(hipStreamSynchronize(stream_));
+		const double inv = box->scal[inv_cur_];
+		for (uint64_t s = 0; s < states_; ++s)
+			out[s] = out[s] * inv;
+	} else {
+		std::vector<float> tmp(states_);
+		BGT_HIP_CHECK(hipMemcpyAsync(tmp.data(), post_, 4 * states_, hipMemcpyDeviceToHost, stream_));
+		BGT_HIP_CHECK(hipStreamSynchronize(stream_));
+		const double inv = box->scal[inv_cur_];
+		for (uint64_t s = 0; s < states_; ++s)
+			out[s] = static_cast<double>(tmp[s]) * inv;
+	}
+}
+
+double DeviceLattice::upset_mass(uint32_t experiment)
+{
+	const uint32_t full = (atoms_ >= 32) ? 0xFFFFFFFFu : ((1u << atoms_) - 1u);
+	if (experiment & ~full)
+		throw Error(BGT_HIP_BAD_INPUT, "experiment has bits beyond the atom count");
+	clock_.begin(stream_);
+	launch_upset_mass(single(), 1, atoms_, experiment, storage_, stream_);
+	clock_.end(stream_, "upset_mass");
+	HostBox *box = static_cast<HostBox *>(host_box_);
+	BGT_HIP_CHECK(hipMemcpyAsync(box->scal, scal_, sizeof(double) * kScalars, hipMemcpyDeviceToHost, stream_));
+	BGT_HIP_CHECK(hipStreamSynchronize(stream_));
+	return box->scal[kScalUpset];
+}
+
+void DeviceLattice::atom_masses(double *out)
+{
+	clock_.begin(stream_);
+	launch_atom_masses(single(), 1, atoms_, storage_, stream_);
+	clock_.end(stream_, "atom_masses");
+	fetch_masses(out);
+}
+
+void DeviceLattice::classify(double up, double lo)
+{
+	// lattice_base.cpp:246-264
+	double m[kMaxAtoms];
+	atom_masses(m);
+	for (int i = 0; i < atoms_; ++i) {
+		const uint32_t placement = 1u << i;
+		if ((pos_ | neg_) & placement)
+			continue;
+		if (m[i] < lo)
+			pos_ |= placement;
+		else if (m[i] > (1 - up))
+			neg_ |= placement;
+	}
+}
+
+std::vector<uint32_t> DeviceLattice::select_experiments(int steps)
+{
+	// lattice_api.cpp:116-168 (PUBLIC coordinates; binary lattices)
+	if (steps < 0)
+		throw Error(BGT_HIP_BAD_INPUT, "k must be non-negative");
+	std::vector<uint32_t> out;
+	if (steps == 0)
+		return out;
+	if (k_ != 1)
+		throw Error(BGT_HIP_BAD_INPUT, "Op-BHA multi-experiment selection is only supported for binary lattices");
+	if (steps > 4 || (steps > 1 && n_ > 20))
+		throw Error(BGT_HIP_UNSUPPORTED, "lookahead is limited to 4 steps and 20 subjects");
+	auto rev = [&](uint32_t v) {
+		uint32_t r = 0;
+		for (int i = 0; i < n_; ++i) {
+			r <<= 1;
+			r |= (v >> i) & 1u;
+		}
+		return r;
+	};
+	out.push_back(rev(select(kSelBha).experiment));
+	std::vector<double> scores(1ull << n_);
+	for (int step = 1; step < steps; ++step) {
+		BGT_HIP_CHECK(hipMemcpyAsync(selected_, out.data(), sizeof(uint32_t) * out.size(), hipMemcpyHostToDevice, stream_));
+		launch_lookahead(single(), n_, storage_, selected_, (int)out.size(), scores_, stream_);
+		BGT_HIP_CHECK(hipMemcpyAsync(scores.data(), scores_, sizeof(double) * scores.size(), hipMemcpyDeviceToHost, stream_));
+		BGT_HIP_CHECK(hipStreamSynchronize(stream_));
+		double best = std::numeric_limits<double>::max();
+		uint32_t best_c = 0;
+		for (uint32_t c = 1; c < (1u << n_); ++c)
+			if (scores[c] < best) {
+				best = scores[c];
+				best_c = c;
+			}
+		out.push_back(best_c);
+	}
+	return out;
+}
+
+// ------------------------------------------------------------------------ fused round
+
+bgt_hip_selection DeviceLattice::round(uint32_t experiment, uint32_t response, const double *dil, double up,
+									   double lo, double *masses_out)
+{
+	if (k_ != 1)
+		throw Error(BGT_HIP_BAD_INPUT, "the fused round is defined for binary lattices");
+	if (atoms_ <= kTileBits) {
+		update(experiment, response, dil);
+		classify(up, lo);
+		if (masses_out)
+			fetch_masses(masses_out);
+		return select(kSelBha);
+	}
+	prepare_update(experiment, response, dil);
+	ensure_z();
+	const Descs ds = single();
+	clock_.begin(stream_);
+	launch_zeta_pass1(ds, 1, atoms_, storage_, true, k_, dilution_, stream_);
+	clock_.end(stream_, "round_pass1");
+	clock_.begin(stream_);
+	launch_zeta_final(ds, 1, atoms_, k_, storage_, kObjHalving, false, true, 0.5, stream_);
+	clock_.end(stream_, "round_pass2");
+	commit_update();
+	const SelResult r = fetch_result();
+	double m[32];
+	fetch_masses(m);
+	for (int i = 0; i < atoms_; ++i) {
+		const uint32_t placement = 1u << i;
+		if ((pos_ | neg_) & placement)
+			continue;
+		if (m[i] < lo)
+			pos_ |= placement;
+		else if (m[i] > (1 - up))
+			neg_ |= placement;
+	}
+	if (masses_out)
+		std::memcpy(masses_out, m, sizeof(double) * atoms_);
+	return finish_selection(kSelBha, r, true);
+}
+
+void DeviceLattice::error_bound(double *posterior_rel, double *mass_rel) const
+{
+	// Relative error of a stored posterior entry after `updates_` updates, against exact
+	// arithmetic: one rounding for the prior, two per update (x * L, then * 1/D), plus the
+	// common-scale error of the summed denominator. Masses add the transform's roundings.
+	const double u64 = std::ldexp(1.0, -53);
+	if (storage_ == Storage::fp64) {
+		*posterior_rel = (2.0 * updates_ + 1.0 + atoms_) * u64;
+		*mass_rel = *posterior_rel + 2.0 * atoms_ * u64;
+	} else {
+		const double u32 = std::ldexp(1.0, -24);
+		*posterior_rel = (2.0 * updates_ + 2.0) * u32;
+		*mass_rel = *posterior_rel + 2.0 * u32;
+	}
+}
+
+} // namespace bgt_hip

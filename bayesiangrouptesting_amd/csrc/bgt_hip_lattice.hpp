@@ -1,0 +1,146 @@
+// Device-resident lattice: the HIP counterpart of model::Lattice for one replicated lattice
+// (src/core/model/lattice_base.cpp, replicated_lattice.cpp, src/kernels/cpu/bbpa.cpp).
+#pragma once
+
+#include "bgt_hip.h"
+#include "bgt_hip_internal.hpp"
+
+#include <string>
+#include <vector>
+
+namespace bgt_hip {
+
+// Per-kernel device time from HIP events recorded on the lattice's stream around each
+// launch. Recording never blocks the host; resolve() waits once and folds the pairs in.
+struct KernelClock {
+	bool enabled = false;
+	struct Entry {
+		std::string name;
+		double total_ms = 0.0;
+		int launches = 0;
+	};
+	std::vector<Entry> entries;
+	void begin(hipStream_t s);
+	void end(hipStream_t s, const char *name);
+	void resolve();
+	void reset();
+	~KernelClock();
+
+private:
+	struct Pending {
+		std::string name;
+		hipEvent_t a, b;
+	};
+	std::vector<Pending> pending_;
+	std::vector<hipEvent_t> pool_;
+	hipEvent_t open_ = nullptr;
+	hipEvent_t take();
+};
+
+class DeviceLattice {
+public:
+	DeviceLattice(bool dilution, int subjects, int variants, const double *prior, Storage storage);
+	~DeviceLattice();
+	DeviceLattice(const DeviceLattice &) = delete;
+	DeviceLattice &operator=(const DeviceLattice &) = delete;
+
+	// Copy of `parent`'s posterior (same shape); the child's classification masks start
+	// from the parent's. Used by the tree builder for out-of-place updates.
+	DeviceLattice(const DeviceLattice &parent, bool copy_posterior);
+
+	int subjects() const { return n_; }
+	int variants() const { return k_; }
+	int atoms() const { return atoms_; }
+	bool dilution() const { return dilution_; }
+	Storage storage() const { return storage_; }
+	uint64_t states() const { return states_; }
+	uint64_t device_bytes() const { return bytes_; }
+	hipStream_t stream() const { return stream_; }
+
+	void update(uint32_t experiment, uint32_t response, const double *dilution_flat);
+	bgt_hip_selection select(int selector);
+	double posterior(uint32_t state);
+	void read_posterior(double *out);
+	double upset_mass(uint32_t experiment);
+	void atom_masses(double *out);
+	void classify(double threshold_up, double threshold_lo);
+	std::vector<uint32_t> select_experiments(int steps);
+	bgt_hip_selection round(uint32_t experiment, uint32_t response, const double *dilution_flat,
+							double threshold_up, double threshold_lo, double *masses_out);
+
+	uint32_t positive_mask() const { return pos_; }
+	uint32_t negative_mask() const { return neg_; }
+	void set_masks(uint32_t pos, uint32_t neg) { pos_ = pos; neg_ = neg; }
+	void set_active(uint32_t active) { active_ = active; }
+	uint32_t active() const { return active_; }
+	int updates_applied() const { return updates_; }
+	void error_bound(double *posterior_rel, double *mass_rel) const;
+
+	KernelClock &clock() { return clock_; }
+
+	// descriptor of the current state (for batched launches)
+	LatDesc desc() const;
+	const double *lik_host() const { return lik_; }
+	void prepare_update(uint32_t experiment, uint32_t response, const double *dilution_flat);
+	void commit_update();
+	void sync();
+	SelResult fetch_result();
+	bgt_hip_selection finish_selection(int selector, const SelResult &r, bool scaled);
+	void fetch_masses(double *out);
+
+	// parameters for the selection launchers
+	static int resolve_selector(int selector, int variants);
+
+private:
+	Descs single() const;
+	void allocate();
+	void check_selector(int selector) const;
+	void ensure_z();
+	void fill_lik(uint32_t experiment, uint32_t response, const double *dilution_flat);
+	void redecide(bgt_hip_selection &sel, int selector);
+	double exact_objective(uint32_t e, int selector);
+
+	bool dilution_;
+	int n_, k_, atoms_;
+	uint64_t states_;
+	Storage storage_;
+	hipStream_t stream_ = nullptr;
+	uint64_t bytes_ = 0;
+
+	void *post_ = nullptr;
+	void *z_ = nullptr;
+	char *arena_ = nullptr; // scalars, results, partials, tables
+	double *scal_ = nullptr;
+	double *part_ = nullptr;
+	SelResult *tops_ = nullptr;
+	SelResult *result_ = nullptr;
+	unsigned *counter_ = nullptr;
+	double *masses_ = nullptr;
+	double *ftab_ = nullptr;
+	double *pairs_ = nullptr;
+	double *scores_ = nullptr;
+	uint32_t *selected_ = nullptr;
+
+	// pinned host mailbox for results
+	void *host_box_ = nullptr;
+
+	double lik_[kLikCapacity] = {};
+	uint32_t e_ = 0, r_ = 0;
+	int len_ = 0;
+	int inv_cur_ = kScalInv0;
+	uint32_t pos_ = 0, neg_ = 0;
+	uint32_t active_ = 0;
+	int updates_ = 0;
+	KernelClock clock_;
+};
+
+// Host-side selector values (bgt_hip.h)
+constexpr int kSelAuto = BGT_HIP_SELECT_AUTO;
+constexpr int kSelBha = BGT_HIP_SELECT_OP_BHA;
+constexpr int kSelBbpa = BGT_HIP_SELECT_OP_BBPA;
+constexpr int kSelBrute = BGT_HIP_SELECT_BRUTE_FORCE;
+
+void require_device();
+int device_count();
+
+} // namespace bgt_hip

@@ -1,0 +1,196 @@
+/*
+ * bgt_hip.h -- C ABI of the MI355X (gfx950) HIP provider for BayesianGroupTesting.
+ *
+ * Plain C types only: no exceptions, no C++ or torch types cross this boundary.
+ * Every entry point returns an int status:
+ *   BGT_HIP_OK            0  success
+ *   BGT_HIP_BAD_INPUT     1  invalid argument (also: missing dilution table)
+ *   BGT_HIP_UNSUPPORTED   2  shape/selector not supported, or no gfx950 device
+ *   BGT_HIP_NO_BACKEND    3  multi-rank entry points (this provider is single GPU)
+ *   BGT_HIP_INTERNAL      4  HIP runtime failure; see bgt_hip_last_error()
+ * These are the reference CUDA provider's codes (src/kernels/cuda/cuda_provider.cu:2384-2446).
+ *
+ * Coordinates at this ABI are the reference's *internal* ones: atom a = v*n + i
+ * (variant-major), state bit a == 1 means atom a is negative, prior[a] = P(bit a == 0).
+ * The public bit reversal for binary lattices (src/core/lattice_api.cpp:17-26) is the
+ * caller's job, exactly as bgt::Lattice does it above model::Lattice.
+ *
+ * Dilution tables are passed flat: subjects x (subjects + 1) doubles, row s-1 holding
+ * pool size s -- the storage of bgt::DilutionTable (include/bgt/lattice.hpp:17-32,
+ * src/core/model/dilution.cpp:44-67) without the row-pointer array.
+ *
+ * Threading: one host thread per lattice handle at a time; distinct handles may be
+ * used from distinct threads. The library holds no mutable global state other than
+ * the per-thread last-error string.
+ */
+#ifndef BGT_HIP_H
+#define BGT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BGT_HIP_OK 0
+#define BGT_HIP_BAD_INPUT 1
+#define BGT_HIP_UNSUPPORTED 2
+#define BGT_HIP_NO_BACKEND 3
+#define BGT_HIP_INTERNAL 4
+
+/* selectors: numbering of bgt::SelectorType (include/bgt/types.hpp:89-94) plus brute_force */
+#define BGT_HIP_SELECT_AUTO 0
+#define BGT_HIP_SELECT_OP_BHA 1
+#define BGT_HIP_SELECT_OP_BBPA 2
+#define BGT_HIP_SELECT_BRUTE_FORCE 3
+
+/* storage forms */
+#define BGT_HIP_FP64 0  /* plain: FP64 posterior and FP64 transform intermediate */
+#define BGT_HIP_FP32C 1 /* FP32 posterior and intermediate; FP64 accumulation on chip */
+
+/* same layout as bgt::TreeStats (include/bgt/types.hpp:109-119) */
+typedef struct bgt_hip_tree_stats {
+	int total_leaves;
+	double correct_probability;
+	double incorrect_probability;
+	double false_positive_probability;
+	double false_negative_probability;
+	double unclassified_probability;
+	double expected_stages;
+	double expected_tests;
+} bgt_hip_tree_stats;
+
+/* Result of one selection. objective is |M(E) - 1/2| for Op-BHA and the L1 bucket
+ * distance for Op-BBPA / brute force. gap = runner_up_objective - objective. When
+ * gap <= gap_bound the pick is rounding-determined (a near tie) and near_tie is 1. */
+typedef struct bgt_hip_selection {
+	uint32_t experiment;
+	uint32_t runner_up;
+	double objective;
+	double runner_up_objective;
+	double gap_bound;
+	int near_tie;
+	int redecided; /* FP32C: 1 when the pick was re-decided in FP64 */
+} bgt_hip_selection;
+
+/* ---- provider entry points: counterparts of include/bgt/detail/cuda_provider.hpp:5-24 ---- */
+
+/* replaces bgt_cuda_provider_available (cuda_provider.hpp:5) */
+int bgt_hip_provider_available(void);
+
+/* replaces bgt_cuda_provider_run (cuda_provider.hpp:6-9); dilution is flat (see above) */
+int bgt_hip_provider_run(int use_dilution, int subjects, int variants, const double *prior,
+						 int search_depth, double threshold_up, double threshold_lo,
+						 double branch_threshold, int selector, const double *dilution,
+						 bgt_hip_tree_stats *stats);
+
+/* run_simulation with the storage form chosen (FP64 plain or FP32C) */
+int bgt_hip_provider_run_precision(int use_dilution, int subjects, int variants,
+								   const double *prior, int search_depth, double threshold_up,
+								   double threshold_lo, double branch_threshold, int selector,
+								   const double *dilution, int precision,
+								   bgt_hip_tree_stats *stats);
+
+/* replaces bgt_cuda_provider_run_distributed (cuda_provider.hpp:10-13): this provider is
+ * single-GPU and returns BGT_HIP_NO_BACKEND */
+int bgt_hip_provider_run_distributed(int use_dilution, int subjects, int variants,
+									 const double *prior, int search_depth, double threshold_up,
+									 double threshold_lo, double branch_threshold, int selector,
+									 int enable_collectives, const double *dilution,
+									 bgt_hip_tree_stats *stats);
+
+/* replaces bgt_cuda_provider_benchmark_select (cuda_provider.hpp:14-24) */
+int bgt_hip_provider_benchmark_select(int subjects, int variants, const double *prior,
+									  int selector, int iterations, int warmup,
+									  uint32_t *candidate, double *init_seconds,
+									  double *mean_seconds, double *min_seconds,
+									  double *max_seconds, double *stddev_seconds);
+
+/* ---- lattice session: the device-resident counterpart of bgt::Lattice / model::Lattice ---- */
+
+typedef struct bgt_hip_lattice bgt_hip_lattice;
+
+/* replaces model::create_lattice + Lattice::Lattice (lattice_api.cpp:66-77,
+ * lattice_base.cpp:71-79): allocates the posterior on the GPU and writes the prior. */
+int bgt_hip_lattice_create(int use_dilution, int subjects, int variants, const double *prior,
+						   int precision, bgt_hip_lattice **out);
+int bgt_hip_lattice_destroy(bgt_hip_lattice *lattice);
+
+/* replaces Lattice::update / update_probs_in_place (lattice_api.cpp:170-174,
+ * lattice_base.cpp:230-244). Asynchronous: returns before the GPU finishes. */
+int bgt_hip_lattice_update(bgt_hip_lattice *lattice, uint32_t experiment, uint32_t response,
+						   const double *dilution);
+
+/* replaces Lattice::select_experiment (lattice_api.cpp:108-114, bbpa.cpp:861-871);
+ * selector BGT_HIP_SELECT_BRUTE_FORCE is the explicit sweep of
+ * benchmarks/experiment_selection_benchmark.cpp:214-247 */
+int bgt_hip_lattice_select(bgt_hip_lattice *lattice, int selector, bgt_hip_selection *out);
+
+/* replaces Lattice::posterior_probability (lattice_api.cpp:89-93, lattice_base.cpp:139-142) */
+int bgt_hip_lattice_posterior(bgt_hip_lattice *lattice, uint32_t state, double *out);
+
+/* bulk read of the normalised posterior (count must equal 2^(subjects*variants)) */
+int bgt_hip_lattice_read_posterior(bgt_hip_lattice *lattice, double *out, uint64_t count);
+
+/* replaces Lattice::upset_probability_mass / get_prob_mass (lattice_api.cpp:95-99,
+ * lattice_base.cpp:318-351) */
+int bgt_hip_lattice_upset_mass(bgt_hip_lattice *lattice, uint32_t experiment, double *out);
+
+/* atom marginals m_a = sum_{S : bit a of S} x_S (compute_atom_masses, lattice_base.cpp:25-41);
+ * out holds subjects*variants doubles */
+int bgt_hip_lattice_atom_masses(bgt_hip_lattice *lattice, double *out);
+
+/* replaces Lattice::update_classification (lattice_api.cpp:176-179, lattice_base.cpp:246-264);
+ * the masks are sticky and kept by the handle */
+int bgt_hip_lattice_classify(bgt_hip_lattice *lattice, double threshold_up, double threshold_lo,
+							 uint32_t *positive_mask, uint32_t *negative_mask);
+
+/* replaces Lattice::positive_classification_mask / negative_classification_mask
+ * (lattice_api.cpp:181-191) in internal coordinates; reads the handle's sticky masks */
+int bgt_hip_lattice_masks(bgt_hip_lattice *lattice, uint32_t *positive_mask, uint32_t *negative_mask);
+
+/* k-step Op-BHA lookahead (lattice_api.cpp:116-168) in the reference's PUBLIC coordinates
+ * (binary lattices only) */
+int bgt_hip_lattice_select_experiments(bgt_hip_lattice *lattice, int steps, uint32_t *out);
+
+/* One fused round of the adaptive loop: update(experiment, response), atom marginals of the
+ * updated posterior and classification, and the next Op-BHA selection, from two HBM passes.
+ * Binary lattices only. Masses of the updated posterior go to masses_out (may be NULL). */
+int bgt_hip_lattice_round(bgt_hip_lattice *lattice, uint32_t experiment, uint32_t response,
+						  const double *dilution, double threshold_up, double threshold_lo,
+						  bgt_hip_selection *next, uint32_t *positive_mask,
+						  uint32_t *negative_mask, double *masses_out);
+
+/* Stated error bounds of the storage form (relative, per posterior entry and per mass),
+ * accumulated over the updates applied so far. Zero-width for nothing; FP64 reports its
+ * own rounding bound. */
+int bgt_hip_lattice_error_bound(bgt_hip_lattice *lattice, double *posterior_rel,
+								double *mass_rel);
+
+/* Average device time (ms) per launch of each kernel class since the last reset, measured
+ * with HIP events on the lattice's stream. names is a ';'-separated list. */
+int bgt_hip_lattice_kernel_times(bgt_hip_lattice *lattice, int enable, double *ms_out,
+								 int *launches_out, int capacity, char *names, int names_len);
+
+int bgt_hip_lattice_info(bgt_hip_lattice *lattice, int *subjects, int *variants, int *precision,
+						 uint64_t *device_bytes);
+
+/* ---- batches and serving ---- */
+
+/* Apply one (experiment, response) update to each lattice, then select with `selector`
+ * on each, in batched launches grouped by shape. Results identical to per-lattice calls. */
+int bgt_hip_batch_step(bgt_hip_lattice **lattices, const uint32_t *experiments,
+					   const uint32_t *responses, const double *const *dilutions, int count,
+					   int selector, bgt_hip_selection *out);
+
+/* ---- misc ---- */
+const char *bgt_hip_last_error(void);
+int bgt_hip_device_synchronize(void);
+int bgt_hip_device_name(char *out, int len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,102 @@
+"""The compensated FP32 storage form and the fused adaptive round, against the FP64 oracle.
+
+FP32C bar: posterior entries within 1e-6 relative of the FP64 oracle (entries above
+1e-30 of the total; smaller ones are below FP32's normal range), masses within 1e-6,
+and the same experiment picked unless the pick is a reported near tie.
+Fused round bar: the FP64 plain form's bar (selection identical unless a near tie,
+masses 1e-12)."""
+import numpy as np
+import pytest
+
+import bayesiangrouptesting_amd as bgt
+from bayesiangrouptesting_amd import workload
+from oracle import pyoracle as orc
+from tests.gpu_helpers import rel_err, seeded_prior
+
+pytestmark = pytest.mark.gpu
+ND = bgt.LatticeType.replicated_non_dilution
+DL = bgt.LatticeType.replicated_dilution
+
+
+def adaptive_trace(n, dilution, seed, rounds, lo=0.005, hi=0.05):
+    """Oracle run of the adaptive loop: select -> sampled response -> update -> classify."""
+    pi = seeded_prior(seed, n, lo, hi)
+    ref = orc.prior(pi)
+    dil = orc.dilution_table(n, 0.05, 0.9) if dilution else None
+    truth = workload.true_state(seed, list(pi))
+    sampler = workload.ResponseSampler(seed, n, 1, truth, list(dil) if dil is not None else None)
+    trace = []
+    e = orc.op_bha(ref, n)[0]
+    pos = neg = 0
+    for _ in range(rounds):
+        r = sampler.draw(e)
+        orc.update_in_place(ref, dilution, n, 1, e, r, dil)
+        m = orc.atom_masses(ref, n)
+        pos, neg = orc.classify(m, 0.01, 0.01, pos, neg)
+        nxt, upset = orc.op_bha(ref, n)
+        trace.append(dict(e=e, r=r, post=ref.copy(), masses=m, pos=pos, neg=neg, next=nxt,
+                          obj=abs(upset[nxt] - 0.5)))
+        e = nxt
+    return pi, dil, trace
+
+
+@pytest.mark.parametrize("n,dilution", [(14, False), (16, True), (20, False), (22, True)])
+def test_fused_round_fp64_matches_oracle(n, dilution):
+    pi, dil, trace = adaptive_trace(n, dilution, 40 + n, 6)
+    lat = bgt.Lattice(DL if dilution else ND, n, pi)
+    t = bgt.DilutionTable(n, 0.05, 0.9) if dilution else None
+    masses = np.empty(n)
+    for step in trace:
+        e_pub = orc.reverse_bits(step["e"], n)
+        sel = lat.round(e_pub, step["r"], 0.01, 0.01, t, masses)
+        assert np.max(np.abs(masses - step["masses"])) <= 1e-12
+        assert rel_err(lat.posterior(), step["post"], floor=1e-300) <= 1e-12
+        got = orc.reverse_bits(sel.experiment, n)
+        if got != step["next"]:
+            assert sel.near_tie, (got, step["next"], sel)
+        assert abs(sel.objective - step["obj"]) <= 1e-11
+        assert orc.reverse_bits(lat.negative_classification_mask(), n) == step["neg"]
+        assert orc.reverse_bits(lat.positive_classification_mask(), n) == step["pos"]
+
+
+@pytest.mark.parametrize("n,dilution", [(16, False), (20, True), (22, False)])
+def test_fp32c_round_errors_within_1e6(n, dilution):
+    pi, dil, trace = adaptive_trace(n, dilution, 90 + n, 8)
+    lat = bgt.Lattice(DL if dilution else ND, n, pi, precision=bgt.Precision.float32_compensated)
+    t = bgt.DilutionTable(n, 0.05, 0.9) if dilution else None
+    masses = np.empty(n)
+    for i, step in enumerate(trace):
+        e_pub = orc.reverse_bits(step["e"], n)
+        sel = lat.round(e_pub, step["r"], 0.01, 0.01, t, masses)
+        post = lat.posterior()
+        ref = step["post"]
+        big = ref >= 1e-30 * ref.sum()
+        assert rel_err(post[big], ref[big]) <= 1e-6, f"round {i}"
+        assert np.max(np.abs(masses - step["masses"]) / np.maximum(step["masses"], 1e-300)) <= 1e-6
+        got = orc.reverse_bits(sel.experiment, n)
+        if got != step["next"]:
+            assert sel.near_tie, (i, got, step["next"], sel)
+            break  # the trajectories legitimately diverge after a near tie
+        bound_post, bound_mass = lat.error_bound()
+        assert bound_mass <= 1e-6
+
+
+@pytest.mark.parametrize("n,k", [(12, 1), (18, 1), (6, 2), (5, 3)])
+def test_fp32c_lattice_api_against_fp64(n, k):
+    pi = seeded_prior(7 + n, n * k, 0.02, 0.2)
+    a = bgt.Lattice(ND, n, pi, variants=k)
+    b = bgt.Lattice(ND, n, pi, variants=k, precision=bgt.Precision.float32_compensated)
+    g = workload.SplitMix64(n)
+    for _ in range(4):
+        e = g.below((1 << n) - 1) + 1
+        r = g.below(1 << k)
+        a.update(e, r)
+        b.update(e, r)
+        pa, pb = a.posterior(), b.posterior()
+        big = pa >= 1e-30
+        assert rel_err(pb[big], pa[big]) <= 1e-6
+        assert np.max(np.abs(a.atom_masses() - b.atom_masses())) <= 1e-6
+        sa = a.select()
+        sb = b.select()
+        if sa.experiment != sb.experiment:
+            assert sb.near_tie or sa.near_tie

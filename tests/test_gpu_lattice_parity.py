@@ -1,0 +1,179 @@
+"""GPU parity of the lattice session (C ABI) against the CPU restatement.
+
+Tolerances (the bar for the FP64 plain form):
+  * prior: bit-identical (same product order);
+  * posteriors and masses: 1e-12 relative; the GPU's denominator is a tree sum and the
+    reference's a serial sum, so posteriors are compared after removing the common scale
+    as well as raw;
+  * selected experiments and classification masks: identical whenever the top-two gap
+    exceeds the reported gap_bound (near ties are reported, not asserted).
+"""
+import numpy as np
+import pytest
+
+import bayesiangrouptesting_amd as bgt
+from oracle import pyoracle as orc
+from tests.gpu_helpers import TOL, op_sequence, oracle_objective, rel_err, seeded_prior
+
+pytestmark = pytest.mark.gpu
+
+ND = bgt.LatticeType.replicated_non_dilution
+DL = bgt.LatticeType.replicated_dilution
+
+
+def make(n, k, dilution, seed, precision=bgt.Precision.float64, lo=0.01, hi=0.15):
+    pi = seeded_prior(seed, n * k, lo, hi)
+    lat = bgt.Lattice(DL if dilution else ND, n, pi, variants=k, precision=precision)
+    return lat, pi
+
+
+def test_device_present():
+    assert bgt.hip_available()
+    assert bgt.Provider.hip in bgt.available_providers()
+
+
+@pytest.mark.parametrize("n,k", [(1, 1), (4, 1), (12, 1), (14, 1), (15, 1), (16, 1), (20, 1), (3, 2), (8, 2),
+                                 (5, 3), (8, 3), (2, 5)])
+def test_prior_bit_identical(n, k):
+    lat, pi = make(n, k, False, 100 + n * k)
+    np.testing.assert_array_equal(lat.posterior(), orc.prior(pi))
+
+
+SHAPES = [(3, 1, False), (10, 1, True), (14, 1, False), (15, 1, True), (16, 1, False), (18, 1, True),
+          (20, 1, False), (22, 1, True), (3, 2, True), (6, 2, False), (4, 3, True), (5, 3, False), (3, 4, False)]
+
+
+@pytest.mark.parametrize("n,k,dilution", SHAPES)
+def test_update_select_masses_sequence(n, k, dilution):
+    seed = 1000 + 31 * n + 7 * k + int(dilution)
+    lat, pi = make(n, k, dilution, seed, lo=0.02, hi=0.3)
+    dil = bgt.DilutionTable(n, 0.05, 0.9) if dilution else None
+    ref = orc.prior(pi)
+    atoms = n * k
+    selectors = [1, 2] if k == 1 else [2]
+    if n <= 10 and n + atoms <= 30:
+        selectors.append(3)
+    for step, (e, r) in enumerate(op_sequence(seed, n, k, 4)):
+        # public coordinates for k=1: flip once for the oracle (internal)
+        e_int = orc.reverse_bits(e, n) if k == 1 else e
+        lat.update(e, r, dil)
+        den = orc.update_in_place(ref, dilution, n, k, e_int, r, dil.values if dil else None)
+        got = lat.posterior()
+        assert rel_err(got, ref, floor=1e-300) <= TOL, f"posterior step {step}"
+        scale = got.sum() / ref.sum()
+        assert rel_err(got / scale, ref, floor=1e-300) <= TOL
+        # atom marginals and upset masses
+        m_gpu = lat.atom_masses()
+        m_ref = orc.atom_masses(ref, atoms)
+        assert np.max(np.abs(m_gpu - m_ref)) <= TOL
+        for probe in (0, 1, e):
+            probe_pub = probe
+            probe_int = orc.reverse_bits(probe, n) if k == 1 else probe
+            assert abs(lat.upset_probability_mass(probe_pub) - orc.upset_mass(ref, atoms, probe_int)) <= TOL
+        for selector in selectors:
+            sel = lat.select(bgt.SelectorType(selector))
+            e_gpu = orc.reverse_bits(sel.experiment, n) if k == 1 else sel.experiment
+            e_ref = orc.select(ref, n, k, selector)
+            obj_ref = oracle_objective(ref, n, k, 1 if selector == 1 else 2, e_ref)
+            assert abs(sel.objective - obj_ref) <= 1e-11, (selector, sel.objective, obj_ref)
+            if e_gpu != e_ref:
+                assert sel.near_tie or (sel.runner_up_objective - sel.objective) <= 1e-11, (selector, e_gpu, e_ref)
+        assert den > 0
+
+
+@pytest.mark.parametrize("n,k,dilution", [(6, 1, False), (12, 1, True), (16, 1, False), (4, 2, True), (4, 3, False)])
+def test_classification_masks_match(n, k, dilution):
+    seed = 77 + n
+    lat, pi = make(n, k, dilution, seed, lo=0.005, hi=0.05)
+    dil = bgt.DilutionTable(n, 0.05, 0.9) if dilution else None
+    ref = orc.prior(pi)
+    pos = neg = 0
+    for e, r in op_sequence(seed, n, k, 6):
+        e_int = orc.reverse_bits(e, n) if k == 1 else e
+        lat.update(e, r, dil)
+        orc.update_in_place(ref, dilution, n, k, e_int, r, dil.values if dil else None)
+        lat.update_classification(0.01, 0.01)
+        pos, neg = orc.classify(orc.atom_masses(ref, n * k), 0.01, 0.01, pos, neg)
+        flip = (lambda v: orc.reverse_bits(v, n)) if k == 1 else (lambda v: v)
+        assert lat.positive_classification_mask() == flip(pos)
+        assert lat.negative_classification_mask() == flip(neg)
+        assert lat.is_classified() == (bin(pos | neg).count("1") == n * k)
+
+
+def test_lattice_api_kats_through_provider(kats):
+    k = kats["lattice_api"]
+    lat = bgt.Lattice(ND, 2, k["prior"])
+    for s, want in enumerate(k["public_prior_posterior"]):
+        assert abs(lat.posterior_probability(s) - want) <= TOL
+    for e, want in k["public_upset_mass"].items():
+        assert abs(lat.upset_probability_mass(int(e)) - want) <= TOL
+    assert lat.select_experiment(bgt.SelectorType.auto_select) == 3
+    assert lat.select_experiment(bgt.SelectorType.op_bha) == 3
+    assert lat.select_experiment(bgt.SelectorType.op_bbpa) == 3
+    assert lat.select_experiments(2, bgt.SelectorType.op_bha) == k["public_lookahead_op_bha_2"]
+    assert lat.select_experiments(1, bgt.SelectorType.op_bbpa) == [3]
+    lat.update(1, 1)
+    den = 0.60 * 0.99 + 0.40 * 0.01
+    for s, (a, b) in enumerate(k["public_update"]["numerators"]):
+        assert abs(lat.posterior_probability(s) - a * b / den) <= TOL
+    lat.update_classification(0.01, 0.01)
+    assert lat.positive_classification_mask() == 0
+    assert lat.negative_classification_mask() == 1
+    assert lat.is_classified() is False
+    m = k["multinomial"]
+    ml = bgt.Lattice(ND, m["subjects"], m["prior"], variants=m["variants"])
+    assert ml.select_experiment(bgt.SelectorType.op_bbpa) == ml.select_experiment(bgt.SelectorType.brute_force)
+
+
+def test_dilution_response_probability_kat(kats):
+    k = kats["dilution"]
+    lat = bgt.Lattice(DL, 2, k["prior"])
+    t = bgt.DilutionTable(2, k["alpha"], k["h"])
+    from tests.conftest import expr
+    for e, r, s, want in k["public_response_probability"]:
+        assert abs(lat.response_probability(e, r, s, t) - expr(want)) <= TOL
+
+
+@pytest.mark.parametrize("n", [8, 12])
+def test_lookahead_matches_oracle(n):
+    lat, pi = make(n, 1, False, 5 + n)
+    ref = orc.prior(pi)
+    assert lat.select_experiments(3) == orc.select_experiments(ref, n, 3)
+
+
+def test_selection_is_deterministic_and_handles_are_independent():
+    a, pi = make(18, 1, True, 9)
+    b, _ = make(18, 1, True, 9)
+    c, _ = make(12, 2, False, 4)  # a different shape in between must not disturb a/b
+    dil = bgt.DilutionTable(18, 0.05, 0.9)
+    for e, r in op_sequence(9, 18, 1, 3):
+        a.update(e, r, dil)
+        c.update(5, 1)
+        b.update(e, r, dil)
+    np.testing.assert_array_equal(a.posterior(), b.posterior())
+    sa, sb = a.select(), b.select()
+    assert (sa.experiment, sa.objective) == (sb.experiment, sb.objective)
+
+
+def test_edge_cases_and_errors():
+    lat, _ = make(3, 1, False, 1)
+    lat.update(0, 1)  # empty pool: every state "contains" it
+    assert abs(lat.posterior().sum() - 1.0) < 1e-14
+    # the public binary API truncates to n bits (lattice_api.cpp:17-26 reads bits 0..n-1);
+    # the C ABI itself rejects experiments and responses outside the lattice
+    from bayesiangrouptesting_amd import _native as nat
+    assert nat.lib().bgt_hip_lattice_update(lat.handle, 1 << 5, 0, None) == nat.BAD_INPUT
+    with pytest.raises(ValueError):
+        lat.update(1, 2)
+    m2, _ = make(3, 2, False, 5)
+    with pytest.raises(ValueError):
+        m2.update(1 << 3, 0)
+    d, _ = make(3, 1, True, 2)
+    with pytest.raises(ValueError):
+        d.update(3, 1)  # dilution lattice without a table
+    m, _ = make(2, 2, False, 3)
+    with pytest.raises(ValueError):
+        m.select_experiment(bgt.SelectorType.op_bha)
+    big, _ = make(20, 1, False, 4)
+    with pytest.raises(bgt.Unsupported):
+        big.select_experiment(bgt.SelectorType.brute_force)
