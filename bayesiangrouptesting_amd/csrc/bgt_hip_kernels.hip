@@ -20,6 +20,8 @@
 #include "bgt_hip_internal.hpp"
 
 #include <cmath>
+#include <cstdlib>
+#include <string>
 
 namespace bgt_hip {
 
@@ -128,20 +130,6 @@ __device__ double block_sum(double v, double *red)
 	return s;
 }
 
-// Returns true in every thread of the last block of the lattice's grid to arrive.
-__device__ bool arrive_last(unsigned *counter, unsigned total)
-{
-	__shared__ unsigned s_last;
-	__threadfence();
-	__syncthreads();
-	if (threadIdx.x == 0)
-		s_last = (atomicAdd(counter, 1u) == total - 1) ? 1u : 0u;
-	__syncthreads();
-	if (s_last)
-		__threadfence();
-	return s_last != 0;
-}
-
 // Tile order: the 8 XCDs receive workgroups round-robin, so give XCD x the contiguous
 // tile range [x*G/8, (x+1)*G/8): neighbouring tiles (and the 64 B runs of pass 2) are
 // then read by CUs that share one L2.
@@ -180,17 +168,6 @@ __device__ __forceinline__ void load_lik_table(const Descs &descs, const LatDesc
 
 // ------------------------------------------------------------------ register tile zeta
 
-template <int Q>
-__device__ __forceinline__ void bfly_lane(double (&v)[kTileRegs], int lane)
-{
-	const bool recv = ((lane >> Q) & 1) == 0;
-#pragma unroll
-	for (int j = 0; j < kTileRegs; ++j) {
-		const double o = __shfl_xor(v[j], 1 << Q);
-		v[j] = recv ? v[j] + o : v[j];
-	}
-}
-
 template <int B>
 __device__ __forceinline__ void bfly_reg(double (&v)[kTileRegs])
 {
@@ -200,18 +177,85 @@ __device__ __forceinline__ void bfly_reg(double (&v)[kTileRegs])
 			v[j] += v[j | (1 << B)];
 }
 
-// Layout A: tile index i = lane | jlo<<6 | wave<<10 | jhi<<14 (j = jlo + 16*jhi)
-// Layout B: tile index i = lane | wave<<6 | jlo<<10 | jhi<<14
+// Load layout: tile index i = lane | jlo<<6 | wave<<10 | jhi<<14 (j = jlo + 16*jhi).
 __device__ __forceinline__ uint32_t tile_index_a(int lane, int wave, int j)
 {
 	return (uint32_t)lane | ((uint32_t)(j & 15) << 6) | ((uint32_t)wave << 10) | ((uint32_t)(j >> 4) << 14);
 }
-__device__ __forceinline__ uint32_t tile_index_b(int lane, int wave, int j)
+
+// Output layout after zeta_tile: lane bits 0..5 hold tile bits 4,5,2,3,0,1; wave bits hold
+// tile bits 6..9; jlo holds 10..13 and jhi bit 14. For a fixed register a wave still
+// covers one contiguous run of 64 states, so stores stay fully coalesced.
+__device__ __forceinline__ uint32_t lane_bits_f(int lane)
 {
-	return (uint32_t)lane | ((uint32_t)wave << 6) | ((uint32_t)(j & 15) << 10) | ((uint32_t)(j >> 4) << 14);
+	const uint32_t l = (uint32_t)lane;
+	return ((l & 1u) << 4) | (((l >> 1) & 1u) << 5) | (l & 0xCu) | ((l >> 4) & 3u);
+}
+__device__ __forceinline__ uint32_t tile_index_f(int lane, int wave, int j)
+{
+	return lane_bits_f(lane) | ((uint32_t)wave << 6) | ((uint32_t)(j & 15) << 10) | ((uint32_t)(j >> 4) << 14);
 }
 
-__device__ __forceinline__ void transpose_ab(double (&v)[kTileRegs], double *lds, int lane, int wave)
+constexpr int kWaveStride = 65; // padded rows of the wave-local transpose (bank-conflict free)
+constexpr int kTileLds = 16 * 16 * kWaveStride; // doubles; also holds the 16x16x64 block transpose
+
+__device__ __forceinline__ void wave_sync()
+{
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+	__builtin_amdgcn_wave_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Swap lane bits 0..3 with register bits 0..3 (jlo) inside one wave, through its own LDS
+// rows: thread lane L, register jlo' receives the value lane (L & 48 | jlo') held in
+// register (L & 15).
+__device__ __forceinline__ void wave_transpose(double (&v)[kTileRegs], double *lw, int lane)
+{
+#pragma unroll
+	for (int h = 0; h < 2; ++h) {
+#pragma unroll
+		for (int j = 0; j < 16; ++j)
+			lw[j * kWaveStride + lane] = v[j + 16 * h];
+		wave_sync();
+		const int rd = (lane & 15) * kWaveStride + (lane & 48);
+#pragma unroll
+		for (int j = 0; j < 16; ++j)
+			v[j + 16 * h] = lw[rd + j];
+		wave_sync();
+	}
+}
+
+// Exchange lane bit 4 (W32 = false, v_permlane16_swap) or lane bit 5 (W32 = true,
+// v_permlane32_swap) with register bit RB: afterwards the pair (j, j | 1<<RB) of every
+// lane holds two states that differ in the former lane bit.
+template <bool W32, int RB>
+__device__ __forceinline__ void perm_swap(double (&v)[kTileRegs])
+{
+#pragma unroll
+	for (int j = 0; j < kTileRegs; ++j) {
+		if (j & (1 << RB))
+			continue;
+		const unsigned long long a = (unsigned long long)__double_as_longlong(v[j]);
+		const unsigned long long b = (unsigned long long)__double_as_longlong(v[j | (1 << RB)]);
+		const unsigned alo = (unsigned)a, ahi = (unsigned)(a >> 32);
+		const unsigned blo = (unsigned)b, bhi = (unsigned)(b >> 32);
+		unsigned nlo0, nlo1, nhi0, nhi1;
+		if (W32) {
+			const auto lo = __builtin_amdgcn_permlane32_swap(alo, blo, false, false);
+			const auto hi = __builtin_amdgcn_permlane32_swap(ahi, bhi, false, false);
+			nlo0 = lo[0]; nlo1 = lo[1]; nhi0 = hi[0]; nhi1 = hi[1];
+		} else {
+			const auto lo = __builtin_amdgcn_permlane16_swap(alo, blo, false, false);
+			const auto hi = __builtin_amdgcn_permlane16_swap(ahi, bhi, false, false);
+			nlo0 = lo[0]; nlo1 = lo[1]; nhi0 = hi[0]; nhi1 = hi[1];
+		}
+		v[j] = __longlong_as_double((long long)(((unsigned long long)nhi0 << 32) | nlo0));
+		v[j | (1 << RB)] = __longlong_as_double((long long)(((unsigned long long)nhi1 << 32) | nlo1));
+	}
+}
+
+// Swap wave bits with register bits 0..3 across the workgroup (two halves through LDS).
+__device__ __forceinline__ void block_transpose(double (&v)[kTileRegs], double *lds, int lane, int wave)
 {
 #pragma unroll
 	for (int h = 0; h < 2; ++h) {
@@ -226,20 +270,27 @@ __device__ __forceinline__ void transpose_ab(double (&v)[kTileRegs], double *lds
 	}
 }
 
-// Sweep tile bits q0..14 in increasing order. Input in layout A, output in layout B.
+// Sweep tile bits q0..14 in increasing order; every butterfly is register-local.
+// Input in the load layout, output in the layout of tile_index_f.
 __device__ __forceinline__ void zeta_tile(double (&v)[kTileRegs], int q0, int lane, int wave, double *lds)
 {
-	if (q0 <= 0) bfly_lane<0>(v, lane);
-	if (q0 <= 1) bfly_lane<1>(v, lane);
-	if (q0 <= 2) bfly_lane<2>(v, lane);
-	if (q0 <= 3) bfly_lane<3>(v, lane);
-	if (q0 <= 4) bfly_lane<4>(v, lane);
-	if (q0 <= 5) bfly_lane<5>(v, lane);
+	double *lw = lds + wave * (16 * kWaveStride);
+	wave_transpose(v, lw, lane); // registers: tile bits 0..3
+	if (q0 <= 0) bfly_reg<0>(v);
+	if (q0 <= 1) bfly_reg<1>(v);
+	if (q0 <= 2) bfly_reg<2>(v);
+	if (q0 <= 3) bfly_reg<3>(v);
+	perm_swap<false, 0>(v); // register bit 0: tile bit 4
+	if (q0 <= 4) bfly_reg<0>(v);
+	perm_swap<true, 1>(v); // register bit 1: tile bit 5
+	if (q0 <= 5) bfly_reg<1>(v);
+	wave_transpose(v, lw, lane); // registers: tile bits 6..9
 	if (q0 <= 6) bfly_reg<0>(v);
 	if (q0 <= 7) bfly_reg<1>(v);
 	if (q0 <= 8) bfly_reg<2>(v);
 	if (q0 <= 9) bfly_reg<3>(v);
-	transpose_ab(v, lds, lane, wave);
+	__syncthreads(); // the block transpose reuses the wave rows
+	block_transpose(v, lds, lane, wave); // registers: tile bits 10..13
 	if (q0 <= 10) bfly_reg<0>(v);
 	if (q0 <= 11) bfly_reg<1>(v);
 	if (q0 <= 12) bfly_reg<2>(v);
@@ -297,29 +348,14 @@ __device__ __forceinline__ void consume(const ObjCtx &c, uint32_t g, double m, T
 	}
 }
 
-__device__ __forceinline__ void finish_selection(const LatDesc &d, unsigned slot, unsigned total,
-												 Top2 t, Top2 *red)
+// Each workgroup publishes its top-2; k_finalize_top2 merges them after the kernel ends
+// (a kernel boundary orders the writes without a device-scope fence per workgroup, which
+// on a multi-XCD part would write back and invalidate the L2 of every XCD).
+__device__ __forceinline__ void publish_top2(const LatDesc &d, unsigned slot, Top2 t, Top2 *red)
 {
 	Top2 b = top2_block(t, red);
 	if (threadIdx.x == 0)
 		reinterpret_cast<Top2 *>(d.tops)[slot] = b;
-	if (arrive_last(d.counter, total)) {
-		Top2 r;
-		top2_init(r);
-		for (unsigned i = threadIdx.x; i < total; i += blockDim.x) {
-			const Top2 p = reinterpret_cast<const Top2 *>(d.tops)[i];
-			top2_insert(r, p.d1, p.e1);
-			top2_insert(r, p.d2, p.e2);
-		}
-		r = top2_block(r, red);
-		if (threadIdx.x == 0) {
-			d.result->d1 = r.d1;
-			d.result->d2 = r.d2;
-			d.result->e1 = r.e1;
-			d.result->e2 = r.e2;
-			*d.counter = 0u;
-		}
-	}
 }
 
 // ----------------------------------------------------------------------------- kernels
@@ -329,12 +365,16 @@ __global__ __launch_bounds__(256) void k_prior(Descs descs, int atoms, const dou
 {
 	// lattice_base.cpp:184-208: P(S) = prod over atoms a = 0..A-1, in order, from 1.0
 	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
+	__shared__ double s_pairs[2 * kMaxAtoms];
+	if (threadIdx.x < 2 * kMaxAtoms)
+		s_pairs[threadIdx.x] = threadIdx.x < 2u * atoms ? pairs[threadIdx.x] : 1.0;
+	__syncthreads();
 	gptr<TP> post = G(static_cast<TP *>(d.post));
 	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
 	for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < states; s += stride) {
 		double prob = 1.0;
 		for (int a = 0; a < atoms; ++a)
-			prob *= ((s >> a) & 1u) ? pairs[2 * a + 1] : pairs[2 * a];
+			prob *= s_pairs[2 * a + ((s >> a) & 1u)];
 		post[s] = static_cast<TP>(prob);
 	}
 }
@@ -361,26 +401,19 @@ __global__ __launch_bounds__(256) void k_update(Descs descs, uint64_t states)
 	const double bs = block_sum(acc, red);
 	if (threadIdx.x == 0)
 		d.part[blockIdx.x] = bs;
-	if (arrive_last(d.counter, gridDim.x)) {
-		double t = 0.0;
-		for (unsigned i = threadIdx.x; i < gridDim.x; i += blockDim.x)
-			t += d.part[i];
-		t = block_sum(t, red);
-		if (threadIdx.x == 0) {
-			d.scal[kScalDen] = t;
-			d.scal[d.inv_out] = 1.0 / t;
-			*d.counter = 0u;
-		}
-	}
 }
 
 // Pass 1: sweep bits 0..14 of contiguous 2^15 tiles. Input is x = y * inv; with FUSED the
 // update y' = x * L is applied first (stored back in place) and the transform runs on y'.
-template <typename TP, typename TZ, bool FUSED>
-__global__ __launch_bounds__(kTileThreads) void k_zeta_pass1(Descs descs)
+// LIK: 0 = general table lookup, 1 = binary non-dilution (two factors chosen by E subset S)
+// BLOCKED writes the intermediate grouped by pass-2 tile: state (row h, column col) goes to
+// ((col >> logc) << 15) | (h << logc) | (col & (c-1)), so that pass 2 reads each of its
+// tiles as one contiguous 2^15 run (c = 2^logc columns per pass-2 tile).
+template <typename TP, typename TZ, bool FUSED, int LIK>
+__global__ __launch_bounds__(kTileThreads) void k_zeta_pass1(Descs descs, int logc_blocked)
 {
 	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
-	__shared__ double lds[16 * 16 * 64];
+	__shared__ double lds[kTileLds];
 	__shared__ double red[16];
 	__shared__ double s_lik[kMaxVariants * (kMaxSubjects + 1)];
 	if (FUSED)
@@ -392,43 +425,49 @@ __global__ __launch_bounds__(kTileThreads) void k_zeta_pass1(Descs descs)
 	// per-thread base pointers; the register index only adds compile-time offsets
 	gptr<TP> pa = G(static_cast<TP *>(d.post)) + base + lane + (wave << 10);
 	const double inv = d.scal[d.inv_in];
+	const double f_in = FUSED ? s_lik[0] : 0.0, f_out = FUSED ? s_lik[1] : 0.0;
 	double v[kTileRegs];
-	double acc = 0.0;
+	// issue every load of the tile before the first store (the fused update writes in place)
+	TP raw[kTileRegs];
+#pragma unroll
+	for (int j = 0; j < kTileRegs; ++j)
+		raw[j] = pa[((uint32_t)(j & 15) << 6) | ((uint32_t)(j >> 4) << 14)];
 #pragma unroll
 	for (int j = 0; j < kTileRegs; ++j) {
 		const uint32_t off = ((uint32_t)(j & 15) << 6) | ((uint32_t)(j >> 4) << 14);
-		double x = static_cast<double>(pa[off]) * inv;
+		double x = static_cast<double>(raw[j]) * inv;
 		if (FUSED) {
 			const uint32_t g = (uint32_t)(base + tile_index_a(lane, wave, j));
-			const TP y = static_cast<TP>(x * likelihood(g, d.e, d.n, d.k, d.len, d.dilution, s_lik));
+			double lk;
+			if (LIK == 1)
+				lk = ((d.e & g) == d.e) ? f_in : f_out;
+			else
+				lk = likelihood(g, d.e, d.n, d.k, d.len, d.dilution, s_lik);
+			const TP y = static_cast<TP>(x * lk);
 			pa[off] = y;
 			x = static_cast<double>(y);
 		}
 		v[j] = x;
-		acc += x;
 	}
 	zeta_tile(v, 0, lane, wave, lds);
-	gptr<TZ> zb = G(static_cast<TZ *>(d.z)) + base + lane + (wave << 6);
+	if (logc_blocked < 0) {
+		gptr<TZ> zb = G(static_cast<TZ *>(d.z)) + base + lane_bits_f(lane) + (wave << 6);
 #pragma unroll
-	for (int j = 0; j < kTileRegs; ++j)
-		zb[((uint32_t)(j & 15) << 10) | ((uint32_t)(j >> 4) << 14)] = static_cast<TZ>(v[j]);
-	const double bs = block_sum(acc, red);
-	if (threadIdx.x == 0)
-		d.part[tile] = bs;
-	if (arrive_last(d.counter, tiles)) {
-		double t = 0.0;
-		for (unsigned i = threadIdx.x; i < tiles; i += blockDim.x)
-			t += d.part[i];
-		t = block_sum(t, red);
-		if (threadIdx.x == 0) {
-			if (FUSED) {
-				d.scal[kScalDen] = t;
-				d.scal[d.inv_out] = 1.0 / t;
-			}
-			d.scal[kScalTotal] = t;
-			*d.counter = 0u;
+		for (int j = 0; j < kTileRegs; ++j)
+			zb[((uint32_t)(j & 15) << 10) | ((uint32_t)(j >> 4) << 14)] = static_cast<TZ>(v[j]);
+	} else {
+		gptr<TZ> zb = G(static_cast<TZ *>(d.z));
+		const uint32_t cmask = (1u << logc_blocked) - 1u;
+		const uint64_t hrow = (uint64_t)tile << logc_blocked;
+#pragma unroll
+		for (int j = 0; j < kTileRegs; ++j) {
+			const uint32_t i = tile_index_f(lane, wave, j);
+			zb[((uint64_t)(i >> logc_blocked) << kTileBits) | hrow | (i & cmask)] = static_cast<TZ>(v[j]);
 		}
 	}
+	// after the full 15-bit sweep, tile index 0 (thread 0, register 0) holds the tile's total
+	if (threadIdx.x == 0)
+		d.part[tile] = v[0];
 }
 
 // Final pass: sweep tile bits q0..14 and evaluate the objective on every output entry.
@@ -437,10 +476,10 @@ __global__ __launch_bounds__(kTileThreads) void k_zeta_pass1(Descs descs)
 // otherwise it reads the pass-1 intermediate, scaled by inv_out when `scaled`.
 template <typename TIN, bool FROM_POST>
 __global__ __launch_bounds__(kTileThreads) void k_zeta_final(Descs descs, int logc, int q0,
-															 int objective, int scaled, double target)
+															 int objective, int scaled, double target, int blocked)
 {
 	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
-	__shared__ double lds[16 * 16 * 64];
+	__shared__ double lds[kTileLds];
 	__shared__ Top2 red2[16];
 	__shared__ double red[16];
 	__shared__ double s_total;
@@ -452,13 +491,19 @@ __global__ __launch_bounds__(kTileThreads) void k_zeta_final(Descs descs, int lo
 	gptr<const TIN> in = G(FROM_POST ? static_cast<const TIN *>(d.post) : static_cast<const TIN *>(d.z));
 	const double inv = FROM_POST ? d.scal[d.inv_in] : 1.0;
 	double v[kTileRegs];
-	double acc = 0.0;
+	if (blocked) {
+		// this tile's 2^15 entries are contiguous in the blocked intermediate
+		gptr<const TIN> tb = in + ((uint64_t)tile << kTileBits) + lane + (wave << 10);
 #pragma unroll
-	for (int j = 0; j < kTileRegs; ++j) {
-		const uint32_t i = tile_index_a(lane, wave, j);
-		const uint64_t g = cbase + (i & cmask) + ((uint64_t)(i >> logc) << kTileBits);
-		v[j] = static_cast<double>(in[g]) * inv;
-		acc += v[j];
+		for (int j = 0; j < kTileRegs; ++j)
+			v[j] = static_cast<double>(tb[((uint32_t)(j & 15) << 6) | ((uint32_t)(j >> 4) << 14)]) * inv;
+	} else {
+#pragma unroll
+		for (int j = 0; j < kTileRegs; ++j) {
+			const uint32_t i = tile_index_a(lane, wave, j);
+			const uint64_t g = cbase + (i & cmask) + ((uint64_t)(i >> logc) << kTileBits);
+			v[j] = static_cast<double>(in[g]) * inv;
+		}
 	}
 	ObjCtx c;
 	c.objective = objective;
@@ -468,19 +513,18 @@ __global__ __launch_bounds__(kTileThreads) void k_zeta_final(Descs descs, int lo
 	c.k = d.k;
 	c.ftab = d.ftab;
 	c.masses = d.masses;
+	zeta_tile(v, q0, lane, wave, lds);
 	if (FROM_POST) {
-		// the whole lattice is this one tile: its total is known here
-		const double t = block_sum(acc, red);
+		// the whole lattice is this one tile: its total is the transform at index 0
 		if (threadIdx.x == 0) {
-			s_total = t;
-			d.scal[kScalTotal] = t;
+			s_total = v[0];
+			d.scal[kScalTotal] = v[0];
 		}
 		__syncthreads();
 		c.total = s_total;
 	} else {
 		c.total = d.scal[kScalTotal];
 	}
-	zeta_tile(v, q0, lane, wave, lds);
 	const double scale = scaled ? d.scal[d.inv_out] : 1.0;
 	if (scaled)
 		c.total = c.total * scale;
@@ -488,12 +532,12 @@ __global__ __launch_bounds__(kTileThreads) void k_zeta_final(Descs descs, int lo
 	top2_init(t);
 #pragma unroll
 	for (int j = 0; j < kTileRegs; ++j) {
-		const uint32_t i = tile_index_b(lane, wave, j);
+		const uint32_t i = tile_index_f(lane, wave, j);
 		const uint32_t g = (uint32_t)(cbase + (i & cmask) + ((uint64_t)(i >> logc) << kTileBits));
 		const double m = scaled ? v[j] * scale : v[j];
 		consume(c, g, m, t);
 	}
-	finish_selection(d, tile, tiles, t, red2);
+	publish_top2(d, tile, t, red2);
 }
 
 // Lattices of at most 2^14 states: the whole transform in the LDS of one workgroup.
@@ -594,7 +638,7 @@ __global__ __launch_bounds__(256) void k_bbpa_score(Descs descs, int ns, int k, 
 		}
 		top2_insert(t, score, e);
 	}
-	finish_selection(d, blockIdx.x, gridDim.x, t, red2);
+	publish_top2(d, blockIdx.x, t, red2);
 }
 
 // Explicit (experiment, state) sweep (benchmarks/experiment_selection_benchmark.cpp:214-247):
@@ -635,7 +679,7 @@ __global__ __launch_bounds__(256) void k_brute_force(Descs descs, int ns)
 	top2_init(t);
 	if (threadIdx.x == 0 && (e & ~d.active) == 0)
 		top2_insert(t, score, e);
-	finish_selection(d, blockIdx.x, gridDim.x, t, red2);
+	publish_top2(d, blockIdx.x, t, red2);
 }
 
 // Atom marginals m_a = sum_{S : bit a} x_S (lattice_base.cpp:25-41). A workgroup covers
@@ -657,13 +701,24 @@ __global__ __launch_bounds__(256) void k_atom_masses(Descs descs, int atoms)
 	double sq[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 	const bool live = (cb >= 8) || tid < (1u << cb);
 	if (live) {
-		for (uint32_t j = 0; j < (1u << jb); ++j) {
-			const double x = static_cast<double>(post[base + tid + (j << 8)]) * inv;
-			total += x;
+		const uint32_t J = 1u << jb;
+		for (uint32_t j0 = 0; j0 < J; j0 += 16) {
+			double x[16];
 #pragma unroll
-			for (int q = 0; q < 6; ++q)
-				if (q < jb && ((j >> q) & 1u))
-					sq[q] += x;
+			for (int q = 0; q < 16; ++q)
+				x[q] = (j0 + q < J) ? static_cast<double>(post[base + tid + ((j0 + q) << 8)]) : 0.0;
+#pragma unroll
+			for (int q = 0; q < 16; ++q) {
+				const uint32_t j = j0 + q;
+				const double xv = x[q] * inv;
+				if (j < J) {
+					total += xv;
+#pragma unroll
+					for (int b = 0; b < 6; ++b)
+						if (b < jb && ((j >> b) & 1u))
+							sq[b] += xv;
+				}
+			}
 		}
 	}
 	for (int a = 0; a < atoms; ++a) {
@@ -682,18 +737,6 @@ __global__ __launch_bounds__(256) void k_atom_masses(Descs descs, int atoms)
 		const double s = block_sum(contrib, red);
 		if (threadIdx.x == 0)
 			d.part[(size_t)chunk * kMaxAtoms + a] = s;
-	}
-	if (arrive_last(d.counter, gridDim.x)) {
-		for (int a = 0; a < atoms; ++a) {
-			double s = 0.0;
-			for (unsigned i = threadIdx.x; i < gridDim.x; i += blockDim.x)
-				s += d.part[(size_t)i * kMaxAtoms + a];
-			s = block_sum(s, red);
-			if (threadIdx.x == 0)
-				d.masses[a] = s;
-		}
-		if (threadIdx.x == 0)
-			*d.counter = 0u;
 	}
 }
 
@@ -730,16 +773,6 @@ __global__ __launch_bounds__(256) void k_upset_mass(Descs descs, int atoms, uint
 	const double bs = block_sum(acc, red);
 	if (threadIdx.x == 0)
 		d.part[blockIdx.x] = bs;
-	if (arrive_last(d.counter, gridDim.x)) {
-		double t = 0.0;
-		for (unsigned i = threadIdx.x; i < gridDim.x; i += blockDim.x)
-			t += d.part[i];
-		t = block_sum(t, red);
-		if (threadIdx.x == 0) {
-			d.scal[kScalUpset] = t;
-			*d.counter = 0u;
-		}
-	}
 }
 
 template <typename TP>
@@ -899,17 +932,78 @@ __global__ __launch_bounds__(256) void k_tree_stats(const FlatNode *nodes, int n
 	if (threadIdx.x == 0)
 		for (int q = 0; q < 7; ++q)
 			partials[(size_t)blockIdx.x * 7 + q] = bs[q];
-	if (arrive_last(counter, gridDim.x)) {
-		for (int q = 0; q < 7; ++q) {
-			double s = 0.0;
-			for (unsigned i2 = threadIdx.x; i2 < gridDim.x; i2 += blockDim.x)
-				s += partials[(size_t)i2 * 7 + q];
-			s = block_sum(s, red);
-			if (threadIdx.x == 0)
-				out[q] = s;
+}
+
+// ------------------------------------------------------------------- finalize kernels
+// One workgroup per lattice folds the per-workgroup partials in index order.
+
+enum SumMode : int { kSumTotal = 0, kSumDenominator = 1, kSumUpset = 2 };
+
+__global__ __launch_bounds__(1024) void k_finalize_sum(Descs descs, unsigned count, int mode)
+{
+	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
+	__shared__ double red[16];
+	double t = 0.0;
+	for (unsigned i = threadIdx.x; i < count; i += blockDim.x)
+		t += d.part[i];
+	t = block_sum(t, red);
+	if (threadIdx.x == 0) {
+		if (mode == kSumDenominator) {
+			d.scal[kScalDen] = t;
+			d.scal[d.inv_out] = 1.0 / t;
+			d.scal[kScalTotal] = t;
+		} else if (mode == kSumUpset) {
+			d.scal[kScalUpset] = t;
+		} else {
+			d.scal[kScalTotal] = t;
 		}
+	}
+}
+
+__global__ __launch_bounds__(1024) void k_finalize_top2(Descs descs, unsigned count)
+{
+	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
+	__shared__ Top2 red2[16];
+	Top2 r;
+	top2_init(r);
+	for (unsigned i = threadIdx.x; i < count; i += blockDim.x) {
+		const Top2 p = reinterpret_cast<const Top2 *>(d.tops)[i];
+		top2_insert(r, p.d1, p.e1);
+		top2_insert(r, p.d2, p.e2);
+	}
+	r = top2_block(r, red2);
+	if (threadIdx.x == 0) {
+		d.result->d1 = r.d1;
+		d.result->d2 = r.d2;
+		d.result->e1 = r.e1;
+		d.result->e2 = r.e2;
+	}
+}
+
+__global__ __launch_bounds__(1024) void k_finalize_masses(Descs descs, unsigned count, int atoms)
+{
+	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
+	__shared__ double red[16];
+	for (int a = 0; a < atoms; ++a) {
+		double s = 0.0;
+		for (unsigned i = threadIdx.x; i < count; i += blockDim.x)
+			s += d.part[(size_t)i * kMaxAtoms + a];
+		s = block_sum(s, red);
 		if (threadIdx.x == 0)
-			*counter = 0u;
+			d.masses[a] = s;
+	}
+}
+
+__global__ __launch_bounds__(1024) void k_finalize_tree(const double *partials, unsigned count, double *out)
+{
+	__shared__ double red[16];
+	for (int q = 0; q < 7; ++q) {
+		double s = 0.0;
+		for (unsigned i = threadIdx.x; i < count; i += blockDim.x)
+			s += partials[(size_t)i * 7 + q];
+		s = block_sum(s, red);
+		if (threadIdx.x == 0)
+			out[q] = s;
 	}
 }
 
@@ -918,6 +1012,18 @@ void launch_checked(const char *name, F f)
 {
 	f();
 	check_hip(hipGetLastError(), name);
+}
+
+// Layout of the transform intermediate between the two passes: natural (state order) by
+// default; BGT_HIP_ZLAYOUT=blocked groups it by pass-2 tile (contiguous pass-2 reads, but
+// pass 1 then scatters 16 B runs and is slower overall on gfx950).
+bool blocked_intermediate()
+{
+	static const bool blocked = [] {
+		const char *v = std::getenv("BGT_HIP_ZLAYOUT");
+		return v && std::string(v) == "blocked";
+	}();
+	return blocked;
 }
 
 unsigned grid_for(uint64_t work, unsigned per_block, unsigned cap)
@@ -955,25 +1061,34 @@ void launch_update(const Descs &descs, int count, uint64_t states, int, int, Sto
 			hipLaunchKernelGGL(k_update<double>, grid, dim3(256), 0, stream, descs, states);
 		else
 			hipLaunchKernelGGL(k_update<float>, grid, dim3(256), 0, stream, descs, states);
+		hipLaunchKernelGGL(k_finalize_sum, dim3(1, count), dim3(1024), 0, stream, descs, grid.x, (int)kSumDenominator);
 	});
 }
 
-void launch_zeta_pass1(const Descs &descs, int count, int n, Storage st, bool fused, int, int,
+void launch_zeta_pass1(const Descs &descs, int count, int n, Storage st, bool fused, int k, int dilution,
 					   hipStream_t stream)
 {
 	const dim3 grid(1u << (n - kTileBits), count);
+	const int lb = blocked_intermediate() ? (2 * kTileBits - n) : -1;
+	const bool binary_nd = (k == 1 && !dilution);
 	launch_checked("k_zeta_pass1", [&] {
 		if (st == Storage::fp64) {
-			if (fused)
-				hipLaunchKernelGGL((k_zeta_pass1<double, double, true>), grid, dim3(kTileThreads), 0, stream, descs);
+			if (fused && binary_nd)
+				hipLaunchKernelGGL((k_zeta_pass1<double, double, true, 1>), grid, dim3(kTileThreads), 0, stream, descs, lb);
+			else if (fused)
+				hipLaunchKernelGGL((k_zeta_pass1<double, double, true, 0>), grid, dim3(kTileThreads), 0, stream, descs, lb);
 			else
-				hipLaunchKernelGGL((k_zeta_pass1<double, double, false>), grid, dim3(kTileThreads), 0, stream, descs);
+				hipLaunchKernelGGL((k_zeta_pass1<double, double, false, 0>), grid, dim3(kTileThreads), 0, stream, descs, lb);
 		} else {
-			if (fused)
-				hipLaunchKernelGGL((k_zeta_pass1<float, float, true>), grid, dim3(kTileThreads), 0, stream, descs);
+			if (fused && binary_nd)
+				hipLaunchKernelGGL((k_zeta_pass1<float, float, true, 1>), grid, dim3(kTileThreads), 0, stream, descs, lb);
+			else if (fused)
+				hipLaunchKernelGGL((k_zeta_pass1<float, float, true, 0>), grid, dim3(kTileThreads), 0, stream, descs, lb);
 			else
-				hipLaunchKernelGGL((k_zeta_pass1<float, float, false>), grid, dim3(kTileThreads), 0, stream, descs);
+				hipLaunchKernelGGL((k_zeta_pass1<float, float, false, 0>), grid, dim3(kTileThreads), 0, stream, descs, lb);
 		}
+		hipLaunchKernelGGL(k_finalize_sum, dim3(1, count), dim3(1024), 0, stream, descs, grid.x,
+						   (int)(fused ? kSumDenominator : kSumTotal));
 	});
 }
 
@@ -989,18 +1104,19 @@ void launch_zeta_final(const Descs &descs, int count, int n, int, Storage st, in
 		if (from_posterior) {
 			if (st == Storage::fp64)
 				hipLaunchKernelGGL((k_zeta_final<double, true>), grid, dim3(kTileThreads), 0, stream, descs, logc, q0,
-								   objective, 0, target);
+								   objective, 0, target, 0);
 			else
 				hipLaunchKernelGGL((k_zeta_final<float, true>), grid, dim3(kTileThreads), 0, stream, descs, logc, q0,
-								   objective, 0, target);
+								   objective, 0, target, 0);
 		} else {
 			if (st == Storage::fp64)
 				hipLaunchKernelGGL((k_zeta_final<double, false>), grid, dim3(kTileThreads), 0, stream, descs, logc,
-								   q0, objective, scaled ? 1 : 0, target);
+								   q0, objective, scaled ? 1 : 0, target, blocked_intermediate() ? 1 : 0);
 			else
 				hipLaunchKernelGGL((k_zeta_final<float, false>), grid, dim3(kTileThreads), 0, stream, descs, logc,
-								   q0, objective, scaled ? 1 : 0, target);
+								   q0, objective, scaled ? 1 : 0, target, blocked_intermediate() ? 1 : 0);
 		}
+		hipLaunchKernelGGL(k_finalize_top2, dim3(1, count), dim3(1024), 0, stream, descs, grid.x);
 	});
 }
 
@@ -1021,6 +1137,7 @@ void launch_bbpa_score(const Descs &descs, int count, int n, int k, double targe
 	const dim3 grid(grid_for(1ull << n, 256, 1u << 20), count);
 	launch_checked("k_bbpa_score", [&] {
 		hipLaunchKernelGGL(k_bbpa_score, grid, dim3(256), 0, stream, descs, n, k, target);
+		hipLaunchKernelGGL(k_finalize_top2, dim3(1, count), dim3(1024), 0, stream, descs, grid.x);
 	});
 }
 
@@ -1034,6 +1151,7 @@ void launch_brute_force(const Descs &descs, int count, int n, int k, Storage st,
 			hipLaunchKernelGGL((k_brute_force<double, KK>), grid, dim3(256), 0, stream, descs, n); \
 		else                                                                                        \
 			hipLaunchKernelGGL((k_brute_force<float, KK>), grid, dim3(256), 0, stream, descs, n);  \
+		hipLaunchKernelGGL(k_finalize_top2, dim3(1, count), dim3(1024), 0, stream, descs, grid.x); \
 		return;                                                                                     \
 	}
 		BGT_BF(1)
@@ -1054,6 +1172,7 @@ void launch_atom_masses(const Descs &descs, int count, int atoms, Storage st, hi
 			hipLaunchKernelGGL(k_atom_masses<double>, grid, dim3(256), 0, stream, descs, atoms);
 		else
 			hipLaunchKernelGGL(k_atom_masses<float>, grid, dim3(256), 0, stream, descs, atoms);
+		hipLaunchKernelGGL(k_finalize_masses, dim3(1, count), dim3(1024), 0, stream, descs, grid.x, atoms);
 	});
 }
 
@@ -1066,6 +1185,7 @@ void launch_upset_mass(const Descs &descs, int count, int atoms, uint32_t e, Sto
 			hipLaunchKernelGGL(k_upset_mass<double>, grid, dim3(256), 0, stream, descs, atoms, e);
 		else
 			hipLaunchKernelGGL(k_upset_mass<float>, grid, dim3(256), 0, stream, descs, atoms, e);
+		hipLaunchKernelGGL(k_finalize_sum, dim3(1, count), dim3(1024), 0, stream, descs, grid.x, (int)kSumUpset);
 	});
 }
 
@@ -1102,6 +1222,7 @@ void launch_tree_stats(const FlatNode *nodes, int node_count, int n, int k, int 
 	launch_checked("k_tree_stats", [&] {
 		hipLaunchKernelGGL(k_tree_stats, grid, dim3(256), 0, stream, nodes, node_count, n, k, dilution, dil, pi,
 						   depth, branch, partials, counter, out);
+		hipLaunchKernelGGL(k_finalize_tree, dim3(1), dim3(1024), 0, stream, partials, grid.x, out);
 	});
 }
 

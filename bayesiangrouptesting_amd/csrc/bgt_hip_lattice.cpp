@@ -120,12 +120,19 @@ constexpr double kNegativeResponse = 0.99; // replicated_lattice.cpp:6
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
+// Arena head: [0,256) scalars, [256,512) selection result, [512,768) counters,
+// [768,1024) atom masses. One 1 KB copy brings all of it to the host.
+constexpr size_t kHeadBytes = 1024;
+constexpr size_t kHeadResult = 256;
+constexpr size_t kHeadMasses = 768;
+
 struct HostBox {
 	SelResult result;
 	double masses[32];
 	double scal[kScalars];
 	double value;
 	double raw;
+	alignas(256) unsigned char head[kHeadBytes];
 };
 
 } // namespace
@@ -196,10 +203,10 @@ void DeviceLattice::allocate()
 		off += align256(bytes);
 		return o;
 	};
-	const size_t o_scal = take(sizeof(double) * kScalars);
-	const size_t o_res = take(sizeof(SelResult));
-	const size_t o_cnt = take(sizeof(unsigned) * 4);
-	const size_t o_mass = take(sizeof(double) * 32);
+	const size_t o_scal = take(sizeof(double) * kScalars); // 0
+	const size_t o_res = take(sizeof(SelResult));         // 256
+	const size_t o_cnt = take(sizeof(unsigned) * 4);      // 512
+	const size_t o_mass = take(sizeof(double) * 32);      // 768
 	const size_t o_pairs = take(sizeof(double) * 2 * kMaxAtoms);
 	const size_t o_part = take(sizeof(double) * part);
 	const size_t o_tops = take(sizeof(SelResult) * tops);
@@ -360,6 +367,16 @@ void DeviceLattice::check_selector(int selector) const
 		throw Error(BGT_HIP_BAD_INPUT, "Op-BHA experiment selection is only supported for binary lattices");
 	if (selector == kSelBrute && (n_ > 16 || k_ > 4 || n_ + atoms_ > 40))
 		throw Error(BGT_HIP_UNSUPPORTED, "brute-force selection is limited to 16 subjects, 4 variants and 2^40 work");
+}
+
+void DeviceLattice::fetch_head()
+{
+	HostBox *box = static_cast<HostBox *>(host_box_);
+	BGT_HIP_CHECK(hipMemcpyAsync(box->head, arena_, kHeadBytes, hipMemcpyDeviceToHost, stream_));
+	BGT_HIP_CHECK(hipStreamSynchronize(stream_));
+	std::memcpy(box->scal, box->head, sizeof(double) * kScalars);
+	std::memcpy(&box->result, box->head + kHeadResult, sizeof(SelResult));
+	std::memcpy(box->masses, box->head + kHeadMasses, sizeof(double) * 32);
 }
 
 SelResult DeviceLattice::fetch_result()
@@ -612,9 +629,11 @@ bgt_hip_selection DeviceLattice::round(uint32_t experiment, uint32_t response, c
 	launch_zeta_final(ds, 1, atoms_, k_, storage_, kObjHalving, false, true, 0.5, stream_);
 	clock_.end(stream_, "round_pass2");
 	commit_update();
-	const SelResult r = fetch_result();
+	fetch_head();
+	HostBox *box = static_cast<HostBox *>(host_box_);
+	const SelResult r = box->result;
 	double m[32];
-	fetch_masses(m);
+	std::memcpy(m, box->masses, sizeof(m));
 	for (int i = 0; i < atoms_; ++i) {
 		const uint32_t placement = 1u << i;
 		if ((pos_ | neg_) & placement)

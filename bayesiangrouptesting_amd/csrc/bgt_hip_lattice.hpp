@@ -85,6 +85,7 @@ public:
 	void commit_update();
 	void sync();
 	SelResult fetch_result();
+	void fetch_head();
 	bgt_hip_selection finish_selection(int selector, const SelResult &r, bool scaled);
 	void fetch_masses(double *out);
 
