@@ -88,6 +88,7 @@ def _sig(L):
         "bgt_hip_lattice_atom_masses": ([_H, _dptr], c_int),
         "bgt_hip_lattice_classify": ([_H, c_double, c_double, _u32p, _u32p], c_int),
         "bgt_hip_lattice_masks": ([_H, _u32p, _u32p], c_int),
+        "bgt_hip_lattice_set_candidates": ([_H, c_u32], c_int),
         "bgt_hip_lattice_select_experiments": ([_H, c_int, _u32p], c_int),
         "bgt_hip_lattice_round": ([_H, c_u32, c_u32, _dptr, c_double, c_double, ctypes.POINTER(SelectionC), _u32p,
                                    _u32p, _dptr], c_int),

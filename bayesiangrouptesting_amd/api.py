@@ -264,6 +264,10 @@ class Lattice:
                                                   threshold_lo, ctypes.byref(s), None, None, mp))
         return Selection.from_c(s, self._flip)
 
+    def restrict_candidates(self, subjects_mask: int) -> None:
+        """Consider only experiments that are subsets of subjects_mask (public coordinates)."""
+        nat.check(nat.lib().bgt_hip_lattice_set_candidates(self._h, self._flip(subjects_mask)))
+
     def update_classification(self, threshold_up: float, threshold_lo: float) -> None:
         nat.check(nat.lib().bgt_hip_lattice_classify(self._h, threshold_up, threshold_lo, None, None))
 

@@ -256,6 +256,17 @@ int bgt_hip_lattice_masks(bgt_hip_lattice *h, uint32_t *pos, uint32_t *neg)
 	});
 }
 
+int bgt_hip_lattice_set_candidates(bgt_hip_lattice *h, uint32_t mask)
+{
+	return guarded([&] {
+		auto &l = L(h);
+		const uint32_t full = l.subjects() >= 32 ? 0xFFFFFFFFu : ((1u << l.subjects()) - 1u);
+		if (mask & ~full)
+			throw bgt_hip::Error(BGT_HIP_BAD_INPUT, "candidate mask has bits beyond the subject count");
+		l.set_active(mask);
+	});
+}
+
 int bgt_hip_lattice_select_experiments(bgt_hip_lattice *h, int steps, uint32_t *out)
 {
 	return guarded([&] {

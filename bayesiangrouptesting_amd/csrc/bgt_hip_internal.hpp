@@ -17,6 +17,16 @@ constexpr int kMaxSubjects = 30;
 
 // Register tile of the transform: 1024 threads x 32 values = 2^15 states per workgroup.
 constexpr int kTileBits = 15;
+// The transform intermediate z holds 2^(A-15) rows of 2^15 entries; rows are spaced by
+// 2^15 entries plus 128 bytes so that the strided 16-32 byte reads of pass 2 spread over
+// the memory channels instead of landing on the same one.
+constexpr int kZSkewBytes = 128;
+template <typename T>
+constexpr uint32_t z_pitch() { return (1u << kTileBits) + kZSkewBytes / sizeof(T); }
+inline uint64_t z_bytes(int atoms, size_t elem)
+{
+	return atoms <= kTileBits ? 0 : (uint64_t)(1ull << (atoms - kTileBits)) * ((1ull << kTileBits) + kZSkewBytes / elem) * elem;
+}
 constexpr int kTileThreads = 1024;
 constexpr int kTileRegs = 32;
 // Lattices with at most this many atoms run the transform in one LDS-resident workgroup.

@@ -39,6 +39,19 @@ __device__ __forceinline__ gptr<T> G(T *p)
 	return (gptr<T>)p;
 }
 
+// Load/store at a wave-uniform base plus a 32-bit per-lane byte offset: the compiler keeps
+// the base in SGPRs (global_* with saddr) instead of one 64-bit address per access.
+template <typename T>
+__device__ __forceinline__ T ld_u(const T *ubase, uint32_t byte_off)
+{
+	return *(gptr<const T>)(reinterpret_cast<const char *>(ubase) + byte_off);
+}
+template <typename T>
+__device__ __forceinline__ void st_u(T *ubase, uint32_t byte_off, T v)
+{
+	*(gptr<T>)(reinterpret_cast<char *>(ubase) + byte_off) = v;
+}
+
 struct Top2 {
 	double d1, d2;
 	uint32_t e1, e2;
@@ -405,12 +418,10 @@ __global__ __launch_bounds__(256) void k_update(Descs descs, uint64_t states)
 
 // Pass 1: sweep bits 0..14 of contiguous 2^15 tiles. Input is x = y * inv; with FUSED the
 // update y' = x * L is applied first (stored back in place) and the transform runs on y'.
-// LIK: 0 = general table lookup, 1 = binary non-dilution (two factors chosen by E subset S)
-// BLOCKED writes the intermediate grouped by pass-2 tile: state (row h, column col) goes to
-// ((col >> logc) << 15) | (h << logc) | (col & (c-1)), so that pass 2 reads each of its
-// tiles as one contiguous 2^15 run (c = 2^logc columns per pass-2 tile).
+// LIK: 0 = general table lookup, 1 = binary non-dilution (two factors chosen by E subset S).
+// The intermediate is written in state order.
 template <typename TP, typename TZ, bool FUSED, int LIK>
-__global__ __launch_bounds__(kTileThreads) void k_zeta_pass1(Descs descs, int logc_blocked)
+__global__ __launch_bounds__(kTileThreads) void k_zeta_pass1(Descs descs)
 {
 	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
 	__shared__ double lds[kTileLds];
@@ -450,21 +461,10 @@ __global__ __launch_bounds__(kTileThreads) void k_zeta_pass1(Descs descs, int lo
 		v[j] = x;
 	}
 	zeta_tile(v, 0, lane, wave, lds);
-	if (logc_blocked < 0) {
-		gptr<TZ> zb = G(static_cast<TZ *>(d.z)) + base + lane_bits_f(lane) + (wave << 6);
+	gptr<TZ> zb = G(static_cast<TZ *>(d.z)) + (uint64_t)tile * z_pitch<TZ>() + lane_bits_f(lane) + (wave << 6);
 #pragma unroll
-		for (int j = 0; j < kTileRegs; ++j)
-			zb[((uint32_t)(j & 15) << 10) | ((uint32_t)(j >> 4) << 14)] = static_cast<TZ>(v[j]);
-	} else {
-		gptr<TZ> zb = G(static_cast<TZ *>(d.z));
-		const uint32_t cmask = (1u << logc_blocked) - 1u;
-		const uint64_t hrow = (uint64_t)tile << logc_blocked;
-#pragma unroll
-		for (int j = 0; j < kTileRegs; ++j) {
-			const uint32_t i = tile_index_f(lane, wave, j);
-			zb[((uint64_t)(i >> logc_blocked) << kTileBits) | hrow | (i & cmask)] = static_cast<TZ>(v[j]);
-		}
-	}
+	for (int j = 0; j < kTileRegs; ++j)
+		zb[((uint32_t)(j & 15) << 10) | ((uint32_t)(j >> 4) << 14)] = static_cast<TZ>(v[j]);
 	// after the full 15-bit sweep, tile index 0 (thread 0, register 0) holds the tile's total
 	if (threadIdx.x == 0)
 		d.part[tile] = v[0];
@@ -476,7 +476,7 @@ __global__ __launch_bounds__(kTileThreads) void k_zeta_pass1(Descs descs, int lo
 // otherwise it reads the pass-1 intermediate, scaled by inv_out when `scaled`.
 template <typename TIN, bool FROM_POST>
 __global__ __launch_bounds__(kTileThreads) void k_zeta_final(Descs descs, int logc, int q0,
-															 int objective, int scaled, double target, int blocked)
+															 int objective, int scaled, double target)
 {
 	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
 	__shared__ double lds[kTileLds];
@@ -491,19 +491,13 @@ __global__ __launch_bounds__(kTileThreads) void k_zeta_final(Descs descs, int lo
 	gptr<const TIN> in = G(FROM_POST ? static_cast<const TIN *>(d.post) : static_cast<const TIN *>(d.z));
 	const double inv = FROM_POST ? d.scal[d.inv_in] : 1.0;
 	double v[kTileRegs];
-	if (blocked) {
-		// this tile's 2^15 entries are contiguous in the blocked intermediate
-		gptr<const TIN> tb = in + ((uint64_t)tile << kTileBits) + lane + (wave << 10);
 #pragma unroll
-		for (int j = 0; j < kTileRegs; ++j)
-			v[j] = static_cast<double>(tb[((uint32_t)(j & 15) << 6) | ((uint32_t)(j >> 4) << 14)]) * inv;
-	} else {
-#pragma unroll
-		for (int j = 0; j < kTileRegs; ++j) {
-			const uint32_t i = tile_index_a(lane, wave, j);
-			const uint64_t g = cbase + (i & cmask) + ((uint64_t)(i >> logc) << kTileBits);
-			v[j] = static_cast<double>(in[g]) * inv;
-		}
+	for (int j = 0; j < kTileRegs; ++j) {
+		const uint32_t i = tile_index_a(lane, wave, j);
+		// state cbase + (i & cmask) + (row << 15); z rows are z_pitch apart
+		const uint64_t g = FROM_POST ? cbase + (i & cmask) + ((uint64_t)(i >> logc) << kTileBits)
+									 : cbase + (i & cmask) + (uint64_t)(i >> logc) * z_pitch<TIN>();
+		v[j] = static_cast<double>(in[g]) * inv;
 	}
 	ObjCtx c;
 	c.objective = objective;
@@ -538,6 +532,323 @@ __global__ __launch_bounds__(kTileThreads) void k_zeta_final(Descs descs, int lo
 		consume(c, g, m, t);
 	}
 	publish_top2(d, tile, t, red2);
+}
+
+// ------------------------------------------- pass 2 for lattices of 21..30 atoms (n >= 21)
+//
+// A pass-2 tile is 2^LC contiguous columns (state bits 0..LC-1) by 2^(15-LC) rows (state
+// bits 15..A-1) of one column block (state bits LC..14 = tile id), LC = 30 - A; only the
+// row bits are swept. Tile bit t (t < LC: column bit t; t >= LC: state bit t - LC + 15) is
+// placed at load time as:
+//   lanes:     column bits 0..min(LC,6)-1, then (LC < 6) tile bits 9+LC..14
+//   registers: tile bits LC..LC+4 (the five lowest row bits)
+//   waves:     tile bits LC+5..LC+8 (LC <= 6), else column bits 6..LC-1 and rows 11+w
+// so a wave load reads runs of 2^min(LC,6) contiguous entries and the sweep needs one block
+// transpose (waves <-> registers) plus, for LC <= 3, one wave transpose, and permlane swaps
+// for lane bits 4 and 5. Bits are still swept in increasing order.
+namespace p2 {
+
+struct Map {
+	int lane[6];
+	int wave[4];
+	int reg[5];
+};
+
+constexpr Map initial_map(int lc)
+{
+	Map m{};
+	for (int l = 0; l < 6; ++l)
+		m.lane[l] = (lc >= 6 || l < lc) ? l : 9 + l;
+	for (int w = 0; w < 4; ++w)
+		m.wave[w] = lc <= 6 ? lc + 5 + w : (w < lc - 6 ? 6 + w : 11 + w);
+	for (int r = 0; r < 5; ++r)
+		m.reg[r] = lc + r;
+	return m;
+}
+
+// after the block transpose (register bits 0..3 <-> wave bits 0..3)
+constexpr Map block_map(int lc)
+{
+	Map m = initial_map(lc);
+	for (int b = 0; b < 4; ++b) {
+		const int t = m.reg[b];
+		m.reg[b] = m.wave[b];
+		m.wave[b] = t;
+	}
+	return m;
+}
+
+// after the wave transpose (lane bits 0..3 <-> register bits 0..3), LC <= 3
+constexpr Map wave_map(int lc)
+{
+	Map m = block_map(lc);
+	if (lc <= 3)
+		for (int b = 0; b < 4; ++b) {
+			const int t = m.reg[b];
+			m.reg[b] = m.lane[b];
+			m.lane[b] = t;
+		}
+	return m;
+}
+
+// after the permlane swaps (lane bit 4 <-> register bit 0 for LC <= 4, lane bit 5 <->
+// register bit 1 for LC <= 5): the layout the objective is evaluated in
+constexpr Map final_map(int lc)
+{
+	Map m = wave_map(lc);
+	if (lc <= 4) {
+		const int t = m.reg[0];
+		m.reg[0] = m.lane[4];
+		m.lane[4] = t;
+	}
+	if (lc <= 5) {
+		const int t = m.reg[1];
+		m.reg[1] = m.lane[5];
+		m.lane[5] = t;
+	}
+	return m;
+}
+
+// offset of tile bit t in the state index, relative to the tile's first column
+constexpr uint32_t weight(int lc, int t) { return t < lc ? (1u << t) : (1u << (t - lc + kTileBits)); }
+// offset of tile bit t in the intermediate z (rows are `pitch` entries apart)
+constexpr uint64_t zweight(int lc, int t, uint32_t pitch) { return t < lc ? (1ull << t) : ((uint64_t)pitch << (t - lc)); }
+
+constexpr uint32_t reg_offset(const Map &m, int lc, int j)
+{
+	uint32_t o = 0;
+	for (int r = 0; r < 5; ++r)
+		if ((j >> r) & 1)
+			o += weight(lc, m.reg[r]);
+	return o;
+}
+
+// registers in increasing state order (the scan keeps the first of equal objectives,
+// which is then the lowest experiment, as the reference's serial scan does)
+struct Order {
+	int j[kTileRegs];
+};
+constexpr Order scan_order(int lc)
+{
+	Order o{};
+	const Map m = final_map(lc);
+	uint32_t key[kTileRegs] = {};
+	for (int j = 0; j < kTileRegs; ++j) {
+		o.j[j] = j;
+		key[j] = reg_offset(m, lc, j);
+	}
+	for (int a = 1; a < kTileRegs; ++a)
+		for (int b = a; b > 0 && key[b - 1] > key[b]; --b) {
+			const uint32_t tk = key[b];
+			key[b] = key[b - 1];
+			key[b - 1] = tk;
+			const int tj = o.j[b];
+			o.j[b] = o.j[b - 1];
+			o.j[b - 1] = tj;
+		}
+	return o;
+}
+
+// state-index offset of (lane, wave) in the layout the objective is evaluated in
+template <int LC>
+__device__ __forceinline__ uint32_t thread_state_offset(int lane, int wave)
+{
+	constexpr Map m = final_map(LC);
+	uint32_t o = 0;
+#pragma unroll
+	for (int l = 0; l < 6; ++l)
+		o += ((lane >> l) & 1) ? weight(LC, m.lane[l]) : 0u;
+#pragma unroll
+	for (int w = 0; w < 4; ++w)
+		o += ((wave >> w) & 1) ? weight(LC, m.wave[w]) : 0u;
+	return o;
+}
+
+// z offset of (lane, wave) in the load layout
+template <int LC, uint32_t PITCH>
+__device__ __forceinline__ uint64_t thread_z_offset(int lane, int wave)
+{
+	constexpr Map m = initial_map(LC);
+	uint64_t o = 0;
+#pragma unroll
+	for (int l = 0; l < 6; ++l)
+		o += ((lane >> l) & 1) ? zweight(LC, m.lane[l], PITCH) : 0ull;
+#pragma unroll
+	for (int w = 0; w < 4; ++w)
+		o += ((wave >> w) & 1) ? zweight(LC, m.wave[w], PITCH) : 0ull;
+	return o;
+}
+
+// largest z offset of any (lane, wave) in the load layout
+constexpr uint64_t max_thread_z_offset(int lc, uint32_t pitch)
+{
+	const Map m = initial_map(lc);
+	uint64_t o = 0;
+	for (int l = 0; l < 6; ++l)
+		o += zweight(lc, m.lane[l], pitch);
+	for (int w = 0; w < 4; ++w)
+		o += zweight(lc, m.wave[w], pitch);
+	return o;
+}
+
+template <int LC>
+__device__ __forceinline__ uint32_t reg_offset_rt(int j)
+{
+	constexpr Map m = final_map(LC);
+	uint32_t o = 0;
+#pragma unroll
+	for (int r = 0; r < 5; ++r)
+		o += ((j >> r) & 1) ? weight(LC, m.reg[r]) : 0u;
+	return o;
+}
+
+// Per-thread top-2 of one tile in scan order: strict comparisons keep the earlier (lower)
+// state on equal objectives. BBPA: |p - t| + |(T - p) - t|; otherwise |p - 1/2|.
+template <int LC, bool BBPA, bool MASKED>
+__device__ __forceinline__ void scan_tile(const double (&v)[kTileRegs], double scale, double total, double target,
+										  uint32_t gt, uint32_t inactive, double &d1, double &d2, int &j1, int &j2)
+{
+	constexpr Order ord = scan_order(LC);
+	constexpr Map mf = final_map(LC);
+#pragma unroll
+	for (int q = 0; q < kTileRegs; ++q) {
+		const int j = ord.j[q];
+		const double p = v[j] * scale;
+		double dd;
+		if (BBPA) {
+			const double negative = total - p;
+			dd = fabs(p - target) + fabs(negative - target);
+		} else {
+			dd = fabs(p - 0.5);
+		}
+		if (MASKED) {
+			const uint32_t g = gt + reg_offset(mf, LC, j);
+			dd = (g & inactive) ? INFINITY : dd;
+		}
+		// one select per update keeps this branch-free
+		const bool c1 = dd < d1, c2 = dd < d2;
+		j2 = c2 ? j : j2;
+		d2 = c2 ? dd : d2;
+		j2 = c1 ? j1 : j2;
+		d2 = c1 ? d1 : d2;
+		j1 = c1 ? j : j1;
+		d1 = c1 ? dd : d1;
+		// keep the scheduler from hoisting every product ahead of the selects
+		__builtin_amdgcn_sched_barrier(0);
+	}
+}
+
+} // namespace p2
+
+// Pass 2 for A = 30 - LC atoms, persistent: workgroup b of G serves the tiles of XCD b % 8
+// (contiguous range, neighbours run on CUs sharing one L2). Objective: kObjHalving or
+// kObjBinaryBbpa; atom singletons are written by the tiles that hold them (tile 0 and
+// power-of-two tiles).
+template <typename TIN, int LC>
+__global__ __launch_bounds__(kTileThreads) void k_zeta_pass2(Descs descs, int objective, int scaled, double target)
+{
+	constexpr p2::Map mb = p2::block_map(LC);
+	constexpr p2::Map mw = p2::wave_map(LC);
+	constexpr unsigned tiles = 1u << (kTileBits - LC);
+	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
+	__shared__ double lds[kTileLds];
+	__shared__ Top2 red2[16];
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const unsigned wg_per_xcd = gridDim.x >> 3;
+	const unsigned per_xcd = tiles >> 3;
+	const unsigned xcd = blockIdx.x & 7u;
+	const unsigned first = xcd * per_xcd + (blockIdx.x >> 3), last = (xcd + 1) * per_xcd;
+	constexpr uint32_t pitch = z_pitch<TIN>();
+	const uint64_t lo = p2::thread_z_offset<LC, pitch>(lane, wave);
+	const uint32_t lf = p2::thread_state_offset<LC>(lane, wave);
+	const TIN *in = static_cast<const TIN *>(d.z);
+	// per-lane byte offset from a wave-uniform base; at 30 atoms (and FP64 at 29) it no
+	// longer fits 32 bits and the lane's own 64-bit pointer is used instead
+	constexpr bool wide = p2::max_thread_z_offset(LC, pitch) * sizeof(TIN) >= (1ull << 32);
+	const uint32_t lob = wide ? 0u : (uint32_t)(lo * sizeof(TIN));
+	const TIN *inw = wide ? in + lo : in;
+	const double scale = scaled ? d.scal[d.inv_out] : 1.0;
+	const double total = scaled ? d.scal[kScalTotal] * scale : d.scal[kScalTotal];
+	const uint32_t full = (d.n >= 32) ? 0xFFFFFFFFu : ((1u << d.n) - 1u);
+	const uint32_t inactive = full & ~d.active;
+	const bool bbpa = objective == kObjBinaryBbpa;
+	Top2 run;
+	top2_init(run);
+	for (unsigned T = first; T < last; T += wg_per_xcd) {
+		// (prefetching the next tile into registers costs more in spills than it hides:
+		// the tile alone takes 64 of the 128 VGPRs a 1024-thread workgroup allows)
+		double v[kTileRegs];
+		{
+			const TIN *tb = inw + ((uint64_t)T << LC);
+#pragma unroll
+			for (int j = 0; j < kTileRegs; ++j)
+				v[j] = static_cast<double>(ld_u(tb + (size_t)j * pitch, lob));
+		}
+		// registers: tile bits LC..LC+4
+		bfly_reg<0>(v);
+		bfly_reg<1>(v);
+		bfly_reg<2>(v);
+		bfly_reg<3>(v);
+		bfly_reg<4>(v);
+		__syncthreads(); // the previous tile's wave transposes used the same LDS
+		block_transpose(v, lds, lane, wave);
+		if constexpr (mb.reg[0] >= LC)
+			bfly_reg<0>(v);
+		if constexpr (mb.reg[1] >= LC)
+			bfly_reg<1>(v);
+		if constexpr (mb.reg[2] >= LC)
+			bfly_reg<2>(v);
+		if constexpr (mb.reg[3] >= LC)
+			bfly_reg<3>(v);
+		if constexpr (LC <= 3) {
+			wave_transpose(v, lds + wave * (16 * kWaveStride), lane);
+			if constexpr (mw.reg[0] >= LC)
+				bfly_reg<0>(v);
+			if constexpr (mw.reg[1] >= LC)
+				bfly_reg<1>(v);
+			if constexpr (mw.reg[2] >= LC)
+				bfly_reg<2>(v);
+			if constexpr (mw.reg[3] >= LC)
+				bfly_reg<3>(v);
+		}
+		if constexpr (LC <= 4) {
+			perm_swap<false, 0>(v);
+			bfly_reg<0>(v);
+		}
+		if constexpr (LC <= 5) {
+			perm_swap<true, 1>(v);
+			bfly_reg<1>(v);
+		}
+		const uint32_t gt = ((uint32_t)T << LC) + lf;
+		if ((T & (T - 1u)) == 0u) {
+			// tile 0 and power-of-two tiles hold the singletons {a}
+			constexpr p2::Map mf = p2::final_map(LC);
+#pragma unroll
+			for (int j = 0; j < kTileRegs; ++j) {
+				const uint32_t g = gt + p2::reg_offset(mf, LC, j);
+				if (__popc(g) == 1)
+					d.masses[__ffs(g) - 1] = v[j] * scale;
+			}
+		}
+		double d1 = INFINITY, d2 = INFINITY;
+		int j1 = 0, j2 = 0;
+		if (inactive == 0u) {
+			if (bbpa)
+				p2::scan_tile<LC, true, false>(v, scale, total, target, gt, inactive, d1, d2, j1, j2);
+			else
+				p2::scan_tile<LC, false, false>(v, scale, total, target, gt, inactive, d1, d2, j1, j2);
+		} else {
+			if (bbpa)
+				p2::scan_tile<LC, true, true>(v, scale, total, target, gt, inactive, d1, d2, j1, j2);
+			else
+				p2::scan_tile<LC, false, true>(v, scale, total, target, gt, inactive, d1, d2, j1, j2);
+		}
+		if (d1 < INFINITY)
+			top2_insert(run, d1, gt + p2::reg_offset_rt<LC>(j1));
+		if (d2 < INFINITY)
+			top2_insert(run, d2, gt + p2::reg_offset_rt<LC>(j2));
+	}
+	publish_top2(d, blockIdx.x, run, red2);
 }
 
 // Lattices of at most 2^14 states: the whole transform in the LDS of one workgroup.
@@ -1014,16 +1325,15 @@ void launch_checked(const char *name, F f)
 	check_hip(hipGetLastError(), name);
 }
 
-// Layout of the transform intermediate between the two passes: natural (state order) by
-// default; BGT_HIP_ZLAYOUT=blocked groups it by pass-2 tile (contiguous pass-2 reads, but
-// pass 1 then scatters 16 B runs and is slower overall on gfx950).
-bool blocked_intermediate()
+int compute_units()
 {
-	static const bool blocked = [] {
-		const char *v = std::getenv("BGT_HIP_ZLAYOUT");
-		return v && std::string(v) == "blocked";
+	static const int cus = [] {
+		int dev = 0, v = 0;
+		if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0)
+			v = 256;
+		return v;
 	}();
-	return blocked;
+	return cus;
 }
 
 unsigned grid_for(uint64_t work, unsigned per_block, unsigned cap)
@@ -1069,23 +1379,22 @@ void launch_zeta_pass1(const Descs &descs, int count, int n, Storage st, bool fu
 					   hipStream_t stream)
 {
 	const dim3 grid(1u << (n - kTileBits), count);
-	const int lb = blocked_intermediate() ? (2 * kTileBits - n) : -1;
 	const bool binary_nd = (k == 1 && !dilution);
 	launch_checked("k_zeta_pass1", [&] {
 		if (st == Storage::fp64) {
 			if (fused && binary_nd)
-				hipLaunchKernelGGL((k_zeta_pass1<double, double, true, 1>), grid, dim3(kTileThreads), 0, stream, descs, lb);
+				hipLaunchKernelGGL((k_zeta_pass1<double, double, true, 1>), grid, dim3(kTileThreads), 0, stream, descs);
 			else if (fused)
-				hipLaunchKernelGGL((k_zeta_pass1<double, double, true, 0>), grid, dim3(kTileThreads), 0, stream, descs, lb);
+				hipLaunchKernelGGL((k_zeta_pass1<double, double, true, 0>), grid, dim3(kTileThreads), 0, stream, descs);
 			else
-				hipLaunchKernelGGL((k_zeta_pass1<double, double, false, 0>), grid, dim3(kTileThreads), 0, stream, descs, lb);
+				hipLaunchKernelGGL((k_zeta_pass1<double, double, false, 0>), grid, dim3(kTileThreads), 0, stream, descs);
 		} else {
 			if (fused && binary_nd)
-				hipLaunchKernelGGL((k_zeta_pass1<float, float, true, 1>), grid, dim3(kTileThreads), 0, stream, descs, lb);
+				hipLaunchKernelGGL((k_zeta_pass1<float, float, true, 1>), grid, dim3(kTileThreads), 0, stream, descs);
 			else if (fused)
-				hipLaunchKernelGGL((k_zeta_pass1<float, float, true, 0>), grid, dim3(kTileThreads), 0, stream, descs, lb);
+				hipLaunchKernelGGL((k_zeta_pass1<float, float, true, 0>), grid, dim3(kTileThreads), 0, stream, descs);
 			else
-				hipLaunchKernelGGL((k_zeta_pass1<float, float, false, 0>), grid, dim3(kTileThreads), 0, stream, descs, lb);
+				hipLaunchKernelGGL((k_zeta_pass1<float, float, false, 0>), grid, dim3(kTileThreads), 0, stream, descs);
 		}
 		hipLaunchKernelGGL(k_finalize_sum, dim3(1, count), dim3(1024), 0, stream, descs, grid.x,
 						   (int)(fused ? kSumDenominator : kSumTotal));
@@ -1095,6 +1404,39 @@ void launch_zeta_pass1(const Descs &descs, int count, int n, Storage st, bool fu
 void launch_zeta_final(const Descs &descs, int count, int n, int, Storage st, int objective,
 					   bool from_posterior, bool scaled, double target, hipStream_t stream)
 {
+	if (!from_posterior && n >= 21 && n <= 30 && (objective == kObjHalving || objective == kObjBinaryBbpa)) {
+		// persistent pass 2: one workgroup per CU (a multiple of 8, one share per XCD)
+		const unsigned tiles = 1u << (n - kTileBits);
+		unsigned g = std::min<unsigned>(tiles, (unsigned)compute_units());
+		g = std::max(8u, g & ~7u);
+		const dim3 grid(g, count);
+		launch_checked("k_zeta_pass2", [&] {
+#define BGT_P2(LCV)                                                                                            \
+	case LCV:                                                                                                  \
+		if (st == Storage::fp64)                                                                               \
+			hipLaunchKernelGGL((k_zeta_pass2<double, LCV>), grid, dim3(kTileThreads), 0, stream, descs, objective, \
+							   scaled ? 1 : 0, target);                                                        \
+		else                                                                                                   \
+			hipLaunchKernelGGL((k_zeta_pass2<float, LCV>), grid, dim3(kTileThreads), 0, stream, descs, objective,  \
+							   scaled ? 1 : 0, target);                                                        \
+		break;
+			switch (2 * kTileBits - n) {
+				BGT_P2(0)
+				BGT_P2(1)
+				BGT_P2(2)
+				BGT_P2(3)
+				BGT_P2(4)
+				BGT_P2(5)
+				BGT_P2(6)
+				BGT_P2(7)
+				BGT_P2(8)
+				BGT_P2(9)
+			}
+#undef BGT_P2
+			hipLaunchKernelGGL(k_finalize_top2, dim3(1, count), dim3(1024), 0, stream, descs, grid.x);
+		});
+		return;
+	}
 	// n == 15: one tile, contiguous, read from the posterior.
 	// n > 15: 2^(n-15) tiles of 2^(30-n) columns by 2^(n-15) rows, read from z.
 	const int logc = from_posterior ? kTileBits : (2 * kTileBits - n);
@@ -1104,17 +1446,17 @@ void launch_zeta_final(const Descs &descs, int count, int n, int, Storage st, in
 		if (from_posterior) {
 			if (st == Storage::fp64)
 				hipLaunchKernelGGL((k_zeta_final<double, true>), grid, dim3(kTileThreads), 0, stream, descs, logc, q0,
-								   objective, 0, target, 0);
+								   objective, 0, target);
 			else
 				hipLaunchKernelGGL((k_zeta_final<float, true>), grid, dim3(kTileThreads), 0, stream, descs, logc, q0,
-								   objective, 0, target, 0);
+								   objective, 0, target);
 		} else {
 			if (st == Storage::fp64)
 				hipLaunchKernelGGL((k_zeta_final<double, false>), grid, dim3(kTileThreads), 0, stream, descs, logc,
-								   q0, objective, scaled ? 1 : 0, target, blocked_intermediate() ? 1 : 0);
+								   q0, objective, scaled ? 1 : 0, target);
 			else
 				hipLaunchKernelGGL((k_zeta_final<float, false>), grid, dim3(kTileThreads), 0, stream, descs, logc,
-								   q0, objective, scaled ? 1 : 0, target, blocked_intermediate() ? 1 : 0);
+								   q0, objective, scaled ? 1 : 0, target);
 		}
 		hipLaunchKernelGGL(k_finalize_top2, dim3(1, count), dim3(1024), 0, stream, descs, grid.x);
 	});

@@ -59,8 +59,10 @@ hipEvent_t KernelClock::take()
 		pool_.pop_back();
 		return e;
 	}
+	// timing-only events: no system-scope fence (which would write back and invalidate the
+	// L2 of every XCD at each record)
 	hipEvent_t e;
-	BGT_HIP_CHECK(hipEventCreate(&e));
+	BGT_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
 	return e;
 }
 
@@ -237,8 +239,9 @@ void DeviceLattice::ensure_z()
 	if (z_ || atoms_ <= kTileBits)
 		return;
 	const size_t eb = storage_ == Storage::fp64 ? 8 : 4;
-	BGT_HIP_CHECK(hipMalloc(&z_, eb * states_));
-	bytes_ += eb * states_;
+	const uint64_t zb = z_bytes(atoms_, eb);
+	BGT_HIP_CHECK(hipMalloc(&z_, zb));
+	bytes_ += zb;
 }
 
 DeviceLattice::~DeviceLattice()

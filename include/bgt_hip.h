@@ -150,6 +150,12 @@ int bgt_hip_lattice_classify(bgt_hip_lattice *lattice, double threshold_up, doub
  * (lattice_api.cpp:181-191) in internal coordinates; reads the handle's sticky masks */
 int bgt_hip_lattice_masks(bgt_hip_lattice *lattice, uint32_t *positive_mask, uint32_t *negative_mask);
 
+/* Restrict the experiments the selectors consider to subsets of `subjects_mask` (internal
+ * coordinates; all subjects by default). The tree builder sets it to the unclassified
+ * subjects, the effect the reference's shrinking tree gets by removing classified subjects
+ * from the lattice (update_metadata_with_shrinking, lattice_base.cpp:266-305). */
+int bgt_hip_lattice_set_candidates(bgt_hip_lattice *lattice, uint32_t subjects_mask);
+
 /* k-step Op-BHA lookahead (lattice_api.cpp:116-168) in the reference's PUBLIC coordinates
  * (binary lattices only) */
 int bgt_hip_lattice_select_experiments(bgt_hip_lattice *lattice, int steps, uint32_t *out);

@@ -1,0 +1,176 @@
+"""Lattices of 21..30 atoms: the persistent pass-2 kernel, one instantiation per column count
+(30 - n columns per tile), on the fused round and on select().
+
+Bars: FP64 as the plain form (masses 1e-12, objectives 1e-11, the oracle's experiment unless
+the pick is a reported near tie); FP32C entries, masses and objectives within 1e-6. At these
+sizes the reference's own serial sums (the denominator D of update_probs_in_place and the
+atom masses) carry rounding of order sqrt(2^n) ulp ~ 1e-12, so FP64 masses are compared with
+pairwise (numpy) sums of the oracle's posterior after removing the common normalisation
+factor, which itself must agree to 1e-9. At 29 and
+30 subjects the CPU oracle would need 8-16 GB and minutes, so those sizes are checked against
+closed forms instead: a prior lattice's upset mass is the product of the atoms' negative
+probabilities, and after one binary non-dilution update it is a two-term mixture of such
+products.
+"""
+import numpy as np
+import pytest
+
+import bayesiangrouptesting_amd as bgt
+from bayesiangrouptesting_amd import workload
+from oracle import pyoracle as orc
+from tests.gpu_helpers import rel_err, seeded_prior
+
+pytestmark = pytest.mark.gpu
+ND = bgt.LatticeType.replicated_non_dilution
+DL = bgt.LatticeType.replicated_dilution
+
+# (subjects, dilution): every column count from 9 (n = 21) down to 2 (n = 28)
+LARGE = [(21, True), (22, False), (23, False), (24, True), (25, False), (26, True), (27, False), (28, False)]
+
+
+def accurate_masses(post, n):
+    """m_a = sum of the entries with bit a set, by numpy's pairwise summation."""
+    return np.array([post.reshape(-1, 2, 1 << a)[:, 1, :].sum() for a in range(n)])
+
+
+def _oracle_run(n, dilution, seed, rounds, precision, tol_mass, tol_obj):
+    pi = seeded_prior(seed, n, 0.005, 0.05)
+    ref = orc.prior(pi)
+    dil = orc.dilution_table(n, 0.05, 0.9) if dilution else None
+    t = bgt.DilutionTable(n, 0.05, 0.9) if dilution else None
+    truth = workload.true_state(seed, list(pi))
+    sampler = workload.ResponseSampler(seed, n, 1, truth, list(dil) if dil is not None else None)
+    lat = bgt.Lattice(DL if dilution else ND, n, pi, precision=precision)
+    e, upset = orc.op_bha(ref, n)
+    s0 = lat.select(bgt.SelectorType.op_bha)
+    got = orc.reverse_bits(s0.experiment, n)
+    assert abs(s0.objective - abs(upset[got] - 0.5)) <= tol_obj
+    assert got == e or s0.near_tie
+    masses = np.empty(n)
+    diverged = False
+    for i in range(rounds):
+        r = sampler.draw(e)
+        orc.update_in_place(ref, dilution, n, 1, e, r, dil)
+        m_ref = accurate_masses(ref, n)
+        nxt, upset = orc.op_bha(ref, n)
+        sel = lat.round(orc.reverse_bits(e, n), r, 0.01, 0.01, t, masses)
+        ratio = lat.upset_probability_mass(0) / ref.sum()
+        assert abs(ratio - 1.0) <= max(1e-9, tol_mass)
+        m_gpu = masses / ratio
+        assert np.max(np.abs(m_gpu - m_ref) / np.maximum(m_ref, 1e-300)) <= tol_mass, f"round {i}"
+        got = orc.reverse_bits(sel.experiment, n)
+        # the GPU's masses carry its own normalisation: compare objectives on a common one
+        assert abs(sel.objective - abs(upset[got] * ratio - 0.5)) <= tol_obj, f"round {i}"
+        if got != nxt:
+            # a different pick must be a reported near tie, or as good as the oracle's pick up
+            # to the oracle's own normalisation rounding
+            slack = 2 * abs(ratio - 1.0) + tol_obj
+            assert sel.near_tie or abs(upset[got] - 0.5) <= abs(upset[nxt] - 0.5) + slack, (i, got, nxt, sel)
+            diverged = True
+        e = nxt  # both sides follow the oracle's trajectory
+    return lat, ref, upset, diverged
+
+
+@pytest.mark.parametrize("n,dilution", LARGE)
+def test_large_rounds_and_selectors_fp64(n, dilution):
+    seed = 300 + n
+    rounds = 2 if n >= 27 else 3
+    lat, ref, upset, _ = _oracle_run(n, dilution, seed, rounds, bgt.Precision.float64, 1e-12, 1e-11)
+    post = lat.posterior()
+    assert rel_err(post, ref, floor=1e-300) <= 1e-9
+    scale = post.sum() / ref.sum()
+    assert rel_err(post / scale, ref, floor=1e-300) <= 1e-12
+    del post
+    # BBPA over the binary response (pass 1 without the update, pass 2 with the BBPA objective)
+    ratio = lat.upset_probability_mass(0) / ref.sum()
+    slack = 2 * abs(ratio - 1.0) + 1e-11
+    e_b, best_b = orc.bbpa(ref, n, 1)
+    sb = lat.select(bgt.SelectorType.op_bbpa)
+    got = orc.reverse_bits(sb.experiment, n)
+    tot = ref.sum()
+
+    def bbpa_obj(e, sc):
+        return abs(upset[e] * sc - 0.5) + abs((tot - upset[e]) * sc - 0.5)
+
+    assert abs(sb.objective - bbpa_obj(got, ratio)) <= 1e-11
+    assert got == e_b or sb.near_tie or bbpa_obj(got, 1.0) <= best_b + slack
+    # Op-BHA restricted to a candidate subset of the subjects (the masked scan)
+    g = workload.SplitMix64(seed)
+    active = 0
+    for a in range(n):
+        if g.below(4) != 0:
+            active |= 1 << a
+    lat.restrict_candidates(orc.reverse_bits(active, n))
+    es = np.arange(1 << n, dtype=np.uint32)
+    obj = np.abs(upset - 0.5)
+    obj[(es & np.uint32(~active & ((1 << n) - 1))) != 0] = np.inf
+    want = int(np.argmin(obj))
+    del es
+    sm = lat.select(bgt.SelectorType.op_bha)
+    got = orc.reverse_bits(sm.experiment, n)
+    assert got & ~active == 0
+    assert abs(sm.objective - abs(upset[got] * ratio - 0.5)) <= 1e-11
+    assert got == want or sm.near_tie or obj[got] <= obj[want] + slack
+    lat.close()
+
+
+def test_n28_fp32c_rounds_within_1e6():
+    # the benchmark's shape: binary non-dilution, 28 subjects, FP32 storage
+    lat, ref, _, diverged = _oracle_run(28, False, 2, 3, bgt.Precision.float32_compensated, 1e-6, 1e-6)
+    post = lat.posterior()
+    big = ref >= 1e-30 * ref.sum()
+    assert rel_err(post[big], ref[big]) <= 1e-6
+    lat.close()
+
+
+def _closed_form_best(q):
+    """min over all E of |prod_{a in E} q_a - 1/2| by meet in the middle in log space."""
+    n = len(q)
+    h = n // 2
+    lq = np.log(q)
+
+    def subset_sums(v):
+        s = np.zeros(1)
+        for x in v:
+            s = np.concatenate([s, s + x])
+        return s
+
+    lo = subset_sums(lq[:h])
+    hi = subset_sums(lq[h:])
+    order = np.argsort(lo)
+    los = lo[order]
+    idx = np.searchsorted(los, np.log(0.5) - hi)
+    best = np.inf
+    for c in (idx - 1, idx):
+        c = np.clip(c, 0, len(los) - 1)
+        best = min(best, float(np.min(np.abs(np.exp(hi + los[c]) - 0.5))))
+    return best
+
+
+@pytest.mark.parametrize("n", [29, 30])
+def test_largest_lattices_closed_form_fp32c(n):
+    pi = seeded_prior(500 + n, n, 0.005, 0.05)
+    q = 1.0 - pi  # probability that atom a is negative (bit set)
+    lat = bgt.Lattice(ND, n, pi, precision=bgt.Precision.float32_compensated)
+    sel = lat.select(bgt.SelectorType.op_bha)
+    e = orc.reverse_bits(sel.experiment, n)
+    me = float(np.prod([q[a] for a in range(n) if (e >> a) & 1]))
+    _, mass_rel = lat.error_bound()
+    assert abs(sel.objective - abs(me - 0.5)) <= 2 * mass_rel
+    assert abs(me - 0.5) - _closed_form_best(q) <= 2 * mass_rel + 1e-12
+    # one fused round: the updated upset masses are L_out M(F) + (L_in - L_out) M(F u E), over Z
+    r = 1
+    l_in, l_out = 0.99, 0.01
+    z = l_out + (l_in - l_out) * me
+    masses = np.empty(n)
+    sel2 = lat.round(sel.experiment, r, 0.01, 0.01, None, masses)
+    want = np.array([(l_out * q[a] + (l_in - l_out) * (me if (e >> a) & 1 else me * q[a])) / z for a in range(n)])
+    _, mass_rel = lat.error_bound()
+    assert np.max(np.abs(masses - want) / want) <= 1e-6
+    assert np.max(np.abs(lat.atom_masses() - want) / want) <= 1e-6
+    e2 = orc.reverse_bits(sel2.experiment, n)
+    mf = float(np.prod([q[a] for a in range(n) if (e2 >> a) & 1]))
+    mfe = float(np.prod([q[a] for a in range(n) if ((e2 | e) >> a) & 1]))
+    m2 = (l_out * mf + (l_in - l_out) * mfe) / z
+    assert abs(sel2.objective - abs(m2 - 0.5)) <= 2 * mass_rel
+    lat.close()
