@@ -24,6 +24,7 @@ from .api import (  # noqa: F401
     TreeStats,
     Unsupported,
     available_providers,
+    exact_upset_mass,
     hip_available,
     reverse_bits,
     run_simulation,
@@ -32,6 +33,7 @@ from .api import (  # noqa: F401
 __all__ = [
     "DilutionParameters", "DilutionMode", "DilutionTable", "Error", "InvalidArgument", "KernelSpec", "Lattice",
     "LatticeType", "Precision", "Provider", "Runtime", "Selection", "SelectorType", "SimulationConfig",
-    "SimulationOptions", "SimulationResult", "TreeStats", "Unsupported", "available_providers", "hip_available",
+    "SimulationOptions", "SimulationResult", "TreeStats", "Unsupported", "available_providers", "exact_upset_mass",
+    "hip_available",
     "reverse_bits", "run_simulation",
 ]

@@ -43,6 +43,8 @@ class SelectionC(ctypes.Structure):
         ("gap_bound", ctypes.c_double),
         ("near_tie", ctypes.c_int),
         ("redecided", ctypes.c_int),
+        ("candidates", ctypes.c_int),
+        ("band", ctypes.c_int),
     ]
 
 
@@ -99,6 +101,7 @@ def _sig(L):
                                   ctypes.POINTER(c_u64)], c_int),
         "bgt_hip_batch_step": ([ctypes.POINTER(_H), _u32p, _u32p, ctypes.POINTER(_dptr), c_int, c_int,
                                 ctypes.POINTER(SelectionC)], c_int),
+        "bgt_hip_exact_upset_mass": ([c_int, _dptr, c_int, _u32p, _u32p, c_int, _u32p, _dptr], c_int),
         "bgt_hip_last_error": ([], ctypes.c_char_p),
         "bgt_hip_device_synchronize": ([], c_int),
         "bgt_hip_device_name": ([ctypes.c_char_p, c_int], c_int),

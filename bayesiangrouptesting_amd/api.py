@@ -111,13 +111,16 @@ class Selection:
     runner_up_objective: float
     gap_bound: float
     near_tie: bool
-    redecided: bool
+    redecided: int  # 0 no, 1 exact over the whole error band, 2 exact over the top two only
+    candidates: int = 0
+    band: int = 0
 
     @classmethod
     def from_c(cls, s: nat.SelectionC, flip=None) -> "Selection":
         f = flip or (lambda v: v)
         return cls(f(s.experiment), f(s.runner_up) if s.runner_up != 0xFFFFFFFF else s.runner_up,
-                   s.objective, s.runner_up_objective, s.gap_bound, bool(s.near_tie), bool(s.redecided))
+                   s.objective, s.runner_up_objective, s.gap_bound, bool(s.near_tie), int(s.redecided),
+                   int(s.candidates), int(s.band))
 
 
 class Lattice:
@@ -292,6 +295,25 @@ class Lattice:
         b = ctypes.c_double()
         nat.check(nat.lib().bgt_hip_lattice_error_bound(self._h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
+
+
+def exact_upset_mass(subjects: int, prior: Sequence[float], history: Sequence[tuple[int, int]],
+                     experiments) -> "float | np.ndarray":
+    """Exact FP64 upset mass(es) of a binary non-dilution lattice after `history` updates
+    (internal coordinates), from the prior and the history alone (no device involved).
+    `experiments` is one experiment or a sequence of them."""
+    pi = np.ascontiguousarray(np.asarray(prior, dtype=np.float64))
+    es = np.ascontiguousarray(np.array([e for e, _ in history], dtype=np.uint32))
+    rs = np.ascontiguousarray(np.array([r for _, r in history], dtype=np.uint32))
+    scalar = np.isscalar(experiments)
+    qs = np.ascontiguousarray(np.atleast_1d(np.asarray(experiments, dtype=np.uint32)))
+    out = np.empty(qs.size, dtype=np.float64)
+    u32p = ctypes.POINTER(ctypes.c_uint32)
+    dptr = ctypes.POINTER(ctypes.c_double)
+    nat.check(nat.lib().bgt_hip_exact_upset_mass(subjects, pi.ctypes.data_as(dptr), len(history),
+                                                 es.ctypes.data_as(u32p), rs.ctypes.data_as(u32p), qs.size,
+                                                 qs.ctypes.data_as(u32p), out.ctypes.data_as(dptr)))
+    return float(out[0]) if scalar else out
 
 
 # ------------------------------------------------------------------------------ simulation

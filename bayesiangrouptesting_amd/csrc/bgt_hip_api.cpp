@@ -1,5 +1,6 @@
 // extern "C" boundary of the provider (include/bgt_hip.h). No exception crosses it.
 #include "bgt_hip.h"
+#include "bgt_hip_exact.hpp"
 #include "bgt_hip_lattice.hpp"
 #include "bgt_hip_tree.hpp"
 
@@ -253,6 +254,27 @@ int bgt_hip_lattice_masks(bgt_hip_lattice *h, uint32_t *pos, uint32_t *neg)
 			*pos = l.positive_mask();
 		if (neg)
 			*neg = l.negative_mask();
+	});
+}
+
+int bgt_hip_exact_upset_mass(int subjects, const double *prior, int updates, const uint32_t *experiments,
+							 const uint32_t *responses, int count, const uint32_t *queries, double *out)
+{
+	return guarded([&] {
+		if (subjects <= 0 || subjects > 32 || prior == nullptr || updates < 0 || count < 0 ||
+			(updates > 0 && (experiments == nullptr || responses == nullptr)) ||
+			(count > 0 && (queries == nullptr || out == nullptr)))
+			throw bgt_hip::Error(BGT_HIP_BAD_INPUT, "bad arguments");
+		bgt_hip::ExactHistory h;
+		h.reset(true, subjects, prior);
+		for (int t = 0; t < updates; ++t)
+			h.push(experiments[t], responses[t] & 1u);
+		std::unique_ptr<bool[]> ok(new bool[count > 0 ? count : 1]);
+		if (!h.masses(queries, (size_t)count, out, ok.get()))
+			throw bgt_hip::Error(BGT_HIP_UNSUPPORTED, "history exceeds the exact program's state budget");
+		for (int i = 0; i < count; ++i)
+			if (!ok[i])
+				throw bgt_hip::Error(BGT_HIP_UNSUPPORTED, "query exceeds the exact program's state budget");
 	});
 }
 

@@ -1,0 +1,69 @@
+// Exact upset masses of a binary non-dilution lattice, recomputed in FP64 from the prior and
+// the applied updates rather than from the stored posterior. Used to re-decide selections
+// whose best candidates are closer than the stored form's error (FP32C storage carries
+// ~1e-7 relative error per entry while top-two gaps at 2^28 states are ~1e-9).
+//
+// With prior P(S) = prod_a (bit a of S ? q_a : pi_a), q_a = 1 - pi_a, and updates
+// (E_t, r_t) whose likelihood depends only on c_t(S) = [S contains E_t]
+// (replicated_lattice.cpp:8-28, non-dilution), the unnormalised upset mass is
+//   U(E) = sum_{S >= E} P(S) prod_t f_t(c_t(S))
+//        = Q(E) * prod_{t : E_t <= E} f_t(1) * sum over the free atoms of the factors of
+//          the remaining pools A_t = E_t \ E,
+// which a dynamic program over the set of pools that can still be contained evaluates as a
+// sum of non-negative terms. The normaliser is the same program for E = {} (all atoms free),
+// so M(E) = U(E) / U({}) carries only a few ulp of rounding. (A recurrence on the per-update
+// denominators would subtract nearly equal masses whenever a pool is likely negative.) The
+// programs stay small in practice: adaptive experiments reuse the same atoms, so the pool
+// sets that can still be contained number in the tens, not 2^updates.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+namespace bgt_hip {
+
+class ExactHistory {
+public:
+	// binary non-dilution lattices only; anything else leaves the history unusable
+	void reset(bool usable, int atoms, const double *prior);
+	bool usable() const { return usable_ && !broken_; }
+	void push(uint32_t experiment, uint32_t response);
+	int updates() const { return (int)hist_.size(); }
+	// M(E) after all pushed updates; false if the state budget of the program is exceeded
+	bool mass(uint32_t experiment, double &out);
+	// M for `count` experiments (ok[i] false where the budget was exceeded); false if the
+	// normaliser itself is out of budget
+	bool masses(const uint32_t *experiments, size_t count, double *out, bool *ok);
+	// relative rounding bound of mass() (a few ulp per atom and per update)
+	double rel_bound() const;
+
+private:
+	struct Upd {
+		uint32_t e;
+		double f_in, f_out; // factor when the pool is / is not contained in S
+	};
+	struct Table {
+		std::vector<uint64_t> key;
+		std::vector<double> w;
+		std::vector<int32_t> slot;
+		void clear(size_t expect);
+		void add(uint64_t k, double v);
+	};
+	struct Scratch {
+		std::vector<uint32_t> pool;
+		std::vector<double> fin, fout;
+		Table a, b;
+	};
+	bool unnormalised(uint32_t e, double &out, Scratch &sc) const;
+	bool normaliser();
+
+	bool usable_ = false, broken_ = false;
+	int atoms_ = 0;
+	std::vector<double> pi_, q_;
+	std::vector<Upd> hist_;
+	double z_ = 1.0;   // U({}) for the current history
+	bool z_ok_ = true; // z_ is up to date
+	Scratch scratch_;
+};
+
+} // namespace bgt_hip

@@ -56,6 +56,17 @@ struct SelResult {
 	uint32_t e1, e2;
 };
 
+// One selection candidate: objective and experiment (internal coordinates).
+struct Cand {
+	double d;
+	uint32_t e;
+	uint32_t pad;
+};
+// Candidates within the error band of the best objective, collected for exact rescoring.
+constexpr int kBandCapacity = 4096;
+// The first kBandHead of them travel with the 2 KB result head in one device-to-host copy.
+constexpr int kBandHead = 64;
+
 // Everything a kernel needs to know about one lattice. Arrays of these live in device
 // memory so that one launch can serve a batch of lattices (blockIdx.y selects).
 struct LatDesc {
@@ -75,6 +86,9 @@ struct LatDesc {
 	uint32_t active;   // selectable subjects (experiments must be subsets)
 	int dilution;
 	int inv_in, inv_out; // scalar slot read / written for 1/D
+	Cand *cand;        // per-thread top-2 of the last selection (null: not collected)
+	Cand *band;        // kBandCapacity candidates within `band_width` of the best
+	double band_width; // error band of the objective (absolute)
 };
 
 constexpr int kLikCapacity = kMaxVariants * (kMaxSubjects + 1);
@@ -106,11 +120,17 @@ void launch_update(const Descs &descs, int count, uint64_t states, int k, int di
 void launch_zeta_pass1(const Descs &descs, int count, int n, Storage st, bool fused_update,
 					   int k, int dilution, hipStream_t stream);
 // pass 2 over bits 15..n-1 (or the whole transform when n == 15), with an objective
-void launch_zeta_final(const Descs &descs, int count, int n, int k, Storage st, int objective,
-					   bool from_posterior, bool scaled, double target, hipStream_t stream);
+unsigned launch_zeta_final(const Descs &descs, int count, int n, int k, Storage st, int objective,
+						   bool from_posterior, bool scaled, double target, hipStream_t stream);
 // n <= kSmallBits: whole transform in one workgroup per lattice
-void launch_zeta_small(const Descs &descs, int count, int n, int k, Storage st, int objective,
+unsigned launch_zeta_small(const Descs &descs, int count, int n, int k, Storage st, int objective,
 					   double target, hipStream_t stream);
+// workgroups of launch_zeta_final / launch_zeta_small for a lattice of `atoms` (1024 threads
+// each; the candidate buffer needs 2 entries per thread)
+unsigned zeta_final_groups(int atoms);
+// candidates of the last zeta_final within d.band_width of the best: d.band[0..counter[0]),
+// counter[1] != 0 when a thread's runner-up was inside the band (the list may be incomplete)
+void launch_band_collect(const Descs &descs, int count, unsigned entries, hipStream_t stream);
 void launch_bbpa_score(const Descs &descs, int count, int n, int k, double target,
 					   hipStream_t stream);
 void launch_brute_force(const Descs &descs, int count, int n, int k, Storage st,

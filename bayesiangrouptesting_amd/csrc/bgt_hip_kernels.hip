@@ -371,6 +371,23 @@ __device__ __forceinline__ void publish_top2(const LatDesc &d, unsigned slot, To
 		reinterpret_cast<Top2 *>(d.tops)[slot] = b;
 }
 
+// Per-thread top-2 for the band collection (exact rescoring of near ties): thread `slot`
+// of the launch writes entries 2*slot and 2*slot+1.
+__device__ __forceinline__ void write_cand(const LatDesc &d, unsigned slot, const Top2 &t)
+{
+	if (d.cand == nullptr)
+		return;
+	Cand a, b;
+	a.d = t.d1;
+	a.e = t.e1;
+	a.pad = 0;
+	b.d = t.d2;
+	b.e = t.e2;
+	b.pad = 0;
+	d.cand[2 * (size_t)slot] = a;
+	d.cand[2 * (size_t)slot + 1] = b;
+}
+
 // ----------------------------------------------------------------------------- kernels
 
 template <typename TP>
@@ -531,6 +548,7 @@ __global__ __launch_bounds__(kTileThreads) void k_zeta_final(Descs descs, int lo
 		const double m = scaled ? v[j] * scale : v[j];
 		consume(c, g, m, t);
 	}
+	write_cand(d, tile * blockDim.x + threadIdx.x, t);
 	publish_top2(d, tile, t, red2);
 }
 
@@ -848,6 +866,7 @@ __global__ __launch_bounds__(kTileThreads) void k_zeta_pass2(Descs descs, int ob
 		if (d2 < INFINITY)
 			top2_insert(run, d2, gt + p2::reg_offset_rt<LC>(j2));
 	}
+	write_cand(d, blockIdx.x * blockDim.x + threadIdx.x, run);
 	publish_top2(d, blockIdx.x, run, red2);
 }
 
@@ -897,12 +916,15 @@ __global__ __launch_bounds__(1024) void k_zeta_small(Descs descs, int atoms, int
 	top2_init(tt);
 	for (uint32_t s = threadIdx.x; s < states; s += blockDim.x)
 		consume(c, s, a[s], tt);
+	write_cand(d, threadIdx.x, tt);
 	Top2 r = top2_block(tt, red2);
 	if (threadIdx.x == 0) {
 		d.result->d1 = r.d1;
 		d.result->d2 = r.d2;
 		d.result->e1 = r.e1;
 		d.result->e2 = r.e2;
+		d.counter[0] = 0; // band collection (runs next, in stream order)
+		d.counter[1] = 0;
 	}
 }
 
@@ -1288,7 +1310,50 @@ __global__ __launch_bounds__(1024) void k_finalize_top2(Descs descs, unsigned co
 		d.result->d2 = r.d2;
 		d.result->e1 = r.e1;
 		d.result->e2 = r.e2;
+		d.counter[0] = 0; // band collection (runs next, in stream order)
+		d.counter[1] = 0;
 	}
+}
+
+// Candidates within band_width of the best objective. Entry 2i+1 is thread i's runner-up;
+// counter[1] counts runner-ups inside the band (threads that may hold further candidates).
+// One atomic per wave: when every candidate is inside the band (a posterior so concentrated
+// that all objectives sit at 1/2) per-entry atomics on one address would serialise.
+__global__ __launch_bounds__(256) void k_band_collect(Descs descs, unsigned entries)
+{
+	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
+	const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+	const int lane = threadIdx.x & 63;
+	Cand c;
+	c.d = INFINITY;
+	c.e = kNone;
+	c.pad = 0;
+	if (i < entries && d.cand != nullptr)
+		c = d.cand[i];
+	c.pad = i & 1u; // 1: a thread's runner-up (the thread may hold further candidates)
+	const double lim = d.result->d1 + d.band_width;
+	const bool in = c.e != kNone && c.d <= lim;
+	const unsigned long long mask = __ballot(in);
+	if (mask == 0ull)
+		return;
+	const int leader = __ffsll((long long)mask) - 1;
+	unsigned base = 0;
+	if (lane == leader) {
+		// once the buffer has overflowed only the count matters: stop adding to it
+		const unsigned seen = __hip_atomic_load(&d.counter[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		base = seen > (unsigned)kBandCapacity ? seen : atomicAdd(&d.counter[0], (unsigned)__popcll(mask));
+	}
+	base = __shfl(base, leader);
+	if (base > (unsigned)kBandCapacity)
+		return;
+	if (in) {
+		const unsigned idx = base + (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
+		if (idx < (unsigned)kBandCapacity)
+			d.band[idx] = c;
+	}
+	const unsigned long long ru = __ballot(in && (i & 1u));
+	if (ru != 0ull && lane == leader)
+		atomicAdd(&d.counter[1], (unsigned)__popcll(ru));
 }
 
 __global__ __launch_bounds__(1024) void k_finalize_masses(Descs descs, unsigned count, int atoms)
@@ -1401,15 +1466,24 @@ void launch_zeta_pass1(const Descs &descs, int count, int n, Storage st, bool fu
 	});
 }
 
-void launch_zeta_final(const Descs &descs, int count, int n, int, Storage st, int objective,
-					   bool from_posterior, bool scaled, double target, hipStream_t stream)
+unsigned zeta_final_groups(int atoms)
+{
+	if (atoms <= kTileBits)
+		return 1;
+	const unsigned tiles = 1u << (atoms - kTileBits);
+	if (atoms >= 21) {
+		unsigned g = std::min<unsigned>(tiles, (unsigned)compute_units());
+		return std::max(8u, g & ~7u);
+	}
+	return tiles;
+}
+
+unsigned launch_zeta_final(const Descs &descs, int count, int n, int, Storage st, int objective,
+						   bool from_posterior, bool scaled, double target, hipStream_t stream)
 {
 	if (!from_posterior && n >= 21 && n <= 30 && (objective == kObjHalving || objective == kObjBinaryBbpa)) {
 		// persistent pass 2: one workgroup per CU (a multiple of 8, one share per XCD)
-		const unsigned tiles = 1u << (n - kTileBits);
-		unsigned g = std::min<unsigned>(tiles, (unsigned)compute_units());
-		g = std::max(8u, g & ~7u);
-		const dim3 grid(g, count);
+		const dim3 grid(zeta_final_groups(n), count);
 		launch_checked("k_zeta_pass2", [&] {
 #define BGT_P2(LCV)                                                                                            \
 	case LCV:                                                                                                  \
@@ -1435,7 +1509,7 @@ void launch_zeta_final(const Descs &descs, int count, int n, int, Storage st, in
 #undef BGT_P2
 			hipLaunchKernelGGL(k_finalize_top2, dim3(1, count), dim3(1024), 0, stream, descs, grid.x);
 		});
-		return;
+		return 2u * grid.x * kTileThreads;
 	}
 	// n == 15: one tile, contiguous, read from the posterior.
 	// n > 15: 2^(n-15) tiles of 2^(30-n) columns by 2^(n-15) rows, read from z.
@@ -1460,9 +1534,10 @@ void launch_zeta_final(const Descs &descs, int count, int n, int, Storage st, in
 		}
 		hipLaunchKernelGGL(k_finalize_top2, dim3(1, count), dim3(1024), 0, stream, descs, grid.x);
 	});
+	return 2u * grid.x * kTileThreads;
 }
 
-void launch_zeta_small(const Descs &descs, int count, int n, int, Storage st, int objective, double target,
+unsigned launch_zeta_small(const Descs &descs, int count, int n, int, Storage st, int objective, double target,
 					   hipStream_t stream)
 {
 	const dim3 grid(1, count);
@@ -1472,7 +1547,18 @@ void launch_zeta_small(const Descs &descs, int count, int n, int, Storage st, in
 		else
 			hipLaunchKernelGGL(k_zeta_small<float>, grid, dim3(1024), 0, stream, descs, n, objective, target);
 	});
+	return 2u * 1024u;
 }
+
+void launch_band_collect(const Descs &descs, int count, unsigned entries, hipStream_t stream)
+{
+	const dim3 grid((entries + 255) / 256, count);
+	launch_checked("k_band_collect", [&] {
+		hipLaunchKernelGGL(k_band_collect, grid, dim3(256), 0, stream, descs, entries);
+	});
+}
+
+
 
 void launch_bbpa_score(const Descs &descs, int count, int n, int k, double target, hipStream_t stream)
 {

@@ -123,18 +123,23 @@ constexpr double kNegativeResponse = 0.99; // replicated_lattice.cpp:6
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 // Arena head: [0,256) scalars, [256,512) selection result, [512,768) counters,
-// [768,1024) atom masses. One 1 KB copy brings all of it to the host.
-constexpr size_t kHeadBytes = 1024;
+// [768,1024) atom masses, then the band candidates (kBandCapacity of them). One 2 KB copy
+// brings all of the head and the first kBandHead candidates to the host.
+constexpr size_t kHeadBytes = 1024 + kBandHead * sizeof(Cand);
 constexpr size_t kHeadResult = 256;
+constexpr size_t kHeadCounter = 512;
 constexpr size_t kHeadMasses = 768;
+constexpr size_t kHeadBand = 1024;
 
 struct HostBox {
 	SelResult result;
+	unsigned counter[4];
 	double masses[32];
 	double scal[kScalars];
 	double value;
 	double raw;
 	alignas(256) unsigned char head[kHeadBytes];
+	Cand band[kBandCapacity];
 };
 
 } // namespace
@@ -155,6 +160,7 @@ DeviceLattice::DeviceLattice(bool dilution, int subjects, int variants, const do
 	active_ = (subjects >= 32) ? 0xFFFFFFFFu : ((1u << subjects) - 1u);
 	BGT_HIP_CHECK(hipStreamCreate(&stream_));
 	allocate();
+	exact_.reset(variants == 1 && !dilution, atoms_, prior);
 	std::vector<double> pairs(2 * kMaxAtoms, 0.0);
 	for (int a = 0; a < atoms_; ++a) {
 		pairs[2 * a] = prior[a];
@@ -176,6 +182,7 @@ DeviceLattice::DeviceLattice(const DeviceLattice &parent, bool copy_posterior)
 	active_ = parent.active_;
 	updates_ = parent.updates_;
 	inv_cur_ = parent.inv_cur_;
+	exact_ = parent.exact_;
 	if (copy_posterior) {
 		// order the copy after the parent's pending work
 		BGT_HIP_CHECK(hipStreamSynchronize(parent.stream_));
@@ -209,6 +216,9 @@ void DeviceLattice::allocate()
 	const size_t o_res = take(sizeof(SelResult));         // 256
 	const size_t o_cnt = take(sizeof(unsigned) * 4);      // 512
 	const size_t o_mass = take(sizeof(double) * 32);      // 768
+	const bool exact = k_ == 1 && !dilution_;
+	const size_t o_band = take(exact ? sizeof(Cand) * kBandCapacity : 0); // 1024
+	const size_t o_cand = take(exact ? sizeof(Cand) * 2 * kTileThreads * zeta_final_groups(atoms_) : 0);
 	const size_t o_pairs = take(sizeof(double) * 2 * kMaxAtoms);
 	const size_t o_part = take(sizeof(double) * part);
 	const size_t o_tops = take(sizeof(SelResult) * tops);
@@ -228,6 +238,10 @@ void DeviceLattice::allocate()
 	ftab_ = reinterpret_cast<double *>(arena_ + o_ftab);
 	scores_ = reinterpret_cast<double *>(arena_ + o_scores);
 	selected_ = reinterpret_cast<uint32_t *>(arena_ + o_sel);
+	if (exact) {
+		band_ = reinterpret_cast<Cand *>(arena_ + o_band);
+		cand_ = reinterpret_cast<Cand *>(arena_ + o_cand);
+	}
 	double init[kScalars] = {1.0, 1.0, 1.0, 1.0, 0.0, 0.0, 0.0, 0.0};
 	BGT_HIP_CHECK(hipMemcpyAsync(scal_, init, sizeof(init), hipMemcpyHostToDevice, stream_));
 	BGT_HIP_CHECK(hipHostMalloc(&host_box_, sizeof(HostBox), hipHostMallocDefault));
@@ -284,6 +298,9 @@ LatDesc DeviceLattice::desc() const
 	d.dilution = dilution_ ? 1 : 0;
 	d.inv_in = inv_cur_;
 	d.inv_out = inv_cur_ == kScalInv0 ? kScalInv1 : kScalInv0;
+	d.cand = cand_;
+	d.band = band_;
+	d.band_width = 0.0;
 	return d;
 }
 
@@ -341,6 +358,7 @@ void DeviceLattice::commit_update()
 {
 	inv_cur_ = inv_cur_ == kScalInv0 ? kScalInv1 : kScalInv0;
 	++updates_;
+	exact_.push(e_, r_);
 }
 
 void DeviceLattice::update(uint32_t experiment, uint32_t response, const double *dil)
@@ -379,7 +397,9 @@ void DeviceLattice::fetch_head()
 	BGT_HIP_CHECK(hipStreamSynchronize(stream_));
 	std::memcpy(box->scal, box->head, sizeof(double) * kScalars);
 	std::memcpy(&box->result, box->head + kHeadResult, sizeof(SelResult));
+	std::memcpy(box->counter, box->head + kHeadCounter, sizeof(box->counter));
 	std::memcpy(box->masses, box->head + kHeadMasses, sizeof(double) * 32);
+	std::memcpy(box->band, box->head + kHeadBand, sizeof(Cand) * kBandHead);
 }
 
 SelResult DeviceLattice::fetch_result()
@@ -410,44 +430,128 @@ bgt_hip_selection DeviceLattice::finish_selection(int selector, const SelResult 
 	out.objective = r.d1;
 	out.runner_up = r.e2;
 	out.runner_up_objective = r.d2;
-	double post_rel = 0.0, mass_rel = 0.0;
-	error_bound(&post_rel, &mass_rel);
-	const double scale = sel == kSelBha ? 1.0 : double(1 << k_);
-	out.gap_bound = 2.0 * scale * mass_rel;
+	out.gap_bound = band_width(sel);
 	out.near_tie = (r.d2 - r.d1) <= out.gap_bound ? 1 : 0;
-	out.redecided = 0;
-	if (storage_ == Storage::fp32c && out.near_tie && found && r.e2 != 0xFFFFFFFFu && k_ == 1)
-		redecide(out, sel);
+	if (out.near_tie && found && (sel == kSelBha || sel == kSelBbpa) && exact_.usable())
+		rescore(out, sel, r);
 	return out;
 }
 
-double DeviceLattice::exact_objective(uint32_t e, int sel)
+double DeviceLattice::band_width(int sel) const
 {
-	// FP64 re-evaluation of one candidate's objective from the stored posterior
-	const double m = upset_mass(e);
-	if (sel == kSelBha)
-		return std::fabs(m - 0.5);
-	HostBox *box = static_cast<HostBox *>(host_box_);
-	BGT_HIP_CHECK(hipMemcpyAsync(box->scal, scal_, sizeof(double) * kScalars, hipMemcpyDeviceToHost, stream_));
-	BGT_HIP_CHECK(hipStreamSynchronize(stream_));
-	const double total = box->scal[kScalTotal];
-	return std::fabs(m - 0.5) + std::fabs((total - m) - 0.5);
+	// Two candidates' device objectives can be misordered only when they are closer than
+	// twice the objective's error eps.
+	// FP32C: each stored entry is the exact unnormalised posterior times (1 + e_S),
+	// |e_S| <= (t+1) u32 (one FP32 rounding for the prior and per update; the FP64 factors
+	// and 1/D are common to all entries). With x = y / sum(y), M(E) / M*(E) =
+	// (1 + avg_{S>=E} e) / (1 + avg_all e), so M - M* = M* (1 - M*) (avg_E e - avg_notE e)
+	// and |M - M*| <= M*(1 - M*) 2 (t+1) u32 <= (t+1) u32 / 2, plus one FP32 rounding of the
+	// transform intermediate (M* u32) and FP64 sums far below; BBPA's |M - t| + |(T - M) - t|
+	// doubles it (T = sum x is 1 up to FP64 rounding).
+	// FP64: the per-mass relative bound of error_bound().
+	double eps;
+	if (storage_ == Storage::fp32c) {
+		const double u32 = std::ldexp(1.0, -24);
+		eps = (0.5 * (updates_ + 1) + 1.5) * u32 * 1.001 + 1e-15;
+	} else {
+		double post_rel = 0.0, mass_rel = 0.0;
+		error_bound(&post_rel, &mass_rel);
+		eps = mass_rel;
+	}
+	const double scale = sel == kSelBha ? 1.0 : (k_ == 1 ? 2.0 : double(1 << k_) * 2.0);
+	return 2.0 * scale * eps;
 }
 
-void DeviceLattice::redecide(bgt_hip_selection &sel, int selector)
+void DeviceLattice::collect_band(const Descs &ds, unsigned entries)
 {
-	const double a = exact_objective(sel.experiment, selector);
-	const double b = exact_objective(sel.runner_up, selector);
-	const bool swap = b < a || (b == a && sel.runner_up < sel.experiment);
-	if (swap) {
-		std::swap(sel.experiment, sel.runner_up);
-		sel.objective = b;
-		sel.runner_up_objective = a;
-	} else {
-		sel.objective = a;
-		sel.runner_up_objective = b;
+	if (cand_ == nullptr)
+		return;
+	clock_.begin(stream_);
+	launch_band_collect(ds, 1, entries, stream_);
+	clock_.end(stream_, "band_collect");
+}
+
+void DeviceLattice::rescore(bgt_hip_selection &out, int sel, const SelResult &r)
+{
+	// Exact FP64 objectives for every candidate within the error band: the device best's
+	// true objective is at most d1 + eps, so the true best has a device objective of at
+	// most d1 + 2 eps (the collected band). Candidates come from per-thread top-2 lists, so
+	// a thread whose runner-up is in the band may hold further candidates, all with device
+	// objectives at least that runner-up's: the re-decision is complete when every such
+	// runner-up is above the exact best plus eps, and the band did not overflow the buffer.
+	HostBox *box = static_cast<HostBox *>(host_box_);
+	const double eps = 0.5 * band_width(sel);
+	std::vector<Cand> cands;
+	const unsigned count = box->counter[0];
+	bool complete = cand_ != nullptr && count <= (unsigned)kBandCapacity;
+	if (complete) {
+		const unsigned inhead = std::min<unsigned>(count, kBandHead);
+		cands.assign(box->band, box->band + inhead);
+		if (count > inhead) {
+			BGT_HIP_CHECK(hipMemcpyAsync(box->band + inhead, band_ + inhead, sizeof(Cand) * (count - inhead),
+										 hipMemcpyDeviceToHost, stream_));
+			BGT_HIP_CHECK(hipStreamSynchronize(stream_));
+			cands.insert(cands.end(), box->band + inhead, box->band + count);
+		}
 	}
-	sel.redecided = 1;
+	for (const Cand c : {Cand{r.d1, r.e1, 0}, Cand{r.d2, r.e2, 0}}) {
+		if (c.e == 0xFFFFFFFFu)
+			continue;
+		bool seen = false;
+		for (const Cand &x : cands)
+			seen = seen || x.e == c.e;
+		if (!seen)
+			cands.push_back(c);
+	}
+	std::sort(cands.begin(), cands.end(),
+			  [](const Cand &a, const Cand &b) { return a.d < b.d || (a.d == b.d && a.e < b.e); });
+	const double t = 1.0 / double(1 << k_);
+	double b1 = INFINITY, b2 = INFINITY;
+	uint32_t e1 = 0xFFFFFFFFu, e2 = 0xFFFFFFFFu;
+	int rescored = 0;
+	constexpr int kMaxRescored = 2048;
+	for (const Cand &c : cands) {
+		if (c.d - eps > b2)
+			break; // its exact objective exceeds the current runner-up, as do all later ones
+		if (rescored == kMaxRescored) {
+			complete = false;
+			break;
+		}
+		double m = 0.0;
+		if (!exact_.mass(c.e, m))
+			return; // beyond the exact program's budget: keep the device result
+		const double d = sel == kSelBha ? std::fabs(m - 0.5) : std::fabs(m - t) + std::fabs((1.0 - m) - t);
+		++rescored;
+		if (d < b1 || (d == b1 && c.e < e1)) {
+			b2 = b1;
+			e2 = e1;
+			b1 = d;
+			e1 = c.e;
+		} else if (d < b2 || (d == b2 && c.e < e2)) {
+			b2 = d;
+			e2 = c.e;
+		}
+	}
+	if (e1 == 0xFFFFFFFFu)
+		return;
+	for (const Cand &c : cands)
+		if (c.pad == 1 && !(c.d - eps > b1))
+			complete = false; // that thread's hidden candidates could beat e1
+	out.experiment = e1;
+	out.objective = b1;
+	out.runner_up = e2;
+	out.runner_up_objective = b2;
+	if (complete) {
+		out.gap_bound = 2.0 * (sel == kSelBha ? 1.0 : 2.0) * exact_.rel_bound();
+		out.near_tie = (b2 - b1) <= out.gap_bound ? 1 : 0;
+	} else {
+		// the band held more candidates than were collected: the exact order of the top
+		// two is known, not that no other candidate beats them
+		out.near_tie = 1;
+	}
+	out.redecided = complete ? 1 : 2;
+	out.candidates = rescored;
+	out.band = (int)count;
 }
 
 bgt_hip_selection DeviceLattice::select(int selector)
@@ -463,30 +567,35 @@ bgt_hip_selection DeviceLattice::select(int selector)
 	}
 	const int objective = sel == kSelBha ? kObjHalving : (k_ == 1 ? kObjBinaryBbpa : kObjGather);
 	const double target = sel == kSelBha ? 0.5 : 1.0 / double(1 << k_);
+	ensure_z();
+	Descs dz = single();
+	dz.one.band_width = band_width(sel);
+	unsigned entries = 0;
 	if (atoms_ <= kSmallBits) {
 		clock_.begin(stream_);
-		launch_zeta_small(ds, 1, atoms_, k_, storage_, objective, target, stream_);
+		entries = launch_zeta_small(dz, 1, atoms_, k_, storage_, objective, target, stream_);
 		clock_.end(stream_, "zeta_small");
 	} else if (atoms_ == kTileBits) {
 		clock_.begin(stream_);
-		launch_zeta_final(ds, 1, atoms_, k_, storage_, objective, true, false, target, stream_);
+		entries = launch_zeta_final(dz, 1, atoms_, k_, storage_, objective, true, false, target, stream_);
 		clock_.end(stream_, "zeta_final");
 	} else {
-		ensure_z();
-		const Descs dz = single();
 		clock_.begin(stream_);
 		launch_zeta_pass1(dz, 1, atoms_, storage_, false, k_, dilution_, stream_);
 		clock_.end(stream_, "zeta_pass1");
 		clock_.begin(stream_);
-		launch_zeta_final(dz, 1, atoms_, k_, storage_, objective, false, false, target, stream_);
+		entries = launch_zeta_final(dz, 1, atoms_, k_, storage_, objective, false, false, target, stream_);
 		clock_.end(stream_, "zeta_pass2");
 	}
 	if (objective == kObjGather) {
 		clock_.begin(stream_);
-		launch_bbpa_score(ds, 1, n_, k_, target, stream_);
+		launch_bbpa_score(dz, 1, n_, k_, target, stream_);
 		clock_.end(stream_, "bbpa_score");
+	} else {
+		collect_band(dz, entries);
 	}
-	return finish_selection(selector, fetch_result(), false);
+	fetch_head();
+	return finish_selection(selector, static_cast<HostBox *>(host_box_)->result, false);
 }
 
 // ------------------------------------------------------------------- mass queries
@@ -624,14 +733,16 @@ bgt_hip_selection DeviceLattice::round(uint32_t experiment, uint32_t response, c
 	}
 	prepare_update(experiment, response, dil);
 	ensure_z();
-	const Descs ds = single();
+	Descs ds = single(); // descriptors of the pre-update state (pass 2 applies inv_out)
 	clock_.begin(stream_);
 	launch_zeta_pass1(ds, 1, atoms_, storage_, true, k_, dilution_, stream_);
 	clock_.end(stream_, "round_pass1");
-	clock_.begin(stream_);
-	launch_zeta_final(ds, 1, atoms_, k_, storage_, kObjHalving, false, true, 0.5, stream_);
-	clock_.end(stream_, "round_pass2");
 	commit_update();
+	ds.one.band_width = band_width(kSelBha); // error band of the updated posterior
+	clock_.begin(stream_);
+	const unsigned entries = launch_zeta_final(ds, 1, atoms_, k_, storage_, kObjHalving, false, true, 0.5, stream_);
+	clock_.end(stream_, "round_pass2");
+	collect_band(ds, entries);
 	fetch_head();
 	HostBox *box = static_cast<HostBox *>(host_box_);
 	const SelResult r = box->result;

@@ -3,6 +3,7 @@
 #pragma once
 
 #include "bgt_hip.h"
+#include "bgt_hip_exact.hpp"
 #include "bgt_hip_internal.hpp"
 
 #include <string>
@@ -98,8 +99,9 @@ private:
 	void check_selector(int selector) const;
 	void ensure_z();
 	void fill_lik(uint32_t experiment, uint32_t response, const double *dilution_flat);
-	void redecide(bgt_hip_selection &sel, int selector);
-	double exact_objective(uint32_t e, int selector);
+	void rescore(bgt_hip_selection &out, int selector, const SelResult &r);
+	double band_width(int selector) const;
+	void collect_band(const Descs &ds, unsigned entries);
 
 	bool dilution_;
 	int n_, k_, atoms_;
@@ -121,6 +123,9 @@ private:
 	double *pairs_ = nullptr;
 	double *scores_ = nullptr;
 	uint32_t *selected_ = nullptr;
+	Cand *cand_ = nullptr; // per-thread top-2 of the last transform (exact rescoring only)
+	Cand *band_ = nullptr; // candidates within the error band
+	ExactHistory exact_;
 
 	// pinned host mailbox for results
 	void *host_box_ = nullptr;

@@ -1,0 +1,95 @@
+"""Short workloads for rocprofv3 runs (kernel traces and HBM counters).
+
+    python -m bayesiangrouptesting_amd.profile_workloads round28 [fp32c|fp64] [rounds]
+        n = 28 binary non-dilution rounds (the benchmark's workload), few rounds
+    python -m bayesiangrouptesting_amd.profile_workloads copy [MiB]
+        one device-to-device copy of a known size, the calibration point for FETCH_SIZE /
+        WRITE_SIZE (bytes the counters report per byte actually moved)
+    python -m bayesiangrouptesting_amd.profile_workloads bbpa24 [fp32c|fp64] [rounds]
+        n = 8, k = 3 dilution lattice (2^24 states): Op-BBPA select + update rounds
+"""
+from __future__ import annotations
+
+import ctypes
+import sys
+import time
+
+
+def round28(form: str = "fp32c", rounds: int = 4) -> None:
+    import bayesiangrouptesting_amd as bgt
+    from bayesiangrouptesting_amd import _native as nat
+    from bayesiangrouptesting_amd import workload
+
+    c = workload.C2
+    n = c.subjects
+    pi = workload.priors(c.seed, n, c.prior_lo, c.prior_hi)
+    truth = bgt.reverse_bits(workload.true_state(c.seed, pi), n)
+    prec = bgt.Precision.float32_compensated if form == "fp32c" else bgt.Precision.float64
+    lat = bgt.Lattice(bgt.LatticeType.replicated_non_dilution, n, pi, precision=prec)
+    sampler = workload.ResponseSampler(c.seed, n, 1, truth)
+    e = lat.select(bgt.SelectorType.op_bha).experiment
+    t0 = time.perf_counter()
+    for _ in range(rounds):
+        r = sampler.draw(e)
+        e = lat.round(e, r, 0.01, 0.01).experiment
+    nat.check(nat.lib().bgt_hip_device_synchronize())
+    print(f"round28 {form}: {rounds} rounds, {(time.perf_counter() - t0) / rounds * 1e3:.3f} ms/round", flush=True)
+    lat.close()
+
+
+def bbpa24(form: str = "fp32c", rounds: int = 4) -> None:
+    import bayesiangrouptesting_amd as bgt
+    from bayesiangrouptesting_amd import _native as nat
+    from bayesiangrouptesting_amd import workload
+
+    c = workload.C3
+    n, k = c.subjects, c.variants
+    pi = workload.priors(c.seed, n * k, c.prior_lo, c.prior_hi)
+    table = bgt.DilutionTable(n, 0.05, 0.9)
+    prec = bgt.Precision.float32_compensated if form == "fp32c" else bgt.Precision.float64
+    lat = bgt.Lattice(bgt.LatticeType.replicated_dilution, n, pi, variants=k, precision=prec)
+    truth = workload.true_state(c.seed, pi)
+    sampler = workload.ResponseSampler(c.seed, n, k, truth, list(table.values))
+    t0 = time.perf_counter()
+    for _ in range(rounds):
+        e = lat.select(bgt.SelectorType.op_bbpa).experiment
+        lat.update(e, sampler.draw(e), table)
+    nat.check(nat.lib().bgt_hip_device_synchronize())
+    print(f"bbpa24 {form}: {rounds} rounds, {(time.perf_counter() - t0) / rounds * 1e3:.3f} ms/round", flush=True)
+    lat.close()
+
+
+def copy(mib: int = 1024) -> None:
+    hip = ctypes.CDLL("libamdhip64.so")
+    nbytes = mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    hip_memcpy_d2d = 3
+    assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), hip_memcpy_d2d) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    print(f"copy: {mib} MiB device to device", flush=True)
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
+def main(argv) -> int:
+    if not argv:
+        print(__doc__)
+        return 2
+    what = argv[0]
+    if what == "round28":
+        round28(argv[1] if len(argv) > 1 else "fp32c", int(argv[2]) if len(argv) > 2 else 4)
+    elif what == "bbpa24":
+        bbpa24(argv[1] if len(argv) > 1 else "fp32c", int(argv[2]) if len(argv) > 2 else 4)
+    elif what == "copy":
+        copy(int(argv[1]) if len(argv) > 1 else 1024)
+    else:
+        print(__doc__)
+        return 2
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv[1:]))

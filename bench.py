@@ -1,20 +1,22 @@
 """Benchmark: Op-BHA select + update + atom marginals at n = 28 on one MI355X.
 
 A step is one round of the adaptive loop on the 2^28-state binary non-dilution lattice
-(priors U[0.005, 0.05], seed 2): apply the sampled response to the current experiment,
-take the atom marginals of the updated posterior and classify, and select the next
-experiment by Op-BHA -- the fused two-pass round of the HIP provider
-(bgt_hip_lattice_round). The headline value is rounds/s of the compensated FP32 form;
-the plain FP64 form runs the same rounds beside it.
+(priors U[0.005, 0.05], seed 2): apply the response to the current experiment, take the atom
+marginals of the updated posterior and classify, and select the next experiment by Op-BHA --
+the fused two-pass round of the HIP provider (bgt_hip_lattice_round). The headline value is
+rounds/s of the compensated FP32 form. The plain FP64 form runs first and drives the
+trajectory (its picks and the responses sampled for them); the FP32C form replays the same
+(experiment, response) sequence, so the two forms' selections and final posteriors are
+compared round by round.
 
-Roofline: algorithmic bytes per round are S*(2*b_p + 2*b_z) (posterior read and
-written once, the transform intermediate spilled and reloaded once), with b = 4 for
-FP32C and 8 for FP64. The dominant kernel is pass 1 (update fused with the low-bit
-transform): S*(2*b_p + b_z) bytes per launch, timed with HIP events on the lattice's
-stream over the timed rounds.
+Roofline: the dominant kernel is pass 1 (update fused with bits 0..14 of the transform):
+it reads the posterior and writes it and the transform intermediate, S*(2 b_p + b_z) bytes
+per launch (b = 4 for FP32C, 8 for FP64), timed with HIP events on the lattice's stream
+over the timed rounds. A round moves S*(2 b_p + 2 b_z) bytes (the intermediate is reloaded
+once by pass 2).
 
-CPU baseline: the C restatement of the reference CPU provider, built with the
-reference's default flags, runs one full round at n = 28 on one host thread.
+CPU baseline: the C restatement of the reference CPU provider (oracle/, built with the
+reference's default flags) runs one full n = 28 round on one host thread.
 """
 from __future__ import annotations
 
@@ -32,6 +34,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Op-BHA select+update/s @n=28 (HBM GB/s %peak); served p50/p99 ms; FP32c vs FP64"
 PEAK_HBM_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_round_n28.json")
 
 
 def kernel_times(lat) -> dict:
@@ -52,37 +55,46 @@ def enable_clock(lat, on: bool) -> None:
     nat.check(nat.lib().bgt_hip_lattice_kernel_times(lat.handle, 1 if on else 0, None, None, 0, None, 0))
 
 
-def run_form(precision, n, pi, truth_pub, steps, warmup, seed):
+def run_form(precision, n, pi, truth_pub, steps, warmup, seed, drive=None):
+    """Rounds of one storage form. Without `drive` the responses are sampled for the form's
+    own picks; with it, the given (experiment, response) sequence is replayed. Returns the
+    lattice, the timed seconds, per-kernel times, the selections (initial one first) and the
+    per-round host latencies (ms) of the timed rounds."""
     import bayesiangrouptesting_amd as bgt
     from bayesiangrouptesting_amd import _native as nat
     from bayesiangrouptesting_amd import workload
 
     lat = bgt.Lattice(bgt.LatticeType.replicated_non_dilution, n, pi, precision=precision)
     sampler = workload.ResponseSampler(seed, n, 1, truth_pub)
-    e = lat.select(bgt.SelectorType.op_bha).experiment
-    trace = []
-    for _ in range(warmup):
-        r = sampler.draw(e)
-        sel = lat.round(e, r, 0.01, 0.01)
-        trace.append((e, r, sel.experiment, sel.near_tie))
-        e = sel.experiment
-    nat.check(nat.lib().bgt_hip_device_synchronize())
-    enable_clock(lat, True)
+    sel = lat.select(bgt.SelectorType.op_bha)
+    picks = [sel]
+    e = sel.experiment
+    lat_ms = []
     t0 = time.perf_counter()
-    for _ in range(steps):
-        r = sampler.draw(e)
+    for i in range(warmup + steps):
+        if i == warmup:
+            nat.check(nat.lib().bgt_hip_device_synchronize())
+            enable_clock(lat, True)
+            t0 = time.perf_counter()
+        if drive is not None:
+            e, r = drive[i]
+        else:
+            r = sampler.draw(e)
+        t1 = time.perf_counter()
         sel = lat.round(e, r, 0.01, 0.01)
-        trace.append((e, r, sel.experiment, sel.near_tie))
-        e = sel.experiment
+        lat_ms.append((time.perf_counter() - t1) * 1e3)
+        picks.append(sel)
+        if drive is None:
+            e = sel.experiment
     nat.check(nat.lib().bgt_hip_device_synchronize())
     dt = time.perf_counter() - t0
     kt = kernel_times(lat)
     enable_clock(lat, False)
-    return lat, dt, kt, trace
+    return lat, dt, kt, picks, lat_ms[warmup:]
 
 
 def cpu_baseline(n, pi, e_int, r):
-    """One full round of the C restatement at n = 28 (select, update, classify), 1 thread."""
+    """One full round of the C restatement at n = 28 (select, update, marginals), 1 thread."""
     from oracle import pyoracle as orc
     L = orc.lib("fast")
     # prior built with numpy (construction is not part of a round)
@@ -114,6 +126,15 @@ def cpu_model() -> str:
     return "unknown"
 
 
+def load_traffic():
+    """HBM bytes per pass-1 launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,8 +143,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: the provider is single-GPU (see DESIGN.md for how the "
-              "lattice would shard); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: this provider runs one lattice per GPU and does not shard "
+              "across devices (DESIGN.md describes how it would); run with --gpus 1", file=sys.stderr)
         return 2
 
     import bayesiangrouptesting_amd as bgt
@@ -138,51 +159,55 @@ def main() -> int:
     truth_int = workload.true_state(c.seed, pi)
     truth_pub = bgt.reverse_bits(truth_int, n)
     S = 1 << n
+    W, K = args.warmup, args.steps
 
-    print(f"[bench] n={n} states={S} steps={args.steps} warmup={args.warmup}", file=sys.stderr, flush=True)
-    lat32, dt32, kt32, tr32 = run_form(bgt.Precision.float32_compensated, n, pi, truth_pub, args.steps,
-                                       args.warmup, c.seed)
-    post32 = lat32.posterior()
-    m32 = lat32.atom_masses()
-    lat32.close()
-    print(f"[bench] fp32c {dt32 / args.steps * 1e3:.3f} ms/round", file=sys.stderr, flush=True)
-    lat64, dt64, kt64, tr64 = run_form(bgt.Precision.float64, n, pi, truth_pub, args.steps, args.warmup, c.seed)
+    print(f"[bench] n={n} states={S} steps={K} warmup={W}", file=sys.stderr, flush=True)
+    lat64, dt64, kt64, pk64, lm64 = run_form(bgt.Precision.float64, n, pi, truth_pub, K, W, c.seed)
+    # the trajectory the FP64 form chose: (experiment applied, response) per round
+    sampler = workload.ResponseSampler(c.seed, n, 1, truth_pub)
+    drive = []
+    for i in range(W + K):
+        e = pk64[i].experiment
+        drive.append((e, sampler.draw(e)))
     post64 = lat64.posterior()
     m64 = lat64.atom_masses()
     lat64.close()
-    print(f"[bench] fp64 {dt64 / args.steps * 1e3:.3f} ms/round", file=sys.stderr, flush=True)
+    print(f"[bench] fp64 {dt64 / K * 1e3:.3f} ms/round", file=sys.stderr, flush=True)
+    lat32, dt32, kt32, pk32, lm32 = run_form(bgt.Precision.float32_compensated, n, pi, truth_pub, K, W, c.seed,
+                                             drive=drive)
+    post32 = lat32.posterior()
+    m32 = lat32.atom_masses()
+    lat32.close()
+    print(f"[bench] fp32c {dt32 / K * 1e3:.3f} ms/round", file=sys.stderr, flush=True)
 
-    same = sum(1 for a, b in zip(tr32, tr64) if a[2] == b[2])
-    first_div = next((i for i, (a, b) in enumerate(zip(tr32, tr64)) if a[2] != b[2]), None)
+    rounds = len(pk64)
+    same = sum(1 for a, b in zip(pk64, pk32) if a.experiment == b.experiment)
+    resolved = [(a, b) for a, b in zip(pk64, pk32) if not (a.near_tie or b.near_tie)]
     big = post64 >= 1e-30 * post64.sum()
-    post_err = float(np.max(np.abs(post32[big] - post64[big]) / post64[big])) if first_div is None else None
-    l1_err = float(np.abs(post32 - post64).sum() / post64.sum()) if first_div is None else None
-    mass_err = float(np.max(np.abs(m32 - m64) / np.maximum(m64, 1e-300))) if first_div is None else None
-    del post32, post64
+    post_err = float(np.max(np.abs(post32[big] - post64[big]) / post64[big]))
+    l1_err = float(np.abs(post32 - post64).sum() / post64.sum())
+    mass_err = float(np.max(np.abs(m32 - m64) / np.maximum(m64, 1e-300)))
+    del post32, post64, big
 
-    def roof(kt, b):
-        p1 = kt.get("round_pass1", (float("nan"), 0))[0]
-        p2 = kt.get("round_pass2", (float("nan"), 0))[0]
-        alg1 = S * (2 * b + b)
-        alg_round = S * (2 * b + 2 * b)
-        return {
-            "pass1_ms": p1, "pass2_ms": p2,
-            "pass1_GBs": alg1 / (p1 * 1e-3) / 1e9,
-            "round_alg_bytes": alg_round,
-            "round_GBs": alg_round / ((p1 + p2) * 1e-3) / 1e9,
-        }
-
-    r32 = roof(kt32, 4)
-    r64 = roof(kt64, 8)
-    value = args.steps / dt32
+    b32, b64 = 4, 8
+    p1_32 = kt32["round_pass1"][0]
+    p2_32 = kt32["round_pass2"][0]
+    p1_64 = kt64["round_pass1"][0]
+    p2_64 = kt64["round_pass2"][0]
+    alg1 = S * (2 * b32 + b32)
+    round_alg32 = S * (2 * b32 + 2 * b32)
+    round_alg64 = S * (2 * b64 + 2 * b64)
+    achieved = alg1 / (p1_32 * 1e-3) / 1e9
+    traffic = load_traffic()
+    value = K / dt32
     line = {
         "metric": METRIC,
         "value": value,
         "unit": "rounds/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": dt32 / args.steps * 1e3,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": dt32 / K * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -190,40 +215,55 @@ def main() -> int:
         "data": "synthetic",
         "config": {
             "workload": "binary non-dilution lattice n=28 k=1 (2^28 states), priors U[0.005,0.05] seed 2; "
-                        "round = update(sampled response) + atom marginals + classification + Op-BHA select",
-            "subjects": n, "variants": 1, "states": S, "form": "fp32c (fp32 storage, fp64 on-chip accumulation)",
+                        "round = update(response) + atom marginals + classification + Op-BHA select; the FP32C "
+                        "run replays the (experiment, response) sequence chosen by the FP64 run",
+            "subjects": n, "variants": 1, "states": S,
+            "form": "fp32c (fp32 storage, fp64 on-chip accumulation, exact fp64 re-decision of near ties)",
         },
         "roofline": {
             "bound": "hbm",
-            "achieved": r32["pass1_GBs"],
+            "achieved": achieved,
             "peak": PEAK_HBM_GBS,
             "unit": "GB/s",
-            "frac": r32["pass1_GBs"] / PEAK_HBM_GBS,
-            "traffic": None,
-            "kernel": "k_zeta_pass1<float,float,true> (update fused with bits 0..14 of the transform)",
-            "alg_bytes_per_launch": S * 12,
-            "round_GBs": r32["round_GBs"],
-            "round_frac": r32["round_GBs"] / PEAK_HBM_GBS,
-            "pass1_ms": r32["pass1_ms"],
-            "pass2_ms": r32["pass2_ms"],
+            "frac": achieved / PEAK_HBM_GBS,
+            "traffic": traffic.get("pass1_bytes") if traffic else None,
+            "kernel": "k_zeta_pass1<float,float,true,1> (update fused with bits 0..14 of the transform)",
+            "alg_bytes_per_launch": alg1,
+            "pass1_ms": p1_32,
+            "pass2_ms": p2_32,
+            "pass2_GBs": S * b32 / (p2_32 * 1e-3) / 1e9,
+            "round_kernels_GBs": round_alg32 / ((p1_32 + p2_32) * 1e-3) / 1e9,
+            "round_GBs": round_alg32 * value / 1e9,
+            "round_frac": round_alg32 * value / 1e9 / PEAK_HBM_GBS,
+            "traffic_detail": traffic,
         },
+        "latency_ms": {"p50": float(np.percentile(lm32, 50)), "p99": float(np.percentile(lm32, 99)),
+                       "note": "per fused round, host call to result (not the serving loop)"},
         "fp64": {
-            "value": args.steps / dt64,
-            "ms_per_step": dt64 / args.steps * 1e3,
-            "pass1_ms": r64["pass1_ms"], "pass2_ms": r64["pass2_ms"],
-            "round_GBs": r64["round_GBs"], "round_frac": r64["round_GBs"] / PEAK_HBM_GBS,
-            "fp32c_speedup": (args.steps / dt32) / (args.steps / dt64),
+            "value": K / dt64,
+            "ms_per_step": dt64 / K * 1e3,
+            "pass1_ms": p1_64, "pass2_ms": p2_64,
+            "round_GBs": round_alg64 * (K / dt64) / 1e9,
+            "round_frac": round_alg64 * (K / dt64) / 1e9 / PEAK_HBM_GBS,
+            "latency_ms": {"p50": float(np.percentile(lm64, 50)), "p99": float(np.percentile(lm64, 99))},
         },
         "fp32c_vs_fp64": {
-            "identical_selections": same, "rounds": len(tr64), "first_divergence": first_div,
+            "fp32c_speedup": (K / dt32) / (K / dt64),
+            "identical_selections": same, "selections": rounds,
+            "resolved_selections": len(resolved),
+            "identical_resolved": sum(1 for a, b in resolved if a.experiment == b.experiment),
             "posterior_max_rel_err": post_err, "posterior_l1_rel_err": l1_err, "mass_max_rel_err": mass_err,
-            "near_ties_fp32c": sum(1 for t in tr32 if t[3]),
+            "near_ties_fp32c": sum(1 for p in pk32 if p.near_tie),
+            "redecided_fp32c": sum(1 for p in pk32 if p.redecided),
+            "band_overflow_fp32c": sum(1 for p in pk32 if p.redecided == 2),
+            "rescored_candidates_fp32c_max": max(p.candidates for p in pk32),
         },
+        "kernels_ms_fp32c": {k: round(v[0], 4) for k, v in kt32.items()},
     }
     if not args.no_cpu:
         print("[bench] cpu baseline (one n=28 round, 1 thread)", file=sys.stderr, flush=True)
-        e_int = bgt.reverse_bits(tr64[0][0], n)
-        cb = cpu_baseline(n, pi, e_int, tr64[0][1])
+        e_int = bgt.reverse_bits(drive[0][0], n)
+        cb = cpu_baseline(n, pi, e_int, drive[0][1])
         line["cpu_baseline"] = {
             "value": 1.0 / cb["round_s"], "unit": "rounds/s", "cores": 1, "kind": "port",
             "sample": f"one full n=28 round of the C restatement (-O3 -ffast-math): select {cb['select_s']:.2f} s, "

@@ -62,8 +62,14 @@ typedef struct bgt_hip_tree_stats {
 } bgt_hip_tree_stats;
 
 /* Result of one selection. objective is |M(E) - 1/2| for Op-BHA and the L1 bucket
- * distance for Op-BBPA / brute force. gap = runner_up_objective - objective. When
- * gap <= gap_bound the pick is rounding-determined (a near tie) and near_tie is 1. */
+ * distance for Op-BBPA / brute force. gap = runner_up_objective - objective. When the
+ * device's gap is within its error bound, the pick is rounding-determined (near_tie = 1)
+ * and, for binary non-dilution lattices, is re-decided by exact FP64 upset masses
+ * recomputed from the prior and the update history over every candidate whose objective
+ * lies within the error band (redecided = 1; 2 when the band held more candidates than
+ * were collected and only the top two were rescored). After a re-decision, objective,
+ * runner_up, runner_up_objective and gap_bound are the exact-rescoring values and
+ * near_tie reports whether even those are within FP64 rounding. */
 typedef struct bgt_hip_selection {
 	uint32_t experiment;
 	uint32_t runner_up;
@@ -71,7 +77,9 @@ typedef struct bgt_hip_selection {
 	double runner_up_objective;
 	double gap_bound;
 	int near_tie;
-	int redecided; /* FP32C: 1 when the pick was re-decided in FP64 */
+	int redecided;
+	int candidates; /* experiments rescored exactly (0 without a re-decision) */
+	int band;       /* candidates the device collected within the error band */
 } bgt_hip_selection;
 
 /* ---- provider entry points: counterparts of include/bgt/detail/cuda_provider.hpp:5-24 ---- */
@@ -189,6 +197,15 @@ int bgt_hip_lattice_info(bgt_hip_lattice *lattice, int *subjects, int *variants,
 int bgt_hip_batch_step(bgt_hip_lattice **lattices, const uint32_t *experiments,
 					   const uint32_t *responses, const double *const *dilutions, int count,
 					   int selector, bgt_hip_selection *out);
+
+/* ---- exact rescoring (host only, no device needed) ---- */
+
+/* Exact FP64 upset masses M(E) of a binary non-dilution lattice after the given updates,
+ * for `count` query experiments, recomputed from the prior and the update history (internal
+ * coordinates): the values the selections re-decide near ties with. Returns
+ * BGT_HIP_UNSUPPORTED when the history is too entangled for the exact program's budget. */
+int bgt_hip_exact_upset_mass(int subjects, const double *prior, int updates, const uint32_t *experiments,
+							 const uint32_t *responses, int count, const uint32_t *queries, double *out);
 
 /* ---- misc ---- */
 const char *bgt_hip_last_error(void);
