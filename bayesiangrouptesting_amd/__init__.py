@@ -24,8 +24,10 @@ from .api import (  # noqa: F401
     TreeStats,
     Unsupported,
     available_providers,
+    batch_round,
     exact_upset_mass,
     hip_available,
+    lattice_footprint,
     reverse_bits,
     run_simulation,
 )
@@ -33,7 +35,7 @@ from .api import (  # noqa: F401
 __all__ = [
     "DilutionParameters", "DilutionMode", "DilutionTable", "Error", "InvalidArgument", "KernelSpec", "Lattice",
     "LatticeType", "Precision", "Provider", "Runtime", "Selection", "SelectorType", "SimulationConfig",
-    "SimulationOptions", "SimulationResult", "TreeStats", "Unsupported", "available_providers", "exact_upset_mass",
-    "hip_available",
+    "SimulationOptions", "SimulationResult", "TreeStats", "Unsupported", "available_providers", "batch_round",
+    "exact_upset_mass", "hip_available", "lattice_footprint",
     "reverse_bits", "run_simulation",
 ]

@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "libbgt_hip.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "bgt_hip.h")
 
 OK, BAD_INPUT, UNSUPPORTED, NO_BACKEND, INTERNAL = 0, 1, 2, 3, 4
+NO_UPDATE = 0xFFFFFFFF
 FP64, FP32C = 0, 1
 SELECT_AUTO, SELECT_OP_BHA, SELECT_OP_BBPA, SELECT_BRUTE_FORCE = 0, 1, 2, 3
 
@@ -99,8 +100,11 @@ def _sig(L):
                                          c_int),
         "bgt_hip_lattice_info": ([_H, ctypes.POINTER(c_int), ctypes.POINTER(c_int), ctypes.POINTER(c_int),
                                   ctypes.POINTER(c_u64)], c_int),
+        "bgt_hip_lattice_footprint": ([c_int, c_int, c_int, c_int, ctypes.POINTER(c_u64)], c_int),
         "bgt_hip_batch_step": ([ctypes.POINTER(_H), _u32p, _u32p, ctypes.POINTER(_dptr), c_int, c_int,
                                 ctypes.POINTER(SelectionC)], c_int),
+        "bgt_hip_batch_round": ([ctypes.POINTER(_H), _u32p, _u32p, ctypes.POINTER(_dptr), c_int, c_int, c_double,
+                                 c_double, ctypes.POINTER(SelectionC), _u32p, _u32p], c_int),
         "bgt_hip_exact_upset_mass": ([c_int, _dptr, c_int, _u32p, _u32p, c_int, _u32p, _dptr], c_int),
         "bgt_hip_last_error": ([], ctypes.c_char_p),
         "bgt_hip_device_synchronize": ([], c_int),

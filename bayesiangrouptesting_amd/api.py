@@ -297,6 +297,57 @@ class Lattice:
         return a.value, b.value
 
 
+def lattice_footprint(type: LatticeType, subjects: int, variants: int = 1,
+                      precision: Precision = Precision.float64) -> int:
+    """Device bytes a lattice of this shape holds once it has selected (admission control)."""
+    out = ctypes.c_uint64()
+    nat.check(nat.lib().bgt_hip_lattice_footprint(int(_uses_dilution(LatticeType(type))), subjects, variants,
+                                                  _storage(precision), ctypes.byref(out)))
+    return out.value
+
+
+def batch_round(lattices: Sequence["Lattice"], experiments: Sequence[Optional[int]], responses: Sequence[Optional[int]],
+                selector: SelectorType = SelectorType.auto_select, threshold_up: float = 0.01,
+                threshold_lo: float = 0.01, dilutions: Optional[Sequence[Optional[DilutionTable]]] = None,
+                classify: bool = True):
+    """One batched step over many lattices (public coordinates): update the lattices whose
+    response is not None, take marginals and classify (classify=True), then select. The
+    results equal per-lattice update / update_classification / select_experiment calls.
+    Returns the selections and, with classify, the (positive, negative) public masks."""
+    count = len(lattices)
+    hs = (nat._H * count)(*[l.handle for l in lattices])
+    es = np.zeros(count, dtype=np.uint32)
+    rs = np.full(count, nat.NO_UPDATE, dtype=np.uint32)
+    for i, (lat, e, r) in enumerate(zip(lattices, experiments, responses)):
+        if r is not None:
+            es[i] = lat._flip(int(e))
+            rs[i] = int(r)
+    dptrs = None
+    keep = []
+    if dilutions is not None:
+        dptrs = (ctypes.POINTER(ctypes.c_double) * count)()
+        for i, t in enumerate(dilutions):
+            if t is not None:
+                dptrs[i] = t.pointer()
+                keep.append(t)
+    sels = (nat.SelectionC * count)()
+    pos = np.zeros(count, dtype=np.uint32)
+    neg = np.zeros(count, dtype=np.uint32)
+    u32p = ctypes.POINTER(ctypes.c_uint32)
+    if classify:
+        nat.check(nat.lib().bgt_hip_batch_round(hs, es.ctypes.data_as(u32p), rs.ctypes.data_as(u32p), dptrs, count,
+                                                int(selector), threshold_up, threshold_lo, sels,
+                                                pos.ctypes.data_as(u32p), neg.ctypes.data_as(u32p)))
+    else:
+        nat.check(nat.lib().bgt_hip_batch_step(hs, es.ctypes.data_as(u32p), rs.ctypes.data_as(u32p), dptrs, count,
+                                               int(selector), sels))
+    out = [Selection.from_c(sels[i], lattices[i]._flip) for i in range(count)]
+    if not classify:
+        return out
+    masks = [(lattices[i]._flip(int(pos[i])), lattices[i]._flip(int(neg[i]))) for i in range(count)]
+    return out, masks
+
+
 def exact_upset_mass(subjects: int, prior: Sequence[float], history: Sequence[tuple[int, int]],
                      experiments) -> "float | np.ndarray":
     """Exact FP64 upset mass(es) of a binary non-dilution lattice after `history` updates

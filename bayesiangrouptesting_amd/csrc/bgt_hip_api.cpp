@@ -1,5 +1,6 @@
 // extern "C" boundary of the provider (include/bgt_hip.h). No exception crosses it.
 #include "bgt_hip.h"
+#include "bgt_hip_batch.hpp"
 #include "bgt_hip_exact.hpp"
 #include "bgt_hip_lattice.hpp"
 #include "bgt_hip_tree.hpp"
@@ -372,17 +373,63 @@ int bgt_hip_lattice_info(bgt_hip_lattice *h, int *subjects, int *variants, int *
 	});
 }
 
+int bgt_hip_lattice_footprint(int use_dilution, int subjects, int variants, int precision, uint64_t *bytes)
+{
+	return guarded([&] {
+		if (bytes == nullptr || subjects <= 0 || variants <= 0)
+			throw bgt_hip::Error(BGT_HIP_BAD_INPUT, "bad arguments");
+		if (variants > bgt_hip::kMaxVariants || subjects * variants > bgt_hip::kMaxAtoms)
+			throw bgt_hip::Error(BGT_HIP_UNSUPPORTED, "lattice larger than 2^30 states or more than 5 variants");
+		*bytes = bgt_hip::DeviceLattice::footprint(use_dilution != 0, subjects, variants, storage_of(precision));
+	});
+}
+
+static void run_batch(bgt_hip_lattice **lattices, const uint32_t *experiments, const uint32_t *responses,
+					  const double *const *dilutions, int count, int selector, bool classify, double up, double lo,
+					  bgt_hip_selection *out, uint32_t *pos, uint32_t *neg)
+{
+	if (count < 0 || (count > 0 && (lattices == nullptr || responses == nullptr || out == nullptr)))
+		throw bgt_hip::Error(BGT_HIP_BAD_INPUT, "null argument");
+	std::vector<bgt_hip::BatchItem> items(count);
+	for (int i = 0; i < count; ++i) {
+		auto &it = items[i];
+		it.lat = &L(lattices[i]);
+		it.update = responses[i] != BGT_HIP_NO_UPDATE;
+		if (it.update && experiments == nullptr)
+			throw bgt_hip::Error(BGT_HIP_BAD_INPUT, "experiments are required for updates");
+		it.experiment = it.update ? experiments[i] : 0u;
+		it.response = it.update ? responses[i] : 0u;
+		it.dilution = dilutions ? dilutions[i] : nullptr;
+		it.classify = classify;
+		it.select = true;
+	}
+	std::vector<bgt_hip::BatchResult> res;
+	bgt_hip::batch_step(items, selector, up, lo, res);
+	for (int i = 0; i < count; ++i) {
+		out[i] = res[i].selection;
+		if (pos)
+			pos[i] = res[i].positive;
+		if (neg)
+			neg[i] = res[i].negative;
+	}
+}
+
 int bgt_hip_batch_step(bgt_hip_lattice **lattices, const uint32_t *experiments, const uint32_t *responses,
 					   const double *const *dilutions, int count, int selector, bgt_hip_selection *out)
 {
 	return guarded([&] {
-		if (count < 0 || (count > 0 && (lattices == nullptr || experiments == nullptr || responses == nullptr ||
-										 out == nullptr)))
-			throw bgt_hip::Error(BGT_HIP_BAD_INPUT, "null argument");
-		for (int i = 0; i < count; ++i)
-			L(lattices[i]).update(experiments[i], responses[i], dilutions ? dilutions[i] : nullptr);
-		for (int i = 0; i < count; ++i)
-			out[i] = L(lattices[i]).select(selector);
+		run_batch(lattices, experiments, responses, dilutions, count, selector, false, 0.0, 0.0, out, nullptr,
+				  nullptr);
+	});
+}
+
+int bgt_hip_batch_round(bgt_hip_lattice **lattices, const uint32_t *experiments, const uint32_t *responses,
+						const double *const *dilutions, int count, int selector, double threshold_up,
+						double threshold_lo, bgt_hip_selection *out, uint32_t *positive, uint32_t *negative)
+{
+	return guarded([&] {
+		run_batch(lattices, experiments, responses, dilutions, count, selector, true, threshold_up, threshold_lo,
+				  out, positive, negative);
 	});
 }
 

@@ -1,0 +1,225 @@
+#include "bgt_hip_batch.hpp"
+
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <tuple>
+#include <vector>
+
+namespace bgt_hip {
+
+namespace {
+
+// Per-thread staging: one stream, device buffers for descriptors and likelihood tables,
+// and the gathered result heads (device and pinned host). Grown on demand, reused.
+struct Staging {
+	hipStream_t stream = nullptr;
+	char *dev = nullptr;
+	size_t dev_bytes = 0;
+	char *host = nullptr; // pinned: descriptors + tables on the way in, heads on the way out
+	size_t host_bytes = 0;
+
+	~Staging()
+	{
+		if (dev)
+			(void)hipFree(dev);
+		if (host)
+			(void)hipHostFree(host);
+		if (stream)
+			(void)hipStreamDestroy(stream);
+	}
+	void reserve(size_t dbytes, size_t hbytes)
+	{
+		if (!stream)
+			BGT_HIP_CHECK(hipStreamCreate(&stream));
+		if (dbytes > dev_bytes) {
+			if (dev)
+				BGT_HIP_CHECK(hipFree(dev));
+			dev = nullptr;
+			dev_bytes = std::max(dbytes, 2 * dev_bytes);
+			BGT_HIP_CHECK(hipMalloc(&dev, dev_bytes));
+		}
+		if (hbytes > host_bytes) {
+			if (host)
+				BGT_HIP_CHECK(hipHostFree(host));
+			host = nullptr;
+			host_bytes = std::max(hbytes, 2 * host_bytes);
+			BGT_HIP_CHECK(hipHostMalloc(&host, host_bytes, hipHostMallocDefault));
+		}
+	}
+};
+
+Staging &staging()
+{
+	static thread_local Staging s;
+	return s;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+// Launch groups: lattices of one shape share grids and templates.
+using ShapeKey = std::tuple<int, int, int, int>; // storage, dilution, subjects, variants
+
+ShapeKey shape_of(const DeviceLattice &l)
+{
+	return {static_cast<int>(l.storage()), l.dilution() ? 1 : 0, l.subjects(), l.variants()};
+}
+
+} // namespace
+
+void batch_step(std::vector<BatchItem> &items, int selector, double up, double lo, std::vector<BatchResult> &out)
+{
+	const size_t count = items.size();
+	out.assign(count, BatchResult{});
+	if (count == 0)
+		return;
+	// validate everything before any lattice changes state
+	std::vector<const DeviceLattice *> seen;
+	seen.reserve(count);
+	for (auto &it : items) {
+		if (it.lat == nullptr)
+			throw Error(BGT_HIP_BAD_INPUT, "null lattice in batch");
+		seen.push_back(it.lat);
+		if (it.select)
+			it.lat->check_selector(selector);
+		if (it.update)
+			it.lat->prepare_update(it.experiment, it.response, it.dilution);
+	}
+	std::sort(seen.begin(), seen.end());
+	if (std::adjacent_find(seen.begin(), seen.end()) != seen.end())
+		throw Error(BGT_HIP_BAD_INPUT, "a lattice appears twice in one batch");
+	for (auto &it : items)
+		it.lat->settle();
+	// host-side preparation in item order: update descriptors carry the pre-update state,
+	// selection descriptors the state after it
+	std::vector<LatDesc> du(count), ds(count);
+	std::vector<const double *> liks(count, nullptr);
+	for (size_t i = 0; i < count; ++i) {
+		auto &it = items[i];
+		if (it.update) {
+			du[i] = it.lat->desc();
+			liks[i] = it.lat->lik_host();
+			it.lat->commit_update();
+		}
+		if (it.select)
+			it.lat->ensure_z();
+		ds[i] = it.lat->desc();
+		ds[i].band_width =
+			it.select ? it.lat->band_width(DeviceLattice::resolve_selector(selector, it.lat->variants())) : 0.0;
+	}
+	// launch groups: identically shaped lattices share grids and templates; each group gets
+	// its own contiguous block of device descriptors
+	struct Group {
+		std::vector<size_t> idx;
+		size_t offset = 0; // byte offset of its descriptor block in the staging buffer
+	};
+	std::map<ShapeKey, Group> gu, gc, gs;
+	for (size_t i = 0; i < count; ++i) {
+		const ShapeKey key = shape_of(*items[i].lat);
+		if (items[i].update)
+			gu[key].idx.push_back(i);
+		if (items[i].classify)
+			gc[key].idx.push_back(i);
+		if (items[i].select)
+			gs[key].idx.push_back(i);
+	}
+	size_t off = 0;
+	const size_t o_lik = off;
+	off = align256(off + sizeof(double) * kLikCapacity * count);
+	for (auto *gm : {&gu, &gc, &gs})
+		for (auto &g : *gm) {
+			g.second.offset = off;
+			off = align256(off + sizeof(LatDesc) * g.second.idx.size());
+		}
+	const size_t o_all = off; // selection descriptors of every item, for the head gather
+	off = align256(off + sizeof(LatDesc) * count);
+	const size_t o_in = off; // bytes uploaded
+	const size_t o_heads = off;
+	off = align256(off + kHeadBytes * count);
+	Staging &st = staging();
+	st.reserve(off, off);
+	char *h = st.host;
+	char *d = st.dev;
+	for (size_t i = 0; i < count; ++i)
+		if (items[i].update) {
+			std::memcpy(h + o_lik + sizeof(double) * kLikCapacity * i, liks[i], sizeof(double) * kLikCapacity);
+			du[i].lik = reinterpret_cast<double *>(d + o_lik) + i * kLikCapacity;
+		}
+	auto fill = [&](std::map<ShapeKey, Group> &gm, const std::vector<LatDesc> &src) {
+		for (auto &g : gm) {
+			LatDesc *dst = reinterpret_cast<LatDesc *>(h + g.second.offset);
+			for (size_t q = 0; q < g.second.idx.size(); ++q)
+				dst[q] = src[g.second.idx[q]];
+		}
+	};
+	fill(gu, du);
+	fill(gc, ds);
+	fill(gs, ds);
+	std::memcpy(h + o_all, ds.data(), sizeof(LatDesc) * count);
+	hipStream_t s = st.stream;
+	BGT_HIP_CHECK(hipMemcpyAsync(d, h, o_in, hipMemcpyHostToDevice, s));
+	auto descs_at = [&](size_t offset) {
+		Descs x;
+		std::memset(&x, 0, sizeof(x));
+		x.many = reinterpret_cast<const LatDesc *>(d + offset);
+		return x;
+	};
+	// 1. updates (x L and the denominator)
+	for (auto &g : gu) {
+		const DeviceLattice &lat = *items[g.second.idx[0]].lat;
+		launch_update(descs_at(g.second.offset), (int)g.second.idx.size(), lat.states(), lat.variants(),
+					  lat.dilution(), lat.storage(), s);
+	}
+	// 2. atom marginals of the updated posteriors
+	for (auto &g : gc) {
+		const DeviceLattice &lat = *items[g.second.idx[0]].lat;
+		launch_atom_masses(descs_at(g.second.offset), (int)g.second.idx.size(), lat.atoms(), lat.storage(), s);
+	}
+	// 3. selection, as DeviceLattice::select does for one lattice
+	for (auto &g : gs) {
+		const DeviceLattice &lat = *items[g.second.idx[0]].lat;
+		const int cnt = (int)g.second.idx.size();
+		const Descs dg = descs_at(g.second.offset);
+		const int sel = DeviceLattice::resolve_selector(selector, lat.variants());
+		if (sel == kSelBrute) {
+			launch_brute_force(dg, cnt, lat.subjects(), lat.variants(), lat.storage(), s);
+			continue;
+		}
+		const int k = lat.variants();
+		const int objective = sel == kSelBha ? kObjHalving : (k == 1 ? kObjBinaryBbpa : kObjGather);
+		const double target = sel == kSelBha ? 0.5 : 1.0 / double(1 << k);
+		const int atoms = lat.atoms();
+		unsigned entries = 0;
+		if (atoms <= kSmallBits) {
+			entries = launch_zeta_small(dg, cnt, atoms, k, lat.storage(), objective, target, s);
+		} else if (atoms == kTileBits) {
+			entries = launch_zeta_final(dg, cnt, atoms, k, lat.storage(), objective, true, false, target, s);
+		} else {
+			launch_zeta_pass1(dg, cnt, atoms, lat.storage(), false, k, lat.dilution(), s);
+			entries = launch_zeta_final(dg, cnt, atoms, k, lat.storage(), objective, false, false, target, s);
+		}
+		if (objective == kObjGather)
+			launch_bbpa_score(dg, cnt, lat.subjects(), k, target, s);
+		else if (k == 1 && !lat.dilution())
+			launch_band_collect(dg, cnt, entries, s);
+	}
+	// 4. one copy brings every head back
+	launch_gather_heads(reinterpret_cast<const LatDesc *>(d + o_all), (int)count,
+						reinterpret_cast<unsigned char *>(d + o_heads), s);
+	BGT_HIP_CHECK(hipMemcpyAsync(h + o_heads, d + o_heads, kHeadBytes * count, hipMemcpyDeviceToHost, s));
+	BGT_HIP_CHECK(hipStreamSynchronize(s));
+	for (size_t i = 0; i < count; ++i) {
+		auto &it = items[i];
+		it.lat->load_head(reinterpret_cast<const unsigned char *>(h + o_heads + kHeadBytes * i));
+		if (it.classify)
+			it.lat->classify_from_head(up, lo, nullptr);
+		if (it.select)
+			out[i].selection = it.lat->finish_selection(selector, it.lat->head_result(), false);
+		out[i].positive = it.lat->positive_mask();
+		out[i].negative = it.lat->negative_mask();
+	}
+}
+
+} // namespace bgt_hip

@@ -1,0 +1,34 @@
+// Batched steps over many device-resident lattices: the continuous-batching core of the
+// serving loop and of the level-batched tree. One step applies an (experiment, response)
+// update to each lattice that has one, takes atom marginals and classifies, and selects the
+// next experiment, with one launch per kernel per group of identically shaped lattices,
+// one upload of descriptors and one download of all results. Each lattice goes through
+// the same kernels, with the same launch shapes, as the per-lattice calls (update,
+// classify, select), so its results are identical whether it steps alone or in a batch.
+#pragma once
+
+#include "bgt_hip_lattice.hpp"
+
+#include <vector>
+
+namespace bgt_hip {
+
+struct BatchItem {
+	DeviceLattice *lat = nullptr;
+	bool update = false; // apply (experiment, response) first
+	uint32_t experiment = 0, response = 0;
+	const double *dilution = nullptr;
+	bool classify = true; // atom marginals + sticky classification
+	bool select = true;   // select the next experiment
+};
+
+struct BatchResult {
+	bgt_hip_selection selection;
+	uint32_t positive, negative;
+};
+
+// `selector` applies to every item (resolved per lattice shape as in select()).
+void batch_step(std::vector<BatchItem> &items, int selector, double threshold_up, double threshold_lo,
+				std::vector<BatchResult> &out);
+
+} // namespace bgt_hip

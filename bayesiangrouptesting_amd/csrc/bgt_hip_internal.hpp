@@ -66,6 +66,9 @@ struct Cand {
 constexpr int kBandCapacity = 4096;
 // The first kBandHead of them travel with the 2 KB result head in one device-to-host copy.
 constexpr int kBandHead = 64;
+// Bytes at the start of every lattice's arena that one result copy brings back: scalars,
+// selection result, counters, atom masses and the first kBandHead band candidates.
+constexpr size_t kHeadBytes = 1024 + kBandHead * sizeof(Cand);
 
 // Everything a kernel needs to know about one lattice. Arrays of these live in device
 // memory so that one launch can serve a batch of lattices (blockIdx.y selects).
@@ -131,6 +134,9 @@ unsigned zeta_final_groups(int atoms);
 // candidates of the last zeta_final within d.band_width of the best: d.band[0..counter[0]),
 // counter[1] != 0 when a thread's runner-up was inside the band (the list may be incomplete)
 void launch_band_collect(const Descs &descs, int count, unsigned entries, hipStream_t stream);
+// copy the kHeadBytes head of each of `count` lattices (arena base = LatDesc::scal) into
+// out[i * kHeadBytes]
+void launch_gather_heads(const LatDesc *many, int count, unsigned char *out, hipStream_t stream);
 void launch_bbpa_score(const Descs &descs, int count, int n, int k, double target,
 					   hipStream_t stream);
 void launch_brute_force(const Descs &descs, int count, int n, int k, Storage st,

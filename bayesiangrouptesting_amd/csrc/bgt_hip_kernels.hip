@@ -1356,6 +1356,16 @@ __global__ __launch_bounds__(256) void k_band_collect(Descs descs, unsigned entr
 		atomicAdd(&d.counter[1], (unsigned)__popcll(ru));
 }
 
+// Result heads of a batch of lattices into one staging buffer (one copy to the host).
+__global__ __launch_bounds__(256) void k_gather_heads(const LatDesc *many, unsigned char *out)
+{
+	const LatDesc &d = many[blockIdx.x];
+	const uint4 *src = reinterpret_cast<const uint4 *>(d.scal);
+	uint4 *dst = reinterpret_cast<uint4 *>(out + (size_t)blockIdx.x * kHeadBytes);
+	for (unsigned i = threadIdx.x; i < kHeadBytes / sizeof(uint4); i += blockDim.x)
+		dst[i] = src[i];
+}
+
 __global__ __launch_bounds__(1024) void k_finalize_masses(Descs descs, unsigned count, int atoms)
 {
 	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
@@ -1548,6 +1558,13 @@ unsigned launch_zeta_small(const Descs &descs, int count, int n, int, Storage st
 			hipLaunchKernelGGL(k_zeta_small<float>, grid, dim3(1024), 0, stream, descs, n, objective, target);
 	});
 	return 2u * 1024u;
+}
+
+void launch_gather_heads(const LatDesc *many, int count, unsigned char *out, hipStream_t stream)
+{
+	launch_checked("k_gather_heads", [&] {
+		hipLaunchKernelGGL(k_gather_heads, dim3(count), dim3(256), 0, stream, many, out);
+	});
 }
 
 void launch_band_collect(const Descs &descs, int count, unsigned entries, hipStream_t stream)

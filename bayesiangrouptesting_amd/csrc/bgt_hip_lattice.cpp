@@ -125,7 +125,6 @@ size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 // Arena head: [0,256) scalars, [256,512) selection result, [512,768) counters,
 // [768,1024) atom masses, then the band candidates (kBandCapacity of them). One 2 KB copy
 // brings all of the head and the first kBandHead candidates to the host.
-constexpr size_t kHeadBytes = 1024 + kBandHead * sizeof(Cand);
 constexpr size_t kHeadResult = 256;
 constexpr size_t kHeadCounter = 512;
 constexpr size_t kHeadMasses = 768;
@@ -168,7 +167,7 @@ DeviceLattice::DeviceLattice(bool dilution, int subjects, int variants, const do
 	}
 	BGT_HIP_CHECK(hipMemcpyAsync(pairs_, pairs.data(), sizeof(double) * pairs.size(), hipMemcpyHostToDevice, stream_));
 	launch_prior(single(), 1, atoms_, storage_, pairs_, states_, stream_);
-	BGT_HIP_CHECK(hipStreamSynchronize(stream_));
+	sync();
 }
 
 DeviceLattice::DeviceLattice(const DeviceLattice &parent, bool copy_posterior)
@@ -189,7 +188,60 @@ DeviceLattice::DeviceLattice(const DeviceLattice &parent, bool copy_posterior)
 		const size_t eb = storage_ == Storage::fp64 ? 8 : 4;
 		BGT_HIP_CHECK(hipMemcpyAsync(post_, parent.post_, eb * states_, hipMemcpyDeviceToDevice, stream_));
 		BGT_HIP_CHECK(hipMemcpyAsync(scal_, parent.scal_, sizeof(double) * kScalars, hipMemcpyDeviceToDevice, stream_));
+		busy_ = true;
 	}
+}
+
+namespace {
+
+struct ArenaLayout {
+	size_t scal, res, cnt, mass, band, cand, pairs, part, tops, ftab, scores, sel, total;
+	bool exact;
+};
+
+ArenaLayout arena_layout(bool dilution, int n, int k)
+{
+	const int atoms = n * k;
+	ArenaLayout a{};
+	const uint64_t tiles = atoms > kTileBits ? (1ull << (atoms - kTileBits)) : 1ull;
+	const uint64_t chunks = atoms > 14 ? (1ull << (atoms - 14)) : 1ull;
+	const uint64_t ec = 1ull << n;
+	const size_t part = std::max<uint64_t>({2048, tiles, (chunks + 1) * kMaxAtoms});
+	size_t tops = std::max<uint64_t>({tiles, ec / 256 + 1, 1});
+	if (n <= 16)
+		tops = std::max<uint64_t>(tops, ec);
+	const size_t ftab = k >= 2 ? (size_t(1) << k) * ec : 1;
+	const size_t scores = (k == 1 && n <= 20) ? ec : 1;
+	size_t off = 0;
+	auto take = [&](size_t bytes) {
+		const size_t o = off;
+		off += align256(bytes);
+		return o;
+	};
+	a.exact = k == 1 && !dilution;
+	a.scal = take(sizeof(double) * kScalars); // 0
+	a.res = take(sizeof(SelResult));         // 256
+	a.cnt = take(sizeof(unsigned) * 4);      // 512
+	a.mass = take(sizeof(double) * 32);      // 768
+	a.band = take(a.exact ? sizeof(Cand) * kBandCapacity : 0); // 1024
+	a.cand = take(a.exact ? sizeof(Cand) * 2 * kTileThreads * zeta_final_groups(atoms) : 0);
+	a.pairs = take(sizeof(double) * 2 * kMaxAtoms);
+	a.part = take(sizeof(double) * part);
+	a.tops = take(sizeof(SelResult) * tops);
+	a.ftab = take(sizeof(double) * ftab);
+	a.scores = take(sizeof(double) * scores);
+	a.sel = take(sizeof(uint32_t) * 16);
+	a.total = std::max<size_t>(off, kHeadBytes);
+	return a;
+}
+
+} // namespace
+
+uint64_t DeviceLattice::footprint(bool dilution, int subjects, int variants, Storage storage)
+{
+	const int atoms = subjects * variants;
+	const size_t eb = storage == Storage::fp64 ? 8 : 4;
+	return (uint64_t)eb * (1ull << atoms) + z_bytes(atoms, eb) + arena_layout(dilution, subjects, variants).total;
 }
 
 void DeviceLattice::allocate()
@@ -197,55 +249,28 @@ void DeviceLattice::allocate()
 	const size_t eb = storage_ == Storage::fp64 ? 8 : 4;
 	BGT_HIP_CHECK(hipMalloc(&post_, eb * states_));
 	bytes_ += eb * states_;
-	const uint64_t tiles = atoms_ > kTileBits ? (1ull << (atoms_ - kTileBits)) : 1ull;
-	const uint64_t chunks = atoms_ > 14 ? (1ull << (atoms_ - 14)) : 1ull;
-	const uint64_t ec = 1ull << n_;
-	size_t part = std::max<uint64_t>({2048, tiles, (chunks + 1) * kMaxAtoms});
-	size_t tops = std::max<uint64_t>({tiles, ec / 256 + 1, 1});
-	if (n_ <= 16)
-		tops = std::max<uint64_t>(tops, ec);
-	const size_t ftab = k_ >= 2 ? (size_t(1) << k_) * ec : 1;
-	const size_t scores = (k_ == 1 && n_ <= 20) ? ec : 1;
-	size_t off = 0;
-	auto take = [&](size_t bytes) {
-		const size_t o = off;
-		off += align256(bytes);
-		return o;
-	};
-	const size_t o_scal = take(sizeof(double) * kScalars); // 0
-	const size_t o_res = take(sizeof(SelResult));         // 256
-	const size_t o_cnt = take(sizeof(unsigned) * 4);      // 512
-	const size_t o_mass = take(sizeof(double) * 32);      // 768
-	const bool exact = k_ == 1 && !dilution_;
-	const size_t o_band = take(exact ? sizeof(Cand) * kBandCapacity : 0); // 1024
-	const size_t o_cand = take(exact ? sizeof(Cand) * 2 * kTileThreads * zeta_final_groups(atoms_) : 0);
-	const size_t o_pairs = take(sizeof(double) * 2 * kMaxAtoms);
-	const size_t o_part = take(sizeof(double) * part);
-	const size_t o_tops = take(sizeof(SelResult) * tops);
-	const size_t o_ftab = take(sizeof(double) * ftab);
-	const size_t o_scores = take(sizeof(double) * scores);
-	const size_t o_sel = take(sizeof(uint32_t) * 16);
-	BGT_HIP_CHECK(hipMalloc(reinterpret_cast<void **>(&arena_), off));
-	bytes_ += off;
-	BGT_HIP_CHECK(hipMemsetAsync(arena_, 0, off, stream_));
-	scal_ = reinterpret_cast<double *>(arena_ + o_scal);
-	result_ = reinterpret_cast<SelResult *>(arena_ + o_res);
-	counter_ = reinterpret_cast<unsigned *>(arena_ + o_cnt);
-	masses_ = reinterpret_cast<double *>(arena_ + o_mass);
-	pairs_ = reinterpret_cast<double *>(arena_ + o_pairs);
-	part_ = reinterpret_cast<double *>(arena_ + o_part);
-	tops_ = reinterpret_cast<SelResult *>(arena_ + o_tops);
-	ftab_ = reinterpret_cast<double *>(arena_ + o_ftab);
-	scores_ = reinterpret_cast<double *>(arena_ + o_scores);
-	selected_ = reinterpret_cast<uint32_t *>(arena_ + o_sel);
-	if (exact) {
-		band_ = reinterpret_cast<Cand *>(arena_ + o_band);
-		cand_ = reinterpret_cast<Cand *>(arena_ + o_cand);
+	const ArenaLayout a = arena_layout(dilution_, n_, k_);
+	BGT_HIP_CHECK(hipMalloc(reinterpret_cast<void **>(&arena_), a.total));
+	bytes_ += a.total;
+	BGT_HIP_CHECK(hipMemsetAsync(arena_, 0, a.total, stream_));
+	scal_ = reinterpret_cast<double *>(arena_ + a.scal);
+	result_ = reinterpret_cast<SelResult *>(arena_ + a.res);
+	counter_ = reinterpret_cast<unsigned *>(arena_ + a.cnt);
+	masses_ = reinterpret_cast<double *>(arena_ + a.mass);
+	pairs_ = reinterpret_cast<double *>(arena_ + a.pairs);
+	part_ = reinterpret_cast<double *>(arena_ + a.part);
+	tops_ = reinterpret_cast<SelResult *>(arena_ + a.tops);
+	ftab_ = reinterpret_cast<double *>(arena_ + a.ftab);
+	scores_ = reinterpret_cast<double *>(arena_ + a.scores);
+	selected_ = reinterpret_cast<uint32_t *>(arena_ + a.sel);
+	if (a.exact) {
+		band_ = reinterpret_cast<Cand *>(arena_ + a.band);
+		cand_ = reinterpret_cast<Cand *>(arena_ + a.cand);
 	}
 	double init[kScalars] = {1.0, 1.0, 1.0, 1.0, 0.0, 0.0, 0.0, 0.0};
 	BGT_HIP_CHECK(hipMemcpyAsync(scal_, init, sizeof(init), hipMemcpyHostToDevice, stream_));
 	BGT_HIP_CHECK(hipHostMalloc(&host_box_, sizeof(HostBox), hipHostMallocDefault));
-	BGT_HIP_CHECK(hipStreamSynchronize(stream_));
+	sync();
 }
 
 void DeviceLattice::ensure_z()
@@ -314,7 +339,17 @@ Descs DeviceLattice::single() const
 	return ds;
 }
 
-void DeviceLattice::sync() { BGT_HIP_CHECK(hipStreamSynchronize(stream_)); }
+void DeviceLattice::sync()
+{
+	BGT_HIP_CHECK(hipStreamSynchronize(stream_));
+	busy_ = false;
+}
+
+void DeviceLattice::settle()
+{
+	if (busy_)
+		sync();
+}
 
 // ----------------------------------------------------------------------- likelihoods
 
@@ -368,6 +403,7 @@ void DeviceLattice::update(uint32_t experiment, uint32_t response, const double 
 	launch_update(single(), 1, states_, k_, dilution_, storage_, stream_);
 	clock_.end(stream_, "update");
 	commit_update();
+	busy_ = true;
 }
 
 // ------------------------------------------------------------------------- selection
@@ -394,7 +430,15 @@ void DeviceLattice::fetch_head()
 {
 	HostBox *box = static_cast<HostBox *>(host_box_);
 	BGT_HIP_CHECK(hipMemcpyAsync(box->head, arena_, kHeadBytes, hipMemcpyDeviceToHost, stream_));
-	BGT_HIP_CHECK(hipStreamSynchronize(stream_));
+	sync();
+	load_head(box->head);
+}
+
+void DeviceLattice::load_head(const unsigned char *head)
+{
+	HostBox *box = static_cast<HostBox *>(host_box_);
+	if (head != box->head)
+		std::memcpy(box->head, head, kHeadBytes);
 	std::memcpy(box->scal, box->head, sizeof(double) * kScalars);
 	std::memcpy(&box->result, box->head + kHeadResult, sizeof(SelResult));
 	std::memcpy(box->counter, box->head + kHeadCounter, sizeof(box->counter));
@@ -406,7 +450,7 @@ SelResult DeviceLattice::fetch_result()
 {
 	HostBox *box = static_cast<HostBox *>(host_box_);
 	BGT_HIP_CHECK(hipMemcpyAsync(&box->result, result_, sizeof(SelResult), hipMemcpyDeviceToHost, stream_));
-	BGT_HIP_CHECK(hipStreamSynchronize(stream_));
+	sync();
 	return box->result;
 }
 
@@ -414,7 +458,7 @@ void DeviceLattice::fetch_masses(double *out)
 {
 	HostBox *box = static_cast<HostBox *>(host_box_);
 	BGT_HIP_CHECK(hipMemcpyAsync(box->masses, masses_, sizeof(double) * 32, hipMemcpyDeviceToHost, stream_));
-	BGT_HIP_CHECK(hipStreamSynchronize(stream_));
+	sync();
 	std::memcpy(out, box->masses, sizeof(double) * atoms_);
 }
 
@@ -490,7 +534,7 @@ void DeviceLattice::rescore(bgt_hip_selection &out, int sel, const SelResult &r)
 		if (count > inhead) {
 			BGT_HIP_CHECK(hipMemcpyAsync(box->band + inhead, band_ + inhead, sizeof(Cand) * (count - inhead),
 										 hipMemcpyDeviceToHost, stream_));
-			BGT_HIP_CHECK(hipStreamSynchronize(stream_));
+			sync();
 			cands.insert(cands.end(), box->band + inhead, box->band + count);
 		}
 	}
@@ -608,7 +652,7 @@ double DeviceLattice::posterior(uint32_t state)
 	const size_t eb = storage_ == Storage::fp64 ? 8 : 4;
 	BGT_HIP_CHECK(hipMemcpyAsync(&box->raw, static_cast<char *>(post_) + eb * state, eb, hipMemcpyDeviceToHost, stream_));
 	BGT_HIP_CHECK(hipMemcpyAsync(box->scal, scal_, sizeof(double) * kScalars, hipMemcpyDeviceToHost, stream_));
-	BGT_HIP_CHECK(hipStreamSynchronize(stream_));
+	sync();
 	double y;
 	if (storage_ == Storage::fp64) {
 		y = box->raw;
@@ -626,14 +670,14 @@ void DeviceLattice::read_posterior(double *out)
 	BGT_HIP_CHECK(hipMemcpyAsync(box->scal, scal_, sizeof(double) * kScalars, hipMemcpyDeviceToHost, stream_));
 	if (storage_ == Storage::fp64) {
 		BGT_HIP_CHECK(hipMemcpyAsync(out, post_, 8 * states_, hipMemcpyDeviceToHost, stream_));
-		BGT_HIP_CHECK(hipStreamSynchronize(stream_));
+		sync();
 		const double inv = box->scal[inv_cur_];
 		for (uint64_t s = 0; s < states_; ++s)
 			out[s] = out[s] * inv;
 	} else {
 		std::vector<float> tmp(states_);
 		BGT_HIP_CHECK(hipMemcpyAsync(tmp.data(), post_, 4 * states_, hipMemcpyDeviceToHost, stream_));
-		BGT_HIP_CHECK(hipStreamSynchronize(stream_));
+		sync();
 		const double inv = box->scal[inv_cur_];
 		for (uint64_t s = 0; s < states_; ++s)
 			out[s] = static_cast<double>(tmp[s]) * inv;
@@ -650,7 +694,7 @@ double DeviceLattice::upset_mass(uint32_t experiment)
 	clock_.end(stream_, "upset_mass");
 	HostBox *box = static_cast<HostBox *>(host_box_);
 	BGT_HIP_CHECK(hipMemcpyAsync(box->scal, scal_, sizeof(double) * kScalars, hipMemcpyDeviceToHost, stream_));
-	BGT_HIP_CHECK(hipStreamSynchronize(stream_));
+	sync();
 	return box->scal[kScalUpset];
 }
 
@@ -704,7 +748,7 @@ std::vector<uint32_t> DeviceLattice::select_experiments(int steps)
 		BGT_HIP_CHECK(hipMemcpyAsync(selected_, out.data(), sizeof(uint32_t) * out.size(), hipMemcpyHostToDevice, stream_));
 		launch_lookahead(single(), n_, storage_, selected_, (int)out.size(), scores_, stream_);
 		BGT_HIP_CHECK(hipMemcpyAsync(scores.data(), scores_, sizeof(double) * scores.size(), hipMemcpyDeviceToHost, stream_));
-		BGT_HIP_CHECK(hipStreamSynchronize(stream_));
+		sync();
 		double best = std::numeric_limits<double>::max();
 		uint32_t best_c = 0;
 		for (uint32_t c = 1; c < (1u << n_); ++c)
@@ -744,22 +788,27 @@ bgt_hip_selection DeviceLattice::round(uint32_t experiment, uint32_t response, c
 	clock_.end(stream_, "round_pass2");
 	collect_band(ds, entries);
 	fetch_head();
-	HostBox *box = static_cast<HostBox *>(host_box_);
-	const SelResult r = box->result;
-	double m[32];
-	std::memcpy(m, box->masses, sizeof(m));
+	classify_from_head(up, lo, masses_out);
+	return finish_selection(kSelBha, static_cast<HostBox *>(host_box_)->result, true);
+}
+
+SelResult DeviceLattice::head_result() const { return static_cast<const HostBox *>(host_box_)->result; }
+
+void DeviceLattice::classify_from_head(double up, double lo, double *masses_out)
+{
+	// lattice_base.cpp:246-264
+	const HostBox *box = static_cast<const HostBox *>(host_box_);
 	for (int i = 0; i < atoms_; ++i) {
 		const uint32_t placement = 1u << i;
 		if ((pos_ | neg_) & placement)
 			continue;
-		if (m[i] < lo)
+		if (box->masses[i] < lo)
 			pos_ |= placement;
-		else if (m[i] > (1 - up))
+		else if (box->masses[i] > (1 - up))
 			neg_ |= placement;
 	}
 	if (masses_out)
-		std::memcpy(masses_out, m, sizeof(double) * atoms_);
-	return finish_selection(kSelBha, r, true);
+		std::memcpy(masses_out, box->masses, sizeof(double) * atoms_);
 }
 
 void DeviceLattice::error_bound(double *posterior_rel, double *mass_rel) const

@@ -92,15 +92,30 @@ public:
 
 	// parameters for the selection launchers
 	static int resolve_selector(int selector, int variants);
+	// device bytes a lattice of this shape holds once it has selected (posterior,
+	// transform intermediate and arena)
+	static uint64_t footprint(bool dilution, int subjects, int variants, Storage storage);
+
+	// ---- hooks for batched steps (bgt_hip_batch.cpp) ----
+	void check_selector(int selector) const;
+	void ensure_z();
+	// wait for work this lattice's own stream may still have in flight
+	void settle();
+	// error band of the selection objective (device objectives closer than this may be
+	// misordered)
+	double band_width(int selector) const;
+	// take a result head gathered by a batched step (the same bytes fetch_head copies)
+	void load_head(const unsigned char *head);
+	// sticky classification from the masses of the last head (lattice_base.cpp:246-264)
+	void classify_from_head(double threshold_up, double threshold_lo, double *masses_out);
+	// selection result of the last head
+	SelResult head_result() const;
 
 private:
 	Descs single() const;
 	void allocate();
-	void check_selector(int selector) const;
-	void ensure_z();
 	void fill_lik(uint32_t experiment, uint32_t response, const double *dilution_flat);
 	void rescore(bgt_hip_selection &out, int selector, const SelResult &r);
-	double band_width(int selector) const;
 	void collect_band(const Descs &ds, unsigned entries);
 
 	bool dilution_;
@@ -137,6 +152,7 @@ private:
 	uint32_t pos_ = 0, neg_ = 0;
 	uint32_t active_ = 0;
 	int updates_ = 0;
+	bool busy_ = false; // work may be pending on stream_ (asynchronous update or copy)
 	KernelClock clock_;
 };
 

@@ -1,0 +1,234 @@
+"""A serving loop for concurrent adaptive-testing sessions with continuous batching.
+
+Each session is one lattice resident in HBM, running the adaptive loop: select an
+experiment, receive its response, update, classify, until every atom is classified or a
+test budget is used up. Sessions arrive over time (a Poisson process in the benchmark
+workload). The engine admits arrived sessions while their device footprint fits the HBM
+budget, and every engine step advances every active session by one round with a single
+batched call (bgt_hip_batch_round: one launch per kernel per group of identically shaped
+lattices, one descriptor upload, one result download). Sessions that arrive while a step
+is on the GPU join the next step, and finished sessions free their memory at once, so
+admitted work keeps flowing rather than waiting for a batch to drain.
+
+Responses come from a seeded simulated lab: each session draws its true state from its
+prior and samples responses with the reference's likelihood (the client side of the
+loop). A session's selections are identical whether it is served alone or in any mix,
+because each lattice runs the same kernels with the same launch shapes either way.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+import time
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import api
+from . import workload
+
+
+@dataclasses.dataclass
+class SessionSpec:
+    sid: int
+    subjects: int
+    variants: int
+    dilution: bool
+    prior: list
+    seed: int
+    arrival: float = 0.0  # seconds after the start of the run
+
+
+@dataclasses.dataclass
+class SessionResult:
+    sid: int
+    subjects: int
+    variants: int
+    dilution: bool
+    tests: int
+    classified: bool
+    positive: int
+    negative: int
+    experiments: list
+    arrival: float
+    admitted: float
+    finished: float
+
+    @property
+    def latency(self) -> float:
+        return self.finished - self.arrival
+
+
+def serving_mix(count: int, rate: float, seed: int = 5, lo: float = 0.005, hi: float = 0.10) -> list[SessionSpec]:
+    """The serving workload: n uniform in 10..24, k in {1, 2} with n*k <= 26, dilution and
+    non-dilution 50/50, priors U[lo, hi], Poisson arrivals at `rate` sessions/s."""
+    g = workload.SplitMix64(seed)
+    out = []
+    t = 0.0
+    for sid in range(count):
+        while True:
+            n = 10 + g.below(15)
+            k = 1 + g.below(2)
+            if n * k <= 26:
+                break
+        dil = g.below(2) == 1
+        s_seed = g.next_u64()
+        prior = workload.priors(s_seed & 0xFFFFFFFF, n * k, lo, hi)
+        t += -math.log(1.0 - g.uniform()) / rate if rate > 0 else 0.0
+        out.append(SessionSpec(sid, n, k, dil, prior, s_seed & 0xFFFFFFFF, t))
+    return out
+
+
+class _Live:
+    """An admitted session: its lattice, simulated lab and progress."""
+
+    def __init__(self, spec: SessionSpec, precision: api.Precision, admitted: float):
+        self.spec = spec
+        t = api.LatticeType.replicated_dilution if spec.dilution else api.LatticeType.replicated_non_dilution
+        self.lattice = api.Lattice(t, spec.subjects, spec.prior, variants=spec.variants, precision=precision)
+        self.table = api.DilutionTable(spec.subjects, 0.05, 0.9) if spec.dilution else None
+        truth = workload.true_state(spec.seed, spec.prior)
+        if spec.variants == 1:
+            truth = api.reverse_bits(truth, spec.subjects)  # public coordinates
+        self.sampler = workload.ResponseSampler(spec.seed, spec.subjects, spec.variants, truth,
+                                                list(self.table.values) if self.table is not None else None)
+        self.admitted = admitted
+        self.pending: Optional[int] = None  # experiment selected and not yet applied
+        self.tests = 0
+        self.experiments: list[int] = []
+        self.pos = self.neg = 0
+
+    def classified(self) -> bool:
+        return bin(self.pos | self.neg).count("1") == self.spec.subjects * self.spec.variants
+
+
+class ServingEngine:
+    def __init__(self, precision: api.Precision = api.Precision.float64, hbm_budget: int = 64 << 30,
+                 max_tests: int = 20, threshold_up: float = 0.01, threshold_lo: float = 0.01,
+                 max_active: int = 1024):
+        self.precision = precision
+        self.hbm_budget = hbm_budget
+        self.max_tests = max_tests
+        self.up = threshold_up
+        self.lo = threshold_lo
+        self.max_active = max_active
+
+    def footprint(self, spec: SessionSpec) -> int:
+        t = api.LatticeType.replicated_dilution if spec.dilution else api.LatticeType.replicated_non_dilution
+        return api.lattice_footprint(t, spec.subjects, spec.variants, self.precision)
+
+    def run(self, specs: Sequence[SessionSpec], realtime: bool = True) -> dict:
+        """Serve `specs` (arrival times relative to the start). With realtime, sessions
+        become visible at their arrival times on the wall clock; otherwise all are
+        queued at time zero. Returns per-session results and aggregate statistics."""
+        queue = sorted(specs, key=lambda s: (s.arrival, s.sid))
+        qi = 0
+        live: list[_Live] = []
+        used = 0
+        peak_used = 0
+        done: list[SessionResult] = []
+        step_ms: list[float] = []
+        batch_sizes: list[int] = []
+        fp = {}
+        t0 = time.perf_counter()
+
+        def now() -> float:
+            return time.perf_counter() - t0
+
+        while qi < len(queue) or live:
+            # admission: arrived sessions, in order, while their footprint fits
+            tnow = now()
+            while qi < len(queue) and len(live) < self.max_active:
+                spec = queue[qi]
+                if realtime and spec.arrival > tnow:
+                    break
+                key = (spec.dilution, spec.subjects, spec.variants)
+                if key not in fp:
+                    fp[key] = self.footprint(spec)
+                if used + fp[key] > self.hbm_budget:
+                    if not live:
+                        raise RuntimeError(f"session {spec.sid} alone exceeds the HBM budget")
+                    break
+                live.append(_Live(spec, self.precision, tnow))
+                used += fp[key]
+                peak_used = max(peak_used, used)
+                qi += 1
+            if not live:
+                # idle until the next arrival
+                wait = queue[qi].arrival - now()
+                if wait > 0:
+                    time.sleep(min(wait, 0.01))
+                continue
+            # one batched round for every active session
+            exps, resps = [], []
+            for s in live:
+                if s.pending is None:
+                    exps.append(None)
+                    resps.append(None)
+                else:
+                    exps.append(s.pending)
+                    resps.append(s.sampler.draw(s.pending))
+            ts = time.perf_counter()
+            sels, masks = api.batch_round([s.lattice for s in live], exps, resps, api.SelectorType.auto_select,
+                                          self.up, self.lo, [s.table for s in live])
+            step_ms.append((time.perf_counter() - ts) * 1e3)
+            batch_sizes.append(len(live))
+            tdone = now()
+            keep = []
+            for s, sel, (p, q), r in zip(live, sels, masks, resps):
+                if r is not None:
+                    s.tests += 1
+                s.pos, s.neg = p, q
+                finished = s.classified() or s.tests >= self.max_tests
+                if not finished:
+                    s.pending = sel.experiment
+                    s.experiments.append(sel.experiment)
+                    keep.append(s)
+                    continue
+                spec = s.spec
+                done.append(SessionResult(spec.sid, spec.subjects, spec.variants, spec.dilution, s.tests,
+                                          s.classified(), s.pos, s.neg, s.experiments, spec.arrival, s.admitted,
+                                          tdone))
+                used -= fp[(spec.dilution, spec.subjects, spec.variants)]
+                s.lattice.close()
+            live = keep
+        wall = now()
+        lat = np.array([r.latency for r in done]) * 1e3
+        return {
+            "sessions": len(done),
+            "wall_s": wall,
+            "sessions_per_s": len(done) / wall if wall > 0 else float("inf"),
+            "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                           "max": float(lat.max())},
+            "step_ms": {"p50": float(np.percentile(step_ms, 50)), "p99": float(np.percentile(step_ms, 99)),
+                        "steps": len(step_ms)},
+            "batch": {"mean": float(np.mean(batch_sizes)), "max": int(np.max(batch_sizes))},
+            "tests_mean": float(np.mean([r.tests for r in done])),
+            "classified": sum(r.classified for r in done),
+            "peak_hbm_bytes": peak_used,
+            "results": done,
+        }
+
+
+def serve_alone(spec: SessionSpec, precision: api.Precision = api.Precision.float64, max_tests: int = 20,
+                up: float = 0.01, lo: float = 0.01) -> SessionResult:
+    """The same session through per-lattice calls (update, classify, select), one at a time."""
+    s = _Live(spec, precision, 0.0)
+    sel = s.lattice.select(api.SelectorType.auto_select)
+    s.lattice.update_classification(up, lo)
+    s.pos, s.neg = s.lattice.positive_classification_mask(), s.lattice.negative_classification_mask()
+    while not (s.classified() or s.tests >= max_tests):
+        e = sel.experiment
+        s.experiments.append(e)
+        r = s.sampler.draw(e)
+        s.lattice.update(e, r, s.table)
+        s.tests += 1
+        s.lattice.update_classification(up, lo)
+        s.pos, s.neg = s.lattice.positive_classification_mask(), s.lattice.negative_classification_mask()
+        if s.classified() or s.tests >= max_tests:
+            break
+        sel = s.lattice.select(api.SelectorType.auto_select)
+    out = SessionResult(spec.sid, spec.subjects, spec.variants, spec.dilution, s.tests, s.classified(), s.pos, s.neg,
+                        s.experiments, 0.0, 0.0, 0.0)
+    s.lattice.close()
+    return out

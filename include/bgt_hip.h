@@ -190,13 +190,34 @@ int bgt_hip_lattice_kernel_times(bgt_hip_lattice *lattice, int enable, double *m
 int bgt_hip_lattice_info(bgt_hip_lattice *lattice, int *subjects, int *variants, int *precision,
 						 uint64_t *device_bytes);
 
+/* Device bytes a lattice of this shape and precision holds once it has selected, without
+ * creating it (admission control by HBM footprint). No reference counterpart. */
+int bgt_hip_lattice_footprint(int use_dilution, int subjects, int variants, int precision, uint64_t *bytes);
+
 /* ---- batches and serving ---- */
 
-/* Apply one (experiment, response) update to each lattice, then select with `selector`
- * on each, in batched launches grouped by shape. Results identical to per-lattice calls. */
+/* response value that means "no update for this lattice in this step" (select only) */
+#define BGT_HIP_NO_UPDATE 0xFFFFFFFFu
+
+/* Apply one (experiment, response) update to each lattice (none where responses[i] is
+ * BGT_HIP_NO_UPDATE), then select with `selector` on each: one launch per kernel per group
+ * of identically shaped lattices, one descriptor upload and one result download for the
+ * whole batch. Each lattice runs the same kernels with the same launch shapes as
+ * bgt_hip_lattice_update + bgt_hip_lattice_select, so its results are identical to those
+ * per-lattice calls. A lattice may appear at most once per batch. No reference
+ * counterpart (the reference steps one lattice at a time). */
 int bgt_hip_batch_step(bgt_hip_lattice **lattices, const uint32_t *experiments,
 					   const uint32_t *responses, const double *const *dilutions, int count,
 					   int selector, bgt_hip_selection *out);
+
+/* As bgt_hip_batch_step, with atom marginals and sticky classification between the update
+ * and the selection (the adaptive loop's round for every lattice: the same results as
+ * update + classify + select per lattice). positive/negative receive each lattice's masks
+ * (may be NULL). */
+int bgt_hip_batch_round(bgt_hip_lattice **lattices, const uint32_t *experiments,
+						const uint32_t *responses, const double *const *dilutions, int count,
+						int selector, double threshold_up, double threshold_lo,
+						bgt_hip_selection *out, uint32_t *positive, uint32_t *negative);
 
 /* ---- exact rescoring (host only, no device needed) ---- */
 
