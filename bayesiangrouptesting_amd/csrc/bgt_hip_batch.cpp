@@ -202,13 +202,17 @@ void batch_step(std::vector<BatchItem> &items, int selector, double up, double l
 		}
 		if (objective == kObjGather)
 			launch_bbpa_score(dg, cnt, lat.subjects(), k, target, s);
-		else if (k == 1 && !lat.dilution())
-			launch_band_collect(dg, cnt, entries, s);
+		else if (k == 1 && !lat.dilution() && atoms > kSmallBits)
+			launch_band_collect(dg, cnt, entries, atoms, k, lat.storage(), objective, atoms == kTileBits, false,
+								target, s);
 	}
 	// 4. one copy brings every head back
 	launch_gather_heads(reinterpret_cast<const LatDesc *>(d + o_all), (int)count,
 						reinterpret_cast<unsigned char *>(d + o_heads), s);
 	BGT_HIP_CHECK(hipMemcpyAsync(h + o_heads, d + o_heads, kHeadBytes * count, hipMemcpyDeviceToHost, s));
+	for (auto &it : items)
+		if (it.select)
+			it.lat->prepare_exact(); // on the host while the device works
 	BGT_HIP_CHECK(hipStreamSynchronize(s));
 	for (size_t i = 0; i < count; ++i) {
 		auto &it = items[i];

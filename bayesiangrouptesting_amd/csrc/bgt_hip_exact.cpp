@@ -1,7 +1,12 @@
 #include "bgt_hip_exact.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <condition_variable>
+#include <functional>
+#include <mutex>
+#include <thread>
 
 namespace bgt_hip {
 
@@ -16,25 +21,132 @@ inline uint32_t mix(uint64_t k)
 	k ^= k >> 33;
 	return (uint32_t)k;
 }
+// A fixed pool of host threads for exact rescoring: run(n, f) calls f(i, worker) for
+// i in [0, n) on the pool and the calling thread (worker 0) and returns when all are done.
+// Calls from different host threads are serialised.
+class Pool {
+public:
+	static Pool &get()
+	{
+		static Pool pool;
+		return pool;
+	}
+	int size() const { return (int)threads_.size() + 1; }
+	void run(size_t n, const std::function<void(size_t, int)> &f)
+	{
+		std::lock_guard<std::mutex> call(call_);
+		{
+			std::lock_guard<std::mutex> lk(m_);
+			job_ = &f;
+			n_ = n;
+			next_.store(0);
+			busy_ = (int)threads_.size();
+			++gen_;
+		}
+		cv_.notify_all();
+		work(0);
+		std::unique_lock<std::mutex> lk(m_);
+		done_.wait(lk, [&] { return busy_ == 0; });
+		job_ = nullptr;
+	}
+	~Pool()
+	{
+		{
+			std::lock_guard<std::mutex> lk(m_);
+			stop_ = true;
+			++gen_;
+		}
+		cv_.notify_all();
+		for (auto &t : threads_)
+			t.join();
+	}
+
+private:
+	Pool()
+	{
+		const unsigned hw = std::thread::hardware_concurrency();
+		const int extra = (int)std::min(7u, hw > 1 ? hw - 1 : 0u);
+		for (int w = 1; w <= extra; ++w)
+			threads_.emplace_back([this, w] { loop(w); });
+	}
+	void work(int w)
+	{
+		// blocks of 8 items: fewer atomics, and neighbouring outputs stay on one thread
+		constexpr size_t kBlock = 8;
+		for (size_t b = next_.fetch_add(kBlock); b < n_; b = next_.fetch_add(kBlock))
+			for (size_t i = b; i < std::min(n_, b + kBlock); ++i)
+				(*job_)(i, w);
+	}
+	void loop(int w)
+	{
+		uint64_t seen = 0;
+		while (true) {
+			{
+				std::unique_lock<std::mutex> lk(m_);
+				cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+				if (stop_)
+					return;
+				seen = gen_;
+			}
+			work(w);
+			std::lock_guard<std::mutex> lk(m_);
+			if (--busy_ == 0)
+				done_.notify_one();
+		}
+	}
+	std::vector<std::thread> threads_;
+	std::mutex call_, m_;
+	std::condition_variable cv_, done_;
+	const std::function<void(size_t, int)> *job_ = nullptr;
+	size_t n_ = 0;
+	std::atomic<size_t> next_{0};
+	int busy_ = 0;
+	uint64_t gen_ = 0;
+	bool stop_ = false;
+};
+
 } // namespace
 
-// Open-addressing map from pool-set state to weight; entries keep insertion order, so the
-// final summation order is deterministic.
-void ExactHistory::Table::clear(size_t expect)
+void ExactHistory::Table::clear(size_t expect, int pools)
 {
+	if (pools <= kDenseBits) {
+		const size_t cap = size_t(1) << pools;
+		if (!dense || slot.size() < cap) {
+			slot.assign(std::max(cap, slot.size()), -1);
+		} else {
+			for (uint64_t k : key)
+				slot[k] = -1;
+		}
+		dense = true;
+		key.clear();
+		w.clear();
+		return;
+	}
 	key.clear();
 	w.clear();
 	size_t cap = 16;
 	while (cap < 2 * expect)
 		cap <<= 1;
-	if (slot.size() != cap)
+	if (slot.size() != cap || dense)
 		slot.assign(cap, -1);
 	else
 		std::fill(slot.begin(), slot.end(), -1);
+	dense = false;
 }
 
 void ExactHistory::Table::add(uint64_t k, double v)
 {
+	if (dense) {
+		int32_t &s = slot[k];
+		if (s < 0) {
+			s = (int32_t)key.size();
+			key.push_back(k);
+			w.push_back(v);
+		} else {
+			w[s] += v;
+		}
+		return;
+	}
 	const size_t m = slot.size() - 1;
 	size_t h = mix(k) & m;
 	while (true) {
@@ -126,7 +238,7 @@ bool ExactHistory::unnormalised(uint32_t e, double &out, Scratch &sc) const
 	for (uint32_t p : sc.pool)
 		relevant |= p;
 	Table *cur = &sc.a, *nxt = &sc.b;
-	cur->clear(1);
+	cur->clear(1, (int)c);
 	cur->add(c == 64 ? ~0ull : ((1ull << c) - 1ull), 1.0);
 	for (int a = 0; a < atoms_; ++a) {
 		if (!((relevant >> a) & 1u))
@@ -135,7 +247,7 @@ bool ExactHistory::unnormalised(uint32_t e, double &out, Scratch &sc) const
 		for (size_t i = 0; i < c; ++i)
 			if ((sc.pool[i] >> a) & 1u)
 				kill |= 1ull << i;
-		nxt->clear(2 * cur->key.size());
+		nxt->clear(2 * cur->key.size(), (int)c);
 		for (size_t s = 0; s < cur->key.size(); ++s) {
 			const uint64_t k = cur->key[s];
 			const double wv = cur->w[s];
@@ -185,11 +297,21 @@ bool ExactHistory::masses(const uint32_t *e, size_t count, double *out, bool *ok
 {
 	if (!usable() || !normaliser())
 		return false;
-	for (size_t i = 0; i < count; ++i) {
+	auto one = [&](size_t i, Scratch &sc) {
 		double u = 0.0;
-		ok[i] = unnormalised(e[i], u, scratch_);
+		ok[i] = unnormalised(e[i], u, sc);
 		out[i] = ok[i] ? u / z_ : NAN;
+	};
+	constexpr size_t kParallelFrom = 16;
+	if (count < kParallelFrom) {
+		for (size_t i = 0; i < count; ++i)
+			one(i, scratch_);
+		return true;
 	}
+	Pool &pool = Pool::get();
+	if (workers_.size() < (size_t)pool.size())
+		workers_.resize(pool.size());
+	pool.run(count, [&](size_t i, int w) { one(i, workers_[w]); });
 	return true;
 }
 

@@ -32,29 +32,44 @@ public:
 	// M(E) after all pushed updates; false if the state budget of the program is exceeded
 	bool mass(uint32_t experiment, double &out);
 	// M for `count` experiments (ok[i] false where the budget was exceeded); false if the
-	// normaliser itself is out of budget
+	// normaliser itself is out of budget. Large requests are spread over a pool of host
+	// worker threads; each result is the same as mass() would give.
 	bool masses(const uint32_t *experiments, size_t count, double *out, bool *ok);
 	// relative rounding bound of mass() (a few ulp per atom and per update)
 	double rel_bound() const;
+	// compute the normaliser of the current history now (host work that can overlap with
+	// device work; mass() would otherwise do it on first use)
+	void prepare()
+	{
+		if (usable())
+			normaliser();
+	}
 
 private:
 	struct Upd {
 		uint32_t e;
 		double f_in, f_out; // factor when the pool is / is not contained in S
 	};
+	// Pool-set states of the program in insertion order with their weights. Up to
+	// kDenseBits pools, states index a direct map (reset entry by entry); beyond that an
+	// open-addressing hash. Either way entries keep insertion order, so the summation order
+	// is the same.
+	static constexpr int kDenseBits = 16;
 	struct Table {
 		std::vector<uint64_t> key;
 		std::vector<double> w;
 		std::vector<int32_t> slot;
-		void clear(size_t expect);
+		bool dense = false;
+		void clear(size_t expect, int pools);
 		void add(uint64_t k, double v);
 	};
-	struct Scratch {
+	struct alignas(128) Scratch { // aligned: the pool's threads each grow their own
 		std::vector<uint32_t> pool;
 		std::vector<double> fin, fout;
 		Table a, b;
 	};
 	bool unnormalised(uint32_t e, double &out, Scratch &sc) const;
+	std::vector<Scratch> workers_; // one per pool thread (masses())
 	bool normaliser();
 
 	bool usable_ = false, broken_ = false;

@@ -56,14 +56,24 @@ struct SelResult {
 	uint32_t e1, e2;
 };
 
-// One selection candidate: objective and experiment (internal coordinates).
+// One selection candidate (internal coordinates). `d` is the objective with a sign: its
+// magnitude is the objective and its sign says on which side of the target the candidate's
+// mass lies (negative: M(E) below 1/2 for Op-BHA, P(0) below P(1) for binary BBPA).
+// Candidates are ordered by |d|, then by experiment. flags: kCandSided when the sign is
+// meaningful.
 struct Cand {
 	double d;
 	uint32_t e;
-	uint32_t pad;
+	uint32_t flags;
 };
-// Candidates within the error band of the best objective, collected for exact rescoring.
-constexpr int kBandCapacity = 4096;
+constexpr uint32_t kCandSided = 2u;
+// Candidates within the error band of the best objective, collected for exact rescoring:
+// up to kBandSmall of them for lattices of at most 20 atoms, kBandLarge above.
+constexpr int kBandSmall = 4096;
+constexpr int kBandLarge = 65536;
+inline int band_capacity(int atoms) { return atoms > 20 ? kBandLarge : kBandSmall; }
+// per-workgroup flags of the final transform launch (at most this many workgroups)
+constexpr int kMaxFinalGroups = 1024;
 // The first kBandHead of them travel with the 2 KB result head in one device-to-host copy.
 constexpr int kBandHead = 64;
 // Bytes at the start of every lattice's arena that one result copy brings back: scalars,
@@ -90,8 +100,10 @@ struct LatDesc {
 	int dilution;
 	int inv_in, inv_out; // scalar slot read / written for 1/D
 	Cand *cand;        // per-thread top-2 of the last selection (null: not collected)
-	Cand *band;        // kBandCapacity candidates within `band_width` of the best
+	Cand *band;        // up to band_cap candidates within `band_width` of the best
 	double band_width; // error band of the objective (absolute)
+	unsigned *wgflag;  // workgroups of the final transform whose runner-ups reached the band
+	unsigned band_cap;
 };
 
 constexpr int kLikCapacity = kMaxVariants * (kMaxSubjects + 1);
@@ -131,9 +143,16 @@ unsigned launch_zeta_small(const Descs &descs, int count, int n, int k, Storage 
 // workgroups of launch_zeta_final / launch_zeta_small for a lattice of `atoms` (1024 threads
 // each; the candidate buffer needs 2 entries per thread)
 unsigned zeta_final_groups(int atoms);
-// candidates of the last zeta_final within d.band_width of the best: d.band[0..counter[0]),
-// counter[1] != 0 when a thread's runner-up was inside the band (the list may be incomplete)
-void launch_band_collect(const Descs &descs, int count, unsigned entries, hipStream_t stream);
+// Candidates of the last zeta_final within d.band_width of the best: d.band[0..counter[0]),
+// complete unless counter[0] exceeds d.band_cap (entries may repeat). Pass 2 of 21..30
+// atoms appends candidates while it scans and this keeps those inside the final band. For
+// 15..20 atoms the per-thread top-2 lists are gathered, and every workgroup with a thread
+// whose runner-up reached the band is transformed again to add all of its entries inside
+// the band. Lattices of at most kSmallBits atoms collect their band inside
+// launch_zeta_small and need no call here. The arguments after `entries` are those of the
+// launch_zeta_final call that produced the candidates, `entries` its return value.
+void launch_band_collect(const Descs &descs, int count, unsigned entries, int n, int k, Storage st, int objective,
+						 bool from_posterior, bool scaled, double target, hipStream_t stream);
 // copy the kHeadBytes head of each of `count` lattices (arena base = LatDesc::scal) into
 // out[i * kHeadBytes]
 void launch_gather_heads(const LatDesc *many, int count, unsigned char *out, hipStream_t stream);

@@ -57,9 +57,13 @@ struct Top2 {
 	uint32_t e1, e2;
 };
 
+// Objectives travel with a sign (the side of the target their mass lies on, see Cand) and
+// are ordered by magnitude, then by experiment: the lowest experiment wins ties, as in the
+// reference's ascending scans with strict comparisons.
 __device__ __forceinline__ bool better(double a, uint32_t ea, double b, uint32_t eb)
 {
-	return a < b || (a == b && ea < eb);
+	const double x = fabs(a), y = fabs(b);
+	return x < y || (x == y && ea < eb);
 }
 
 __device__ __forceinline__ void top2_init(Top2 &t)
@@ -313,6 +317,16 @@ __device__ __forceinline__ void zeta_tile(double (&v)[kTileRegs], int q0, int la
 
 // --------------------------------------------------------------- selection objective
 
+// Op-BHA: M - 1/2 (objective |M - 1/2|, bbpa.cpp:537-551). Binary BBPA:
+// |P0 - t| + |P1 - t| with P0 = M, P1 = T - M (bbpa.cpp:201-220), signed by P0 - P1.
+__device__ __forceinline__ double signed_objective(int objective, double m, double total, double target)
+{
+	if (objective == kObjHalving)
+		return m - 0.5;
+	const double negative = total - m;
+	return copysign(fabs(m - target) + fabs(negative - target), m - negative);
+}
+
 struct ObjCtx {
 	int objective;
 	uint32_t active;
@@ -326,15 +340,9 @@ __device__ __forceinline__ void consume(const ObjCtx &c, uint32_t g, double m, T
 {
 	if (__popc(g) == 1)
 		c.masses[__ffs(g) - 1] = m;
-	if (c.objective == kObjHalving) {
+	if (c.objective == kObjHalving || c.objective == kObjBinaryBbpa) {
 		if ((g & ~c.active) == 0)
-			top2_insert(t, fabs(m - 0.5), g);
-	} else if (c.objective == kObjBinaryBbpa) {
-		if ((g & ~c.active) == 0) {
-			const double positive = m;
-			const double negative = c.total - positive;
-			top2_insert(t, fabs(positive - c.target) + fabs(negative - c.target), g);
-		}
+			top2_insert(t, signed_objective(c.objective, m, c.total, c.target), g);
 	} else if (c.objective == kObjGather) {
 		// F_B(E) = M(spread_B(E)): g's non-zero variant blocks all equal E, B = those blocks
 		const uint32_t mask = (1u << c.ns) - 1u;
@@ -380,12 +388,73 @@ __device__ __forceinline__ void write_cand(const LatDesc &d, unsigned slot, cons
 	Cand a, b;
 	a.d = t.d1;
 	a.e = t.e1;
-	a.pad = 0;
+	a.flags = 0;
 	b.d = t.d2;
 	b.e = t.e2;
-	b.pad = 0;
+	b.flags = 0;
 	d.cand[2 * (size_t)slot] = a;
 	d.cand[2 * (size_t)slot + 1] = b;
+}
+
+// Append the lanes' candidates with `in` set to buf[0..cap), counting in *counter: one
+// atomic per wave. Once the buffer has overflowed only the count matters, so the atomic is
+// skipped (a posterior so concentrated that every objective sits at 1/2 would otherwise
+// serialise on one address). Every lane of the wave must call this.
+__device__ __forceinline__ void cand_append(Cand *buf, unsigned *counter, unsigned cap, bool in, double sd,
+											uint32_t e)
+{
+	const unsigned long long mask = __ballot(in);
+	if (mask == 0ull)
+		return;
+	const int lane = threadIdx.x & 63;
+	const int leader = __ffsll((long long)mask) - 1;
+	unsigned base = 0;
+	if (lane == leader) {
+		const unsigned seen = __hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		base = seen > cap ? seen : atomicAdd(counter, (unsigned)__popcll(mask));
+	}
+	base = __shfl(base, leader);
+	if (in && base <= cap) {
+		const unsigned idx = base + (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
+		if (idx < cap) {
+			Cand c;
+			c.d = sd;
+			c.e = e;
+			c.flags = kCandSided;
+			buf[idx] = c;
+		}
+	}
+}
+
+__device__ __forceinline__ void band_append(const LatDesc &d, bool in, double sd, uint32_t e)
+{
+	cand_append(d.band, &d.counter[0], d.band_cap, in, sd, e);
+}
+
+// Pass 2 of 21..30 atoms collects its band while it scans: counter[kRawCount] counts the
+// entries it appended to the `cand` buffer, and eight slots from counter[kBestSlots] hold
+// running bests of |objective|, one per XCD. A slot holds the high word of a non-negative
+// double plus one: the double that key decodes to (key << 32) is at least the value
+// recorded, and keys order like the values. Pass 1 resets both.
+constexpr int kRawCount = 2;
+constexpr int kBestSlots = 16;
+constexpr unsigned kKeyInf = 0x7FF00000u; // +inf
+__device__ __forceinline__ unsigned *best_slots(const LatDesc &d) { return d.counter + kBestSlots; }
+__device__ __forceinline__ unsigned best_key(double v)
+{
+	return (unsigned)((unsigned long long)__double_as_longlong(v) >> 32) + 1u;
+}
+__device__ __forceinline__ double key_value(unsigned key)
+{
+	return key >= kKeyInf ? INFINITY : __longlong_as_double((long long)((unsigned long long)key << 32));
+}
+
+// Rescan launches (collect mode) transform only the workgroups flagged by k_band_collect,
+// and nothing at all once the band has overflowed.
+__device__ __forceinline__ bool rescan_skip(const LatDesc &d, unsigned group)
+{
+	return d.band == nullptr || d.wgflag[group] == 0u ||
+		   __hip_atomic_load(&d.counter[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > d.band_cap;
 }
 
 // ----------------------------------------------------------------------------- kernels
@@ -450,6 +519,13 @@ __global__ __launch_bounds__(kTileThreads) void k_zeta_pass1(Descs descs)
 	const unsigned tiles = gridDim.x;
 	const unsigned tile = xcd_tile(blockIdx.x, tiles);
 	const uint64_t base = (uint64_t)tile << kTileBits;
+	if (blockIdx.x == 0 && d.band != nullptr) {
+		// the band collection of the pass 2 that follows
+		if (threadIdx.x < 8)
+			best_slots(d)[threadIdx.x] = kKeyInf;
+		else if (threadIdx.x == 8)
+			d.counter[kRawCount] = 0u;
+	}
 	// per-thread base pointers; the register index only adds compile-time offsets
 	gptr<TP> pa = G(static_cast<TP *>(d.post)) + base + lane + (wave << 10);
 	const double inv = d.scal[d.inv_in];
@@ -491,9 +567,11 @@ __global__ __launch_bounds__(kTileThreads) void k_zeta_pass1(Descs descs)
 // Tile index i maps to state g = tile*c + (i & (c-1)) + ((i >> logc) << 15), c = 2^logc.
 // FROM_POST reads x = y * inv from the posterior (single-pass lattices, A == 15);
 // otherwise it reads the pass-1 intermediate, scaled by inv_out when `scaled`.
+// With `collect` (a rescan after k_band_collect) only flagged tiles run, and every entry
+// within the band of the final best is appended to the band; nothing else is written.
 template <typename TIN, bool FROM_POST>
 __global__ __launch_bounds__(kTileThreads) void k_zeta_final(Descs descs, int logc, int q0,
-															 int objective, int scaled, double target)
+															 int objective, int scaled, double target, int collect)
 {
 	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
 	__shared__ double lds[kTileLds];
@@ -503,6 +581,8 @@ __global__ __launch_bounds__(kTileThreads) void k_zeta_final(Descs descs, int lo
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 	const unsigned tiles = gridDim.x;
 	const unsigned tile = xcd_tile(blockIdx.x, tiles);
+	if (collect && rescan_skip(d, tile))
+		return;
 	const uint32_t cmask = (1u << logc) - 1u;
 	const uint64_t cbase = (uint64_t)tile << logc;
 	gptr<const TIN> in = G(FROM_POST ? static_cast<const TIN *>(d.post) : static_cast<const TIN *>(d.z));
@@ -525,6 +605,19 @@ __global__ __launch_bounds__(kTileThreads) void k_zeta_final(Descs descs, int lo
 	c.ftab = d.ftab;
 	c.masses = d.masses;
 	zeta_tile(v, q0, lane, wave, lds);
+	if (collect) {
+		const double sc = scaled ? d.scal[d.inv_out] : 1.0;
+		const double total = d.scal[kScalTotal] * sc;
+		const double lim = fabs(d.result->d1) + d.band_width;
+#pragma unroll
+		for (int j = 0; j < kTileRegs; ++j) {
+			const uint32_t i = tile_index_f(lane, wave, j);
+			const uint32_t g = (uint32_t)(cbase + (i & cmask) + ((uint64_t)(i >> logc) << kTileBits));
+			const double sd = signed_objective(objective, scaled ? v[j] * sc : v[j], total, target);
+			band_append(d, (g & ~d.active) == 0u && fabs(sd) <= lim, sd, g);
+		}
+		return;
+	}
 	if (FROM_POST) {
 		// the whole lattice is this one tile: its total is the transform at index 0
 		if (threadIdx.x == 0) {
@@ -720,8 +813,9 @@ __device__ __forceinline__ uint32_t reg_offset_rt(int j)
 	return o;
 }
 
-// Per-thread top-2 of one tile in scan order: strict comparisons keep the earlier (lower)
-// state on equal objectives. BBPA: |p - t| + |(T - p) - t|; otherwise |p - 1/2|.
+// Per-thread top-2 of one tile in scan order: strict comparisons of magnitudes keep the
+// earlier (lower) state on equal objectives. Signed objectives as signed_objective():
+// BBPA |p - t| + |(T - p) - t| signed by p - (T - p); otherwise p - 1/2.
 template <int LC, bool BBPA, bool MASKED>
 __device__ __forceinline__ void scan_tile(const double (&v)[kTileRegs], double scale, double total, double target,
 										  uint32_t gt, uint32_t inactive, double &d1, double &d2, int &j1, int &j2)
@@ -735,16 +829,16 @@ __device__ __forceinline__ void scan_tile(const double (&v)[kTileRegs], double s
 		double dd;
 		if (BBPA) {
 			const double negative = total - p;
-			dd = fabs(p - target) + fabs(negative - target);
+			dd = copysign(fabs(p - target) + fabs(negative - target), p - negative);
 		} else {
-			dd = fabs(p - 0.5);
+			dd = p - 0.5;
 		}
 		if (MASKED) {
 			const uint32_t g = gt + reg_offset(mf, LC, j);
 			dd = (g & inactive) ? INFINITY : dd;
 		}
 		// one select per update keeps this branch-free
-		const bool c1 = dd < d1, c2 = dd < d2;
+		const bool c1 = fabs(dd) < fabs(d1), c2 = fabs(dd) < fabs(d2);
 		j2 = c2 ? j : j2;
 		d2 = c2 ? dd : d2;
 		j2 = c1 ? j1 : j2;
@@ -762,6 +856,11 @@ __device__ __forceinline__ void scan_tile(const double (&v)[kTileRegs], double s
 // (contiguous range, neighbours run on CUs sharing one L2). Objective: kObjHalving or
 // kObjBinaryBbpa; atom singletons are written by the tiles that hold them (tile 0 and
 // power-of-two tiles).
+// Band collection (lattices with exact rescoring, d.band set): every entry within
+// band_width of the best |objective| seen so far is appended to the `cand` buffer. The
+// best seen is a running minimum over this wave's tiles and its XCD's slot, never below the
+// final best, so every entry within band_width of the final best is appended;
+// k_band_compact keeps those after the merge.
 template <typename TIN, int LC>
 __global__ __launch_bounds__(kTileThreads) void k_zeta_pass2(Descs descs, int objective, int scaled, double target)
 {
@@ -790,6 +889,11 @@ __global__ __launch_bounds__(kTileThreads) void k_zeta_pass2(Descs descs, int ob
 	const uint32_t full = (d.n >= 32) ? 0xFFFFFFFFu : ((1u << d.n) - 1u);
 	const uint32_t inactive = full & ~d.active;
 	const bool bbpa = objective == kObjBinaryBbpa;
+	const bool banding = d.band != nullptr;
+	unsigned *slot = best_slots(d) + xcd;
+	const unsigned raw_cap = 2u * gridDim.x * blockDim.x;
+	double seen = INFINITY; // wave-uniform: the best |objective| this wave knows of
+	bool raw_full = false;  // wave-uniform: the candidate buffer has overflowed
 	Top2 run;
 	top2_init(run);
 	for (unsigned T = first; T < last; T += wg_per_xcd) {
@@ -861,12 +965,49 @@ __global__ __launch_bounds__(kTileThreads) void k_zeta_pass2(Descs descs, int ob
 			else
 				p2::scan_tile<LC, false, true>(v, scale, total, target, gt, inactive, d1, d2, j1, j2);
 		}
-		if (d1 < INFINITY)
+		if (fabs(d1) < INFINITY)
 			top2_insert(run, d1, gt + p2::reg_offset_rt<LC>(j1));
-		if (d2 < INFINITY)
+		if (fabs(d2) < INFINITY)
 			top2_insert(run, d2, gt + p2::reg_offset_rt<LC>(j2));
+		if (banding) {
+			double wm = fabs(d1);
+#pragma unroll
+			for (int o = 32; o > 0; o >>= 1)
+				wm = fmin(wm, __shfl_xor(wm, o));
+			seen = fmin(seen, key_value(__builtin_amdgcn_readfirstlane(
+								  __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))));
+			if (wm < seen) {
+				seen = wm;
+				if (lane == 0)
+					atomicMin(slot, best_key(wm));
+			}
+			const double thr = seen + d.band_width;
+			if (!raw_full && __ballot(fabs(d1) <= thr) != 0ull) {
+				constexpr p2::Map mf = p2::final_map(LC);
+#pragma unroll
+				for (int j = 0; j < kTileRegs; ++j) {
+					const uint32_t g = gt + p2::reg_offset(mf, LC, j);
+					const double sd = signed_objective(objective, v[j] * scale, total, target);
+					const bool in = (g & inactive) == 0u && fabs(sd) <= thr;
+					const unsigned long long mask = __ballot(in);
+					if (mask != 0ull && !raw_full) {
+						unsigned base = 0u;
+						if (lane == 0)
+							base = atomicAdd(&d.counter[kRawCount], (unsigned)__popcll(mask));
+						base = __shfl(base, 0);
+						raw_full = base + (unsigned)__popcll(mask) > raw_cap;
+						if (in && !raw_full) {
+							Cand c;
+							c.d = sd;
+							c.e = g;
+							c.flags = kCandSided;
+							d.cand[base + (unsigned)__popcll(mask & ((1ull << lane) - 1ull))] = c;
+						}
+					}
+				}
+			}
+		}
 	}
-	write_cand(d, blockIdx.x * blockDim.x + threadIdx.x, run);
 	publish_top2(d, blockIdx.x, run, red2);
 }
 
@@ -879,6 +1020,8 @@ __global__ __launch_bounds__(1024) void k_zeta_small(Descs descs, int atoms, int
 	__shared__ Top2 red2[16];
 	__shared__ double red[16];
 	__shared__ double s_total;
+	__shared__ double s_lim;
+	__shared__ unsigned s_count;
 	const uint32_t states = 1u << atoms;
 	gptr<const TP> post = G(static_cast<const TP *>(d.post));
 	const double inv = d.scal[d.inv_in];
@@ -916,15 +1059,37 @@ __global__ __launch_bounds__(1024) void k_zeta_small(Descs descs, int atoms, int
 	top2_init(tt);
 	for (uint32_t s = threadIdx.x; s < states; s += blockDim.x)
 		consume(c, s, a[s], tt);
-	write_cand(d, threadIdx.x, tt);
 	Top2 r = top2_block(tt, red2);
 	if (threadIdx.x == 0) {
-		d.result->d1 = r.d1;
-		d.result->d2 = r.d2;
+		d.result->d1 = fabs(r.d1);
+		d.result->d2 = fabs(r.d2);
 		d.result->e1 = r.e1;
 		d.result->e2 = r.e2;
-		d.counter[0] = 0; // band collection (runs next, in stream order)
+		d.counter[0] = 0;
 		d.counter[1] = 0;
+		s_lim = fabs(r.d1) + d.band_width;
+		s_count = 0u;
+	}
+	__syncthreads();
+	// the whole transform is in this workgroup's LDS: collect the complete band directly
+	if (d.band != nullptr && (objective == kObjHalving || objective == kObjBinaryBbpa)) {
+		const double lim = s_lim;
+		for (uint32_t s = threadIdx.x; s < states; s += blockDim.x) {
+			const double sd = signed_objective(objective, a[s], s_total, target);
+			if ((s & ~d.active) == 0u && fabs(sd) <= lim) {
+				const unsigned idx = atomicAdd(&s_count, 1u);
+				if (idx < d.band_cap) {
+					Cand cd;
+					cd.d = sd;
+					cd.e = s;
+					cd.flags = kCandSided;
+					d.band[idx] = cd;
+				}
+			}
+		}
+		__syncthreads();
+		if (threadIdx.x == 0)
+			d.counter[0] = s_count;
 	}
 }
 
@@ -1306,54 +1471,66 @@ __global__ __launch_bounds__(1024) void k_finalize_top2(Descs descs, unsigned co
 	}
 	r = top2_block(r, red2);
 	if (threadIdx.x == 0) {
-		d.result->d1 = r.d1;
-		d.result->d2 = r.d2;
+		d.result->d1 = fabs(r.d1);
+		d.result->d2 = fabs(r.d2);
 		d.result->e1 = r.e1;
 		d.result->e2 = r.e2;
 		d.counter[0] = 0; // band collection (runs next, in stream order)
 		d.counter[1] = 0;
 	}
+	// workgroup flags of the band rescan, one per workgroup of the launch just finished
+	if (d.wgflag != nullptr)
+		for (unsigned i = threadIdx.x; i < count && i < (unsigned)kMaxFinalGroups; i += blockDim.x)
+			d.wgflag[i] = 0u;
 }
 
-// Candidates within band_width of the best objective. Entry 2i+1 is thread i's runner-up;
-// counter[1] counts runner-ups inside the band (threads that may hold further candidates).
-// One atomic per wave: when every candidate is inside the band (a posterior so concentrated
-// that all objectives sit at 1/2) per-entry atomics on one address would serialise.
+// Candidates within band_width of the best objective from the per-thread top-2 lists
+// (entry 2i + 1 is thread i's runner-up). A thread whose runner-up is inside the band may
+// hold further candidates: its workgroup is flagged for the rescan that follows, and
+// counter[1] counts such runner-ups.
 __global__ __launch_bounds__(256) void k_band_collect(Descs descs, unsigned entries)
 {
 	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
 	const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
-	const int lane = threadIdx.x & 63;
 	Cand c;
 	c.d = INFINITY;
 	c.e = kNone;
-	c.pad = 0;
 	if (i < entries && d.cand != nullptr)
 		c = d.cand[i];
-	c.pad = i & 1u; // 1: a thread's runner-up (the thread may hold further candidates)
-	const double lim = d.result->d1 + d.band_width;
-	const bool in = c.e != kNone && c.d <= lim;
-	const unsigned long long mask = __ballot(in);
-	if (mask == 0ull)
+	const double lim = fabs(d.result->d1) + d.band_width;
+	const bool in = c.e != kNone && fabs(c.d) <= lim;
+	if (__ballot(in) == 0ull)
 		return;
-	const int leader = __ffsll((long long)mask) - 1;
-	unsigned base = 0;
-	if (lane == leader) {
-		// once the buffer has overflowed only the count matters: stop adding to it
-		const unsigned seen = __hip_atomic_load(&d.counter[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-		base = seen > (unsigned)kBandCapacity ? seen : atomicAdd(&d.counter[0], (unsigned)__popcll(mask));
-	}
-	base = __shfl(base, leader);
-	if (base > (unsigned)kBandCapacity)
-		return;
-	if (in) {
-		const unsigned idx = base + (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
-		if (idx < (unsigned)kBandCapacity)
-			d.band[idx] = c;
-	}
+	band_append(d, in, c.d, c.e);
+	if (in && (i & 1u))
+		d.wgflag[i / (2u * kTileThreads)] = 1u;
 	const unsigned long long ru = __ballot(in && (i & 1u));
-	if (ru != 0ull && lane == leader)
+	if (ru != 0ull && (threadIdx.x & 63) == __ffsll((long long)ru) - 1)
 		atomicAdd(&d.counter[1], (unsigned)__popcll(ru));
+}
+
+// The band of a pass 2 that collected while scanning: its appended entries (relative to
+// running bests) that lie within band_width of the final best. counter[0] was cleared by
+// k_finalize_top2; an overflow of the appended entries is reported as an overflowed band.
+__global__ __launch_bounds__(256) void k_band_compact(Descs descs, unsigned raw_cap)
+{
+	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
+	const unsigned raw = d.counter[kRawCount];
+	if (raw > raw_cap) {
+		if (blockIdx.x == 0 && threadIdx.x == 0)
+			d.counter[0] = d.band_cap + 1u;
+		return;
+	}
+	const double lim = d.result->d1 + d.band_width;
+	for (unsigned base = blockIdx.x * blockDim.x; base < raw; base += gridDim.x * blockDim.x) {
+		const unsigned i = base + threadIdx.x;
+		Cand c;
+		c.d = INFINITY;
+		c.e = kNone;
+		if (i < raw)
+			c = d.cand[i];
+		band_append(d, c.e != kNone && fabs(c.d) <= lim, c.d, c.e);
+	}
 }
 
 // Result heads of a batch of lattices into one staging buffer (one copy to the host).
@@ -1488,10 +1665,18 @@ unsigned zeta_final_groups(int atoms)
 	return tiles;
 }
 
-unsigned launch_zeta_final(const Descs &descs, int count, int n, int, Storage st, int objective,
-						   bool from_posterior, bool scaled, double target, hipStream_t stream)
+bool persistent_pass2(int n, int objective, bool from_posterior)
 {
-	if (!from_posterior && n >= 21 && n <= 30 && (objective == kObjHalving || objective == kObjBinaryBbpa)) {
+	return !from_posterior && n >= 21 && n <= 30 && (objective == kObjHalving || objective == kObjBinaryBbpa);
+}
+
+// The final transform kernel of a selection (pass 2, or the single pass of 15..20 atoms),
+// without the merge of the workgroups' top-2 lists; returns its workgroup count. `collect`
+// (single pass of 15..20 atoms only) rescans the workgroups flagged by k_band_collect.
+unsigned launch_final_kernel(const Descs &descs, int count, int n, Storage st, int objective, bool from_posterior,
+							 bool scaled, double target, int collect, hipStream_t stream)
+{
+	if (persistent_pass2(n, objective, from_posterior)) {
 		// persistent pass 2: one workgroup per CU (a multiple of 8, one share per XCD)
 		const dim3 grid(zeta_final_groups(n), count);
 		launch_checked("k_zeta_pass2", [&] {
@@ -1517,9 +1702,8 @@ unsigned launch_zeta_final(const Descs &descs, int count, int n, int, Storage st
 				BGT_P2(9)
 			}
 #undef BGT_P2
-			hipLaunchKernelGGL(k_finalize_top2, dim3(1, count), dim3(1024), 0, stream, descs, grid.x);
 		});
-		return 2u * grid.x * kTileThreads;
+		return grid.x;
 	}
 	// n == 15: one tile, contiguous, read from the posterior.
 	// n > 15: 2^(n-15) tiles of 2^(30-n) columns by 2^(n-15) rows, read from z.
@@ -1530,21 +1714,31 @@ unsigned launch_zeta_final(const Descs &descs, int count, int n, int, Storage st
 		if (from_posterior) {
 			if (st == Storage::fp64)
 				hipLaunchKernelGGL((k_zeta_final<double, true>), grid, dim3(kTileThreads), 0, stream, descs, logc, q0,
-								   objective, 0, target);
+								   objective, 0, target, collect);
 			else
 				hipLaunchKernelGGL((k_zeta_final<float, true>), grid, dim3(kTileThreads), 0, stream, descs, logc, q0,
-								   objective, 0, target);
+								   objective, 0, target, collect);
 		} else {
 			if (st == Storage::fp64)
 				hipLaunchKernelGGL((k_zeta_final<double, false>), grid, dim3(kTileThreads), 0, stream, descs, logc,
-								   q0, objective, scaled ? 1 : 0, target);
+								   q0, objective, scaled ? 1 : 0, target, collect);
 			else
 				hipLaunchKernelGGL((k_zeta_final<float, false>), grid, dim3(kTileThreads), 0, stream, descs, logc,
-								   q0, objective, scaled ? 1 : 0, target);
+								   q0, objective, scaled ? 1 : 0, target, collect);
 		}
-		hipLaunchKernelGGL(k_finalize_top2, dim3(1, count), dim3(1024), 0, stream, descs, grid.x);
 	});
-	return 2u * grid.x * kTileThreads;
+	return grid.x;
+}
+
+unsigned launch_zeta_final(const Descs &descs, int count, int n, int, Storage st, int objective,
+						   bool from_posterior, bool scaled, double target, hipStream_t stream)
+{
+	const unsigned groups = launch_final_kernel(descs, count, n, st, objective, from_posterior, scaled, target, 0,
+												stream);
+	launch_checked("k_finalize_top2", [&] {
+		hipLaunchKernelGGL(k_finalize_top2, dim3(1, count), dim3(1024), 0, stream, descs, groups);
+	});
+	return 2u * groups * kTileThreads;
 }
 
 unsigned launch_zeta_small(const Descs &descs, int count, int n, int, Storage st, int objective, double target,
@@ -1567,12 +1761,22 @@ void launch_gather_heads(const LatDesc *many, int count, unsigned char *out, hip
 	});
 }
 
-void launch_band_collect(const Descs &descs, int count, unsigned entries, hipStream_t stream)
+void launch_band_collect(const Descs &descs, int count, unsigned entries, int n, int, Storage st, int objective,
+						 bool from_posterior, bool scaled, double target, hipStream_t stream)
 {
+	if (persistent_pass2(n, objective, from_posterior)) {
+		// pass 2 appended its candidates while scanning: keep those inside the final band
+		launch_checked("k_band_compact", [&] {
+			hipLaunchKernelGGL(k_band_compact, dim3(64, count), dim3(256), 0, stream, descs, entries);
+		});
+		return;
+	}
 	const dim3 grid((entries + 255) / 256, count);
 	launch_checked("k_band_collect", [&] {
 		hipLaunchKernelGGL(k_band_collect, grid, dim3(256), 0, stream, descs, entries);
 	});
+	// rescan of the flagged workgroups (the others return at once)
+	launch_final_kernel(descs, count, n, st, objective, from_posterior, scaled, target, 1, stream);
 }
 
 

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <memory>
 
 namespace bgt_hip {
 
@@ -123,8 +124,8 @@ constexpr double kNegativeResponse = 0.99; // replicated_lattice.cpp:6
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 // Arena head: [0,256) scalars, [256,512) selection result, [512,768) counters,
-// [768,1024) atom masses, then the band candidates (kBandCapacity of them). One 2 KB copy
-// brings all of the head and the first kBandHead candidates to the host.
+// [768,1024) atom masses, then the band candidates (band_capacity(atoms) of them). One 2 KB
+// copy brings all of the head and the first kBandHead candidates to the host.
 constexpr size_t kHeadResult = 256;
 constexpr size_t kHeadCounter = 512;
 constexpr size_t kHeadMasses = 768;
@@ -138,7 +139,7 @@ struct HostBox {
 	double value;
 	double raw;
 	alignas(256) unsigned char head[kHeadBytes];
-	Cand band[kBandCapacity];
+	Cand band[kBandHead];
 };
 
 } // namespace
@@ -195,7 +196,7 @@ DeviceLattice::DeviceLattice(const DeviceLattice &parent, bool copy_posterior)
 namespace {
 
 struct ArenaLayout {
-	size_t scal, res, cnt, mass, band, cand, pairs, part, tops, ftab, scores, sel, total;
+	size_t scal, res, cnt, mass, band, cand, flags, pairs, part, tops, ftab, scores, sel, total;
 	bool exact;
 };
 
@@ -223,8 +224,9 @@ ArenaLayout arena_layout(bool dilution, int n, int k)
 	a.res = take(sizeof(SelResult));         // 256
 	a.cnt = take(sizeof(unsigned) * 4);      // 512
 	a.mass = take(sizeof(double) * 32);      // 768
-	a.band = take(a.exact ? sizeof(Cand) * kBandCapacity : 0); // 1024
+	a.band = take(a.exact ? sizeof(Cand) * band_capacity(atoms) : 0); // 1024
 	a.cand = take(a.exact ? sizeof(Cand) * 2 * kTileThreads * zeta_final_groups(atoms) : 0);
+	a.flags = take(a.exact ? sizeof(unsigned) * kMaxFinalGroups : 0);
 	a.pairs = take(sizeof(double) * 2 * kMaxAtoms);
 	a.part = take(sizeof(double) * part);
 	a.tops = take(sizeof(SelResult) * tops);
@@ -266,6 +268,10 @@ void DeviceLattice::allocate()
 	if (a.exact) {
 		band_ = reinterpret_cast<Cand *>(arena_ + a.band);
 		cand_ = reinterpret_cast<Cand *>(arena_ + a.cand);
+		wgflag_ = reinterpret_cast<unsigned *>(arena_ + a.flags);
+		band_cap_ = (unsigned)band_capacity(atoms_);
+		BGT_HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&band_host_), sizeof(Cand) * band_cap_,
+									hipHostMallocDefault));
 	}
 	double init[kScalars] = {1.0, 1.0, 1.0, 1.0, 0.0, 0.0, 0.0, 0.0};
 	BGT_HIP_CHECK(hipMemcpyAsync(scal_, init, sizeof(init), hipMemcpyHostToDevice, stream_));
@@ -295,6 +301,8 @@ DeviceLattice::~DeviceLattice()
 		(void)hipFree(arena_);
 	if (host_box_)
 		(void)hipHostFree(host_box_);
+	if (band_host_)
+		(void)hipHostFree(band_host_);
 	if (stream_)
 		(void)hipStreamDestroy(stream_);
 }
@@ -326,6 +334,8 @@ LatDesc DeviceLattice::desc() const
 	d.cand = cand_;
 	d.band = band_;
 	d.band_width = 0.0;
+	d.wgflag = wgflag_;
+	d.band_cap = band_cap_;
 	return d;
 }
 
@@ -444,6 +454,8 @@ void DeviceLattice::load_head(const unsigned char *head)
 	std::memcpy(box->counter, box->head + kHeadCounter, sizeof(box->counter));
 	std::memcpy(box->masses, box->head + kHeadMasses, sizeof(double) * 32);
 	std::memcpy(box->band, box->head + kHeadBand, sizeof(Cand) * kBandHead);
+	if (band_host_)
+		std::memcpy(band_host_, box->band, sizeof(Cand) * kBandHead);
 }
 
 SelResult DeviceLattice::fetch_result()
@@ -506,64 +518,159 @@ double DeviceLattice::band_width(int sel) const
 	return 2.0 * scale * eps;
 }
 
-void DeviceLattice::collect_band(const Descs &ds, unsigned entries)
+void DeviceLattice::collect_band(const Descs &ds, unsigned entries, int objective, bool from_posterior, bool scaled,
+								 double target)
 {
-	if (cand_ == nullptr)
+	// lattices of at most kSmallBits atoms collect their band inside the transform kernel
+	if (cand_ == nullptr || atoms_ <= kSmallBits)
 		return;
 	clock_.begin(stream_);
-	launch_band_collect(ds, 1, entries, stream_);
+	launch_band_collect(ds, 1, entries, atoms_, k_, storage_, objective, from_posterior, scaled, target, stream_);
 	clock_.end(stream_, "band_collect");
 }
 
+namespace {
+
+// Open-addressing set of band experiments with their side of the target (+1 above, -1
+// below, 0 not known for sure).
+class SideTable {
+public:
+	explicit SideTable(size_t n)
+	{
+		size_t cap = 16;
+		while (cap < 2 * n)
+			cap <<= 1;
+		key_.assign(cap, kEmpty);
+		side_.assign(cap, 0);
+	}
+	void put(uint32_t e, int side)
+	{
+		size_t h = slot(e);
+		while (key_[h] != kEmpty && key_[h] != e)
+			h = (h + 1) & (key_.size() - 1);
+		key_[h] = e;
+		side_[h] = (int8_t)side;
+	}
+	int side(uint32_t e) const
+	{
+		size_t h = slot(e);
+		while (key_[h] != kEmpty) {
+			if (key_[h] == e)
+				return side_[h];
+			h = (h + 1) & (key_.size() - 1);
+		}
+		return 0;
+	}
+
+private:
+	static constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+	size_t slot(uint32_t e) const { return (size_t)((e * 0x9E3779B1u) >> 7) & (key_.size() - 1); }
+	std::vector<uint32_t> key_;
+	std::vector<int8_t> side_;
+};
+
+} // namespace
+
 void DeviceLattice::rescore(bgt_hip_selection &out, int sel, const SelResult &r)
 {
-	// Exact FP64 objectives for every candidate within the error band: the device best's
-	// true objective is at most d1 + eps, so the true best has a device objective of at
-	// most d1 + 2 eps (the collected band). Candidates come from per-thread top-2 lists, so
-	// a thread whose runner-up is in the band may hold further candidates, all with device
-	// objectives at least that runner-up's: the re-decision is complete when every such
-	// runner-up is above the exact best plus eps, and the band did not overflow the buffer.
+	// Exact FP64 objectives for the candidates within the error band. The device best's
+	// true objective is at most d1 + eps, so the true best has a device objective of at most
+	// d1 + 2 eps: the band, which the device collects completely (the per-thread top-2 lists
+	// plus a rescan of every workgroup whose lists may have hidden more) unless it overflows
+	// its capacity.
+	// Dominance: M(E) is strictly decreasing under inclusion (every state has positive
+	// mass), and both objectives are increasing in |M - 1/2| (binary BBPA's is 2|M - 1/2|).
+	// A candidate whose mass is surely above 1/2 therefore loses to any one-subject extension
+	// whose mass is surely above 1/2, and one surely below 1/2 to any one-subject reduction
+	// surely below 1/2. Such candidates are not rescored; the rest are, in order of device
+	// objective, until the next one's lower bound exceeds the exact runner-up.
 	HostBox *box = static_cast<HostBox *>(host_box_);
 	const double eps = 0.5 * band_width(sel);
-	std::vector<Cand> cands;
 	const unsigned count = box->counter[0];
-	bool complete = cand_ != nullptr && count <= (unsigned)kBandCapacity;
+	bool complete = cand_ != nullptr && count <= band_cap_;
+	std::vector<Cand> cands;
 	if (complete) {
 		const unsigned inhead = std::min<unsigned>(count, kBandHead);
-		cands.assign(box->band, box->band + inhead);
 		if (count > inhead) {
-			BGT_HIP_CHECK(hipMemcpyAsync(box->band + inhead, band_ + inhead, sizeof(Cand) * (count - inhead),
+			BGT_HIP_CHECK(hipMemcpyAsync(band_host_ + inhead, band_ + inhead, sizeof(Cand) * (count - inhead),
 										 hipMemcpyDeviceToHost, stream_));
 			sync();
-			cands.insert(cands.end(), box->band + inhead, box->band + count);
 		}
+		cands.assign(band_host_, band_host_ + count);
 	}
-	for (const Cand c : {Cand{r.d1, r.e1, 0}, Cand{r.d2, r.e2, 0}}) {
-		if (c.e == 0xFFFFFFFFu)
-			continue;
-		bool seen = false;
-		for (const Cand &x : cands)
-			seen = seen || x.e == c.e;
-		if (!seen)
+	for (const Cand c : {Cand{r.d1, r.e1, 0}, Cand{r.d2, r.e2, 0}})
+		if (c.e != 0xFFFFFFFFu)
 			cands.push_back(c);
-	}
+	// one entry per experiment (a rescan repeats what the lists held), signed ones first
 	std::sort(cands.begin(), cands.end(),
-			  [](const Cand &a, const Cand &b) { return a.d < b.d || (a.d == b.d && a.e < b.e); });
+			  [](const Cand &a, const Cand &b) { return a.e < b.e || (a.e == b.e && a.flags > b.flags); });
+	cands.erase(std::unique(cands.begin(), cands.end(), [](const Cand &a, const Cand &b) { return a.e == b.e; }),
+				cands.end());
+	auto side_of = [&](const Cand &c) {
+		if (!(c.flags & kCandSided) || !(std::fabs(c.d) > eps))
+			return 0;
+		return c.d > 0.0 ? 1 : -1;
+	};
+	SideTable sides(cands.size());
+	for (const Cand &c : cands)
+		sides.put(c.e, side_of(c));
+	std::vector<Cand> live;
+	live.reserve(cands.size());
+	for (const Cand &c : cands) {
+		const int sd = side_of(c);
+		bool dominated = false;
+		if (sd > 0) {
+			for (uint32_t rest = active_ & ~c.e; rest && !dominated; rest &= rest - 1u)
+				dominated = sides.side(c.e | (rest & (0u - rest))) > 0;
+		} else if (sd < 0) {
+			for (uint32_t rest = c.e; rest && !dominated; rest &= rest - 1u)
+				dominated = sides.side(c.e & ~(rest & (0u - rest))) < 0;
+		}
+		if (!dominated)
+			live.push_back(c);
+	}
+	std::sort(live.begin(), live.end(), [](const Cand &a, const Cand &b) {
+		const double x = std::fabs(a.d), y = std::fabs(b.d);
+		return x < y || (x == y && a.e < b.e);
+	});
 	const double t = 1.0 / double(1 << k_);
 	double b1 = INFINITY, b2 = INFINITY;
 	uint32_t e1 = 0xFFFFFFFFu, e2 = 0xFFFFFFFFu;
 	int rescored = 0;
-	constexpr int kMaxRescored = 2048;
-	for (const Cand &c : cands) {
-		if (c.d - eps > b2)
+	constexpr int kMaxRescored = 4096;
+	// The scan below is sequential; the exact masses it needs are computed ahead in chunks
+	// (in parallel), so the result is that of the sequential scan.
+	std::vector<uint32_t> ex;
+	std::vector<double> mass;
+	std::unique_ptr<bool[]> okv;
+	size_t ready = 0, chunk = 64;
+	for (size_t i = 0; i < live.size(); ++i) {
+		const Cand &c = live[i];
+		if (std::fabs(c.d) - eps > b2)
 			break; // its exact objective exceeds the current runner-up, as do all later ones
 		if (rescored == kMaxRescored) {
 			complete = false;
 			break;
 		}
-		double m = 0.0;
-		if (!exact_.mass(c.e, m))
+		if (i == ready) {
+			// next chunk: candidates that may still be needed under the current runner-up
+			size_t j = i;
+			while (j < live.size() && j - i < chunk && !(std::fabs(live[j].d) - eps > b2))
+				++j;
+			ex.resize(j - i);
+			mass.resize(j - i);
+			okv.reset(new bool[j - i]);
+			for (size_t q = i; q < j; ++q)
+				ex[q - i] = live[q].e;
+			if (!exact_.masses(ex.data(), ex.size(), mass.data(), okv.get()))
+				return; // the normaliser is beyond the exact program's budget: keep the device result
+			ready = j;
+			chunk *= 2;
+		}
+		const size_t slot = i - (ready - ex.size());
+		if (!okv[slot])
 			return; // beyond the exact program's budget: keep the device result
+		const double m = mass[slot];
 		const double d = sel == kSelBha ? std::fabs(m - 0.5) : std::fabs(m - t) + std::fabs((1.0 - m) - t);
 		++rescored;
 		if (d < b1 || (d == b1 && c.e < e1)) {
@@ -578,9 +685,6 @@ void DeviceLattice::rescore(bgt_hip_selection &out, int sel, const SelResult &r)
 	}
 	if (e1 == 0xFFFFFFFFu)
 		return;
-	for (const Cand &c : cands)
-		if (c.pad == 1 && !(c.d - eps > b1))
-			complete = false; // that thread's hidden candidates could beat e1
 	out.experiment = e1;
 	out.objective = b1;
 	out.runner_up = e2;
@@ -623,6 +727,8 @@ bgt_hip_selection DeviceLattice::select(int selector)
 		clock_.begin(stream_);
 		entries = launch_zeta_final(dz, 1, atoms_, k_, storage_, objective, true, false, target, stream_);
 		clock_.end(stream_, "zeta_final");
+		if (objective != kObjGather)
+			collect_band(dz, entries, objective, true, false, target);
 	} else {
 		clock_.begin(stream_);
 		launch_zeta_pass1(dz, 1, atoms_, storage_, false, k_, dilution_, stream_);
@@ -630,14 +736,15 @@ bgt_hip_selection DeviceLattice::select(int selector)
 		clock_.begin(stream_);
 		entries = launch_zeta_final(dz, 1, atoms_, k_, storage_, objective, false, false, target, stream_);
 		clock_.end(stream_, "zeta_pass2");
+		if (objective != kObjGather)
+			collect_band(dz, entries, objective, false, false, target);
 	}
 	if (objective == kObjGather) {
 		clock_.begin(stream_);
 		launch_bbpa_score(dz, 1, n_, k_, target, stream_);
 		clock_.end(stream_, "bbpa_score");
-	} else {
-		collect_band(dz, entries);
 	}
+	exact_.prepare(); // on the host while the device selects
 	fetch_head();
 	return finish_selection(selector, static_cast<HostBox *>(host_box_)->result, false);
 }
@@ -786,7 +893,8 @@ bgt_hip_selection DeviceLattice::round(uint32_t experiment, uint32_t response, c
 	clock_.begin(stream_);
 	const unsigned entries = launch_zeta_final(ds, 1, atoms_, k_, storage_, kObjHalving, false, true, 0.5, stream_);
 	clock_.end(stream_, "round_pass2");
-	collect_band(ds, entries);
+	collect_band(ds, entries, kObjHalving, false, true, 0.5);
+	exact_.prepare(); // on the host while the device runs the round
 	fetch_head();
 	classify_from_head(up, lo, masses_out);
 	return finish_selection(kSelBha, static_cast<HostBox *>(host_box_)->result, true);

@@ -110,13 +110,18 @@ public:
 	void classify_from_head(double threshold_up, double threshold_lo, double *masses_out);
 	// selection result of the last head
 	SelResult head_result() const;
+	// host-side preparation of exact rescoring, to overlap with device work
+	void prepare_exact() { exact_.prepare(); }
 
 private:
 	Descs single() const;
 	void allocate();
 	void fill_lik(uint32_t experiment, uint32_t response, const double *dilution_flat);
 	void rescore(bgt_hip_selection &out, int selector, const SelResult &r);
-	void collect_band(const Descs &ds, unsigned entries);
+	// complete band of candidates for exact rescoring (binary non-dilution lattices); the
+	// arguments after `entries` are those of the final transform launch
+	void collect_band(const Descs &ds, unsigned entries, int objective, bool from_posterior, bool scaled,
+					  double target);
 
 	bool dilution_;
 	int n_, k_, atoms_;
@@ -140,6 +145,9 @@ private:
 	uint32_t *selected_ = nullptr;
 	Cand *cand_ = nullptr; // per-thread top-2 of the last transform (exact rescoring only)
 	Cand *band_ = nullptr; // candidates within the error band
+	unsigned *wgflag_ = nullptr; // workgroups of the final transform to rescan
+	unsigned band_cap_ = 0;
+	Cand *band_host_ = nullptr; // pinned host copy of the band
 	ExactHistory exact_;
 
 	// pinned host mailbox for results
