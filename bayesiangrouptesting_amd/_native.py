@@ -46,6 +46,7 @@ class SelectionC(ctypes.Structure):
         ("redecided", ctypes.c_int),
         ("candidates", ctypes.c_int),
         ("band", ctypes.c_int),
+        ("collected", ctypes.c_int),
     ]
 
 

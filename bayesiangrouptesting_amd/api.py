@@ -114,13 +114,14 @@ class Selection:
     redecided: int  # 0 no, 1 exact over the whole error band, 2 exact over the top two only
     candidates: int = 0
     band: int = 0
+    collected: int = 0
 
     @classmethod
     def from_c(cls, s: nat.SelectionC, flip=None) -> "Selection":
         f = flip or (lambda v: v)
         return cls(f(s.experiment), f(s.runner_up) if s.runner_up != 0xFFFFFFFF else s.runner_up,
                    s.objective, s.runner_up_objective, s.gap_bound, bool(s.near_tie), int(s.redecided),
-                   int(s.candidates), int(s.band))
+                   int(s.candidates), int(s.band), int(s.collected))
 
 
 class Lattice:

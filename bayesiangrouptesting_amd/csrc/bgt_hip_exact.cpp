@@ -237,43 +237,94 @@ bool ExactHistory::unnormalised(uint32_t e, double &out, Scratch &sc) const
 	uint32_t relevant = 0;
 	for (uint32_t p : sc.pool)
 		relevant |= p;
+	// State: bit i set while pool i can still be contained. Atoms are processed in order_;
+	// once a pool's last atom has been processed its factor is applied and its bit cleared,
+	// so only the pools that are partly processed tell states apart.
 	Table *cur = &sc.a, *nxt = &sc.b;
 	cur->clear(1, (int)c);
 	cur->add(c == 64 ? ~0ull : ((1ull << c) - 1ull), 1.0);
-	for (int a = 0; a < atoms_; ++a) {
+	uint32_t done = 0;
+	for (int a : order_) {
 		if (!((relevant >> a) & 1u))
 			continue;
-		uint64_t kill = 0;
-		for (size_t i = 0; i < c; ++i)
-			if ((sc.pool[i] >> a) & 1u)
+		done |= 1u << a;
+		uint64_t kill = 0, close = 0;
+		for (size_t i = 0; i < c; ++i) {
+			if ((sc.pool[i] >> a) & 1u) {
 				kill |= 1ull << i;
+				if ((sc.pool[i] & ~done) == 0)
+					close |= 1ull << i;
+			}
+		}
 		nxt->clear(2 * cur->key.size(), (int)c);
 		for (size_t s = 0; s < cur->key.size(); ++s) {
 			const uint64_t k = cur->key[s];
 			const double wv = cur->w[s];
-			nxt->add(k, wv * q_[a]);          // atom negative: pools unchanged
-			nxt->add(k & ~kill, wv * pi_[a]); // atom positive: its pools are out
+			// atom negative: pools unchanged; atom positive: its pools are out
+			uint64_t k0 = k, k1 = k & ~kill;
+			double w0 = wv * q_[a], w1 = wv * pi_[a];
+			for (uint64_t rest = close; rest; rest &= rest - 1) {
+				const int i = __builtin_ctzll(rest);
+				w0 *= ((k0 >> i) & 1u) ? sc.fin[i] : sc.fout[i];
+				w1 *= ((k1 >> i) & 1u) ? sc.fin[i] : sc.fout[i];
+			}
+			nxt->add(k0 & ~close, w0);
+			nxt->add(k1 & ~close, w1);
 		}
 		if (nxt->key.size() > kMaxStates)
 			return false;
 		std::swap(cur, nxt);
 	}
+	// every pool has closed: one state remains
 	double sum = 0.0;
-	for (size_t s = 0; s < cur->key.size(); ++s) {
-		const uint64_t k = cur->key[s];
-		double f = 1.0;
-		for (size_t i = 0; i < c; ++i)
-			f *= ((k >> i) & 1u) ? sc.fin[i] : sc.fout[i];
-		sum += cur->w[s] * f;
-	}
+	for (size_t s = 0; s < cur->key.size(); ++s)
+		sum += cur->w[s];
 	out = qe * always * sum;
 	return true;
+}
+
+void ExactHistory::plan_order()
+{
+	// Atom order of the program: greedily the atom after which the fewest pools are partly
+	// processed (lowest index on ties), for the pools of the whole history; irrelevant
+	// atoms last. Fewer partly processed pools mean fewer distinct states.
+	std::vector<uint32_t> pools;
+	uint32_t relevant = 0;
+	for (const Upd &u : hist_) {
+		if (u.e == 0 || std::find(pools.begin(), pools.end(), u.e) != pools.end())
+			continue;
+		pools.push_back(u.e);
+		relevant |= u.e;
+	}
+	order_.clear();
+	uint32_t done = 0;
+	while (relevant & ~done) {
+		int best = -1, best_open = 1 << 30;
+		for (int a = 0; a < atoms_; ++a) {
+			if (!((relevant & ~done) >> a & 1u))
+				continue;
+			const uint32_t after = done | (1u << a);
+			int open = 0;
+			for (uint32_t p : pools)
+				open += (p & after) != 0 && (p & ~after) != 0;
+			if (open < best_open) {
+				best_open = open;
+				best = a;
+			}
+		}
+		order_.push_back(best);
+		done |= 1u << best;
+	}
+	for (int a = 0; a < atoms_; ++a)
+		if (!((done >> a) & 1u))
+			order_.push_back(a);
 }
 
 bool ExactHistory::normaliser()
 {
 	if (z_ok_)
 		return true;
+	plan_order();
 	if (!unnormalised(0u, z_, scratch_)) {
 		broken_ = true;
 		return false;

@@ -69,6 +69,7 @@ private:
 		Table a, b;
 	};
 	bool unnormalised(uint32_t e, double &out, Scratch &sc) const;
+	void plan_order(); // atom order of the program for the current history (normaliser())
 	std::vector<Scratch> workers_; // one per pool thread (masses())
 	bool normaliser();
 
@@ -76,6 +77,7 @@ private:
 	int atoms_ = 0;
 	std::vector<double> pi_, q_;
 	std::vector<Upd> hist_;
+	std::vector<int> order_; // atom order of the program
 	double z_ = 1.0;   // U({}) for the current history
 	bool z_ok_ = true; // z_ is up to date
 	Scratch scratch_;

@@ -488,6 +488,8 @@ bgt_hip_selection DeviceLattice::finish_selection(int selector, const SelResult 
 	out.runner_up_objective = r.d2;
 	out.gap_bound = band_width(sel);
 	out.near_tie = (r.d2 - r.d1) <= out.gap_bound ? 1 : 0;
+	if (cand_ != nullptr)
+		out.collected = (int)static_cast<const HostBox *>(host_box_)->counter[2];
 	if (out.near_tie && found && (sel == kSelBha || sel == kSelBbpa) && exact_.usable())
 		rescore(out, sel, r);
 	return out;
@@ -628,6 +630,21 @@ void DeviceLattice::rescore(bgt_hip_selection &out, int sel, const SelResult &r)
 		}
 		if (!dominated)
 			live.push_back(c);
+	}
+	if (complete && live.size() == 1) {
+		// every other candidate loses to one that beats it strictly: the survivor is the
+		// exact best, decided without its exact mass (objective: the device's value)
+		out.experiment = live[0].e;
+		out.objective = std::fabs(live[0].d);
+		if (out.runner_up == out.experiment) {
+			out.runner_up = r.e1 == out.experiment ? r.e2 : r.e1;
+			out.runner_up_objective = r.e1 == out.experiment ? r.d2 : r.d1;
+		}
+		out.near_tie = 0;
+		out.redecided = 1;
+		out.candidates = 0;
+		out.band = (int)count;
+		return;
 	}
 	std::sort(live.begin(), live.end(), [](const Cand &a, const Cand &b) {
 		const double x = std::fabs(a.d), y = std::fabs(b.d);

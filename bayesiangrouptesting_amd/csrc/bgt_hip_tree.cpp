@@ -9,6 +9,8 @@
 // and index order is preserved by Tree::true_ex, so ties resolve the same way.
 #include "bgt_hip_tree.hpp"
 
+#include "bgt_hip_batch.hpp"
+
 #include <hip/hip_runtime_api.h>
 
 #include <cstring>
@@ -46,36 +48,84 @@ uint32_t active_subjects(uint32_t classified, int n, int k)
 	return active;
 }
 
-std::unique_ptr<HostNode> build(const TreeCtx &c, std::unique_ptr<DeviceLattice> lat, uint32_t ex, uint32_t res,
-								int stage, TreeBuildStats &bs)
+bool is_classified(uint32_t pos, uint32_t neg, int atoms) { return __builtin_popcount(pos | neg) == atoms; }
+
+// LocalTree construction (global_tree_intra.cpp:8-40), one level at a time: every
+// unfinished node of a level selects in one batched step, and all of their children are
+// updated and classified in one more. A node becomes a leaf when it is classified or at the
+// depth limit; a leaf's posterior is freed at once.
+std::unique_ptr<HostNode> build_levels(const TreeCtx &c, std::unique_ptr<DeviceLattice> root_lat, TreeBuildStats &bs)
 {
-	auto node = std::make_unique<HostNode>();
-	node->ex = ex;
-	node->res = res;
-	node->stage = stage;
-	node->pos = lat->positive_mask();
-	node->neg = lat->negative_mask();
+	struct Open {
+		HostNode *node;
+		std::unique_ptr<DeviceLattice> lat;
+	};
 	const int atoms = c.n * c.k;
-	const bool classified = __builtin_popcount(node->pos | node->neg) == atoms;
-	if (classified || stage >= c.depth)
-		return node;
-	lat->set_active(active_subjects(node->pos | node->neg, c.n, c.k));
-	const bgt_hip_selection sel = lat->select(c.selector);
-	bs.selections += 1;
-	bs.near_ties += sel.near_tie;
-	const uint32_t e = sel.experiment;
 	const int rc = 1 << c.k;
-	for (int r = 0; r < rc; ++r) {
-		std::unique_ptr<DeviceLattice> child;
-		if (r != rc - 1)
-			child = std::make_unique<DeviceLattice>(*lat, true);
-		else
-			child = std::move(lat);
-		child->update(e, (uint32_t)r, c.dil);
-		child->classify(c.up, c.lo);
-		node->children.push_back(build(c, std::move(child), e, (uint32_t)r, stage + 1, bs));
+	auto root = std::make_unique<HostNode>();
+	root->pos = root_lat->positive_mask();
+	root->neg = root_lat->negative_mask();
+	std::vector<Open> frontier;
+	if (!is_classified(root->pos, root->neg, atoms) && c.depth > 0)
+		frontier.push_back({root.get(), std::move(root_lat)});
+	std::vector<BatchItem> items;
+	std::vector<BatchResult> res;
+	while (!frontier.empty()) {
+		// select on every open node of this level
+		items.assign(frontier.size(), BatchItem{});
+		for (size_t i = 0; i < frontier.size(); ++i) {
+			const HostNode &nd = *frontier[i].node;
+			frontier[i].lat->set_active(active_subjects(nd.pos | nd.neg, c.n, c.k));
+			items[i].lat = frontier[i].lat.get();
+			items[i].classify = false;
+		}
+		batch_step(items, c.selector, c.up, c.lo, res);
+		// children: copies of the parent for all responses but the last, which takes the
+		// parent's own lattice (global_tree_intra.cpp:24-36)
+		std::vector<Open> kids;
+		std::vector<uint32_t> kid_e;
+		kids.reserve(frontier.size() * rc);
+		for (size_t i = 0; i < frontier.size(); ++i) {
+			const uint32_t e = res[i].selection.experiment;
+			bs.selections += 1;
+			bs.near_ties += res[i].selection.near_tie;
+			HostNode &nd = *frontier[i].node;
+			for (int r = 0; r < rc; ++r) {
+				auto child = std::make_unique<HostNode>();
+				child->ex = e;
+				child->res = (uint32_t)r;
+				child->stage = nd.stage + 1;
+				HostNode *cp = child.get();
+				nd.children.push_back(std::move(child));
+				std::unique_ptr<DeviceLattice> lat = r != rc - 1
+					? std::make_unique<DeviceLattice>(*frontier[i].lat, true)
+					: std::move(frontier[i].lat);
+				kids.push_back({cp, std::move(lat)});
+				kid_e.push_back(e);
+			}
+		}
+		// update and classify every child
+		items.assign(kids.size(), BatchItem{});
+		for (size_t i = 0; i < kids.size(); ++i) {
+			items[i].lat = kids[i].lat.get();
+			items[i].update = true;
+			items[i].experiment = kid_e[i];
+			items[i].response = kids[i].node->res;
+			items[i].dilution = c.dil;
+			items[i].select = false;
+		}
+		batch_step(items, c.selector, c.up, c.lo, res);
+		frontier.clear();
+		for (size_t i = 0; i < kids.size(); ++i) {
+			HostNode &nd = *kids[i].node;
+			nd.pos = res[i].positive;
+			nd.neg = res[i].negative;
+			if (is_classified(nd.pos, nd.neg, atoms) || nd.stage >= c.depth)
+				continue; // a leaf: its lattice is released here
+			frontier.push_back(std::move(kids[i]));
+		}
 	}
-	return node;
+	return root;
 }
 
 int count_leaves(const HostNode &n)
@@ -126,7 +176,7 @@ bgt_hip_tree_stats run_tree(bool dilution, int n, int k, const double *prior, in
 	auto root_lat = std::make_unique<DeviceLattice>(dilution, n, k, prior, storage);
 	hipStream_t s = nullptr;
 	TreeBuildStats bs;
-	auto root = build(c, std::move(root_lat), 0, 0, 0, bs);
+	auto root = build_levels(c, std::move(root_lat), bs);
 	std::vector<FlatNode> nodes;
 	flatten(*root, nodes);
 	bgt_hip_tree_stats stats;

@@ -1,19 +1,27 @@
 """Benchmark: Op-BHA select + update + atom marginals at n = 28 on one MI355X.
 
-A step is one round of the adaptive loop on the 2^28-state binary non-dilution lattice
-(priors U[0.005, 0.05], seed 2): apply the response to the current experiment, take the atom
-marginals of the updated posterior and classify, and select the next experiment by Op-BHA --
-the fused two-pass round of the HIP provider (bgt_hip_lattice_round). The headline value is
-rounds/s of the compensated FP32 form. The plain FP64 form runs first and drives the
-trajectory (its picks and the responses sampled for them); the FP32C form replays the same
-(experiment, response) sequence, so the two forms' selections and final posteriors are
-compared round by round.
+Headline workload (one step = one round of the adaptive loop on the 2^28-state binary
+non-dilution lattice, priors U[0.005, 0.05] from seed 2): apply the response to the current
+experiment, take the atom marginals of the updated posterior and classify, and select the
+next experiment by Op-BHA -- the fused two-pass round of the HIP provider
+(bgt_hip_lattice_round). Rounds come from seeded adaptive sessions run back to back: session
+i draws its true state from the prior with seed 2000 + i and stops, as the reference's
+adaptive loop does, once every subject is classified or after 20 tests. Each session starts
+from a lattice holding the prior and the prior's Op-BHA selection (the same for every
+session), prepared before the timed region; the timed region is exactly `steps` rounds. The
+headline value is rounds/s of the compensated FP32 form. The plain FP64 form runs first and
+drives the trajectory (its picks and the responses sampled for them); the FP32C form replays
+the same (experiment, response) sequence, so the two forms' selections are compared round by
+round.
 
-Roofline: the dominant kernel is pass 1 (update fused with bits 0..14 of the transform):
-it reads the posterior and writes it and the transform intermediate, S*(2 b_p + b_z) bytes
-per launch (b = 4 for FP32C, 8 for FP64), timed with HIP events on the lattice's stream
-over the timed rounds. A round moves S*(2 b_p + 2 b_z) bytes (the intermediate is reloaded
-once by pass 2).
+Roofline: the dominant kernel is pass 1 (update fused with bits 0..14 of the transform): it
+reads the posterior and writes it and the transform intermediate, S*(2 b_p + b_z) bytes per
+launch (b = 4 for FP32C, 8 for FP64), timed with HIP events on the lattice's stream over the
+timed rounds. A round moves S*(2 b_p + 2 b_z) bytes (pass 2 reloads the intermediate once).
+
+Also reported, each for both storage forms: the batched tree simulation (n = 16 dilution,
+depth 6) to TreeStats, Op-BBPA rounds of the n = 8, k = 3 dilution lattice, and the serving
+loop over a mix of adaptive sessions with Poisson arrivals (p50/p99 latency, sessions/s).
 
 CPU baseline: the C restatement of the reference CPU provider (oracle/, built with the
 reference's default flags) runs one full n = 28 round on one host thread.
@@ -35,6 +43,12 @@ sys.path.insert(0, ROOT)
 METRIC = "Op-BHA select+update/s @n=28 (HBM GB/s %peak); served p50/p99 ms; FP32c vs FP64"
 PEAK_HBM_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_round_n28.json")
+MAX_TESTS = 20
+THRESHOLD = 0.01
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def kernel_times(lat) -> dict:
@@ -55,42 +69,109 @@ def enable_clock(lat, on: bool) -> None:
     nat.check(nat.lib().bgt_hip_lattice_kernel_times(lat.handle, 1 if on else 0, None, None, 0, None, 0))
 
 
-def run_form(precision, n, pi, truth_pub, steps, warmup, seed, drive=None):
-    """Rounds of one storage form. Without `drive` the responses are sampled for the form's
-    own picks; with it, the given (experiment, response) sequence is replayed. Returns the
-    lattice, the timed seconds, per-kernel times, the selections (initial one first) and the
-    per-round host latencies (ms) of the timed rounds."""
-    import bayesiangrouptesting_amd as bgt
+def device_sync() -> None:
     from bayesiangrouptesting_amd import _native as nat
+    nat.check(nat.lib().bgt_hip_device_synchronize())
+
+
+class Timer:
+    """Wall time of device work in segments, each bracketed by a device synchronisation."""
+
+    def __init__(self):
+        self.total = 0.0
+        self._t0 = None
+
+    @property
+    def running(self) -> bool:
+        return self._t0 is not None
+
+    def start(self) -> None:
+        device_sync()
+        self._t0 = time.perf_counter()
+
+    def stop(self) -> None:
+        device_sync()
+        self.total += time.perf_counter() - self._t0
+        self._t0 = None
+
+
+def run_sessions(precision, n, pi, steps, warmup, seed, drive=None):
+    """`warmup` + `steps` rounds of seeded adaptive sessions. Without `drive` each session
+    samples responses for its own picks and ends when classified or after MAX_TESTS tests;
+    with it, the given (session, experiment, response) rounds are replayed. Returns the
+    timed seconds, per-kernel times, the plan that was run, the selection after every round,
+    per-round host latencies (ms) of the timed rounds, the classification of every round and
+    the last session's lattice."""
+    import bayesiangrouptesting_amd as bgt
     from bayesiangrouptesting_amd import workload
 
-    lat = bgt.Lattice(bgt.LatticeType.replicated_non_dilution, n, pi, precision=precision)
-    sampler = workload.ResponseSampler(seed, n, 1, truth_pub)
-    sel = lat.select(bgt.SelectorType.op_bha)
-    picks = [sel]
-    e = sel.experiment
-    lat_ms = []
-    t0 = time.perf_counter()
-    for i in range(warmup + steps):
+    total = warmup + steps
+    timer = Timer()
+    done, clocked = [], []
+    plan, picks, lat_ms, classified = [], [], [], []
+    lat = None
+    sess = -1
+    e = 0
+    sampler = None
+    tests = 0
+    for i in range(total):
+        if lat is None:
+            # a new session: its lattice is prepared outside the timed region
+            was = timer.running
+            if was:
+                timer.stop()
+            sess += 1
+            lat = bgt.Lattice(bgt.LatticeType.replicated_non_dilution, n, pi, precision=precision)
+            first = lat.select(bgt.SelectorType.op_bha)
+            e = first.experiment
+            s_seed = seed * 1000 + sess
+            sampler = workload.ResponseSampler(s_seed, n, 1, bgt.reverse_bits(workload.true_state(s_seed, pi), n))
+            tests = 0
+            if i >= warmup:
+                enable_clock(lat, True)
+                clocked.append(lat)
+            if was:
+                timer.start()
         if i == warmup:
-            nat.check(nat.lib().bgt_hip_device_synchronize())
-            enable_clock(lat, True)
-            t0 = time.perf_counter()
+            if lat not in clocked:
+                enable_clock(lat, True)
+                clocked.append(lat)
+            timer.start()
         if drive is not None:
-            e, r = drive[i]
+            s_drive, e, r = drive[i]
+            assert s_drive == sess, "replay out of step with the session plan"
         else:
             r = sampler.draw(e)
+        plan.append((sess, e, r))
         t1 = time.perf_counter()
-        sel = lat.round(e, r, 0.01, 0.01)
-        lat_ms.append((time.perf_counter() - t1) * 1e3)
+        sel = lat.round(e, r, THRESHOLD, THRESHOLD)
+        if i >= warmup:
+            lat_ms.append((time.perf_counter() - t1) * 1e3)
         picks.append(sel)
-        if drive is None:
+        tests += 1
+        classified.append(lat.is_classified())
+        if drive is not None:
+            end = i + 1 < total and drive[i + 1][0] != sess
+        else:
+            end = classified[-1] or tests >= MAX_TESTS
+        if end and i + 1 < total:
+            done.append(lat)
+            lat = None
+        else:
             e = sel.experiment
-    nat.check(nat.lib().bgt_hip_device_synchronize())
-    dt = time.perf_counter() - t0
-    kt = kernel_times(lat)
-    enable_clock(lat, False)
-    return lat, dt, kt, picks, lat_ms[warmup:]
+    timer.stop()
+    # per-kernel averages over the timed rounds of every clocked lattice
+    agg: dict = {}
+    for l in clocked:
+        for name, (ms, cnt) in kernel_times(l).items():
+            t, c = agg.get(name, (0.0, 0))
+            agg[name] = (t + ms * cnt, c + cnt)
+        enable_clock(l, False)
+    kt = {k: (t / c if c else 0.0, c) for k, (t, c) in agg.items()}
+    for l in done:
+        l.close()
+    return {"seconds": timer.total, "kernels": kt, "plan": plan, "picks": picks, "lat_ms": lat_ms,
+            "classified": classified, "sessions": sess + 1, "lattice": lat}
 
 
 def cpu_baseline(n, pi, e_int, r):
@@ -135,12 +216,112 @@ def load_traffic():
         return None
 
 
+# ----------------------------------------------------------------- other workloads
+
+def tree_workload(repeats: int = 3) -> dict:
+    """Batched tree simulation: binary dilution n = 16, DilutionTable(16, 0.05, 0.9), priors
+    U[0.01, 0.10], Op-BHA to depth 6 -> TreeStats; best of `repeats` runs per form."""
+    import bayesiangrouptesting_amd as bgt
+    from bayesiangrouptesting_amd import workload
+    from oracle import pyoracle as orc
+
+    c = workload.C4
+    pi = workload.priors(c.seed, c.subjects, c.prior_lo, c.prior_hi)
+    out = {"workload": "binary dilution n=16 k=1, DilutionTable(16,0.05,0.9), priors U[0.01,0.10] seed 4, "
+                       "Op-BHA, depth 6, one batched launch per kernel per tree level"}
+    stats = {}
+    for prec in (bgt.Precision.float64, bgt.Precision.float32_compensated):
+        cfg = bgt.SimulationConfig(lattice_type=bgt.LatticeType.replicated_dilution, subjects=c.subjects,
+                                   variants=1, prior=pi)
+        cfg.options.search_depth = 6
+        cfg.options.precision = prec
+        cfg.dilution = bgt.DilutionParameters(c.alpha, c.h)
+        best = float("inf")
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            st = bgt.run_simulation(cfg).stats
+            best = min(best, time.perf_counter() - t0)
+        stats[prec] = st
+        out["fp64" if prec == bgt.Precision.float64 else "fp32c"] = {"ms": best * 1e3, "leaves": st.total_leaves}
+    t0 = time.perf_counter()
+    ref = orc.run_simulation(True, c.subjects, 1, pi, 6, alpha=c.alpha, h=c.h, variant="fast")
+    out["cpu_port_ms"] = (time.perf_counter() - t0) * 1e3
+    fields = ["correct_probability", "incorrect_probability", "false_positive_probability",
+              "false_negative_probability", "unclassified_probability", "expected_stages", "expected_tests"]
+    for prec, key in ((bgt.Precision.float64, "fp64"), (bgt.Precision.float32_compensated, "fp32c")):
+        st = stats[prec]
+        out[key]["max_abs_diff_vs_cpu"] = max(abs(getattr(st, f) - getattr(ref, f)) for f in fields)
+        out[key]["leaves_equal_cpu"] = st.total_leaves == ref.total_leaves
+    return out
+
+
+def bbpa_workload(rounds: int = 8) -> dict:
+    """Multinomial dilution lattice n = 8, k = 3 (2^24 states), DilutionTable(8, 0.05, 0.9),
+    priors U[0.01, 0.10]: `rounds` rounds of Op-BBPA select -> update with seeded responses."""
+    import bayesiangrouptesting_amd as bgt
+    from bayesiangrouptesting_amd import workload
+
+    c = workload.C3
+    pi = workload.priors(c.seed, c.subjects * c.variants, c.prior_lo, c.prior_hi)
+    tab = bgt.DilutionTable(c.subjects, c.alpha, c.h)
+    out = {"workload": "multinomial dilution n=8 k=3 (2^24 states), DilutionTable(8,0.05,0.9), priors U[0.01,0.10] "
+                       "seed 3; round = Op-BBPA select + update with a seeded response"}
+    picks = {}
+    for prec, key in ((bgt.Precision.float64, "fp64"), (bgt.Precision.float32_compensated, "fp32c")):
+        lat = bgt.Lattice(bgt.LatticeType.replicated_dilution, c.subjects, pi, variants=c.variants, precision=prec)
+        lat.select(bgt.SelectorType.op_bbpa)  # allocates the transform intermediate
+        g = workload.SplitMix64(c.seed)
+        sel = []
+        device_sync()
+        t0 = time.perf_counter()
+        for _ in range(rounds):
+            s = lat.select(bgt.SelectorType.op_bbpa)
+            sel.append(s.experiment)
+            lat.update(s.experiment, g.below(1 << c.variants), tab)
+        device_sync()
+        dt = time.perf_counter() - t0
+        lat.close()
+        picks[key] = sel
+        states = 1 << (c.subjects * c.variants)
+        b = 8 if key == "fp64" else 4
+        out[key] = {"ms_per_round": dt / rounds * 1e3, "round_GBs_alg": 4 * states * b / (dt / rounds) / 1e9}
+    out["identical_selections"] = sum(a == b for a, b in zip(picks["fp64"], picks["fp32c"]))
+    out["selections"] = rounds
+    out["note"] = "2^24 states (128 MiB FP64) stay in the 256 MiB Infinity Cache: cache-bound, not HBM-bound"
+    return out
+
+
+def serving_workload(sessions: int, rate: float, seed: int = 5) -> dict:
+    """The serving mix: `sessions` adaptive sessions, n uniform in 10..24, k in {1, 2} with
+    n*k <= 26, dilution 50/50, priors U[0.005, 0.10], Poisson arrivals at `rate` per second;
+    each session runs until classified or 20 tests. Continuous batching by HBM footprint."""
+    import bayesiangrouptesting_amd as bgt
+    from bayesiangrouptesting_amd import serving
+
+    out = {"workload": f"{sessions} sessions, n in 10..24, k in {{1,2}}, n*k<=26, dilution 50/50, priors "
+                       f"U[0.005,0.10], Poisson arrivals at {rate:g}/s (seed {seed}), up to 20 tests each"}
+    specs = serving.serving_mix(sessions, rate=rate, seed=seed)
+    exps = {}
+    for prec, key in ((bgt.Precision.float64, "fp64"), (bgt.Precision.float32_compensated, "fp32c")):
+        eng = serving.ServingEngine(precision=prec, hbm_budget=96 << 30)
+        rep = eng.run(specs, realtime=True)
+        exps[key] = {r.sid: r.experiments for r in rep.pop("results")}
+        out[key] = rep
+        log(f"serving {key}: {rep['sessions_per_s']:.1f} sessions/s p50 {rep['latency_ms']['p50']:.1f} ms "
+            f"p99 {rep['latency_ms']['p99']:.1f} ms")
+    out["identical_sessions"] = sum(exps["fp64"][s] == exps["fp32c"][s] for s in exps["fp64"])
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=16)
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-aux", action="store_true", help="skip the tree, BBPA and serving workloads")
+    ap.add_argument("--serving-sessions", type=int, default=4096)
+    ap.add_argument("--serving-rate", type=float, default=200.0, help="Poisson arrival rate, sessions/s")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this provider runs one lattice per GPU and does not shard "
@@ -156,30 +337,22 @@ def main() -> int:
     c = workload.C2
     n = c.subjects
     pi = workload.priors(c.seed, n, c.prior_lo, c.prior_hi)
-    truth_int = workload.true_state(c.seed, pi)
-    truth_pub = bgt.reverse_bits(truth_int, n)
     S = 1 << n
     W, K = args.warmup, args.steps
 
-    print(f"[bench] n={n} states={S} steps={K} warmup={W}", file=sys.stderr, flush=True)
-    lat64, dt64, kt64, pk64, lm64 = run_form(bgt.Precision.float64, n, pi, truth_pub, K, W, c.seed)
-    # the trajectory the FP64 form chose: (experiment applied, response) per round
-    sampler = workload.ResponseSampler(c.seed, n, 1, truth_pub)
-    drive = []
-    for i in range(W + K):
-        e = pk64[i].experiment
-        drive.append((e, sampler.draw(e)))
-    post64 = lat64.posterior()
-    m64 = lat64.atom_masses()
-    lat64.close()
-    print(f"[bench] fp64 {dt64 / K * 1e3:.3f} ms/round", file=sys.stderr, flush=True)
-    lat32, dt32, kt32, pk32, lm32 = run_form(bgt.Precision.float32_compensated, n, pi, truth_pub, K, W, c.seed,
-                                             drive=drive)
-    post32 = lat32.posterior()
-    m32 = lat32.atom_masses()
-    lat32.close()
-    print(f"[bench] fp32c {dt32 / K * 1e3:.3f} ms/round", file=sys.stderr, flush=True)
+    log(f"n={n} states={S} steps={K} warmup={W}")
+    r64 = run_sessions(bgt.Precision.float64, n, pi, K, W, c.seed)
+    post64 = r64["lattice"].posterior()
+    m64 = r64["lattice"].atom_masses()
+    r64["lattice"].close()
+    log(f"fp64 {r64['seconds'] / K * 1e3:.3f} ms/round over {r64['sessions']} sessions")
+    r32 = run_sessions(bgt.Precision.float32_compensated, n, pi, K, W, c.seed, drive=r64["plan"])
+    post32 = r32["lattice"].posterior()
+    m32 = r32["lattice"].atom_masses()
+    r32["lattice"].close()
+    log(f"fp32c {r32['seconds'] / K * 1e3:.3f} ms/round")
 
+    pk64, pk32 = r64["picks"], r32["picks"]
     rounds = len(pk64)
     same = sum(1 for a, b in zip(pk64, pk32) if a.experiment == b.experiment)
     resolved = [(a, b) for a, b in zip(pk64, pk32) if not (a.near_tie or b.near_tie)]
@@ -190,16 +363,17 @@ def main() -> int:
     del post32, post64, big
 
     b32, b64 = 4, 8
-    p1_32 = kt32["round_pass1"][0]
-    p2_32 = kt32["round_pass2"][0]
-    p1_64 = kt64["round_pass1"][0]
-    p2_64 = kt64["round_pass2"][0]
+    kt32, kt64 = r32["kernels"], r64["kernels"]
+    p1_32, p2_32 = kt32["round_pass1"][0], kt32["round_pass2"][0]
+    p1_64, p2_64 = kt64["round_pass1"][0], kt64["round_pass2"][0]
     alg1 = S * (2 * b32 + b32)
     round_alg32 = S * (2 * b32 + 2 * b32)
     round_alg64 = S * (2 * b64 + 2 * b64)
     achieved = alg1 / (p1_32 * 1e-3) / 1e9
     traffic = load_traffic()
+    dt32, dt64 = r32["seconds"], r64["seconds"]
     value = K / dt32
+    lm32, lm64 = r32["lat_ms"], r64["lat_ms"]
     line = {
         "metric": METRIC,
         "value": value,
@@ -215,9 +389,10 @@ def main() -> int:
         "data": "synthetic",
         "config": {
             "workload": "binary non-dilution lattice n=28 k=1 (2^28 states), priors U[0.005,0.05] seed 2; "
-                        "round = update(response) + atom marginals + classification + Op-BHA select; the FP32C "
-                        "run replays the (experiment, response) sequence chosen by the FP64 run",
-            "subjects": n, "variants": 1, "states": S,
+                        "round = update(response) + atom marginals + classification + Op-BHA select; adaptive "
+                        "sessions with true states drawn from the prior (seeds 2000+i), each until classified or "
+                        "20 tests; the FP32C run replays the (experiment, response) sequence chosen by the FP64 run",
+            "subjects": n, "variants": 1, "states": S, "sessions": r64["sessions"],
             "form": "fp32c (fp32 storage, fp64 on-chip accumulation, exact fp64 re-decision of near ties)",
         },
         "roofline": {
@@ -252,18 +427,28 @@ def main() -> int:
             "identical_selections": same, "selections": rounds,
             "resolved_selections": len(resolved),
             "identical_resolved": sum(1 for a, b in resolved if a.experiment == b.experiment),
-            "posterior_max_rel_err": post_err, "posterior_l1_rel_err": l1_err, "mass_max_rel_err": mass_err,
+            "near_ties_fp64": sum(1 for p in pk64 if p.near_tie),
             "near_ties_fp32c": sum(1 for p in pk32 if p.near_tie),
             "redecided_fp32c": sum(1 for p in pk32 if p.redecided),
             "band_overflow_fp32c": sum(1 for p in pk32 if p.redecided == 2),
             "rescored_candidates_fp32c_max": max(p.candidates for p in pk32),
+            "same_classification": r64["classified"] == r32["classified"],
+            "posterior_max_rel_err": post_err, "posterior_l1_rel_err": l1_err, "mass_max_rel_err": mass_err,
+            "note": "errors of the last session's final posterior and masses, FP32C against FP64",
         },
         "kernels_ms_fp32c": {k: round(v[0], 4) for k, v in kt32.items()},
     }
+    if not args.no_aux:
+        log("tree (n=16 dilution, depth 6)")
+        line["tree"] = tree_workload()
+        log("bbpa (n=8, k=3)")
+        line["bbpa"] = bbpa_workload()
+        log(f"serving ({args.serving_sessions} sessions at {args.serving_rate:g}/s)")
+        line["serving"] = serving_workload(args.serving_sessions, args.serving_rate)
     if not args.no_cpu:
-        print("[bench] cpu baseline (one n=28 round, 1 thread)", file=sys.stderr, flush=True)
-        e_int = bgt.reverse_bits(drive[0][0], n)
-        cb = cpu_baseline(n, pi, e_int, drive[0][1])
+        log("cpu baseline (one n=28 round, 1 thread)")
+        e_int = bgt.reverse_bits(r64["plan"][0][1], n)
+        cb = cpu_baseline(n, pi, e_int, r64["plan"][0][2])
         line["cpu_baseline"] = {
             "value": 1.0 / cb["round_s"], "unit": "rounds/s", "cores": 1, "kind": "port",
             "sample": f"one full n=28 round of the C restatement (-O3 -ffast-math): select {cb['select_s']:.2f} s, "

@@ -80,6 +80,7 @@ typedef struct bgt_hip_selection {
 	int redecided;
 	int candidates; /* experiments rescored exactly (0 without a re-decision) */
 	int band;       /* candidates the device collected within the error band */
+	int collected;  /* entries the device set aside before the final band filter */
 } bgt_hip_selection;
 
 /* ---- provider entry points: counterparts of include/bgt/detail/cuda_provider.hpp:5-24 ---- */
