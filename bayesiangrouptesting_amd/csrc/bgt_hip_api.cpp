@@ -173,15 +173,18 @@ int bgt_hip_lattice_create(int use_dilution, int subjects, int variants, const d
 			throw bgt_hip::Error(BGT_HIP_BAD_INPUT, "null output handle");
 		*out = nullptr;
 		auto h = std::make_unique<bgt_hip_lattice>();
-		h->impl = std::make_unique<bgt_hip::DeviceLattice>(use_dilution != 0, subjects, variants, prior,
-														   storage_of(precision));
+		h->impl = bgt_hip::DeviceLattice::make(use_dilution != 0, subjects, variants, prior, storage_of(precision));
 		*out = h.release();
 	});
 }
 
 int bgt_hip_lattice_destroy(bgt_hip_lattice *lattice)
 {
-	return guarded([&] { delete lattice; });
+	return guarded([&] {
+		if (lattice != nullptr)
+			bgt_hip::DeviceLattice::recycle(std::move(lattice->impl));
+		delete lattice;
+	});
 }
 
 int bgt_hip_lattice_update(bgt_hip_lattice *h, uint32_t experiment, uint32_t response, const double *dilution)

@@ -90,8 +90,22 @@ void batch_step(std::vector<BatchItem> &items, int selector, double up, double l
 	std::sort(seen.begin(), seen.end());
 	if (std::adjacent_find(seen.begin(), seen.end()) != seen.end())
 		throw Error(BGT_HIP_BAD_INPUT, "a lattice appears twice in one batch");
-	for (auto &it : items)
+	for (auto &it : items) {
+		if (it.source == nullptr)
+			continue;
+		const DeviceLattice &src = *it.source;
+		if (!it.update || src.dilution() != it.lat->dilution() || src.subjects() != it.lat->subjects() ||
+			src.variants() != it.lat->variants() || src.storage() != it.lat->storage())
+			throw Error(BGT_HIP_BAD_INPUT, "an update source must match its lattice's shape and storage");
+		for (auto &other : items)
+			if (other.update && other.lat == it.source)
+				throw Error(BGT_HIP_BAD_INPUT, "an update source is itself updated in the same batch");
+	}
+	for (auto &it : items) {
 		it.lat->settle();
+		if (it.source)
+			const_cast<DeviceLattice *>(it.source)->settle();
+	}
 	// host-side preparation in item order: update descriptors carry the pre-update state,
 	// selection descriptors the state after it
 	std::vector<LatDesc> du(count), ds(count);
@@ -100,6 +114,8 @@ void batch_step(std::vector<BatchItem> &items, int selector, double up, double l
 		auto &it = items[i];
 		if (it.update) {
 			du[i] = it.lat->desc();
+			if (it.source)
+				it.source->as_source(du[i]);
 			liks[i] = it.lat->lik_host();
 			it.lat->commit_update();
 		}

@@ -17,6 +17,10 @@ struct BatchItem {
 	DeviceLattice *lat = nullptr;
 	bool update = false; // apply (experiment, response) first
 	uint32_t experiment = 0, response = 0;
+	// with update: read the posterior to update from this lattice (same shape and storage,
+	// adopted by `lat` beforehand) instead of lat's own -- the out-of-place update of the
+	// tree's children (lattice_base.cpp:210-228). It must not be updated in the same batch.
+	const DeviceLattice *source = nullptr;
 	const double *dilution = nullptr;
 	bool classify = true; // atom marginals + sticky classification
 	bool select = true;   // select the next experiment

@@ -104,6 +104,11 @@ struct LatDesc {
 	double band_width; // error band of the objective (absolute)
 	unsigned *wgflag;  // workgroups of the final transform whose runner-ups reached the band
 	unsigned band_cap;
+	// out-of-place update (k_update only): read x = src_post * src_scal[src_inv] instead of
+	// this lattice's own posterior (null: in place)
+	const void *src_post;
+	const double *src_scal;
+	int src_inv;
 };
 
 constexpr int kLikCapacity = kMaxVariants * (kMaxSubjects + 1);

@@ -482,17 +482,20 @@ template <typename TP>
 __global__ __launch_bounds__(256) void k_update(Descs descs, uint64_t states)
 {
 	// lattice_base.cpp:230-244: y' = x * L(E, r, S); D = sum y'; the pending 1/D is applied
-	// by the next reader (x' = y' * (1/D))
+	// by the next reader (x' = y' * (1/D)). With d.src_post set the input x is another
+	// lattice's posterior (update_probs, the out-of-place form the tree uses,
+	// lattice_base.cpp:210-228).
 	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
 	__shared__ double s_lik[kMaxVariants * (kMaxSubjects + 1)];
 	__shared__ double red[16];
 	load_lik_table(descs, d, s_lik);
 	gptr<TP> post = G(static_cast<TP *>(d.post));
-	const double inv = d.scal[d.inv_in];
+	gptr<const TP> src = G(static_cast<const TP *>(d.src_post ? d.src_post : d.post));
+	const double inv = d.src_post ? d.src_scal[d.src_inv] : d.scal[d.inv_in];
 	double acc = 0.0;
 	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
 	for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < states; s += stride) {
-		const double x = static_cast<double>(post[s]) * inv;
+		const double x = static_cast<double>(src[s]) * inv;
 		const TP y = static_cast<TP>(x * likelihood((uint32_t)s, d.e, d.n, d.k, d.len, d.dilution, s_lik));
 		post[s] = y;
 		acc += static_cast<double>(y);

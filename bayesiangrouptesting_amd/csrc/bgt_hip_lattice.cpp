@@ -1,13 +1,19 @@
 // Host orchestration of one device-resident lattice.
 #include "bgt_hip_lattice.hpp"
 
+#include "bgt_hip_resources.hpp"
+
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <memory>
+#include <mutex>
+#include <tuple>
 
 namespace bgt_hip {
 
@@ -157,25 +163,126 @@ DeviceLattice::DeviceLattice(bool dilution, int subjects, int variants, const do
 		throw Error(BGT_HIP_BAD_INPUT, "prior is required");
 	require_device();
 	states_ = 1ull << atoms_;
-	active_ = (subjects >= 32) ? 0xFFFFFFFFu : ((1u << subjects) - 1u);
-	BGT_HIP_CHECK(hipStreamCreate(&stream_));
+	stream_ = stream_acquire();
 	allocate();
-	exact_.reset(variants == 1 && !dilution, atoms_, prior);
+	init_prior(prior);
+}
+
+void DeviceLattice::init_prior(const double *prior)
+{
+	pos_ = neg_ = 0;
+	active_ = (n_ >= 32) ? 0xFFFFFFFFu : ((1u << n_) - 1u);
+	updates_ = 0;
+	inv_cur_ = kScalInv0;
+	e_ = r_ = 0;
+	len_ = 0;
+	clock_.reset();
+	clock_.enabled = false;
+	exact_.reset(k_ == 1 && !dilution_, atoms_, prior);
 	std::vector<double> pairs(2 * kMaxAtoms, 0.0);
 	for (int a = 0; a < atoms_; ++a) {
 		pairs[2 * a] = prior[a];
 		pairs[2 * a + 1] = 1.0 - prior[a];
 	}
+	const double init[kScalars] = {1.0, 1.0, 1.0, 1.0, 0.0, 0.0, 0.0, 0.0};
+	BGT_HIP_CHECK(hipMemcpyAsync(scal_, init, sizeof(init), hipMemcpyHostToDevice, stream_));
 	BGT_HIP_CHECK(hipMemcpyAsync(pairs_, pairs.data(), sizeof(double) * pairs.size(), hipMemcpyHostToDevice, stream_));
 	launch_prior(single(), 1, atoms_, storage_, pairs_, states_, stream_);
 	sync();
+}
+
+// ------------------------------------------------------------------- lattice cache
+
+namespace {
+
+// Released lattices by shape and storage, kept for re-use while their device bytes stay
+// within a budget (1/8 of the device's memory, or BGT_HIP_LATTICE_CACHE_BYTES bytes when
+// that environment variable is set; 0 disables the cache): creating a lattice allocates device and
+// pinned host memory and a stream, and releasing them synchronises the device, which
+// dominates small lattices' lifetimes (tree children, serving sessions). The cache lives
+// until the process ends and is never destroyed, so no HIP call runs during static
+// destruction.
+class LatticeCache {
+public:
+	static LatticeCache &get()
+	{
+		static LatticeCache *cache = new LatticeCache();
+		return *cache;
+	}
+	std::unique_ptr<DeviceLattice> take(bool dilution, int n, int k, Storage st)
+	{
+		std::lock_guard<std::mutex> lk(m_);
+		auto it = free_.find(Key{dilution, n, k, (int)st});
+		if (it == free_.end() || it->second.empty())
+			return nullptr;
+		auto l = std::move(it->second.back());
+		it->second.pop_back();
+		bytes_ -= l->device_bytes();
+		return l;
+	}
+	void give(std::unique_ptr<DeviceLattice> l)
+	{
+		{
+			std::lock_guard<std::mutex> lk(m_);
+			if (!budget_set_) {
+				size_t free_b = 0, total_b = 0;
+				budget_ = hipMemGetInfo(&free_b, &total_b) == hipSuccess ? total_b / 8 : (1ull << 30);
+				if (const char *env = std::getenv("BGT_HIP_LATTICE_CACHE_BYTES"))
+					budget_ = std::strtoull(env, nullptr, 10);
+				budget_set_ = true;
+			}
+			if (bytes_ + l->device_bytes() <= budget_) {
+				bytes_ += l->device_bytes();
+				free_[Key{l->dilution(), l->subjects(), l->variants(), (int)l->storage()}].push_back(std::move(l));
+				return;
+			}
+		}
+		l.reset(); // over budget: released outside the lock
+	}
+
+private:
+	using Key = std::tuple<bool, int, int, int>;
+	std::mutex m_;
+	std::map<Key, std::vector<std::unique_ptr<DeviceLattice>>> free_;
+	uint64_t bytes_ = 0, budget_ = 0;
+	bool budget_set_ = false;
+};
+
+} // namespace
+
+std::unique_ptr<DeviceLattice> DeviceLattice::make(bool dilution, int subjects, int variants, const double *prior,
+												   Storage storage)
+{
+	if (subjects > 0 && variants > 0 && prior != nullptr && subjects * variants <= kMaxAtoms) {
+		if (auto l = LatticeCache::get().take(dilution, subjects, variants, storage)) {
+			l->settle();
+			l->init_prior(prior);
+			return l;
+		}
+	}
+	return std::make_unique<DeviceLattice>(dilution, subjects, variants, prior, storage);
+}
+
+std::unique_ptr<DeviceLattice> DeviceLattice::make_like(const DeviceLattice &like)
+{
+	if (auto l = LatticeCache::get().take(like.dilution_, like.n_, like.k_, like.storage_)) {
+		l->settle();
+		return l;
+	}
+	return std::make_unique<DeviceLattice>(like, false);
+}
+
+void DeviceLattice::recycle(std::unique_ptr<DeviceLattice> lattice)
+{
+	if (lattice)
+		LatticeCache::get().give(std::move(lattice));
 }
 
 DeviceLattice::DeviceLattice(const DeviceLattice &parent, bool copy_posterior)
 	: dilution_(parent.dilution_), n_(parent.n_), k_(parent.k_), atoms_(parent.atoms_), states_(parent.states_),
 	  storage_(parent.storage_)
 {
-	BGT_HIP_CHECK(hipStreamCreate(&stream_));
+	stream_ = stream_acquire();
 	allocate();
 	pos_ = parent.pos_;
 	neg_ = parent.neg_;
@@ -249,10 +356,10 @@ uint64_t DeviceLattice::footprint(bool dilution, int subjects, int variants, Sto
 void DeviceLattice::allocate()
 {
 	const size_t eb = storage_ == Storage::fp64 ? 8 : 4;
-	BGT_HIP_CHECK(hipMalloc(&post_, eb * states_));
+	post_ = device_alloc(eb * states_, stream_);
 	bytes_ += eb * states_;
 	const ArenaLayout a = arena_layout(dilution_, n_, k_);
-	BGT_HIP_CHECK(hipMalloc(reinterpret_cast<void **>(&arena_), a.total));
+	arena_ = static_cast<char *>(device_alloc(a.total, stream_));
 	bytes_ += a.total;
 	BGT_HIP_CHECK(hipMemsetAsync(arena_, 0, a.total, stream_));
 	scal_ = reinterpret_cast<double *>(arena_ + a.scal);
@@ -270,13 +377,12 @@ void DeviceLattice::allocate()
 		cand_ = reinterpret_cast<Cand *>(arena_ + a.cand);
 		wgflag_ = reinterpret_cast<unsigned *>(arena_ + a.flags);
 		band_cap_ = (unsigned)band_capacity(atoms_);
-		BGT_HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&band_host_), sizeof(Cand) * band_cap_,
-									hipHostMallocDefault));
+		band_host_ = static_cast<Cand *>(pinned_alloc(sizeof(Cand) * band_cap_));
 	}
 	double init[kScalars] = {1.0, 1.0, 1.0, 1.0, 0.0, 0.0, 0.0, 0.0};
 	BGT_HIP_CHECK(hipMemcpyAsync(scal_, init, sizeof(init), hipMemcpyHostToDevice, stream_));
-	BGT_HIP_CHECK(hipHostMalloc(&host_box_, sizeof(HostBox), hipHostMallocDefault));
-	sync();
+	host_box_ = pinned_alloc(sizeof(HostBox));
+	sync(); // the allocations are complete before any other stream uses them
 }
 
 void DeviceLattice::ensure_z()
@@ -285,26 +391,42 @@ void DeviceLattice::ensure_z()
 		return;
 	const size_t eb = storage_ == Storage::fp64 ? 8 : 4;
 	const uint64_t zb = z_bytes(atoms_, eb);
-	BGT_HIP_CHECK(hipMalloc(&z_, zb));
+	z_ = device_alloc(zb, stream_);
 	bytes_ += zb;
+	sync(); // batched steps use it from another stream
 }
 
 DeviceLattice::~DeviceLattice()
 {
+	// every use of this lattice's memory is on its own stream or finished (batched steps
+	// synchronise before they return), so the frees need only order after its own stream
 	if (stream_)
 		(void)hipStreamSynchronize(stream_);
-	if (post_)
-		(void)hipFree(post_);
-	if (z_)
-		(void)hipFree(z_);
-	if (arena_)
-		(void)hipFree(arena_);
-	if (host_box_)
-		(void)hipHostFree(host_box_);
-	if (band_host_)
-		(void)hipHostFree(band_host_);
-	if (stream_)
-		(void)hipStreamDestroy(stream_);
+	device_free(post_, stream_);
+	device_free(z_, stream_);
+	device_free(arena_, stream_);
+	pinned_release(host_box_, sizeof(HostBox));
+	pinned_release(band_host_, sizeof(Cand) * band_cap_);
+	stream_release(stream_);
+}
+
+void DeviceLattice::adopt(const DeviceLattice &parent)
+{
+	if (parent.dilution_ != dilution_ || parent.n_ != n_ || parent.k_ != k_ || parent.storage_ != storage_)
+		throw Error(BGT_HIP_BAD_INPUT, "adopt: lattices differ in shape or storage");
+	pos_ = parent.pos_;
+	neg_ = parent.neg_;
+	active_ = parent.active_;
+	updates_ = parent.updates_;
+	inv_cur_ = parent.inv_cur_;
+	exact_ = parent.exact_;
+}
+
+void DeviceLattice::as_source(LatDesc &d) const
+{
+	d.src_post = post_;
+	d.src_scal = scal_;
+	d.src_inv = inv_cur_;
 }
 
 LatDesc DeviceLattice::desc() const

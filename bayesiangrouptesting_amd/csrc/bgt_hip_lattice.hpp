@@ -6,6 +6,7 @@
 #include "bgt_hip_exact.hpp"
 #include "bgt_hip_internal.hpp"
 
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -48,6 +49,16 @@ public:
 	// Copy of `parent`'s posterior (same shape); the child's classification masks start
 	// from the parent's. Used by the tree builder for out-of-place updates.
 	DeviceLattice(const DeviceLattice &parent, bool copy_posterior);
+
+	// Lattices through the process-wide cache of released lattices (see LatticeCache):
+	// make() gives a lattice holding `prior` (a cached one of this shape re-initialised, or
+	// a new one); make_like() a lattice of `like`'s shape and storage whose posterior and
+	// state are to be set by the caller (adopt + an out-of-place update); recycle() hands a
+	// lattice back to the cache, which keeps it while within its byte budget.
+	static std::unique_ptr<DeviceLattice> make(bool dilution, int subjects, int variants, const double *prior,
+											   Storage storage);
+	static std::unique_ptr<DeviceLattice> make_like(const DeviceLattice &like);
+	static void recycle(std::unique_ptr<DeviceLattice> lattice);
 
 	int subjects() const { return n_; }
 	int variants() const { return k_; }
@@ -112,10 +123,18 @@ public:
 	SelResult head_result() const;
 	// host-side preparation of exact rescoring, to overlap with device work
 	void prepare_exact() { exact_.prepare(); }
+	// Take over `parent`'s host-side state (masks, candidate restriction, update history,
+	// pending normalisation slot) ahead of an out-of-place update from it; the posterior
+	// itself is produced by that update. Both must have the same shape and storage.
+	void adopt(const DeviceLattice &parent);
+	// source fields of an out-of-place update reading this lattice's posterior
+	void as_source(LatDesc &d) const;
 
 private:
 	Descs single() const;
 	void allocate();
+	// write the prior and reset every piece of state (construction, and re-use from the cache)
+	void init_prior(const double *prior);
 	void fill_lik(uint32_t experiment, uint32_t response, const double *dilution_flat);
 	void rescore(bgt_hip_selection &out, int selector, const SelResult &r);
 	// complete band of candidates for exact rescoring (binary non-dilution lattices); the

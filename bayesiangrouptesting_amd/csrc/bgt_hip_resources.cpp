@@ -1,0 +1,137 @@
+#include "bgt_hip_resources.hpp"
+
+#include "bgt_hip_internal.hpp"
+
+#include <cstdint>
+#include <mutex>
+#include <vector>
+
+namespace bgt_hip {
+
+namespace {
+
+// Shared state, created on first use and never destroyed: no HIP call may run during
+// static destruction, after the runtime has gone.
+struct Resources {
+	std::mutex m;
+	int async_alloc = -1; // -1 unknown, 0 unsupported, 1 stream-ordered allocation in use
+	std::vector<void *> pinned_free[24]; // by log2 of the block size
+	std::vector<hipStream_t> streams;
+};
+
+Resources &res()
+{
+	static Resources *r = new Resources();
+	return *r;
+}
+
+int pinned_class(size_t bytes)
+{
+	int c = 8; // 256 bytes
+	while ((size_t(1) << c) < bytes)
+		++c;
+	return c;
+}
+
+bool async_supported()
+{
+	Resources &r = res();
+	std::lock_guard<std::mutex> lk(r.m);
+	if (r.async_alloc < 0) {
+		int dev = 0, ok = 0;
+		r.async_alloc = 0;
+		if (hipGetDevice(&dev) == hipSuccess &&
+			hipDeviceGetAttribute(&ok, hipDeviceAttributeMemoryPoolsSupported, dev) == hipSuccess && ok) {
+			hipMemPool_t pool;
+			if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+				// keep freed memory in the pool instead of returning it at each synchronisation
+				uint64_t keep = UINT64_MAX;
+				if (hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep) == hipSuccess)
+					r.async_alloc = 1;
+			}
+		}
+	}
+	return r.async_alloc == 1;
+}
+
+} // namespace
+
+void *device_alloc(size_t bytes, hipStream_t stream)
+{
+	void *p = nullptr;
+	if (async_supported())
+		BGT_HIP_CHECK(hipMallocAsync(&p, bytes, stream));
+	else
+		BGT_HIP_CHECK(hipMalloc(&p, bytes));
+	return p;
+}
+
+void device_free(void *p, hipStream_t stream)
+{
+	if (p == nullptr)
+		return;
+	if (async_supported())
+		(void)hipFreeAsync(p, stream);
+	else
+		(void)hipFree(p);
+}
+
+void *pinned_alloc(size_t bytes)
+{
+	const int c = pinned_class(bytes);
+	Resources &r = res();
+	{
+		std::lock_guard<std::mutex> lk(r.m);
+		auto &fl = r.pinned_free[c];
+		if (!fl.empty()) {
+			void *p = fl.back();
+			fl.pop_back();
+			return p;
+		}
+	}
+	// a fresh chunk of at least 64 KiB, carved into blocks of this class
+	const size_t block = size_t(1) << c;
+	const size_t chunk = block < (size_t(1) << 16) ? (size_t(1) << 16) : block;
+	void *base = nullptr;
+	BGT_HIP_CHECK(hipHostMalloc(&base, chunk, hipHostMallocDefault));
+	std::lock_guard<std::mutex> lk(r.m);
+	for (size_t off = block; off < chunk; off += block)
+		r.pinned_free[c].push_back(static_cast<char *>(base) + off);
+	return base;
+}
+
+void pinned_release(void *p, size_t bytes)
+{
+	if (p == nullptr)
+		return;
+	Resources &r = res();
+	std::lock_guard<std::mutex> lk(r.m);
+	r.pinned_free[pinned_class(bytes)].push_back(p);
+}
+
+hipStream_t stream_acquire()
+{
+	Resources &r = res();
+	{
+		std::lock_guard<std::mutex> lk(r.m);
+		if (!r.streams.empty()) {
+			hipStream_t s = r.streams.back();
+			r.streams.pop_back();
+			return s;
+		}
+	}
+	hipStream_t s = nullptr;
+	BGT_HIP_CHECK(hipStreamCreate(&s));
+	return s;
+}
+
+void stream_release(hipStream_t s)
+{
+	if (s == nullptr)
+		return;
+	Resources &r = res();
+	std::lock_guard<std::mutex> lk(r.m);
+	r.streams.push_back(s);
+}
+
+} // namespace bgt_hip

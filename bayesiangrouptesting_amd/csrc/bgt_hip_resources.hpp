@@ -1,0 +1,32 @@
+// Process-wide device and host resources of the provider: stream-ordered device memory,
+// pinned host blocks and streams, each recycled rather than released. Releasing device or
+// pinned memory with hipFree / hipHostFree synchronises the whole device, and creating a
+// stream or a pinned buffer costs tens to hundreds of microseconds; a serving loop or a
+// tree creates and retires lattices continually, so they come from here.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstddef>
+
+namespace bgt_hip {
+
+// Device memory ordered on `stream` (hipMallocAsync on the device's default pool, which
+// keeps freed memory for re-use; plain hipMalloc where stream-ordered allocation is not
+// supported). The allocation is usable by work on `stream`; other streams must wait for it.
+void *device_alloc(size_t bytes, hipStream_t stream);
+// Release after the work already queued on `stream`. Whoever frees must make sure no
+// other stream still uses the memory.
+void device_free(void *p, hipStream_t stream);
+
+// Pinned host memory in power-of-two blocks (at least 256 bytes), never returned to the
+// system; pinned_release takes the block back for re-use.
+void *pinned_alloc(size_t bytes);
+void pinned_release(void *p, size_t bytes);
+
+// A stream of default priority from a pool; stream_release returns it (the caller has
+// synchronised it).
+hipStream_t stream_acquire();
+void stream_release(hipStream_t s);
+
+} // namespace bgt_hip

@@ -10,6 +10,7 @@
 #include "bgt_hip_tree.hpp"
 
 #include "bgt_hip_batch.hpp"
+#include "bgt_hip_resources.hpp"
 
 #include <hip/hip_runtime_api.h>
 
@@ -52,8 +53,11 @@ bool is_classified(uint32_t pos, uint32_t neg, int atoms) { return __builtin_pop
 
 // LocalTree construction (global_tree_intra.cpp:8-40), one level at a time: every
 // unfinished node of a level selects in one batched step, and all of their children are
-// updated and classified in one more. A node becomes a leaf when it is classified or at the
-// depth limit; a leaf's posterior is freed at once.
+// produced in one more -- each child's posterior written by the out-of-place update of its
+// parent's (read the parent once, write the children; update_probs, lattice_base.cpp:210-228)
+// and classified. A node becomes a leaf when it is classified or at the depth limit. Device
+// lattices are recycled: a leaf's lattice, and a parent's once its children exist, return to
+// a pool that the next level draws from, so a tree allocates about two levels' worth.
 std::unique_ptr<HostNode> build_levels(const TreeCtx &c, std::unique_ptr<DeviceLattice> root_lat, TreeBuildStats &bs)
 {
 	struct Open {
@@ -62,12 +66,22 @@ std::unique_ptr<HostNode> build_levels(const TreeCtx &c, std::unique_ptr<DeviceL
 	};
 	const int atoms = c.n * c.k;
 	const int rc = 1 << c.k;
+	std::vector<std::unique_ptr<DeviceLattice>> pool;
+	auto take = [&](const DeviceLattice &like) {
+		if (pool.empty())
+			return DeviceLattice::make_like(like);
+		auto l = std::move(pool.back());
+		pool.pop_back();
+		return l;
+	};
 	auto root = std::make_unique<HostNode>();
 	root->pos = root_lat->positive_mask();
 	root->neg = root_lat->negative_mask();
 	std::vector<Open> frontier;
 	if (!is_classified(root->pos, root->neg, atoms) && c.depth > 0)
 		frontier.push_back({root.get(), std::move(root_lat)});
+	else
+		DeviceLattice::recycle(std::move(root_lat));
 	std::vector<BatchItem> items;
 	std::vector<BatchResult> res;
 	while (!frontier.empty()) {
@@ -80,16 +94,17 @@ std::unique_ptr<HostNode> build_levels(const TreeCtx &c, std::unique_ptr<DeviceL
 			items[i].classify = false;
 		}
 		batch_step(items, c.selector, c.up, c.lo, res);
-		// children: copies of the parent for all responses but the last, which takes the
-		// parent's own lattice (global_tree_intra.cpp:24-36)
+		// children: one per response, each a lattice from the pool that adopts its parent's
+		// state and receives the parent's posterior times the likelihood
 		std::vector<Open> kids;
-		std::vector<uint32_t> kid_e;
 		kids.reserve(frontier.size() * rc);
+		items.clear();
 		for (size_t i = 0; i < frontier.size(); ++i) {
 			const uint32_t e = res[i].selection.experiment;
 			bs.selections += 1;
 			bs.near_ties += res[i].selection.near_tie;
 			HostNode &nd = *frontier[i].node;
+			const DeviceLattice &parent = *frontier[i].lat;
 			for (int r = 0; r < rc; ++r) {
 				auto child = std::make_unique<HostNode>();
 				child->ex = e;
@@ -97,34 +112,36 @@ std::unique_ptr<HostNode> build_levels(const TreeCtx &c, std::unique_ptr<DeviceL
 				child->stage = nd.stage + 1;
 				HostNode *cp = child.get();
 				nd.children.push_back(std::move(child));
-				std::unique_ptr<DeviceLattice> lat = r != rc - 1
-					? std::make_unique<DeviceLattice>(*frontier[i].lat, true)
-					: std::move(frontier[i].lat);
+				auto lat = take(parent);
+				lat->adopt(parent);
+				BatchItem it;
+				it.lat = lat.get();
+				it.update = true;
+				it.experiment = e;
+				it.response = (uint32_t)r;
+				it.source = &parent;
+				it.dilution = c.dil;
+				it.select = false;
+				items.push_back(it);
 				kids.push_back({cp, std::move(lat)});
-				kid_e.push_back(e);
 			}
 		}
-		// update and classify every child
-		items.assign(kids.size(), BatchItem{});
-		for (size_t i = 0; i < kids.size(); ++i) {
-			items[i].lat = kids[i].lat.get();
-			items[i].update = true;
-			items[i].experiment = kid_e[i];
-			items[i].response = kids[i].node->res;
-			items[i].dilution = c.dil;
-			items[i].select = false;
-		}
 		batch_step(items, c.selector, c.up, c.lo, res);
+		for (auto &o : frontier)
+			pool.push_back(std::move(o.lat));
 		frontier.clear();
 		for (size_t i = 0; i < kids.size(); ++i) {
 			HostNode &nd = *kids[i].node;
 			nd.pos = res[i].positive;
 			nd.neg = res[i].negative;
 			if (is_classified(nd.pos, nd.neg, atoms) || nd.stage >= c.depth)
-				continue; // a leaf: its lattice is released here
-			frontier.push_back(std::move(kids[i]));
+				pool.push_back(std::move(kids[i].lat)); // a leaf
+			else
+				frontier.push_back(std::move(kids[i]));
 		}
 	}
+	for (auto &l : pool)
+		DeviceLattice::recycle(std::move(l));
 	return root;
 }
 
@@ -173,7 +190,7 @@ bgt_hip_tree_stats run_tree(bool dilution, int n, int k, const double *prior, in
 	if (dilution && dil == nullptr)
 		throw Error(BGT_HIP_BAD_INPUT, "dilution table is required for dilution models");
 	TreeCtx c{n, k, depth, DeviceLattice::resolve_selector(selector, k), up, lo, dil};
-	auto root_lat = std::make_unique<DeviceLattice>(dilution, n, k, prior, storage);
+	auto root_lat = DeviceLattice::make(dilution, n, k, prior, storage);
 	hipStream_t s = nullptr;
 	TreeBuildStats bs;
 	auto root = build_levels(c, std::move(root_lat), bs);
@@ -188,13 +205,13 @@ bgt_hip_tree_stats run_tree(bool dilution, int n, int k, const double *prior, in
 	double *d_dil = nullptr, *d_pi = nullptr, *d_part = nullptr, *d_out = nullptr;
 	unsigned *d_counter = nullptr;
 	const size_t dil_count = dilution ? (size_t)n * (n + 1) : 1;
-	BGT_HIP_CHECK(hipStreamCreate(&s));
-	BGT_HIP_CHECK(hipMalloc(&d_nodes, sizeof(FlatNode) * nodes.size()));
-	BGT_HIP_CHECK(hipMalloc(&d_dil, sizeof(double) * dil_count));
-	BGT_HIP_CHECK(hipMalloc(&d_pi, sizeof(double) * atoms));
-	BGT_HIP_CHECK(hipMalloc(&d_part, sizeof(double) * 7 * 4096));
-	BGT_HIP_CHECK(hipMalloc(&d_out, sizeof(double) * 8));
-	BGT_HIP_CHECK(hipMalloc(&d_counter, sizeof(unsigned) * 4));
+	s = stream_acquire();
+	d_nodes = static_cast<FlatNode *>(device_alloc(sizeof(FlatNode) * nodes.size(), s));
+	d_dil = static_cast<double *>(device_alloc(sizeof(double) * dil_count, s));
+	d_pi = static_cast<double *>(device_alloc(sizeof(double) * atoms, s));
+	d_part = static_cast<double *>(device_alloc(sizeof(double) * 7 * 4096, s));
+	d_out = static_cast<double *>(device_alloc(sizeof(double) * 8, s));
+	d_counter = static_cast<unsigned *>(device_alloc(sizeof(unsigned) * 4, s));
 	BGT_HIP_CHECK(hipMemsetAsync(d_counter, 0, sizeof(unsigned) * 4, s));
 	BGT_HIP_CHECK(hipMemcpyAsync(d_nodes, nodes.data(), sizeof(FlatNode) * nodes.size(), hipMemcpyHostToDevice, s));
 	if (dilution)
@@ -205,13 +222,13 @@ bgt_hip_tree_stats run_tree(bool dilution, int n, int k, const double *prior, in
 	double out[8] = {};
 	BGT_HIP_CHECK(hipMemcpyAsync(out, d_out, sizeof(double) * 7, hipMemcpyDeviceToHost, s));
 	BGT_HIP_CHECK(hipStreamSynchronize(s));
-	(void)hipFree(d_nodes);
-	(void)hipFree(d_dil);
-	(void)hipFree(d_pi);
-	(void)hipFree(d_part);
-	(void)hipFree(d_out);
-	(void)hipFree(d_counter);
-	(void)hipStreamDestroy(s);
+	device_free(d_nodes, s);
+	device_free(d_dil, s);
+	device_free(d_pi, s);
+	device_free(d_part, s);
+	device_free(d_out, s);
+	device_free(d_counter, s);
+	stream_release(s);
 	stats.correct_probability = out[0];
 	stats.incorrect_probability = out[1];
 	stats.false_positive_probability = out[2];

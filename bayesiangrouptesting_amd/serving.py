@@ -129,6 +129,7 @@ class ServingEngine:
         done: list[SessionResult] = []
         step_ms: list[float] = []
         batch_sizes: list[int] = []
+        admit_s = 0.0  # host time spent creating admitted sessions' lattices
         fp = {}
         t0 = time.perf_counter()
 
@@ -149,7 +150,9 @@ class ServingEngine:
                     if not live:
                         raise RuntimeError(f"session {spec.sid} alone exceeds the HBM budget")
                     break
+                ta = time.perf_counter()
                 live.append(_Live(spec, self.precision, tnow))
+                admit_s += time.perf_counter() - ta
                 used += fp[key]
                 peak_used = max(peak_used, used)
                 qi += 1
@@ -201,7 +204,8 @@ class ServingEngine:
             "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                            "max": float(lat.max())},
             "step_ms": {"p50": float(np.percentile(step_ms, 50)), "p99": float(np.percentile(step_ms, 99)),
-                        "steps": len(step_ms)},
+                        "steps": len(step_ms), "total_s": float(np.sum(step_ms)) / 1e3},
+            "admission_s": admit_s,
             "batch": {"mean": float(np.mean(batch_sizes)), "max": int(np.max(batch_sizes))},
             "tests_mean": float(np.mean([r.tests for r in done])),
             "classified": sum(r.classified for r in done),
