@@ -1,7 +1,8 @@
 """Short workloads for rocprofv3 runs (kernel traces and HBM counters).
 
     python -m bayesiangrouptesting_amd.profile_workloads round28 [fp32c|fp64] [rounds]
-        n = 28 binary non-dilution rounds (the benchmark's workload), few rounds
+        n = 28 binary non-dilution rounds of the benchmark's adaptive sessions (true state of
+        session i drawn with seed 2000 + i, a session ends when classified), few rounds
     python -m bayesiangrouptesting_amd.profile_workloads copy [MiB]
         one device-to-device copy of a known size, the calibration point for FETCH_SIZE /
         WRITE_SIZE (bytes the counters report per byte actually moved)
@@ -23,17 +24,27 @@ def round28(form: str = "fp32c", rounds: int = 4) -> None:
     c = workload.C2
     n = c.subjects
     pi = workload.priors(c.seed, n, c.prior_lo, c.prior_hi)
-    truth = bgt.reverse_bits(workload.true_state(c.seed, pi), n)
     prec = bgt.Precision.float32_compensated if form == "fp32c" else bgt.Precision.float64
-    lat = bgt.Lattice(bgt.LatticeType.replicated_non_dilution, n, pi, precision=prec)
-    sampler = workload.ResponseSampler(c.seed, n, 1, truth)
-    e = lat.select(bgt.SelectorType.op_bha).experiment
-    t0 = time.perf_counter()
-    for _ in range(rounds):
-        r = sampler.draw(e)
-        e = lat.round(e, r, 0.01, 0.01).experiment
-    nat.check(nat.lib().bgt_hip_device_synchronize())
-    print(f"round28 {form}: {rounds} rounds, {(time.perf_counter() - t0) / rounds * 1e3:.3f} ms/round", flush=True)
+    lat = None
+    session = -1
+    done = 0
+    elapsed = 0.0
+    while done < rounds:
+        if lat is None or lat.is_classified():
+            if lat is not None:
+                lat.close()
+            session += 1
+            seed = c.seed * 1000 + session
+            lat = bgt.Lattice(bgt.LatticeType.replicated_non_dilution, n, pi, precision=prec)
+            sampler = workload.ResponseSampler(seed, n, 1, bgt.reverse_bits(workload.true_state(seed, pi), n))
+            e = lat.select(bgt.SelectorType.op_bha).experiment
+        t0 = time.perf_counter()
+        e = lat.round(e, sampler.draw(e), 0.01, 0.01).experiment
+        nat.check(nat.lib().bgt_hip_device_synchronize())
+        elapsed += time.perf_counter() - t0
+        done += 1
+    print(f"round28 {form}: {rounds} rounds over {session + 1} sessions, {elapsed / rounds * 1e3:.3f} ms/round",
+          flush=True)
     lat.close()
 
 

@@ -120,7 +120,8 @@ class ServingEngine:
     def run(self, specs: Sequence[SessionSpec], realtime: bool = True) -> dict:
         """Serve `specs` (arrival times relative to the start). With realtime, sessions
         become visible at their arrival times on the wall clock; otherwise all are
-        queued at time zero. Returns per-session results and aggregate statistics."""
+        queued at time zero (and latency counts from there). Returns per-session results
+        and aggregate statistics."""
         queue = sorted(specs, key=lambda s: (s.arrival, s.sid))
         qi = 0
         live: list[_Live] = []
@@ -190,7 +191,8 @@ class ServingEngine:
                     continue
                 spec = s.spec
                 done.append(SessionResult(spec.sid, spec.subjects, spec.variants, spec.dilution, s.tests,
-                                          s.classified(), s.pos, s.neg, s.experiments, spec.arrival, s.admitted,
+                                          s.classified(), s.pos, s.neg, s.experiments,
+                                          spec.arrival if realtime else 0.0, s.admitted,
                                           tdone))
                 used -= fp[(spec.dilution, spec.subjects, spec.variants)]
                 s.lattice.close()

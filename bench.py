@@ -293,23 +293,32 @@ def bbpa_workload(rounds: int = 8) -> dict:
 
 def serving_workload(sessions: int, rate: float, seed: int = 5) -> dict:
     """The serving mix: `sessions` adaptive sessions, n uniform in 10..24, k in {1, 2} with
-    n*k <= 26, dilution 50/50, priors U[0.005, 0.10], Poisson arrivals at `rate` per second;
-    each session runs until classified or 20 tests. Continuous batching by HBM footprint."""
+    n*k <= 26, dilution 50/50, priors U[0.005, 0.10]; each session runs until classified or
+    20 tests, with continuous batching and admission by HBM footprint (96 GiB). Two runs per
+    form: Poisson arrivals at `rate` per second on the wall clock (latency under a load both
+    forms sustain), and every session queued at once (capacity: sessions/s at saturation)."""
     import bayesiangrouptesting_amd as bgt
     from bayesiangrouptesting_amd import serving
 
     out = {"workload": f"{sessions} sessions, n in 10..24, k in {{1,2}}, n*k<=26, dilution 50/50, priors "
-                       f"U[0.005,0.10], Poisson arrivals at {rate:g}/s (seed {seed}), up to 20 tests each"}
+                       f"U[0.005,0.10] (seed {seed}), up to 20 tests each; 'at_rate': Poisson arrivals at "
+                       f"{rate:g}/s; 'saturated': all sessions queued at time 0"}
     specs = serving.serving_mix(sessions, rate=rate, seed=seed)
     exps = {}
     for prec, key in ((bgt.Precision.float64, "fp64"), (bgt.Precision.float32_compensated, "fp32c")):
-        eng = serving.ServingEngine(precision=prec, hbm_budget=96 << 30)
-        rep = eng.run(specs, realtime=True)
-        exps[key] = {r.sid: r.experiments for r in rep.pop("results")}
-        out[key] = rep
-        log(f"serving {key}: {rep['sessions_per_s']:.1f} sessions/s p50 {rep['latency_ms']['p50']:.1f} ms "
-            f"p99 {rep['latency_ms']['p99']:.1f} ms")
+        out[key] = {}
+        for mode, realtime in (("at_rate", True), ("saturated", False)):
+            eng = serving.ServingEngine(precision=prec, hbm_budget=96 << 30)
+            rep = eng.run(specs, realtime=realtime)
+            results = rep.pop("results")
+            if mode == "at_rate":
+                exps[key] = {r.sid: r.experiments for r in results}
+            out[key][mode] = rep
+            log(f"serving {key} {mode}: {rep['sessions_per_s']:.1f} sessions/s p50 {rep['latency_ms']['p50']:.1f} ms "
+                f"p99 {rep['latency_ms']['p99']:.1f} ms")
     out["identical_sessions"] = sum(exps["fp64"][s] == exps["fp32c"][s] for s in exps["fp64"])
+    out["fp32c_vs_fp64_capacity"] = (out["fp32c"]["saturated"]["sessions_per_s"] /
+                                     out["fp64"]["saturated"]["sessions_per_s"])
     return out
 
 
@@ -321,7 +330,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-aux", action="store_true", help="skip the tree, BBPA and serving workloads")
     ap.add_argument("--serving-sessions", type=int, default=4096)
-    ap.add_argument("--serving-rate", type=float, default=200.0, help="Poisson arrival rate, sessions/s")
+    ap.add_argument("--serving-rate", type=float, default=300.0, help="Poisson arrival rate, sessions/s")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this provider runs one lattice per GPU and does not shard "
