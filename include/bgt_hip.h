@@ -20,8 +20,12 @@
  * src/core/model/dilution.cpp:44-67) without the row-pointer array.
  *
  * Threading: one host thread per lattice handle at a time; distinct handles may be
- * used from distinct threads. The library holds no mutable global state other than
- * the per-thread last-error string.
+ * used from distinct threads. Process-wide state (mutex-guarded, never torn down): the
+ * per-thread last-error string, a cache of released lattices for re-use by later
+ * creations of the same shape and storage (1/8 of device memory by default; the
+ * environment variable BGT_HIP_LATTICE_CACHE_BYTES sets the budget, 0 disables it),
+ * pools of stream-ordered device memory, pinned host blocks and streams, and the host
+ * threads of the exact rescoring.
  */
 #ifndef BGT_HIP_H
 #define BGT_HIP_H
@@ -67,9 +71,12 @@ typedef struct bgt_hip_tree_stats {
  * and, for binary non-dilution lattices, is re-decided by exact FP64 upset masses
  * recomputed from the prior and the update history over every candidate whose objective
  * lies within the error band (redecided = 1; 2 when the band held more candidates than
- * were collected and only the top two were rescored). After a re-decision, objective,
- * runner_up, runner_up_objective and gap_bound are the exact-rescoring values and
- * near_tie reports whether even those are within FP64 rounding. */
+ * were collected and only the top two were rescored). Candidates that a one-subject
+ * extension or reduction in the band beats for certain are not rescored; when a single
+ * candidate remains, it is the exact best and objective keeps the device's value.
+ * Otherwise, after a re-decision, objective, runner_up, runner_up_objective and gap_bound
+ * are the exact-rescoring values and near_tie reports whether even those are within FP64
+ * rounding. */
 typedef struct bgt_hip_selection {
 	uint32_t experiment;
 	uint32_t runner_up;
