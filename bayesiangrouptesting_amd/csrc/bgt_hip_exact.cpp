@@ -65,7 +65,7 @@ private:
 	Pool()
 	{
 		const unsigned hw = std::thread::hardware_concurrency();
-		const int extra = (int)std::min(7u, hw > 1 ? hw - 1 : 0u);
+		const int extra = (int)std::min(15u, hw > 1 ? hw - 1 : 0u);
 		for (int w = 1; w <= extra; ++w)
 			threads_.emplace_back([this, w] { loop(w); });
 	}
@@ -106,6 +106,9 @@ private:
 };
 
 } // namespace
+
+void parallel_for(size_t n, const std::function<void(size_t, int)> &f) { Pool::get().run(n, f); }
+int parallel_width() { return Pool::get().size(); }
 
 void ExactHistory::Table::clear(size_t expect, int pools)
 {

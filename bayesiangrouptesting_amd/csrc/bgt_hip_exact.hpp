@@ -18,9 +18,15 @@
 #pragma once
 
 #include <cstdint>
+#include <functional>
 #include <vector>
 
 namespace bgt_hip {
+
+// f(i, worker) for i in [0, n) on the host threads of the exact rescoring (up to 16, the
+// calling thread included); returns when all are done. worker < parallel_width().
+void parallel_for(size_t n, const std::function<void(size_t, int)> &f);
+int parallel_width();
 
 class ExactHistory {
 public:
@@ -37,6 +43,21 @@ public:
 	bool masses(const uint32_t *experiments, size_t count, double *out, bool *ok);
 	// relative rounding bound of mass() (a few ulp per atom and per update)
 	double rel_bound() const;
+	// update t of the history: its pool and its factors when contained / not contained
+	void record(int t, uint32_t &experiment, double &f_in, double &f_out) const
+	{
+		experiment = hist_[t].e;
+		f_in = hist_[t].f_in;
+		f_out = hist_[t].f_out;
+	}
+	// U(E) / U({}) for an unnormalised mass computed elsewhere; false without a normaliser
+	bool normalise(double u, double &out)
+	{
+		if (!usable() || !normaliser())
+			return false;
+		out = u / z_;
+		return true;
+	}
 	// compute the normaliser of the current history now (host work that can overlap with
 	// device work; mass() would otherwise do it on first use)
 	void prepare()

@@ -67,6 +67,8 @@ struct Cand {
 	uint32_t flags;
 };
 constexpr uint32_t kCandSided = 2u;
+// host side only: band index + 1 of a candidate from here up in flags
+constexpr int kCandIndexShift = 8;
 // Candidates within the error band of the best objective, collected for exact rescoring:
 // up to kBandSmall of them for lattices of at most 20 atoms, kBandLarge above.
 constexpr int kBandSmall = 4096;
@@ -79,6 +81,10 @@ constexpr int kBandHead = 64;
 // Bytes at the start of every lattice's arena that one result copy brings back: scalars,
 // selection result, counters, atom masses and the first kBandHead band candidates.
 constexpr size_t kHeadBytes = 1024 + kBandHead * sizeof(Cand);
+// A single lattice with exact rescoring brings back a wider head, with the first
+// kBandHeadWide band candidates, so that a band of ordinary size needs no second copy.
+constexpr int kBandHeadWide = 1024;
+constexpr size_t kHeadBytesWide = 1024 + kBandHeadWide * sizeof(Cand);
 
 // Everything a kernel needs to know about one lattice. Arrays of these live in device
 // memory so that one launch can serve a batch of lattices (blockIdx.y selects).
@@ -109,7 +115,29 @@ struct LatDesc {
 	const void *src_post;
 	const double *src_scal;
 	int src_inv;
+	// exact rescoring on the device (k_exact_band): the update history, the prior's
+	// (pi_a, 1 - pi_a) pairs and the output array
+	const struct ExactRec *hist;
+	int hist_len;
+	const double *pairs;
+	double *exact_u;
 };
+
+// One update of a binary non-dilution lattice for the exact program: the pool and the
+// likelihood factors when the pool is / is not contained in the state.
+struct ExactRec {
+	uint32_t e;
+	uint32_t pad;
+	double fin, fout;
+};
+// Device exact rescoring covers the first kExactBand band candidates, histories of at
+// most kExactMaxHist updates and candidates whose remaining pools number at most
+// kExactMaxPools; the host program takes the rest.
+constexpr int kExactBand = 1024;
+constexpr int kExactMaxHist = 64;
+constexpr int kExactMaxPools = 10;
+// the exact band kernel stages one update and one prior value per lane of its wavefront
+static_assert(kExactMaxHist <= 64 && 2 * kMaxAtoms <= 64, "exact band staging is one wavefront wide");
 
 constexpr int kLikCapacity = kMaxVariants * (kMaxSubjects + 1);
 
@@ -163,6 +191,10 @@ void launch_band_collect(const Descs &descs, int count, unsigned entries, int n,
 void launch_gather_heads(const LatDesc *many, int count, unsigned char *out, hipStream_t stream);
 void launch_bbpa_score(const Descs &descs, int count, int n, int k, double target,
 					   hipStream_t stream);
+// Unnormalised exact upset masses U(E) of the first kExactBand band candidates of each
+// lattice (d.exact_u[i], -1 where the device program does not apply), from d.pairs and
+// d.hist: the quantity ExactHistory computes on the host, as a dense dynamic program.
+void launch_exact_band(const Descs &descs, int count, int atoms, hipStream_t stream);
 void launch_brute_force(const Descs &descs, int count, int n, int k, Storage st,
 						hipStream_t stream);
 void launch_atom_masses(const Descs &descs, int count, int atoms, Storage st, hipStream_t stream);

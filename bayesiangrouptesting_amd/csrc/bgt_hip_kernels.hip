@@ -1536,6 +1536,147 @@ __global__ __launch_bounds__(256) void k_band_compact(Descs descs, unsigned raw_
 	}
 }
 
+// Exact unnormalised upset masses of band candidates, one wave per candidate:
+//   U(E) = Q(E) * prod_{t : E_t <= E} f_t(in) * sum over the free atoms of the factors of
+//          the pools A_t = E_t \ E that remain
+// (the program of the host's ExactHistory, replicated_lattice.cpp:8-28 likelihoods), as a
+// dense dynamic program over the 2^c subsets of pools that can still be contained: atom a
+// negative keeps the state (weight 1 - pi_a), positive removes its pools (weight pi_a); a
+// pool's factor is applied, and its bit cleared, after its last atom. Both steps run as
+// sweeps over one pool bit at a time, 2^(c-1) independent pairs per sweep spread over the
+// wavefront. Every term is non-negative and every sum runs in a fixed order. Candidates
+// with more than kExactMaxPools distinct pools, or histories longer than kExactMaxHist,
+// get -1.
+__global__ __launch_bounds__(64) void k_exact_band(Descs descs, int atoms)
+{
+	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
+	if (d.exact_u == nullptr || d.band == nullptr)
+		return;
+	const unsigned count = min(d.counter[0], d.band_cap);
+	const unsigned i = blockIdx.x;
+	if (i >= count || i >= (unsigned)kExactBand)
+		return;
+	__shared__ double a[1 << kExactMaxPools];
+	__shared__ double b[1 << kExactMaxPools];
+	__shared__ double s_pairs[2 * kMaxAtoms];
+	__shared__ uint32_t s_rest[kExactMaxHist];
+	__shared__ double s_hin[kExactMaxHist], s_hout[kExactMaxHist];
+	__shared__ uint32_t s_pool[kExactMaxPools];
+	__shared__ double s_fin[kExactMaxPools], s_fout[kExactMaxPools];
+	const int lane = threadIdx.x; // one wavefront: lane t stages update t and pair t
+	const int h = d.hist_len;
+	const uint32_t e = d.band[i].e;
+	if (h > kExactMaxHist) {
+		if (lane == 0)
+			d.exact_u[i] = -1.0;
+		return;
+	}
+	if (lane < 2 * kMaxAtoms)
+		s_pairs[lane] = lane < 2 * atoms ? d.pairs[lane] : 1.0;
+	uint32_t rest = 0;
+	double fin = 1.0, fout = 1.0;
+	if (lane < h) {
+		const ExactRec r = d.hist[lane];
+		rest = r.e & ~e;
+		fin = r.fin;
+		fout = r.fout;
+	}
+	s_rest[lane] = rest;
+	s_hin[lane] = fin;
+	s_hout[lane] = fout;
+	__syncthreads();
+	// Updates whose pool lies inside E always see it contained (factor fin); the others
+	// group by their remaining atoms, one pool per distinct remainder in order of first
+	// occurrence, with the factors of the group multiplied in update order.
+	bool first = lane < h && rest != 0u;
+	for (int t = 0; t < lane && first; ++t)
+		first = s_rest[t] != rest;
+	if (first) {
+		for (int t = lane + 1; t < h; ++t)
+			if (s_rest[t] == rest) {
+				fin *= s_hin[t];
+				fout *= s_hout[t];
+			}
+	}
+	const unsigned long long firsts = __ballot(first);
+	const int c = __popcll(firsts);
+	if (c > kExactMaxPools) {
+		if (lane == 0)
+			d.exact_u[i] = -1.0;
+		return;
+	}
+	if (first) {
+		const int j = __popcll(firsts & ((1ull << lane) - 1ull));
+		s_pool[j] = rest;
+		s_fin[j] = fin;
+		s_fout[j] = fout;
+	}
+	double lead = 1.0; // Q(E) times the always-contained factors, in the host's order
+	if (lane == 0) {
+		for (int at = 0; at < atoms; ++at)
+			if ((e >> at) & 1u)
+				lead *= s_pairs[2 * at + 1];
+		for (int t = 0; t < h; ++t)
+			if (s_rest[t] == 0u)
+				lead *= s_hin[t];
+	}
+	const uint32_t states = 1u << c;
+	for (uint32_t k = lane; k < states; k += 64)
+		a[k] = k == states - 1 ? 1.0 : 0.0;
+	__syncthreads();
+	uint32_t relevant = 0;
+	for (int j = 0; j < c; ++j)
+		relevant |= s_pool[j];
+	uint32_t done = 0;
+	for (uint32_t left = relevant; left; left &= left - 1u) {
+		const int at = __ffs(left) - 1;
+		done |= 1u << at;
+		uint32_t kill = 0, close = 0;
+		for (int j = 0; j < c; ++j)
+			if ((s_pool[j] >> at) & 1u) {
+				kill |= 1u << j;
+				if ((s_pool[j] & ~done) == 0)
+					close |= 1u << j;
+			}
+		const double p = s_pairs[2 * at], q = s_pairs[2 * at + 1];
+		const uint32_t half = states >> 1;
+		// a[k] <- a[k] q + [k has no kill bit] p sum_{sub <= kill} a[k | sub]: the sum
+		// projects the kill bits out of a copy, one pool bit per sweep
+		for (uint32_t k = lane; k < states; k += 64)
+			b[k] = a[k];
+		__syncthreads();
+		for (uint32_t bits = kill; bits; bits &= bits - 1u) {
+			const int j = __ffs(bits) - 1;
+			const uint32_t low = (1u << j) - 1u;
+			for (uint32_t x = lane; x < half; x += 64) {
+				const uint32_t k = ((x & ~low) << 1) | (x & low); // bit j clear
+				b[k] += b[k | (1u << j)];
+				b[k | (1u << j)] = 0.0;
+			}
+			__syncthreads();
+		}
+		for (uint32_t k = lane; k < states; k += 64)
+			a[k] = a[k] * q + b[k] * p;
+		__syncthreads();
+		// close the pools whose last atom this was, one per sweep: the state without the
+		// pool's bit takes fout times its weight plus fin times the weight with it
+		for (uint32_t bits = close; bits; bits &= bits - 1u) {
+			const int j = __ffs(bits) - 1;
+			const uint32_t low = (1u << j) - 1u;
+			const double fi = s_fin[j], fo = s_fout[j];
+			for (uint32_t x = lane; x < half; x += 64) {
+				const uint32_t k = ((x & ~low) << 1) | (x & low);
+				a[k] = a[k] * fo + a[k | (1u << j)] * fi;
+				a[k | (1u << j)] = 0.0;
+			}
+			__syncthreads();
+		}
+	}
+	// every pool has closed: the weight sits in state 0
+	if (lane == 0)
+		d.exact_u[i] = lead * a[0];
+}
+
 // Result heads of a batch of lattices into one staging buffer (one copy to the host).
 __global__ __launch_bounds__(256) void k_gather_heads(const LatDesc *many, unsigned char *out)
 {
@@ -1783,6 +1924,13 @@ void launch_band_collect(const Descs &descs, int count, unsigned entries, int n,
 }
 
 
+
+void launch_exact_band(const Descs &descs, int count, int atoms, hipStream_t stream)
+{
+	launch_checked("k_exact_band", [&] {
+		hipLaunchKernelGGL(k_exact_band, dim3(kExactBand, count), dim3(64), 0, stream, descs, atoms);
+	});
+}
 
 void launch_bbpa_score(const Descs &descs, int count, int n, int k, double target, hipStream_t stream)
 {

@@ -144,8 +144,9 @@ struct HostBox {
 	double scal[kScalars];
 	double value;
 	double raw;
-	alignas(256) unsigned char head[kHeadBytes];
-	Cand band[kBandHead];
+	ExactRec hist[kExactMaxHist];   // staging of the update history for the device
+	double exact_u[kExactBand];     // exact masses of band candidates from the device
+	alignas(256) unsigned char head[kHeadBytesWide];
 };
 
 } // namespace
@@ -173,12 +174,15 @@ void DeviceLattice::init_prior(const double *prior)
 	pos_ = neg_ = 0;
 	active_ = (n_ >= 32) ? 0xFFFFFFFFu : ((1u << n_) - 1u);
 	updates_ = 0;
+	hist_uploaded_ = 0;
+	exact_pending_ = exact_valid_ = false;
 	inv_cur_ = kScalInv0;
 	e_ = r_ = 0;
 	len_ = 0;
 	clock_.reset();
 	clock_.enabled = false;
 	exact_.reset(k_ == 1 && !dilution_, atoms_, prior);
+	pairs_own_ = true;
 	std::vector<double> pairs(2 * kMaxAtoms, 0.0);
 	for (int a = 0; a < atoms_; ++a) {
 		pairs[2 * a] = prior[a];
@@ -303,7 +307,7 @@ DeviceLattice::DeviceLattice(const DeviceLattice &parent, bool copy_posterior)
 namespace {
 
 struct ArenaLayout {
-	size_t scal, res, cnt, mass, band, cand, flags, pairs, part, tops, ftab, scores, sel, total;
+	size_t scal, res, cnt, mass, band, cand, flags, hist, exact_u, pairs, part, tops, ftab, scores, sel, total;
 	bool exact;
 };
 
@@ -334,13 +338,15 @@ ArenaLayout arena_layout(bool dilution, int n, int k)
 	a.band = take(a.exact ? sizeof(Cand) * band_capacity(atoms) : 0); // 1024
 	a.cand = take(a.exact ? sizeof(Cand) * 2 * kTileThreads * zeta_final_groups(atoms) : 0);
 	a.flags = take(a.exact ? sizeof(unsigned) * kMaxFinalGroups : 0);
+	a.hist = take(a.exact ? sizeof(ExactRec) * kExactMaxHist : 0);
+	a.exact_u = take(a.exact ? sizeof(double) * kExactBand : 0);
 	a.pairs = take(sizeof(double) * 2 * kMaxAtoms);
 	a.part = take(sizeof(double) * part);
 	a.tops = take(sizeof(SelResult) * tops);
 	a.ftab = take(sizeof(double) * ftab);
 	a.scores = take(sizeof(double) * scores);
 	a.sel = take(sizeof(uint32_t) * 16);
-	a.total = std::max<size_t>(off, kHeadBytes);
+	a.total = std::max<size_t>(off, a.exact ? kHeadBytesWide : kHeadBytes);
 	return a;
 }
 
@@ -376,6 +382,8 @@ void DeviceLattice::allocate()
 		band_ = reinterpret_cast<Cand *>(arena_ + a.band);
 		cand_ = reinterpret_cast<Cand *>(arena_ + a.cand);
 		wgflag_ = reinterpret_cast<unsigned *>(arena_ + a.flags);
+		hist_dev_ = reinterpret_cast<ExactRec *>(arena_ + a.hist);
+		exact_dev_ = reinterpret_cast<double *>(arena_ + a.exact_u);
 		band_cap_ = (unsigned)band_capacity(atoms_);
 		band_host_ = static_cast<Cand *>(pinned_alloc(sizeof(Cand) * band_cap_));
 	}
@@ -420,6 +428,9 @@ void DeviceLattice::adopt(const DeviceLattice &parent)
 	updates_ = parent.updates_;
 	inv_cur_ = parent.inv_cur_;
 	exact_ = parent.exact_;
+	hist_uploaded_ = 0;
+	exact_pending_ = exact_valid_ = false;
+	pairs_own_ = false; // the device prior pairs are those this lattice was created with
 }
 
 void DeviceLattice::as_source(LatDesc &d) const
@@ -458,6 +469,10 @@ LatDesc DeviceLattice::desc() const
 	d.band_width = 0.0;
 	d.wgflag = wgflag_;
 	d.band_cap = band_cap_;
+	d.hist = hist_dev_;
+	d.hist_len = exact_.updates();
+	d.pairs = pairs_;
+	d.exact_u = nullptr; // set by the launches that want exact masses
 	return d;
 }
 
@@ -561,23 +576,68 @@ void DeviceLattice::check_selector(int selector) const
 void DeviceLattice::fetch_head()
 {
 	HostBox *box = static_cast<HostBox *>(host_box_);
-	BGT_HIP_CHECK(hipMemcpyAsync(box->head, arena_, kHeadBytes, hipMemcpyDeviceToHost, stream_));
+	// lattices with an error band (their arena holds at least kHeadBytesWide) bring back
+	// the wide head, and the band's exact masses when they were computed
+	const bool wide = band_host_ != nullptr;
+	BGT_HIP_CHECK(hipMemcpyAsync(box->head, arena_, wide ? kHeadBytesWide : kHeadBytes, hipMemcpyDeviceToHost,
+								 stream_));
+	if (exact_pending_)
+		BGT_HIP_CHECK(hipMemcpyAsync(box->exact_u, exact_dev_, sizeof(double) * kExactBand, hipMemcpyDeviceToHost,
+									 stream_));
 	sync();
-	load_head(box->head);
+	exact_valid_ = exact_pending_;
+	exact_pending_ = false;
+	unpack_head(wide ? kBandHeadWide : kBandHead);
 }
 
 void DeviceLattice::load_head(const unsigned char *head)
 {
 	HostBox *box = static_cast<HostBox *>(host_box_);
-	if (head != box->head)
-		std::memcpy(box->head, head, kHeadBytes);
+	std::memcpy(box->head, head, kHeadBytes);
+	exact_valid_ = exact_pending_ = false;
+	unpack_head(kBandHead);
+}
+
+void DeviceLattice::launch_exact(const Descs &ds_in)
+{
+	exact_pending_ = false;
+	if (exact_dev_ == nullptr || !pairs_own_ || !exact_.usable() || exact_.updates() > kExactMaxHist)
+		return;
+	HostBox *box = static_cast<HostBox *>(host_box_);
+	const int n = exact_.updates();
+	if (hist_uploaded_ < n) {
+		for (int t = hist_uploaded_; t < n; ++t) {
+			ExactRec &rec = box->hist[t];
+			exact_.record(t, rec.e, rec.fin, rec.fout);
+			rec.pad = 0;
+		}
+		BGT_HIP_CHECK(hipMemcpyAsync(hist_dev_ + hist_uploaded_, box->hist + hist_uploaded_,
+									 sizeof(ExactRec) * (n - hist_uploaded_), hipMemcpyHostToDevice, stream_));
+		hist_uploaded_ = n;
+	}
+	Descs ds = ds_in;
+	ds.one.hist = hist_dev_;
+	ds.one.hist_len = n;
+	ds.one.pairs = pairs_;
+	ds.one.exact_u = exact_dev_;
+	clock_.begin(stream_);
+	launch_exact_band(ds, 1, atoms_, stream_);
+	clock_.end(stream_, "exact_band");
+	exact_pending_ = true;
+}
+
+void DeviceLattice::unpack_head(int band_entries)
+{
+	HostBox *box = static_cast<HostBox *>(host_box_);
 	std::memcpy(box->scal, box->head, sizeof(double) * kScalars);
 	std::memcpy(&box->result, box->head + kHeadResult, sizeof(SelResult));
 	std::memcpy(box->counter, box->head + kHeadCounter, sizeof(box->counter));
 	std::memcpy(box->masses, box->head + kHeadMasses, sizeof(double) * 32);
-	std::memcpy(box->band, box->head + kHeadBand, sizeof(Cand) * kBandHead);
-	if (band_host_)
-		std::memcpy(band_host_, box->band, sizeof(Cand) * kBandHead);
+	head_band_ = 0;
+	if (band_host_) {
+		head_band_ = (unsigned)std::min<unsigned>({(unsigned)band_entries, box->counter[0], band_cap_});
+		std::memcpy(band_host_, box->head + kHeadBand, sizeof(Cand) * head_band_);
+	}
 }
 
 SelResult DeviceLattice::fetch_result()
@@ -714,20 +774,29 @@ void DeviceLattice::rescore(bgt_hip_selection &out, int sel, const SelResult &r)
 	bool complete = cand_ != nullptr && count <= band_cap_;
 	std::vector<Cand> cands;
 	if (complete) {
-		const unsigned inhead = std::min<unsigned>(count, kBandHead);
+		const unsigned inhead = std::min<unsigned>(count, head_band_);
 		if (count > inhead) {
 			BGT_HIP_CHECK(hipMemcpyAsync(band_host_ + inhead, band_ + inhead, sizeof(Cand) * (count - inhead),
 										 hipMemcpyDeviceToHost, stream_));
 			sync();
 		}
 		cands.assign(band_host_, band_host_ + count);
+		if (exact_valid_) {
+			// band entry i carries i + 1 in the upper flag bits: its exact mass came back
+			// from the device when box->exact_u[i] is not negative
+			const unsigned tagged = std::min<unsigned>(count, kExactBand);
+			for (unsigned i = 0; i < tagged; ++i)
+				cands[i].flags |= (i + 1u) << kCandIndexShift;
+		}
 	}
 	for (const Cand c : {Cand{r.d1, r.e1, 0}, Cand{r.d2, r.e2, 0}})
 		if (c.e != 0xFFFFFFFFu)
 			cands.push_back(c);
-	// one entry per experiment (a rescan repeats what the lists held), signed ones first
+	// one entry per experiment (a rescan repeats what the lists held): signed ones first,
+	// then those with a device mass
+	auto rank = [](const Cand &c) { return (c.flags & kCandSided) * 2u + ((c.flags >> kCandIndexShift) != 0u); };
 	std::sort(cands.begin(), cands.end(),
-			  [](const Cand &a, const Cand &b) { return a.e < b.e || (a.e == b.e && a.flags > b.flags); });
+			  [&](const Cand &a, const Cand &b) { return a.e < b.e || (a.e == b.e && rank(a) > rank(b)); });
 	cands.erase(std::unique(cands.begin(), cands.end(), [](const Cand &a, const Cand &b) { return a.e == b.e; }),
 				cands.end());
 	auto side_of = [&](const Cand &c) {
@@ -735,23 +804,39 @@ void DeviceLattice::rescore(bgt_hip_selection &out, int sel, const SelResult &r)
 			return 0;
 		return c.d > 0.0 ? 1 : -1;
 	};
-	SideTable sides(cands.size());
+	size_t sided = 0;
 	for (const Cand &c : cands)
-		sides.put(c.e, side_of(c));
+		sided += side_of(c) != 0;
 	std::vector<Cand> live;
-	live.reserve(cands.size());
-	for (const Cand &c : cands) {
-		const int sd = side_of(c);
-		bool dominated = false;
-		if (sd > 0) {
-			for (uint32_t rest = active_ & ~c.e; rest && !dominated; rest &= rest - 1u)
-				dominated = sides.side(c.e | (rest & (0u - rest))) > 0;
-		} else if (sd < 0) {
-			for (uint32_t rest = c.e; rest && !dominated; rest &= rest - 1u)
-				dominated = sides.side(c.e & ~(rest & (0u - rest))) < 0;
-		}
-		if (!dominated)
-			live.push_back(c);
+	if (sided < 2) {
+		live = std::move(cands); // no candidate can dominate another
+	} else {
+		SideTable sides(cands.size());
+		for (const Cand &c : cands)
+			sides.put(c.e, side_of(c));
+		std::vector<char> dominated(cands.size(), 0);
+		auto check = [&](size_t i) {
+			const Cand &c = cands[i];
+			const int sd = side_of(c);
+			bool dom = false;
+			if (sd > 0) {
+				for (uint32_t rest = active_ & ~c.e; rest && !dom; rest &= rest - 1u)
+					dom = sides.side(c.e | (rest & (0u - rest))) > 0;
+			} else if (sd < 0) {
+				for (uint32_t rest = c.e; rest && !dom; rest &= rest - 1u)
+					dom = sides.side(c.e & ~(rest & (0u - rest))) < 0;
+			}
+			dominated[i] = dom;
+		};
+		if (cands.size() >= 256)
+			parallel_for(cands.size(), [&](size_t i, int) { check(i); });
+		else
+			for (size_t i = 0; i < cands.size(); ++i)
+				check(i);
+		live.reserve(cands.size());
+		for (size_t i = 0; i < cands.size(); ++i)
+			if (!dominated[i])
+				live.push_back(cands[i]);
 	}
 	if (complete && live.size() == 1) {
 		// every other candidate loses to one that beats it strictly: the survivor is the
@@ -777,12 +862,23 @@ void DeviceLattice::rescore(bgt_hip_selection &out, int sel, const SelResult &r)
 	uint32_t e1 = 0xFFFFFFFFu, e2 = 0xFFFFFFFFu;
 	int rescored = 0;
 	constexpr int kMaxRescored = 4096;
-	// The scan below is sequential; the exact masses it needs are computed ahead in chunks
-	// (in parallel), so the result is that of the sequential scan.
+	// The scan below is sequential; the exact masses it needs are computed ahead, in
+	// parallel, so the result is that of the sequential scan. It never goes past a device
+	// objective of |d_2| + 2 eps (d_2: the second candidate's): the exact objectives of the
+	// first two candidates are at most |d_2| + eps, so the exact runner-up is too, and the
+	// scan stops at the first candidate whose lower bound |d| - eps exceeds it. Those masses
+	// are computed in one parallel run; later chunks are only a guard.
 	std::vector<uint32_t> ex;
 	std::vector<double> mass;
 	std::unique_ptr<bool[]> okv;
 	size_t ready = 0, chunk = 64;
+	if (live.size() >= 2) {
+		const double reach = std::fabs(live[1].d) + 2.0 * eps;
+		size_t need = 0;
+		while (need < live.size() && need < (size_t)kMaxRescored && !(std::fabs(live[need].d) > reach))
+			++need;
+		chunk = std::max<size_t>(need, 1);
+	}
 	for (size_t i = 0; i < live.size(); ++i) {
 		const Cand &c = live[i];
 		if (std::fabs(c.d) - eps > b2)
@@ -799,14 +895,32 @@ void DeviceLattice::rescore(bgt_hip_selection &out, int sel, const SelResult &r)
 			ex.resize(j - i);
 			mass.resize(j - i);
 			okv.reset(new bool[j - i]);
-			for (size_t q = i; q < j; ++q)
-				ex[q - i] = live[q].e;
-			if (!exact_.masses(ex.data(), ex.size(), mass.data(), okv.get()))
-				return; // the normaliser is beyond the exact program's budget: keep the device result
+			// masses the device computed are normalised here; the rest run on the host
+			std::vector<size_t> host;
+			for (size_t q = i; q < j; ++q) {
+				const uint32_t tag = live[q].flags >> kCandIndexShift;
+				const double u = tag ? box->exact_u[tag - 1] : -1.0;
+				if (u >= 0.0 && exact_.normalise(u, mass[q - i])) {
+					okv[q - i] = true;
+				} else {
+					ex[host.size()] = live[q].e;
+					host.push_back(q - i);
+				}
+			}
+			if (!host.empty()) {
+				std::vector<double> hm(host.size());
+				std::unique_ptr<bool[]> hok(new bool[host.size()]);
+				if (!exact_.masses(ex.data(), host.size(), hm.data(), hok.get()))
+					return; // the normaliser is beyond the exact program's budget: keep the device result
+				for (size_t h = 0; h < host.size(); ++h) {
+					mass[host[h]] = hm[h];
+					okv[host[h]] = hok[h];
+				}
+			}
 			ready = j;
 			chunk *= 2;
 		}
-		const size_t slot = i - (ready - ex.size());
+		const size_t slot = i - (ready - mass.size());
 		if (!okv[slot])
 			return; // beyond the exact program's budget: keep the device result
 		const double m = mass[slot];
@@ -882,6 +996,8 @@ bgt_hip_selection DeviceLattice::select(int selector)
 		clock_.begin(stream_);
 		launch_bbpa_score(dz, 1, n_, k_, target, stream_);
 		clock_.end(stream_, "bbpa_score");
+	} else {
+		launch_exact(dz);
 	}
 	exact_.prepare(); // on the host while the device selects
 	fetch_head();
@@ -1033,6 +1149,7 @@ bgt_hip_selection DeviceLattice::round(uint32_t experiment, uint32_t response, c
 	const unsigned entries = launch_zeta_final(ds, 1, atoms_, k_, storage_, kObjHalving, false, true, 0.5, stream_);
 	clock_.end(stream_, "round_pass2");
 	collect_band(ds, entries, kObjHalving, false, true, 0.5);
+	launch_exact(ds);
 	exact_.prepare(); // on the host while the device runs the round
 	fetch_head();
 	classify_from_head(up, lo, masses_out);

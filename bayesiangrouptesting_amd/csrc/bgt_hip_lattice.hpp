@@ -115,7 +115,7 @@ public:
 	// error band of the selection objective (device objectives closer than this may be
 	// misordered)
 	double band_width(int selector) const;
-	// take a result head gathered by a batched step (the same bytes fetch_head copies)
+	// take a result head gathered by a batched step (kHeadBytes of the arena's start)
 	void load_head(const unsigned char *head);
 	// sticky classification from the masses of the last head (lattice_base.cpp:246-264)
 	void classify_from_head(double threshold_up, double threshold_lo, double *masses_out);
@@ -137,6 +137,10 @@ private:
 	void init_prior(const double *prior);
 	void fill_lik(uint32_t experiment, uint32_t response, const double *dilution_flat);
 	void rescore(bgt_hip_selection &out, int selector, const SelResult &r);
+	// exact masses of the band on the device (binary non-dilution), after the band kernels
+	void launch_exact(const Descs &ds);
+	// the fields of the head now in the host mailbox, with up to band_entries band candidates
+	void unpack_head(int band_entries);
 	// complete band of candidates for exact rescoring (binary non-dilution lattices); the
 	// arguments after `entries` are those of the final transform launch
 	void collect_band(const Descs &ds, unsigned entries, int objective, bool from_posterior, bool scaled,
@@ -167,6 +171,13 @@ private:
 	unsigned *wgflag_ = nullptr; // workgroups of the final transform to rescan
 	unsigned band_cap_ = 0;
 	Cand *band_host_ = nullptr; // pinned host copy of the band
+	unsigned head_band_ = 0;    // band candidates that came back with the last head
+	ExactRec *hist_dev_ = nullptr; // update history for the device exact program
+	double *exact_dev_ = nullptr;  // its exact masses of the first kExactBand band candidates
+	int hist_uploaded_ = 0;        // history records already on the device
+	bool exact_pending_ = false;   // exact masses were launched for the band being fetched
+	bool exact_valid_ = false;     // the host mailbox holds them for the current band
+	bool pairs_own_ = false;       // the prior pairs on the device are this lattice's own
 	ExactHistory exact_;
 
 	// pinned host mailbox for results
