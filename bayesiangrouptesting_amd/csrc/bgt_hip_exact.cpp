@@ -231,8 +231,8 @@ bool ExactHistory::unnormalised(uint32_t e, double &out, Scratch &sc) const
 		}
 	}
 	const size_t c = sc.pool.size();
-	if (c == 0) {
-		out = qe * always;
+	if (c <= (size_t)kDenseMaxPools && hist_.size() <= (size_t)kDenseMaxHist) {
+		out = dense(e, sc);
 		return true;
 	}
 	if (c > 64)
@@ -284,6 +284,66 @@ bool ExactHistory::unnormalised(uint32_t e, double &out, Scratch &sc) const
 		sum += cur->w[s];
 	out = qe * always * sum;
 	return true;
+}
+
+double ExactHistory::dense(uint32_t e, Scratch &sc) const
+{
+	// Q(E) and the always-contained factors, multiplied in atom and then update order
+	double lead = 1.0;
+	for (int a = 0; a < atoms_; ++a)
+		if ((e >> a) & 1u)
+			lead *= q_[a];
+	for (const Upd &u : hist_)
+		if ((u.e & ~e) == 0u)
+			lead *= u.f_in;
+	const int c = (int)sc.pool.size();
+	const uint32_t states = 1u << c, half = states >> 1;
+	std::vector<double> &A = sc.da, &B = sc.db;
+	A.assign(states, 0.0);
+	A[states - 1] = 1.0;
+	B.resize(states);
+	uint32_t relevant = 0;
+	for (uint32_t p : sc.pool)
+		relevant |= p;
+	// atoms in increasing order; state bit j set while pool j can still be contained
+	uint32_t done = 0;
+	for (uint32_t left = relevant; left; left &= left - 1u) {
+		const int at = __builtin_ctz(left);
+		done |= 1u << at;
+		uint32_t kill = 0, close = 0;
+		for (int j = 0; j < c; ++j)
+			if ((sc.pool[j] >> at) & 1u) {
+				kill |= 1u << j;
+				if ((sc.pool[j] & ~done) == 0u)
+					close |= 1u << j;
+			}
+		const double p = pi_[at], q = q_[at];
+		// atom positive: its pools are out (the kill bits projected away, one per sweep)
+		std::copy(A.begin(), A.end(), B.begin());
+		for (uint32_t bits = kill; bits; bits &= bits - 1u) {
+			const int j = __builtin_ctz(bits);
+			const uint32_t low = (1u << j) - 1u;
+			for (uint32_t x = 0; x < half; ++x) {
+				const uint32_t k = ((x & ~low) << 1) | (x & low);
+				B[k] += B[k | (1u << j)];
+				B[k | (1u << j)] = 0.0;
+			}
+		}
+		for (uint32_t k = 0; k < states; ++k)
+			A[k] = A[k] * q + B[k] * p;
+		// pools whose last atom this was take their factor and leave the state
+		for (uint32_t bits = close; bits; bits &= bits - 1u) {
+			const int j = __builtin_ctz(bits);
+			const uint32_t low = (1u << j) - 1u;
+			const double fi = sc.fin[j], fo = sc.fout[j];
+			for (uint32_t x = 0; x < half; ++x) {
+				const uint32_t k = ((x & ~low) << 1) | (x & low);
+				A[k] = A[k] * fo + A[k | (1u << j)] * fi;
+				A[k | (1u << j)] = 0.0;
+			}
+		}
+	}
+	return lead * A[0];
 }
 
 void ExactHistory::plan_order()

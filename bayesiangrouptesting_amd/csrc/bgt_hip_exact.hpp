@@ -23,6 +23,12 @@
 
 namespace bgt_hip {
 
+// Candidates whose remaining pools number at most kDenseMaxPools, under histories of at
+// most kDenseMaxHist updates, run the dense program: the same operations in the same
+// order as the device's band kernel, so host and device masses are bit-identical.
+constexpr int kDenseMaxPools = 10;
+constexpr int kDenseMaxHist = 64;
+
 // f(i, worker) for i in [0, n) on the host threads of the exact rescoring (up to 16, the
 // calling thread included); returns when all are done. worker < parallel_width().
 void parallel_for(size_t n, const std::function<void(size_t, int)> &f);
@@ -88,8 +94,10 @@ private:
 		std::vector<uint32_t> pool;
 		std::vector<double> fin, fout;
 		Table a, b;
+		std::vector<double> da, db; // dense program
 	};
 	bool unnormalised(uint32_t e, double &out, Scratch &sc) const;
+	double dense(uint32_t e, Scratch &sc) const; // sc.pool/fin/fout built by unnormalised
 	void plan_order(); // atom order of the program for the current history (normaliser())
 	std::vector<Scratch> workers_; // one per pool thread (masses())
 	bool normaliser();

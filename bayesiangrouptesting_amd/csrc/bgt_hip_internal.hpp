@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include "bgt_hip_exact.hpp"
+
 #include <cstddef>
 #include <cstdint>
 #include <stdexcept>
@@ -19,7 +21,9 @@ constexpr int kMaxSubjects = 30;
 constexpr int kTileBits = 15;
 // The transform intermediate z holds 2^(A-15) rows of 2^15 entries; rows are spaced by
 // 2^15 entries plus 128 bytes so that the strided 16-32 byte reads of pass 2 spread over
-// the memory channels instead of landing on the same one.
+// the memory channels instead of landing on the same one. (A layout holding each pass-2
+// tile contiguously turns those reads into runs of 64 entries but pass 1's stores into
+// 16-byte pieces; at 2^28 states it made pass 1 slower by more than it saved in pass 2.)
 constexpr int kZSkewBytes = 128;
 template <typename T>
 constexpr uint32_t z_pitch() { return (1u << kTileBits) + kZSkewBytes / sizeof(T); }
@@ -134,8 +138,8 @@ struct ExactRec {
 // most kExactMaxHist updates and candidates whose remaining pools number at most
 // kExactMaxPools; the host program takes the rest.
 constexpr int kExactBand = 1024;
-constexpr int kExactMaxHist = 64;
-constexpr int kExactMaxPools = 10;
+constexpr int kExactMaxHist = kDenseMaxHist;
+constexpr int kExactMaxPools = kDenseMaxPools;
 // the exact band kernel stages one update and one prior value per lane of its wavefront
 static_assert(kExactMaxHist <= 64 && 2 * kMaxAtoms <= 64, "exact band staging is one wavefront wide");
 

@@ -20,10 +20,19 @@ def _history(seed, n, steps):
     return [(g.below(1 << n), g.below(2)) for _ in range(steps)]
 
 
-@pytest.mark.parametrize("n,steps,seed", [(4, 3, 1), (8, 6, 2), (10, 10, 3), (12, 8, 4), (14, 12, 5)])
+# Up to 10 remaining pools under at most 64 updates the dense program runs (the device
+# kernel's); beyond either limit the sparse one. (14, 16) has more than 10 distinct pools
+# for most probes; the last case repeats 5 experiments 70 times (more than 64 updates).
+@pytest.mark.parametrize("n,steps,seed", [(4, 3, 1), (8, 6, 2), (10, 10, 3), (12, 8, 4), (14, 12, 5), (14, 16, 7),
+                                          (10, -70, 6)])
 def test_exact_mass_matches_oracle_posterior(n, steps, seed):
     pi = np.array(workload.priors(seed, n, 0.01, 0.3))
-    hist = _history(seed + 100, n, steps)
+    if steps > 0:
+        hist = _history(seed + 100, n, steps)
+    else:
+        few = _history(seed + 100, n, 5)
+        g = workload.SplitMix64(seed + 200)
+        hist = [(few[g.below(5)][0], g.below(2)) for _ in range(-steps)]
     post = orc.prior(pi)
     for e, r in hist:
         orc.update_in_place(post, False, n, 1, e, r)
