@@ -180,6 +180,9 @@ unsigned launch_zeta_small(const Descs &descs, int count, int n, int k, Storage 
 // workgroups of launch_zeta_final / launch_zeta_small for a lattice of `atoms` (1024 threads
 // each; the candidate buffer needs 2 entries per thread)
 unsigned zeta_final_groups(int atoms);
+// the final transform of `atoms` atoms is the persistent pass 2 (which collects its error
+// band in the pass, each state at most once)
+bool persistent_pass2(int atoms, int objective, bool from_posterior);
 // Candidates of the last zeta_final within d.band_width of the best: d.band[0..counter[0]),
 // complete unless counter[0] exceeds d.band_cap (entries may repeat). Pass 2 of 21..30
 // atoms appends candidates while it scans and this keeps those inside the final band. For

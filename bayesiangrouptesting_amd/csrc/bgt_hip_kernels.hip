@@ -29,6 +29,26 @@ namespace {
 
 constexpr unsigned kNone = 0xFFFFFFFFu;
 
+// A pointer every lane holds alike (a descriptor field), moved to scalar registers so
+// that accesses take the scalar-base form with a 32-bit lane offset.
+template <typename T>
+__device__ __forceinline__ T *uniform_ptr(T *p)
+{
+	const unsigned long long b = (unsigned long long)p;
+	const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b);
+	const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
+	return reinterpret_cast<T *>(((unsigned long long)hi << 32) | lo);
+}
+
+// A value every lane holds alike, moved to scalar registers.
+__device__ __forceinline__ double uniform(double x)
+{
+	const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+	const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b);
+	const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
+	return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // Lattice buffers arrive through a descriptor struct, so the compiler cannot infer their
 // address space; cast to global so loads and stores are global_* rather than flat_*.
 template <typename T>
@@ -509,36 +529,38 @@ __global__ __launch_bounds__(256) void k_update(Descs descs, uint64_t states)
 // update y' = x * L is applied first (stored back in place) and the transform runs on y'.
 // LIK: 0 = general table lookup, 1 = binary non-dilution (two factors chosen by E subset S).
 // The intermediate is written in state order.
-template <typename TP, typename TZ, bool FUSED, int LIK>
-__global__ __launch_bounds__(kTileThreads) void k_zeta_pass1(Descs descs)
+// The 32 posterior entries of one thread in pass-1 tile `tile` (load layout).
+template <typename TP>
+__device__ __forceinline__ void pass1_load(const LatDesc &d, unsigned tile, int lane, int wave, TP (&raw)[kTileRegs])
 {
-	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
-	__shared__ double lds[kTileLds];
-	__shared__ double red[16];
-	__shared__ double s_lik[kMaxVariants * (kMaxSubjects + 1)];
-	if (FUSED)
-		load_lik_table(descs, d, s_lik);
-	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-	const unsigned tiles = gridDim.x;
-	const unsigned tile = xcd_tile(blockIdx.x, tiles);
-	const uint64_t base = (uint64_t)tile << kTileBits;
-	if (blockIdx.x == 0 && d.band != nullptr) {
-		// the band collection of the pass 2 that follows
-		if (threadIdx.x < 8)
-			best_slots(d)[threadIdx.x] = kKeyInf;
-		else if (threadIdx.x == 8)
-			d.counter[kRawCount] = 0u;
-	}
-	// per-thread base pointers; the register index only adds compile-time offsets
-	gptr<TP> pa = G(static_cast<TP *>(d.post)) + base + lane + (wave << 10);
-	const double inv = d.scal[d.inv_in];
-	const double f_in = FUSED ? s_lik[0] : 0.0, f_out = FUSED ? s_lik[1] : 0.0;
-	double v[kTileRegs];
-	// issue every load of the tile before the first store (the fused update writes in place)
-	TP raw[kTileRegs];
+	// wave-uniform tile base plus one 32-bit lane offset: no per-register address VGPRs
+	const TP *tb = uniform_ptr(static_cast<const TP *>(d.post)) + ((uint64_t)tile << kTileBits);
+	const uint32_t lob = ((uint32_t)lane | ((uint32_t)wave << 10)) * (uint32_t)sizeof(TP);
 #pragma unroll
 	for (int j = 0; j < kTileRegs; ++j)
-		raw[j] = pa[((uint32_t)(j & 15) << 6) | ((uint32_t)(j >> 4) << 14)];
+		raw[j] = ld_u(tb + (((j & 15) << 6) | ((j >> 4) << 14)), lob);
+}
+
+// Pass-1 variants that run persistent with a prefetch of the next tile (FP32 storage, 32
+// registers per thread; not the general likelihood, whose table lookups leave too few).
+// The launcher picks the grid by the same rule.
+template <typename TP, bool FUSED, int LIK>
+constexpr bool pass1_persistent()
+{
+	return sizeof(TP) == 4 && (!FUSED || LIK == 1);
+}
+
+// One pass-1 tile from its loaded entries: the fused update (stored in place), the sweep
+// of bits 0..14, the intermediate z and the tile total.
+template <typename TP, typename TZ, bool FUSED, int LIK>
+__device__ __forceinline__ void pass1_tile(const LatDesc &d, unsigned tile, const TP (&raw)[kTileRegs], int lane,
+										   int wave, double inv, double f_in, double f_out, const double *s_lik,
+										   double *lds)
+{
+	const uint64_t base = (uint64_t)tile << kTileBits;
+	TP *tb = uniform_ptr(static_cast<TP *>(d.post)) + base;
+	const uint32_t lob = ((uint32_t)lane | ((uint32_t)wave << 10)) * (uint32_t)sizeof(TP);
+	double v[kTileRegs];
 #pragma unroll
 	for (int j = 0; j < kTileRegs; ++j) {
 		const uint32_t off = ((uint32_t)(j & 15) << 6) | ((uint32_t)(j >> 4) << 14);
@@ -551,19 +573,73 @@ __global__ __launch_bounds__(kTileThreads) void k_zeta_pass1(Descs descs)
 			else
 				lk = likelihood(g, d.e, d.n, d.k, d.len, d.dilution, s_lik);
 			const TP y = static_cast<TP>(x * lk);
-			pa[off] = y;
+			st_u(tb + off, lob, y);
 			x = static_cast<double>(y);
 		}
 		v[j] = x;
 	}
 	zeta_tile(v, 0, lane, wave, lds);
-	gptr<TZ> zb = G(static_cast<TZ *>(d.z)) + (uint64_t)tile * z_pitch<TZ>() + lane_bits_f(lane) + (wave << 6);
+	TZ *zt = uniform_ptr(static_cast<TZ *>(d.z)) + (uint64_t)tile * z_pitch<TZ>();
+	const uint32_t loz = (lane_bits_f(lane) + ((uint32_t)wave << 6)) * (uint32_t)sizeof(TZ);
 #pragma unroll
 	for (int j = 0; j < kTileRegs; ++j)
-		zb[((uint32_t)(j & 15) << 10) | ((uint32_t)(j >> 4) << 14)] = static_cast<TZ>(v[j]);
+		st_u(zt + (((j & 15) << 10) | ((j >> 4) << 14)), loz, static_cast<TZ>(v[j]));
 	// after the full 15-bit sweep, tile index 0 (thread 0, register 0) holds the tile's total
 	if (threadIdx.x == 0)
 		d.part[tile] = v[0];
+}
+
+// Pass 1. With one workgroup per tile (gridDim.x = tile count) each workgroup loads, then
+// transforms, its tile. With fewer workgroups (persistent, a multiple of 8: FP32 storage,
+// where the next tile's entries fit in the registers the transform leaves free) workgroup
+// b serves the contiguous tile range of XCD b % 8 in turn and loads its next tile while it
+// transforms the current one; the workgroup barriers of the transform wait for LDS only,
+// so those loads stay in flight across them.
+template <typename TP, typename TZ, bool FUSED, int LIK>
+__global__ __launch_bounds__(kTileThreads) void k_zeta_pass1(Descs descs)
+{
+	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
+	__shared__ double lds[kTileLds];
+	__shared__ double s_lik[kMaxVariants * (kMaxSubjects + 1)];
+	if (FUSED)
+		load_lik_table(descs, d, s_lik);
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	// uniform by construction; said so, the tile loop stays scalar
+	const unsigned tiles = 1u << (__builtin_amdgcn_readfirstlane(d.atoms) - kTileBits);
+	if (blockIdx.x == 0 && d.band != nullptr) {
+		// the band collection of the pass 2 that follows
+		if (threadIdx.x < 8)
+			best_slots(d)[threadIdx.x] = kKeyInf;
+		else if (threadIdx.x == 8)
+			d.counter[kRawCount] = 0u;
+	}
+	const double inv = uniform(d.scal[d.inv_in]);
+	const double f_in = FUSED ? uniform(s_lik[0]) : 0.0, f_out = FUSED ? uniform(s_lik[1]) : 0.0;
+	if (!pass1_persistent<TP, FUSED, LIK>() || gridDim.x >= tiles) {
+		const unsigned tile = xcd_tile(blockIdx.x, tiles);
+		TP raw[kTileRegs];
+		pass1_load(d, tile, lane, wave, raw);
+		pass1_tile<TP, TZ, FUSED, LIK>(d, tile, raw, lane, wave, inv, f_in, f_out, s_lik, lds);
+		return;
+	}
+	if constexpr (pass1_persistent<TP, FUSED, LIK>()) {
+		const unsigned wg_per_xcd = gridDim.x >> 3;
+		const unsigned per_xcd = tiles >> 3;
+		const unsigned xcd = blockIdx.x & 7u;
+		const unsigned first = xcd * per_xcd + (blockIdx.x >> 3), last = (xcd + 1) * per_xcd;
+		TP next[kTileRegs];
+		if (first < last)
+			pass1_load(d, first, lane, wave, next);
+		for (unsigned T = first; T < last; T += wg_per_xcd) {
+			TP cur[kTileRegs];
+#pragma unroll
+			for (int j = 0; j < kTileRegs; ++j)
+				cur[j] = next[j];
+			if (T + wg_per_xcd < last)
+				pass1_load(d, T + wg_per_xcd, lane, wave, next);
+			pass1_tile<TP, TZ, FUSED, LIK>(d, T, cur, lane, wave, inv, f_in, f_out, s_lik, lds);
+		}
+	}
 }
 
 // Final pass: sweep tile bits q0..14 and evaluate the objective on every output entry.
@@ -1546,7 +1622,7 @@ __global__ __launch_bounds__(256) void k_band_compact(Descs descs, unsigned raw_
 // sweeps over one pool bit at a time, 2^(c-1) independent pairs per sweep spread over the
 // wavefront. Every term is non-negative and every sum runs in a fixed order. Candidates
 // with more than kExactMaxPools distinct pools, or histories longer than kExactMaxHist,
-// get -1.
+// get -1; candidates dominated by another band entry get -2 (see below).
 __global__ __launch_bounds__(64) void k_exact_band(Descs descs, int atoms)
 {
 	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
@@ -1565,7 +1641,41 @@ __global__ __launch_bounds__(64) void k_exact_band(Descs descs, int atoms)
 	__shared__ double s_fin[kExactMaxPools], s_fout[kExactMaxPools];
 	const int lane = threadIdx.x; // one wavefront: lane t stages update t and pair t
 	const int h = d.hist_len;
-	const uint32_t e = d.band[i].e;
+	const Cand me = d.band[i];
+	const uint32_t e = me.e;
+	// Dominance (every state has positive mass, so M strictly decreases under inclusion):
+	// a candidate surely above 1/2 loses to a one-atom extension in the band that is surely
+	// above 1/2, one surely below to a one-atom reduction surely below. Such candidates
+	// need no mass: -2. Sure side: signed and |d| beyond half the band width. (The host
+	// uses these marks only for bands of at most kExactBand entries.)
+	const double half_band = 0.5 * d.band_width;
+	const int side = ((me.flags & kCandSided) && fabs(me.d) > half_band) ? (me.d > 0.0 ? 1 : -1) : 0;
+	if (side != 0 && count <= (unsigned)kExactBand) {
+		bool dom = false;
+		for (unsigned base = 0; base < count; base += 64 * 8) {
+			Cand o[8]; // eight loads in flight per lane
+#pragma unroll
+			for (int u = 0; u < 8; ++u) {
+				const unsigned j = base + (unsigned)lane + 64u * u;
+				o[u] = j < count ? d.band[j] : me;
+			}
+#pragma unroll
+			for (int u = 0; u < 8; ++u) {
+				const uint32_t x = o[u].e ^ e;
+				if (x == 0u || (x & (x - 1u)) != 0u)
+					continue;
+				const int os =
+					((o[u].flags & kCandSided) && fabs(o[u].d) > half_band) ? (o[u].d > 0.0 ? 1 : -1) : 0;
+				dom = dom || (side > 0 && os > 0 && (o[u].e & x) != 0u) ||
+					  (side < 0 && os < 0 && (o[u].e & x) == 0u);
+			}
+		}
+		if (__ballot(dom) != 0ull) {
+			if (lane == 0)
+				d.exact_u[i] = -2.0;
+			return;
+		}
+	}
 	if (h > kExactMaxHist) {
 		if (lane == 0)
 			d.exact_u[i] = -1.0;
@@ -1774,7 +1884,12 @@ void launch_update(const Descs &descs, int count, uint64_t states, int, int, Sto
 void launch_zeta_pass1(const Descs &descs, int count, int n, Storage st, bool fused, int k, int dilution,
 					   hipStream_t stream)
 {
-	const dim3 grid(1u << (n - kTileBits), count);
+	// FP32 storage: persistent workgroups (one per CU) that prefetch their next tile, once
+	// there are more tiles than CUs; otherwise one workgroup per tile
+	const unsigned tiles = 1u << (n - kTileBits);
+	const bool persistent = st == Storage::fp32c && (!fused || (k == 1 && !dilution));
+	const unsigned groups = persistent ? zeta_final_groups(n) : tiles;
+	const dim3 grid(groups, count);
 	const bool binary_nd = (k == 1 && !dilution);
 	launch_checked("k_zeta_pass1", [&] {
 		if (st == Storage::fp64) {
@@ -1792,7 +1907,7 @@ void launch_zeta_pass1(const Descs &descs, int count, int n, Storage st, bool fu
 			else
 				hipLaunchKernelGGL((k_zeta_pass1<float, float, false, 0>), grid, dim3(kTileThreads), 0, stream, descs);
 		}
-		hipLaunchKernelGGL(k_finalize_sum, dim3(1, count), dim3(1024), 0, stream, descs, grid.x,
+		hipLaunchKernelGGL(k_finalize_sum, dim3(1, count), dim3(1024), 0, stream, descs, tiles,
 						   (int)(fused ? kSumDenominator : kSumTotal));
 	});
 }

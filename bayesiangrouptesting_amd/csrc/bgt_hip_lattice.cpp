@@ -6,6 +6,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -91,6 +92,19 @@ void KernelClock::end(hipStream_t s, const char *name)
 	BGT_HIP_CHECK(hipEventRecord(b, s));
 	pending_.push_back({name, open_, b});
 	open_ = nullptr;
+}
+
+void KernelClock::host(const char *name, double ms)
+{
+	if (!enabled)
+		return;
+	for (auto &e : entries)
+		if (e.name == name) {
+			e.total_ms += ms;
+			e.launches += 1;
+			return;
+		}
+	entries.push_back({name, ms, 1});
 }
 
 void KernelClock::resolve()
@@ -584,7 +598,14 @@ void DeviceLattice::fetch_head()
 	if (exact_pending_)
 		BGT_HIP_CHECK(hipMemcpyAsync(box->exact_u, exact_dev_, sizeof(double) * kExactBand, hipMemcpyDeviceToHost,
 									 stream_));
-	sync();
+	if (clock_.enabled) {
+		const auto t0 = std::chrono::steady_clock::now();
+		sync();
+		clock_.host("host_fetch_wait",
+					std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+	} else {
+		sync();
+	}
 	exact_valid_ = exact_pending_;
 	exact_pending_ = false;
 	unpack_head(wide ? kBandHeadWide : kBandHead);
@@ -757,6 +778,17 @@ private:
 
 void DeviceLattice::rescore(bgt_hip_selection &out, int sel, const SelResult &r)
 {
+	if (!clock_.enabled) {
+		rescore_band(out, sel, r);
+		return;
+	}
+	const auto t0 = std::chrono::steady_clock::now();
+	rescore_band(out, sel, r);
+	clock_.host("host_rescore", std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+}
+
+void DeviceLattice::rescore_band(bgt_hip_selection &out, int sel, const SelResult &r)
+{
 	// Exact FP64 objectives for the candidates within the error band. The device best's
 	// true objective is at most d1 + eps, so the true best has a device objective of at most
 	// d1 + 2 eps: the band, which the device collects completely (the per-thread top-2 lists
@@ -771,6 +803,14 @@ void DeviceLattice::rescore(bgt_hip_selection &out, int sel, const SelResult &r)
 	HostBox *box = static_cast<HostBox *>(host_box_);
 	const double eps = 0.5 * band_width(sel);
 	const unsigned count = box->counter[0];
+	auto tp = std::chrono::steady_clock::now();
+	auto phase = [&](const char *name) {
+		if (!clock_.enabled)
+			return;
+		const auto now = std::chrono::steady_clock::now();
+		clock_.host(name, std::chrono::duration<double, std::milli>(now - tp).count());
+		tp = now;
+	};
 	bool complete = cand_ != nullptr && count <= band_cap_;
 	std::vector<Cand> cands;
 	if (complete) {
@@ -789,26 +829,56 @@ void DeviceLattice::rescore(bgt_hip_selection &out, int sel, const SelResult &r)
 				cands[i].flags |= (i + 1u) << kCandIndexShift;
 		}
 	}
-	for (const Cand c : {Cand{r.d1, r.e1, 0}, Cand{r.d2, r.e2, 0}})
-		if (c.e != 0xFFFFFFFFu)
-			cands.push_back(c);
-	// one entry per experiment (a rescan repeats what the lists held): signed ones first,
-	// then those with a device mass
-	auto rank = [](const Cand &c) { return (c.flags & kCandSided) * 2u + ((c.flags >> kCandIndexShift) != 0u); };
-	std::sort(cands.begin(), cands.end(),
-			  [&](const Cand &a, const Cand &b) { return a.e < b.e || (a.e == b.e && rank(a) > rank(b)); });
-	cands.erase(std::unique(cands.begin(), cands.end(), [](const Cand &a, const Cand &b) { return a.e == b.e; }),
-				cands.end());
+	// one entry per experiment: the band of the persistent pass 2 holds each state once, so
+	// only the device's top two need a look; a rescan repeats what the lists held, so there
+	// the entries are sorted and the signed copy, then the one with a device mass, is kept
+	const int objective = sel == kSelBha ? kObjHalving : (k_ == 1 ? kObjBinaryBbpa : kObjGather);
+	if (complete && persistent_pass2(atoms_, objective, false)) {
+		const size_t nb = cands.size();
+		for (const Cand c : {Cand{r.d1, r.e1, 0}, Cand{r.d2, r.e2, 0}}) {
+			if (c.e == 0xFFFFFFFFu)
+				continue;
+			bool seen = false;
+			for (size_t q = 0; q < nb && !seen; ++q)
+				seen = cands[q].e == c.e;
+			if (!seen && !(cands.size() > nb && cands.back().e == c.e))
+				cands.push_back(c);
+		}
+	} else {
+		for (const Cand c : {Cand{r.d1, r.e1, 0}, Cand{r.d2, r.e2, 0}})
+			if (c.e != 0xFFFFFFFFu)
+				cands.push_back(c);
+		auto rank = [](const Cand &c) {
+			return (c.flags & kCandSided) * 2u + ((c.flags >> kCandIndexShift) != 0u);
+		};
+		std::sort(cands.begin(), cands.end(),
+				  [&](const Cand &a, const Cand &b) { return a.e < b.e || (a.e == b.e && rank(a) > rank(b)); });
+		cands.erase(
+			std::unique(cands.begin(), cands.end(), [](const Cand &a, const Cand &b) { return a.e == b.e; }),
+			cands.end());
+	}
+	phase("t_dedupe");
 	auto side_of = [&](const Cand &c) {
 		if (!(c.flags & kCandSided) || !(std::fabs(c.d) > eps))
 			return 0;
 		return c.d > 0.0 ? 1 : -1;
 	};
+	// the device marked the dominated band entries (exact mass -2) when it computed the
+	// band's masses over the whole band; otherwise the same test runs here
+	const bool device_dom = complete && exact_valid_ && count <= (unsigned)kExactBand;
 	size_t sided = 0;
-	for (const Cand &c : cands)
-		sided += side_of(c) != 0;
+	if (!device_dom)
+		for (const Cand &c : cands)
+			sided += side_of(c) != 0;
 	std::vector<Cand> live;
-	if (sided < 2) {
+	if (device_dom) {
+		live.reserve(cands.size());
+		for (const Cand &c : cands) {
+			const uint32_t tag = c.flags >> kCandIndexShift;
+			if (tag == 0u || box->exact_u[tag - 1] != -2.0)
+				live.push_back(c);
+		}
+	} else if (sided < 2) {
 		live = std::move(cands); // no candidate can dominate another
 	} else {
 		SideTable sides(cands.size());
@@ -838,6 +908,9 @@ void DeviceLattice::rescore(bgt_hip_selection &out, int sel, const SelResult &r)
 			if (!dominated[i])
 				live.push_back(cands[i]);
 	}
+	phase("t_dominance");
+	clock_.host("n_band_unique", (double)cands.size());
+	clock_.host("n_band_live", (double)live.size());
 	if (complete && live.size() == 1) {
 		// every other candidate loses to one that beats it strictly: the survivor is the
 		// exact best, decided without its exact mass (objective: the device's value)
@@ -907,6 +980,8 @@ void DeviceLattice::rescore(bgt_hip_selection &out, int sel, const SelResult &r)
 					host.push_back(q - i);
 				}
 			}
+			clock_.host("n_exact_device", (double)(j - i - host.size()));
+			clock_.host("n_exact_host", (double)host.size());
 			if (!host.empty()) {
 				std::vector<double> hm(host.size());
 				std::unique_ptr<bool[]> hok(new bool[host.size()]);
@@ -936,6 +1011,7 @@ void DeviceLattice::rescore(bgt_hip_selection &out, int sel, const SelResult &r)
 			e2 = c.e;
 		}
 	}
+	phase("t_scan");
 	if (e1 == 0xFFFFFFFFu)
 		return;
 	out.experiment = e1;

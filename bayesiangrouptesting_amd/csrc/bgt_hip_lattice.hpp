@@ -24,6 +24,8 @@ struct KernelClock {
 	std::vector<Entry> entries;
 	void begin(hipStream_t s);
 	void end(hipStream_t s, const char *name);
+	// host-side entries: `ms` of host time, or a count carried in total_ms
+	void host(const char *name, double ms);
 	void resolve();
 	void reset();
 	~KernelClock();
@@ -137,6 +139,7 @@ private:
 	void init_prior(const double *prior);
 	void fill_lik(uint32_t experiment, uint32_t response, const double *dilution_flat);
 	void rescore(bgt_hip_selection &out, int selector, const SelResult &r);
+	void rescore_band(bgt_hip_selection &out, int selector, const SelResult &r);
 	// exact masses of the band on the device (binary non-dilution), after the band kernels
 	void launch_exact(const Descs &ds);
 	// the fields of the head now in the host mailbox, with up to band_entries band candidates

@@ -445,7 +445,11 @@ def main() -> int:
             "posterior_max_rel_err": post_err, "posterior_l1_rel_err": l1_err, "mass_max_rel_err": mass_err,
             "note": "errors of the last session's final posterior and masses, FP32C against FP64",
         },
-        "kernels_ms_fp32c": {k: round(v[0], 4) for k, v in kt32.items()},
+        # device kernels (HIP events on the lattice's stream), host phases of a round and
+        # error-band sizes, each averaged over the rounds that had them
+        "kernels_ms_fp32c": {k: round(v[0], 4) for k, v in kt32.items() if not k.startswith(("n_", "t_", "host_"))},
+        "host_ms_fp32c": {k: round(v[0], 4) for k, v in kt32.items() if k.startswith(("t_", "host_"))},
+        "band_fp32c": {k[2:]: round(v[0], 1) for k, v in kt32.items() if k.startswith("n_")},
     }
     if not args.no_aux:
         log("tree (n=16 dilution, depth 6)")
