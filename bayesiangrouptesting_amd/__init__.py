@@ -18,6 +18,8 @@ from .api import (  # noqa: F401
     Runtime,
     Selection,
     SelectorType,
+    ShrinkPlan,
+    ShrinkingLattice,
     SimulationConfig,
     SimulationOptions,
     SimulationResult,
@@ -28,14 +30,15 @@ from .api import (  # noqa: F401
     exact_upset_mass,
     hip_available,
     lattice_footprint,
+    make_shrink_plan,
     reverse_bits,
     run_simulation,
 )
 
 __all__ = [
     "DilutionParameters", "DilutionMode", "DilutionTable", "Error", "InvalidArgument", "KernelSpec", "Lattice",
-    "LatticeType", "Precision", "Provider", "Runtime", "Selection", "SelectorType", "SimulationConfig",
+    "LatticeType", "Precision", "Provider", "Runtime", "Selection", "SelectorType", "ShrinkPlan", "ShrinkingLattice", "SimulationConfig",
     "SimulationOptions", "SimulationResult", "TreeStats", "Unsupported", "available_providers", "batch_round",
-    "exact_upset_mass", "hip_available", "lattice_footprint",
+    "exact_upset_mass", "hip_available", "lattice_footprint", "make_shrink_plan",
     "reverse_bits", "run_simulation",
 ]

@@ -50,6 +50,22 @@ class SelectionC(ctypes.Structure):
     ]
 
 
+class ShrinkPlanC(ctypes.Structure):
+    _fields_ = [
+        ("original_subjects", ctypes.c_int),
+        ("current_subjects", ctypes.c_int),
+        ("variants", ctypes.c_int),
+        ("source_atoms", ctypes.c_int),
+        ("base_atom_count", ctypes.c_int),
+        ("reduced_atom_count", ctypes.c_int),
+        ("new_current_subjects", ctypes.c_int),
+        ("current_classified_atoms", ctypes.c_uint32),
+        ("removable_current_atoms", ctypes.c_uint32),
+        ("new_classified_subjects", ctypes.c_uint32),
+        ("original_to_current_atom_position", ctypes.c_int * 32),
+    ]
+
+
 class Error(RuntimeError):
     """A provider call failed; ``status`` carries the C ABI status code."""
 
@@ -93,6 +109,8 @@ def _sig(L):
         "bgt_hip_lattice_classify": ([_H, c_double, c_double, _u32p, _u32p], c_int),
         "bgt_hip_lattice_masks": ([_H, _u32p, _u32p], c_int),
         "bgt_hip_lattice_set_candidates": ([_H, c_u32], c_int),
+        "bgt_hip_shrink_plan_make": ([c_int, c_int, c_int, c_u32, c_u32, ctypes.POINTER(ShrinkPlanC)], c_int),
+        "bgt_hip_lattice_project": ([_H, ctypes.POINTER(ShrinkPlanC), ctypes.POINTER(_H)], c_int),
         "bgt_hip_lattice_select_experiments": ([_H, c_int, _u32p], c_int),
         "bgt_hip_lattice_round": ([_H, c_u32, c_u32, _dptr, c_double, c_double, ctypes.POINTER(SelectionC), _u32p,
                                    _u32p, _dptr], c_int),

@@ -291,6 +291,20 @@ class Lattice:
         p, q = self._masks()
         return bin(p | q).count("1") == self._n * self._k
 
+    def project(self, plan: "ShrinkPlan") -> "Lattice":
+        """A new lattice without the plan's removable atoms, holding this one's posterior
+        summed over them (project_posterior, shrink.hpp:70-86). This lattice is unchanged."""
+        h = nat._H()
+        c = plan.to_c()
+        nat.check(nat.lib().bgt_hip_lattice_project(self._h, ctypes.byref(c), ctypes.byref(h)))
+        out = Lattice.__new__(Lattice)
+        out._type = self._type
+        out._n = plan.new_current_subjects
+        out._k = self._k
+        out._prior = None
+        out._h = h
+        return out
+
     def error_bound(self) -> tuple[float, float]:
         a = ctypes.c_double()
         b = ctypes.c_double()
@@ -305,6 +319,134 @@ def lattice_footprint(type: LatticeType, subjects: int, variants: int = 1,
     nat.check(nat.lib().bgt_hip_lattice_footprint(int(_uses_dilution(LatticeType(type))), subjects, variants,
                                                   _storage(precision), ctypes.byref(out)))
     return out.value
+
+
+@dataclass
+class ShrinkPlan:
+    """model::ShrinkPlan (shrink.hpp:27-46). Atoms are variant-major (atom v*n + i)."""
+    original_subjects: int
+    current_subjects: int
+    variants: int
+    source_atoms: int
+    base_atom_count: int
+    reduced_atom_count: int
+    new_current_subjects: int
+    current_classified_atoms: int
+    removable_current_atoms: int
+    new_classified_subjects: int
+    original_to_current_atom_position: list
+
+    def has_shrink(self) -> bool:
+        return self.removable_current_atoms != 0
+
+    @property
+    def base_atom_masks(self) -> list[int]:
+        return [1 << a for a in range(self.source_atoms) if not (self.removable_current_atoms >> a) & 1]
+
+    @property
+    def reduced_atom_masks(self) -> list[int]:
+        return [1 << a for a in range(self.source_atoms) if (self.removable_current_atoms >> a) & 1]
+
+    @classmethod
+    def from_c(cls, c: nat.ShrinkPlanC) -> "ShrinkPlan":
+        atoms = c.original_subjects * c.variants
+        return cls(c.original_subjects, c.current_subjects, c.variants, c.source_atoms, c.base_atom_count,
+                   c.reduced_atom_count, c.new_current_subjects, c.current_classified_atoms,
+                   c.removable_current_atoms, c.new_classified_subjects,
+                   list(c.original_to_current_atom_position[:atoms]))
+
+    def to_c(self) -> nat.ShrinkPlanC:
+        c = nat.ShrinkPlanC()
+        for f, _ in nat.ShrinkPlanC._fields_:
+            if f != "original_to_current_atom_position":
+                setattr(c, f, getattr(self, f))
+        for i in range(32):
+            c.original_to_current_atom_position[i] = (self.original_to_current_atom_position[i]
+                                                      if i < len(self.original_to_current_atom_position) else -1)
+        return c
+
+
+def make_shrink_plan(original_subjects: int, current_subjects: int, variants: int, classified_atoms: int,
+                     already_classified_subjects: int) -> ShrinkPlan:
+    """make_shrink_plan (shrink_plan.cpp:124-177); classified_atoms in the original numbering."""
+    c = nat.ShrinkPlanC()
+    nat.check(nat.lib().bgt_hip_shrink_plan_make(original_subjects, current_subjects, variants, classified_atoms,
+                                                 already_classified_subjects, ctypes.byref(c)))
+    return ShrinkPlan.from_c(c)
+
+
+class ShrinkingLattice:
+    """The shrinking bookkeeping of model::Lattice (lattice_base.cpp:266-316) over a device
+    lattice: classification masks and removed subjects in the ORIGINAL numbering, the device
+    lattice holding only the subjects still in play (INTERNAL coordinates, as model::Lattice).
+    Each update_metadata_with_shrinking classifies from the device's atom masses and, when
+    some subject became fully classified, replaces the lattice by its projection."""
+
+    def __init__(self, type: LatticeType, subjects: int, prior: Sequence[float], variants: int = 1,
+                 precision: Precision = Precision.float64, dilution: Optional[DilutionTable] = None):
+        self.lattice = Lattice(type, subjects, prior, variants=variants, precision=precision)
+        self.original_subjects = subjects
+        self.current_subjects = subjects
+        self.variants = variants
+        self.classified_subjects = 0
+        self.positive = 0
+        self.negative = 0
+        self._table = dilution
+        self._tables: dict[int, np.ndarray] = {}
+
+    def _flat_table(self) -> Optional[np.ndarray]:
+        # the device takes D[s-1][m] for the current subject count (pool sizes 1..n)
+        if self._table is None:
+            return None
+        n, full = self.current_subjects, self._table.subjects()
+        if n not in self._tables:
+            rows = self._table.values.reshape(full, full + 1)
+            self._tables[n] = np.ascontiguousarray(rows[:n, : n + 1])  # D[s-1][m], s, m <= n
+        return self._tables[n]
+
+    def update(self, experiment: int, response: int) -> None:
+        """update_probs_in_place in the current numbering (internal coordinates)."""
+        t = self._flat_table()
+        ptr = t.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if t is not None else None
+        nat.check(nat.lib().bgt_hip_lattice_update(self.lattice.handle, experiment, response, ptr))
+
+    def select_experiment(self, selector: SelectorType = SelectorType.auto_select) -> int:
+        """The selection in the current numbering (internal coordinates)."""
+        s = nat.SelectionC()
+        nat.check(nat.lib().bgt_hip_lattice_select(self.lattice.handle, int(selector), ctypes.byref(s)))
+        return s.experiment
+
+    def update_metadata_with_shrinking(self, threshold_up: float, threshold_lo: float) -> None:
+        """lattice_base.cpp:266-305, then apply_shrink_plan (307-316)."""
+        k, orig = self.variants, self.original_subjects
+        classified = self.positive | self.negative
+        old = make_shrink_plan(orig, self.current_subjects, k, classified, self.classified_subjects)
+        masses = self.lattice.atom_masses()
+        new_pos = new_neg = 0
+        for i in range(orig * k):
+            if (classified >> i) & 1:
+                continue
+            cur = old.original_to_current_atom_position[i]
+            if cur < 0:
+                continue
+            m = masses[cur]
+            if m < threshold_lo:
+                new_pos |= 1 << i
+            elif m > 1 - threshold_up:
+                new_neg |= 1 << i
+        self.positive |= new_pos
+        self.negative |= new_neg
+        plan = make_shrink_plan(orig, self.current_subjects, k, self.positive | self.negative,
+                                self.classified_subjects)
+        if plan.has_shrink() and plan.new_current_subjects > 0:
+            projected = self.lattice.project(plan)
+            self.lattice.close()
+            self.lattice = projected
+            self.classified_subjects = plan.new_classified_subjects
+            self.current_subjects = plan.new_current_subjects
+
+    def close(self) -> None:
+        self.lattice.close()
 
 
 def batch_round(lattices: Sequence["Lattice"], experiments: Sequence[Optional[int]], responses: Sequence[Optional[int]],

@@ -282,6 +282,35 @@ int bgt_hip_exact_upset_mass(int subjects, const double *prior, int updates, con
 	});
 }
 
+int bgt_hip_shrink_plan_make(int original_subjects, int current_subjects, int variants, uint32_t classified_atoms,
+							 uint32_t already_classified_subjects, bgt_hip_shrink_plan *out)
+{
+	return guarded([&] {
+		if (out == nullptr)
+			throw bgt_hip::Error(BGT_HIP_BAD_INPUT, "null plan");
+		bgt_hip::make_shrink_plan(original_subjects, current_subjects, variants, classified_atoms,
+								  already_classified_subjects, *out);
+	});
+}
+
+int bgt_hip_lattice_project(bgt_hip_lattice *h, const bgt_hip_shrink_plan *plan, bgt_hip_lattice **out)
+{
+	return guarded([&] {
+		if (plan == nullptr || out == nullptr)
+			throw bgt_hip::Error(BGT_HIP_BAD_INPUT, "null plan or output handle");
+		*out = nullptr;
+		auto &l = L(h);
+		if (plan->current_subjects != l.subjects() || plan->variants != l.variants() ||
+			plan->source_atoms != l.atoms())
+			throw bgt_hip::Error(BGT_HIP_BAD_INPUT, "the plan is for another lattice shape");
+		auto r = std::make_unique<bgt_hip_lattice>();
+		r->impl = bgt_hip::DeviceLattice::project(l, plan->removable_current_atoms);
+		if (r->impl->subjects() != plan->new_current_subjects)
+			throw bgt_hip::Error(BGT_HIP_BAD_INPUT, "the plan's subject count does not match its removable atoms");
+		*out = r.release();
+	});
+}
+
 int bgt_hip_lattice_set_candidates(bgt_hip_lattice *h, uint32_t mask)
 {
 	return guarded([&] {

@@ -207,6 +207,10 @@ void launch_brute_force(const Descs &descs, int count, int n, int k, Storage st,
 void launch_atom_masses(const Descs &descs, int count, int atoms, Storage st, hipStream_t stream);
 void launch_upset_mass(const Descs &descs, int count, int atoms, uint32_t experiment,
 					   Storage st, hipStream_t stream);
+// destination entry b = sum over the subsets `red` of `removed` (increasing order) of
+// source[base(b) | red] * (1/D), base(b) = b's bits placed at the atoms of `keep`
+void launch_project(const void *src, const double *src_scal, int inv_slot, void *dst, int dst_atoms, uint32_t keep,
+					uint32_t removed, Storage st, hipStream_t stream);
 void launch_read_posterior(const Descs &descs, int atoms, Storage st, double *out,
 						   hipStream_t stream);
 void launch_lookahead(const Descs &descs, int n, Storage st, const uint32_t *selected,
@@ -224,5 +228,9 @@ struct FlatNode {
 void launch_tree_stats(const FlatNode *nodes, int node_count, int n, int k, int dilution,
 					   const double *dil, const double *pi, int depth, double branch,
 					   double *partials, unsigned *counter, double *out, hipStream_t stream);
+
+// make_shrink_plan (shrink_plan.cpp:124-177) into the C ABI's plan struct
+void make_shrink_plan(int original_subjects, int current_subjects, int variants, uint32_t classified_atoms,
+					  uint32_t already_classified_subjects, struct bgt_hip_shrink_plan &plan);
 
 } // namespace bgt_hip

@@ -1352,6 +1352,33 @@ __global__ __launch_bounds__(256) void k_upset_mass(Descs descs, int atoms, uint
 		d.part[blockIdx.x] = bs;
 }
 
+// Projection of a posterior onto the atoms it keeps (project_posterior, shrink.hpp:70-86):
+// entry b of the destination is the sum, in increasing order of the removed atoms' index,
+// of the source entries base(b) | red, where base(b) places b's bits at the kept atoms and
+// red runs over the subsets of the removed atoms. Sources are read as x = y * (1/D) and
+// summed in FP64 (the reference's accumulator_t); the destination holds the sums.
+template <typename TS, typename TD>
+__global__ __launch_bounds__(256) void k_project(const TS *src, const double *src_scal, int inv_slot, TD *dst,
+												 uint64_t dst_states, uint32_t keep, uint32_t removed)
+{
+	const double inv = src_scal[inv_slot];
+	for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < dst_states;
+		 b += (uint64_t)gridDim.x * blockDim.x) {
+		// scatter b's bits into the kept atoms, lowest first
+		uint32_t base = 0, bits = (uint32_t)b;
+		for (uint32_t m = keep; m && bits; m &= m - 1u, bits >>= 1)
+			base |= (bits & 1u) ? (m & (0u - m)) : 0u;
+		double sum = 0.0;
+		// subsets of the removed atoms in increasing order (0, then (red - removed) & removed)
+		uint32_t red = 0;
+		do {
+			sum += static_cast<double>(src[base | red]) * inv;
+			red = (red - removed) & removed;
+		} while (red != 0u);
+		dst[b] = static_cast<TD>(sum);
+	}
+}
+
 template <typename TP>
 __global__ __launch_bounds__(256) void k_read_posterior(Descs descs, uint64_t states, double *out)
 {
@@ -2101,6 +2128,21 @@ void launch_upset_mass(const Descs &descs, int count, int atoms, uint32_t e, Sto
 		else
 			hipLaunchKernelGGL(k_upset_mass<float>, grid, dim3(256), 0, stream, descs, atoms, e);
 		hipLaunchKernelGGL(k_finalize_sum, dim3(1, count), dim3(1024), 0, stream, descs, grid.x, (int)kSumUpset);
+	});
+}
+
+void launch_project(const void *src, const double *src_scal, int inv_slot, void *dst, int dst_atoms, uint32_t keep,
+					uint32_t removed, Storage st, hipStream_t stream)
+{
+	const uint64_t states = 1ull << dst_atoms;
+	const dim3 grid(grid_for(states, 256, 8192));
+	launch_checked("k_project", [&] {
+		if (st == Storage::fp64)
+			hipLaunchKernelGGL((k_project<double, double>), grid, dim3(256), 0, stream, static_cast<const double *>(src),
+							   src_scal, inv_slot, static_cast<double *>(dst), states, keep, removed);
+		else
+			hipLaunchKernelGGL((k_project<float, float>), grid, dim3(256), 0, stream, static_cast<const float *>(src),
+							   src_scal, inv_slot, static_cast<float *>(dst), states, keep, removed);
 	});
 }
 

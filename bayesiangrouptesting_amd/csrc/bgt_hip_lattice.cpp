@@ -183,6 +183,19 @@ DeviceLattice::DeviceLattice(bool dilution, int subjects, int variants, const do
 	init_prior(prior);
 }
 
+DeviceLattice::DeviceLattice(bool dilution, int subjects, int variants, Storage storage, Blank)
+	: dilution_(dilution), n_(subjects), k_(variants), atoms_(subjects * variants), storage_(storage)
+{
+	if (subjects <= 0 || variants <= 0)
+		throw Error(BGT_HIP_BAD_INPUT, "subjects and variants must be positive");
+	if (variants > kMaxVariants || subjects > kMaxSubjects || atoms_ > kMaxAtoms)
+		throw Error(BGT_HIP_UNSUPPORTED, "lattice larger than 2^30 states or more than 5 variants");
+	require_device();
+	states_ = 1ull << atoms_;
+	stream_ = stream_acquire();
+	allocate();
+}
+
 void DeviceLattice::init_prior(const double *prior)
 {
 	pos_ = neg_ = 0;
@@ -294,6 +307,63 @@ void DeviceLattice::recycle(std::unique_ptr<DeviceLattice> lattice)
 {
 	if (lattice)
 		LatticeCache::get().give(std::move(lattice));
+}
+
+std::unique_ptr<DeviceLattice> DeviceLattice::project(const DeviceLattice &src, uint32_t removed)
+{
+	const uint32_t all = src.atoms_ >= 32 ? 0xFFFFFFFFu : ((1u << src.atoms_) - 1u);
+	if (removed & ~all)
+		throw Error(BGT_HIP_BAD_INPUT, "removed atoms outside the lattice");
+	int removed_subjects = 0;
+	for (int i = 0; i < src.n_; ++i) {
+		uint32_t subject = 0;
+		for (int v = 0; v < src.k_; ++v)
+			subject |= 1u << (v * src.n_ + i);
+		if ((removed & subject) == subject)
+			++removed_subjects;
+		else if (removed & subject)
+			throw Error(BGT_HIP_BAD_INPUT, "removed atoms must cover every variant of each removed subject");
+	}
+	const int n2 = src.n_ - removed_subjects;
+	if (n2 < 1)
+		throw Error(BGT_HIP_UNSUPPORTED, "projection would remove every subject");
+	std::unique_ptr<DeviceLattice> l = LatticeCache::get().take(src.dilution_, n2, src.k_, src.storage_);
+	if (l)
+		l->settle();
+	else
+		l.reset(new DeviceLattice(src.dilution_, n2, src.k_, src.storage_, Blank{}));
+	const uint32_t keep = all & ~removed;
+	// the source's pending work (its last update) comes first
+	BGT_HIP_CHECK(hipStreamSynchronize(src.stream_));
+	launch_project(src.post_, src.scal_, src.inv_cur_, l->post_, l->atoms_, keep, removed, src.storage_, l->stream_);
+	HostBox *box = static_cast<HostBox *>(l->host_box_);
+	for (int s = 0; s < kScalars; ++s)
+		box->scal[s] = s < 4 ? 1.0 : 0.0; // pending normalisation 1: the sums are the values
+	BGT_HIP_CHECK(hipMemcpyAsync(l->scal_, box->scal, sizeof(double) * kScalars, hipMemcpyHostToDevice, l->stream_));
+	auto compact = [&](uint32_t mask) { // the kept atoms' bits, in order
+		uint32_t out = 0;
+		int at = 0;
+		for (uint32_t m = keep; m; m &= m - 1u, ++at)
+			if (mask & (m & (0u - m)))
+				out |= 1u << at;
+		return out;
+	};
+	l->pos_ = compact(src.pos_);
+	l->neg_ = compact(src.neg_);
+	l->active_ = (1u << n2) - 1u;
+	l->updates_ = src.updates_ + 1;
+	l->inv_cur_ = kScalInv0;
+	l->e_ = l->r_ = 0;
+	l->len_ = 0;
+	l->hist_uploaded_ = 0;
+	l->exact_pending_ = l->exact_valid_ = false;
+	l->pairs_own_ = false;
+	const std::vector<double> none((size_t)l->atoms_, 0.0);
+	l->exact_.reset(false, l->atoms_, none.data());
+	l->clock_.reset();
+	l->clock_.enabled = false;
+	l->busy_ = true;
+	return l;
 }
 
 DeviceLattice::DeviceLattice(const DeviceLattice &parent, bool copy_posterior)

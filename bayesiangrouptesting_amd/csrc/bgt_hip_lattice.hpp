@@ -61,6 +61,12 @@ public:
 											   Storage storage);
 	static std::unique_ptr<DeviceLattice> make_like(const DeviceLattice &like);
 	static void recycle(std::unique_ptr<DeviceLattice> lattice);
+	// A lattice without the atoms in `removed` (every variant of some subjects): its
+	// posterior is the projection of `source`'s (project_posterior, shrink.hpp:70-86), its
+	// classification masks are `source`'s on the kept atoms, every kept subject is a
+	// candidate again. It has no exact re-decision (the update history no longer applies)
+	// and its error bound counts the projection as one more rounding.
+	static std::unique_ptr<DeviceLattice> project(const DeviceLattice &source, uint32_t removed);
 
 	int subjects() const { return n_; }
 	int variants() const { return k_; }
@@ -133,6 +139,9 @@ public:
 	void as_source(LatDesc &d) const;
 
 private:
+	struct Blank {};
+	// a lattice of this shape whose posterior and state the caller sets
+	DeviceLattice(bool dilution, int subjects, int variants, Storage storage, Blank);
 	Descs single() const;
 	void allocate();
 	// write the prior and reset every piece of state (construction, and re-use from the cache)

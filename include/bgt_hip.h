@@ -172,6 +172,42 @@ int bgt_hip_lattice_masks(bgt_hip_lattice *lattice, uint32_t *positive_mask, uin
  * from the lattice (update_metadata_with_shrinking, lattice_base.cpp:266-305). */
 int bgt_hip_lattice_set_candidates(bgt_hip_lattice *lattice, uint32_t subjects_mask);
 
+/* ---- shrinking: removing classified subjects from a lattice ---- */
+
+/* Plan of a shrink (ShrinkPlan, shrink.hpp:27-46). Atoms are variant-major: atom v*n + i
+ * is variant v of subject i, in the original numbering (original_subjects) or the current
+ * one (current_subjects, the subjects still in the lattice). The reference's base and
+ * reduced atom mask lists are the atoms of the current layout outside and inside
+ * removable_current_atoms, in increasing order. */
+typedef struct bgt_hip_shrink_plan {
+	int original_subjects;
+	int current_subjects;
+	int variants;
+	int source_atoms;          /* current_subjects * variants */
+	int base_atom_count;       /* atoms kept */
+	int reduced_atom_count;    /* atoms removed */
+	int new_current_subjects;  /* subjects left after the shrink */
+	uint32_t current_classified_atoms;  /* classified atoms, current numbering */
+	uint32_t removable_current_atoms;   /* every atom of each fully classified subject */
+	uint32_t new_classified_subjects;   /* fully classified subjects, original numbering */
+	int original_to_current_atom_position[32]; /* -1 for atoms of subjects already removed */
+} bgt_hip_shrink_plan;
+
+/* replaces make_shrink_plan (shrink_plan.cpp:124-177): classified_atoms in the original
+ * numbering, already_classified_subjects the subjects an earlier shrink removed. Host only. */
+int bgt_hip_shrink_plan_make(int original_subjects, int current_subjects, int variants,
+							 uint32_t classified_atoms, uint32_t already_classified_subjects,
+							 bgt_hip_shrink_plan *out);
+
+/* replaces project_posterior (shrink.hpp:70-86) as apply_shrink_plan uses it
+ * (lattice_base.cpp:307-316): a new lattice of plan->new_current_subjects subjects whose
+ * posterior is the source's summed over the removed atoms, each sum in FP64 in increasing
+ * order of the removed atoms' index, as the reference's. The source lattice is unchanged.
+ * Classification masks carry over on the kept atoms; the new lattice has no exact
+ * re-decision. Fails (status 2) when the plan would remove every subject. */
+int bgt_hip_lattice_project(bgt_hip_lattice *source, const bgt_hip_shrink_plan *plan,
+							bgt_hip_lattice **out);
+
 /* k-step Op-BHA lookahead (lattice_api.cpp:116-168) in the reference's PUBLIC coordinates
  * (binary lattices only) */
 int bgt_hip_lattice_select_experiments(bgt_hip_lattice *lattice, int steps, uint32_t *out);

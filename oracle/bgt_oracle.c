@@ -607,6 +607,27 @@ static void update_metadata_with_shrinking(const orc_tree_cfg *c, double **post,
 	}
 }
 
+int orc_shrink_step(int orig, int k, double up, double lo, const double *post, int curr, uint32_t *pos,
+					uint32_t *neg, uint32_t *clas_subjs, double *out)
+{
+	/* one update_metadata_with_shrinking (lattice_base.cpp:266-316) on a copy of post; out
+	 * receives the posterior after it, 2^(curr' k) entries for the returned curr' */
+	orc_tree_cfg c;
+	memset(&c, 0, sizeof c);
+	c.orig = orig;
+	c.k = k;
+	c.up = up;
+	c.lo = lo;
+	const size_t n = (size_t)1 << (curr * k);
+	double *p = (double *)malloc(n * sizeof(double));
+	memcpy(p, post, n * sizeof(double));
+	int cc = curr;
+	update_metadata_with_shrinking(&c, &p, &cc, pos, neg, clas_subjs);
+	memcpy(out, p, ((size_t)1 << (cc * k)) * sizeof(double));
+	free(p);
+	return cc;
+}
+
 static orc_node *build_local_tree(const orc_tree_cfg *c, double *post, int curr, uint32_t pos,
 								  uint32_t neg, uint32_t clas_subjs, uint32_t ex, uint32_t res,
 								  int stage)

@@ -71,6 +71,11 @@ void orc_make_shrink_plan(int orig, int curr, int k, uint32_t classified_atoms,
 						  uint32_t already_classified_subjects, orc_shrink_plan *plan);
 void orc_project_posterior(const double *src, double *dst, const orc_shrink_plan *plan);
 uint32_t orc_true_ex(uint32_t halving, uint32_t clas_subjs, int orig_subjs);
+/* one update_metadata_with_shrinking step (lattice_base.cpp:266-316): classification from the
+ * atom masses of post (curr subjects), then the projection when subjects became fully
+ * classified; out (2^(curr k) entries) receives the posterior after it; returns curr' */
+int orc_shrink_step(int orig, int k, double up, double lo, const double *post, int curr, uint32_t *pos,
+					uint32_t *neg, uint32_t *clas_subjs, double *out);
 
 /* run_simulation -> TreeStats (simulation_runner.cpp:92-132) */
 int orc_run_simulation(int dilution, int n, int k, const double *prior, int depth, double up,

@@ -97,6 +97,10 @@ def lib(variant: str = "strict") -> ctypes.CDLL:
     L.orc_make_shrink_plan.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
                                        ctypes.c_uint32, ctypes.POINTER(OrcShrinkPlan)]
     L.orc_project_posterior.argtypes = [_dptr, _dptr, ctypes.POINTER(OrcShrinkPlan)]
+    L.orc_shrink_step.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _dptr, ctypes.c_int,
+                                  ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32),
+                                  ctypes.POINTER(ctypes.c_uint32), _dptr]
+    L.orc_shrink_step.restype = ctypes.c_int
     L.orc_true_ex.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int]
     L.orc_true_ex.restype = ctypes.c_uint32
     L.orc_run_simulation.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dptr, ctypes.c_int,
@@ -232,6 +236,18 @@ def project_posterior(src: np.ndarray, plan: OrcShrinkPlan) -> np.ndarray:
     out = np.empty(1 << plan.base_atom_count, dtype=np.float64)
     lib().orc_project_posterior(_p(_f64(src)), _p(out), ctypes.byref(plan))
     return out
+
+
+def shrink_step(orig: int, k: int, up: float, lo: float, post: np.ndarray, curr: int, pos: int, neg: int,
+                clas_subjs: int):
+    """One update_metadata_with_shrinking (lattice_base.cpp:266-316): returns (posterior,
+    current subjects, positive mask, negative mask, classified subjects) after it."""
+    src = _f64(post)
+    out = np.empty(src.size, dtype=np.float64)
+    p, q, c = ctypes.c_uint32(pos), ctypes.c_uint32(neg), ctypes.c_uint32(clas_subjs)
+    cc = lib().orc_shrink_step(orig, k, up, lo, _p(src), curr, ctypes.byref(p), ctypes.byref(q), ctypes.byref(c),
+                               _p(out))
+    return out[: 1 << (cc * k)].copy(), cc, p.value, q.value, c.value
 
 
 def true_ex(halving: int, clas_subjs: int, orig: int) -> int:
