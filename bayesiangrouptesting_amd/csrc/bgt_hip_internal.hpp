@@ -208,9 +208,9 @@ void launch_atom_masses(const Descs &descs, int count, int atoms, Storage st, hi
 void launch_upset_mass(const Descs &descs, int count, int atoms, uint32_t experiment,
 					   Storage st, hipStream_t stream);
 // destination entry b = sum over the subsets `red` of `removed` (increasing order) of
-// source[base(b) | red] * (1/D), base(b) = b's bits placed at the atoms of `keep`
-void launch_project(const void *src, const double *src_scal, int inv_slot, void *dst, int dst_atoms, uint32_t keep,
-					uint32_t removed, Storage st, hipStream_t stream);
+// source[base(b) | red] * (1/D), base(b) = b's bits placed at the atoms outside `removed`
+void launch_project(const void *src, const double *src_scal, int inv_slot, void *dst, int dst_atoms, uint32_t removed,
+					Storage st, hipStream_t stream);
 void launch_read_posterior(const Descs &descs, int atoms, Storage st, double *out,
 						   hipStream_t stream);
 void launch_lookahead(const Descs &descs, int n, Storage st, const uint32_t *selected,

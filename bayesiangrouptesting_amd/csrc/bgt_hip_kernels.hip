@@ -1359,15 +1359,17 @@ __global__ __launch_bounds__(256) void k_upset_mass(Descs descs, int atoms, uint
 // summed in FP64 (the reference's accumulator_t); the destination holds the sums.
 template <typename TS, typename TD>
 __global__ __launch_bounds__(256) void k_project(const TS *src, const double *src_scal, int inv_slot, TD *dst,
-												 uint64_t dst_states, uint32_t keep, uint32_t removed)
+												 uint64_t dst_states, uint32_t removed)
 {
 	const double inv = src_scal[inv_slot];
 	for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < dst_states;
 		 b += (uint64_t)gridDim.x * blockDim.x) {
-		// scatter b's bits into the kept atoms, lowest first
-		uint32_t base = 0, bits = (uint32_t)b;
-		for (uint32_t m = keep; m && bits; m &= m - 1u, bits >>= 1)
-			base |= (bits & 1u) ? (m & (0u - m)) : 0u;
+		// b's bits at the kept atoms: a zero inserted at each removed atom, lowest first
+		uint32_t base = (uint32_t)b;
+		for (uint32_t m = removed; m; m &= m - 1u) {
+			const uint32_t low = (m & (0u - m)) - 1u;
+			base = ((base & ~low) << 1) | (base & low);
+		}
 		double sum = 0.0;
 		// subsets of the removed atoms in increasing order (0, then (red - removed) & removed)
 		uint32_t red = 0;
@@ -2131,18 +2133,18 @@ void launch_upset_mass(const Descs &descs, int count, int atoms, uint32_t e, Sto
 	});
 }
 
-void launch_project(const void *src, const double *src_scal, int inv_slot, void *dst, int dst_atoms, uint32_t keep,
-					uint32_t removed, Storage st, hipStream_t stream)
+void launch_project(const void *src, const double *src_scal, int inv_slot, void *dst, int dst_atoms, uint32_t removed,
+					Storage st, hipStream_t stream)
 {
 	const uint64_t states = 1ull << dst_atoms;
 	const dim3 grid(grid_for(states, 256, 8192));
 	launch_checked("k_project", [&] {
 		if (st == Storage::fp64)
 			hipLaunchKernelGGL((k_project<double, double>), grid, dim3(256), 0, stream, static_cast<const double *>(src),
-							   src_scal, inv_slot, static_cast<double *>(dst), states, keep, removed);
+							   src_scal, inv_slot, static_cast<double *>(dst), states, removed);
 		else
 			hipLaunchKernelGGL((k_project<float, float>), grid, dim3(256), 0, stream, static_cast<const float *>(src),
-							   src_scal, inv_slot, static_cast<float *>(dst), states, keep, removed);
+							   src_scal, inv_slot, static_cast<float *>(dst), states, removed);
 	});
 }
 

@@ -335,7 +335,7 @@ std::unique_ptr<DeviceLattice> DeviceLattice::project(const DeviceLattice &src, 
 	const uint32_t keep = all & ~removed;
 	// the source's pending work (its last update) comes first
 	BGT_HIP_CHECK(hipStreamSynchronize(src.stream_));
-	launch_project(src.post_, src.scal_, src.inv_cur_, l->post_, l->atoms_, keep, removed, src.storage_, l->stream_);
+	launch_project(src.post_, src.scal_, src.inv_cur_, l->post_, l->atoms_, removed, src.storage_, l->stream_);
 	HostBox *box = static_cast<HostBox *>(l->host_box_);
 	for (int s = 0; s < kScalars; ++s)
 		box->scal[s] = s < 4 ? 1.0 : 0.0; // pending normalisation 1: the sums are the values

@@ -255,6 +255,37 @@ def tree_workload(repeats: int = 3) -> dict:
     return out
 
 
+def shrink_workload(repeats: int = 10) -> dict:
+    """Projection of a 2^26-state binary lattice onto 24 of its subjects (two fully classified
+    subjects summed out, project_posterior): per form, the median of `repeats` projections
+    (host call to completion, the destination from the lattice cache) and its rate in
+    algorithmic bytes (source read once, destination written once)."""
+    import bayesiangrouptesting_amd as bgt
+    from bayesiangrouptesting_amd import workload
+
+    n, removed = 26, (3, 17)
+    pi = workload.priors(7, n, 0.01, 0.10)
+    classified = sum(1 << i for i in removed)
+    plan = bgt.make_shrink_plan(n, n, 1, classified, 0)
+    out = {"workload": f"binary non-dilution n={n} -> {plan.new_current_subjects} subjects "
+                       f"(subjects {removed[0]} and {removed[1]} summed out), priors U[0.01,0.10] seed 7"}
+    for prec, key, b in ((bgt.Precision.float64, "fp64", 8), (bgt.Precision.float32_compensated, "fp32c", 4)):
+        lat = bgt.Lattice(bgt.LatticeType.replicated_non_dilution, n, pi, precision=prec)
+        device_sync()
+        times = []
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            proj = lat.project(plan)
+            device_sync()
+            times.append(time.perf_counter() - t0)
+            proj.close()
+        ms = float(np.median(times)) * 1e3
+        alg = (1 << n) * b + (1 << plan.base_atom_count) * b
+        out[key] = {"ms": ms, "GBs_alg": alg / (ms * 1e-3) / 1e9, "alg_bytes": alg}
+        lat.close()
+    return out
+
+
 def bbpa_workload(rounds: int = 8) -> dict:
     """Multinomial dilution lattice n = 8, k = 3 (2^24 states), DilutionTable(8, 0.05, 0.9),
     priors U[0.01, 0.10]: `rounds` rounds of Op-BBPA select -> update with seeded responses."""
@@ -456,6 +487,8 @@ def main() -> int:
         line["tree"] = tree_workload()
         log("bbpa (n=8, k=3)")
         line["bbpa"] = bbpa_workload()
+        log("shrink (n=26 -> 24)")
+        line["shrink"] = shrink_workload()
         log(f"serving ({args.serving_sessions} sessions at {args.serving_rate:g}/s)")
         line["serving"] = serving_workload(args.serving_sessions, args.serving_rate)
     if not args.no_cpu:
