@@ -940,21 +940,34 @@ __device__ __forceinline__ void scan_tile(const double (&v)[kTileRegs], double s
 // best seen is a running minimum over this wave's tiles and its XCD's slot, never below the
 // final best, so every entry within band_width of the final best is appended;
 // k_band_compact keeps those after the merge.
-template <typename TIN, int LC>
+//
+// SPLIT (FP32 intermediate, A >= 28): the lattice is run as one of A - 1 atoms (this LC is
+// 31 - A) whose top row bit, state bit A - 1, is split into two work units per tile. The
+// unit of the lower half (bit A-1 clear) sums each entry with its partner in the upper half
+// before the sweep, which is the sweep of bit A-1 done first; the unit of the upper half
+// sweeps its own half. Tiles are then twice as wide (runs of 32 bytes instead of 16), so
+// the pass needs 3/4 of the memory requests for 3/2 of the bytes, and the two units of a
+// tile run side by side on one XCD, where the half both read is one L2 fetch. The sum order
+// differs from the increasing bit order only in FP64 additions of FP32 values, inside the
+// FP32C error band; FP64 storage keeps the exact order.
+template <typename TIN, int LC, bool SPLIT = false>
 __global__ __launch_bounds__(kTileThreads) void k_zeta_pass2(Descs descs, int objective, int scaled, double target)
 {
 	constexpr p2::Map mb = p2::block_map(LC);
 	constexpr p2::Map mw = p2::wave_map(LC);
 	constexpr unsigned tiles = 1u << (kTileBits - LC);
+	constexpr unsigned units = SPLIT ? 2 * tiles : tiles;
 	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
 	__shared__ double lds[kTileLds];
 	__shared__ Top2 red2[16];
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 	const unsigned wg_per_xcd = gridDim.x >> 3;
-	const unsigned per_xcd = tiles >> 3;
+	const unsigned per_xcd = units >> 3;
 	const unsigned xcd = blockIdx.x & 7u;
 	const unsigned first = xcd * per_xcd + (blockIdx.x >> 3), last = (xcd + 1) * per_xcd;
 	constexpr uint32_t pitch = z_pitch<TIN>();
+	// SPLIT: the upper half's rows start this many entries after the lower half's
+	constexpr size_t half_rows = (size_t)pitch << (kTileBits - LC);
 	const uint64_t lo = p2::thread_z_offset<LC, pitch>(lane, wave);
 	const uint32_t lf = p2::thread_state_offset<LC>(lane, wave);
 	const TIN *in = static_cast<const TIN *>(d.z);
@@ -975,15 +988,26 @@ __global__ __launch_bounds__(kTileThreads) void k_zeta_pass2(Descs descs, int ob
 	bool raw_full = false;  // wave-uniform: the candidate buffer has overflowed
 	Top2 run;
 	top2_init(run);
-	for (unsigned T = first; T < last; T += wg_per_xcd) {
+	for (unsigned U = first; U < last; U += wg_per_xcd) {
 		// (prefetching the next tile into registers costs more in spills than it hides:
 		// the tile alone takes 64 of the 128 VGPRs a 1024-thread workgroup allows)
+		// units 2T and 2T+1 are tile T's halves; which is which alternates from one turn of a
+		// workgroup to the next, so every workgroup does as many two-half units as one-half
+		const unsigned T = SPLIT ? U >> 1 : U;
+		const bool upper = SPLIT && (((U & 1u) ^ ((T / (wg_per_xcd >> 1)) & 1u)) != 0u);
 		double v[kTileRegs];
 		{
-			const TIN *tb = inw + ((uint64_t)T << LC);
+			const TIN *tb = inw + ((uint64_t)T << LC) + (upper ? half_rows : 0);
+			if (SPLIT && !upper) {
 #pragma unroll
-			for (int j = 0; j < kTileRegs; ++j)
-				v[j] = static_cast<double>(ld_u(tb + (size_t)j * pitch, lob));
+				for (int j = 0; j < kTileRegs; ++j)
+					v[j] = static_cast<double>(ld_u(tb + (size_t)j * pitch, lob)) +
+						   static_cast<double>(ld_u(tb + half_rows + (size_t)j * pitch, lob));
+			} else {
+#pragma unroll
+				for (int j = 0; j < kTileRegs; ++j)
+					v[j] = static_cast<double>(ld_u(tb + (size_t)j * pitch, lob));
+			}
 		}
 		// registers: tile bits LC..LC+4
 		bfly_reg<0>(v);
@@ -1020,7 +1044,7 @@ __global__ __launch_bounds__(kTileThreads) void k_zeta_pass2(Descs descs, int ob
 			perm_swap<true, 1>(v);
 			bfly_reg<1>(v);
 		}
-		const uint32_t gt = ((uint32_t)T << LC) + lf;
+		const uint32_t gt = ((uint32_t)T << LC) + lf + (upper ? (1u << (2 * kTileBits - LC)) : 0u);
 		if ((T & (T - 1u)) == 0u) {
 			// tile 0 and power-of-two tiles hold the singletons {a}
 			constexpr p2::Map mf = p2::final_map(LC);
@@ -1967,6 +1991,22 @@ unsigned launch_final_kernel(const Descs &descs, int count, int n, Storage st, i
 	if (persistent_pass2(n, objective, from_posterior)) {
 		// persistent pass 2: one workgroup per CU (a multiple of 8, one share per XCD)
 		const dim3 grid(zeta_final_groups(n), count);
+		if (st == Storage::fp32c && n >= 28) {
+			// FP32 runs of 16 bytes or less: the split form, one atom smaller and twice as wide
+			launch_checked("k_zeta_pass2", [&] {
+				const int sc = scaled ? 1 : 0;
+				if (n == 28)
+					hipLaunchKernelGGL((k_zeta_pass2<float, 3, true>), grid, dim3(kTileThreads), 0, stream, descs,
+									   objective, sc, target);
+				else if (n == 29)
+					hipLaunchKernelGGL((k_zeta_pass2<float, 2, true>), grid, dim3(kTileThreads), 0, stream, descs,
+									   objective, sc, target);
+				else
+					hipLaunchKernelGGL((k_zeta_pass2<float, 1, true>), grid, dim3(kTileThreads), 0, stream, descs,
+									   objective, sc, target);
+			});
+			return grid.x;
+		}
 		launch_checked("k_zeta_pass2", [&] {
 #define BGT_P2(LCV)                                                                                            \
 	case LCV:                                                                                                  \
