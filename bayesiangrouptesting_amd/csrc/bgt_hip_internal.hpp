@@ -233,4 +233,32 @@ void launch_tree_stats(const FlatNode *nodes, int node_count, int n, int k, int 
 void make_shrink_plan(int original_subjects, int current_subjects, int variants, uint32_t classified_atoms,
 					  uint32_t already_classified_subjects, struct bgt_hip_shrink_plan &plan);
 
+// One step of the k-step lookahead on down-set sums: the m experiments selected so far
+// (internal coordinates) as the intersections over each subset of them (index k: bit i set
+// = experiment i in the subset; the empty subset is every atom).
+struct LookStep {
+	uint32_t inter[16];
+	int m;
+	int n;
+};
+// lattices from this many subjects take the down-set lookahead (below, the O(4^n) scan)
+constexpr int kLookDownMin = 13;
+// D(X) = sum over s <= X of x(s), internal coordinates, FP64 (2^atoms entries)
+void launch_down_sums(const Descs &descs, int atoms, Storage st, double *D, hipStream_t stream);
+// every candidate's score (internal index order) and each of `blocks` blocks' lowest score
+void launch_lookahead_down(const double *D, const LookStep &step, double *scores, double *best_score,
+						   unsigned blocks, hipStream_t stream);
+// each block's lowest public index scoring at most `limit` (same grid as the scoring launch,
+// whose per-block lowest scores are block_best)
+void launch_lookahead_pick(const double *scores, const double *block_best, int n, double limit,
+						   uint32_t *best_exp, unsigned blocks, hipStream_t stream);
+// Bound on the FP64 rounding of one lookahead score over 2^(m+1) groups of n-subject states,
+// with masses formed from down-set sums (n levels) and a Moebius inversion (m+1 levels):
+// each group mass carries at most 2^(m+1)(n+1) u, the score sums 2^(m+1) such terms.
+inline double lookahead_score_error(int n, int m)
+{
+	const double g = (double)(1 << (m + 1));
+	return g * (g * (double)(n + 1) + 3.0) * 0x1p-53;
+}
+
 } // namespace bgt_hip

@@ -1465,6 +1465,157 @@ __global__ __launch_bounds__(256) void k_lookahead(Descs descs, int n, const uin
 		scores[c] = score;
 }
 
+// ------------------------------------------- k-step lookahead through down-set sums
+//
+// The reference scores each candidate c by the masses of the 2^(m+1) groups the states fall
+// into by which of the m selected experiments and c contain them (s <= e_i), summing the
+// posterior once per candidate (lattice_api.cpp:116-168, O(4^n)). With the down-set sums
+// D(X) = sum_{s <= X} x(s), the states contained in every experiment of a set K have mass
+// F(K) = D(the intersection of K's experiments), and a group's mass is the Moebius
+// inversion sum_{K >= J} (-1)^{|K \ J|} F(K): O(2^n) for D once, then 2^(m+1) lookups per
+// candidate. Subset relations are kept by the public bit reversal, so D is built in internal
+// coordinates; candidates are ranked by public index, as the reference's loop.
+
+// Pass A: D over bits 0..nb-1 of contiguous blocks of 2^nb entries (x = y * (1/D) read from
+// the posterior, the sums in FP64).
+template <typename TP>
+__global__ __launch_bounds__(256) void k_down_first(Descs descs, double *D, int nb)
+{
+	const LatDesc &d = descs.many ? descs.many[0] : descs.one;
+	extern __shared__ double t[];
+	const uint32_t size = 1u << nb;
+	const uint64_t base = (uint64_t)blockIdx.x << nb;
+	gptr<const TP> post = G(static_cast<const TP *>(d.post));
+	const double inv = d.scal[d.inv_in];
+	for (uint32_t i = threadIdx.x; i < size; i += blockDim.x)
+		t[i] = static_cast<double>(post[base + i]) * inv;
+	__syncthreads();
+	for (int b = 0; b < nb; ++b) {
+		const uint32_t low = (1u << b) - 1u;
+		for (uint32_t x = threadIdx.x; x < (size >> 1); x += blockDim.x) {
+			const uint32_t k = ((x & ~low) << 1) | (x & low); // bit b clear
+			t[k | (1u << b)] += t[k];
+		}
+		__syncthreads();
+	}
+	for (uint32_t i = threadIdx.x; i < size; i += blockDim.x)
+		D[base + i] = t[i];
+}
+
+// Pass B: D over bits b0..b0+nb-1 (nb <= 8) on tiles of 32 contiguous entries (bits 0..4,
+// already summed) by 2^nb rows.
+__global__ __launch_bounds__(256) void k_down_more(double *D, int n, int b0, int nb)
+{
+	__shared__ double t[32 * 256];
+	const uint32_t rows = 1u << nb;
+	// tile: columns c0..c0+31 (bits 0..b0-1 of the tile's start), the bits above b0+nb
+	const uint64_t cols = 1ull << (b0 - 5); // column groups of 32 below bit b0
+	const uint64_t cg = blockIdx.x % cols, hi = blockIdx.x / cols;
+	const uint64_t base = (cg << 5) | (hi << (b0 + nb));
+	for (uint32_t i = threadIdx.x; i < 32 * rows; i += blockDim.x) {
+		const uint32_t r = i >> 5, c = i & 31u;
+		t[i] = D[base + c + ((uint64_t)r << b0)];
+	}
+	__syncthreads();
+	for (int b = 0; b < nb; ++b) {
+		const uint32_t low = (1u << b) - 1u;
+		for (uint32_t x = threadIdx.x; x < 32 * (rows >> 1); x += blockDim.x) {
+			const uint32_t r = x >> 5, c = x & 31u;
+			const uint32_t rk = ((r & ~low) << 1) | (r & low);
+			t[((rk | (1u << b)) << 5) + c] += t[(rk << 5) + c];
+		}
+		__syncthreads();
+	}
+	for (uint32_t i = threadIdx.x; i < 32 * rows; i += blockDim.x) {
+		const uint32_t r = i >> 5, c = i & 31u;
+		D[base + c + ((uint64_t)r << b0)] = t[i];
+	}
+	(void)n;
+}
+
+// Scores of every candidate into scores[internal index] (candidate 0 scores +inf), and each
+// block's lowest score into best_score[blockIdx.x]. M experiments are selected (1..4), so
+// every register index below is a compile-time constant.
+template <int M>
+__global__ __launch_bounds__(256) void k_lookahead_down(const double *D, LookStep st, double *scores,
+														double *best_score)
+{
+	constexpr int K = 1 << M;       // subsets of the selected experiments
+	constexpr int GROUPS = 2 * K;   // with the candidate
+	__shared__ double s_sc[256];
+	const uint32_t states = 1u << st.n;
+	const double target = 1.0 / (double)GROUPS;
+	double bs = INFINITY;
+	// the candidate-independent masses F(K), K without the candidate
+	double fk[K];
+	uint32_t inter[K];
+#pragma unroll
+	for (int k = 0; k < K; ++k) {
+		inter[k] = st.inter[k];
+		fk[k] = D[inter[k]];
+	}
+	for (uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x; ci < states; ci += gridDim.x * blockDim.x) {
+		// g[J]: J's bit M is the candidate; F(K) first, then Moebius inversion over the
+		// superset lattice: mass(J) = sum_{K >= J} (-1)^|K\J| F(K)
+		double g[GROUPS];
+#pragma unroll
+		for (int k = 0; k < K; ++k) {
+			g[k] = fk[k];
+			g[k | K] = D[inter[k] & ci];
+		}
+#pragma unroll
+		for (int b = 0; b <= M; ++b) {
+#pragma unroll
+			for (int j = 0; j < GROUPS; ++j)
+				if (!(j & (1 << b)))
+					g[j] -= g[j | (1 << b)];
+		}
+		double score = 0.0;
+#pragma unroll
+		for (int j = 0; j < GROUPS; ++j)
+			score += fabs(g[j] - target);
+		score = ci == 0u ? INFINITY : score; // the empty experiment is not a candidate
+		scores[ci] = score;
+		bs = fmin(bs, score);
+	}
+	s_sc[threadIdx.x] = bs;
+	__syncthreads();
+	for (int o = blockDim.x >> 1; o > 0; o >>= 1) {
+		if ((int)threadIdx.x < o)
+			s_sc[threadIdx.x] = fmin(s_sc[threadIdx.x], s_sc[threadIdx.x + o]);
+		__syncthreads();
+	}
+	if (threadIdx.x == 0)
+		best_score[blockIdx.x] = s_sc[0];
+}
+
+// Each block's lowest public index whose score is at most `limit` (0xFFFFFFFF if none). The
+// grid is the scoring launch's, so a block whose lowest score exceeds the limit has nothing.
+__global__ __launch_bounds__(256) void k_lookahead_pick(const double *scores, const double *block_best, int n,
+														double limit, uint32_t *best_exp)
+{
+	__shared__ uint32_t s_ex[256];
+	if (block_best[blockIdx.x] > limit) {
+		if (threadIdx.x == 0)
+			best_exp[blockIdx.x] = 0xFFFFFFFFu;
+		return;
+	}
+	const uint32_t states = 1u << n;
+	uint32_t be = 0xFFFFFFFFu;
+	for (uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x; ci < states; ci += gridDim.x * blockDim.x)
+		if (scores[ci] <= limit)
+			be = min(be, __brev(ci) >> (32 - n));
+	s_ex[threadIdx.x] = be;
+	__syncthreads();
+	for (int o = blockDim.x >> 1; o > 0; o >>= 1) {
+		if ((int)threadIdx.x < o)
+			s_ex[threadIdx.x] = min(s_ex[threadIdx.x], s_ex[threadIdx.x + o]);
+		__syncthreads();
+	}
+	if (threadIdx.x == 0)
+		best_exp[blockIdx.x] = s_ex[0];
+}
+
 // ------------------------------------------------------------------ tree statistics
 
 __device__ __forceinline__ double orig_likelihood(uint32_t e, uint32_t r, uint32_t s, int n, int k,
@@ -2209,6 +2360,50 @@ void launch_lookahead(const Descs &descs, int n, Storage st, const uint32_t *sel
 			hipLaunchKernelGGL(k_lookahead<double>, grid, dim3(256), 0, stream, descs, n, selected, m, scores);
 		else
 			hipLaunchKernelGGL(k_lookahead<float>, grid, dim3(256), 0, stream, descs, n, selected, m, scores);
+	});
+}
+
+void launch_down_sums(const Descs &descs, int atoms, Storage st, double *D, hipStream_t stream)
+{
+	const int nb = atoms < 12 ? atoms : 12;
+	launch_checked("k_down_first", [&] {
+		const dim3 grid(1u << (atoms - nb));
+		const size_t lds = sizeof(double) << nb;
+		if (st == Storage::fp64)
+			hipLaunchKernelGGL(k_down_first<double>, grid, dim3(256), lds, stream, descs, D, nb);
+		else
+			hipLaunchKernelGGL(k_down_first<float>, grid, dim3(256), lds, stream, descs, D, nb);
+	});
+	for (int b0 = nb; b0 < atoms; b0 += 8) {
+		const int w = atoms - b0 < 8 ? atoms - b0 : 8;
+		const dim3 grid((unsigned)(1ull << (atoms - w - 5)));
+		launch_checked("k_down_more", [&] {
+			hipLaunchKernelGGL(k_down_more, grid, dim3(256), 0, stream, D, atoms, b0, w);
+		});
+	}
+}
+
+void launch_lookahead_down(const double *D, const LookStep &step, double *scores, double *best_score,
+						   unsigned blocks, hipStream_t stream)
+{
+	if (step.m < 1 || step.m > 4)
+		throw Error(4, "lookahead step out of range");
+	launch_checked("k_lookahead_down", [&] {
+		switch (step.m) {
+		case 1: hipLaunchKernelGGL(k_lookahead_down<1>, dim3(blocks), dim3(256), 0, stream, D, step, scores, best_score); break;
+		case 2: hipLaunchKernelGGL(k_lookahead_down<2>, dim3(blocks), dim3(256), 0, stream, D, step, scores, best_score); break;
+		case 3: hipLaunchKernelGGL(k_lookahead_down<3>, dim3(blocks), dim3(256), 0, stream, D, step, scores, best_score); break;
+		default: hipLaunchKernelGGL(k_lookahead_down<4>, dim3(blocks), dim3(256), 0, stream, D, step, scores, best_score); break;
+		}
+	});
+}
+
+void launch_lookahead_pick(const double *scores, const double *block_best, int n, double limit,
+						   uint32_t *best_exp, unsigned blocks, hipStream_t stream)
+{
+	launch_checked("k_lookahead_pick", [&] {
+		hipLaunchKernelGGL(k_lookahead_pick, dim3(blocks), dim3(256), 0, stream, scores, block_best, n, limit,
+						   best_exp);
 	});
 }
 

@@ -435,6 +435,44 @@ int orc_select_experiments(const double *post, int n, int steps, uint32_t *out)
 	return steps;
 }
 
+int orc_lookahead_scores(const double *post, int n, const uint32_t *selected, int m, double *scores)
+{
+	/* lattice_api.cpp:139-158: the score of every public candidate c (scores[c], c >= 1)
+	 * after the public experiments selected[0..m-1], in the reference's summation order:
+	 * states ascending, each added to its group's mass, then |mass - 1/2^(m+1)| summed in
+	 * group order. scores[0] is left untouched. */
+	if (m < 1 || m > 15)
+		return -1;
+	const uint32_t total = bit(n);
+	const int groups = 1 << (m + 1);
+	double *mass = (double *)malloc(sizeof(double) * (size_t)groups);
+	int *cls = (int *)malloc(sizeof(int) * total);
+	double *pub = (double *)malloc(sizeof(double) * total);
+	for (uint32_t s = 0; s < total; s++) {
+		pub[s] = post[orc_reverse_bits(s, n)];
+		int gi = 0;
+		for (int i = 0; i < m; i++)
+			if ((selected[i] & s) == s)
+				gi |= 1 << i;
+		cls[s] = gi;
+	}
+	const double target = 1.0 / (double)groups;
+	for (uint32_t c = 1; c < total; c++) {
+		for (int g = 0; g < groups; g++)
+			mass[g] = 0.0;
+		for (uint32_t s = 0; s < total; s++)
+			mass[cls[s] | ((c & s) == s ? 1 << m : 0)] += pub[s];
+		double score = 0.0;
+		for (int g = 0; g < groups; g++)
+			score += fabs(mass[g] - target);
+		scores[c] = score;
+	}
+	free(pub);
+	free(cls);
+	free(mass);
+	return 0;
+}
+
 /* ------------------------------------------------------------------ shrink */
 
 static uint32_t curr_shrinkable_atoms(uint32_t curr_clas_atoms, int curr, int k)

@@ -55,6 +55,8 @@ void orc_atom_masses(const double *post, int atoms, double *out);
 void orc_classify(const double *masses, int atoms, double up, double lo, uint32_t *pos, uint32_t *neg);
 double orc_upset_mass(const double *post, int atoms, uint32_t e);
 int orc_select_experiments(const double *post, int n, int steps, uint32_t *out);
+/* every candidate's lookahead score after m selected public experiments (lattice_api.cpp:139-158) */
+int orc_lookahead_scores(const double *post, int n, const uint32_t *selected, int m, double *scores);
 uint32_t orc_reverse_bits(uint32_t value, int width);
 
 /* shrink plan (shrink_plan.cpp:124-177) and projection (shrink.hpp:70-86) */

@@ -92,6 +92,8 @@ def lib(variant: str = "strict") -> ctypes.CDLL:
     L.orc_upset_mass.restype = ctypes.c_double
     L.orc_select_experiments.argtypes = [_dptr, ctypes.c_int, ctypes.c_int, _u32p]
     L.orc_select_experiments.restype = ctypes.c_int
+    L.orc_lookahead_scores.argtypes = [_dptr, ctypes.c_int, _u32p, ctypes.c_int, _dptr]
+    L.orc_lookahead_scores.restype = ctypes.c_int
     L.orc_reverse_bits.argtypes = [ctypes.c_uint32, ctypes.c_int]
     L.orc_reverse_bits.restype = ctypes.c_uint32
     L.orc_make_shrink_plan.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
@@ -220,6 +222,16 @@ def select_experiments(post: np.ndarray, n: int, steps: int) -> list[int]:
     out = (ctypes.c_uint32 * max(steps, 1))()
     lib().orc_select_experiments(_p(post), n, steps, out)
     return [int(out[i]) for i in range(steps)]
+
+
+def lookahead_scores(post: np.ndarray, n: int, selected) -> np.ndarray:
+    """Every public candidate's lookahead score after `selected` (index 0 is +inf)."""
+    sel = np.ascontiguousarray(selected, dtype=np.uint32)
+    out = np.full(1 << n, np.inf)
+    p = _f64(post)
+    if lib().orc_lookahead_scores(_p(p), n, sel.ctypes.data_as(_u32p), len(sel), _p(out)) != 0:
+        raise ValueError("lookahead_scores: bad selection")
+    return out
 
 
 def reverse_bits(v: int, width: int) -> int:
