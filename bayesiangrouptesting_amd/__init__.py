@@ -14,31 +14,46 @@ from .api import (  # noqa: F401
     Lattice,
     LatticeType,
     Precision,
+    ProgressConfig,
+    ProgressSnapshot,
     Provider,
+    ReportConfig,
     Runtime,
     Selection,
     SelectorType,
     ShrinkPlan,
     ShrinkingLattice,
     SimulationConfig,
+    SimulationMode,
     SimulationOptions,
     SimulationResult,
+    SimulationRunStatus,
+    SimulationTimings,
+    TerminationReason,
     TreeStats,
+    TrueStatePolicy,
     Unsupported,
     available_providers,
     batch_round,
+    clear_simulation_cancellation,
     exact_upset_mass,
     hip_available,
     lattice_footprint,
     make_shrink_plan,
+    request_simulation_cancellation,
     reverse_bits,
     run_simulation,
+    simulation_cancellation_reason,
+    simulation_cancellation_requested,
 )
 
 __all__ = [
     "DilutionParameters", "DilutionMode", "DilutionTable", "Error", "InvalidArgument", "KernelSpec", "Lattice",
-    "LatticeType", "Precision", "Provider", "Runtime", "Selection", "SelectorType", "ShrinkPlan", "ShrinkingLattice", "SimulationConfig",
-    "SimulationOptions", "SimulationResult", "TreeStats", "Unsupported", "available_providers", "batch_round",
-    "exact_upset_mass", "hip_available", "lattice_footprint", "make_shrink_plan",
-    "reverse_bits", "run_simulation",
+    "LatticeType", "Precision", "ProgressConfig", "ProgressSnapshot", "Provider", "ReportConfig", "Runtime",
+    "Selection", "SelectorType", "ShrinkPlan", "ShrinkingLattice", "SimulationConfig", "SimulationMode",
+    "SimulationOptions", "SimulationResult", "SimulationRunStatus", "SimulationTimings", "TerminationReason",
+    "TreeStats", "TrueStatePolicy", "Unsupported", "available_providers", "batch_round",
+    "clear_simulation_cancellation", "exact_upset_mass", "hip_available", "lattice_footprint", "make_shrink_plan",
+    "request_simulation_cancellation", "reverse_bits", "run_simulation", "simulation_cancellation_reason",
+    "simulation_cancellation_requested",
 ]

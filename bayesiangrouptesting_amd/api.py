@@ -10,12 +10,14 @@ from __future__ import annotations
 
 import ctypes
 import enum
+import time
 from dataclasses import dataclass, field
 from typing import Optional, Sequence
 
 import numpy as np
 
 from . import _native as nat
+from . import reporting
 from ._native import Error, InvalidArgument, Unsupported  # noqa: F401
 
 
@@ -548,15 +550,83 @@ class DilutionParameters:
     h: float = 0.005
 
 
+class SimulationMode(enum.IntEnum):
+    # include/bgt/simulation.hpp:13-21
+    local_tree = 0
+    parallel_global_tree = 1
+    parallel_dynamic_tree = 2
+    parallel_hybrid_tree = 3
+    parallel_partial_tree = 4
+    parallel_fusion_tree = 5
+
+
+class TrueStatePolicy(enum.IntEnum):
+    # include/bgt/simulation.hpp:23-28
+    all = 0
+    trimmed = 1
+    symmetric = 2
+
+
+class SimulationRunStatus(enum.IntEnum):
+    # include/bgt/simulation.hpp:30-35
+    completed = reporting.STATUS_COMPLETED
+    interrupted = reporting.STATUS_INTERRUPTED
+    running = reporting.STATUS_RUNNING
+
+
+class TerminationReason(enum.IntEnum):
+    # include/bgt/simulation.hpp:37-42
+    none = reporting.REASON_NONE
+    user_signal = reporting.REASON_USER_SIGNAL
+    cancellation_requested = reporting.REASON_CANCELLATION_REQUESTED
+
+
+@dataclass
+class ReportConfig:
+    # include/bgt/simulation.hpp:50-55
+    write_csv: bool = False
+    output_directory: str = "."
+    run_name: str = ""
+
+
+@dataclass
+class SimulationTimings:
+    # include/bgt/simulation.hpp:57-63
+    setup_seconds: float = 0.0
+    tree_build_seconds: float = 0.0
+    evaluation_seconds: float = 0.0
+    total_seconds: float = 0.0
+
+
+@dataclass
+class ProgressConfig:
+    # include/bgt/simulation.hpp:65-72
+    enabled: bool = False
+    interval_seconds: float = 5.0
+    output_path: str = ""
+    write_jsonl: bool = True
+    print_stderr: bool = False
+
+
 @dataclass
 class SimulationConfig:
-    # include/bgt/simulation.hpp:87-103 (hot-path fields)
+    # include/bgt/simulation.hpp:87-103. mode, true_state_policy, global_tree_depth,
+    # workload_granularity, trim_percent and evaluation_prior configure the reference's
+    # multi-rank modes; they are validated here as there, and the local tree ignores them.
     lattice_type: LatticeType = LatticeType.replicated_non_dilution
     subjects: int = 0
     variants: int = 1
     prior: list = field(default_factory=list)
+    evaluation_prior: Optional[list] = None
     options: SimulationOptions = field(default_factory=SimulationOptions)
+    mode: SimulationMode = SimulationMode.local_tree
+    true_state_policy: TrueStatePolicy = TrueStatePolicy.all
+    global_tree_depth: int = 0
+    workload_granularity: int = 1
+    trim_percent: float = 1.0
     dilution: DilutionParameters = field(default_factory=DilutionParameters)
+    report: ReportConfig = field(default_factory=ReportConfig)
+    progress: ProgressConfig = field(default_factory=ProgressConfig)
 
 
 @dataclass
@@ -579,11 +649,53 @@ class KernelSpec:
 
 @dataclass
 class SimulationResult:
-    stats: TreeStats
-    kernel: KernelSpec
-    provider: Provider
-    total_states: int
-    evaluated_states: int
+    # include/bgt/simulation.hpp:105-121
+    stats: TreeStats = field(default_factory=TreeStats)
+    kernel: KernelSpec = field(default_factory=KernelSpec)
+    mode: SimulationMode = SimulationMode.local_tree
+    provider: Provider = Provider.auto_select
+    rank: int = 0
+    world_size: int = 1
+    total_states: int = 0
+    evaluated_states: int = 0
+    report_path: str = ""
+    timings: SimulationTimings = field(default_factory=SimulationTimings)
+    status: SimulationRunStatus = SimulationRunStatus.completed
+    termination_reason: TerminationReason = TerminationReason.none
+    evaluated_prior_mass: float = 0.0
+    progress_path: str = ""
+
+
+@dataclass
+class ProgressSnapshot:
+    # include/bgt/simulation.hpp:74-85
+    status: SimulationRunStatus = SimulationRunStatus.completed
+    reason: TerminationReason = TerminationReason.none
+    total_states: int = 0
+    evaluated_states: int = 0
+    state_coverage_fraction: float = 0.0
+    evaluated_prior_mass: float = 0.0
+    stats: TreeStats = field(default_factory=TreeStats)
+    timings: SimulationTimings = field(default_factory=SimulationTimings)
+    message: str = ""
+
+
+def clear_simulation_cancellation() -> None:
+    # progress.cpp:87-90
+    reporting.clear_cancellation()
+
+
+def request_simulation_cancellation(reason: TerminationReason = TerminationReason.cancellation_requested) -> None:
+    # progress.cpp:92-98: the first reason requested stays until cleared
+    reporting.request_cancellation(int(reason))
+
+
+def simulation_cancellation_requested() -> bool:
+    return reporting.cancellation_reason() != reporting.REASON_NONE
+
+
+def simulation_cancellation_reason() -> TerminationReason:
+    return TerminationReason(reporting.cancellation_reason())
 
 
 def _resolve_selector(selector: SelectorType, variants: int) -> SelectorType:
@@ -594,17 +706,38 @@ def _resolve_selector(selector: SelectorType, variants: int) -> SelectorType:
 
 
 def _validate(config: SimulationConfig) -> None:
-    # simulation_runner.cpp:30-68 (local-tree subset)
+    # simulation_runner.cpp:30-68
     if config.subjects <= 0 or config.variants <= 0 or config.options.search_depth < 0:
         raise InvalidArgument(nat.BAD_INPUT, "invalid simulation dimensions.")
+    if config.workload_granularity <= 0:
+        raise InvalidArgument(nat.BAD_INPUT, "workload_granularity must be positive.")
+    if config.progress.interval_seconds <= 0.0:
+        raise InvalidArgument(nat.BAD_INPUT, "progress interval must be positive.")
     if len(config.prior) != config.subjects * config.variants:
         raise InvalidArgument(nat.BAD_INPUT, "prior length must equal subjects * variants.")
+    if config.evaluation_prior is not None and len(config.evaluation_prior) != config.subjects * config.variants:
+        raise InvalidArgument(nat.BAD_INPUT, "evaluation_prior length must equal subjects * variants.")
+    if config.options.selector not in (SelectorType.auto_select, SelectorType.op_bha, SelectorType.op_bbpa):
+        raise InvalidArgument(nat.BAD_INPUT, "unsupported selector.")
     if _resolve_selector(config.options.selector, config.variants) == SelectorType.op_bha and config.variants != 1:
         raise InvalidArgument(nat.BAD_INPUT, "Op-BHA tree selection is only supported for binary lattices.")
     if _uses_dilution(config.lattice_type) and (config.dilution.alpha <= 0.0 or config.dilution.h <= 0.0):
         raise InvalidArgument(nat.BAD_INPUT, "dilution parameters must be positive.")
-    if config.lattice_type in (LatticeType.distributed_dilution, LatticeType.distributed_non_dilution):
+    distributed = config.lattice_type in (LatticeType.distributed_dilution, LatticeType.distributed_non_dilution)
+    if config.mode == SimulationMode.local_tree and distributed:
         raise InvalidArgument(nat.BAD_INPUT, "local_tree requires a replicated lattice.")
+    if config.mode in (SimulationMode.parallel_dynamic_tree, SimulationMode.parallel_hybrid_tree) and distributed:
+        raise InvalidArgument(nat.BAD_INPUT,
+                              "dynamic and hybrid parallel tree modes currently require replicated lattices.")
+    if config.mode == SimulationMode.parallel_partial_tree:
+        if not distributed:
+            raise InvalidArgument(nat.BAD_INPUT, "partial parallel tree mode requires a distributed lattice.")
+        if config.true_state_policy == TrueStatePolicy.all:
+            raise InvalidArgument(nat.BAD_INPUT,
+                                  "partial parallel tree mode requires trimmed or symmetric true_state_policy.")
+    if config.mode != SimulationMode.local_tree:
+        raise Unsupported(nat.UNSUPPORTED, f"{reporting.mode_name(config.mode)} runs over several ranks; this "
+                          "provider runs the local tree on one GPU")
 
 
 class Runtime:
@@ -632,15 +765,42 @@ class Runtime:
         return len(self._registry)
 
     def run_simulation(self, config: SimulationConfig) -> SimulationResult:
+        # simulation_runner.cpp:181-237: the GPU provider's path (progress forced at the
+        # start, one device evaluation of every true state, the final snapshot, the report)
+        total_start = time.perf_counter()
         _validate(config)
-        spec = self.make_kernel_spec(config)
-        self._registry.add(spec.key())
-        if config.options.provider in (Provider.cpu, Provider.cuda):
-            raise Unsupported(nat.UNSUPPORTED, f"{config.options.provider.name} provider is not part of this build")
-        prior = np.ascontiguousarray(np.asarray(config.prior, dtype=np.float64))
         dil = None
         if _uses_dilution(config.lattice_type):
             dil = DilutionTable(config.subjects, config.dilution.alpha, config.dilution.h)
+        spec = self.make_kernel_spec(config)
+        self._registry.add(spec.key())
+        result = SimulationResult(kernel=spec, mode=config.mode, rank=0, world_size=1,
+                                  total_states=1 << (config.subjects * config.variants), evaluated_states=0)
+
+        def snapshot(status, reason, message):
+            # progress.cpp:203-221
+            t = SimulationTimings(**vars(result.timings))
+            t.total_seconds = time.perf_counter() - total_start
+            done = progress.evaluated_states
+            return ProgressSnapshot(SimulationRunStatus(status), TerminationReason(reason), result.total_states, done,
+                                    0.0 if result.total_states == 0 else done / result.total_states,
+                                    progress.evaluated_prior_mass,
+                                    progress.stats if progress.stats is not None else TreeStats(), t, message)
+
+        progress = reporting.ProgressTracker(config, result.rank, result.total_states, total_start, snapshot)
+        eval_start = time.perf_counter()
+        if config.options.provider == Provider.cpu:
+            raise Unsupported(nat.UNSUPPORTED, "cpu provider is not part of this build")
+        if progress.should_stop():
+            result.status = SimulationRunStatus.interrupted
+            result.termination_reason = simulation_cancellation_reason()
+            result.progress_path = progress.progress_path
+            result.timings.total_seconds = time.perf_counter() - total_start
+            progress.finish(result.status, result.termination_reason, "cancelled before evaluation started")
+            result.report_path = reporting.write_report_if_requested(config, result)
+            return result
+        progress.maybe_emit("starting HIP evaluation", True)
+        prior = np.ascontiguousarray(np.asarray(config.prior, dtype=np.float64))
         st = nat.TreeStatsC()
         o = config.options
         nat.check(nat.lib().bgt_hip_provider_run_precision(
@@ -648,9 +808,18 @@ class Runtime:
             prior.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), o.search_depth, o.threshold_up, o.threshold_lo,
             o.branch_threshold, int(o.selector), dil.pointer() if dil is not None else None,
             _storage(o.precision), ctypes.byref(st)))
+        result.stats = TreeStats.from_c(st)
+        result.provider = Provider.hip
         spec.provider = Provider.hip
-        total = 1 << (config.subjects * config.variants)
-        return SimulationResult(TreeStats.from_c(st), spec, Provider.hip, total, total)
+        result.evaluated_states = result.total_states
+        result.evaluated_prior_mass = 1.0
+        progress.replace_state(result.evaluated_states, result.evaluated_prior_mass, TreeStats(**vars(result.stats)))
+        result.timings.evaluation_seconds = time.perf_counter() - eval_start
+        result.timings.total_seconds = time.perf_counter() - total_start
+        result.progress_path = progress.progress_path
+        progress.finish(result.status, result.termination_reason, "completed")
+        result.report_path = reporting.write_report_if_requested(config, result)
+        return result
 
 
 def run_simulation(config: SimulationConfig) -> SimulationResult:

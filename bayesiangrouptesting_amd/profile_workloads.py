@@ -8,6 +8,9 @@
         WRITE_SIZE (bytes the counters report per byte actually moved)
     python -m bayesiangrouptesting_amd.profile_workloads bbpa24 [fp32c|fp64] [rounds]
         n = 8, k = 3 dilution lattice (2^24 states): Op-BBPA select + update rounds
+    python -m bayesiangrouptesting_amd.profile_workloads lookahead28 [fp32c|fp64] [calls]
+        n = 28 binary lattice (the benchmark's priors) after 4 seeded updates: 5-step
+        select_experiments calls
 """
 from __future__ import annotations
 
@@ -70,6 +73,30 @@ def bbpa24(form: str = "fp32c", rounds: int = 4) -> None:
     lat.close()
 
 
+def lookahead28(form: str = "fp64", calls: int = 2) -> None:
+    import bayesiangrouptesting_amd as bgt
+    from bayesiangrouptesting_amd import _native as nat
+    from bayesiangrouptesting_amd import workload
+
+    c = workload.C2
+    n = c.subjects
+    pi = workload.priors(c.seed, n, c.prior_lo, c.prior_hi)
+    prec = bgt.Precision.float32_compensated if form == "fp32c" else bgt.Precision.float64
+    lat = bgt.Lattice(bgt.LatticeType.replicated_non_dilution, n, pi, precision=prec)
+    seed = c.seed * 1000
+    sampler = workload.ResponseSampler(seed, n, 1, bgt.reverse_bits(workload.true_state(seed, pi), n))
+    for _ in range(4):
+        e = lat.select(bgt.SelectorType.op_bha).experiment
+        lat.update(e, sampler.draw(e))
+    for i in range(calls):
+        t0 = time.perf_counter()
+        picks = lat.select_experiments(5)
+        nat.check(nat.lib().bgt_hip_device_synchronize())
+        print(f"lookahead28 {form}: call {i}, 5 steps, {(time.perf_counter() - t0) * 1e3:.3f} ms, picks {picks}",
+              flush=True)
+    lat.close()
+
+
 def copy(mib: int = 1024) -> None:
     hip = ctypes.CDLL("libamdhip64.so")
     nbytes = mib << 20
@@ -94,6 +121,8 @@ def main(argv) -> int:
         round28(argv[1] if len(argv) > 1 else "fp32c", int(argv[2]) if len(argv) > 2 else 4)
     elif what == "bbpa24":
         bbpa24(argv[1] if len(argv) > 1 else "fp32c", int(argv[2]) if len(argv) > 2 else 4)
+    elif what == "lookahead28":
+        lookahead28(argv[1] if len(argv) > 1 else "fp64", int(argv[2]) if len(argv) > 2 else 2)
     elif what == "copy":
         copy(int(argv[1]) if len(argv) > 1 else 1024)
     else:

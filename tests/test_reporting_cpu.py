@@ -1,0 +1,172 @@
+"""Run formats on the CPU: the CSV report (reporting.cpp:10-46), the progress JSON and its
+files (progress.cpp:46-279), cancellation before evaluation (simulation_runner.cpp:205-215,
+the interrupted half of simulation_progress_test.cpp), configuration checks
+(simulation_runner.cpp:30-68) and the bgt_simulate flag parsing (run_simulation.cpp:98-240).
+None of these reaches the device."""
+import json
+import os
+
+import pytest
+
+import bayesiangrouptesting_amd as bgt
+from bayesiangrouptesting_amd import reporting, simulate
+from oracle import pyoracle as orc
+
+CSV_FIELDS = ["status", "termination_reason", "is_partial", "mode", "provider", "subjects", "variants", "world_size",
+              "total_states", "evaluated_states", "state_coverage_fraction", "evaluated_prior_mass", "progress_path",
+              "total_leaves", "correct_probability", "incorrect_probability", "false_positive_probability",
+              "false_negative_probability", "unclassified_probability", "expected_stages", "expected_tests",
+              "setup_seconds", "tree_build_seconds", "evaluation_seconds", "total_seconds"]
+
+
+def sample_result():
+    st = bgt.TreeStats(11, 0.123456789, 0.25, 1e-7, 0.0, 2.5e-10, 3.0, 4.75)
+    t = bgt.SimulationTimings(0.0, 0.0, 0.015625, 1.0 / 3.0)
+    return bgt.SimulationResult(stats=st, provider=bgt.Provider.hip, total_states=8, evaluated_states=4,
+                                timings=t, evaluated_prior_mass=0.9, progress_path="runs/x.progress.json")
+
+
+def test_progress_defaults():
+    # simulation_progress_test.cpp:26-30
+    d = bgt.ProgressConfig()
+    assert not d.enabled and d.interval_seconds == 5.0 and d.write_jsonl and not d.print_stderr
+
+
+def test_csv_report_fields_and_number_format():
+    cfg = bgt.SimulationConfig(subjects=3, variants=1, prior=[0.1] * 3)
+    text = reporting.report_text(cfg, sample_result())
+    lines = text.splitlines()
+    assert lines[0] == "field,value"
+    rows = dict(line.split(",", 1) for line in lines[1:])
+    assert [line.split(",", 1)[0] for line in lines[1:]] == CSV_FIELDS
+    # doubles as a default-formatted ostream writes them (6 significant digits)
+    assert rows["correct_probability"] == "0.123457"
+    assert rows["false_positive_probability"] == "1e-07"
+    assert rows["unclassified_probability"] == "2.5e-10"
+    assert rows["expected_stages"] == "3"
+    assert rows["state_coverage_fraction"] == "0.5"
+    assert rows["total_seconds"] == "0.333333"
+    assert rows["is_partial"] == "false" and rows["status"] == "completed" and rows["mode"] == "local_tree"
+    assert rows["provider"] == str(int(bgt.Provider.hip)) and rows["total_leaves"] == "11"
+
+
+def test_progress_json_round_trips_every_double():
+    r = sample_result()
+    snap = bgt.ProgressSnapshot(bgt.SimulationRunStatus.running, bgt.TerminationReason.none, 8, 4, 0.5, 0.9,
+                                r.stats, r.timings, 'line "one"\n\tback\\slash')
+    body = reporting.snapshot_json(snap)
+    assert "\n" not in body
+    d = json.loads(body)
+    assert list(d) == ["status", "termination_reason", "total_states", "evaluated_states", "state_coverage_fraction",
+                       "evaluated_prior_mass", "stats", "timings", "message"]
+    assert d["status"] == "running" and d["termination_reason"] == "none"
+    assert d["stats"]["correct_probability"] == 0.123456789 and d["timings"]["total_seconds"] == 1.0 / 3.0
+    assert d["message"] == 'line "one"\n\tback\\slash'
+    assert '"evaluated_states":4,' in body and '"evaluated_prior_mass":0.90000000000000002,' in body
+
+
+def test_stderr_progress_line():
+    r = sample_result()
+    snap = bgt.ProgressSnapshot(bgt.SimulationRunStatus.running, bgt.TerminationReason.none, 8, 4, 0.5, 0.9,
+                                r.stats, r.timings, "evaluating")
+    assert reporting.stderr_line(snap) == ("bgt progress: status=running evaluated=4/8 coverage=0.5 prior_mass=0.9 "
+                                           "elapsed=0.333333s evaluating")
+
+
+def test_cancellation_keeps_the_first_reason():
+    bgt.clear_simulation_cancellation()
+    assert not bgt.simulation_cancellation_requested()
+    bgt.request_simulation_cancellation(bgt.TerminationReason.user_signal)
+    bgt.request_simulation_cancellation(bgt.TerminationReason.cancellation_requested)
+    assert bgt.simulation_cancellation_reason() == bgt.TerminationReason.user_signal
+    bgt.clear_simulation_cancellation()
+    bgt.request_simulation_cancellation(bgt.TerminationReason.none)
+    assert bgt.simulation_cancellation_reason() == bgt.TerminationReason.cancellation_requested
+    bgt.clear_simulation_cancellation()
+
+
+def test_interrupted_before_evaluation_writes_partial_files(tmp_path):
+    # simulation_progress_test.cpp:68-84: cancellation requested before the run
+    cfg = bgt.SimulationConfig(subjects=2, prior=[0.2, 0.3])
+    cfg.options.search_depth = 2
+    cfg.report.write_csv = True
+    cfg.report.output_directory = str(tmp_path)
+    cfg.report.run_name = "interrupted"
+    cfg.progress.enabled = True
+    cfg.progress.interval_seconds = 60.0
+    bgt.request_simulation_cancellation(bgt.TerminationReason.cancellation_requested)
+    try:
+        r = bgt.run_simulation(cfg)
+    finally:
+        bgt.clear_simulation_cancellation()
+    assert r.status == bgt.SimulationRunStatus.interrupted
+    assert r.termination_reason == bgt.TerminationReason.cancellation_requested
+    assert r.evaluated_states == 0 and r.evaluated_prior_mass == 0.0 and r.total_states == 4
+    assert os.path.exists(tmp_path / "interrupted.partial.csv")
+    body = (tmp_path / "interrupted.progress.json").read_text()
+    assert '"status":"interrupted"' in body and '"evaluated_states":0' in body
+    assert (tmp_path / "interrupted.progress.jsonl").read_text() == body
+    report = (tmp_path / "interrupted.partial.csv").read_text()
+    assert "status,interrupted" in report and "is_partial,true" in report
+    assert f"progress_path,{tmp_path / 'interrupted.progress.json'}" in report
+
+
+@pytest.mark.parametrize("change,message", [
+    (lambda c: setattr(c, "workload_granularity", 0), "workload_granularity must be positive"),
+    (lambda c: setattr(c.progress, "interval_seconds", 0.0), "progress interval must be positive"),
+    (lambda c: setattr(c, "evaluation_prior", [0.1]), "evaluation_prior length must equal"),
+    (lambda c: setattr(c.options, "selector", bgt.SelectorType.brute_force), "unsupported selector"),
+    (lambda c: setattr(c, "lattice_type", bgt.LatticeType.distributed_dilution), "local_tree requires a replicated"),
+])
+def test_configuration_checks(change, message):
+    cfg = bgt.SimulationConfig(subjects=2, prior=[0.2, 0.3])
+    change(cfg)
+    with pytest.raises(bgt.InvalidArgument, match=message):
+        bgt.run_simulation(cfg)
+
+
+def test_parallel_modes_are_checked_then_refused():
+    cfg = bgt.SimulationConfig(subjects=2, prior=[0.2, 0.3], mode=bgt.SimulationMode.parallel_partial_tree)
+    with pytest.raises(bgt.InvalidArgument, match="requires a distributed lattice"):
+        bgt.run_simulation(cfg)
+    cfg.mode = bgt.SimulationMode.parallel_dynamic_tree
+    with pytest.raises(bgt.Unsupported, match="parallel_dynamic_tree"):
+        bgt.run_simulation(cfg)
+
+
+def test_cli_parsing(capsys):
+    cfg, prior, eprior, help_ = simulate.parse_args(
+        ["--subjects", "3", "--variants", "2", "--prior", "0.02", "--lattice", "replicated_dilution", "--search-depth",
+         "5", "--provider", "cuda", "--selector", "op_bbpa", "--threshold-up", "0.02", "--branch-threshold", "0.001",
+         "--dilution-alpha", "0.05", "--dilution-h", "0.9", "--write-csv", "--output-dir", "out", "--run-name", "r",
+         "--progress", "--no-progress-stderr", "--progress-interval", "2", "--evaluation-prior", "0.1,0.2,,0.3",
+         "--true-state-policy", "trimmed", "--trim-percent", "5", "--precision", "float32_compensated"])
+    assert not help_ and prior == "0.02" and eprior == "0.1,0.2,,0.3"
+    assert cfg.subjects == 3 and cfg.variants == 2 and cfg.lattice_type == bgt.LatticeType.replicated_dilution
+    assert cfg.options.search_depth == 5 and cfg.options.provider == bgt.Provider.hip
+    assert cfg.options.selector == bgt.SelectorType.op_bbpa and cfg.options.threshold_up == 0.02
+    assert cfg.options.branch_threshold == 0.001 and cfg.dilution.alpha == 0.05 and cfg.dilution.h == 0.9
+    assert cfg.report.write_csv and cfg.report.output_directory == "out" and cfg.report.run_name == "r"
+    assert cfg.progress.enabled and not cfg.progress.print_stderr and cfg.progress.interval_seconds == 2.0
+    assert cfg.true_state_policy == bgt.TrueStatePolicy.trimmed and cfg.trim_percent == 5.0
+    assert cfg.options.precision == bgt.Precision.float32_compensated
+    assert simulate.parse_prior("0.02", 6) == [0.02] * 6
+    assert simulate.parse_prior("0.1,0.2,,0.3", 6) == [0.1, 0.2, 0.3]
+    assert simulate.main(["--help"]) == 0
+    assert simulate.main(["--subjects", "3"]) == 2
+    assert simulate.main(["--subjects", "3", "--prior", "0.1", "--bogus"]) == 1
+    assert "bgt_simulate: unknown argument: --bogus" in capsys.readouterr().err
+    assert simulate.main(["--subjects"]) == 1
+    assert "missing value for --subjects" in capsys.readouterr().err
+    assert simulate.main(["--subjects", "2", "--prior", "0.1", "--mode", "sideways"]) == 1
+    assert "unknown mode: sideways" in capsys.readouterr().err
+    # a configuration error surfaces as a failed run with the runtime's message
+    assert simulate.main(["--subjects", "2", "--prior", "0.1,0.2,0.3"]) == 1
+    assert "bgt_simulate: prior length must equal subjects * variants." in capsys.readouterr().err
+
+
+def test_reference_cli_golden_on_the_oracle():
+    # CMakeLists.txt:529-531: bgt_simulate --subjects 3 --prior 0.02 --search-depth 5 --selector op_bbpa
+    # --threshold-up 0.01 --threshold-lo 0.01 --branch-threshold 0.001 reports total_leaves,11
+    st = orc.run_simulation(False, 3, 1, [0.02] * 3, 5, 0.01, 0.01, 0.001, selector=2)
+    assert st.total_leaves == 11
