@@ -142,6 +142,7 @@ constexpr int kExactMaxHist = kDenseMaxHist;
 constexpr int kExactMaxPools = kDenseMaxPools;
 // the exact band kernel stages one update and one prior value per lane of its wavefront
 static_assert(kExactMaxHist <= 64 && 2 * kMaxAtoms <= 64, "exact band staging is one wavefront wide");
+static_assert(kExactMaxPools <= 10, "the device's dense program holds at most 2^10 pool states in registers");
 
 constexpr int kLikCapacity = kMaxVariants * (kMaxSubjects + 1);
 

@@ -1827,6 +1827,87 @@ __global__ __launch_bounds__(256) void k_band_compact(Descs descs, unsigned raw_
 // wavefront. Every term is non-negative and every sum runs in a fixed order. Candidates
 // with more than kExactMaxPools distinct pools, or histories longer than kExactMaxHist,
 // get -1; candidates dominated by another band entry get -2 (see below).
+// Lane l's value, for a uniform l (no memory access).
+__device__ __forceinline__ uint32_t lane_u(uint32_t v, int l)
+{
+	return (uint32_t)__builtin_amdgcn_readlane((int)v, l);
+}
+__device__ __forceinline__ double lane_d(double v, int l)
+{
+	const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+	const uint32_t lo = lane_u((uint32_t)b, l), hi = lane_u((uint32_t)(b >> 32), l);
+	return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// The dense pool-state program of k_exact_band with the 2^c states in registers: state
+// k = lane + 64 r, RB = max(0, c - 6) register bits (R = 2^RB doubles per lane). A sweep
+// over a lane bit exchanges partners with a lane shuffle, one over a register bit pairs
+// registers; each element sees exactly the operations of the host's dense program, in the
+// same order, so the result is bit-identical to it. Lanes beyond 2^c (c < 6) hold zeros
+// and pair only with each other. Lane j < c holds pool j (pool, fin, fout); lane a < atoms
+// holds atom a's prior pair (pv, qv).
+template <int RB>
+__device__ __forceinline__ double exact_dense_regs(int c, uint32_t relevant, uint32_t pool, double pfin,
+												   double pfout, double pv, double qv)
+{
+	constexpr int R = 1 << RB;
+	const int lane = threadIdx.x;
+	const uint32_t states = 1u << c;
+	double a[R], b[R];
+#pragma unroll
+	for (int r = 0; r < R; ++r)
+		a[r] = ((uint32_t)lane + ((uint32_t)r << 6)) == states - 1u ? 1.0 : 0.0;
+	// one sweep over pool bit j: OP(low, high) gives the new value of the bit-clear
+	// element; the bit-set element becomes 0
+	auto sweep = [&](double *v, int j, auto op) {
+		if (j < 6) {
+			const bool hi = (lane >> j) & 1;
+#pragma unroll
+			for (int r = 0; r < R; ++r) {
+				const double o = __shfl_xor(v[r], 1 << j);
+				v[r] = hi ? 0.0 : op(v[r], o);
+			}
+		} else {
+			const int rj = j - 6;
+#pragma unroll
+			for (int bit = 0; bit < RB; ++bit) {
+				if (bit == rj) {
+#pragma unroll
+					for (int r = 0; r < R; ++r)
+						if (!((r >> bit) & 1)) {
+							v[r] = op(v[r], v[r | (1 << bit)]);
+							v[r | (1 << bit)] = 0.0;
+						}
+				}
+			}
+		}
+	};
+	const bool mine = lane < c;
+	uint32_t done = 0;
+	for (uint32_t left = relevant; left; left &= left - 1u) {
+		const int at = __ffs(left) - 1;
+		done |= 1u << at;
+		const bool in = mine && ((pool >> at) & 1u);
+		const uint32_t kill = (uint32_t)__ballot(in);
+		const uint32_t close = (uint32_t)__ballot(in && (pool & ~done) == 0u);
+		const double p = lane_d(pv, at), q = lane_d(qv, at);
+#pragma unroll
+		for (int r = 0; r < R; ++r)
+			b[r] = a[r];
+		for (uint32_t bits = kill; bits; bits &= bits - 1u)
+			sweep(b, __ffs(bits) - 1, [](double lo, double hi) { return lo + hi; });
+#pragma unroll
+		for (int r = 0; r < R; ++r)
+			a[r] = a[r] * q + b[r] * p;
+		for (uint32_t bits = close; bits; bits &= bits - 1u) {
+			const int j = __ffs(bits) - 1;
+			const double fi = lane_d(pfin, j), fo = lane_d(pfout, j);
+			sweep(a, j, [fi, fo](double lo, double hi) { return lo * fo + hi * fi; });
+		}
+	}
+	return lane_d(a[0], 0); // every pool has closed: the weight sits in state 0 (lane 0)
+}
+
 __global__ __launch_bounds__(64) void k_exact_band(Descs descs, int atoms)
 {
 	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
@@ -1836,14 +1917,9 @@ __global__ __launch_bounds__(64) void k_exact_band(Descs descs, int atoms)
 	const unsigned i = blockIdx.x;
 	if (i >= count || i >= (unsigned)kExactBand)
 		return;
-	__shared__ double a[1 << kExactMaxPools];
-	__shared__ double b[1 << kExactMaxPools];
-	__shared__ double s_pairs[2 * kMaxAtoms];
-	__shared__ uint32_t s_rest[kExactMaxHist];
-	__shared__ double s_hin[kExactMaxHist], s_hout[kExactMaxHist];
 	__shared__ uint32_t s_pool[kExactMaxPools];
 	__shared__ double s_fin[kExactMaxPools], s_fout[kExactMaxPools];
-	const int lane = threadIdx.x; // one wavefront: lane t stages update t and pair t
+	const int lane = threadIdx.x; // one wavefront
 	const int h = d.hist_len;
 	const Cand me = d.band[i];
 	const uint32_t e = me.e;
@@ -1885,32 +1961,42 @@ __global__ __launch_bounds__(64) void k_exact_band(Descs descs, int atoms)
 			d.exact_u[i] = -1.0;
 		return;
 	}
-	if (lane < 2 * kMaxAtoms)
-		s_pairs[lane] = lane < 2 * atoms ? d.pairs[lane] : 1.0;
+	// lane t stages update t (its remainder outside E and factors), lane a atom a's pair
 	uint32_t rest = 0;
-	double fin = 1.0, fout = 1.0;
+	double fin0 = 1.0, fout0 = 1.0;
 	if (lane < h) {
 		const ExactRec r = d.hist[lane];
 		rest = r.e & ~e;
-		fin = r.fin;
-		fout = r.fout;
+		fin0 = r.fin;
+		fout0 = r.fout;
 	}
-	s_rest[lane] = rest;
-	s_hin[lane] = fin;
-	s_hout[lane] = fout;
-	__syncthreads();
-	// Updates whose pool lies inside E always see it contained (factor fin); the others
-	// group by their remaining atoms, one pool per distinct remainder in order of first
-	// occurrence, with the factors of the group multiplied in update order.
+	const double pv = lane < atoms ? d.pairs[2 * lane] : 1.0;
+	const double qv = lane < atoms ? d.pairs[2 * lane + 1] : 1.0;
+	// Q(E) times the always-contained factors (updates whose pool lies inside E see it
+	// contained), in the host's order
+	double lead = 1.0;
+	for (int at = 0; at < atoms; ++at) {
+		const double x = lane_d(qv, at);
+		if ((e >> at) & 1u)
+			lead *= x;
+	}
+	// The other updates group by their remaining atoms, one pool per distinct remainder in
+	// order of first occurrence, with the factors of the group multiplied in update order.
 	bool first = lane < h && rest != 0u;
-	for (int t = 0; t < lane && first; ++t)
-		first = s_rest[t] != rest;
-	if (first) {
-		for (int t = lane + 1; t < h; ++t)
-			if (s_rest[t] == rest) {
-				fin *= s_hin[t];
-				fout *= s_hout[t];
+	double fin = fin0, fout = fout0;
+	for (int t = 0; t < h; ++t) {
+		const uint32_t rt = lane_u(rest, t);
+		const double fi = lane_d(fin0, t), fo = lane_d(fout0, t);
+		if (rt == 0u)
+			lead *= fi;
+		if (rt == rest) {
+			if (t < lane)
+				first = false;
+			else if (t > lane) {
+				fin *= fi;
+				fout *= fo;
 			}
+		}
 	}
 	const unsigned long long firsts = __ballot(first);
 	const int c = __popcll(firsts);
@@ -1925,70 +2011,22 @@ __global__ __launch_bounds__(64) void k_exact_band(Descs descs, int atoms)
 		s_fin[j] = fin;
 		s_fout[j] = fout;
 	}
-	double lead = 1.0; // Q(E) times the always-contained factors, in the host's order
-	if (lane == 0) {
-		for (int at = 0; at < atoms; ++at)
-			if ((e >> at) & 1u)
-				lead *= s_pairs[2 * at + 1];
-		for (int t = 0; t < h; ++t)
-			if (s_rest[t] == 0u)
-				lead *= s_hin[t];
-	}
-	const uint32_t states = 1u << c;
-	for (uint32_t k = lane; k < states; k += 64)
-		a[k] = k == states - 1 ? 1.0 : 0.0;
 	__syncthreads();
+	const uint32_t pool = lane < c ? s_pool[lane] : 0u;
+	const double pfin = lane < c ? s_fin[lane] : 1.0, pfout = lane < c ? s_fout[lane] : 1.0;
 	uint32_t relevant = 0;
 	for (int j = 0; j < c; ++j)
-		relevant |= s_pool[j];
-	uint32_t done = 0;
-	for (uint32_t left = relevant; left; left &= left - 1u) {
-		const int at = __ffs(left) - 1;
-		done |= 1u << at;
-		uint32_t kill = 0, close = 0;
-		for (int j = 0; j < c; ++j)
-			if ((s_pool[j] >> at) & 1u) {
-				kill |= 1u << j;
-				if ((s_pool[j] & ~done) == 0)
-					close |= 1u << j;
-			}
-		const double p = s_pairs[2 * at], q = s_pairs[2 * at + 1];
-		const uint32_t half = states >> 1;
-		// a[k] <- a[k] q + [k has no kill bit] p sum_{sub <= kill} a[k | sub]: the sum
-		// projects the kill bits out of a copy, one pool bit per sweep
-		for (uint32_t k = lane; k < states; k += 64)
-			b[k] = a[k];
-		__syncthreads();
-		for (uint32_t bits = kill; bits; bits &= bits - 1u) {
-			const int j = __ffs(bits) - 1;
-			const uint32_t low = (1u << j) - 1u;
-			for (uint32_t x = lane; x < half; x += 64) {
-				const uint32_t k = ((x & ~low) << 1) | (x & low); // bit j clear
-				b[k] += b[k | (1u << j)];
-				b[k | (1u << j)] = 0.0;
-			}
-			__syncthreads();
-		}
-		for (uint32_t k = lane; k < states; k += 64)
-			a[k] = a[k] * q + b[k] * p;
-		__syncthreads();
-		// close the pools whose last atom this was, one per sweep: the state without the
-		// pool's bit takes fout times its weight plus fin times the weight with it
-		for (uint32_t bits = close; bits; bits &= bits - 1u) {
-			const int j = __ffs(bits) - 1;
-			const uint32_t low = (1u << j) - 1u;
-			const double fi = s_fin[j], fo = s_fout[j];
-			for (uint32_t x = lane; x < half; x += 64) {
-				const uint32_t k = ((x & ~low) << 1) | (x & low);
-				a[k] = a[k] * fo + a[k | (1u << j)] * fi;
-				a[k | (1u << j)] = 0.0;
-			}
-			__syncthreads();
-		}
+		relevant |= lane_u(pool, j);
+	double w;
+	switch (c <= 6 ? 0 : c - 6) {
+	case 0: w = exact_dense_regs<0>(c, relevant, pool, pfin, pfout, pv, qv); break;
+	case 1: w = exact_dense_regs<1>(c, relevant, pool, pfin, pfout, pv, qv); break;
+	case 2: w = exact_dense_regs<2>(c, relevant, pool, pfin, pfout, pv, qv); break;
+	case 3: w = exact_dense_regs<3>(c, relevant, pool, pfin, pfout, pv, qv); break;
+	default: w = exact_dense_regs<4>(c, relevant, pool, pfin, pfout, pv, qv); break;
 	}
-	// every pool has closed: the weight sits in state 0
 	if (lane == 0)
-		d.exact_u[i] = lead * a[0];
+		d.exact_u[i] = lead * w;
 }
 
 // Result heads of a batch of lattices into one staging buffer (one copy to the host).

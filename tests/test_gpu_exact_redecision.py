@@ -75,3 +75,32 @@ def test_fp32c_and_fp64_pick_the_same_along_a_trajectory():
         sb = b.round(e, r, 0.01, 0.01)
         assert sb.redecided or not sb.near_tie
     assert resolved >= 4 and same >= resolved
+
+
+def test_device_exact_masses_equal_the_host_program_bitwise():
+    # k_exact_band runs the host's dense pool-state program on the device (same operations per
+    # state, same order): a round re-decided by exact masses (candidates > 0; a pick that
+    # dominance alone decides keeps the device's objective) reports the objective built from
+    # the device's mass of the pick, equal to the host program's to the bit
+    n = 24
+    seed = 2
+    pi = seeded_prior(seed, n, 0.005, 0.05)
+    lat = bgt.Lattice(ND, n, pi, precision=FP32C)
+    truth = workload.true_state(seed, list(pi))
+    sampler = workload.ResponseSampler(seed, n, 1, bgt.reverse_bits(truth, n))
+    hist = []
+    seen = []
+    checked = 0
+    s = lat.select(bgt.SelectorType.op_bha)
+    for step in range(12):
+        e_pub = s.experiment
+        r = sampler.draw(e_pub)
+        hist.append((orc.reverse_bits(e_pub, n), r))
+        s = lat.round(e_pub, r, 0.01, 0.01)
+        seen.append((s.redecided, s.near_tie, s.band))
+        if s.redecided == 1 and not s.near_tie and s.candidates > 0:
+            e = orc.reverse_bits(s.experiment, n)
+            m = float(bgt.exact_upset_mass(n, pi, hist, np.array([e]))[0])
+            assert s.objective == abs(m - 0.5), (step, s, m)
+            checked += 1
+    assert checked >= 4, seen
