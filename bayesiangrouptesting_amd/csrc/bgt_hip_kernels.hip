@@ -1502,35 +1502,35 @@ __global__ __launch_bounds__(256) void k_down_first(Descs descs, double *D, int 
 		D[base + i] = t[i];
 }
 
-// Pass B: D over bits b0..b0+nb-1 (nb <= 8) on tiles of 32 contiguous entries (bits 0..4,
-// already summed) by 2^nb rows.
-__global__ __launch_bounds__(256) void k_down_more(double *D, int n, int b0, int nb)
+// Pass B: D over bits b0..b0+nb-1 (nb <= 8) on tiles of 16 contiguous entries (bits 0..3,
+// 128-byte rows) by 2^nb rows; 32 KB of LDS, so several workgroups share a CU.
+constexpr int kDownCols = 16;
+__global__ __launch_bounds__(256) void k_down_more(double *D, int b0, int nb)
 {
-	__shared__ double t[32 * 256];
+	__shared__ double t[kDownCols * 256];
 	const uint32_t rows = 1u << nb;
-	// tile: columns c0..c0+31 (bits 0..b0-1 of the tile's start), the bits above b0+nb
-	const uint64_t cols = 1ull << (b0 - 5); // column groups of 32 below bit b0
+	// tile: kDownCols columns below bit b0 (group cg), the bits above b0+nb (hi)
+	const uint64_t cols = 1ull << (b0 - 4); // column groups below bit b0
 	const uint64_t cg = blockIdx.x % cols, hi = blockIdx.x / cols;
-	const uint64_t base = (cg << 5) | (hi << (b0 + nb));
-	for (uint32_t i = threadIdx.x; i < 32 * rows; i += blockDim.x) {
-		const uint32_t r = i >> 5, c = i & 31u;
+	const uint64_t base = (cg << 4) | (hi << (b0 + nb));
+	for (uint32_t i = threadIdx.x; i < kDownCols * rows; i += blockDim.x) {
+		const uint32_t r = i >> 4, c = i & 15u;
 		t[i] = D[base + c + ((uint64_t)r << b0)];
 	}
 	__syncthreads();
 	for (int b = 0; b < nb; ++b) {
 		const uint32_t low = (1u << b) - 1u;
-		for (uint32_t x = threadIdx.x; x < 32 * (rows >> 1); x += blockDim.x) {
-			const uint32_t r = x >> 5, c = x & 31u;
+		for (uint32_t x = threadIdx.x; x < kDownCols * (rows >> 1); x += blockDim.x) {
+			const uint32_t r = x >> 4, c = x & 15u;
 			const uint32_t rk = ((r & ~low) << 1) | (r & low);
-			t[((rk | (1u << b)) << 5) + c] += t[(rk << 5) + c];
+			t[((rk | (1u << b)) << 4) + c] += t[(rk << 4) + c];
 		}
 		__syncthreads();
 	}
-	for (uint32_t i = threadIdx.x; i < 32 * rows; i += blockDim.x) {
-		const uint32_t r = i >> 5, c = i & 31u;
+	for (uint32_t i = threadIdx.x; i < kDownCols * rows; i += blockDim.x) {
+		const uint32_t r = i >> 4, c = i & 15u;
 		D[base + c + ((uint64_t)r << b0)] = t[i];
 	}
-	(void)n;
 }
 
 // Scores of every candidate into scores[internal index] (candidate 0 scores +inf), and each
@@ -2414,9 +2414,9 @@ void launch_down_sums(const Descs &descs, int atoms, Storage st, double *D, hipS
 	});
 	for (int b0 = nb; b0 < atoms; b0 += 8) {
 		const int w = atoms - b0 < 8 ? atoms - b0 : 8;
-		const dim3 grid((unsigned)(1ull << (atoms - w - 5)));
+		const dim3 grid((unsigned)(1ull << (atoms - w - 4)));
 		launch_checked("k_down_more", [&] {
-			hipLaunchKernelGGL(k_down_more, grid, dim3(256), 0, stream, D, atoms, b0, w);
+			hipLaunchKernelGGL(k_down_more, grid, dim3(256), 0, stream, D, b0, w);
 		});
 	}
 }
