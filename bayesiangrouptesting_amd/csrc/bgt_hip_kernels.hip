@@ -1087,24 +1087,45 @@ __global__ __launch_bounds__(kTileThreads) void k_zeta_pass2(Descs descs, int ob
 			const double thr = seen + d.band_width;
 			if (!raw_full && __ballot(fabs(d1) <= thr) != 0ull) {
 				constexpr p2::Map mf = p2::final_map(LC);
+				// count the wave's entries over all registers first (bit j of `inbits`: register
+				// j is in), so the wave takes its slots with one atomic: on plateaus many waves
+				// append, and per-register atomics on one counter serialise
+				uint32_t inbits = 0u;
+				unsigned count_in = 0u;
 #pragma unroll
 				for (int j = 0; j < kTileRegs; ++j) {
 					const uint32_t g = gt + p2::reg_offset(mf, LC, j);
 					const double sd = signed_objective(objective, v[j] * scale, total, target);
 					const bool in = (g & inactive) == 0u && fabs(sd) <= thr;
-					const unsigned long long mask = __ballot(in);
-					if (mask != 0ull && !raw_full) {
-						unsigned base = 0u;
-						if (lane == 0)
-							base = atomicAdd(&d.counter[kRawCount], (unsigned)__popcll(mask));
-						base = __shfl(base, 0);
-						raw_full = base + (unsigned)__popcll(mask) > raw_cap;
-						if (in && !raw_full) {
-							Cand c;
-							c.d = sd;
-							c.e = g;
-							c.flags = kCandSided;
-							d.cand[base + (unsigned)__popcll(mask & ((1ull << lane) - 1ull))] = c;
+					inbits |= (uint32_t)in << j;
+					count_in += (unsigned)__popcll(__ballot(in));
+				}
+				if (count_in != 0u) {
+					unsigned base = 0u;
+					if (lane == 0)
+						base = atomicAdd(&d.counter[kRawCount], count_in);
+					base = __builtin_amdgcn_readfirstlane(base);
+					raw_full = base + count_in > raw_cap;
+					if (!raw_full) {
+						const unsigned long long below = (1ull << lane) - 1ull;
+						// objectives and states are recomputed (same operations, same bits)
+						// from opaque copies of the scale and the tile base, so the compiler
+						// cannot keep all 32 of the first loop's alive across the atomic
+						double scale2 = scale;
+						uint32_t gt2 = gt;
+						asm volatile("" : "+v"(scale2), "+v"(gt2));
+#pragma unroll
+						for (int j = 0; j < kTileRegs; ++j) {
+							const bool in = (inbits >> j) & 1u;
+							const unsigned long long mask = __ballot(in);
+							if (in) {
+								Cand c;
+								c.d = signed_objective(objective, v[j] * scale2, total, target);
+								c.e = gt2 + p2::reg_offset(mf, LC, j);
+								c.flags = kCandSided;
+								d.cand[base + (unsigned)__popcll(mask & below)] = c;
+							}
+							base += (unsigned)__popcll(mask);
 						}
 					}
 				}
