@@ -717,7 +717,10 @@ def _validate(config: SimulationConfig) -> None:
         raise InvalidArgument(nat.BAD_INPUT, "prior length must equal subjects * variants.")
     if config.evaluation_prior is not None and len(config.evaluation_prior) != config.subjects * config.variants:
         raise InvalidArgument(nat.BAD_INPUT, "evaluation_prior length must equal subjects * variants.")
-    if config.options.selector not in (SelectorType.auto_select, SelectorType.op_bha, SelectorType.op_bbpa):
+    # simulation_helpers.cpp:34-38 admits auto/op_bha/op_bbpa; brute_force is this provider's
+    # addition (the device tree runs it), anything else is refused as there
+    if config.options.selector not in (SelectorType.auto_select, SelectorType.op_bha, SelectorType.op_bbpa,
+                                       SelectorType.brute_force):
         raise InvalidArgument(nat.BAD_INPUT, "unsupported selector.")
     if _resolve_selector(config.options.selector, config.variants) == SelectorType.op_bha and config.variants != 1:
         raise InvalidArgument(nat.BAD_INPUT, "Op-BHA tree selection is only supported for binary lattices.")

@@ -85,10 +85,6 @@ constexpr int kBandHead = 64;
 // Bytes at the start of every lattice's arena that one result copy brings back: scalars,
 // selection result, counters, atom masses and the first kBandHead band candidates.
 constexpr size_t kHeadBytes = 1024 + kBandHead * sizeof(Cand);
-// A single lattice with exact rescoring brings back a wider head, with the first
-// kBandHeadWide band candidates, so that a band of ordinary size needs no second copy.
-constexpr int kBandHeadWide = 1024;
-constexpr size_t kHeadBytesWide = 1024 + kBandHeadWide * sizeof(Cand);
 
 // Everything a kernel needs to know about one lattice. Arrays of these live in device
 // memory so that one launch can serve a batch of lattices (blockIdx.y selects).
@@ -138,6 +134,14 @@ struct ExactRec {
 // most kExactMaxHist updates and candidates whose remaining pools number at most
 // kExactMaxPools; the host program takes the rest.
 constexpr int kExactBand = 1024;
+// A single lattice with exact rescoring brings back a wider head in the same one copy: the
+// 1 KB block, the device's exact masses of the first kExactBand band candidates, and the
+// first kBandHeadWide band candidates (its arena places the exact masses at 1 KB and the band
+// right after them), so that a band of ordinary size needs no second copy.
+constexpr int kBandHeadWide = 1024;
+constexpr size_t kHeadExactWide = 1024;
+constexpr size_t kHeadBandWide = kHeadExactWide + sizeof(double) * kExactBand;
+constexpr size_t kHeadBytesWide = kHeadBandWide + kBandHeadWide * sizeof(Cand);
 constexpr int kExactMaxHist = kDenseMaxHist;
 constexpr int kExactMaxPools = kDenseMaxPools;
 // the exact band kernel stages one update and one prior value per lane of its wavefront

@@ -2053,11 +2053,15 @@ __global__ __launch_bounds__(64) void k_exact_band(Descs descs, int atoms)
 // Result heads of a batch of lattices into one staging buffer (one copy to the host).
 __global__ __launch_bounds__(256) void k_gather_heads(const LatDesc *many, unsigned char *out)
 {
+	// the 1 KB block from the arena's start, then the first kBandHead band candidates from
+	// wherever the lattice's band lives (lattices without a band: the next 1 KB, unused)
+	static_assert(kHeadBytes == 2048 && kBandHead * sizeof(Cand) == 1024, "head: 1 KB block + 1 KB of band");
 	const LatDesc &d = many[blockIdx.x];
 	const uint4 *src = reinterpret_cast<const uint4 *>(d.scal);
+	const uint4 *band = d.band ? reinterpret_cast<const uint4 *>(d.band) : src + 1024 / sizeof(uint4);
 	uint4 *dst = reinterpret_cast<uint4 *>(out + (size_t)blockIdx.x * kHeadBytes);
 	for (unsigned i = threadIdx.x; i < kHeadBytes / sizeof(uint4); i += blockDim.x)
-		dst[i] = src[i];
+		dst[i] = i < 1024 / sizeof(uint4) ? src[i] : band[i - 1024 / sizeof(uint4)];
 }
 
 __global__ __launch_bounds__(1024) void k_finalize_masses(Descs descs, unsigned count, int atoms)

@@ -144,8 +144,12 @@ constexpr double kNegativeResponse = 0.99; // replicated_lattice.cpp:6
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 // Arena head: [0,256) scalars, [256,512) selection result, [512,768) counters,
-// [768,1024) atom masses, then the band candidates (band_capacity(atoms) of them). One 2 KB
-// copy brings all of the head and the first kBandHead candidates to the host.
+// [768,1024) atom masses. Lattices with exact rescoring continue with the device's exact
+// masses of the band (kExactBand doubles) and then the band candidates; the others have
+// neither. The head copy of a lattice brings back this 1 KB block with the first band
+// candidates: kHeadBytes (band at 1 KB: the batched heads gather it there) or, for a single
+// lattice with exact rescoring, kHeadBytesWide (exact masses at kHeadExactWide, band at
+// kHeadBandWide).
 constexpr size_t kHeadResult = 256;
 constexpr size_t kHeadCounter = 512;
 constexpr size_t kHeadMasses = 768;
@@ -419,11 +423,11 @@ ArenaLayout arena_layout(bool dilution, int n, int k)
 	a.res = take(sizeof(SelResult));         // 256
 	a.cnt = take(sizeof(unsigned) * 4);      // 512
 	a.mass = take(sizeof(double) * 32);      // 768
-	a.band = take(a.exact ? sizeof(Cand) * band_capacity(atoms) : 0); // 1024
+	a.exact_u = take(a.exact ? sizeof(double) * kExactBand : 0);        // kHeadExactWide
+	a.band = take(a.exact ? sizeof(Cand) * band_capacity(atoms) : 0); // kHeadBandWide (exact)
 	a.cand = take(a.exact ? sizeof(Cand) * 2 * kTileThreads * zeta_final_groups(atoms) : 0);
 	a.flags = take(a.exact ? sizeof(unsigned) * kMaxFinalGroups : 0);
 	a.hist = take(a.exact ? sizeof(ExactRec) * kExactMaxHist : 0);
-	a.exact_u = take(a.exact ? sizeof(double) * kExactBand : 0);
 	a.pairs = take(sizeof(double) * 2 * kMaxAtoms);
 	a.part = take(sizeof(double) * part);
 	a.tops = take(sizeof(SelResult) * tops);
@@ -449,6 +453,8 @@ void DeviceLattice::allocate()
 	post_ = device_alloc(eb * states_, stream_);
 	bytes_ += eb * states_;
 	const ArenaLayout a = arena_layout(dilution_, n_, k_);
+	if (a.exact ? (a.exact_u != kHeadExactWide || a.band != kHeadBandWide) : a.band != kHeadBand)
+		throw Error(BGT_HIP_INTERNAL, "arena layout does not match the head copy");
 	arena_ = static_cast<char *>(device_alloc(a.total, stream_));
 	bytes_ += a.total;
 	BGT_HIP_CHECK(hipMemsetAsync(arena_, 0, a.total, stream_));
@@ -662,12 +668,10 @@ void DeviceLattice::fetch_head()
 	HostBox *box = static_cast<HostBox *>(host_box_);
 	// lattices with an error band (their arena holds at least kHeadBytesWide) bring back
 	// the wide head, and the band's exact masses when they were computed
+	// in one copy (a second one, issued on another engine, waited about 30 us for the first)
 	const bool wide = band_host_ != nullptr;
 	BGT_HIP_CHECK(hipMemcpyAsync(box->head, arena_, wide ? kHeadBytesWide : kHeadBytes, hipMemcpyDeviceToHost,
 								 stream_));
-	if (exact_pending_)
-		BGT_HIP_CHECK(hipMemcpyAsync(box->exact_u, exact_dev_, sizeof(double) * kExactBand, hipMemcpyDeviceToHost,
-									 stream_));
 	if (clock_.enabled) {
 		const auto t0 = std::chrono::steady_clock::now();
 		sync();
@@ -678,7 +682,9 @@ void DeviceLattice::fetch_head()
 	}
 	exact_valid_ = exact_pending_;
 	exact_pending_ = false;
-	unpack_head(wide ? kBandHeadWide : kBandHead);
+	if (exact_valid_)
+		std::memcpy(box->exact_u, box->head + kHeadExactWide, sizeof(double) * kExactBand);
+	unpack_head(wide ? kBandHeadWide : kBandHead, wide ? kHeadBandWide : kHeadBand);
 }
 
 void DeviceLattice::load_head(const unsigned char *head)
@@ -686,7 +692,7 @@ void DeviceLattice::load_head(const unsigned char *head)
 	HostBox *box = static_cast<HostBox *>(host_box_);
 	std::memcpy(box->head, head, kHeadBytes);
 	exact_valid_ = exact_pending_ = false;
-	unpack_head(kBandHead);
+	unpack_head(kBandHead, kHeadBand);
 }
 
 void DeviceLattice::launch_exact(const Descs &ds_in)
@@ -717,7 +723,7 @@ void DeviceLattice::launch_exact(const Descs &ds_in)
 	exact_pending_ = true;
 }
 
-void DeviceLattice::unpack_head(int band_entries)
+void DeviceLattice::unpack_head(int band_entries, size_t band_offset)
 {
 	HostBox *box = static_cast<HostBox *>(host_box_);
 	std::memcpy(box->scal, box->head, sizeof(double) * kScalars);
@@ -727,7 +733,7 @@ void DeviceLattice::unpack_head(int band_entries)
 	head_band_ = 0;
 	if (band_host_) {
 		head_band_ = (unsigned)std::min<unsigned>({(unsigned)band_entries, box->counter[0], band_cap_});
-		std::memcpy(band_host_, box->head + kHeadBand, sizeof(Cand) * head_band_);
+		std::memcpy(band_host_, box->head + band_offset, sizeof(Cand) * head_band_);
 	}
 }
 

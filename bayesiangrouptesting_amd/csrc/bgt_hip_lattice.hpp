@@ -152,7 +152,7 @@ private:
 	// exact masses of the band on the device (binary non-dilution), after the band kernels
 	void launch_exact(const Descs &ds);
 	// the fields of the head now in the host mailbox, with up to band_entries band candidates
-	void unpack_head(int band_entries);
+	void unpack_head(int band_entries, size_t band_offset);
 	// complete band of candidates for exact rescoring (binary non-dilution lattices); the
 	// arguments after `entries` are those of the final transform launch
 	void collect_band(const Descs &ds, unsigned entries, int objective, bool from_posterior, bool scaled,

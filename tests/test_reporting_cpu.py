@@ -115,7 +115,7 @@ def test_interrupted_before_evaluation_writes_partial_files(tmp_path):
     (lambda c: setattr(c, "workload_granularity", 0), "workload_granularity must be positive"),
     (lambda c: setattr(c.progress, "interval_seconds", 0.0), "progress interval must be positive"),
     (lambda c: setattr(c, "evaluation_prior", [0.1]), "evaluation_prior length must equal"),
-    (lambda c: setattr(c.options, "selector", bgt.SelectorType.brute_force), "unsupported selector"),
+    (lambda c: setattr(c.options, "selector", 7), "unsupported selector"),
     (lambda c: setattr(c, "lattice_type", bgt.LatticeType.distributed_dilution), "local_tree requires a replicated"),
 ])
 def test_configuration_checks(change, message):
