@@ -1002,89 +1002,118 @@ void DeviceLattice::rescore_band(bgt_hip_selection &out, int sel, const SelResul
 		out.band = (int)count;
 		return;
 	}
-	std::sort(live.begin(), live.end(), [](const Cand &a, const Cand &b) {
-		const double x = std::fabs(a.d), y = std::fabs(b.d);
-		return x < y || (x == y && a.e < b.e);
-	});
 	const double t = 1.0 / double(1 << k_);
 	double b1 = INFINITY, b2 = INFINITY;
 	uint32_t e1 = 0xFFFFFFFFu, e2 = 0xFFFFFFFFu;
 	int rescored = 0;
 	constexpr int kMaxRescored = 4096;
-	// The scan below is sequential; the exact masses it needs are computed ahead, in
-	// parallel, so the result is that of the sequential scan. It never goes past a device
-	// objective of |d_2| + 2 eps (d_2: the second candidate's): the exact objectives of the
-	// first two candidates are at most |d_2| + eps, so the exact runner-up is too, and the
-	// scan stops at the first candidate whose lower bound |d| - eps exceeds it. Those masses
-	// are computed in one parallel run; later chunks are only a guard.
-	std::vector<uint32_t> ex;
-	std::vector<double> mass;
-	std::unique_ptr<bool[]> okv;
-	size_t ready = 0, chunk = 64;
-	if (live.size() >= 2) {
-		const double reach = std::fabs(live[1].d) + 2.0 * eps;
-		size_t need = 0;
-		while (need < live.size() && need < (size_t)kMaxRescored && !(std::fabs(live[need].d) > reach))
-			++need;
-		chunk = std::max<size_t>(need, 1);
-	}
-	for (size_t i = 0; i < live.size(); ++i) {
-		const Cand &c = live[i];
-		if (std::fabs(c.d) - eps > b2)
-			break; // its exact objective exceeds the current runner-up, as do all later ones
-		if (rescored == kMaxRescored) {
-			complete = false;
-			break;
-		}
-		if (i == ready) {
-			// next chunk: candidates that may still be needed under the current runner-up
-			size_t j = i;
-			while (j < live.size() && j - i < chunk && !(std::fabs(live[j].d) - eps > b2))
-				++j;
-			ex.resize(j - i);
-			mass.resize(j - i);
-			okv.reset(new bool[j - i]);
-			// masses the device computed are normalised here; the rest run on the host
-			std::vector<size_t> host;
-			for (size_t q = i; q < j; ++q) {
-				const uint32_t tag = live[q].flags >> kCandIndexShift;
-				const double u = tag ? box->exact_u[tag - 1] : -1.0;
-				if (u >= 0.0 && exact_.normalise(u, mass[q - i])) {
-					okv[q - i] = true;
-				} else {
-					ex[host.size()] = live[q].e;
-					host.push_back(q - i);
-				}
-			}
-			clock_.host("n_exact_device", (double)(j - i - host.size()));
-			clock_.host("n_exact_host", (double)host.size());
-			if (!host.empty()) {
-				std::vector<double> hm(host.size());
-				std::unique_ptr<bool[]> hok(new bool[host.size()]);
-				if (!exact_.masses(ex.data(), host.size(), hm.data(), hok.get()))
-					return; // the normaliser is beyond the exact program's budget: keep the device result
-				for (size_t h = 0; h < host.size(); ++h) {
-					mass[host[h]] = hm[h];
-					okv[host[h]] = hok[h];
-				}
-			}
-			ready = j;
-			chunk *= 2;
-		}
-		const size_t slot = i - (ready - mass.size());
-		if (!okv[slot])
-			return; // beyond the exact program's budget: keep the device result
-		const double m = mass[slot];
-		const double d = sel == kSelBha ? std::fabs(m - 0.5) : std::fabs(m - t) + std::fabs((1.0 - m) - t);
-		++rescored;
-		if (d < b1 || (d == b1 && c.e < e1)) {
+	auto objective_of = [&](double m) {
+		return sel == kSelBha ? std::fabs(m - 0.5) : std::fabs(m - t) + std::fabs((1.0 - m) - t);
+	};
+	// the best two by (exact objective, experiment): a strict total order over distinct
+	// experiments, so the result does not depend on the order candidates arrive in
+	auto consider = [&](double d, uint32_t e) {
+		if (d < b1 || (d == b1 && e < e1)) {
 			b2 = b1;
 			e2 = e1;
 			b1 = d;
-			e1 = c.e;
-		} else if (d < b2 || (d == b2 && c.e < e2)) {
+			e1 = e;
+		} else if (d < b2 || (d == b2 && e < e2)) {
 			b2 = d;
-			e2 = c.e;
+			e2 = e;
+		}
+	};
+	// When the device returned the exact mass of every live candidate, one pass over them
+	// gives the same best two as the ordered scan below (which only spares exact masses that
+	// would have to be computed on the host), without sorting the band.
+	bool linear = exact_valid_ && !live.empty() && live.size() <= (size_t)kMaxRescored;
+	for (size_t i = 0; linear && i < live.size(); ++i) {
+		const uint32_t tag = live[i].flags >> kCandIndexShift;
+		linear = tag != 0u && box->exact_u[tag - 1] >= 0.0;
+	}
+	double m0;
+	if (linear)
+		linear = exact_.normalise(box->exact_u[(live[0].flags >> kCandIndexShift) - 1], m0);
+	if (linear) {
+		for (const Cand &c : live) {
+			double m;
+			exact_.normalise(box->exact_u[(c.flags >> kCandIndexShift) - 1], m);
+			consider(objective_of(m), c.e);
+		}
+		rescored = (int)live.size();
+		clock_.host("n_exact_device", (double)live.size());
+		clock_.host("n_exact_host", 0.0);
+	} else {
+		std::sort(live.begin(), live.end(), [](const Cand &a, const Cand &b) {
+			const double x = std::fabs(a.d), y = std::fabs(b.d);
+			return x < y || (x == y && a.e < b.e);
+		});
+		// The scan below is sequential; the exact masses it needs are computed ahead, in
+		// parallel, so the result is that of the sequential scan. It never goes past a device
+		// objective of |d_2| + 2 eps (d_2: the second candidate's): the exact objectives of the
+		// first two candidates are at most |d_2| + eps, so the exact runner-up is too, and the
+		// scan stops at the first candidate whose lower bound |d| - eps exceeds it. Those masses
+		// are computed in one parallel run; later chunks are only a guard.
+		std::vector<uint32_t> ex;
+		std::vector<double> mass;
+		std::unique_ptr<bool[]> okv;
+		size_t ready = 0, chunk = 64;
+		if (live.size() >= 2) {
+			const double reach = std::fabs(live[1].d) + 2.0 * eps;
+			size_t need = 0;
+			while (need < live.size() && need < (size_t)kMaxRescored && !(std::fabs(live[need].d) > reach))
+				++need;
+			chunk = std::max<size_t>(need, 1);
+		}
+		for (size_t i = 0; i < live.size(); ++i) {
+			const Cand &c = live[i];
+			if (std::fabs(c.d) - eps > b2)
+				break; // its exact objective exceeds the current runner-up, as do all later ones
+			if (rescored == kMaxRescored) {
+				complete = false;
+				break;
+			}
+			if (i == ready) {
+				// next chunk: candidates that may still be needed under the current runner-up
+				size_t j = i;
+				while (j < live.size() && j - i < chunk && !(std::fabs(live[j].d) - eps > b2))
+					++j;
+				ex.resize(j - i);
+				mass.resize(j - i);
+				okv.reset(new bool[j - i]);
+				// masses the device computed are normalised here; the rest run on the host
+				std::vector<size_t> host;
+				for (size_t q = i; q < j; ++q) {
+					const uint32_t tag = live[q].flags >> kCandIndexShift;
+					const double u = tag ? box->exact_u[tag - 1] : -1.0;
+					if (u >= 0.0 && exact_.normalise(u, mass[q - i])) {
+						okv[q - i] = true;
+					} else {
+						ex[host.size()] = live[q].e;
+						host.push_back(q - i);
+					}
+				}
+				clock_.host("n_exact_device", (double)(j - i - host.size()));
+				clock_.host("n_exact_host", (double)host.size());
+				if (!host.empty()) {
+					std::vector<double> hm(host.size());
+					std::unique_ptr<bool[]> hok(new bool[host.size()]);
+					if (!exact_.masses(ex.data(), host.size(), hm.data(), hok.get()))
+						return; // the normaliser is beyond the exact program's budget: keep the device result
+					for (size_t h = 0; h < host.size(); ++h) {
+						mass[host[h]] = hm[h];
+						okv[host[h]] = hok[h];
+					}
+				}
+				ready = j;
+				chunk *= 2;
+			}
+			const size_t slot = i - (ready - mass.size());
+			if (!okv[slot])
+				return; // beyond the exact program's budget: keep the device result
+			const double m = mass[slot];
+			++rescored;
+			consider(objective_of(m), c.e);
 		}
 	}
 	phase("t_scan");
