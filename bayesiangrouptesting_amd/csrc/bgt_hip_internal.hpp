@@ -173,12 +173,17 @@ void launch_prior(const Descs &descs, int count, int atoms, Storage st, const do
 				  uint64_t states, hipStream_t stream);
 void launch_update(const Descs &descs, int count, uint64_t states, int k, int dilution,
 				   Storage st, hipStream_t stream);
-// exact path: transform of x = y * inv (n >= kTileBits + 1): pass 1 over the low 15 bits
+// exact path: transform of x = y * inv (n >= kTileBits + 1): pass 1 over the low 15 bits.
+// defer_sum: leave the sum of its per-tile partials to a persistent pass 2 (fold below)
 void launch_zeta_pass1(const Descs &descs, int count, int n, Storage st, bool fused_update,
-					   int k, int dilution, hipStream_t stream);
-// pass 2 over bits 15..n-1 (or the whole transform when n == 15), with an objective
+					   int k, int dilution, hipStream_t stream, bool defer_sum = false);
+// pass 2 over bits 15..n-1 (or the whole transform when n == 15), with an objective.
+// fold > 0 (persistent pass 2 after a fused pass 1 with defer_sum): every workgroup sums
+// the `fold` per-tile partials itself, as the separate finalize launch would, and the first
+// stores the denominator, its inverse and the total
 unsigned launch_zeta_final(const Descs &descs, int count, int n, int k, Storage st, int objective,
-						   bool from_posterior, bool scaled, double target, hipStream_t stream);
+						   bool from_posterior, bool scaled, double target, hipStream_t stream,
+						   unsigned fold = 0);
 // n <= kSmallBits: whole transform in one workgroup per lattice
 unsigned launch_zeta_small(const Descs &descs, int count, int n, int k, Storage st, int objective,
 					   double target, hipStream_t stream);

@@ -951,7 +951,8 @@ __device__ __forceinline__ void scan_tile(const double (&v)[kTileRegs], double s
 // differs from the increasing bit order only in FP64 additions of FP32 values, inside the
 // FP32C error band; FP64 storage keeps the exact order.
 template <typename TIN, int LC, bool SPLIT = false>
-__global__ __launch_bounds__(kTileThreads) void k_zeta_pass2(Descs descs, int objective, int scaled, double target)
+__global__ __launch_bounds__(kTileThreads) void k_zeta_pass2(Descs descs, int objective, int scaled, double target,
+																unsigned fold)
 {
 	constexpr p2::Map mb = p2::block_map(LC);
 	constexpr p2::Map mw = p2::wave_map(LC);
@@ -976,8 +977,32 @@ __global__ __launch_bounds__(kTileThreads) void k_zeta_pass2(Descs descs, int ob
 	constexpr bool wide = p2::max_thread_z_offset(LC, pitch) * sizeof(TIN) >= (1ull << 32);
 	const uint32_t lob = wide ? 0u : (uint32_t)(lo * sizeof(TIN));
 	const TIN *inw = wide ? in + lo : in;
-	const double scale = scaled ? d.scal[d.inv_out] : 1.0;
-	const double total = scaled ? d.scal[kScalTotal] * scale : d.scal[kScalTotal];
+	double scale, total;
+	if (fold != 0u) {
+		// the fused update's denominator D from pass 1's per-tile partials, formed exactly as
+		// k_finalize_sum forms it (1024 threads, the same strides and reduction), so the
+		// values are identical to the separate launch's; the first workgroup stores them
+		double t = 0.0;
+		for (unsigned i = threadIdx.x; i < fold; i += blockDim.x)
+			t += d.part[i];
+		t = block_sum(t, lds);
+		if (threadIdx.x == 0)
+			lds[16] = t;
+		__syncthreads();
+		t = uniform(lds[16]); // workgroup-uniform values live in scalar registers
+		__syncthreads();
+		const double inv = uniform(1.0 / t);
+		if (blockIdx.x == 0 && threadIdx.x == 0) {
+			d.scal[kScalDen] = t;
+			d.scal[d.inv_out] = inv;
+			d.scal[kScalTotal] = t;
+		}
+		scale = uniform(scaled ? inv : 1.0);
+		total = uniform(scaled ? t * scale : t);
+	} else {
+		scale = uniform(scaled ? d.scal[d.inv_out] : 1.0);
+		total = uniform(scaled ? d.scal[kScalTotal] * scale : d.scal[kScalTotal]);
+	}
 	const uint32_t full = (d.n >= 32) ? 0xFFFFFFFFu : ((1u << d.n) - 1u);
 	const uint32_t inactive = full & ~d.active;
 	const bool bbpa = objective == kObjBinaryBbpa;
@@ -2149,7 +2174,7 @@ void launch_update(const Descs &descs, int count, uint64_t states, int, int, Sto
 }
 
 void launch_zeta_pass1(const Descs &descs, int count, int n, Storage st, bool fused, int k, int dilution,
-					   hipStream_t stream)
+					   hipStream_t stream, bool defer_sum)
 {
 	// FP32 storage: persistent workgroups (one per CU) that prefetch their next tile, once
 	// there are more tiles than CUs; otherwise one workgroup per tile
@@ -2174,8 +2199,9 @@ void launch_zeta_pass1(const Descs &descs, int count, int n, Storage st, bool fu
 			else
 				hipLaunchKernelGGL((k_zeta_pass1<float, float, false, 0>), grid, dim3(kTileThreads), 0, stream, descs);
 		}
-		hipLaunchKernelGGL(k_finalize_sum, dim3(1, count), dim3(1024), 0, stream, descs, tiles,
-						   (int)(fused ? kSumDenominator : kSumTotal));
+		if (!(defer_sum && fused))
+			hipLaunchKernelGGL(k_finalize_sum, dim3(1, count), dim3(1024), 0, stream, descs, tiles,
+							   (int)(fused ? kSumDenominator : kSumTotal));
 	});
 }
 
@@ -2200,7 +2226,7 @@ bool persistent_pass2(int n, int objective, bool from_posterior)
 // without the merge of the workgroups' top-2 lists; returns its workgroup count. `collect`
 // (single pass of 15..20 atoms only) rescans the workgroups flagged by k_band_collect.
 unsigned launch_final_kernel(const Descs &descs, int count, int n, Storage st, int objective, bool from_posterior,
-							 bool scaled, double target, int collect, hipStream_t stream)
+							 bool scaled, double target, int collect, hipStream_t stream, unsigned fold = 0)
 {
 	if (persistent_pass2(n, objective, from_posterior)) {
 		// persistent pass 2: one workgroup per CU (a multiple of 8, one share per XCD)
@@ -2211,13 +2237,13 @@ unsigned launch_final_kernel(const Descs &descs, int count, int n, Storage st, i
 				const int sc = scaled ? 1 : 0;
 				if (n == 28)
 					hipLaunchKernelGGL((k_zeta_pass2<float, 3, true>), grid, dim3(kTileThreads), 0, stream, descs,
-									   objective, sc, target);
+									   objective, sc, target, fold);
 				else if (n == 29)
 					hipLaunchKernelGGL((k_zeta_pass2<float, 2, true>), grid, dim3(kTileThreads), 0, stream, descs,
-									   objective, sc, target);
+									   objective, sc, target, fold);
 				else
 					hipLaunchKernelGGL((k_zeta_pass2<float, 1, true>), grid, dim3(kTileThreads), 0, stream, descs,
-									   objective, sc, target);
+									   objective, sc, target, fold);
 			});
 			return grid.x;
 		}
@@ -2226,10 +2252,10 @@ unsigned launch_final_kernel(const Descs &descs, int count, int n, Storage st, i
 	case LCV:                                                                                                  \
 		if (st == Storage::fp64)                                                                               \
 			hipLaunchKernelGGL((k_zeta_pass2<double, LCV>), grid, dim3(kTileThreads), 0, stream, descs, objective, \
-							   scaled ? 1 : 0, target);                                                        \
+							   scaled ? 1 : 0, target, fold);                                                  \
 		else                                                                                                   \
 			hipLaunchKernelGGL((k_zeta_pass2<float, LCV>), grid, dim3(kTileThreads), 0, stream, descs, objective,  \
-							   scaled ? 1 : 0, target);                                                        \
+							   scaled ? 1 : 0, target, fold);                                                  \
 		break;
 			switch (2 * kTileBits - n) {
 				BGT_P2(0)
@@ -2273,10 +2299,12 @@ unsigned launch_final_kernel(const Descs &descs, int count, int n, Storage st, i
 }
 
 unsigned launch_zeta_final(const Descs &descs, int count, int n, int, Storage st, int objective,
-						   bool from_posterior, bool scaled, double target, hipStream_t stream)
+						   bool from_posterior, bool scaled, double target, hipStream_t stream, unsigned fold)
 {
+	if (fold != 0u && !persistent_pass2(n, objective, from_posterior))
+		throw Error(4, "folding the pass-1 partials needs the persistent pass 2");
 	const unsigned groups = launch_final_kernel(descs, count, n, st, objective, from_posterior, scaled, target, 0,
-												stream);
+												stream, fold);
 	launch_checked("k_finalize_top2", [&] {
 		hipLaunchKernelGGL(k_finalize_top2, dim3(1, count), dim3(1024), 0, stream, descs, groups);
 	});

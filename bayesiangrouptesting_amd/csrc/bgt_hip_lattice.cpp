@@ -1372,13 +1372,16 @@ bgt_hip_selection DeviceLattice::round(uint32_t experiment, uint32_t response, c
 	prepare_update(experiment, response, dil);
 	ensure_z();
 	Descs ds = single(); // descriptors of the pre-update state (pass 2 applies inv_out)
+	// a persistent pass 2 sums pass 1's partials itself (one launch and its gap fewer)
+	const bool fold = persistent_pass2(atoms_, kObjHalving, false);
 	clock_.begin(stream_);
-	launch_zeta_pass1(ds, 1, atoms_, storage_, true, k_, dilution_, stream_);
+	launch_zeta_pass1(ds, 1, atoms_, storage_, true, k_, dilution_, stream_, fold);
 	clock_.end(stream_, "round_pass1");
 	commit_update();
 	ds.one.band_width = band_width(kSelBha); // error band of the updated posterior
 	clock_.begin(stream_);
-	const unsigned entries = launch_zeta_final(ds, 1, atoms_, k_, storage_, kObjHalving, false, true, 0.5, stream_);
+	const unsigned entries = launch_zeta_final(ds, 1, atoms_, k_, storage_, kObjHalving, false, true, 0.5, stream_,
+											   fold ? (1u << (atoms_ - kTileBits)) : 0u);
 	clock_.end(stream_, "round_pass2");
 	collect_band(ds, entries, kObjHalving, false, true, 0.5);
 	launch_exact(ds);
