@@ -86,6 +86,14 @@ constexpr int kBandHead = 64;
 // selection result, counters, atom masses and the first kBandHead band candidates.
 constexpr size_t kHeadBytes = 1024 + kBandHead * sizeof(Cand);
 
+// One update of a binary non-dilution lattice for the exact program: the pool and the
+// likelihood factors when the pool is / is not contained in the state.
+struct ExactRec {
+	uint32_t e;
+	uint32_t pad;
+	double fin, fout;
+};
+
 // Everything a kernel needs to know about one lattice. Arrays of these live in device
 // memory so that one launch can serve a batch of lattices (blockIdx.y selects).
 struct LatDesc {
@@ -117,19 +125,16 @@ struct LatDesc {
 	int src_inv;
 	// exact rescoring on the device (k_exact_band): the update history, the prior's
 	// (pi_a, 1 - pi_a) pairs and the output array
-	const struct ExactRec *hist;
+	ExactRec *hist;
 	int hist_len;
+	// the newest update's record travels here when it is the only one not yet on the device
+	// (hist_new_at >= 0); k_exact_band uses it for that index and stores it for later rounds
+	int hist_new_at;
+	ExactRec hist_new;
 	const double *pairs;
 	double *exact_u;
 };
 
-// One update of a binary non-dilution lattice for the exact program: the pool and the
-// likelihood factors when the pool is / is not contained in the state.
-struct ExactRec {
-	uint32_t e;
-	uint32_t pad;
-	double fin, fout;
-};
 // Device exact rescoring covers the first kExactBand band candidates, histories of at
 // most kExactMaxHist updates and candidates whose remaining pools number at most
 // kExactMaxPools; the host program takes the rest.

@@ -1959,6 +1959,10 @@ __global__ __launch_bounds__(64) void k_exact_band(Descs descs, int atoms)
 	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
 	if (d.exact_u == nullptr || d.band == nullptr)
 		return;
+	// the newest record arrived as an argument: keep it on the device for later rounds (no
+	// block reads that slot in this launch; every lane takes the argument instead)
+	if (d.hist_new_at >= 0 && blockIdx.x == 0 && threadIdx.x == 0)
+		d.hist[d.hist_new_at] = d.hist_new;
 	const unsigned count = min(d.counter[0], d.band_cap);
 	const unsigned i = blockIdx.x;
 	if (i >= count || i >= (unsigned)kExactBand)
@@ -2011,7 +2015,7 @@ __global__ __launch_bounds__(64) void k_exact_band(Descs descs, int atoms)
 	uint32_t rest = 0;
 	double fin0 = 1.0, fout0 = 1.0;
 	if (lane < h) {
-		const ExactRec r = d.hist[lane];
+		const ExactRec r = lane == d.hist_new_at ? d.hist_new : d.hist[lane];
 		rest = r.e & ~e;
 		fin0 = r.fin;
 		fout0 = r.fout;

@@ -561,6 +561,7 @@ LatDesc DeviceLattice::desc() const
 	d.band_cap = band_cap_;
 	d.hist = hist_dev_;
 	d.hist_len = exact_.updates();
+	d.hist_new_at = -1;
 	d.pairs = pairs_;
 	d.exact_u = nullptr; // set by the launches that want exact masses
 	return d;
@@ -702,19 +703,26 @@ void DeviceLattice::launch_exact(const Descs &ds_in)
 		return;
 	HostBox *box = static_cast<HostBox *>(host_box_);
 	const int n = exact_.updates();
+	int new_at = -1;
 	if (hist_uploaded_ < n) {
 		for (int t = hist_uploaded_; t < n; ++t) {
 			ExactRec &rec = box->hist[t];
 			exact_.record(t, rec.e, rec.fin, rec.fout);
 			rec.pad = 0;
 		}
-		BGT_HIP_CHECK(hipMemcpyAsync(hist_dev_ + hist_uploaded_, box->hist + hist_uploaded_,
-									 sizeof(ExactRec) * (n - hist_uploaded_), hipMemcpyHostToDevice, stream_));
+		if (n - hist_uploaded_ == 1)
+			new_at = n - 1; // one new record: a kernel argument, no copy ahead of the kernel
+		else
+			BGT_HIP_CHECK(hipMemcpyAsync(hist_dev_ + hist_uploaded_, box->hist + hist_uploaded_,
+										 sizeof(ExactRec) * (n - hist_uploaded_), hipMemcpyHostToDevice, stream_));
 		hist_uploaded_ = n;
 	}
 	Descs ds = ds_in;
 	ds.one.hist = hist_dev_;
 	ds.one.hist_len = n;
+	ds.one.hist_new_at = new_at;
+	if (new_at >= 0)
+		ds.one.hist_new = box->hist[new_at];
 	ds.one.pairs = pairs_;
 	ds.one.exact_u = exact_dev_;
 	clock_.begin(stream_);
