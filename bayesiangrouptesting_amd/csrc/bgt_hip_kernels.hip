@@ -607,11 +607,16 @@ __global__ __launch_bounds__(kTileThreads) void k_zeta_pass1(Descs descs)
 	// uniform by construction; said so, the tile loop stays scalar
 	const unsigned tiles = 1u << (__builtin_amdgcn_readfirstlane(d.atoms) - kTileBits);
 	if (blockIdx.x == 0 && d.band != nullptr) {
-		// the band collection of the pass 2 that follows
+		// the band collection of the pass 2 that follows (raw appends, running bests) and of
+		// the compaction after it (band count and its companion counter)
 		if (threadIdx.x < 8)
 			best_slots(d)[threadIdx.x] = kKeyInf;
 		else if (threadIdx.x == 8)
 			d.counter[kRawCount] = 0u;
+		else if (threadIdx.x == 9)
+			d.counter[0] = 0u;
+		else if (threadIdx.x == 10)
+			d.counter[1] = 0u;
 	}
 	const double inv = uniform(d.scal[d.inv_in]);
 	const double f_in = FUSED ? uniform(s_lik[0]) : 0.0, f_out = FUSED ? uniform(s_lik[1]) : 0.0;
@@ -939,7 +944,7 @@ __device__ __forceinline__ void scan_tile(const double (&v)[kTileRegs], double s
 // band_width of the best |objective| seen so far is appended to the `cand` buffer. The
 // best seen is a running minimum over this wave's tiles and its XCD's slot, never below the
 // final best, so every entry within band_width of the final best is appended;
-// k_band_compact keeps those after the merge.
+// k_top2_compact keeps those after the merge.
 //
 // SPLIT (FP32 intermediate, A >= 28): the lattice is run as one of A - 1 atoms (this LC is
 // 31 - A) whose top row bit, state bit A - 1, is split into two work units per tile. The
@@ -1841,16 +1846,47 @@ __global__ __launch_bounds__(256) void k_band_collect(Descs descs, unsigned entr
 // The band of a pass 2 that collected while scanning: its appended entries (relative to
 // running bests) that lie within band_width of the final best. counter[0] was cleared by
 // k_finalize_top2; an overflow of the appended entries is reported as an overflowed band.
-__global__ __launch_bounds__(256) void k_band_compact(Descs descs, unsigned raw_cap)
+// After a persistent pass 2: the merge of its workgroups' top-2 lists (k_finalize_top2's
+// result; the top two by (|objective|, experiment) do not depend on how the lists are
+// split, so every block finds the same) and, for lattices with a band, the compaction of
+// the candidates it appended (those within band_width of the final best), in one launch.
+// Pass 1 reset the counters.
+__global__ __launch_bounds__(256) void k_top2_compact(Descs descs, unsigned count, unsigned raw_cap)
 {
 	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
+	__shared__ Top2 red2[4];
+	__shared__ double s_lim;
+	Top2 r;
+	top2_init(r);
+	for (unsigned i = threadIdx.x; i < count; i += blockDim.x) {
+		const Top2 p = reinterpret_cast<const Top2 *>(d.tops)[i];
+		top2_insert(r, p.d1, p.e1);
+		top2_insert(r, p.d2, p.e2);
+	}
+	r = top2_block(r, red2);
+	if (threadIdx.x == 0) {
+		if (blockIdx.x == 0) {
+			d.result->d1 = fabs(r.d1);
+			d.result->d2 = fabs(r.d2);
+			d.result->e1 = r.e1;
+			d.result->e2 = r.e2;
+			if (d.band == nullptr) {
+				d.counter[0] = 0;
+				d.counter[1] = 0;
+			}
+		}
+		s_lim = fabs(r.d1) + d.band_width;
+	}
+	__syncthreads();
+	if (d.band == nullptr)
+		return;
 	const unsigned raw = d.counter[kRawCount];
 	if (raw > raw_cap) {
 		if (blockIdx.x == 0 && threadIdx.x == 0)
 			d.counter[0] = d.band_cap + 1u;
 		return;
 	}
-	const double lim = d.result->d1 + d.band_width;
+	const double lim = s_lim;
 	for (unsigned base = blockIdx.x * blockDim.x; base < raw; base += gridDim.x * blockDim.x) {
 		const unsigned i = base + threadIdx.x;
 		Cand c;
@@ -2309,10 +2345,18 @@ unsigned launch_zeta_final(const Descs &descs, int count, int n, int, Storage st
 		throw Error(4, "folding the pass-1 partials needs the persistent pass 2");
 	const unsigned groups = launch_final_kernel(descs, count, n, st, objective, from_posterior, scaled, target, 0,
 												stream, fold);
+	const unsigned entries = 2u * groups * kTileThreads;
+	if (persistent_pass2(n, objective, from_posterior)) {
+		// the merge of the top-2 lists and the band compaction in one launch
+		launch_checked("k_top2_compact", [&] {
+			hipLaunchKernelGGL(k_top2_compact, dim3(64, count), dim3(256), 0, stream, descs, groups, entries);
+		});
+		return entries;
+	}
 	launch_checked("k_finalize_top2", [&] {
 		hipLaunchKernelGGL(k_finalize_top2, dim3(1, count), dim3(1024), 0, stream, descs, groups);
 	});
-	return 2u * groups * kTileThreads;
+	return entries;
 }
 
 unsigned launch_zeta_small(const Descs &descs, int count, int n, int, Storage st, int objective, double target,
@@ -2339,10 +2383,9 @@ void launch_band_collect(const Descs &descs, int count, unsigned entries, int n,
 						 bool from_posterior, bool scaled, double target, hipStream_t stream)
 {
 	if (persistent_pass2(n, objective, from_posterior)) {
-		// pass 2 appended its candidates while scanning: keep those inside the final band
-		launch_checked("k_band_compact", [&] {
-			hipLaunchKernelGGL(k_band_compact, dim3(64, count), dim3(256), 0, stream, descs, entries);
-		});
+		// pass 2 appended its candidates while scanning; launch_zeta_final's k_top2_compact
+		// already kept those inside the final band
+		(void)entries;
 		return;
 	}
 	const dim3 grid((entries + 255) / 256, count);
