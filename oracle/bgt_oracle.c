@@ -8,9 +8,13 @@
  * Built twice by oracle/Makefile:
  *   liboracle.so       strict IEEE (no contraction), the parity checker;
  *   liboracle_fast.so  the reference's default flags (-O3 -ffast-math), timed
- *                      as the CPU baseline.
+ *                      as the CPU baseline; with OpenMP, used only by
+ *                      orc_op_bha_omp (the reference's own OpenMP loops).
  */
 #define _POSIX_C_SOURCE 199309L
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include "bgt_oracle.h"
 
 #include <math.h>
@@ -173,6 +177,80 @@ uint32_t orc_op_bha(const double *post, int n, double *upset_out)
 	if (!upset_out)
 		free(a);
 	return candidate;
+}
+
+uint32_t orc_op_bha_omp(const double *post, int n, double *upset_out)
+{
+	/* bbpa.cpp:920-940 (op_bha_omp): the copy, subset_zeta_transform_omp (bbpa.cpp:517-533:
+	 * sweeps in order, the blocks of one sweep in parallel) and best_op_bha_candidate_omp
+	 * (bbpa.cpp:555-571: the lowest distance, ties to the lowest index), from 2^16 states.
+	 * Built without OpenMP it is orc_op_bha. Used only as the all-cores CPU baseline. */
+	const uint32_t count = bit(n);
+	if (count < (1u << 16))
+		return orc_op_bha(post, n, upset_out);
+	double *a = upset_out ? upset_out : (double *)malloc(sizeof(double) * count);
+	const long long cnt = (long long)count;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+	for (long long s = 0; s < cnt; s++)
+		a[s] = post[s];
+	for (int subject = 0; subject < n; subject++) {
+		const long long b = (long long)bit(subject);
+		const long long step = b << 1;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+		for (long long base = 0; base < cnt; base += step)
+			for (long long off = 0; off < b; off++)
+				a[base + off] += a[base + off + b];
+	}
+	uint32_t candidate = 0;
+	double best = 2.0;
+#ifdef _OPENMP
+#pragma omp parallel
+#endif
+	{
+		uint32_t lc = 0;
+		double lb = 2.0;
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+		for (long long e = 0; e < cnt; e++) {
+			const double d = fabs(a[e] - 0.5);
+			if (d < lb || (d == lb && (uint32_t)e < lc)) {
+				lb = d;
+				lc = (uint32_t)e;
+			}
+		}
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+		{
+			if (lb < best || (lb == best && lc < candidate)) {
+				best = lb;
+				candidate = lc;
+			}
+		}
+	}
+	if (!upset_out)
+		free(a);
+	return candidate;
+}
+
+int orc_threads(void)
+{
+#ifdef _OPENMP
+	int t = 1;
+#pragma omp parallel
+	{
+#pragma omp single
+		t = omp_get_num_threads();
+	}
+	return t;
+#else
+	return 1;
+#endif
 }
 
 uint32_t orc_bbpa(const double *post, int n, int k, double target, double *best_score)

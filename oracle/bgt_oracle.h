@@ -48,6 +48,9 @@ double orc_update_out(int dilution, int curr_subjs, int variants, const double *
 					  uint32_t e, uint32_t r, const double *dil, int dil_stride);
 void orc_zeta(double *a, int n);
 uint32_t orc_op_bha(const double *post, int n, double *upset_out);
+/* op_bha with the reference's OpenMP loops (all-cores CPU baseline only) */
+uint32_t orc_op_bha_omp(const double *post, int n, double *upset_out);
+int orc_threads(void);
 uint32_t orc_bbpa(const double *post, int n, int k, double target, double *best_score);
 uint32_t orc_brute_force(const double *post, int n, int k, double *best_score);
 uint32_t orc_select(const double *post, int n, int k, int selector);

@@ -80,6 +80,10 @@ def lib(variant: str = "strict") -> ctypes.CDLL:
     L.orc_zeta.argtypes = [_dptr, ctypes.c_int]
     L.orc_op_bha.argtypes = [_dptr, ctypes.c_int, _dptr]
     L.orc_op_bha.restype = ctypes.c_uint32
+    L.orc_op_bha_omp.argtypes = [_dptr, ctypes.c_int, _dptr]
+    L.orc_op_bha_omp.restype = ctypes.c_uint32
+    L.orc_threads.argtypes = []
+    L.orc_threads.restype = ctypes.c_int
     L.orc_bbpa.argtypes = [_dptr, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dptr]
     L.orc_bbpa.restype = ctypes.c_uint32
     L.orc_brute_force.argtypes = [_dptr, ctypes.c_int, ctypes.c_int, _dptr]

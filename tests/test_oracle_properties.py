@@ -199,3 +199,19 @@ def test_planted_wrong_bit_order_in_zeta_is_caught():
                 wrong[e | (1 << b)] += wrong[e]
     assert np.max(np.abs(wrong - superset_sums(x, n))) > 1e-3
     np.testing.assert_allclose(orc.zeta(x, n), superset_sums(x, n), rtol=1e-13)
+
+
+def test_openmp_op_bha_picks_the_serial_pick():
+    # the all-cores baseline (bbpa.cpp:920-940 restated with OpenMP, fast build) must select
+    # what the serial scan selects, ties to the lowest index included
+    rng = np.random.default_rng(17)
+    L = orc.lib("fast")
+    for n in (16, 17):
+        for post in (rng.random(1 << n), orc.prior(rng.uniform(0.01, 0.2, n)), np.full(1 << n, 2.0 ** -n)):
+            post = np.ascontiguousarray(post / post.sum())
+            a, b = np.empty(1 << n), np.empty(1 << n)
+            p = post.ctypes.data_as(orc._dptr)
+            e1 = L.orc_op_bha(p, n, a.ctypes.data_as(orc._dptr))
+            e2 = L.orc_op_bha_omp(p, n, b.ctypes.data_as(orc._dptr))
+            assert e1 == e2
+    assert L.orc_threads() >= 1
