@@ -1356,8 +1356,10 @@ std::vector<uint32_t> DeviceLattice::select_experiments(int steps)
 			best = std::min(best, scores[c]);
 		const double limit = best + 2.0 * lookahead_score_error(n_, m);
 		uint32_t best_c = 1;
-		while (scores[best_c] > limit)
+		while (best_c < (1u << n_) && !(scores[best_c] <= limit))
 			++best_c;
+		if (best_c == (1u << n_))
+			throw Error(BGT_HIP_INTERNAL, "lookahead found no candidate");
 		out.push_back(best_c);
 	}
 	return out;
