@@ -71,6 +71,13 @@ __device__ __forceinline__ void st_u(T *ubase, uint32_t byte_off, T v)
 {
 	*(gptr<T>)(reinterpret_cast<char *>(ubase) + byte_off) = v;
 }
+// The same as a streaming (non-temporal) store: for data not read again soon, such as the
+// updated posterior, which the next round reads after a pass over the whole lattice.
+template <typename T>
+__device__ __forceinline__ void st_u_nt(T *ubase, uint32_t byte_off, T v)
+{
+	__builtin_nontemporal_store(v, (gptr<T>)(reinterpret_cast<char *>(ubase) + byte_off));
+}
 
 struct Top2 {
 	double d1, d2;
@@ -573,7 +580,7 @@ __device__ __forceinline__ void pass1_tile(const LatDesc &d, unsigned tile, cons
 			else
 				lk = likelihood(g, d.e, d.n, d.k, d.len, d.dilution, s_lik);
 			const TP y = static_cast<TP>(x * lk);
-			st_u(tb + off, lob, y);
+			st_u_nt(tb + off, lob, y); // read again only by the next round
 			x = static_cast<double>(y);
 		}
 		v[j] = x;
