@@ -595,14 +595,15 @@ __device__ __forceinline__ void pass1_tile(const LatDesc &d, unsigned tile, cons
 	zeta_tile(v, 0, lane, wave, lds);
 	TZ *zt = uniform_ptr(static_cast<TZ *>(d.z)) + (uint64_t)tile * z_pitch<TZ>();
 	const uint32_t loz = (lane_bits_f(lane) + ((uint32_t)wave << 6)) * (uint32_t)sizeof(TZ);
+	// FP64 z streams past the caches (measured faster in both passes); FP32 z stays cached,
+	// as pass 2 reads it next (streaming it moved time from pass 2 to pass 1)
 #pragma unroll
-	for (int j = 0; j < kTileRegs; ++j)
-		// FP64 z streams past the caches (measured faster in both passes); FP32 z stays
-		// cached, which pass 2 reads next (streaming it moved time from pass 2 to pass 1)
+	for (int j = 0; j < kTileRegs; ++j) {
 		if constexpr (sizeof(TZ) == 8)
 			st_u_nt(zt + (((j & 15) << 10) | ((j >> 4) << 14)), loz, static_cast<TZ>(v[j]));
 		else
 			st_u(zt + (((j & 15) << 10) | ((j >> 4) << 14)), loz, static_cast<TZ>(v[j]));
+	}
 	// after the full 15-bit sweep, tile index 0 (thread 0, register 0) holds the tile's total
 	if (threadIdx.x == 0)
 		d.part[tile] = v[0];
