@@ -369,6 +369,7 @@ int bgt_hip_lattice_kernel_times(bgt_hip_lattice *h, int enable, double *ms_out,
 		if (enable >= 0) {
 			clk.reset();
 			clk.enabled = enable != 0;
+			clk.light = enable == 2;
 			return;
 		}
 		clk.resolve();

@@ -74,10 +74,12 @@ hipEvent_t KernelClock::take()
 	return e;
 }
 
-void KernelClock::begin(hipStream_t s)
+void KernelClock::begin(hipStream_t s, bool essential)
 {
-	if (!enabled)
+	if (!enabled || (light && !essential)) {
+		open_ = nullptr;
 		return;
+	}
 	if (pending_.size() >= 4096)
 		resolve();
 	open_ = take();
@@ -1172,7 +1174,7 @@ bgt_hip_selection DeviceLattice::select(int selector)
 		if (objective != kObjGather)
 			collect_band(dz, entries, objective, true, false, target);
 	} else {
-		clock_.begin(stream_);
+		clock_.begin(stream_, true);
 		launch_zeta_pass1(dz, 1, atoms_, storage_, false, k_, dilution_, stream_);
 		clock_.end(stream_, "zeta_pass1");
 		clock_.begin(stream_);
@@ -1384,7 +1386,7 @@ bgt_hip_selection DeviceLattice::round(uint32_t experiment, uint32_t response, c
 	Descs ds = single(); // descriptors of the pre-update state (pass 2 applies inv_out)
 	// a persistent pass 2 sums pass 1's partials itself (one launch and its gap fewer)
 	const bool fold = persistent_pass2(atoms_, kObjHalving, false);
-	clock_.begin(stream_);
+	clock_.begin(stream_, true);
 	launch_zeta_pass1(ds, 1, atoms_, storage_, true, k_, dilution_, stream_, fold);
 	clock_.end(stream_, "round_pass1");
 	commit_update();

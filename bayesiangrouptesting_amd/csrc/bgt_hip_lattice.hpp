@@ -16,13 +16,16 @@ namespace bgt_hip {
 // launch. Recording never blocks the host; resolve() waits once and folds the pairs in.
 struct KernelClock {
 	bool enabled = false;
+	// light: only the launches marked essential (pass 1) are bracketed by events; each event
+	// record costs the stream a few microseconds, which a benchmark's timed rounds then carry
+	bool light = false;
 	struct Entry {
 		std::string name;
 		double total_ms = 0.0;
 		int launches = 0;
 	};
 	std::vector<Entry> entries;
-	void begin(hipStream_t s);
+	void begin(hipStream_t s, bool essential = false);
 	void end(hipStream_t s, const char *name);
 	// host-side entries: `ms` of host time, or a count carried in total_ms
 	void host(const char *name, double ms);
