@@ -65,12 +65,18 @@ def _storage(precision: Precision) -> int:
     raise InvalidArgument(nat.BAD_INPUT, "unknown precision")
 
 
+# bit reversal of every byte value
+_REV8 = bytes(int(format(i, "08b")[::-1], 2) for i in range(256))
+
+
 def reverse_bits(value: int, width: int) -> int:
-    """lattice_api.cpp:17-26"""
-    ret = 0
-    for i in range(width):
-        ret = (ret << 1) | ((value >> i) & 1)
-    return ret
+    """lattice_api.cpp:17-26: the low `width` bits of `value` (width <= 32) in reverse order.
+    By bytes through a table: a round calls it several times, on its host critical path."""
+    if width <= 0:
+        return 0
+    v = value & 0xFFFFFFFF
+    r = (_REV8[v & 0xFF] << 24) | (_REV8[(v >> 8) & 0xFF] << 16) | (_REV8[(v >> 16) & 0xFF] << 8) | _REV8[v >> 24]
+    return r >> (32 - width)
 
 
 def _uses_dilution(t: LatticeType) -> bool:

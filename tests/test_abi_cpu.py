@@ -62,3 +62,13 @@ def test_no_device_is_reported_as_unsupported_when_absent():
         pytest.skip("a device is present")
     with pytest.raises(bgt.Unsupported):
         bgt.Lattice(bgt.LatticeType.replicated_non_dilution, 2, [0.2, 0.4])
+
+
+def test_public_bit_flip_matches_the_oracle():
+    # lattice_api.cpp:17-26, the low `width` bits reversed; the mirror's byte-table form
+    # against the oracle's bit loop, widths 0..32
+    from oracle import pyoracle as orc
+    rng = np.random.default_rng(5)
+    for width in range(0, 33):
+        for v in rng.integers(0, 1 << 32, 64, dtype=np.uint64):
+            assert bgt.reverse_bits(int(v), width) == (orc.reverse_bits(int(v), width) if width else 0)
