@@ -379,7 +379,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this provider runs one lattice per GPU and does not shard "
-              "across devices (DESIGN.md describes how it would); run with --gpus 1", file=sys.stderr)
+              "across devices; run with --gpus 1", file=sys.stderr)
         return 2
 
     import bayesiangrouptesting_amd as bgt

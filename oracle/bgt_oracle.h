@@ -93,7 +93,7 @@ int orc_run_simulation_noshrink(int dilution, int n, int k, const double *prior,
 								double up, double lo, double branch, double alpha, double h,
 								orc_tree_stats *stats);
 
-/* one C2-style round with timings, used for the CPU baseline */
+/* one binary round (Op-BHA select, update, atom marginals) with timings, used for the CPU baseline */
 double orc_now(void);
 
 #ifdef __cplusplus
