@@ -812,8 +812,9 @@ double DeviceLattice::band_width(int sel) const
 void DeviceLattice::collect_band(const Descs &ds, unsigned entries, int objective, bool from_posterior, bool scaled,
 								 double target)
 {
-	// lattices of at most kSmallBits atoms collect their band inside the transform kernel
-	if (cand_ == nullptr || atoms_ <= kSmallBits)
+	// lattices of at most kSmallBits atoms collect their band inside the transform kernel, and
+	// after a persistent pass 2 the final-kernel launch already compacted it
+	if (cand_ == nullptr || atoms_ <= kSmallBits || persistent_pass2(atoms_, objective, from_posterior))
 		return;
 	clock_.begin(stream_);
 	launch_band_collect(ds, 1, entries, atoms_, k_, storage_, objective, from_posterior, scaled, target, stream_);
