@@ -1547,61 +1547,64 @@ __global__ __launch_bounds__(256) void k_lookahead(Descs descs, int n, const uin
 // candidate. Subset relations are kept by the public bit reversal, so D is built in internal
 // coordinates; candidates are ranked by public index, as the reference's loop.
 
-// Pass A: D over bits 0..nb-1 of contiguous blocks of 2^nb entries (x = y * (1/D) read from
-// the posterior, the sums in FP64).
+// Pass A: D over bits 0..11. A wave owns 4096 contiguous entries: lane = bits 0..5 (every
+// load and store a 512-byte run in FP64), register r = bits 6..11. x = y * (1/D) is read from
+// the posterior and summed in FP64; no LDS and no barrier.
+constexpr int kDownFirstBits = 12;
 template <typename TP>
-__global__ __launch_bounds__(256) void k_down_first(Descs descs, double *D, int nb)
+__global__ __launch_bounds__(256) void k_down_first(Descs descs, double *D)
 {
 	const LatDesc &d = descs.many ? descs.many[0] : descs.one;
-	extern __shared__ double t[];
-	const uint32_t size = 1u << nb;
-	const uint64_t base = (uint64_t)blockIdx.x << nb;
+	const uint32_t lane = threadIdx.x & 63u;
+	const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+	const uint64_t base = (wave << kDownFirstBits) | lane;
 	gptr<const TP> post = G(static_cast<const TP *>(d.post));
 	const double inv = d.scal[d.inv_in];
-	for (uint32_t i = threadIdx.x; i < size; i += blockDim.x)
-		t[i] = static_cast<double>(post[base + i]) * inv;
-	__syncthreads();
-	for (int b = 0; b < nb; ++b) {
-		const uint32_t low = (1u << b) - 1u;
-		for (uint32_t x = threadIdx.x; x < (size >> 1); x += blockDim.x) {
-			const uint32_t k = ((x & ~low) << 1) | (x & low); // bit b clear
-			t[k | (1u << b)] += t[k];
+	double v[64];
+#pragma unroll
+	for (int r = 0; r < 64; ++r)
+		v[r] = static_cast<double>(post[base + ((uint64_t)r << 6)]) * inv;
+#pragma unroll
+	for (int b = 0; b < 6; ++b)
+#pragma unroll
+		for (int r = 0; r < 64; ++r)
+			if (r & (1 << b))
+				v[r] += v[r ^ (1 << b)];
+#pragma unroll
+	for (int b = 0; b < 6; ++b) {
+		const bool up = (lane >> b) & 1u; // bit b set: add the partner's value (bit b clear)
+#pragma unroll
+		for (int r = 0; r < 64; ++r) {
+			const double o = __shfl_xor(v[r], 1 << b);
+			v[r] += up ? o : 0.0;
 		}
-		__syncthreads();
 	}
-	for (uint32_t i = threadIdx.x; i < size; i += blockDim.x)
-		D[base + i] = t[i];
+#pragma unroll
+	for (int r = 0; r < 64; ++r)
+		D[base + ((uint64_t)r << 6)] = v[r];
 }
 
-// Pass B: D over bits b0..b0+nb-1 (nb <= 8) on tiles of 16 contiguous entries (bits 0..3,
-// 128-byte rows) by 2^nb rows; 32 KB of LDS, so several workgroups share a CU.
-constexpr int kDownCols = 16;
-__global__ __launch_bounds__(256) void k_down_more(double *D, int b0, int nb)
+// Pass B: D over bits b0..b0+R-1 (b0 >= 12, R <= 6) in registers: a thread owns one column
+// below b0 (consecutive columns across the lanes, so 512-byte runs) and its 2^R entries.
+template <int R>
+__global__ __launch_bounds__(256) void k_down_more(double *D, int b0)
 {
-	__shared__ double t[kDownCols * 256];
-	const uint32_t rows = 1u << nb;
-	// tile: kDownCols columns below bit b0 (group cg), the bits above b0+nb (hi)
-	const uint64_t cols = 1ull << (b0 - 4); // column groups below bit b0
-	const uint64_t cg = blockIdx.x % cols, hi = blockIdx.x / cols;
-	const uint64_t base = (cg << 4) | (hi << (b0 + nb));
-	for (uint32_t i = threadIdx.x; i < kDownCols * rows; i += blockDim.x) {
-		const uint32_t r = i >> 4, c = i & 15u;
-		t[i] = D[base + c + ((uint64_t)r << b0)];
-	}
-	__syncthreads();
-	for (int b = 0; b < nb; ++b) {
-		const uint32_t low = (1u << b) - 1u;
-		for (uint32_t x = threadIdx.x; x < kDownCols * (rows >> 1); x += blockDim.x) {
-			const uint32_t r = x >> 4, c = x & 15u;
-			const uint32_t rk = ((r & ~low) << 1) | (r & low);
-			t[((rk | (1u << b)) << 4) + c] += t[(rk << 4) + c];
-		}
-		__syncthreads();
-	}
-	for (uint32_t i = threadIdx.x; i < kDownCols * rows; i += blockDim.x) {
-		const uint32_t r = i >> 4, c = i & 15u;
-		D[base + c + ((uint64_t)r << b0)] = t[i];
-	}
+	const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	const uint64_t col = t & ((1ull << b0) - 1ull), hi = t >> b0;
+	const uint64_t base = col | (hi << (b0 + R));
+	double v[1 << R];
+#pragma unroll
+	for (int r = 0; r < (1 << R); ++r)
+		v[r] = D[base + ((uint64_t)r << b0)];
+#pragma unroll
+	for (int b = 0; b < R; ++b)
+#pragma unroll
+		for (int r = 0; r < (1 << R); ++r)
+			if (r & (1 << b))
+				v[r] += v[r ^ (1 << b)];
+#pragma unroll
+	for (int r = 0; r < (1 << R); ++r)
+		D[base + ((uint64_t)r << b0)] = v[r];
 }
 
 // Scores of every candidate into scores[internal index] (candidate 0 scores +inf), and each
@@ -2523,21 +2526,37 @@ void launch_lookahead(const Descs &descs, int n, Storage st, const uint32_t *sel
 
 void launch_down_sums(const Descs &descs, int atoms, Storage st, double *D, hipStream_t stream)
 {
-	const int nb = atoms < 12 ? atoms : 12;
+	// pass A: 2^(atoms-12) waves of 4096 entries, up to four waves per workgroup
+	if (atoms < kDownFirstBits || atoms > 32)
+		throw Error(4, "down-set sums need 12..32 atoms");
 	launch_checked("k_down_first", [&] {
-		const dim3 grid(1u << (atoms - nb));
-		const size_t lds = sizeof(double) << nb;
+		const uint64_t threads = 1ull << (atoms - 6);
+		const unsigned block = (unsigned)std::min<uint64_t>(256, threads);
+		const dim3 grid((unsigned)(threads / block));
 		if (st == Storage::fp64)
-			hipLaunchKernelGGL(k_down_first<double>, grid, dim3(256), lds, stream, descs, D, nb);
+			hipLaunchKernelGGL(k_down_first<double>, grid, dim3(block), 0, stream, descs, D);
 		else
-			hipLaunchKernelGGL(k_down_first<float>, grid, dim3(256), lds, stream, descs, D, nb);
+			hipLaunchKernelGGL(k_down_first<float>, grid, dim3(block), 0, stream, descs, D);
 	});
-	for (int b0 = nb; b0 < atoms; b0 += 8) {
-		const int w = atoms - b0 < 8 ? atoms - b0 : 8;
-		const dim3 grid((unsigned)(1ull << (atoms - w - 4)));
+	// pass B: the bits above 12 in ceil(rest / 6) register passes of near-equal width; each
+	// pass runs 2^(atoms-R) threads, 256 per workgroup (atoms - R >= 12)
+	const int rest = atoms - kDownFirstBits;
+	const int passes = (rest + 5) / 6;
+	for (int p = 0, b0 = kDownFirstBits; p < passes; ++p) {
+		const int w = (rest - (b0 - kDownFirstBits) + (passes - p) - 1) / (passes - p);
+		const dim3 grid((unsigned)(1ull << (atoms - w - 8)));
 		launch_checked("k_down_more", [&] {
-			hipLaunchKernelGGL(k_down_more, grid, dim3(256), 0, stream, D, b0, w);
+			switch (w) {
+			case 1: hipLaunchKernelGGL(k_down_more<1>, grid, dim3(256), 0, stream, D, b0); break;
+			case 2: hipLaunchKernelGGL(k_down_more<2>, grid, dim3(256), 0, stream, D, b0); break;
+			case 3: hipLaunchKernelGGL(k_down_more<3>, grid, dim3(256), 0, stream, D, b0); break;
+			case 4: hipLaunchKernelGGL(k_down_more<4>, grid, dim3(256), 0, stream, D, b0); break;
+			case 5: hipLaunchKernelGGL(k_down_more<5>, grid, dim3(256), 0, stream, D, b0); break;
+			case 6: hipLaunchKernelGGL(k_down_more<6>, grid, dim3(256), 0, stream, D, b0); break;
+			default: throw Error(4, "down-set pass width out of range");
+			}
 		});
+		b0 += w;
 	}
 }
 

@@ -60,7 +60,9 @@ def test_lookahead_fp32c_against_its_own_posterior(n):
 
 @pytest.mark.parametrize("n,updates,steps,precision", [(18, 4, 5, bgt.Precision.float64),
                                                        (20, 6, 4, bgt.Precision.float64),
-                                                       (20, 6, 4, bgt.Precision.float32_compensated)])
+                                                       (20, 6, 4, bgt.Precision.float32_compensated),
+                                                       (22, 5, 3, bgt.Precision.float64),
+                                                       (24, 4, 2, bgt.Precision.float32_compensated)])
 def test_lookahead_larger_lattices(n, updates, steps, precision):
     lat = session(n, 700 + n, updates, precision=precision)
     print(f"lookahead n={n} steps={steps} {precision}", flush=True)
