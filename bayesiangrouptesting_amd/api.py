@@ -132,6 +132,34 @@ class Selection:
                    int(s.candidates), int(s.band), int(s.collected))
 
 
+def response_probability(lattice_type: "LatticeType", subjects: int, variants: int, experiment: int,
+                         response: int, true_state: int, dilution: Optional[DilutionTable] = None) -> float:
+    """L(E, r, S) in internal coordinates (replicated_lattice.cpp:8-28): the product over
+    variants of P(r_v | E, S_v). Non-dilution: P(r_v = 1) = 0.99 when the pool E lies inside
+    S_v's negatives, 0.01 otherwise. Dilution: P(r_v = 1) = D[|E| - 1][missing], missing =
+    |E| - |E & S_v|; the empty pool has no table row (the reference would index row -1), and
+    P(r_v = 1 | E = 0) = 0 here as on the device."""
+    ret = 1.0
+    mask = (1 << subjects) - 1
+    if not _uses_dilution(lattice_type):
+        for v in range(variants):
+            inside = (experiment & (true_state >> (v * subjects)) & mask) == experiment
+            rb = (response >> v) & 1
+            ret *= (0.99 if rb else 1.0 - 0.99) if inside else (1.0 - 0.99 if rb else 0.99)
+        return ret
+    if dilution is None:
+        raise InvalidArgument(nat.BAD_INPUT, "dilution table is required for dilution lattices")
+    size = bin(experiment).count("1")
+    for v in range(variants):
+        if size == 0:
+            p1 = 0.0
+        else:
+            missing = size - bin(experiment & (true_state >> (v * subjects)) & mask).count("1")
+            p1 = dilution.values[(size - 1) * (dilution.subjects() + 1) + missing]
+        ret *= p1 if (response >> v) & 1 else 1.0 - p1
+    return ret
+
+
 class Lattice:
     """bgt::Lattice (include/bgt/lattice.hpp:34-67) backed by a device-resident posterior."""
 
@@ -210,22 +238,8 @@ class Lattice:
     def response_probability(self, experiment: int, response: int, true_state: int,
                              dilution: Optional[DilutionTable] = None) -> float:
         # lattice_api.cpp:101-106 -> replicated_lattice.cpp:8-28 (host-side scalar query)
-        e = self._flip(experiment)
-        s = self._flip(true_state)
-        ret = 1.0
-        if not _uses_dilution(self._type):
-            for v in range(self._k):
-                inside = (e & (s >> (v * self._n))) == e
-                rb = (response >> v) & 1
-                ret *= (0.99 if rb else (1.0 - 0.99)) if inside else ((1.0 - 0.99) if rb else 0.99)
-            return ret
-        if dilution is None:
-            raise InvalidArgument(nat.BAD_INPUT, "dilution table is required for dilution lattices")
-        size = bin(e).count("1")
-        for v in range(self._k):
-            p1 = dilution.values[(size - 1) * (dilution.subjects() + 1) + size - bin(e & (s >> (v * self._n))).count("1")]
-            ret *= p1 if (response >> v) & 1 else 1.0 - p1
-        return ret
+        return response_probability(self._type, self._n, self._k, self._flip(experiment), response,
+                                    self._flip(true_state), dilution)
 
     # -- selection
     def select(self, selector: SelectorType = SelectorType.auto_select) -> Selection:

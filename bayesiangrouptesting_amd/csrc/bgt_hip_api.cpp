@@ -123,9 +123,22 @@ int bgt_hip_provider_benchmark_select(int subjects, int variants, const double *
 									  double *min_seconds, double *max_seconds, double *stddev_seconds)
 {
 	// timing shape of bgt_cuda_provider_benchmark_select (cuda_provider.cu:2449-2551)
+	// argument checks and codes as the reference's: 1 for a missing output, 2 for a shape or
+	// selector it cannot run (Op-BHA needs one variant)
+	if (candidate == nullptr || init_seconds == nullptr || mean_seconds == nullptr || min_seconds == nullptr ||
+		max_seconds == nullptr || stddev_seconds == nullptr) {
+		g_last_error = "benchmark_select: every output pointer is required";
+		return BGT_HIP_BAD_INPUT;
+	}
+	*candidate = 0;
+	*init_seconds = *mean_seconds = *min_seconds = *max_seconds = *stddev_seconds = 0.0;
+	if (subjects <= 0 || variants <= 0 || iterations <= 0 || warmup < 0 || prior == nullptr ||
+		subjects * variants > 30 || variants > 10 ||
+		(variants != 1 && selector == BGT_HIP_SELECT_OP_BHA)) {
+		g_last_error = "benchmark_select: unsupported shape or selector";
+		return BGT_HIP_UNSUPPORTED;
+	}
 	return guarded([&] {
-		if (iterations <= 0 || warmup < 0 || prior == nullptr)
-			throw bgt_hip::Error(BGT_HIP_BAD_INPUT, "invalid benchmark arguments");
 		using clk = std::chrono::steady_clock;
 		const auto t0 = clk::now();
 		bgt_hip::DeviceLattice lat(false, subjects, variants, prior, bgt_hip::Storage::fp64);
@@ -150,18 +163,12 @@ int bgt_hip_provider_benchmark_select(int subjects, int variants, const double *
 		for (double x : t)
 			var += (x - mean) * (x - mean);
 		var /= iterations;
-		if (candidate)
-			*candidate = sel.experiment;
-		if (init_seconds)
-			*init_seconds = init;
-		if (mean_seconds)
-			*mean_seconds = mean;
-		if (min_seconds)
-			*min_seconds = mn;
-		if (max_seconds)
-			*max_seconds = mx;
-		if (stddev_seconds)
-			*stddev_seconds = std::sqrt(var);
+		*candidate = sel.experiment;
+		*init_seconds = init;
+		*mean_seconds = mean;
+		*min_seconds = mn;
+		*max_seconds = mx;
+		*stddev_seconds = std::sqrt(var);
 	});
 }
 

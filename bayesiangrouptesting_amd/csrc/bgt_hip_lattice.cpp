@@ -794,7 +794,8 @@ double DeviceLattice::band_width(int sel) const
 	// (1 + avg_{S>=E} e) / (1 + avg_all e), so M - M* = M* (1 - M*) (avg_E e - avg_notE e)
 	// and |M - M*| <= M*(1 - M*) 2 (t+1) u32 <= (t+1) u32 / 2, plus one FP32 rounding of the
 	// transform intermediate (M* u32) and FP64 sums far below; BBPA's |M - t| + |(T - M) - t|
-	// doubles it (T = sum x is 1 up to FP64 rounding).
+	// doubles it (T = sum x is 1 up to FP64 rounding). With k >= 2 variants each of the 2^k
+	// response masses is an inclusion-exclusion over 2^|R| feature masses, 3^k in all.
 	// FP64: the per-mass relative bound of error_bound().
 	double eps;
 	if (storage_ == Storage::fp32c) {
@@ -805,7 +806,9 @@ double DeviceLattice::band_width(int sel) const
 		error_bound(&post_rel, &mass_rel);
 		eps = mass_rel;
 	}
-	const double scale = sel == kSelBha ? 1.0 : (k_ == 1 ? 2.0 : double(1 << k_) * 2.0);
+	double scale = 1.0;
+	if (sel != kSelBha)
+		scale = k_ == 1 ? 2.0 : std::pow(3.0, k_);
 	return 2.0 * scale * eps;
 }
 

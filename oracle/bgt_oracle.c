@@ -253,9 +253,10 @@ int orc_threads(void)
 #endif
 }
 
-uint32_t orc_bbpa(const double *post, int n, int k, double target, double *best_score)
+static uint32_t bbpa_core(const double *post, int n, int k, double target, double *best_score, double *scores)
 {
-	/* bbpa.cpp:950-970 (BBPA_serial) */
+	/* bbpa.cpp:950-970 (BBPA_serial); scores[e], when given, receives every experiment's
+	 * objective as the scorers form it */
 	const uint32_t ec = bit(n);
 	const uint32_t rc = bit(k);
 	const uint32_t mask = ec - 1;
@@ -275,6 +276,8 @@ uint32_t orc_bbpa(const double *post, int n, int k, double target, double *best_
 			const double positive = fm[e];
 			const double negative = total - positive;
 			const double score = fabs(positive - target) + fabs(negative - target);
+			if (scores)
+				scores[e] = score;
 			if (score < best || (score == best && e < cand)) {
 				best = score;
 				cand = e;
@@ -331,6 +334,8 @@ uint32_t orc_bbpa(const double *post, int n, int k, double target, double *best_
 			const double r2 = p0 - pb;
 			const double r3 = total - p0 - p1 + pb;
 			const double score = fabs(r0 - target) + fabs(r1 - target) + fabs(r2 - target) + fabs(r3 - target);
+			if (scores)
+				scores[e] = score;
 			if (score < best || (score == best && e < cand)) {
 				best = score;
 				cand = e;
@@ -357,6 +362,8 @@ uint32_t orc_bbpa(const double *post, int n, int k, double target, double *best_
 				}
 				score += fabs(exact - target);
 			}
+			if (scores)
+				scores[e] = score;
 			if (score < best || (score == best && e < cand)) {
 				best = score;
 				cand = e;
@@ -367,6 +374,16 @@ uint32_t orc_bbpa(const double *post, int n, int k, double target, double *best_
 	if (best_score)
 		*best_score = best;
 	return cand;
+}
+
+uint32_t orc_bbpa(const double *post, int n, int k, double target, double *best_score)
+{
+	return bbpa_core(post, n, k, target, best_score, NULL);
+}
+
+uint32_t orc_bbpa_scores(const double *post, int n, int k, double target, double *scores)
+{
+	return bbpa_core(post, n, k, target, NULL, scores);
 }
 
 static inline uint32_t response_bucket(uint32_t e, uint32_t s, int n, int k)

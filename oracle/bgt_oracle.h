@@ -52,6 +52,8 @@ uint32_t orc_op_bha(const double *post, int n, double *upset_out);
 uint32_t orc_op_bha_omp(const double *post, int n, double *upset_out);
 int orc_threads(void);
 uint32_t orc_bbpa(const double *post, int n, int k, double target, double *best_score);
+/* orc_bbpa with every experiment's objective written to scores[0 .. 2^n) */
+uint32_t orc_bbpa_scores(const double *post, int n, int k, double target, double *scores);
 uint32_t orc_brute_force(const double *post, int n, int k, double *best_score);
 uint32_t orc_select(const double *post, int n, int k, int selector);
 void orc_atom_masses(const double *post, int atoms, double *out);

@@ -86,6 +86,8 @@ def lib(variant: str = "strict") -> ctypes.CDLL:
     L.orc_threads.restype = ctypes.c_int
     L.orc_bbpa.argtypes = [_dptr, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dptr]
     L.orc_bbpa.restype = ctypes.c_uint32
+    L.orc_bbpa_scores.argtypes = [_dptr, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dptr]
+    L.orc_bbpa_scores.restype = ctypes.c_uint32
     L.orc_brute_force.argtypes = [_dptr, ctypes.c_int, ctypes.c_int, _dptr]
     L.orc_brute_force.restype = ctypes.c_uint32
     L.orc_select.argtypes = [_dptr, ctypes.c_int, ctypes.c_int, ctypes.c_int]
@@ -192,6 +194,14 @@ def bbpa(post: np.ndarray, n: int, k: int, target: float | None = None, variant:
     best = ctypes.c_double()
     e = lib(variant).orc_bbpa(_p(post), n, k, t, ctypes.byref(best))
     return int(e), best.value
+
+
+def bbpa_scores(post: np.ndarray, n: int, k: int, target: float | None = None) -> tuple[int, np.ndarray]:
+    """The Op-BBPA pick and every experiment's objective (index e holds experiment e's)."""
+    t = 1.0 / (1 << k) if target is None else target
+    scores = np.empty(1 << n, dtype=np.float64)
+    e = lib().orc_bbpa_scores(_p(post), n, k, t, _p(scores))
+    return int(e), scores
 
 
 def brute_force(post: np.ndarray, n: int, k: int):

@@ -31,19 +31,126 @@ def rel_err(a, b, floor=0.0):
     return float(np.max(r)) if r.size else 0.0
 
 
-def oracle_objective(post, n, k, selector, e):
-    """Objective value the oracle assigns to experiment e."""
+U64 = 2.0 ** -53
+
+
+def oracle_scores(post, n, k, selector):
+    """The oracle's objective for every experiment (internal coordinates): |M(E) - 1/2| for
+    Op-BHA (bbpa.cpp:537-551), the bucket distance for Op-BBPA (bbpa.cpp:201-220,332-365,
+    416-460), and the oracle's pick."""
     if selector == 1:
-        m = orc.upset_mass(post, n, e)
-        return abs(m - 0.5)
+        e, upset = orc.op_bha(post, n)
+        return e, np.abs(upset - 0.5)
+    return orc.bbpa_scores(post, n, k)
+
+
+def accurate_masses(post):
+    """Atom masses m_a = sum of the entries with bit a set, by halving: the upper half of the
+    array is bit A-1's mass; folding it onto the lower half leaves the masses of the rest.
+    numpy's pairwise sums keep the rounding near log2(S) ulp (the oracle's serial per-atom sums
+    carry ~1e-11 at 2^26 states)."""
+    atoms = int(len(post)).bit_length() - 1
+    out = np.empty(atoms)
+    y = post
+    for a in range(atoms - 1, -1, -1):
+        h = len(y) // 2
+        out[a] = y[h:].sum()
+        y = y[:h] + y[h:]
+    return out
+
+
+def accurate_scores(post, n, k, selector):
+    """Every experiment's objective from the subset zeta transform of the whole posterior
+    (M(E) = sum over S containing E, each output a chain of at most A two-operand adds, as the
+    device forms it): F_B(E) = M(E spread over the variant blocks in B), F_{} = M(0) the total,
+    response masses by inclusion-exclusion, objective sum |P - 1/2^k| (Op-BHA: |M - 1/2|).
+    The oracle's BBPA builds its feature tables and total by serial sums over the states
+    (bbpa.cpp:79-141,237-285), whose rounding grows with the state count; the difference of the
+    two measures that rounding."""
+    if selector == 1:
+        return np.abs(orc.op_bha(post, n)[1] - 0.5)
+    z = orc.zeta(post, n * k)
+    es = np.arange(1 << n, dtype=np.int64)
+    feats = [np.full(1 << n, z[0])]
+    for b in range(1, 1 << k):
+        idx = np.zeros(1 << n, dtype=np.int64)
+        for v in range(k):
+            if (b >> v) & 1:
+                idx |= es << (v * n)
+        feats.append(z[idx])
+    del z
     t = 1.0 / (1 << k)
-    mass = np.zeros(1 << k)
-    mask = (1 << n) - 1
-    states = np.arange(1 << (n * k), dtype=np.uint64)
-    rid = np.zeros(states.shape, dtype=np.int64)
-    for v in range(k):
-        vs = (states >> np.uint64(v * n)) & np.uint64(mask)
-        rid |= ((np.uint64(e) & ~vs) != 0).astype(np.int64) << v
-    for b in range(1 << k):
-        mass[b] = post[rid == b].sum()
-    return float(np.abs(mass - t).sum())
+    full = (1 << k) - 1
+    score = np.zeros(1 << n)
+    for resp in range(1 << k):
+        p = np.zeros(1 << n)
+        x = resp
+        while True:
+            f = feats[(~resp & full) | x]
+            p = p + f if bin(x).count("1") % 2 == 0 else p - f
+            if x == 0:
+                break
+            x = (x - 1) & resp
+        score += np.abs(p - t)
+    return score
+
+
+def pick_bound(objs_ref, objs_acc, e_gpu, e_ref, bound):
+    """tie bound for comparing the device's pick with the oracle's: the FP64 bound of the
+    accurate values plus the oracle's own measured rounding on the two candidates."""
+    err = max(abs(objs_ref[e_gpu] - objs_acc[e_gpu]), abs(objs_ref[e_ref] - objs_acc[e_ref]))
+    return bound + 2.0 * float(err)
+
+
+def tie_bound(atoms, k, selector, scale_err=0.0, updates=20):
+    """Largest difference of two objectives that FP64 rounding can invert.
+
+    A mass is a chain of at most `atoms` additions over posterior entries that carry an
+    `atoms`-term product and two roundings per update each, so its rounding is below
+    4 * (atoms + updates) * 2^-52 relative (masses are at most 1); the posteriors of two FP64 implementations also differ by a common factor
+    (each normalises by its own sum; the reference's serial sum is the less accurate), whose
+    relative size is `scale_err`. An Op-BHA objective holds one mass, a binary BBPA objective
+    two, and a k-variant BBPA objective 3^k (each of the 2^k response masses is an
+    inclusion-exclusion over 2^|R| feature masses). Two objectives are compared, so twice that.
+    """
+    per_mass = 4.0 * (atoms + updates) * 2.0 ** -52 + scale_err
+    terms = 1 if selector == 1 else (2 if k == 1 else 3 ** k)
+    return 2.0 * terms * per_mass
+
+
+def fp32c_band(updates, k, selector):
+    """Width of the FP32C error band: twice the bound on one objective's error (stored entries
+    carry one FP32 rounding for the prior and one per update, the transform intermediate one
+    more; a mass then errs by at most (0.5 (t + 1) + 1.5) 2^-24), times the number of masses
+    an objective holds (1 for Op-BHA, 2 for binary BBPA, 3^k for k variants)."""
+    eps = (0.5 * (updates + 1) + 1.5) * 2.0 ** -24 * 1.001
+    terms = 1 if selector == 1 else (2 if k == 1 else 3 ** k)
+    return 2.0 * terms * eps
+
+
+def assert_pick(e_gpu, scores, e_ref, bound, what=""):
+    """The device's pick is the oracle's, or the oracle itself ranks the two within `bound`."""
+    if e_gpu == e_ref:
+        return
+    assert e_gpu < len(scores), (what, e_gpu)
+    gap = float(scores[e_gpu] - scores[e_ref])
+    assert 0.0 <= gap <= bound, (what, e_gpu, e_ref, gap, bound)
+
+
+def check_selection(e_gpu, objective, post_ref, n, k, selector, bound, what="", obj_bound=None):
+    """The device's selection (internal coordinates) against the oracle on the oracle's
+    posterior: its reported objective within `obj_bound` (default `bound`) of the accurate
+    value, and its pick the oracle's unless the oracle ranks the two within `bound` plus the
+    oracle's own rounding. Brute force (selector 3) shares Op-BBPA's objective. Returns the
+    oracle's pick."""
+    sel = 1 if selector == 1 else 2
+    if selector == 3:
+        _, objs = orc.bbpa_scores(post_ref, n, k)
+        e_ref = orc.select(post_ref, n, k, 3)
+    else:
+        e_ref, objs = oracle_scores(post_ref, n, k, sel)
+    acc = accurate_scores(post_ref, n, k, sel)
+    ob = bound if obj_bound is None else obj_bound
+    assert abs(objective - acc[e_gpu]) <= ob, (what, objective, acc[e_gpu], ob)
+    assert_pick(e_gpu, objs, e_ref, pick_bound(objs, acc, e_gpu, e_ref, bound), what)
+    return e_ref

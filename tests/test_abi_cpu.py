@@ -72,3 +72,53 @@ def test_public_bit_flip_matches_the_oracle():
     for width in range(0, 33):
         for v in rng.integers(0, 1 << 32, 64, dtype=np.uint64):
             assert bgt.reverse_bits(int(v), width) == (orc.reverse_bits(int(v), width) if width else 0)
+
+
+def test_dilution_response_probability_kat(kats):
+    # replicated_lattice.cpp:17-28 through the host mirror, public coordinates flipped as
+    # bgt::Lattice does for a binary lattice (lattice_api.cpp:101-106)
+    from tests.conftest import expr
+    k = kats["dilution"]
+    t = bgt.DilutionTable(2, k["alpha"], k["h"])
+    for e, r, s, want in k["public_response_probability"]:
+        got = bgt.response_probability(bgt.LatticeType.replicated_dilution, 2, 1, bgt.reverse_bits(e, 2), r,
+                                       bgt.reverse_bits(s, 2), t)
+        assert abs(got - expr(want)) <= 1e-12
+
+
+def test_response_probability_defines_the_empty_pool():
+    # no table row exists for |E| = 0 (the reference would read row -1): P(r = 1 | E = 0) = 0,
+    # the device's definition; non-dilution lattices treat E = 0 as inside every state
+    t = bgt.DilutionTable(3, 0.05, 0.9)
+    dl, nd = bgt.LatticeType.replicated_dilution, bgt.LatticeType.replicated_non_dilution
+    assert bgt.response_probability(dl, 3, 1, 0, 1, 5, t) == 0.0
+    assert bgt.response_probability(dl, 3, 2, 0, 0, 5, t) == 1.0
+    assert bgt.response_probability(nd, 3, 1, 0, 1, 0, None) == 0.99
+    with pytest.raises(bgt.InvalidArgument):
+        bgt.response_probability(dl, 3, 1, 1, 1, 0, None)
+
+
+def test_response_probability_matches_the_oracle():
+    from oracle import pyoracle as orc
+    n, k = 3, 2
+    t = bgt.DilutionTable(n, 0.05, 0.9)
+    for dilution, lt in ((True, bgt.LatticeType.replicated_dilution), (False, bgt.LatticeType.replicated_non_dilution)):
+        for e in range(1, 1 << n):
+            for r in range(1 << k):
+                for s in range(1 << (n * k)):
+                    want = orc.response_prob(dilution, n, k, e, r, s, orc.dilution_table(n, 0.05, 0.9))
+                    assert bgt.response_probability(lt, n, k, e, r, s, t if dilution else None) == want
+
+
+def test_benchmark_select_argument_codes():
+    # cuda_provider.cu:2449-2486: a missing output is 1, a shape or selector it cannot run is 2
+    prior = np.full(4, 0.1)
+    p = prior.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    c = ctypes.c_uint32()
+    d = [ctypes.c_double() for _ in range(5)]
+    refs = [ctypes.byref(x) for x in d]
+    L = nat.lib()
+    assert L.bgt_hip_provider_benchmark_select(4, 1, p, 1, 1, 0, None, *refs) == nat.BAD_INPUT
+    assert L.bgt_hip_provider_benchmark_select(2, 2, p, 1, 1, 0, ctypes.byref(c), *refs) == nat.UNSUPPORTED
+    assert L.bgt_hip_provider_benchmark_select(4, 1, p, 1, 0, 0, ctypes.byref(c), *refs) == nat.UNSUPPORTED
+    assert L.bgt_hip_provider_benchmark_select(31, 1, p, 1, 1, 0, ctypes.byref(c), *refs) == nat.UNSUPPORTED

@@ -1,0 +1,142 @@
+"""The multinomial Op-BBPA workload and the serving mix, at full size, against the oracle.
+
+Multinomial dilution lattice: n = 8 subjects, k = 3 variants (24 atoms, 2^24 states),
+DilutionTable(8, 0.05, 0.9), priors U[0.01, 0.10] (seed 3); 8 rounds of Op-BBPA select over
+the 2^3 response masks, then an update with a response sampled from a seeded true state.
+Both sides follow the oracle's picks, so every round is checked. Bars per round:
+  * the selection is the oracle's, unless the oracle's own objectives of the two picks lie
+    within the FP64 tie bound (FP32C: plus its error band, since no exact re-decision
+    applies to dilution lattices); the reported objective within the same bound;
+  * FP64 posterior within 1e-12 relative after removing the common normalisation factor,
+    FP32C within 1e-6 (entries above 1e-30 of the total); atom masses likewise.
+
+Serving mix (4096 sessions, n uniform in 10..24, k in {1, 2}, n*k <= 26, dilution 50/50,
+priors U[0.005, 0.10], seed 5): a slice holding the largest k = 2 sessions (24 and 26 atoms,
+with and without dilution), large binary sessions and small ones is served together by the
+engine in FP64, then each session is replayed by an oracle-driven adaptive loop with the same
+simulated lab: oracle update, oracle selection, classification from accurately summed oracle
+masses. Every served experiment must be the oracle's pick or one the oracle ranks within the
+FP64 tie bound; where they are equal all the way, test counts and classification masks must be
+equal too.
+"""
+import numpy as np
+import pytest
+
+import bayesiangrouptesting_amd as bgt
+from bayesiangrouptesting_amd import serving, workload
+from oracle import pyoracle as orc
+from tests.gpu_helpers import (accurate_masses, accurate_scores, assert_pick, check_selection, fp32c_band, oracle_scores,
+                               pick_bound, rel_err, tie_bound)
+
+pytestmark = pytest.mark.gpu
+DL = bgt.LatticeType.replicated_dilution
+ND = bgt.LatticeType.replicated_non_dilution
+
+
+@pytest.mark.parametrize("precision", [bgt.Precision.float64, bgt.Precision.float32_compensated])
+def test_multinomial_dilution_bbpa_rounds(precision):
+    c = workload.C3
+    n, k = c.subjects, c.variants
+    atoms = n * k
+    pi = workload.priors(c.seed, atoms, c.prior_lo, c.prior_hi)
+    tab = bgt.DilutionTable(n, c.alpha, c.h)
+    dil = orc.dilution_table(n, c.alpha, c.h)
+    assert np.array_equal(np.asarray(tab.values), dil)
+    lat = bgt.Lattice(DL, n, pi, variants=k, precision=precision)
+    ref = orc.prior(pi)
+    truth = workload.true_state(c.seed, pi)
+    sampler = workload.ResponseSampler(c.seed, n, k, truth, list(dil))
+    fp32c = precision == bgt.Precision.float32_compensated
+    tol = 1e-6 if fp32c else 1e-12
+    scale_err = 0.0
+    for rnd in range(8):
+        sel = lat.select(bgt.SelectorType.op_bbpa)
+        bound = tie_bound(atoms, k, 2, scale_err, updates=rnd)
+        if fp32c:
+            bound += fp32c_band(rnd, k, 2)
+        e_ref = check_selection(sel.experiment, sel.objective, ref, n, k, 2, bound, f"round {rnd}")
+        r = sampler.draw(e_ref)
+        lat.update(e_ref, r, tab)
+        orc.update_in_place(ref, True, n, k, e_ref, r, dil)
+        post = lat.posterior()
+        scale = post.sum() / ref.sum()
+        scale_err = abs(scale - 1.0)
+        big = ref >= 1e-30 * ref.sum()
+        assert rel_err(post[big] / scale, ref[big]) <= tol, f"round {rnd}"
+        m_ref = accurate_masses(ref)
+        assert np.max(np.abs(lat.atom_masses() / scale - m_ref) / m_ref) <= tol, f"round {rnd}"
+    lat.close()
+
+
+def _slice(specs):
+    """Per dilution flag: the first session of each of the largest k = 2 shapes (26, 24 and 20
+    atoms), of the large binary shapes (23-24 and 19-21 subjects), and two small sessions."""
+    picks, seen = [], {}
+    rules = [
+        ("k2-26", lambda s: s.variants == 2 and s.subjects == 13, 1),
+        ("k2-24", lambda s: s.variants == 2 and s.subjects == 12, 1),
+        ("k2-20", lambda s: s.variants == 2 and s.subjects == 10, 1),
+        ("k1-24", lambda s: s.variants == 1 and s.subjects >= 23, 1),
+        ("k1-20", lambda s: s.variants == 1 and 19 <= s.subjects <= 21, 1),
+        ("small", lambda s: s.subjects * s.variants <= 14, 2),
+    ]
+    for s in specs:
+        for name, rule, want in rules:
+            key = (name, s.dilution)
+            if rule(s) and seen.get(key, 0) < want:
+                seen[key] = seen.get(key, 0) + 1
+                picks.append(s)
+                break
+    return picks
+
+
+SPECS = _slice(serving.serving_mix(4096, rate=300.0, seed=5))
+
+
+@pytest.fixture(scope="module")
+def served():
+    eng = serving.ServingEngine(precision=bgt.Precision.float64, hbm_budget=16 << 30)
+    rep = eng.run(SPECS, realtime=False)
+    assert rep["sessions"] == len(SPECS)
+    return {r.sid: r for r in rep["results"]}
+
+
+def test_slice_covers_the_large_multinomial_sessions():
+    shapes = {(s.subjects, s.variants, s.dilution) for s in SPECS}
+    for shape in ((13, 2, True), (13, 2, False), (12, 2, True), (12, 2, False)):
+        assert shape in shapes, shape
+
+
+@pytest.mark.parametrize("spec", SPECS, ids=lambda s: f"sid{s.sid}-n{s.subjects}k{s.variants}{'d' if s.dilution else ''}")
+def test_served_session_follows_the_oracle(served, spec):
+    got = served[spec.sid]
+    n, k = spec.subjects, spec.variants
+    atoms = n * k
+    selector = 1 if k == 1 else 2
+    flip = (lambda v: orc.reverse_bits(v, n)) if k == 1 else (lambda v: v)
+    dil = orc.dilution_table(n, 0.05, 0.9) if spec.dilution else None
+    truth = workload.true_state(spec.seed, spec.prior)
+    # the engine's simulated lab works in public coordinates (bit-reversed for k = 1)
+    sampler = workload.ResponseSampler(spec.seed, n, k, flip(truth), list(dil) if dil is not None else None)
+    ref = orc.prior(spec.prior)
+    up = lo = 0.01
+    pos, neg = orc.classify(accurate_masses(ref), up, lo)
+    tests = 0
+    while bin(pos | neg).count("1") != atoms and tests < 20:
+        e_ref, objs = oracle_scores(ref, n, k, selector)
+        acc = accurate_scores(ref, n, k, selector)
+        assert tests < len(got.experiments), (tests, got.experiments)
+        e_dev = flip(got.experiments[tests])
+        # the oracle's normalisation against an accurate one: the device's own error is far below
+        scale_err = abs(ref.sum() - 1.0) + 2.0 ** -45
+        bound = tie_bound(atoms, k, selector, scale_err, updates=tests)
+        assert_pick(e_dev, objs, e_ref, pick_bound(objs, acc, e_dev, e_ref, bound), f"test {tests}")
+        if e_dev != e_ref:
+            return  # a tie within FP64 rounding: the trajectories part here
+        r = sampler.draw(flip(e_ref))
+        orc.update_in_place(ref, spec.dilution, n, k, e_ref, r, dil)
+        tests += 1
+        pos, neg = orc.classify(accurate_masses(ref), up, lo, pos, neg)
+    assert got.tests == tests
+    assert (flip(got.positive), flip(got.negative)) == (pos, neg)
+    assert len(got.experiments) == tests

@@ -4,16 +4,19 @@ Tolerances (the bar for the FP64 plain form):
   * prior: bit-identical (same product order);
   * posteriors and masses: 1e-12 relative; the GPU's denominator is a tree sum and the
     reference's a serial sum, so posteriors are compared after removing the common scale
-    as well as raw;
-  * selected experiments and classification masks: identical whenever the top-two gap
-    exceeds the reported gap_bound (near ties are reported, not asserted).
+    as well as raw (raw at 1e-9 above 2^22 states, where the serial sum itself carries
+    ~1e-12); masses against accurately summed oracle masses on the device's normalisation,
+    and up to 2^22 states against the oracle's own serial sums too;
+  * selected experiments: the oracle's, unless the oracle's own objectives of the two picks
+    lie within the FP64 tie bound (tests/gpu_helpers.tie_bound); objectives within that bound;
+  * classification masks: identical.
 """
 import numpy as np
 import pytest
 
 import bayesiangrouptesting_amd as bgt
 from oracle import pyoracle as orc
-from tests.gpu_helpers import TOL, op_sequence, oracle_objective, rel_err, seeded_prior
+from tests.gpu_helpers import TOL, accurate_masses, check_selection, op_sequence, rel_err, seeded_prior, tie_bound
 
 pytestmark = pytest.mark.gpu
 
@@ -40,7 +43,10 @@ def test_prior_bit_identical(n, k):
 
 
 SHAPES = [(3, 1, False), (10, 1, True), (14, 1, False), (15, 1, True), (16, 1, False), (18, 1, True),
-          (20, 1, False), (22, 1, True), (3, 2, True), (6, 2, False), (4, 3, True), (5, 3, False), (3, 4, False)]
+          (20, 1, False), (22, 1, True), (3, 2, True), (6, 2, False), (4, 3, True), (5, 3, False), (3, 4, False),
+          # k >= 2 above 15 atoms: the transform over pass 1 and the multi-tile final pass,
+          # feature tables gathered from it, scored by inclusion-exclusion
+          (8, 2, True), (10, 2, False), (11, 2, True), (12, 2, False), (13, 2, True), (6, 3, True), (4, 4, True)]
 
 
 @pytest.mark.parametrize("n,k,dilution", SHAPES)
@@ -59,25 +65,32 @@ def test_update_select_masses_sequence(n, k, dilution):
         lat.update(e, r, dil)
         den = orc.update_in_place(ref, dilution, n, k, e_int, r, dil.values if dil else None)
         got = lat.posterior()
-        assert rel_err(got, ref, floor=1e-300) <= TOL, f"posterior step {step}"
+        # above 2^22 states the oracle's serial denominator itself carries ~1e-12
+        assert rel_err(got, ref, floor=1e-300) <= (TOL if atoms <= 22 else 1e-9), f"posterior step {step}"
         scale = got.sum() / ref.sum()
         assert rel_err(got / scale, ref, floor=1e-300) <= TOL
-        # atom marginals and upset masses
+        del got
+        # atom marginals and upset masses, against accurately summed oracle values on the
+        # device's normalisation (the oracle's own serial sums are compared up to 2^22 states)
         m_gpu = lat.atom_masses()
-        m_ref = orc.atom_masses(ref, atoms)
-        assert np.max(np.abs(m_gpu - m_ref)) <= TOL
+        m_acc = accurate_masses(ref) * scale
+        assert np.max(np.abs(m_gpu - m_acc)) <= TOL
+        if atoms <= 22:
+            assert np.max(np.abs(m_gpu - orc.atom_masses(ref, atoms))) <= TOL
+        z = orc.zeta(ref, atoms)
         for probe in (0, 1, e):
             probe_pub = probe
             probe_int = orc.reverse_bits(probe, n) if k == 1 else probe
-            assert abs(lat.upset_probability_mass(probe_pub) - orc.upset_mass(ref, atoms, probe_int)) <= TOL
+            got_m = lat.upset_probability_mass(probe_pub)
+            assert abs(got_m - z[probe_int] * scale) <= TOL
+            if atoms <= 22:
+                assert abs(got_m - orc.upset_mass(ref, atoms, probe_int)) <= TOL
+        del z
         for selector in selectors:
             sel = lat.select(bgt.SelectorType(selector))
             e_gpu = orc.reverse_bits(sel.experiment, n) if k == 1 else sel.experiment
-            e_ref = orc.select(ref, n, k, selector)
-            obj_ref = oracle_objective(ref, n, k, 1 if selector == 1 else 2, e_ref)
-            assert abs(sel.objective - obj_ref) <= 1e-11, (selector, sel.objective, obj_ref)
-            if e_gpu != e_ref:
-                assert sel.near_tie or (sel.runner_up_objective - sel.objective) <= 1e-11, (selector, e_gpu, e_ref)
+            bound = tie_bound(atoms, k, selector, abs(scale - 1.0), updates=step + 1)
+            check_selection(e_gpu, sel.objective, ref, n, k, selector, bound, (selector, step))
         assert den > 0
 
 
@@ -125,15 +138,6 @@ def test_lattice_api_kats_through_provider(kats):
     assert ml.select_experiment(bgt.SelectorType.op_bbpa) == ml.select_experiment(bgt.SelectorType.brute_force)
 
 
-def test_dilution_response_probability_kat(kats):
-    k = kats["dilution"]
-    lat = bgt.Lattice(DL, 2, k["prior"])
-    t = bgt.DilutionTable(2, k["alpha"], k["h"])
-    from tests.conftest import expr
-    for e, r, s, want in k["public_response_probability"]:
-        assert abs(lat.response_probability(e, r, s, t) - expr(want)) <= TOL
-
-
 @pytest.mark.parametrize("n", [8, 12])
 def test_lookahead_matches_oracle(n):
     lat, pi = make(n, 1, False, 5 + n)
@@ -177,3 +181,29 @@ def test_edge_cases_and_errors():
     big, _ = make(20, 1, False, 4)
     with pytest.raises(bgt.Unsupported):
         big.select_experiment(bgt.SelectorType.brute_force)
+
+
+@pytest.mark.parametrize("n,k,selector", [(14, 1, 1), (20, 1, 2), (6, 2, 2), (4, 3, 0)])
+def test_benchmark_select_through_the_c_abi(n, k, selector):
+    # bgt_hip_provider_benchmark_select (cuda_provider.hpp:14-24): the prior's selection,
+    # internal coordinates, with the timing statistics of the timed iterations
+    import ctypes
+    from bayesiangrouptesting_amd import _native as nat
+    from tests.gpu_helpers import accurate_scores, assert_pick, oracle_scores, pick_bound
+    pi = np.ascontiguousarray(seeded_prior(60 + n * k, n * k, 0.02, 0.3))
+    c = ctypes.c_uint32()
+    init, mean, mn, mx, sd = (ctypes.c_double() for _ in range(5))
+    rc = nat.lib().bgt_hip_provider_benchmark_select(n, k, pi.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                                     selector, 5, 2, ctypes.byref(c), ctypes.byref(init),
+                                                     ctypes.byref(mean), ctypes.byref(mn), ctypes.byref(mx),
+                                                     ctypes.byref(sd))
+    assert rc == nat.OK, nat.lib().bgt_hip_last_error()
+    ref = orc.prior(pi)
+    sel = 1 if (selector == 1 or (selector == 0 and k == 1)) else 2
+    # objective not returned by this entry point: check the pick alone
+    e_ref, objs = oracle_scores(ref, n, k, sel)
+    acc = accurate_scores(ref, n, k, sel)
+    bound = tie_bound(n * k, k, sel, updates=0)
+    assert_pick(c.value, objs, e_ref, pick_bound(objs, acc, c.value, e_ref, bound))
+    assert 0.0 < mn.value <= mean.value <= mx.value
+    assert init.value > 0.0 and sd.value >= 0.0

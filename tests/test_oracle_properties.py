@@ -215,3 +215,32 @@ def test_openmp_op_bha_picks_the_serial_pick():
             e2 = L.orc_op_bha_omp(p, n, b.ctypes.data_as(orc._dptr))
             assert e1 == e2
     assert L.orc_threads() >= 1
+
+
+def test_fast_library_leaves_denormals_on():
+    # the timed baseline is built with -ffast-math; linked with it, gcc adds crtfastmath.o,
+    # which switches the loading process to flush-to-zero / denormals-are-zero
+    orc.lib("fast")
+    assert np.finfo(np.float64).smallest_subnormal > 0
+    tiny = np.array([1e-310])
+    assert float((tiny * 1.0)[0]) != 0.0
+    assert float((tiny / 2.0)[0]) > 0.0
+
+
+@pytest.mark.parametrize("n,k,dilution", [(5, 1, False), (4, 2, True), (3, 3, False), (3, 3, True), (2, 4, False)])
+def test_bbpa_scores_are_the_bucket_sweep_objectives(n, k, dilution):
+    # every experiment's objective from the restated scorers equals the explicit bucket sweep
+    post = random_posterior(40 + n * k, n, k, dilution)
+    t = 1.0 / (1 << k)
+    scores = np.empty(1 << n)
+    e_best = orc.lib().orc_bbpa_scores(orc._p(post), n, k, t, orc._p(scores))
+    states = np.arange(1 << (n * k), dtype=np.uint64)
+    mask = np.uint64((1 << n) - 1)
+    for e in range(1 << n):
+        rid = np.zeros(states.shape, dtype=np.int64)
+        for v in range(k):
+            vs = (states >> np.uint64(v * n)) & mask
+            rid |= ((np.uint64(e) & ~vs) != 0).astype(np.int64) << v
+        mass = np.array([post[rid == b].sum() for b in range(1 << k)])
+        assert abs(scores[e] - np.abs(mass - t).sum()) <= 1e-13
+    assert e_best == orc.bbpa(post, n, k)[0] == int(np.argmin(scores))
