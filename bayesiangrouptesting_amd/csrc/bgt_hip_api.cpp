@@ -3,6 +3,7 @@
 #include "bgt_hip_batch.hpp"
 #include "bgt_hip_exact.hpp"
 #include "bgt_hip_lattice.hpp"
+#include "bgt_hip_noalloc.hpp"
 #include "bgt_hip_tree.hpp"
 
 #include <hip/hip_runtime_api.h>
@@ -470,6 +471,15 @@ int bgt_hip_batch_round(bgt_hip_lattice **lattices, const uint32_t *experiments,
 	return guarded([&] {
 		run_batch(lattices, experiments, responses, dilutions, count, selector, true, threshold_up, threshold_lo,
 				  out, positive, negative);
+	});
+}
+
+int bgt_hip_select_noalloc(const void *posterior, int subjects, int precision, int selector, double scale,
+						   bgt_hip_selection *result, double *masses, void *stream)
+{
+	return guarded([&] {
+		bgt_hip::select_noalloc(posterior, subjects, precision, selector, scale, result, masses,
+								static_cast<hipStream_t>(stream));
 	});
 }
 

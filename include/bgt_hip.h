@@ -271,6 +271,36 @@ int bgt_hip_batch_round(bgt_hip_lattice **lattices, const uint32_t *experiments,
 						int selector, double threshold_up, double threshold_lo,
 						bgt_hip_selection *out, uint32_t *positive, uint32_t *negative);
 
+/* ---- selection without device allocation ---- */
+
+/* Largest lattice and static scratch size of bgt_hip_select_noalloc. */
+#define BGT_HIP_NOALLOC_MAX_SUBJECTS 20
+#define BGT_HIP_NOALLOC_SCRATCH_BYTES 8454144 /* 8.06 MiB per GPU, held until the process ends */
+
+/* No counterpart in the reference. For callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops and harnesses that forbid hipMalloc): the selection
+ * of bgt_hip_lattice_select (same kernels, same launch shapes) on a caller-owned posterior of
+ * a binary lattice, with every other buffer in a static device array of
+ * BGT_HIP_NOALLOC_SCRATCH_BYTES that each GPU holds until the process ends. It calls no
+ * hipMalloc, hipFree or allocating library routine.
+ *   posterior    device memory, 2^subjects entries in internal coordinates, double
+ *                (precision BGT_HIP_FP64) or float (BGT_HIP_FP32C); entry S is read as
+ *                posterior[S] * scale (scale > 0: a pending normalisation, 1.0 for none)
+ *   subjects     1..BGT_HIP_NOALLOC_MAX_SUBJECTS (status 2 beyond)
+ *   selector     BGT_HIP_SELECT_AUTO / _OP_BHA (Op-BHA) or BGT_HIP_SELECT_OP_BBPA (binary)
+ *   result       device memory, one bgt_hip_selection: experiment, runner-up and their
+ *                objectives as bgt_hip_lattice_select reports them before any exact
+ *                re-decision; gap_bound is the transform's own FP64 rounding on the top two
+ *                (errors the posterior already carries are the caller's to add), near_tie
+ *                whether the gap lies within it; redecided, candidates, band, collected = 0
+ *   masses       device memory for the atom marginals (subjects doubles) or NULL
+ *   stream       the hipStream_t to run on (NULL: the default stream); the call returns at
+ *                once, and its results are ready when the stream reaches them
+ * Every call on one GPU shares that GPU's scratch array: calls that overlap on one GPU (on
+ * different streams, or from different threads) race and must be serialised by the caller. */
+int bgt_hip_select_noalloc(const void *posterior, int subjects, int precision, int selector, double scale,
+						   bgt_hip_selection *result, double *masses, void *stream);
+
 /* ---- exact rescoring (host only, no device needed) ---- */
 
 /* Exact FP64 upset masses M(E) of a binary non-dilution lattice after the given updates,
