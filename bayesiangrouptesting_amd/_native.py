@@ -98,6 +98,9 @@ def _sig(L):
                                               c_int, _dptr, ctypes.POINTER(TreeStatsC)], c_int),
         "bgt_hip_provider_benchmark_select": ([c_int, c_int, _dptr, c_int, c_int, c_int, _u32p, _dptr, _dptr, _dptr,
                                                _dptr, _dptr], c_int),
+        "bgt_hip_tune": ([c_int, c_int, c_int, c_int, c_int, c_int, ctypes.POINTER(c_int), _dptr, _dptr,
+                          ctypes.c_char_p, c_int], c_int),
+        "bgt_hip_tune_force": ([c_int], c_int),
         "bgt_hip_select_noalloc": ([ctypes.c_void_p, c_int, c_int, c_int, c_double, ctypes.c_void_p, ctypes.c_void_p,
                                     ctypes.c_void_p], c_int),
         "bgt_hip_lattice_create": ([c_int, c_int, c_int, _dptr, c_int, ctypes.POINTER(_H)], c_int),

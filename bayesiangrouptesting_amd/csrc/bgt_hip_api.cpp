@@ -4,6 +4,7 @@
 #include "bgt_hip_exact.hpp"
 #include "bgt_hip_lattice.hpp"
 #include "bgt_hip_noalloc.hpp"
+#include "bgt_hip_tune.hpp"
 #include "bgt_hip_tree.hpp"
 
 #include <hip/hip_runtime_api.h>
@@ -471,6 +472,43 @@ int bgt_hip_batch_round(bgt_hip_lattice **lattices, const uint32_t *experiments,
 	return guarded([&] {
 		run_batch(lattices, experiments, responses, dilutions, count, selector, true, threshold_up, threshold_lo,
 				  out, positive, negative);
+	});
+}
+
+int bgt_hip_tune(int use_dilution, int subjects, int variants, int precision, int selector, int force, int *form,
+				 double *ms_two_pass, double *ms_three_pass, char *spec_id, int spec_len)
+{
+	return guarded([&] {
+		if (subjects <= 0 || variants <= 0 || subjects * variants > bgt_hip::kMaxAtoms)
+			throw bgt_hip::Error(BGT_HIP_BAD_INPUT, "tune: shape out of range");
+		bgt_hip::require_device();
+		bgt_hip::TuneKey key;
+		key.dilution = use_dilution != 0;
+		key.subjects = subjects;
+		key.variants = variants;
+		key.selector = selector == BGT_HIP_SELECT_AUTO ? (variants == 1 ? BGT_HIP_SELECT_OP_BHA : BGT_HIP_SELECT_OP_BBPA)
+													   : selector;
+		key.storage = storage_of(precision);
+		const bgt_hip::TuneResult r = bgt_hip::tune(key, force != 0);
+		if (form)
+			*form = r.form;
+		if (ms_two_pass)
+			*ms_two_pass = r.ms[0];
+		if (ms_three_pass)
+			*ms_three_pass = r.ms[1];
+		if (spec_id && spec_len > 0) {
+			std::strncpy(spec_id, r.id.c_str(), (size_t)spec_len - 1);
+			spec_id[spec_len - 1] = '\0';
+		}
+	});
+}
+
+int bgt_hip_tune_force(int form)
+{
+	return guarded([&] {
+		if (form < -1 || form > 1)
+			throw bgt_hip::Error(BGT_HIP_BAD_INPUT, "tune_force: form is -1, 0 or 1");
+		bgt_hip::set_pass2_form(form);
 	});
 }
 

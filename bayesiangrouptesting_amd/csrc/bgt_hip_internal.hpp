@@ -198,6 +198,12 @@ unsigned zeta_final_groups(int atoms);
 // the final transform of `atoms` atoms is the persistent pass 2 (which collects its error
 // band in the pass, each state at most once)
 bool persistent_pass2(int atoms, int objective, bool from_posterior);
+// the FP32 three-pass form of the high bits (pass M over bits 15..24, pass T over the rest)
+// rather than one persistent pass 2, for lattices of 26..30 atoms
+bool three_pass_form(int atoms, Storage st);
+// -1: the tuned or default choice per atom count; 0: always two passes; 1: three where it
+// applies (a process-wide override, for tests and the tuner's own measurements)
+void set_pass2_form(int form);
 // Candidates of the last zeta_final within d.band_width of the best: d.band[0..counter[0]),
 // complete unless counter[0] exceeds d.band_cap (entries may repeat). Pass 2 of 21..30
 // atoms appends candidates while it scans and this keeps those inside the final band. For

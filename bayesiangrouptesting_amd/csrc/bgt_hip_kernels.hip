@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include "bgt_hip_internal.hpp"
+#include "bgt_hip_tune.hpp"
 
 #include <cmath>
 #include <cstdlib>
@@ -1185,6 +1186,247 @@ __global__ __launch_bounds__(kTileThreads) void k_zeta_pass2(Descs descs, int ob
 	publish_top2(d, blockIdx.x, run, red2);
 }
 
+// ----------------------------- three-pass transform of FP32 lattices of 26..30 atoms
+//
+// With 2^15-entry tiles the two passes of the transform share at most 30 - A state bits, so
+// pass 2 of a large FP32 lattice reads pieces of 16 or 32 bytes (see k_zeta_pass2). The
+// three-pass form gives the high bits two passes that both read whole 128-byte lines:
+//   pass M (k_zeta_mid): state bits 15..24 on tiles of 32 contiguous columns (128 bytes of
+//     z per row) by 1024 rows, in place in z, persistent with the next tile's entries loaded
+//     during the current sweep (as pass 1);
+//   pass T (k_zeta_top): bits 25..A-1, 2^(A-25) rows per column, consecutive lanes on
+//     consecutive columns, fused with the objective, the top-2, the error band and the
+//     atom singletons (as pass 2).
+// It moves 2 S b_z bytes more than the two-pass form. Sums stay FP64 on chip; z is rounded
+// to FP32 once more, between pass M and pass T. Bits are still swept in increasing order.
+constexpr int kMidBits = 10;                  // pass M sweeps bits 15..24
+constexpr int kMidCols = 5;                   // 32 columns per pass-M tile
+constexpr int kTopLo = kTileBits + kMidBits;  // pass T sweeps bits 25..A-1
+constexpr unsigned kTopGroups = 512;          // pass T: 512 x 512 threads (2^18, a power of two)
+// Untuned default: the two-pass form everywhere. Measured at 2^28 FP32 states (rocprofv3,
+// fused rounds): pass M 0.48 ms and pass T 0.50-0.66 ms against 0.64 ms for the split pass 2,
+// so the three-pass form is a candidate of the tuner only.
+constexpr int kThreePassFrom = 31;
+
+// z entries of a pass-M tile: column block T & 1023 (32 columns), row group T >> 10 (1024 rows)
+template <typename TZ>
+__device__ __forceinline__ TZ *mid_tile(TZ *z, unsigned T)
+{
+	return z + ((uint64_t)(T >> kMidBits) << kMidBits) * z_pitch<TZ>() + ((uint64_t)(T & 1023u) << kMidCols);
+}
+
+// Load layout (tile_index_a): column = lane & 31; row = lane bit 5 | jlo << 1 | wave << 5 | jhi << 9.
+template <typename TZ>
+__device__ __forceinline__ void mid_load(const TZ *tb, uint32_t lob, TZ (&raw)[kTileRegs])
+{
+	constexpr uint32_t pitch = z_pitch<TZ>();
+#pragma unroll
+	for (int j = 0; j < kTileRegs; ++j)
+		raw[j] = ld_u(tb + (size_t)((((j & 15) << 1) | ((j >> 4) << 9)) * pitch), lob);
+}
+
+template <typename TZ>
+__global__ __launch_bounds__(kTileThreads) void k_zeta_mid(Descs descs)
+{
+	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
+	__shared__ double lds[kTileLds];
+	constexpr uint32_t pitch = z_pitch<TZ>();
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const unsigned tiles = 1u << (__builtin_amdgcn_readfirstlane(d.atoms) - kTileBits);
+	TZ *z = uniform_ptr(static_cast<TZ *>(d.z));
+	const uint32_t lob = ((uint32_t)(lane & 31) + ((uint32_t)(lane >> 5) + ((uint32_t)wave << 5)) * pitch) *
+						 (uint32_t)sizeof(TZ);
+	// output layout (tile_index_f): column = lane_bits_f & 31; row = its bit 5 | wave << 1 | jlo << 5 | jhi << 9
+	const uint32_t lbf = lane_bits_f(lane);
+	const uint32_t los = ((lbf & 31u) + ((lbf >> 5) + ((uint32_t)wave << 1)) * pitch) * (uint32_t)sizeof(TZ);
+	// workgroup b serves the contiguous tile range of XCD b % 8: neighbouring column blocks
+	// (adjacent 128-byte pieces of the same rows) are read behind one L2
+	const unsigned wg_per_xcd = gridDim.x >> 3;
+	const unsigned per_xcd = tiles >> 3;
+	const unsigned xcd = blockIdx.x & 7u;
+	const unsigned first = xcd * per_xcd + (blockIdx.x >> 3), last = (xcd + 1) * per_xcd;
+	TZ next[kTileRegs];
+	if (first < last)
+		mid_load(mid_tile(z, first), lob, next);
+	for (unsigned T = first; T < last; T += wg_per_xcd) {
+		double v[kTileRegs];
+#pragma unroll
+		for (int j = 0; j < kTileRegs; ++j)
+			v[j] = static_cast<double>(next[j]);
+		if (T + wg_per_xcd < last)
+			mid_load(mid_tile(z, T + wg_per_xcd), lob, next);
+		zeta_tile(v, kMidCols, lane, wave, lds); // tile bits 5..14: state bits 15..24
+		TZ *tb = mid_tile(z, T);
+#pragma unroll
+		for (int j = 0; j < kTileRegs; ++j)
+			st_u(tb + (size_t)((((j & 15) << 5) | ((j >> 4) << 9)) * pitch), los, static_cast<TZ>(v[j]));
+	}
+}
+
+// Pass T: thread t of the launch's kTopThreads threads takes positions p = (it*P + q)*NT + t
+// (column p & 32767, row p >> 15 below bit 25) and their 2^HB rows of bits 25..A-1,
+// P = 32 / 2^HB positions per turn, so each turn holds 32 entries (state p + (h << 25) in
+// register q * 2^HB + h). NT is a multiple of 2^15, so every load is a wave-uniform base
+// plus the thread's own 32-bit offset. Objective, band and singletons as in k_zeta_pass2;
+// ties resolve by the order (|objective|, experiment), as everywhere.
+constexpr unsigned kTopThreads = kTopGroups * 512u;
+constexpr int kTopRegs = 16;
+template <typename TZ, int HB>
+__global__ __launch_bounds__(512) void k_zeta_top(Descs descs, int objective, int scaled, double target,
+												  unsigned fold, unsigned raw_cap)
+{
+	constexpr int H = 1 << HB;
+	constexpr int R = H > kTopRegs ? H : kTopRegs; // entries per thread per turn
+	constexpr int P = R / H;
+	constexpr uint32_t pitch = z_pitch<TZ>();
+	constexpr uint64_t hstride = (uint64_t)pitch << kMidBits;
+	constexpr uint32_t positions = 1u << kTopLo;
+	constexpr uint32_t NT = kTopThreads;
+	static_assert((NT & ((1u << kTileBits) - 1u)) == 0u, "pass T thread count must be a multiple of 2^15");
+	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
+	__shared__ double red[16];
+	__shared__ Top2 red2[16];
+	const int lane = threadIdx.x & 63;
+	double scale, total;
+	if (fold != 0u) {
+		// the fused update's denominator from pass 1's per-tile partials, formed exactly as
+		// k_finalize_sum (1024 threads) forms it: this launch has 512 threads per workgroup,
+		// so each thread folds the two strides of k_finalize_sum's threads t and t + 512 in
+		// their order, and the sum over waves takes them in that kernel's order
+		double t0 = 0.0, t1 = 0.0;
+		for (unsigned i = threadIdx.x; i < fold; i += 1024u)
+			t0 += d.part[i];
+		for (unsigned i = threadIdx.x + 512u; i < fold; i += 1024u)
+			t1 += d.part[i];
+		const double w0 = wave_sum(t0), w1 = wave_sum(t1);
+		const int w = threadIdx.x >> 6;
+		if (lane == 0) {
+			red[w] = w0;
+			red[w + 8] = w1;
+		}
+		__syncthreads();
+		if (threadIdx.x == 0) {
+			double s = 0.0;
+			for (int i = 0; i < 16; ++i)
+				s += red[i];
+			red[0] = s;
+		}
+		__syncthreads();
+		const double tsum = uniform(red[0]);
+		const double inv = uniform(1.0 / tsum);
+		if (blockIdx.x == 0 && threadIdx.x == 0) {
+			d.scal[kScalDen] = tsum;
+			d.scal[d.inv_out] = inv;
+			d.scal[kScalTotal] = tsum;
+		}
+		scale = uniform(scaled ? inv : 1.0);
+		total = uniform(scaled ? tsum * scale : tsum);
+	} else {
+		scale = uniform(scaled ? d.scal[d.inv_out] : 1.0);
+		total = uniform(scaled ? d.scal[kScalTotal] * scale : d.scal[kScalTotal]);
+	}
+	const uint32_t full = (d.n >= 32) ? 0xFFFFFFFFu : ((1u << d.n) - 1u);
+	const uint32_t inactive = full & ~d.active;
+	const bool banding = d.band != nullptr;
+	unsigned *slot = best_slots(d) + (blockIdx.x & 7u);
+	const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+	const TZ *z = uniform_ptr(static_cast<const TZ *>(d.z));
+	const uint32_t lob = ((t >> kTileBits) * pitch + (t & ((1u << kTileBits) - 1u))) * (uint32_t)sizeof(TZ);
+	double seen = INFINITY; // wave-uniform: the best |objective| this wave knows of
+	bool raw_full = false;  // wave-uniform: the candidate buffer has overflowed
+	Top2 run;
+	top2_init(run);
+	unsigned turn = 0;
+	for (uint32_t base = 0; base < positions; base += P * NT, ++turn) {
+		// the XCD's running best, read every eighth turn (its latency would otherwise stall
+		// each turn): a stale value only widens this wave's threshold
+		unsigned key = kKeyInf;
+		if (banding && (turn & 7u) == 0u)
+			key = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		double v[R];
+		const TZ *zb = z + (uint64_t)(base >> kTileBits) * pitch;
+#pragma unroll
+		for (int q = 0; q < P; ++q)
+#pragma unroll
+			for (int h = 0; h < H; ++h)
+				v[q * H + h] = static_cast<double>(
+					ld_u(zb + (uint64_t)q * (NT >> kTileBits) * pitch + (uint64_t)h * hstride, lob));
+		// bits 25..A-1 in increasing order
+#pragma unroll
+		for (int b = 0; b < HB; ++b)
+#pragma unroll
+			for (int r = 0; r < R; ++r)
+				if (!(r & (1 << b)))
+					v[r] += v[r | (1 << b)];
+		double best = INFINITY;
+#pragma unroll
+		for (int r = 0; r < R; ++r) {
+			const uint32_t g = base + (uint32_t)(r / H) * NT + t + ((uint32_t)(r % H) << kTopLo);
+			const double m = v[r] * scale;
+			if (__popc(g) == 1)
+				d.masses[__ffs(g) - 1] = m;
+			if ((g & inactive) == 0u) {
+				const double dd = signed_objective(objective, m, total, target);
+				top2_insert(run, dd, g);
+				best = fmin(best, fabs(dd));
+			}
+		}
+		if (banding) {
+			double wm = best;
+#pragma unroll
+			for (int o = 32; o > 0; o >>= 1)
+				wm = fmin(wm, __shfl_xor(wm, o));
+			if ((turn & 7u) == 0u)
+				seen = fmin(seen, key_value(__builtin_amdgcn_readfirstlane(key)));
+			if (wm < seen) {
+				seen = wm;
+				if (lane == 0)
+					atomicMin(slot, best_key(wm));
+			}
+			const double thr = seen + d.band_width;
+			if (!raw_full && __ballot(best <= thr) != 0ull) {
+				uint32_t inbits = 0u;
+				unsigned count_in = 0u;
+#pragma unroll
+				for (int r = 0; r < R; ++r) {
+					const uint32_t g = base + (uint32_t)(r / H) * NT + t + ((uint32_t)(r % H) << kTopLo);
+					const double sd = signed_objective(objective, v[r] * scale, total, target);
+					const bool in = (g & inactive) == 0u && fabs(sd) <= thr;
+					inbits |= (uint32_t)in << r;
+					count_in += (unsigned)__popcll(__ballot(in));
+				}
+				if (count_in != 0u) {
+					unsigned at = 0u;
+					if (lane == 0)
+						at = atomicAdd(&d.counter[kRawCount], count_in);
+					at = __builtin_amdgcn_readfirstlane(at);
+					raw_full = at + count_in > raw_cap;
+					if (!raw_full) {
+						const unsigned long long below = (1ull << lane) - 1ull;
+						double scale2 = scale;
+						uint32_t base2 = base;
+						asm volatile("" : "+v"(scale2), "+v"(base2));
+#pragma unroll
+						for (int r = 0; r < R; ++r) {
+							const bool in = (inbits >> r) & 1u;
+							const unsigned long long mask = __ballot(in);
+							if (in) {
+								Cand c;
+								c.d = signed_objective(objective, v[r] * scale2, total, target);
+								c.e = base2 + (uint32_t)(r / H) * NT + t + ((uint32_t)(r % H) << kTopLo);
+								c.flags = kCandSided;
+								d.cand[at + (unsigned)__popcll(mask & below)] = c;
+							}
+							at += (unsigned)__popcll(mask);
+						}
+					}
+				}
+			}
+		}
+	}
+	publish_top2(d, blockIdx.x, run, red2);
+}
+
 // Lattices of at most 2^14 states: the whole transform in the LDS of one workgroup.
 template <typename TP>
 __global__ __launch_bounds__(1024) void k_zeta_small(Descs descs, int atoms, int objective, double target)
@@ -2280,6 +2522,16 @@ unsigned zeta_final_groups(int atoms)
 	return tiles;
 }
 
+bool three_pass_form(int n, Storage st)
+{
+	if (st != Storage::fp32c || n < kTopLo + 1 || n > 30)
+		return false;
+	const int tuned = tuned_pass2_form(n, st);
+	if (tuned >= 0)
+		return tuned == 1;
+	return n >= kThreePassFrom;
+}
+
 bool persistent_pass2(int n, int objective, bool from_posterior)
 {
 	return !from_posterior && n >= 21 && n <= 30 && (objective == kObjHalving || objective == kObjBinaryBbpa);
@@ -2291,6 +2543,27 @@ bool persistent_pass2(int n, int objective, bool from_posterior)
 unsigned launch_final_kernel(const Descs &descs, int count, int n, Storage st, int objective, bool from_posterior,
 							 bool scaled, double target, int collect, hipStream_t stream, unsigned fold = 0)
 {
+	if (persistent_pass2(n, objective, from_posterior) && three_pass_form(n, st)) {
+		// FP32, 26..30 atoms: pass M over bits 15..24, then pass T over the rest with the objective
+		const dim3 gm(zeta_final_groups(n), count);
+		const dim3 gt(kTopGroups, count);
+		const unsigned raw_cap = 2u * zeta_final_groups(n) * kTileThreads;
+		const int sc = scaled ? 1 : 0;
+		launch_checked("k_zeta_mid", [&] {
+			hipLaunchKernelGGL(k_zeta_mid<float>, gm, dim3(kTileThreads), 0, stream, descs);
+		});
+		launch_checked("k_zeta_top", [&] {
+			switch (n - kTopLo) {
+			case 1: hipLaunchKernelGGL((k_zeta_top<float, 1>), gt, dim3(512), 0, stream, descs, objective, sc, target, fold, raw_cap); break;
+			case 2: hipLaunchKernelGGL((k_zeta_top<float, 2>), gt, dim3(512), 0, stream, descs, objective, sc, target, fold, raw_cap); break;
+			case 3: hipLaunchKernelGGL((k_zeta_top<float, 3>), gt, dim3(512), 0, stream, descs, objective, sc, target, fold, raw_cap); break;
+			case 4: hipLaunchKernelGGL((k_zeta_top<float, 4>), gt, dim3(512), 0, stream, descs, objective, sc, target, fold, raw_cap); break;
+			case 5: hipLaunchKernelGGL((k_zeta_top<float, 5>), gt, dim3(512), 0, stream, descs, objective, sc, target, fold, raw_cap); break;
+			default: throw Error(4, "three-pass transform needs 26..30 atoms");
+			}
+		});
+		return kTopGroups;
+	}
 	if (persistent_pass2(n, objective, from_posterior)) {
 		// persistent pass 2: one workgroup per CU (a multiple of 8, one share per XCD)
 		const dim3 grid(zeta_final_groups(n), count);
@@ -2370,11 +2643,13 @@ unsigned launch_zeta_final(const Descs &descs, int count, int n, int, Storage st
 												stream, fold);
 	const unsigned entries = 2u * groups * kTileThreads;
 	if (persistent_pass2(n, objective, from_posterior)) {
-		// the merge of the top-2 lists and the band compaction in one launch
+		// the merge of the top-2 lists and the band compaction in one launch; the raw candidate
+		// buffer holds two entries per thread of a persistent pass 2
+		const unsigned raw_cap = 2u * zeta_final_groups(n) * kTileThreads;
 		launch_checked("k_top2_compact", [&] {
-			hipLaunchKernelGGL(k_top2_compact, dim3(64, count), dim3(256), 0, stream, descs, groups, entries);
+			hipLaunchKernelGGL(k_top2_compact, dim3(64, count), dim3(256), 0, stream, descs, groups, raw_cap);
 		});
-		return entries;
+		return raw_cap;
 	}
 	launch_checked("k_finalize_top2", [&] {
 		hipLaunchKernelGGL(k_finalize_top2, dim3(1, count), dim3(1024), 0, stream, descs, groups);

@@ -800,7 +800,9 @@ double DeviceLattice::band_width(int sel) const
 	double eps;
 	if (storage_ == Storage::fp32c) {
 		const double u32 = std::ldexp(1.0, -24);
-		eps = (0.5 * (updates_ + 1) + 1.5) * u32 * 1.001 + 1e-15;
+		// the three-pass transform rounds its intermediate to FP32 once more (M* u32 more)
+		const double z_roundings = three_pass_form(atoms_, storage_) ? 2.0 : 1.0;
+		eps = (0.5 * (updates_ + 1) + 0.5 + z_roundings) * u32 * 1.001 + 1e-15;
 	} else {
 		double post_rel = 0.0, mass_rel = 0.0;
 		error_bound(&post_rel, &mass_rel);
@@ -1438,7 +1440,7 @@ void DeviceLattice::error_bound(double *posterior_rel, double *mass_rel) const
 	} else {
 		const double u32 = std::ldexp(1.0, -24);
 		*posterior_rel = (2.0 * updates_ + 2.0) * u32;
-		*mass_rel = *posterior_rel + 2.0 * u32;
+		*mass_rel = *posterior_rel + (three_pass_form(atoms_, storage_) ? 3.0 : 2.0) * u32;
 	}
 }
 

@@ -1,8 +1,9 @@
 """Short workloads for rocprofv3 runs (kernel traces and HBM counters).
 
-    python -m bayesiangrouptesting_amd.profile_workloads round28 [fp32c|fp64] [rounds]
+    python -m bayesiangrouptesting_amd.profile_workloads round28 [fp32c|fp64] [rounds] [two|three]
         n = 28 binary non-dilution rounds of the benchmark's adaptive sessions (true state of
-        session i drawn with seed 2000 + i, a session ends when classified), few rounds
+        session i drawn with seed 2000 + i, a session ends when classified), few rounds; the
+        last argument pins the FP32C transform form (default: the tuned or built-in choice)
     python -m bayesiangrouptesting_amd.profile_workloads copy [MiB]
         one device-to-device copy of a known size, the calibration point for FETCH_SIZE /
         WRITE_SIZE (bytes the counters report per byte actually moved)
@@ -19,11 +20,13 @@ import sys
 import time
 
 
-def round28(form: str = "fp32c", rounds: int = 4) -> None:
+def round28(form: str = "fp32c", rounds: int = 4, passes: str = "") -> None:
     import bayesiangrouptesting_amd as bgt
     from bayesiangrouptesting_amd import _native as nat
     from bayesiangrouptesting_amd import workload
 
+    if passes:
+        nat.check(nat.lib().bgt_hip_tune_force({"two": 0, "three": 1}[passes]))
     c = workload.C2
     n = c.subjects
     pi = workload.priors(c.seed, n, c.prior_lo, c.prior_hi)
@@ -118,7 +121,8 @@ def main(argv) -> int:
         return 2
     what = argv[0]
     if what == "round28":
-        round28(argv[1] if len(argv) > 1 else "fp32c", int(argv[2]) if len(argv) > 2 else 4)
+        round28(argv[1] if len(argv) > 1 else "fp32c", int(argv[2]) if len(argv) > 2 else 4,
+                argv[3] if len(argv) > 3 else "")
     elif what == "bbpa24":
         bbpa24(argv[1] if len(argv) > 1 else "fp32c", int(argv[2]) if len(argv) > 2 else 4)
     elif what == "lookahead28":

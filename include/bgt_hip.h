@@ -271,6 +271,25 @@ int bgt_hip_batch_round(bgt_hip_lattice **lattices, const uint32_t *experiments,
 						int selector, double threshold_up, double threshold_lo,
 						bgt_hip_selection *out, uint32_t *positive, uint32_t *negative);
 
+/* ---- autotuning ---- */
+
+/* Measure once per KernelSpec key (include/bgt/types.hpp:132-181) which launch form of the
+ * transform's high bits is fastest on this device for lattices of this shape -- the two-pass
+ * form or the three-pass form for FP32C lattices of 26..30 atoms -- and use it from then on.
+ * Results are cached in memory and in a text file per key named as the reference's JIT
+ * artefacts (kernel_spec_id, src/runtime/jit_compiler.cpp:13-38) in BGT_HIP_TUNE_DIR
+ * (default: the temporary directory's bayesian_group_testing_hip_tune); force = 1 measures
+ * again. form receives 0 (two passes) or 1 (three passes); ms_* the measured selection times
+ * (0 when taken from a cache or when the shape has one form only); spec_id (may be NULL) the
+ * key. Run it before sessions of that shape start: the choice applies process-wide. The
+ * reference's KernelRegistry / JitCompiler have no device measurement. */
+int bgt_hip_tune(int use_dilution, int subjects, int variants, int precision, int selector, int force, int *form,
+				 double *ms_two_pass, double *ms_three_pass, char *spec_id, int spec_len);
+
+/* Pin the transform form process-wide (0: two passes, 1: three passes where it applies);
+ * -1 returns to the tuned choice, or the built-in default (two passes). */
+int bgt_hip_tune_force(int form);
+
 /* ---- selection without device allocation ---- */
 
 /* Largest lattice and static scratch size of bgt_hip_select_noalloc. */

@@ -121,9 +121,10 @@ def tie_bound(atoms, k, selector, scale_err=0.0, updates=20):
 def fp32c_band(updates, k, selector):
     """Width of the FP32C error band: twice the bound on one objective's error (stored entries
     carry one FP32 rounding for the prior and one per update, the transform intermediate one
-    more; a mass then errs by at most (0.5 (t + 1) + 1.5) 2^-24), times the number of masses
-    an objective holds (1 for Op-BHA, 2 for binary BBPA, 3^k for k variants)."""
-    eps = (0.5 * (updates + 1) + 1.5) * 2.0 ** -24 * 1.001
+    more per spill, two in the three-pass form; a mass then errs by at most
+    (0.5 (t + 1) + 2.5) 2^-24), times the number of masses an objective holds (1 for Op-BHA,
+    2 for binary BBPA, 3^k for k variants)."""
+    eps = (0.5 * (updates + 1) + 2.5) * 2.0 ** -24 * 1.001
     terms = 1 if selector == 1 else (2 if k == 1 else 3 ** k)
     return 2.0 * terms * eps
 

@@ -1,0 +1,125 @@
+"""The three-pass transform of FP32C lattices of 26..30 atoms (pass M over state bits 15..24
+in place, pass T over the rest with the objective), against the oracle and against the
+two-pass form, and the autotuner that chooses between them.
+
+Bars: as the FP32C bars elsewhere -- masses within 1e-6 of the oracle's (accurately summed),
+objectives within the FP32C error band, picks the oracle's unless the oracle ranks the two
+within the FP64 tie bound (near ties are re-decided exactly for binary non-dilution
+lattices) or, for dilution lattices, within the FP32C band. Both forms re-decide near ties
+exactly, so on non-dilution lattices they pick the same experiment.
+"""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import bayesiangrouptesting_amd as bgt
+from bayesiangrouptesting_amd import _native as nat
+from bayesiangrouptesting_amd import workload
+from oracle import pyoracle as orc
+from tests.gpu_helpers import accurate_masses, check_selection, fp32c_band, seeded_prior, tie_bound
+from tests.test_gpu_large_lattices import _oracle_run
+
+pytestmark = pytest.mark.gpu
+ND = bgt.LatticeType.replicated_non_dilution
+DL = bgt.LatticeType.replicated_dilution
+FP32C = bgt.Precision.float32_compensated
+
+
+@pytest.fixture
+def form():
+    def force(f):
+        assert nat.lib().bgt_hip_tune_force(f) == nat.OK
+    yield force
+    force(-1)
+
+
+@pytest.mark.parametrize("n,dilution", [(26, False), (27, True), (28, False)])
+def test_three_pass_selection_against_the_oracle(form, n, dilution):
+    form(1)
+    pi = seeded_prior(700 + n, n, 0.005, 0.05)
+    lat = bgt.Lattice(DL if dilution else ND, n, pi, precision=FP32C)
+    t = bgt.DilutionTable(n, 0.05, 0.9) if dilution else None
+    dil = orc.dilution_table(n, 0.05, 0.9) if dilution else None
+    ref = orc.prior(pi)
+    truth = workload.true_state(700 + n, list(pi))
+    sampler = workload.ResponseSampler(700 + n, n, 1, truth, list(dil) if dil is not None else None)
+    for step in range(2):
+        for selector in (1, 2):
+            sel = lat.select(bgt.SelectorType(selector))
+            band = fp32c_band(step, 1, selector)
+            bound = tie_bound(n, 1, selector, abs(ref.sum() - 1.0) + 2.0 ** -40, updates=step)
+            pick = bound if (not dilution and sel.redecided == 1) else bound + band
+            check_selection(orc.reverse_bits(sel.experiment, n), sel.objective, ref, n, 1, selector, pick,
+                            (step, selector), obj_bound=bound + band)
+        e = orc.op_bha(ref, n)[0]
+        r = sampler.draw(e)
+        lat.update(orc.reverse_bits(e, n), r, t)
+        orc.update_in_place(ref, dilution, n, 1, e, r, dil)
+    m = lat.atom_masses()
+    m_ref = accurate_masses(ref)
+    assert np.max(np.abs(m - m_ref) / m_ref) <= 1e-6
+    lat.close()
+
+
+def test_three_pass_fused_rounds_n28(form):
+    # the benchmark's round under the three-pass form, along the oracle's trajectory
+    form(1)
+    lat, ref, upset, _ = _oracle_run(28, False, 2, 3, FP32C, 1e-6, 1e-6)
+    lat.close()
+
+
+@pytest.mark.parametrize("n", [27, 29])
+def test_forms_agree(form, n):
+    # the same lattice under both forms: the same exactly re-decided picks, masses and
+    # objectives within the FP32C band of each other, the same restricted (masked) pick
+    pi = seeded_prior(800 + n, n, 0.005, 0.05)
+    out = {}
+    for f in (0, 1):
+        form(f)
+        lat = bgt.Lattice(ND, n, pi, precision=FP32C)
+        picks = []
+        masses = np.empty(n)
+        for e, r in [(0b1011, 1), (0b110000, 0), (1 << (n - 1), 1)]:
+            sel = lat.round(e, r, 0.01, 0.01, None, masses)
+            picks.append((sel.experiment, sel.objective, sel.redecided, masses.copy()))
+        lat.restrict_candidates((1 << (n // 2)) - 1)
+        ms = lat.select(bgt.SelectorType.op_bha)
+        picks.append((ms.experiment, ms.objective, ms.redecided, None))
+        out[f] = picks
+        lat.close()
+    band = fp32c_band(3, 1, 1)
+    for a, b in zip(out[0], out[1]):
+        if a[2] == 1 and b[2] == 1:
+            assert a[0] == b[0]
+        assert abs(a[1] - b[1]) <= band
+        if a[3] is not None:
+            assert np.max(np.abs(a[3] - b[3]) / a[3]) <= 2e-6
+
+
+def test_tuner_measures_caches_and_applies(tmp_path, monkeypatch, form):
+    monkeypatch.setenv("BGT_HIP_TUNE_DIR", str(tmp_path))
+    L = nat.lib()
+    f = ctypes.c_int(-1)
+    two, three = ctypes.c_double(), ctypes.c_double()
+    sid = ctypes.create_string_buffer(256)
+    assert L.bgt_hip_tune(0, 28, 1, nat.FP32C, 1, 1, ctypes.byref(f), ctypes.byref(two), ctypes.byref(three),
+                          sid, 256) == nat.OK, L.bgt_hip_last_error()
+    assert f.value in (0, 1)
+    assert two.value > 0 and three.value > 0
+    assert f.value == (1 if three.value < two.value else 0)
+    name = sid.value.decode()
+    assert name.startswith("kernel-") and "-n28-v1-s32-d0-q1-r2-" in name and name.endswith("gfx950")
+    path = os.path.join(str(tmp_path), name + ".bgthip")
+    assert os.path.exists(path)
+    with open(path) as fh:
+        assert f"pass2_form={f.value}" in fh.read()
+    # a second call is served from the cache without measuring
+    g = ctypes.c_int(-1)
+    assert L.bgt_hip_tune(0, 28, 1, nat.FP32C, 1, 0, ctypes.byref(g), ctypes.byref(two), ctypes.byref(three),
+                          None, 0) == nat.OK
+    assert g.value == f.value and two.value == 0.0 and three.value == 0.0
+    # FP64 and small shapes have one form only
+    assert L.bgt_hip_tune(0, 28, 1, nat.FP64, 1, 0, ctypes.byref(g), None, None, None, 0) == nat.OK
+    assert g.value == 0
