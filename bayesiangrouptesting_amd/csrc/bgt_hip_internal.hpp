@@ -7,6 +7,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <functional>
 #include <stdexcept>
 #include <string>
 
@@ -186,9 +187,11 @@ void launch_zeta_pass1(const Descs &descs, int count, int n, Storage st, bool fu
 // fold > 0 (persistent pass 2 after a fused pass 1 with defer_sum): every workgroup sums
 // the `fold` per-tile partials itself, as the separate finalize launch would, and the first
 // stores the denominator, its inverse and the total
+// after_transform (optional) runs on the host between the transform kernel(s) and the
+// merge of their top-2 lists (a timing hook).
 unsigned launch_zeta_final(const Descs &descs, int count, int n, int k, Storage st, int objective,
 						   bool from_posterior, bool scaled, double target, hipStream_t stream,
-						   unsigned fold = 0);
+						   unsigned fold = 0, const std::function<void()> &after_transform = {});
 // n <= kSmallBits: whole transform in one workgroup per lattice
 unsigned launch_zeta_small(const Descs &descs, int count, int n, int k, Storage st, int objective,
 					   double target, hipStream_t stream);

@@ -21,6 +21,7 @@
 #include "bgt_hip_tune.hpp"
 
 #include <cmath>
+#include <functional>
 #include <cstdlib>
 #include <string>
 
@@ -2635,12 +2636,15 @@ unsigned launch_final_kernel(const Descs &descs, int count, int n, Storage st, i
 }
 
 unsigned launch_zeta_final(const Descs &descs, int count, int n, int, Storage st, int objective,
-						   bool from_posterior, bool scaled, double target, hipStream_t stream, unsigned fold)
+						   bool from_posterior, bool scaled, double target, hipStream_t stream, unsigned fold,
+						   const std::function<void()> &after_transform)
 {
 	if (fold != 0u && !persistent_pass2(n, objective, from_posterior))
 		throw Error(4, "folding the pass-1 partials needs the persistent pass 2");
 	const unsigned groups = launch_final_kernel(descs, count, n, st, objective, from_posterior, scaled, target, 0,
 												stream, fold);
+	if (after_transform)
+		after_transform();
 	const unsigned entries = 2u * groups * kTileThreads;
 	if (persistent_pass2(n, objective, from_posterior)) {
 		// the merge of the top-2 lists and the band compaction in one launch; the raw candidate

@@ -1392,15 +1392,20 @@ bgt_hip_selection DeviceLattice::round(uint32_t experiment, uint32_t response, c
 	Descs ds = single(); // descriptors of the pre-update state (pass 2 applies inv_out)
 	// a persistent pass 2 sums pass 1's partials itself (one launch and its gap fewer)
 	const bool fold = persistent_pass2(atoms_, kObjHalving, false);
-	clock_.begin(stream_, true);
+	// the light clock brackets one transform pass per round, pass 1 and pass 2 in turn
+	const int light_pass = 1 + (int)(rounds_++ & 1u);
+	clock_.begin(stream_, light_pass == 1);
 	launch_zeta_pass1(ds, 1, atoms_, storage_, true, k_, dilution_, stream_, fold);
 	clock_.end(stream_, "round_pass1");
 	commit_update();
 	ds.one.band_width = band_width(kSelBha); // error band of the updated posterior
-	clock_.begin(stream_);
+	clock_.begin(stream_, light_pass == 2);
 	const unsigned entries = launch_zeta_final(ds, 1, atoms_, k_, storage_, kObjHalving, false, true, 0.5, stream_,
-											   fold ? (1u << (atoms_ - kTileBits)) : 0u);
-	clock_.end(stream_, "round_pass2");
+											   fold ? (1u << (atoms_ - kTileBits)) : 0u, [&] {
+												   clock_.end(stream_, "round_pass2");
+												   clock_.begin(stream_);
+											   });
+	clock_.end(stream_, "top2_compact");
 	collect_band(ds, entries, kObjHalving, false, true, 0.5);
 	launch_exact(ds);
 	exact_.prepare(); // on the host while the device runs the round

@@ -165,6 +165,7 @@ private:
 	int n_, k_, atoms_;
 	uint64_t states_;
 	Storage storage_;
+	uint64_t rounds_ = 0; // fused rounds run (the light clock alternates its bracket)
 	hipStream_t stream_ = nullptr;
 	uint64_t bytes_ = 0;
 

@@ -14,10 +14,14 @@ drives the trajectory (its picks and the responses sampled for them); the FP32C 
 the same (experiment, response) sequence, so the two forms' selections are compared round by
 round.
 
-Roofline: the dominant kernel is pass 1 (update fused with bits 0..14 of the transform): it
-reads the posterior and writes it and the transform intermediate, S*(2 b_p + b_z) bytes per
-launch (b = 4 for FP32C, 8 for FP64), timed with HIP events on the lattice's stream over the
-timed rounds. A round moves S*(2 b_p + 2 b_z) bytes (pass 2 reloads the intermediate once).
+Roofline: the dominant kernel of the FP32C round is pass 2 (the transform's bits 15..27 fused
+with the Op-BHA objective): it reads the transform intermediate once, S*b_z bytes per launch
+(b = 4 for FP32C, 8 for FP64). Pass 1 (the update fused with bits 0..14) reads the posterior
+and writes it and the intermediate, S*(2 b_p + b_z). Both are timed with HIP events on the
+lattice's stream over the timed rounds, one bracket per round (pass 1 and pass 2 in turn). A
+round moves S*(2 b_p + 2 b_z) bytes; its share of peak is reported beside the kernel's. Before
+timing, the autotuner measures the transform forms for this KernelSpec (untimed); the line
+names the form it chose.
 
 Also reported, each for both storage forms: the batched tree simulation (n = 16 dilution,
 depth 6) to TreeStats, Op-BBPA rounds of the n = 8, k = 3 dilution lattice, and the serving
@@ -65,7 +69,7 @@ def kernel_times(lat) -> dict:
 
 
 def enable_clock(lat, mode) -> None:
-    """0 off, 1 every kernel class and host phase, 2 light (events around pass 1 only)."""
+    """0 off, 1 every kernel class and host phase, 2 light (events around one transform pass per round)."""
     from bayesiangrouptesting_amd import _native as nat
     nat.check(nat.lib().bgt_hip_lattice_kernel_times(lat.handle, int(mode), None, None, 0, None, 0))
 
@@ -173,6 +177,18 @@ def run_sessions(precision, n, pi, steps, warmup, seed, drive=None, clock=2):
         l.close()
     return {"seconds": timer.total, "kernels": kt, "plan": plan, "picks": picks, "lat_ms": lat_ms,
             "classified": classified, "sessions": sess + 1, "lattice": lat}
+
+
+def tune_forms(n) -> dict:
+    """The autotuner's choice of the FP32C transform form at this size (measured once, untimed)."""
+    from bayesiangrouptesting_amd import _native as nat
+    f = ctypes.c_int(-1)
+    two, three = ctypes.c_double(), ctypes.c_double()
+    sid = ctypes.create_string_buffer(256)
+    nat.check(nat.lib().bgt_hip_tune(0, n, 1, nat.FP32C, 1, 1, ctypes.byref(f), ctypes.byref(two),
+                                     ctypes.byref(three), sid, 256))
+    return {"spec": sid.value.decode(), "form": "three-pass" if f.value == 1 else "two-pass",
+            "select_ms_two_pass": two.value, "select_ms_three_pass": three.value}
 
 
 def cpu_baseline(n, pi, steps, all_cores: bool = False):
@@ -394,7 +410,8 @@ def main() -> int:
     S = 1 << n
     W, K = args.warmup, args.steps
 
-    log(f"n={n} states={S} steps={K} warmup={W}")
+    tuned = tune_forms(n)
+    log(f"n={n} states={S} steps={K} warmup={W}; transform form {tuned}")
     r64 = run_sessions(bgt.Precision.float64, n, pi, K, W, c.seed)
     post64 = r64["lattice"].posterior()
     m64 = r64["lattice"].atom_masses()
@@ -425,12 +442,16 @@ def main() -> int:
 
     b32, b64 = 4, 8
     kt32, kt64 = d32["kernels"], d64["kernels"]
-    p1_32, p2_32 = r32["kernels"]["round_pass1"][0], kt32["round_pass2"][0]
-    p1_64, p2_64 = r64["kernels"]["round_pass1"][0], kt64["round_pass2"][0]
+    # both passes timed in the timed rounds (one bracket per round, in turn)
+    p1_32, p2_32 = r32["kernels"]["round_pass1"][0], r32["kernels"]["round_pass2"][0]
+    p1_64, p2_64 = r64["kernels"]["round_pass1"][0], r64["kernels"]["round_pass2"][0]
     alg1 = S * (2 * b32 + b32)
+    alg2 = S * b32
     round_alg32 = S * (2 * b32 + 2 * b32)
     round_alg64 = S * (2 * b64 + 2 * b64)
-    achieved = alg1 / (p1_32 * 1e-3) / 1e9
+    # the dominant kernel: the longer of the two passes
+    dom2 = p2_32 >= p1_32
+    achieved = (alg2 / (p2_32 * 1e-3) if dom2 else alg1 / (p1_32 * 1e-3)) / 1e9
     traffic = load_traffic()
     dt32, dt64 = r32["seconds"], r64["seconds"]
     value = K / dt32
@@ -462,19 +483,25 @@ def main() -> int:
             "peak": PEAK_HBM_GBS,
             "unit": "GB/s",
             "frac": achieved / PEAK_HBM_GBS,
-            "traffic": traffic.get("pass1_bytes") if traffic else None,
-            "kernel": "k_zeta_pass1<float,float,true,1> (update fused with bits 0..14 of the transform)",
-            "alg_bytes_per_launch": alg1,
+            "traffic": (traffic.get("pass2_bytes") if dom2 else traffic.get("pass1_bytes")) if traffic else None,
+            "kernel": ("k_zeta_pass2<float,3,true> (bits 15..27 of the transform fused with the Op-BHA objective, "
+                       "the top-2 and the error band; reads the intermediate once)") if dom2 else
+                      "k_zeta_pass1<float,float,true,1> (update fused with bits 0..14 of the transform)",
+            "alg_bytes_per_launch": alg2 if dom2 else alg1,
             "pass1_ms": p1_32,
+            "pass1_GBs": alg1 / (p1_32 * 1e-3) / 1e9,
+            "pass1_frac": alg1 / (p1_32 * 1e-3) / 1e9 / PEAK_HBM_GBS,
             "pass2_ms": p2_32,
-            "pass2_GBs": S * b32 / (p2_32 * 1e-3) / 1e9,
+            "pass2_GBs": alg2 / (p2_32 * 1e-3) / 1e9,
+            "pass2_frac": alg2 / (p2_32 * 1e-3) / 1e9 / PEAK_HBM_GBS,
             "round_kernels_GBs": round_alg32 / ((p1_32 + p2_32) * 1e-3) / 1e9,
             "round_GBs": round_alg32 * value / 1e9,
             "round_frac": round_alg32 * value / 1e9 / PEAK_HBM_GBS,
+            "transform_form": tuned,
             "traffic_detail": traffic,
-            "timing_note": "pass1_ms: HIP events around pass 1 in the timed rounds (nothing else bracketed); "
-                           "pass2_ms and the kernel/host breakdowns: a replay of the same rounds with every "
-                           "phase bracketed",
+            "timing_note": "pass1_ms / pass2_ms: HIP events around one pass per timed round (pass 1 and pass 2 "
+                           "in turn, nothing else bracketed); the kernel/host breakdowns: a replay of the same "
+                           "rounds with every phase bracketed",
         },
         "latency_ms": {"p50": float(np.percentile(lm32, 50)), "p99": float(np.percentile(lm32, 99)),
                        "note": "per fused round, host call to result (not the serving loop)"},

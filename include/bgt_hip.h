@@ -234,8 +234,8 @@ int bgt_hip_lattice_error_bound(bgt_hip_lattice *lattice, double *posterior_rel,
 /* Average device time (ms) per launch of each kernel class since the last reset, measured
  * with HIP events on the lattice's stream. names is a ';'-separated list. enable >= 0 resets
  * and switches the clock: 0 off, 1 every kernel class and host phase, 2 light (events only
- * around pass 1, the rest of the round unbracketed: each event record costs the stream a
- * few microseconds); enable < 0 reads. */
+ * around one transform pass per fused round, pass 1 and pass 2 in turn, the rest unbracketed:
+ * each event record costs the stream a few microseconds); enable < 0 reads. */
 int bgt_hip_lattice_kernel_times(bgt_hip_lattice *lattice, int enable, double *ms_out,
 								 int *launches_out, int capacity, char *names, int names_len);
 
