@@ -282,7 +282,8 @@ class Lattice:
 
     def round(self, experiment: int, response: int, threshold_up: float, threshold_lo: float,
               dilution: Optional[DilutionTable] = None, masses: Optional[np.ndarray] = None) -> Selection:
-        """Fused update + classification + next Op-BHA selection (binary lattices)."""
+        """Fused update + classification + next selection (Op-BHA for binary lattices, Op-BBPA
+        for multinomial ones)."""
         s = nat.SelectionC()
         ptr = dilution.pointer() if dilution is not None else None
         mp = masses.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if masses is not None else None

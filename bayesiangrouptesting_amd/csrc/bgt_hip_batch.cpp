@@ -207,6 +207,12 @@ void batch_step(std::vector<BatchItem> &items, int selector, double up, double l
 		const int objective = sel == kSelBha ? kObjHalving : (k == 1 ? kObjBinaryBbpa : kObjGather);
 		const double target = sel == kSelBha ? 0.5 : 1.0 / double(1 << k);
 		const int atoms = lat.atoms();
+		if (objective == kObjGather && bbpa_hist_path(lat.subjects(), k)) {
+			// as DeviceLattice::select: the feature tables from one histogram pass
+			launch_bbpa_hist(dg, cnt, lat.subjects(), k, lat.storage(), false, s);
+			launch_bbpa_score(dg, cnt, lat.subjects(), k, target, s);
+			continue;
+		}
 		unsigned entries = 0;
 		if (atoms <= kSmallBits) {
 			entries = launch_zeta_small(dg, cnt, atoms, k, lat.storage(), objective, target, s);

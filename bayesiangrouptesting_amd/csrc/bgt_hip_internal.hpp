@@ -134,6 +134,9 @@ struct LatDesc {
 	ExactRec hist_new;
 	const double *pairs;
 	double *exact_u;
+	// Op-BBPA histogram tables (k >= 2): fixed-point bins, three 64-bit limbs each, zero
+	// between uses
+	unsigned long long *fixtab;
 };
 
 // Device exact rescoring covers the first kExactBand band candidates, histories of at
@@ -222,6 +225,14 @@ void launch_band_collect(const Descs &descs, int count, unsigned entries, int n,
 void launch_gather_heads(const LatDesc *many, int count, unsigned char *out, hipStream_t stream);
 void launch_bbpa_score(const Descs &descs, int count, int n, int k, double target,
 					   hipStream_t stream);
+// Op-BBPA feature tables of k >= 2 lattices in one pass over the posterior (bbpa.cpp:237-313)
+// where each lane can hold its entries of every table (n >= 6, (2^k - 1) 2^(n-6) <= 64, more
+// than 14 atoms): d.ftab, the total and the atom masses, ready for launch_bbpa_score. fused:
+// apply the pending update (d.e, the likelihood factors) in the same pass and store the new
+// denominator (slot inv_out).
+bool bbpa_hist_path(int n, int k);
+size_t bbpa_fix_bytes(int n, int k);
+void launch_bbpa_hist(const Descs &descs, int count, int n, int k, Storage st, bool fused, hipStream_t stream);
 // Unnormalised exact upset masses U(E) of the first kExactBand band candidates of each
 // lattice (d.exact_u[i], -1 where the device program does not apply), from d.pairs and
 // d.hist: the quantity ExactHistory computes on the host, as a dense dynamic program.

@@ -51,6 +51,18 @@ __device__ __forceinline__ double uniform(double x)
 	return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+// Lane l's value, for a uniform l (no memory access).
+__device__ __forceinline__ uint32_t lane_u(uint32_t v, int l)
+{
+	return (uint32_t)__builtin_amdgcn_readlane((int)v, l);
+}
+__device__ __forceinline__ double lane_d(double v, int l)
+{
+	const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+	const uint32_t lo = lane_u((uint32_t)b, l), hi = lane_u((uint32_t)(b >> 32), l);
+	return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // Lattice buffers arrive through a descriptor struct, so the compiler cannot infer their
 // address space; cast to global so loads and stores are global_* rather than flat_*.
 template <typename T>
@@ -1510,6 +1522,191 @@ __global__ __launch_bounds__(1024) void k_zeta_small(Descs descs, int atoms, int
 	}
 }
 
+// ----------------------------------------- Op-BBPA feature tables in one pass (k >= 2)
+//
+// The reference builds, for every non-empty variant subset B, the table
+// H_B[I] = sum of x_S over the states whose variant blocks in B intersect in I
+// (bbpa.cpp:237-285), and zetas each table over the n subject bits (bbpa.cpp:292-313), giving
+// F_B(E) = sum of x_S over the states with S_v >= E for every v in B.
+//
+// Here the zeta comes first, per row: a row is the 2^n states of fixed blocks S_1..S_{k-1},
+// and its own zeta over the subject bits, Z(E) = sum over S_0 >= E of the row, gives every
+// table's contribution at once: F_B(E) += Z(E) when B holds variant 0 and E lies inside the
+// AND c of B's other blocks (all subjects for B = {0}), and F_B(E) += Z(0) when B does not
+// hold variant 0 and E lies inside the AND of its blocks. Lane l, register r of a wave hold
+// E = l + 64 r (n >= 6), so each lane accumulates its own entries of every table over the
+// rows it visits, with no exchange; the row's zeta runs over the lane bits by butterflies and
+// over the register bits in registers, bits in increasing order.
+//
+// The workgroup's waves then add their tables in wave order, and workgroups add theirs to
+// the device table in fixed point, 96 fractional bits as three 32-bit digits in 64-bit
+// integer limbs: integer additions commute, so the tables do not depend on the order in
+// which workgroups finish (the same bits alone or batched, run after run).
+constexpr int kHistThreads = 256;
+constexpr int kHistWaves = kHistThreads / 64;
+
+__device__ __forceinline__ void fix_add(unsigned long long *bin3, double x)
+{
+	// x in [0, 2): the digits of floor(x 2^96) (each step exact: t - floor(t) is the
+	// fraction of a double, times 2^32 is a power-of-two scaling)
+	const double t0 = x * 4294967296.0;
+	const double f0 = floor(t0);
+	const double t1 = (t0 - f0) * 4294967296.0;
+	const double f1 = floor(t1);
+	const double f2 = floor((t1 - f1) * 4294967296.0);
+	if (f0 != 0.0)
+		atomicAdd(&bin3[0], (unsigned long long)f0);
+	if (f1 != 0.0)
+		atomicAdd(&bin3[1], (unsigned long long)f1);
+	if (f2 != 0.0)
+		atomicAdd(&bin3[2], (unsigned long long)f2);
+}
+
+// value of a bin's three limbs: L0 2^-32 + L1 2^-64 + L2 2^-96 after the carries
+__device__ __forceinline__ double fix_value(const unsigned long long *bin3)
+{
+	unsigned long long l0 = bin3[0], l1 = bin3[1], l2 = bin3[2];
+	l1 += l2 >> 32;
+	l2 &= 0xFFFFFFFFull;
+	l0 += l1 >> 32;
+	l1 &= 0xFFFFFFFFull;
+	return ((double)l0 * 0x1p-32 + (double)l1 * 0x1p-64) + (double)l2 * 0x1p-96;
+}
+
+// FUSED: apply the update first (y' = x L, stored in place) and tabulate y'; otherwise
+// x = y * inv. K variants, NH = n - 6 (2^NH registers per lane). The device table d.fixtab
+// holds (2^K - 1) tables of 2^n bins (table B at (B - 1) 2^n) and the total, three limbs each.
+template <typename TP, bool FUSED, int K, int NH>
+__global__ __launch_bounds__(kHistThreads) void k_bbpa_hist(Descs descs)
+{
+	constexpr int R = 1 << NH;
+	constexpr int T = (1 << K) - 1;
+	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
+	extern __shared__ double s_acc[]; // T * 2^n + 1 doubles: the workgroup's tables
+	__shared__ double s_lik[kLikCapacity];
+	const int n = NH + 6;
+	const uint32_t ec = 1u << n, mask = ec - 1u;
+	if (FUSED)
+		load_lik_table(descs, d, s_lik);
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const unsigned gwave = blockIdx.x * kHistWaves + wave;
+	const unsigned nwaves = gridDim.x * kHistWaves;
+	const uint64_t rows = 1ull << (n * (K - 1));
+	TP *post = static_cast<TP *>(d.post);
+	const double inv = d.scal[d.inv_in];
+	double f[T][R];
+#pragma unroll
+	for (int b = 0; b < T; ++b)
+#pragma unroll
+		for (int r = 0; r < R; ++r)
+			f[b][r] = 0.0;
+	double total = 0.0; // lane 0's row totals
+	for (uint64_t row = gwave; row < rows; row += nwaves) {
+		const uint64_t base = row << n;
+		double v[R];
+#pragma unroll
+		for (int r = 0; r < R; ++r) {
+			const uint64_t st = base + (uint64_t)lane + 64u * r;
+			double x = static_cast<double>(post[st]) * inv;
+			if (FUSED) {
+				const TP y = static_cast<TP>(x * likelihood((uint32_t)st, d.e, n, K, d.len, d.dilution, s_lik));
+				post[st] = y;
+				x = static_cast<double>(y);
+			}
+			v[r] = x;
+		}
+		// the row's zeta over S_0: lane bits 0..5, then register bits 6..n-1
+#pragma unroll
+		for (int lb = 0; lb < 6; ++lb) {
+			const bool low = !((lane >> lb) & 1);
+#pragma unroll
+			for (int r = 0; r < R; ++r) {
+				const double o = __shfl_xor(v[r], 1 << lb);
+				v[r] = low ? v[r] + o : v[r];
+			}
+		}
+#pragma unroll
+		for (int hb = 0; hb < NH; ++hb)
+#pragma unroll
+			for (int r = 0; r < R; ++r)
+				if (!(r & (1 << hb)))
+					v[r] += v[r | (1 << hb)];
+		const double z0 = lane_d(v[0], 0); // the row's total, Z(0)
+		total += z0;
+		uint32_t blk[K];
+		blk[0] = mask;
+#pragma unroll
+		for (int q = 1; q < K; ++q)
+			blk[q] = (uint32_t)(row >> ((q - 1) * n)) & mask;
+#pragma unroll
+		for (int b = 1; b <= T; ++b) {
+			uint32_t c = mask;
+#pragma unroll
+			for (int q = 1; q < K; ++q)
+				if ((b >> q) & 1)
+					c &= blk[q];
+#pragma unroll
+			for (int r = 0; r < R; ++r) {
+				const uint32_t e = (uint32_t)lane + 64u * r;
+				const double add = (b & 1) ? v[r] : z0;
+				f[b - 1][r] += ((e & ~c) == 0u) ? add : 0.0;
+			}
+		}
+	}
+	// the workgroup's waves in wave order, then the device table in fixed point
+	const unsigned cells = (unsigned)T * ec;
+	for (int w = 0; w < kHistWaves; ++w) {
+		if (wave == w) {
+#pragma unroll
+			for (int b = 0; b < T; ++b)
+#pragma unroll
+				for (int r = 0; r < R; ++r) {
+					const unsigned i = (unsigned)b * ec + (unsigned)lane + 64u * r;
+					s_acc[i] = (w == 0) ? f[b][r] : s_acc[i] + f[b][r];
+				}
+			if (lane == 0)
+				s_acc[cells] = (w == 0) ? total : s_acc[cells] + total;
+		}
+		__syncthreads();
+	}
+	unsigned long long *g = d.fixtab;
+	for (unsigned i = threadIdx.x; i <= cells; i += blockDim.x)
+		fix_add(&g[3u * i], s_acc[i]);
+}
+
+// The tables in FP64 into d.ftab (table B at B * 2^n), the total into the scalars, the atom
+// masses (F_{v}({i})) into d.masses; the fixed-point table is cleared for the next use.
+// FUSED: the tables were of the updated, unnormalised posterior: their total is the
+// denominator D, stored with 1/D in the inv_out slot, and the tables are scaled by 1/D.
+__global__ __launch_bounds__(1024) void k_bbpa_tables(Descs descs, int fused)
+{
+	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
+	const int n = d.n, k = d.k;
+	const uint32_t ec = 1u << n;
+	const unsigned cells = ((1u << k) - 1u) * ec;
+	unsigned long long *g = d.fixtab;
+	const double t = fix_value(&g[3u * cells]);
+	const double scale = fused ? 1.0 / t : 1.0;
+	for (unsigned i = threadIdx.x; i < cells; i += blockDim.x)
+		d.ftab[(size_t)ec + i] = fix_value(&g[3u * i]) * scale;
+	for (unsigned i = threadIdx.x; i < (unsigned)(k * n); i += blockDim.x) {
+		const unsigned v = i / n, a = i % n;
+		d.masses[i] = fix_value(&g[3u * (((1u << v) - 1u) * ec + (1u << a))]) * scale;
+	}
+	if (threadIdx.x == 0) {
+		if (fused) {
+			d.scal[kScalDen] = t;
+			d.scal[d.inv_out] = scale;
+			d.scal[kScalTotal] = t * scale;
+		} else {
+			d.scal[kScalTotal] = t;
+		}
+	}
+	__syncthreads();
+	for (unsigned i = threadIdx.x; i < 3u * (cells + 1u); i += blockDim.x)
+		g[i] = 0ull;
+}
+
 // BBPA k >= 2 scoring from the feature tables (bbpa.cpp:332-365 for k = 2 and 416-460
 // otherwise); same operation order per experiment.
 __global__ __launch_bounds__(256) void k_bbpa_score(Descs descs, int ns, int k, double target)
@@ -2175,18 +2372,6 @@ __global__ __launch_bounds__(256) void k_top2_compact(Descs descs, unsigned coun
 // wavefront. Every term is non-negative and every sum runs in a fixed order. Candidates
 // with more than kExactMaxPools distinct pools, or histories longer than kExactMaxHist,
 // get -1; candidates dominated by another band entry get -2 (see below).
-// Lane l's value, for a uniform l (no memory access).
-__device__ __forceinline__ uint32_t lane_u(uint32_t v, int l)
-{
-	return (uint32_t)__builtin_amdgcn_readlane((int)v, l);
-}
-__device__ __forceinline__ double lane_d(double v, int l)
-{
-	const unsigned long long b = (unsigned long long)__double_as_longlong(v);
-	const uint32_t lo = lane_u((uint32_t)b, l), hi = lane_u((uint32_t)(b >> 32), l);
-	return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
-
 // The dense pool-state program of k_exact_band with the 2^c states in registers: state
 // k = lane + 64 r, RB = max(0, c - 6) register bits (R = 2^RB doubles per lane). A sweep
 // over a lane bit exchanges partners with a lane shuffle, one over a register bit pairs
@@ -2713,6 +2898,57 @@ void launch_bbpa_score(const Descs &descs, int count, int n, int k, double targe
 	launch_checked("k_bbpa_score", [&] {
 		hipLaunchKernelGGL(k_bbpa_score, grid, dim3(256), 0, stream, descs, n, k, target);
 		hipLaunchKernelGGL(k_finalize_top2, dim3(1, count), dim3(1024), 0, stream, descs, grid.x);
+	});
+}
+
+bool bbpa_hist_path(int n, int k)
+{
+	// per lane (2^k - 1) 2^(n-6) FP64 accumulators: at most 64 (128 registers)
+	if (k < 2 || k > kMaxVariants || n < 6 || n * k <= kSmallBits)
+		return false;
+	return ((1u << k) - 1u) << (n - 6) <= 64u;
+}
+
+size_t bbpa_fix_bytes(int n, int k)
+{
+	return bbpa_hist_path(n, k) ? sizeof(unsigned long long) * 3u * (((1u << k) - 1u) * (1u << n) + 1u) : 0;
+}
+
+void launch_bbpa_hist(const Descs &descs, int count, int n, int k, Storage st, bool fused, hipStream_t stream)
+{
+	if (!bbpa_hist_path(n, k))
+		throw Error(4, "one-pass feature tables need k >= 2, n >= 6 and (2^k - 1) 2^(n-6) <= 64");
+	const unsigned cells = ((1u << k) - 1u) * (1u << n);
+	const size_t lds = sizeof(double) * (cells + 1u);
+	const uint64_t rows = 1ull << (n * k - n);
+	const unsigned groups =
+		(unsigned)std::max<uint64_t>(1, std::min<uint64_t>((rows + kHistWaves - 1) / kHistWaves, 2ull * compute_units()));
+	const dim3 grid(groups, count);
+	launch_checked("k_bbpa_hist", [&] {
+#define BGT_HIST(KV, NHV)                                                                                       \
+	if (k == KV && n - 6 == NHV) {                                                                              \
+		if (st == Storage::fp64) {                                                                              \
+			if (fused)                                                                                          \
+				hipLaunchKernelGGL((k_bbpa_hist<double, true, KV, NHV>), grid, dim3(kHistThreads), lds, stream, descs); \
+			else                                                                                                \
+				hipLaunchKernelGGL((k_bbpa_hist<double, false, KV, NHV>), grid, dim3(kHistThreads), lds, stream, descs); \
+		} else {                                                                                                \
+			if (fused)                                                                                          \
+				hipLaunchKernelGGL((k_bbpa_hist<float, true, KV, NHV>), grid, dim3(kHistThreads), lds, stream, descs); \
+			else                                                                                                \
+				hipLaunchKernelGGL((k_bbpa_hist<float, false, KV, NHV>), grid, dim3(kHistThreads), lds, stream, descs); \
+		}                                                                                                       \
+		return;                                                                                                 \
+	}
+		BGT_HIST(2, 0) BGT_HIST(2, 1) BGT_HIST(2, 2) BGT_HIST(2, 3) BGT_HIST(2, 4)
+		BGT_HIST(3, 0) BGT_HIST(3, 1) BGT_HIST(3, 2) BGT_HIST(3, 3)
+		BGT_HIST(4, 0) BGT_HIST(4, 1) BGT_HIST(4, 2)
+		BGT_HIST(5, 0) BGT_HIST(5, 1)
+#undef BGT_HIST
+		throw Error(4, "no one-pass table kernel for this shape");
+	});
+	launch_checked("k_bbpa_tables", [&] {
+		hipLaunchKernelGGL(k_bbpa_tables, dim3(1, count), dim3(1024), 0, stream, descs, fused ? 1 : 0);
 	});
 }
 

@@ -224,6 +224,10 @@ void DeviceLattice::init_prior(const double *prior)
 	const double init[kScalars] = {1.0, 1.0, 1.0, 1.0, 0.0, 0.0, 0.0, 0.0};
 	BGT_HIP_CHECK(hipMemcpyAsync(scal_, init, sizeof(init), hipMemcpyHostToDevice, stream_));
 	BGT_HIP_CHECK(hipMemcpyAsync(pairs_, pairs.data(), sizeof(double) * pairs.size(), hipMemcpyHostToDevice, stream_));
+	// the histogram bins start at zero (each use clears them after reading; a lattice taken
+	// back from the cache after a failed call may not have)
+	if (fixtab_)
+		BGT_HIP_CHECK(hipMemsetAsync(fixtab_, 0, fix_bytes_, stream_));
 	launch_prior(single(), 1, atoms_, storage_, pairs_, states_, stream_);
 	sync();
 }
@@ -397,7 +401,8 @@ DeviceLattice::DeviceLattice(const DeviceLattice &parent, bool copy_posterior)
 namespace {
 
 struct ArenaLayout {
-	size_t scal, res, cnt, mass, band, cand, flags, hist, exact_u, pairs, part, tops, ftab, scores, sel, total;
+	size_t scal, res, cnt, mass, band, cand, flags, hist, exact_u, pairs, part, tops, ftab, fix, fix_bytes, scores, sel,
+		total;
 	bool exact;
 };
 
@@ -434,6 +439,8 @@ ArenaLayout arena_layout(bool dilution, int n, int k)
 	a.part = take(sizeof(double) * part);
 	a.tops = take(sizeof(SelResult) * tops);
 	a.ftab = take(sizeof(double) * ftab);
+	a.fix_bytes = bbpa_fix_bytes(n, k);
+	a.fix = take(a.fix_bytes);
 	a.scores = take(sizeof(double) * scores);
 	a.sel = take(sizeof(uint32_t) * 16);
 	a.total = std::max<size_t>(off, a.exact ? kHeadBytesWide : kHeadBytes);
@@ -446,7 +453,8 @@ uint64_t DeviceLattice::footprint(bool dilution, int subjects, int variants, Sto
 {
 	const int atoms = subjects * variants;
 	const size_t eb = storage == Storage::fp64 ? 8 : 4;
-	return (uint64_t)eb * (1ull << atoms) + z_bytes(atoms, eb) + arena_layout(dilution, subjects, variants).total;
+	const uint64_t zb = bbpa_hist_path(subjects, variants) ? 0 : z_bytes(atoms, eb);
+	return (uint64_t)eb * (1ull << atoms) + zb + arena_layout(dilution, subjects, variants).total;
 }
 
 void DeviceLattice::allocate()
@@ -468,6 +476,10 @@ void DeviceLattice::allocate()
 	part_ = reinterpret_cast<double *>(arena_ + a.part);
 	tops_ = reinterpret_cast<SelResult *>(arena_ + a.tops);
 	ftab_ = reinterpret_cast<double *>(arena_ + a.ftab);
+	if (a.fix_bytes) {
+		fixtab_ = reinterpret_cast<unsigned long long *>(arena_ + a.fix);
+		fix_bytes_ = a.fix_bytes;
+	}
 	scores_ = reinterpret_cast<double *>(arena_ + a.scores);
 	selected_ = reinterpret_cast<uint32_t *>(arena_ + a.sel);
 	if (a.exact) {
@@ -487,7 +499,8 @@ void DeviceLattice::allocate()
 
 void DeviceLattice::ensure_z()
 {
-	if (z_ || atoms_ <= kTileBits)
+	// lattices whose Op-BBPA tables come from the histogram pass never transform
+	if (z_ || atoms_ <= kTileBits || bbpa_hist_path(n_, k_))
 		return;
 	const size_t eb = storage_ == Storage::fp64 ? 8 : 4;
 	const uint64_t zb = z_bytes(atoms_, eb);
@@ -546,6 +559,7 @@ LatDesc DeviceLattice::desc() const
 	d.lik = nullptr;
 	d.masses = masses_;
 	d.ftab = ftab_;
+	d.fixtab = fixtab_;
 	d.n = n_;
 	d.k = k_;
 	d.atoms = atoms_;
@@ -1165,6 +1179,17 @@ bgt_hip_selection DeviceLattice::select(int selector)
 	}
 	const int objective = sel == kSelBha ? kObjHalving : (k_ == 1 ? kObjBinaryBbpa : kObjGather);
 	const double target = sel == kSelBha ? 0.5 : 1.0 / double(1 << k_);
+	if (objective == kObjGather && bbpa_hist_path(n_, k_)) {
+		// the feature tables from one histogram pass (bbpa.cpp:237-313), no transform
+		clock_.begin(stream_, true);
+		launch_bbpa_hist(ds, 1, n_, k_, storage_, false, stream_);
+		clock_.end(stream_, "bbpa_hist");
+		clock_.begin(stream_);
+		launch_bbpa_score(ds, 1, n_, k_, target, stream_);
+		clock_.end(stream_, "bbpa_score");
+		fetch_head();
+		return finish_selection(selector, static_cast<HostBox *>(host_box_)->result, false);
+	}
 	ensure_z();
 	Descs dz = single();
 	dz.one.band_width = band_width(sel);
@@ -1378,8 +1403,30 @@ std::vector<uint32_t> DeviceLattice::select_experiments(int steps)
 bgt_hip_selection DeviceLattice::round(uint32_t experiment, uint32_t response, const double *dil, double up,
 									   double lo, double *masses_out)
 {
-	if (k_ != 1)
-		throw Error(BGT_HIP_BAD_INPUT, "the fused round is defined for binary lattices");
+	if (k_ != 1) {
+		// multinomial lattices: Op-BBPA (auto_select's choice, simulation_helpers.cpp:40-45);
+		// with tables on chip, the update and the histogram of the updated posterior are one
+		// pass, and the atom masses come with the single-variant tables
+		if (!bbpa_hist_path(n_, k_)) {
+			update(experiment, response, dil);
+			classify(up, lo);
+			if (masses_out)
+				fetch_masses(masses_out);
+			return select(kSelBbpa);
+		}
+		prepare_update(experiment, response, dil);
+		const Descs ds = single(); // the new experiment's factors, the pre-update 1/D
+		clock_.begin(stream_, true);
+		launch_bbpa_hist(ds, 1, n_, k_, storage_, true, stream_);
+		clock_.end(stream_, "round_hist");
+		commit_update();
+		clock_.begin(stream_);
+		launch_bbpa_score(ds, 1, n_, k_, 1.0 / double(1 << k_), stream_);
+		clock_.end(stream_, "bbpa_score");
+		fetch_head();
+		classify_from_head(up, lo, masses_out);
+		return finish_selection(kSelBbpa, static_cast<HostBox *>(host_box_)->result, true);
+	}
 	if (atoms_ <= kTileBits) {
 		update(experiment, response, dil);
 		classify(up, lo);

@@ -8,7 +8,8 @@
         one device-to-device copy of a known size, the calibration point for FETCH_SIZE /
         WRITE_SIZE (bytes the counters report per byte actually moved)
     python -m bayesiangrouptesting_amd.profile_workloads bbpa24 [fp32c|fp64] [rounds]
-        n = 8, k = 3 dilution lattice (2^24 states): Op-BBPA select + update rounds
+        n = 8, k = 3 dilution lattice (2^24 states): fused rounds (update, classification and
+        Op-BBPA select from one histogram pass)
     python -m bayesiangrouptesting_amd.profile_workloads lookahead28 [fp32c|fp64] [calls]
         n = 28 binary lattice (the benchmark's priors) after 4 seeded updates: 5-step
         select_experiments calls
@@ -67,10 +68,10 @@ def bbpa24(form: str = "fp32c", rounds: int = 4) -> None:
     lat = bgt.Lattice(bgt.LatticeType.replicated_dilution, n, pi, variants=k, precision=prec)
     truth = workload.true_state(c.seed, pi)
     sampler = workload.ResponseSampler(c.seed, n, k, truth, list(table.values))
+    e = lat.select(bgt.SelectorType.op_bbpa).experiment
     t0 = time.perf_counter()
     for _ in range(rounds):
-        e = lat.select(bgt.SelectorType.op_bbpa).experiment
-        lat.update(e, sampler.draw(e), table)
+        e = lat.round(e, sampler.draw(e), 0.01, 0.01, table).experiment
     nat.check(nat.lib().bgt_hip_device_synchronize())
     print(f"bbpa24 {form}: {rounds} rounds, {(time.perf_counter() - t0) / rounds * 1e3:.3f} ms/round", flush=True)
     lat.close()

@@ -316,39 +316,49 @@ def shrink_workload(repeats: int = 10) -> dict:
     return out
 
 
-def bbpa_workload(rounds: int = 8) -> dict:
+def bbpa_workload(rounds: int = 8, repeats: int = 3) -> dict:
     """Multinomial dilution lattice n = 8, k = 3 (2^24 states), DilutionTable(8, 0.05, 0.9),
-    priors U[0.01, 0.10]: `rounds` rounds of Op-BBPA select -> update with seeded responses."""
+    priors U[0.01, 0.10]: `rounds` rounds of Op-BBPA select -> update with responses sampled
+    from a seeded true state. A round is the fused lattice round: the update and the
+    histogram of the updated posterior in one pass (the feature tables on chip), then the
+    tables' zeta and the inclusion-exclusion scoring. Algorithmic bytes per round: the
+    posterior read and written once, 2 S b_p. Best of `repeats` runs of the rounds."""
     import bayesiangrouptesting_amd as bgt
     from bayesiangrouptesting_amd import workload
 
     c = workload.C3
-    pi = workload.priors(c.seed, c.subjects * c.variants, c.prior_lo, c.prior_hi)
-    tab = bgt.DilutionTable(c.subjects, c.alpha, c.h)
+    n, k = c.subjects, c.variants
+    pi = workload.priors(c.seed, n * k, c.prior_lo, c.prior_hi)
+    tab = bgt.DilutionTable(n, c.alpha, c.h)
     out = {"workload": "multinomial dilution n=8 k=3 (2^24 states), DilutionTable(8,0.05,0.9), priors U[0.01,0.10] "
-                       "seed 3; round = Op-BBPA select + update with a seeded response"}
+                       "seed 3; round = update with a response sampled from a seeded true state + classification "
+                       "+ Op-BBPA select (one histogram pass over the posterior)"}
     picks = {}
-    for prec, key in ((bgt.Precision.float64, "fp64"), (bgt.Precision.float32_compensated, "fp32c")):
-        lat = bgt.Lattice(bgt.LatticeType.replicated_dilution, c.subjects, pi, variants=c.variants, precision=prec)
-        lat.select(bgt.SelectorType.op_bbpa)  # allocates the transform intermediate
-        g = workload.SplitMix64(c.seed)
-        sel = []
-        device_sync()
-        t0 = time.perf_counter()
-        for _ in range(rounds):
-            s = lat.select(bgt.SelectorType.op_bbpa)
-            sel.append(s.experiment)
-            lat.update(s.experiment, g.below(1 << c.variants), tab)
-        device_sync()
-        dt = time.perf_counter() - t0
-        lat.close()
+    states = 1 << (n * k)
+    for prec, key, b in ((bgt.Precision.float64, "fp64", 8), (bgt.Precision.float32_compensated, "fp32c", 4)):
+        best = float("inf")
+        for _ in range(repeats):
+            lat = bgt.Lattice(bgt.LatticeType.replicated_dilution, n, pi, variants=k, precision=prec)
+            e = lat.select(bgt.SelectorType.op_bbpa).experiment
+            sampler = workload.ResponseSampler(c.seed, n, k, workload.true_state(c.seed, pi), list(tab.values))
+            sel = []
+            device_sync()
+            t0 = time.perf_counter()
+            for _ in range(rounds):
+                e = lat.round(e, sampler.draw(e), THRESHOLD, THRESHOLD, tab).experiment
+                sel.append(e)
+            device_sync()
+            best = min(best, time.perf_counter() - t0)
+            lat.close()
         picks[key] = sel
-        states = 1 << (c.subjects * c.variants)
-        b = 8 if key == "fp64" else 4
-        out[key] = {"ms_per_round": dt / rounds * 1e3, "round_GBs_alg": 4 * states * b / (dt / rounds) / 1e9}
+        ms = best / rounds * 1e3
+        alg = 2 * states * b
+        out[key] = {"ms_per_round": ms, "alg_bytes_per_round": alg, "round_GBs_alg": alg / (ms * 1e-3) / 1e9,
+                    "round_frac": alg / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS}
     out["identical_selections"] = sum(a == b for a, b in zip(picks["fp64"], picks["fp32c"]))
     out["selections"] = rounds
-    out["note"] = "2^24 states (128 MiB FP64) stay in the 256 MiB Infinity Cache: cache-bound, not HBM-bound"
+    out["note"] = ("2^24 states (128 MiB FP64) stay in the 256 MiB Infinity Cache: cache-bound; reported against "
+                   "HBM peak")
     return out
 
 

@@ -218,8 +218,10 @@ int bgt_hip_lattice_project(bgt_hip_lattice *source, const bgt_hip_shrink_plan *
 int bgt_hip_lattice_select_experiments(bgt_hip_lattice *lattice, int steps, uint32_t *out);
 
 /* One fused round of the adaptive loop: update(experiment, response), atom marginals of the
- * updated posterior and classification, and the next Op-BHA selection, from two HBM passes.
- * Binary lattices only. Masses of the updated posterior go to masses_out (may be NULL). */
+ * updated posterior and classification, and the next selection -- Op-BHA for binary lattices
+ * (two HBM passes), Op-BBPA for multinomial ones (one pass where the feature tables fit on
+ * chip: 6..11 subjects, (2^k - 1) 2^n <= 4095; otherwise update, classify and select in
+ * turn). Masses of the updated posterior go to masses_out (may be NULL). */
 int bgt_hip_lattice_round(bgt_hip_lattice *lattice, uint32_t experiment, uint32_t response,
 						  const double *dilution, double threshold_up, double threshold_lo,
 						  bgt_hip_selection *next, uint32_t *positive_mask,

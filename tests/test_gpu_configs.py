@@ -140,3 +140,43 @@ def test_served_session_follows_the_oracle(served, spec):
     assert got.tests == tests
     assert (flip(got.positive), flip(got.negative)) == (pos, neg)
     assert len(got.experiments) == tests
+
+
+@pytest.mark.parametrize("n,k,dilution", [(8, 3, True), (10, 2, False), (6, 3, False), (6, 4, True)])
+@pytest.mark.parametrize("precision", [bgt.Precision.float64, bgt.Precision.float32_compensated])
+def test_multinomial_fused_round(n, k, dilution, precision):
+    # lattice.round on a multinomial lattice: the update and the histogram of the updated
+    # posterior in one pass, masses from the single-variant tables, then Op-BBPA; along the
+    # oracle's picks, against the oracle's update, masses and selection
+    atoms = n * k
+    pi = workload.priors(40 + atoms, atoms, 0.01, 0.10)
+    tab = bgt.DilutionTable(n, 0.05, 0.9) if dilution else None
+    dil = orc.dilution_table(n, 0.05, 0.9) if dilution else None
+    lat = bgt.Lattice(DL if dilution else ND, n, pi, variants=k, precision=precision)
+    ref = orc.prior(pi)
+    truth = workload.true_state(40 + atoms, pi)
+    sampler = workload.ResponseSampler(40 + atoms, n, k, truth, list(dil) if dil is not None else None)
+    fp32c = precision == bgt.Precision.float32_compensated
+    tol = 1e-6 if fp32c else 1e-12
+    e = orc.bbpa(ref, n, k)[0]
+    masses = np.empty(atoms)
+    pos = neg = 0
+    for rnd in range(4):
+        r = sampler.draw(e)
+        sel = lat.round(e, r, 0.01, 0.01, tab, masses)
+        orc.update_in_place(ref, dilution, n, k, e, r, dil)
+        post = lat.posterior()
+        # the oracle normalises by its serial sum (~1e-11 relative at 2^24 states): compare
+        # on the device's normalisation
+        scale = post.sum() / ref.sum()
+        m_ref = accurate_masses(ref)
+        assert np.max(np.abs(masses / scale - m_ref) / m_ref) <= tol, f"round {rnd}"
+        big = ref >= 1e-30 * ref.sum()
+        assert rel_err(post[big] / scale, ref[big]) <= tol, f"round {rnd}"
+        bound = tie_bound(atoms, k, 2, abs(scale - 1.0), updates=rnd + 1)
+        if fp32c:
+            bound += fp32c_band(rnd + 1, k, 2)
+        e = check_selection(sel.experiment, sel.objective, ref, n, k, 2, bound, f"round {rnd}")
+        pos, neg = orc.classify(m_ref, 0.01, 0.01, pos, neg)
+        assert (lat.positive_classification_mask(), lat.negative_classification_mask()) == (pos, neg)
+    lat.close()
