@@ -130,6 +130,7 @@ def _sig(L):
         "bgt_hip_batch_round": ([ctypes.POINTER(_H), _u32p, _u32p, ctypes.POINTER(_dptr), c_int, c_int, c_double,
                                  c_double, ctypes.POINTER(SelectionC), _u32p, _u32p], c_int),
         "bgt_hip_exact_upset_mass": ([c_int, _dptr, c_int, _u32p, _u32p, c_int, _u32p, _dptr], c_int),
+        "bgt_hip_exact_upset_mass_dilution": ([c_int, _dptr, _dptr, c_int, _u32p, _u32p, c_int, _u32p, _dptr], c_int),
         "bgt_hip_last_error": ([], ctypes.c_char_p),
         "bgt_hip_device_synchronize": ([], c_int),
         "bgt_hip_device_name": ([ctypes.c_char_p, c_int], c_int),

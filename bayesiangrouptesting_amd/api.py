@@ -515,10 +515,11 @@ def batch_round(lattices: Sequence["Lattice"], experiments: Sequence[Optional[in
 
 
 def exact_upset_mass(subjects: int, prior: Sequence[float], history: Sequence[tuple[int, int]],
-                     experiments) -> "float | np.ndarray":
-    """Exact FP64 upset mass(es) of a binary non-dilution lattice after `history` updates
-    (internal coordinates), from the prior and the history alone (no device involved).
-    `experiments` is one experiment or a sequence of them."""
+                     experiments, dilution: Optional[DilutionTable] = None) -> "float | np.ndarray":
+    """Exact FP64 upset mass(es) of a binary lattice after `history` updates (internal
+    coordinates), from the prior and the history alone (no device involved): non-dilution,
+    or dilution with the table the updates were applied with. `experiments` is one
+    experiment or a sequence of them."""
     pi = np.ascontiguousarray(np.asarray(prior, dtype=np.float64))
     es = np.ascontiguousarray(np.array([e for e, _ in history], dtype=np.uint32))
     rs = np.ascontiguousarray(np.array([r for _, r in history], dtype=np.uint32))
@@ -527,9 +528,14 @@ def exact_upset_mass(subjects: int, prior: Sequence[float], history: Sequence[tu
     out = np.empty(qs.size, dtype=np.float64)
     u32p = ctypes.POINTER(ctypes.c_uint32)
     dptr = ctypes.POINTER(ctypes.c_double)
-    nat.check(nat.lib().bgt_hip_exact_upset_mass(subjects, pi.ctypes.data_as(dptr), len(history),
-                                                 es.ctypes.data_as(u32p), rs.ctypes.data_as(u32p), qs.size,
-                                                 qs.ctypes.data_as(u32p), out.ctypes.data_as(dptr)))
+    if dilution is None:
+        nat.check(nat.lib().bgt_hip_exact_upset_mass(subjects, pi.ctypes.data_as(dptr), len(history),
+                                                     es.ctypes.data_as(u32p), rs.ctypes.data_as(u32p), qs.size,
+                                                     qs.ctypes.data_as(u32p), out.ctypes.data_as(dptr)))
+    else:
+        nat.check(nat.lib().bgt_hip_exact_upset_mass_dilution(
+            subjects, pi.ctypes.data_as(dptr), dilution.pointer(), len(history), es.ctypes.data_as(u32p),
+            rs.ctypes.data_as(u32p), qs.size, qs.ctypes.data_as(u32p), out.ctypes.data_as(dptr)))
     return float(out[0]) if scalar else out
 
 

@@ -291,6 +291,28 @@ int bgt_hip_exact_upset_mass(int subjects, const double *prior, int updates, con
 	});
 }
 
+int bgt_hip_exact_upset_mass_dilution(int subjects, const double *prior, const double *dilution, int updates,
+									  const uint32_t *experiments, const uint32_t *responses, int count,
+									  const uint32_t *queries, double *out)
+{
+	return guarded([&] {
+		if (subjects <= 0 || subjects > 32 || prior == nullptr || dilution == nullptr || updates < 0 || count < 0 ||
+			(updates > 0 && (experiments == nullptr || responses == nullptr)) ||
+			(count > 0 && (queries == nullptr || out == nullptr)))
+			throw bgt_hip::Error(BGT_HIP_BAD_INPUT, "bad arguments");
+		bgt_hip::ExactHistory h;
+		h.reset(true, subjects, prior, true);
+		for (int t = 0; t < updates; ++t)
+			h.push(experiments[t], responses[t] & 1u, dilution);
+		std::unique_ptr<bool[]> ok(new bool[count > 0 ? count : 1]);
+		if (!h.masses(queries, (size_t)count, out, ok.get()))
+			throw bgt_hip::Error(BGT_HIP_UNSUPPORTED, "history exceeds the exact program's state budget");
+		for (int i = 0; i < count; ++i)
+			if (!ok[i])
+				throw bgt_hip::Error(BGT_HIP_UNSUPPORTED, "query exceeds the exact program's state budget");
+	});
+}
+
 int bgt_hip_shrink_plan_make(int original_subjects, int current_subjects, int variants, uint32_t classified_atoms,
 							 uint32_t already_classified_subjects, bgt_hip_shrink_plan *out)
 {

@@ -224,7 +224,7 @@ void batch_step(std::vector<BatchItem> &items, int selector, double up, double l
 		}
 		if (objective == kObjGather)
 			launch_bbpa_score(dg, cnt, lat.subjects(), k, target, s);
-		else if (k == 1 && !lat.dilution() && atoms > kSmallBits)
+		else if (k == 1 && atoms > kSmallBits)
 			launch_band_collect(dg, cnt, entries, atoms, k, lat.storage(), objective, atoms == kTileBits, false,
 								target, s);
 	}

@@ -168,10 +168,11 @@ void ExactHistory::Table::add(uint64_t k, double v)
 	}
 }
 
-void ExactHistory::reset(bool usable, int atoms, const double *prior)
+void ExactHistory::reset(bool usable, int atoms, const double *prior, bool dilution)
 {
 	usable_ = usable && atoms <= 32;
 	broken_ = false;
+	dilution_ = dilution;
 	atoms_ = atoms;
 	pi_.assign(prior, prior + atoms);
 	q_.resize(atoms);
@@ -182,14 +183,33 @@ void ExactHistory::reset(bool usable, int atoms, const double *prior)
 	z_ok_ = true;
 }
 
-void ExactHistory::push(uint32_t e, uint32_t r)
+void ExactHistory::push(uint32_t e, uint32_t r, const double *dil)
 {
 	if (!usable_)
 		return;
+	if (dilution_) {
+		if (dil == nullptr) {
+			broken_ = true;
+			return;
+		}
+		// the factors of the device likelihood tables (same FP64 expressions): pool size s,
+		// m positive atoms, P(r = 1) = D[s-1][m]; the empty pool has P(r = 1) = 0
+		const int len = __builtin_popcount(e);
+		Upd u{e, 0.0, 0.0, {}};
+		u.f.resize(len + 1);
+		for (int m = 0; m <= len; ++m) {
+			const double p1 = len > 0 ? dil[(size_t)(len - 1) * (atoms_ + 1) + m] : 0.0;
+			u.f[m] = r ? p1 : 1.0 - p1;
+		}
+		u.f_in = u.f[0];
+		hist_.push_back(std::move(u));
+		z_ok_ = false;
+		return;
+	}
 	// the factors of the device likelihood tables (same FP64 expressions)
 	const double contained = r ? kNegative : (1.0 - kNegative);
 	const double missing = r ? (1.0 - kNegative) : kNegative;
-	hist_.push_back({e, contained, missing});
+	hist_.push_back({e, contained, missing, {}});
 	z_ok_ = false;
 }
 
@@ -201,8 +221,112 @@ double ExactHistory::rel_bound() const
 	return 8.0 * (atoms_ + 2.0 * hist_.size() + 8.0) * std::ldexp(1.0, -53);
 }
 
+bool ExactHistory::counts(uint32_t e, double &out, Scratch &sc) const
+{
+	double lead = 1.0;
+	for (int a = 0; a < atoms_; ++a)
+		if ((e >> a) & 1u)
+			lead *= q_[a];
+	sc.pool.clear();
+	sc.g.resize(0);
+	for (const Upd &u : hist_) {
+		const uint32_t rest = u.e & ~e;
+		if (rest == 0) {
+			lead *= u.f[0]; // every pooled atom is negative: no positive atom in the pool
+			continue;
+		}
+		// the candidate's atoms are negative, so the pool's count is over its remaining atoms
+		size_t i = 0;
+		while (i < sc.pool.size() && sc.pool[i] != rest)
+			++i;
+		const int len = __builtin_popcount(rest);
+		if (i == sc.pool.size()) {
+			sc.pool.push_back(rest);
+			sc.g.emplace_back(u.f.begin(), u.f.begin() + len + 1);
+		} else {
+			for (int m = 0; m <= len; ++m)
+				sc.g[i][m] *= u.f[m];
+		}
+	}
+	const size_t c = sc.pool.size();
+	uint32_t relevant = 0;
+	for (uint32_t p : sc.pool)
+		relevant |= p;
+	// Fields of the 64-bit state key: each partly processed pool holds its count of positive
+	// atoms so far in a field just wide enough for its size, taken (lowest free bits first)
+	// at the pool's first atom and given back after its last.
+	std::vector<int> first(c, -1), last(c, -1), shift(c, -1), width(c, 0);
+	std::vector<int> pos;
+	for (int a : order_)
+		if ((relevant >> a) & 1u)
+			pos.push_back(a);
+	for (size_t t = 0; t < pos.size(); ++t)
+		for (size_t i = 0; i < c; ++i)
+			if ((sc.pool[i] >> pos[t]) & 1u) {
+				if (first[i] < 0)
+					first[i] = (int)t;
+				last[i] = (int)t;
+			}
+	for (size_t i = 0; i < c; ++i)
+		width[i] = 32 - __builtin_clz((uint32_t)__builtin_popcount(sc.pool[i]));
+	Table *cur = &sc.a, *nxt = &sc.b;
+	cur->clear(1, 64);
+	cur->add(0ull, 1.0);
+	uint64_t used = 0; // key bits in use
+	for (size_t t = 0; t < pos.size(); ++t) {
+		const int a = pos[t];
+		uint64_t inc = 0; // key increment of a positive atom
+		std::vector<int> closing;
+		for (size_t i = 0; i < c; ++i) {
+			if (!((sc.pool[i] >> a) & 1u))
+				continue;
+			if (first[i] == (int)t) {
+				const uint64_t field = (1ull << width[i]) - 1ull;
+				int at = 0;
+				while (at + width[i] <= 64 && ((used >> at) & field) != 0ull)
+					++at;
+				if (at + width[i] > 64)
+					return false; // more partly processed pools than the key holds
+				shift[i] = at;
+				used |= field << at;
+			}
+			inc += 1ull << shift[i];
+			if (last[i] == (int)t)
+				closing.push_back((int)i);
+		}
+		nxt->clear(2 * cur->key.size(), 64);
+		for (size_t s = 0; s < cur->key.size(); ++s) {
+			const uint64_t k = cur->key[s];
+			const double wv = cur->w[s];
+			uint64_t k0 = k, k1 = k + inc;
+			double w0 = wv * q_[a], w1 = wv * pi_[a];
+			for (int i : closing) {
+				const uint64_t fm = (1ull << width[i]) - 1ull;
+				w0 *= sc.g[i][(k0 >> shift[i]) & fm];
+				w1 *= sc.g[i][(k1 >> shift[i]) & fm];
+				k0 &= ~(fm << shift[i]);
+				k1 &= ~(fm << shift[i]);
+			}
+			nxt->add(k0, w0);
+			nxt->add(k1, w1);
+		}
+		for (int i : closing)
+			used &= ~(((1ull << width[i]) - 1ull) << shift[i]);
+		if (nxt->key.size() > kMaxStates)
+			return false;
+		std::swap(cur, nxt);
+	}
+	double sum = 0.0;
+	for (size_t s = 0; s < cur->key.size(); ++s)
+		sum += cur->w[s];
+	out = lead * sum;
+	return true;
+}
+
 bool ExactHistory::unnormalised(uint32_t e, double &out, Scratch &sc) const
 {
+	if (dilution_)
+		return counts(e, out, sc);
 	double qe = 1.0;
 	for (int a = 0; a < atoms_; ++a)
 		if ((e >> a) & 1u)

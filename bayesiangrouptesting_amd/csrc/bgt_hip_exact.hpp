@@ -1,5 +1,5 @@
-// Exact upset masses of a binary non-dilution lattice, recomputed in FP64 from the prior and
-// the applied updates rather than from the stored posterior. Used to re-decide selections
+// Exact upset masses of a binary lattice, recomputed in FP64 from the prior and the applied
+// updates rather than from the stored posterior. Used to re-decide selections
 // whose best candidates are closer than the stored form's error (FP32C storage carries
 // ~1e-7 relative error per entry while top-two gaps at 2^28 states are ~1e-9).
 //
@@ -15,6 +15,13 @@
 // denominators would subtract nearly equal masses whenever a pool is likely negative.) The
 // programs stay small in practice: adaptive experiments reuse the same atoms, so the pool
 // sets that can still be contained number in the tens, not 2^updates.
+//
+// Dilution lattices (replicated_lattice.cpp:17-28): the factor of update t depends on the
+// number m_t of positive atoms in the pool, f_t(m) = D[|E_t| - 1][m] or 1 - D[|E_t| - 1][m]
+// (dilution.cpp:44-67). The program then carries, for each pool that is partly processed,
+// its count of positive atoms so far (pools with the same remaining atoms merge, their factor
+// vectors multiplied entry by entry), and applies f(count) when the pool's last atom has been
+// processed.
 #pragma once
 
 #include <cstdint>
@@ -36,10 +43,12 @@ int parallel_width();
 
 class ExactHistory {
 public:
-	// binary non-dilution lattices only; anything else leaves the history unusable
-	void reset(bool usable, int atoms, const double *prior);
+	// binary lattices only; anything else leaves the history unusable. dilution: the pushes
+	// carry dilution tables (subjects x (subjects + 1), row s-1 for pool size s)
+	void reset(bool usable, int atoms, const double *prior, bool dilution = false);
 	bool usable() const { return usable_ && !broken_; }
-	void push(uint32_t experiment, uint32_t response);
+	bool dilution() const { return dilution_; }
+	void push(uint32_t experiment, uint32_t response, const double *dilution_table = nullptr);
 	int updates() const { return (int)hist_.size(); }
 	// M(E) after all pushed updates; false if the state budget of the program is exceeded
 	bool mass(uint32_t experiment, double &out);
@@ -75,7 +84,8 @@ public:
 private:
 	struct Upd {
 		uint32_t e;
-		double f_in, f_out; // factor when the pool is / is not contained in S
+		double f_in, f_out;   // factor when the pool is / is not contained in S
+		std::vector<double> f; // dilution: factor by the pool's count of positive atoms
 	};
 	// Pool-set states of the program in insertion order with their weights. Up to
 	// kDenseBits pools, states index a direct map (reset entry by entry); beyond that an
@@ -93,16 +103,18 @@ private:
 	struct alignas(128) Scratch { // aligned: the pool's threads each grow their own
 		std::vector<uint32_t> pool;
 		std::vector<double> fin, fout;
+		std::vector<std::vector<double>> g; // dilution: merged factor vectors
 		Table a, b;
 		std::vector<double> da, db; // dense program
 	};
 	bool unnormalised(uint32_t e, double &out, Scratch &sc) const;
+	bool counts(uint32_t e, double &out, Scratch &sc) const; // the dilution program
 	double dense(uint32_t e, Scratch &sc) const; // sc.pool/fin/fout built by unnormalised
 	void plan_order(); // atom order of the program for the current history (normaliser())
 	std::vector<Scratch> workers_; // one per pool thread (masses())
 	bool normaliser();
 
-	bool usable_ = false, broken_ = false;
+	bool usable_ = false, broken_ = false, dilution_ = false;
 	int atoms_ = 0;
 	std::vector<double> pi_, q_;
 	std::vector<Upd> hist_;

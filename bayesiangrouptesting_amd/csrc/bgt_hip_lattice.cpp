@@ -214,7 +214,7 @@ void DeviceLattice::init_prior(const double *prior)
 	len_ = 0;
 	clock_.reset();
 	clock_.enabled = false;
-	exact_.reset(k_ == 1 && !dilution_, atoms_, prior);
+	exact_.reset(k_ == 1, atoms_, prior, dilution_);
 	pairs_own_ = true;
 	std::vector<double> pairs(2 * kMaxAtoms, 0.0);
 	for (int a = 0; a < atoms_; ++a) {
@@ -425,7 +425,9 @@ ArenaLayout arena_layout(bool dilution, int n, int k)
 		off += align256(bytes);
 		return o;
 	};
-	a.exact = k == 1 && !dilution;
+	// binary lattices collect their error band for exact re-decision; the device's dense
+	// program (k_exact_band) covers the non-dilution ones
+	a.exact = k == 1;
 	a.scal = take(sizeof(double) * kScalars); // 0
 	a.res = take(sizeof(SelResult));         // 256
 	a.cnt = take(sizeof(unsigned) * 4);      // 512
@@ -641,13 +643,15 @@ void DeviceLattice::prepare_update(uint32_t experiment, uint32_t response, const
 	if (dilution_ && dil == nullptr)
 		throw Error(BGT_HIP_BAD_INPUT, "dilution table is required for dilution lattices");
 	fill_lik(experiment, response, dil);
+	pending_dil_ = dil;
 }
 
 void DeviceLattice::commit_update()
 {
 	inv_cur_ = inv_cur_ == kScalInv0 ? kScalInv1 : kScalInv0;
 	++updates_;
-	exact_.push(e_, r_);
+	exact_.push(e_, r_, pending_dil_);
+	pending_dil_ = nullptr;
 }
 
 void DeviceLattice::update(uint32_t experiment, uint32_t response, const double *dil)
@@ -715,7 +719,7 @@ void DeviceLattice::load_head(const unsigned char *head)
 void DeviceLattice::launch_exact(const Descs &ds_in)
 {
 	exact_pending_ = false;
-	if (exact_dev_ == nullptr || !pairs_own_ || !exact_.usable() || exact_.updates() > kExactMaxHist)
+	if (exact_dev_ == nullptr || dilution_ || !pairs_own_ || !exact_.usable() || exact_.updates() > kExactMaxHist)
 		return;
 	HostBox *box = static_cast<HostBox *>(host_box_);
 	const int n = exact_.updates();

@@ -172,6 +172,7 @@ private:
 	void *post_ = nullptr;
 	void *z_ = nullptr;
 	unsigned long long *fixtab_ = nullptr; // Op-BBPA histogram bins (k >= 2, tables on chip)
+	const double *pending_dil_ = nullptr;  // dilution table of the update being applied
 	size_t fix_bytes_ = 0;
 	char *arena_ = nullptr; // scalars, results, partials, tables
 	double *scal_ = nullptr;

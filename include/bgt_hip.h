@@ -331,6 +331,12 @@ int bgt_hip_select_noalloc(const void *posterior, int subjects, int precision, i
 int bgt_hip_exact_upset_mass(int subjects, const double *prior, int updates, const uint32_t *experiments,
 							 const uint32_t *responses, int count, const uint32_t *queries, double *out);
 
+/* The same for a binary dilution lattice: dilution is the flat subjects x (subjects + 1)
+ * table the updates were applied with (replicated_lattice.cpp:17-28, dilution.cpp:44-67). */
+int bgt_hip_exact_upset_mass_dilution(int subjects, const double *prior, const double *dilution, int updates,
+									  const uint32_t *experiments, const uint32_t *responses, int count,
+									  const uint32_t *queries, double *out);
+
 /* ---- misc ---- */
 const char *bgt_hip_last_error(void);
 int bgt_hip_device_synchronize(void);

@@ -101,3 +101,97 @@ def test_batch_matches_single_queries():
     qs = list(range(0, 1 << n, 37))
     batch = bgt.exact_upset_mass(n, pi, hist, qs)
     assert np.array_equal(batch, np.array([bgt.exact_upset_mass(n, pi, hist, e) for e in qs]))
+
+
+# ---- dilution lattices: the program over counts of positive pooled atoms
+
+def _dilution_history(seed, n, steps):
+    g = workload.SplitMix64(seed)
+    if steps < 0:
+        # an adaptive-like history: a few experiments, repeated
+        few = [g.below((1 << n) - 1) + 1 for _ in range(5)]
+        return [(few[g.below(5)], g.below(2)) for _ in range(-steps)]
+    return [(g.below((1 << n) - 1) + 1, g.below(2)) for _ in range(steps)]
+
+
+@pytest.mark.parametrize("n,steps,seed,alpha,h", [(4, 3, 1, 0.05, 0.9), (8, 6, 2, 0.05, 0.9), (10, 10, 3, 0.05, 0.9),
+                                                  (12, 8, 4, 0.99, 0.005), (14, 12, 5, 0.05, 0.9),
+                                                  (12, -30, 7, 0.05, 0.9)])
+def test_dilution_exact_mass_matches_oracle_posterior(n, steps, seed, alpha, h):
+    pi = np.array(workload.priors(seed, n, 0.01, 0.3))
+    hist = _dilution_history(seed + 100, n, steps)
+    tab = bgt.DilutionTable(n, alpha, h)
+    dil = orc.dilution_table(n, alpha, h)
+    post = orc.prior(pi)
+    for e, r in hist:
+        orc.update_in_place(post, True, n, 1, e, r, dil)
+    g = workload.SplitMix64(seed + 7)
+    probes = [0, (1 << n) - 1] + [g.below(1 << n) for _ in range(12)] + [e for e, _ in hist]
+    got = bgt.exact_upset_mass(n, pi, hist, probes, dilution=tab)
+    for e, m in zip(probes, got):
+        want = orc.upset_mass(post, n, e)
+        assert abs(m - want) <= 1e-12 * max(want, 1e-300) + 1e-300, (e, m, want)
+
+
+def _brute_exact_dilution(n, pi, hist, e, alpha, h):
+    """M(E) with exact rationals under the dilution likelihood (dilution.cpp:44-67)."""
+    fpi = [Fraction(p) for p in pi]
+    fa, fh = Fraction(alpha), Fraction(h)
+
+    def p1(s, m):
+        return fa if m == 0 else 1 - fa * m / ((s - m) * fh + m)
+
+    num = Fraction(0)
+    den = Fraction(0)
+    for st in range(1 << n):
+        w = Fraction(1)
+        for a in range(n):
+            w *= (1 - fpi[a]) if (st >> a) & 1 else fpi[a]
+        for ex, r in hist:
+            s = bin(ex).count("1")
+            m = s - bin(ex & st).count("1")
+            w *= p1(s, m) if r else 1 - p1(s, m)
+        den += w
+        if (st & e) == e:
+            num += w
+    return num / den
+
+
+@pytest.mark.parametrize("seed", [21, 22])
+def test_dilution_exact_mass_matches_rational_brute_force(seed):
+    n = 6
+    pi = np.array(workload.priors(seed, n, 0.05, 0.4))
+    hist = _dilution_history(seed, n, 5)
+    tab = bgt.DilutionTable(n, 0.05, 0.9)
+    for e in [0, 1, 5, 12, 33, 63]:
+        want = float(_brute_exact_dilution(n, pi, hist, e, 0.05, 0.9))
+        # the table's entries are FP64 roundings of the rationals: a few ulp per factor
+        assert abs(bgt.exact_upset_mass(n, pi, hist, e, dilution=tab) - want) <= 1e-13 * want
+
+
+def test_dilution_single_update_closed_form():
+    # one pool E0 of size s: U(E) sums, over the count m of positive atoms of E0 \ E, the
+    # probability of m positives among those atoms times the factor for m
+    n = 9
+    pi = np.array(workload.priors(5, n, 0.05, 0.3))
+    q = 1.0 - pi
+    e0 = 0b101101101
+    tab = bgt.DilutionTable(n, 0.05, 0.9)
+    s = bin(e0).count("1")
+    for r in (0, 1):
+        f = [(tab.values[(s - 1) * (n + 1) + m] if r else 1.0 - tab.values[(s - 1) * (n + 1) + m])
+             for m in range(s + 1)]
+
+        def u(e):
+            rest = [a for a in range(n) if (e0 >> a) & 1 and not (e >> a) & 1]
+            qe = float(np.prod([q[a] for a in range(n) if (e >> a) & 1]))
+            dist = np.zeros(len(rest) + 1)  # distribution of the positive count over rest
+            dist[0] = 1.0
+            for a in rest:
+                dist[1:] = dist[1:] * q[a] + dist[:-1] * pi[a]
+                dist[0] *= q[a]
+            return qe * float(np.dot(dist, f[:len(rest) + 1]))
+
+        z = u(0)
+        for e in (0, 1, 0b11, e0, e0 | 2, 0b111111111):
+            assert abs(bgt.exact_upset_mass(n, pi, [(e0, r)], e, dilution=tab) - u(e) / z) <= 1e-14 * (u(e) / z)
