@@ -236,9 +236,19 @@ void batch_step(std::vector<BatchItem> &items, int selector, double up, double l
 		if (it.select)
 			it.lat->prepare_exact(); // on the host while the device works
 	BGT_HIP_CHECK(hipStreamSynchronize(s));
+	for (size_t i = 0; i < count; ++i)
+		items[i].lat->load_head(reinterpret_cast<const unsigned char *>(h + o_heads + kHeadBytes * i));
+	// the exact normalisers of the lattices whose selections will be re-decided, side by side
+	// on the host threads (each lattice's program is its own; the results do not depend on
+	// which thread computes them)
+	std::vector<size_t> tied;
+	for (size_t i = 0; i < count; ++i)
+		if (items[i].select)
+			tied.push_back(i);
+	if (tied.size() > 1)
+		parallel_for(tied.size(), [&](size_t j, int) { items[tied[j]].lat->prepare_rescore(selector); });
 	for (size_t i = 0; i < count; ++i) {
 		auto &it = items[i];
-		it.lat->load_head(reinterpret_cast<const unsigned char *>(h + o_heads + kHeadBytes * i));
 		if (it.classify)
 			it.lat->classify_from_head(up, lo, nullptr);
 		if (it.select)

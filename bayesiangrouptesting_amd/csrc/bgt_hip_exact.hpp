@@ -74,10 +74,12 @@ public:
 		return true;
 	}
 	// compute the normaliser of the current history now (host work that can overlap with
-	// device work; mass() would otherwise do it on first use)
-	void prepare()
+	// device work; mass() would otherwise do it on first use). Dilution programs, whose
+	// normaliser carries counts of every pool and costs far more, wait for a re-decision
+	// that needs them.
+	void prepare(bool dilution_too = false)
 	{
-		if (usable())
+		if (usable() && (!dilution_ || dilution_too))
 			normaliser();
 	}
 

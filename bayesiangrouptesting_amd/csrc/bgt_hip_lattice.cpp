@@ -802,6 +802,19 @@ bgt_hip_selection DeviceLattice::finish_selection(int selector, const SelResult 
 	return out;
 }
 
+bool DeviceLattice::prepare_rescore(int selector)
+{
+	const int sel = resolve_selector(selector, k_);
+	if (!(sel == kSelBha || sel == kSelBbpa) || !exact_.usable())
+		return false;
+	const SelResult r = static_cast<const HostBox *>(host_box_)->result;
+	const bool found = r.e1 != 0xFFFFFFFFu && r.d1 < 2.0;
+	if (!found || !((r.d2 - r.d1) <= band_width(sel)))
+		return false;
+	exact_.prepare(true);
+	return true;
+}
+
 double DeviceLattice::band_width(int sel) const
 {
 	// Two candidates' device objectives can be misordered only when they are closer than

@@ -134,6 +134,10 @@ public:
 	SelResult head_result() const;
 	// host-side preparation of exact rescoring, to overlap with device work
 	void prepare_exact() { exact_.prepare(); }
+	// After load_head: whether finish_selection will re-decide this selection exactly (a near
+	// tie on a lattice with an exact program); if so, its normaliser is computed now, so that
+	// the normalisers of many lattices can run side by side on the host threads.
+	bool prepare_rescore(int selector);
 	// Take over `parent`'s host-side state (masks, candidate restriction, update history,
 	// pending normalisation slot) ahead of an out-of-place update from it; the posterior
 	// itself is produced by that update. Both must have the same shape and storage.

@@ -6,9 +6,11 @@ test budget is used up. Sessions arrive over time (a Poisson process in the benc
 workload). The engine admits arrived sessions while their device footprint fits the HBM
 budget, and every engine step advances every active session by one round with a single
 batched call (bgt_hip_batch_round: one launch per kernel per group of identically shaped
-lattices, one descriptor upload, one result download). Sessions that arrive while a step
-is on the GPU join the next step, and finished sessions free their memory at once, so
-admitted work keeps flowing rather than waiting for a batch to drain.
+lattices, one descriptor upload, one result download). An admitted session's lattice is
+built (allocation and prior kernel on its own stream) by a host worker thread while the
+current step runs on the GPU, and the session joins the first step after it is ready;
+finished sessions free their memory at once, so admitted work keeps flowing rather than
+waiting for a batch to drain.
 
 Responses come from a seeded simulated lab: each session draws its true state from its
 prior and samples responses with the reference's likelihood (the client side of the
@@ -17,6 +19,7 @@ because each lattice runs the same kernels with the same launch shapes either wa
 """
 from __future__ import annotations
 
+import concurrent.futures
 import dataclasses
 import math
 import time
@@ -53,6 +56,8 @@ class SessionResult:
     arrival: float
     admitted: float
     finished: float
+    near_ties: list = dataclasses.field(default_factory=list)  # per experiment: reported near tie
+    masks: list = dataclasses.field(default_factory=list)  # per experiment: (positive, negative) before it
 
     @property
     def latency(self) -> float:
@@ -96,6 +101,8 @@ class _Live:
         self.pending: Optional[int] = None  # experiment selected and not yet applied
         self.tests = 0
         self.experiments: list[int] = []
+        self.near_ties: list[bool] = []
+        self.masks: list[tuple[int, int]] = []
         self.pos = self.neg = 0
 
     def classified(self) -> bool:
@@ -121,82 +128,110 @@ class ServingEngine:
         """Serve `specs` (arrival times relative to the start). With realtime, sessions
         become visible at their arrival times on the wall clock; otherwise all are
         queued at time zero (and latency counts from there). Returns per-session results
-        and aggregate statistics."""
+        and aggregate statistics.
+
+        Admission overlaps the device work: an admitted session's lattice (allocation and
+        prior kernel, on the lattice's own stream) is built by a host worker thread while
+        the current batched step runs, and the session joins the first step after it is
+        ready. The provider's calls release the interpreter lock, and distinct lattices may
+        be used from distinct threads."""
         queue = sorted(specs, key=lambda s: (s.arrival, s.sid))
         qi = 0
         live: list[_Live] = []
+        building: list = []  # (future, spec) of admitted sessions whose lattice is being built
         used = 0
         peak_used = 0
         done: list[SessionResult] = []
         step_ms: list[float] = []
         batch_sizes: list[int] = []
-        admit_s = 0.0  # host time spent creating admitted sessions' lattices
+        admit_wait_s = 0.0  # time the serving thread waited for admissions with nothing to step
         fp = {}
         t0 = time.perf_counter()
+        builder = concurrent.futures.ThreadPoolExecutor(max_workers=1)
 
         def now() -> float:
             return time.perf_counter() - t0
 
-        while qi < len(queue) or live:
-            # admission: arrived sessions, in order, while their footprint fits
-            tnow = now()
-            while qi < len(queue) and len(live) < self.max_active:
-                spec = queue[qi]
-                if realtime and spec.arrival > tnow:
-                    break
-                key = (spec.dilution, spec.subjects, spec.variants)
-                if key not in fp:
-                    fp[key] = self.footprint(spec)
-                if used + fp[key] > self.hbm_budget:
+        try:
+            while qi < len(queue) or live or building:
+                # admission: arrived sessions, in order, while their footprint fits
+                tnow = now()
+                while qi < len(queue) and len(live) + len(building) < self.max_active:
+                    spec = queue[qi]
+                    if realtime and spec.arrival > tnow:
+                        break
+                    key = (spec.dilution, spec.subjects, spec.variants)
+                    if key not in fp:
+                        fp[key] = self.footprint(spec)
+                    if used + fp[key] > self.hbm_budget:
+                        if not live and not building:
+                            raise RuntimeError(f"session {spec.sid} alone exceeds the HBM budget")
+                        break
+                    building.append((builder.submit(_Live, spec, self.precision, tnow), spec))
+                    used += fp[key]
+                    peak_used = max(peak_used, used)
+                    qi += 1
+                # sessions whose lattices are ready join this step
+                if building:
                     if not live:
-                        raise RuntimeError(f"session {spec.sid} alone exceeds the HBM budget")
-                    break
-                ta = time.perf_counter()
-                live.append(_Live(spec, self.precision, tnow))
-                admit_s += time.perf_counter() - ta
-                used += fp[key]
-                peak_used = max(peak_used, used)
-                qi += 1
-            if not live:
-                # idle until the next arrival
-                wait = queue[qi].arrival - now()
-                if wait > 0:
-                    time.sleep(min(wait, 0.01))
-                continue
-            # one batched round for every active session
-            exps, resps = [], []
-            for s in live:
-                if s.pending is None:
-                    exps.append(None)
-                    resps.append(None)
-                else:
-                    exps.append(s.pending)
-                    resps.append(s.sampler.draw(s.pending))
-            ts = time.perf_counter()
-            sels, masks = api.batch_round([s.lattice for s in live], exps, resps, api.SelectorType.auto_select,
-                                          self.up, self.lo, [s.table for s in live])
-            step_ms.append((time.perf_counter() - ts) * 1e3)
-            batch_sizes.append(len(live))
-            tdone = now()
-            keep = []
-            for s, sel, (p, q), r in zip(live, sels, masks, resps):
-                if r is not None:
-                    s.tests += 1
-                s.pos, s.neg = p, q
-                finished = s.classified() or s.tests >= self.max_tests
-                if not finished:
-                    s.pending = sel.experiment
-                    s.experiments.append(sel.experiment)
-                    keep.append(s)
+                        ta = time.perf_counter()
+                        concurrent.futures.wait([f for f, _ in building],
+                                                return_when=concurrent.futures.FIRST_COMPLETED)
+                        admit_wait_s += time.perf_counter() - ta
+                    still = []
+                    for fut, spec in building:
+                        if fut.done():
+                            live.append(fut.result())
+                        else:
+                            still.append((fut, spec))
+                    building = still
+                if not live:
+                    if qi < len(queue):
+                        # idle until the next arrival
+                        wait = queue[qi].arrival - now()
+                        if wait > 0:
+                            time.sleep(min(wait, 0.01))
                     continue
-                spec = s.spec
-                done.append(SessionResult(spec.sid, spec.subjects, spec.variants, spec.dilution, s.tests,
-                                          s.classified(), s.pos, s.neg, s.experiments,
-                                          spec.arrival if realtime else 0.0, s.admitted,
-                                          tdone))
-                used -= fp[(spec.dilution, spec.subjects, spec.variants)]
-                s.lattice.close()
-            live = keep
+                # one batched round for every active session
+                exps, resps = [], []
+                for s in live:
+                    if s.pending is None:
+                        exps.append(None)
+                        resps.append(None)
+                    else:
+                        exps.append(s.pending)
+                        resps.append(s.sampler.draw(s.pending))
+                ts = time.perf_counter()
+                sels, masks = api.batch_round([s.lattice for s in live], exps, resps, api.SelectorType.auto_select,
+                                              self.up, self.lo, [s.table for s in live])
+                step_ms.append((time.perf_counter() - ts) * 1e3)
+                batch_sizes.append(len(live))
+                tdone = now()
+                keep = []
+                for s, sel, (p, q), r in zip(live, sels, masks, resps):
+                    if r is not None:
+                        s.tests += 1
+                    s.pos, s.neg = p, q
+                    finished = s.classified() or s.tests >= self.max_tests
+                    if not finished:
+                        s.pending = sel.experiment
+                        s.experiments.append(sel.experiment)
+                        s.near_ties.append(bool(sel.near_tie))
+                        s.masks.append((p, q))
+                        keep.append(s)
+                        continue
+                    spec = s.spec
+                    done.append(SessionResult(spec.sid, spec.subjects, spec.variants, spec.dilution, s.tests,
+                                              s.classified(), s.pos, s.neg, s.experiments,
+                                              spec.arrival if realtime else 0.0, s.admitted,
+                                              tdone, s.near_ties, s.masks))
+                    used -= fp[(spec.dilution, spec.subjects, spec.variants)]
+                    s.lattice.close()
+                live = keep
+        finally:
+            for fut, _ in building:
+                fut.cancel()
+            builder.shutdown(wait=True)
         wall = now()
         lat = np.array([r.latency for r in done]) * 1e3
         return {
@@ -207,7 +242,7 @@ class ServingEngine:
                            "max": float(lat.max())},
             "step_ms": {"p50": float(np.percentile(step_ms, 50)), "p99": float(np.percentile(step_ms, 99)),
                         "steps": len(step_ms), "total_s": float(np.sum(step_ms)) / 1e3},
-            "admission_s": admit_s,
+            "admission_wait_s": admit_wait_s,
             "batch": {"mean": float(np.mean(batch_sizes)), "max": int(np.max(batch_sizes))},
             "tests_mean": float(np.mean([r.tests for r in done])),
             "classified": sum(r.classified for r in done),

@@ -375,7 +375,7 @@ def serving_workload(sessions: int, rate: float, seed: int = 5) -> dict:
                        f"U[0.005,0.10] (seed {seed}), up to 20 tests each; 'at_rate': Poisson arrivals at "
                        f"{rate:g}/s; 'saturated': all sessions queued at time 0"}
     specs = serving.serving_mix(sessions, rate=rate, seed=seed)
-    exps = {}
+    exps, ties = {}, {}
     for prec, key in ((bgt.Precision.float64, "fp64"), (bgt.Precision.float32_compensated, "fp32c")):
         out[key] = {}
         for mode, realtime in (("at_rate", True), ("saturated", False)):
@@ -384,9 +384,69 @@ def serving_workload(sessions: int, rate: float, seed: int = 5) -> dict:
             results = rep.pop("results")
             if mode == "at_rate":
                 exps[key] = {r.sid: r.experiments for r in results}
+                ties[key] = {r.sid: (r.near_ties, r.masks) for r in results}
             out[key][mode] = rep
             log(f"serving {key} {mode}: {rep['sessions_per_s']:.1f} sessions/s p50 {rep['latency_ms']['p50']:.1f} ms "
                 f"p99 {rep['latency_ms']['p99']:.1f} ms")
+    out["identical_sessions"] = sum(exps["fp64"][s] == exps["fp32c"][s] for s in exps["fp64"])
+    # each session whose FP32C experiments part from FP64's: the first differing test, and
+    # whether either form reported that pick as a near tie (FP64 rounding decides it)
+    div = []
+    for sid, a in exps["fp64"].items():
+        b = exps["fp32c"][sid]
+        if a == b:
+            continue
+        i = next((j for j in range(min(len(a), len(b))) if a[j] != b[j]), min(len(a), len(b)))
+        (n64, m64), (n32, m32) = ties["fp64"][sid], ties["fp32c"][sid]
+        t64 = i < len(a) and n64[i]
+        t32 = i < len(b) and n32[i]
+        # a classification that differed first (a mass within FP32C's error of a threshold)
+        cls = next((j for j in range(min(len(m64), len(m32), i + 1)) if m64[j] != m32[j]), None)
+        div.append({"sid": sid, "first": i, "near_tie_fp64": bool(t64), "near_tie_fp32c": bool(t32),
+                    "classification_first": cls is not None})
+    out["divergent_sessions"] = len(div)
+    out["divergent_at_reported_near_tie"] = sum(d["near_tie_fp64"] or d["near_tie_fp32c"] for d in div)
+    out["divergent_after_classification"] = sum(
+        d["classification_first"] and not (d["near_tie_fp64"] or d["near_tie_fp32c"]) for d in div)
+    out["divergent_unexplained"] = [d for d in div if not (d["near_tie_fp64"] or d["near_tie_fp32c"]
+                                                            or d["classification_first"])][:20]
+    out["fp32c_vs_fp64_capacity"] = (out["fp32c"]["saturated"]["sessions_per_s"] /
+                                     out["fp64"]["saturated"]["sessions_per_s"])
+    return out
+
+
+def serving28_workload(sessions: int = 32, rate: float = 100.0) -> dict:
+    """The serving loop on the headline lattice: `sessions` adaptive sessions of the n = 28
+    binary non-dilution configuration (priors U[0.005, 0.05] seed 2, true state of session i
+    drawn with seed 2000 + i), each until classified or 20 tests, batched per step; Poisson
+    arrivals at `rate` per second and all queued at once, FP64 and FP32C, admission by HBM
+    footprint (96 GiB)."""
+    import bayesiangrouptesting_amd as bgt
+    from bayesiangrouptesting_amd import serving, workload
+
+    c = workload.C2
+    pi = workload.priors(c.seed, c.subjects, c.prior_lo, c.prior_hi)
+    g = workload.SplitMix64(c.seed)
+    t = 0.0
+    specs = []
+    for i in range(sessions):
+        t += -np.log(1.0 - g.uniform()) / rate
+        specs.append(serving.SessionSpec(i, c.subjects, 1, False, list(pi), c.seed * 1000 + i, t))
+    out = {"workload": f"{sessions} sessions of the n=28 binary non-dilution lattice (priors U[0.005,0.05] seed 2), "
+                       f"each until classified or 20 tests; 'at_rate': Poisson arrivals at {rate:g}/s; 'saturated': "
+                       "all sessions queued at time 0"}
+    exps = {}
+    for prec, key in ((bgt.Precision.float64, "fp64"), (bgt.Precision.float32_compensated, "fp32c")):
+        out[key] = {}
+        for mode, realtime in (("at_rate", True), ("saturated", False)):
+            eng = serving.ServingEngine(precision=prec, hbm_budget=96 << 30)
+            rep = eng.run(specs, realtime=realtime)
+            results = rep.pop("results")
+            if mode == "saturated":
+                exps[key] = {r.sid: r.experiments for r in results}
+            out[key][mode] = rep
+            log(f"serving n=28 {key} {mode}: {rep['sessions_per_s']:.1f} sessions/s p50 "
+                f"{rep['latency_ms']['p50']:.1f} ms p99 {rep['latency_ms']['p99']:.1f} ms")
     out["identical_sessions"] = sum(exps["fp64"][s] == exps["fp32c"][s] for s in exps["fp64"])
     out["fp32c_vs_fp64_capacity"] = (out["fp32c"]["saturated"]["sessions_per_s"] /
                                      out["fp64"]["saturated"]["sessions_per_s"])
@@ -552,6 +612,8 @@ def main() -> int:
         line["shrink"] = shrink_workload()
         log(f"serving ({args.serving_sessions} sessions at {args.serving_rate:g}/s)")
         line["serving"] = serving_workload(args.serving_sessions, args.serving_rate)
+        log("serving n=28 sessions")
+        line["serving_n28"] = serving28_workload()
     if not args.no_cpu:
         # the FP64 run's first two rounds (internal experiment, response)
         steps = [(bgt.reverse_bits(p[1], n), p[2]) for p in r64["plan"][:2]]
