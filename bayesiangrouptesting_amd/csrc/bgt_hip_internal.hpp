@@ -278,6 +278,27 @@ struct LookStep {
 };
 // lattices from this many subjects take the down-set lookahead (below, the O(4^n) scan)
 constexpr int kLookDownMin = 13;
+// the lookahead selects at most this many experiments (2^16 groups at the last step)
+constexpr int kLookMaxSteps = 16;
+// from this many earlier picks on, a step takes the class-decomposed scores (below, the
+// 2^(m+1) group masses of a candidate are kept in registers)
+constexpr int kLookClassMin = 5;
+// up to this many subjects a class's down-set sums are formed in LDS (2^n doubles)
+constexpr int kLookLdsMax = 12;
+// The earlier picks of a lookahead step in internal coordinates, for the class-decomposed
+// scores: a state's class has bit i set when the state lies inside pick i.
+struct LookClasses {
+	uint32_t sel[kLookMaxSteps - 1];
+	int m;
+	int n;
+};
+// classes whose down-set sums one chunk holds (rows of 2^n doubles)
+size_t look_class_rows(int n);
+// every candidate's score (internal index order) after m >= 1 earlier picks through the
+// class-decomposed sums (P: look_class_rows(n) rows of 2^n doubles), and each of `blocks`
+// blocks' lowest score, as launch_lookahead_down
+void launch_look_classes(const Descs &descs, Storage st, const LookClasses &lc, double *P, double *scores,
+						 double *best_score, unsigned blocks, hipStream_t stream);
 // D(X) = sum over s <= X of x(s), internal coordinates, FP64 (2^atoms entries)
 void launch_down_sums(const Descs &descs, int atoms, Storage st, double *D, hipStream_t stream);
 // every candidate's score (internal index order) and each of `blocks` blocks' lowest score

@@ -2025,10 +2025,12 @@ __global__ __launch_bounds__(256) void k_down_first(Descs descs, double *D)
 }
 
 // Pass B: D over bits b0..b0+R-1 (b0 >= 12, R <= 6) in registers: a thread owns one column
-// below b0 (consecutive columns across the lanes, so 512-byte runs) and its 2^R entries.
+// below b0 (consecutive columns across the lanes, so 512-byte runs) and its 2^R entries; grid
+// row y works on the transform at D + y * row_stride.
 template <int R>
-__global__ __launch_bounds__(256) void k_down_more(double *D, int b0)
+__global__ __launch_bounds__(256) void k_down_more(double *D, int b0, uint64_t row_stride)
 {
+	D += blockIdx.y * row_stride; // several masked transforms side by side (grid y)
 	const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
 	const uint64_t col = t & ((1ull << b0) - 1ull), hi = t >> b0;
 	const uint64_t base = col | (hi << (b0 + R));
@@ -2128,6 +2130,126 @@ __global__ __launch_bounds__(256) void k_lookahead_pick(const double *scores, co
 	}
 	if (threadIdx.x == 0)
 		best_exp[blockIdx.x] = s_ex[0];
+}
+
+// ------------------------------- k-step lookahead after five or more earlier picks
+//
+// With m >= 5 earlier picks the 2^(m+1) group masses of a candidate no longer fit in
+// registers. The score is regrouped by the class J of a state under the m earlier picks (bit
+// i: the state lies inside pick i). A candidate c splits class J into the states inside c,
+// of mass H_J(c) = sum_{s <= c, class(s) = J} x(s) (the down-set sums of the posterior masked
+// to class J), and the rest, of mass H_J(full) - H_J(c); the score is
+// sum_J |H_J(full) - H_J(c) - t| + |H_J(c) - t| with t = 2^-(m+1). A step costs 2^m masked
+// down-set transforms of 2^n entries, every sum over non-negative terms.
+
+__device__ __forceinline__ uint32_t look_class(uint32_t s, const LookClasses &lc)
+{
+	uint32_t j = 0;
+	for (int i = 0; i < lc.m; ++i)
+		j |= ((lc.sel[i] & s) == s ? 1u : 0u) << i;
+	return j;
+}
+
+// Up to kLookLdsMax subjects: one workgroup per class (j0 + blockIdx.x), the masked posterior
+// and its down-set sums in LDS (2^n doubles), the class's row of sums into P (row stride 2^n).
+template <typename TP>
+__global__ __launch_bounds__(256) void k_look_class_lds(Descs descs, LookClasses lc, uint32_t j0, double *P)
+{
+	extern __shared__ double L[];
+	const LatDesc &d = descs.many ? descs.many[0] : descs.one;
+	const uint32_t states = 1u << lc.n;
+	const uint32_t j = j0 + blockIdx.x;
+	gptr<const TP> post = G(static_cast<const TP *>(d.post));
+	const double inv = d.scal[d.inv_in];
+	for (uint32_t s = threadIdx.x; s < states; s += blockDim.x)
+		L[s] = look_class(s, lc) == j ? static_cast<double>(post[s]) * inv : 0.0;
+	__syncthreads();
+	for (int b = 0; b < lc.n; ++b) {
+		const uint32_t bit = 1u << b;
+		for (uint32_t h = threadIdx.x; h < (states >> 1); h += blockDim.x) {
+			// the h-th state with bit b set: h's bits below b, bit b, h's higher bits above it
+			const uint32_t s = ((h >> b) << (b + 1)) | bit | (h & (bit - 1u));
+			L[s] += L[s ^ bit];
+		}
+		__syncthreads();
+	}
+	double *row = P + (size_t)blockIdx.x * states;
+	for (uint32_t s = threadIdx.x; s < states; s += blockDim.x)
+		row[s] = L[s];
+}
+
+// From 13 subjects: the first 12 bits of the down-set sums of the posterior masked to class
+// j0 + blockIdx.y, into row blockIdx.y of P (row stride 2^atoms); the layout of k_down_first.
+template <typename TP>
+__global__ __launch_bounds__(256) void k_look_class_first(Descs descs, LookClasses lc, uint32_t j0, double *P)
+{
+	const LatDesc &d = descs.many ? descs.many[0] : descs.one;
+	const uint32_t lane = threadIdx.x & 63u;
+	const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+	const uint64_t base = (wave << kDownFirstBits) | lane;
+	const uint32_t j = j0 + blockIdx.y;
+	double *D = P + ((uint64_t)blockIdx.y << lc.n);
+	gptr<const TP> post = G(static_cast<const TP *>(d.post));
+	const double inv = d.scal[d.inv_in];
+	double v[64];
+#pragma unroll
+	for (int r = 0; r < 64; ++r) {
+		const uint64_t s = base + ((uint64_t)r << 6);
+		v[r] = look_class((uint32_t)s, lc) == j ? static_cast<double>(post[s]) * inv : 0.0;
+	}
+#pragma unroll
+	for (int b = 0; b < 6; ++b)
+#pragma unroll
+		for (int r = 0; r < 64; ++r)
+			if (r & (1 << b))
+				v[r] += v[r ^ (1 << b)];
+#pragma unroll
+	for (int b = 0; b < 6; ++b) {
+		const bool up = (lane >> b) & 1u;
+#pragma unroll
+		for (int r = 0; r < 64; ++r) {
+			const double o = __shfl_xor(v[r], 1 << b);
+			v[r] += up ? o : 0.0;
+		}
+	}
+#pragma unroll
+	for (int r = 0; r < 64; ++r)
+		D[base + ((uint64_t)r << 6)] = v[r];
+}
+
+// Adds classes 0..rows-1 of P (row stride 2^n, row r holding H_J for J = j0 + r) to every
+// candidate's running score, scores[internal index], class by class in ascending order.
+// `first` starts the scores at zero; `last` closes them: the empty experiment scores +inf and
+// each block's lowest score goes to best_score[blockIdx.x] (the grid of k_lookahead_pick).
+__global__ __launch_bounds__(256) void k_look_class_accum(const double *P, int rows, int n, double target, int first,
+														  int last, double *scores, double *best_score)
+{
+	__shared__ double s_sc[256];
+	const uint32_t states = 1u << n;
+	double bs = INFINITY;
+	for (uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x; ci < states; ci += gridDim.x * blockDim.x) {
+		double sc = first ? 0.0 : scores[ci];
+		for (int r = 0; r < rows; ++r) {
+			const double *row = P + ((size_t)r << n);
+			const double h = row[ci];
+			sc += fabs(row[states - 1u] - h - target) + fabs(h - target);
+		}
+		if (last)
+			sc = ci == 0u ? INFINITY : sc;
+		scores[ci] = sc;
+		bs = fmin(bs, sc);
+	}
+	if (!last)
+		return;
+	s_sc[threadIdx.x] = bs;
+	__syncthreads();
+	for (int o = blockDim.x >> 1; o > 0; o >>= 1) {
+		if ((int)threadIdx.x < o)
+			s_sc[threadIdx.x] = fmin(s_sc[threadIdx.x], s_sc[threadIdx.x + o]);
+		__syncthreads();
+	}
+	if (threadIdx.x == 0)
+		best_score[blockIdx.x] = s_sc[0];
 }
 
 // ------------------------------------------------------------------ tree statistics
@@ -3039,6 +3161,36 @@ void launch_lookahead(const Descs &descs, int n, Storage st, const uint32_t *sel
 	});
 }
 
+namespace {
+
+// the bits of the down-set sums above the first 12, for `rows` transforms of 2^atoms entries
+// side by side: ceil(rest / 6) register passes of near-equal width; each pass runs
+// 2^(atoms-R) threads per transform, 256 per workgroup (atoms - R >= 12)
+void launch_down_rest(double *D, int atoms, unsigned rows, hipStream_t stream)
+{
+	const int rest = atoms - kDownFirstBits;
+	const int passes = (rest + 5) / 6;
+	const uint64_t stride = 1ull << atoms;
+	for (int p = 0, b0 = kDownFirstBits; p < passes; ++p) {
+		const int w = (rest - (b0 - kDownFirstBits) + (passes - p) - 1) / (passes - p);
+		const dim3 grid((unsigned)(1ull << (atoms - w - 8)), rows);
+		launch_checked("k_down_more", [&] {
+			switch (w) {
+			case 1: hipLaunchKernelGGL(k_down_more<1>, grid, dim3(256), 0, stream, D, b0, stride); break;
+			case 2: hipLaunchKernelGGL(k_down_more<2>, grid, dim3(256), 0, stream, D, b0, stride); break;
+			case 3: hipLaunchKernelGGL(k_down_more<3>, grid, dim3(256), 0, stream, D, b0, stride); break;
+			case 4: hipLaunchKernelGGL(k_down_more<4>, grid, dim3(256), 0, stream, D, b0, stride); break;
+			case 5: hipLaunchKernelGGL(k_down_more<5>, grid, dim3(256), 0, stream, D, b0, stride); break;
+			case 6: hipLaunchKernelGGL(k_down_more<6>, grid, dim3(256), 0, stream, D, b0, stride); break;
+			default: throw Error(4, "down-set pass width out of range");
+			}
+		});
+		b0 += w;
+	}
+}
+
+} // namespace
+
 void launch_down_sums(const Descs &descs, int atoms, Storage st, double *D, hipStream_t stream)
 {
 	// pass A: 2^(atoms-12) waves of 4096 entries, up to four waves per workgroup
@@ -3053,25 +3205,54 @@ void launch_down_sums(const Descs &descs, int atoms, Storage st, double *D, hipS
 		else
 			hipLaunchKernelGGL(k_down_first<float>, grid, dim3(block), 0, stream, descs, D);
 	});
-	// pass B: the bits above 12 in ceil(rest / 6) register passes of near-equal width; each
-	// pass runs 2^(atoms-R) threads, 256 per workgroup (atoms - R >= 12)
-	const int rest = atoms - kDownFirstBits;
-	const int passes = (rest + 5) / 6;
-	for (int p = 0, b0 = kDownFirstBits; p < passes; ++p) {
-		const int w = (rest - (b0 - kDownFirstBits) + (passes - p) - 1) / (passes - p);
-		const dim3 grid((unsigned)(1ull << (atoms - w - 8)));
-		launch_checked("k_down_more", [&] {
-			switch (w) {
-			case 1: hipLaunchKernelGGL(k_down_more<1>, grid, dim3(256), 0, stream, D, b0); break;
-			case 2: hipLaunchKernelGGL(k_down_more<2>, grid, dim3(256), 0, stream, D, b0); break;
-			case 3: hipLaunchKernelGGL(k_down_more<3>, grid, dim3(256), 0, stream, D, b0); break;
-			case 4: hipLaunchKernelGGL(k_down_more<4>, grid, dim3(256), 0, stream, D, b0); break;
-			case 5: hipLaunchKernelGGL(k_down_more<5>, grid, dim3(256), 0, stream, D, b0); break;
-			case 6: hipLaunchKernelGGL(k_down_more<6>, grid, dim3(256), 0, stream, D, b0); break;
-			default: throw Error(4, "down-set pass width out of range");
-			}
+	launch_down_rest(D, atoms, 1, stream);
+}
+
+size_t look_class_rows(int n)
+{
+	// classes per chunk: at most 64, and at most 256 MiB of rows
+	const uint64_t row = sizeof(double) << n;
+	return (size_t)std::max<uint64_t>(1, std::min<uint64_t>(64, (256ull << 20) / row));
+}
+
+void launch_look_classes(const Descs &descs, Storage st, const LookClasses &lc, double *P, double *scores,
+						 double *best_score, unsigned blocks, hipStream_t stream)
+{
+	const int n = lc.n;
+	if (lc.m < 1 || lc.m >= kLookMaxSteps || n < 1 || n > 31)
+		throw Error(4, "lookahead classes out of range");
+	if (n > kLookLdsMax && n < kDownFirstBits)
+		throw Error(4, "lookahead classes: no transform for this subject count");
+	const uint32_t classes = 1u << lc.m;
+	const uint32_t chunk = (uint32_t)look_class_rows(n);
+	const double target = 1.0 / (double)(2u * classes);
+	for (uint32_t j0 = 0; j0 < classes; j0 += chunk) {
+		const uint32_t rows = std::min(chunk, classes - j0);
+		if (n <= kLookLdsMax) {
+			const size_t lds = sizeof(double) << n;
+			launch_checked("k_look_class_lds", [&] {
+				if (st == Storage::fp64)
+					hipLaunchKernelGGL(k_look_class_lds<double>, dim3(rows), dim3(256), lds, stream, descs, lc, j0, P);
+				else
+					hipLaunchKernelGGL(k_look_class_lds<float>, dim3(rows), dim3(256), lds, stream, descs, lc, j0, P);
+			});
+		} else {
+			launch_checked("k_look_class_first", [&] {
+				const uint64_t threads = 1ull << (n - 6);
+				const unsigned block = (unsigned)std::min<uint64_t>(256, threads);
+				const dim3 grid((unsigned)(threads / block), rows);
+				if (st == Storage::fp64)
+					hipLaunchKernelGGL(k_look_class_first<double>, grid, dim3(block), 0, stream, descs, lc, j0, P);
+				else
+					hipLaunchKernelGGL(k_look_class_first<float>, grid, dim3(block), 0, stream, descs, lc, j0, P);
+			});
+			launch_down_rest(P, n, rows, stream);
+		}
+		const int first = j0 == 0 ? 1 : 0, last = j0 + rows == classes ? 1 : 0;
+		launch_checked("k_look_class_accum", [&] {
+			hipLaunchKernelGGL(k_look_class_accum, dim3(blocks), dim3(256), 0, stream, P, (int)rows, n, target, first,
+							   last, scores, best_score);
 		});
-		b0 += w;
 	}
 }
 

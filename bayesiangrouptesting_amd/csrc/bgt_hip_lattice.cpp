@@ -1333,8 +1333,8 @@ std::vector<uint32_t> DeviceLattice::select_experiments(int steps)
 		return out;
 	if (k_ != 1)
 		throw Error(BGT_HIP_BAD_INPUT, "Op-BHA multi-experiment selection is only supported for binary lattices");
-	if (steps > 5)
-		throw Error(BGT_HIP_UNSUPPORTED, "lookahead is limited to 5 steps (32 groups)");
+	if (steps > kLookMaxSteps)
+		throw Error(BGT_HIP_UNSUPPORTED, "lookahead is limited to 16 steps (2^16 groups at the last step)");
 	auto rev = [&](uint32_t v) {
 		uint32_t r = 0;
 		for (int i = 0; i < n_; ++i) {
@@ -1344,73 +1344,110 @@ std::vector<uint32_t> DeviceLattice::select_experiments(int steps)
 		return r;
 	};
 	out.push_back(rev(select(kSelBha).experiment));
+	if (steps == 1)
+		return out;
 	// Tie rule: the lowest public index among the candidates whose score lies within twice
 	// the score's rounding bound of the lowest score. Candidates that tie in exact arithmetic
 	// (common: whenever both halves of every earlier group fall on one side of the target,
 	// the score does not depend on the candidate) are then decided by index, not by rounding.
-	if (steps > 1 && n_ >= kLookDownMin) {
-		// down-set sums once, then every candidate from 2^(m+1) of them per step
-		const uint64_t states = 1ull << n_;
-		double *D = static_cast<double *>(device_alloc(sizeof(double) * states, stream_));
-		double *sc = static_cast<double *>(device_alloc(sizeof(double) * states, stream_));
+	//
+	// Scores per step: up to 4 earlier picks, every candidate's 2^(m+1) group masses in
+	// registers (from 13 subjects through down-set sums built once, below by a pass over all
+	// states per candidate); from 5 on, the class-decomposed sums (2^m masked down-set
+	// transforms per step).
+	const uint64_t states = 1ull << n_;
+	const unsigned blocks = (unsigned)std::min<uint64_t>(2048, std::max<uint64_t>(1, states / 256));
+	struct Buffers {
+		hipStream_t s;
+		std::vector<void *> held;
+		void *get(size_t bytes)
+		{
+			held.push_back(device_alloc(bytes, s));
+			return held.back();
+		}
+		~Buffers()
+		{
+			for (auto it = held.rbegin(); it != held.rend(); ++it)
+				device_free(*it, s);
+		}
+	} buf{stream_, {}};
+	double *D = nullptr, *P = nullptr, *sc = nullptr, *bsc = nullptr;
+	uint32_t *bex = nullptr;
+	std::vector<double> hs(blocks), host_scores;
+	std::vector<uint32_t> he(blocks);
+	// the lowest public index within the tie limit, from per-block lowest scores on the device
+	auto pick_on_device = [&](int m) {
+		BGT_HIP_CHECK(hipMemcpyAsync(hs.data(), bsc, sizeof(double) * blocks, hipMemcpyDeviceToHost, stream_));
+		sync();
+		double best = INFINITY;
+		for (double v : hs)
+			best = std::min(best, v);
+		const double limit = best + 2.0 * lookahead_score_error(n_, m);
+		launch_lookahead_pick(sc, bsc, n_, limit, bex, blocks, stream_);
+		BGT_HIP_CHECK(hipMemcpyAsync(he.data(), bex, sizeof(uint32_t) * blocks, hipMemcpyDeviceToHost, stream_));
+		sync();
+		uint32_t best_c = 0xFFFFFFFFu;
+		for (uint32_t v : he)
+			best_c = std::min(best_c, v);
+		if (best_c == 0xFFFFFFFFu || best_c == 0u)
+			throw Error(BGT_HIP_INTERNAL, "lookahead found no candidate");
+		return best_c;
+	};
+	const bool down = n_ >= kLookDownMin;
+	if (down || steps > kLookClassMin) {
+		sc = static_cast<double *>(buf.get(sizeof(double) * states));
+		bsc = static_cast<double *>(buf.get(sizeof(double) * blocks));
+		bex = static_cast<uint32_t *>(buf.get(sizeof(uint32_t) * blocks));
+	}
+	if (down) {
+		D = static_cast<double *>(buf.get(sizeof(double) * states));
 		launch_down_sums(single(), atoms_, storage_, D, stream_);
-		const unsigned blocks = (unsigned)std::min<uint64_t>(2048, std::max<uint64_t>(1, states / 256));
-		double *bsc = static_cast<double *>(device_alloc(sizeof(double) * blocks, stream_));
-		uint32_t *bex = static_cast<uint32_t *>(device_alloc(sizeof(uint32_t) * blocks, stream_));
-		std::vector<double> hs(blocks);
-		std::vector<uint32_t> he(blocks);
-		const uint32_t full = (uint32_t)(states - 1);
-		for (int step = 1; step < steps; ++step) {
+	}
+	const uint32_t full = (uint32_t)(states - 1);
+	for (int step = 1; step < steps; ++step) {
+		const int m = (int)out.size();
+		if (m >= kLookClassMin) {
+			if (P == nullptr)
+				P = static_cast<double *>(buf.get(sizeof(double) * look_class_rows(n_) * states));
+			LookClasses lc{};
+			lc.m = m;
+			lc.n = n_;
+			for (int i = 0; i < m; ++i)
+				lc.sel[i] = rev(out[i]);
+			launch_look_classes(single(), storage_, lc, P, sc, bsc, blocks, stream_);
+			out.push_back(pick_on_device(m));
+		} else if (down) {
 			LookStep st{};
 			st.n = n_;
-			st.m = (int)out.size();
-			for (int k = 0; k < (1 << st.m); ++k) {
+			st.m = m;
+			for (int k = 0; k < (1 << m); ++k) {
 				uint32_t x = full;
-				for (int i = 0; i < st.m; ++i)
+				for (int i = 0; i < m; ++i)
 					if ((k >> i) & 1)
 						x &= rev(out[i]);
 				st.inter[k] = x;
 			}
 			launch_lookahead_down(D, st, sc, bsc, blocks, stream_);
-			BGT_HIP_CHECK(hipMemcpyAsync(hs.data(), bsc, sizeof(double) * blocks, hipMemcpyDeviceToHost, stream_));
+			out.push_back(pick_on_device(m));
+		} else {
+			host_scores.resize(states);
+			BGT_HIP_CHECK(
+				hipMemcpyAsync(selected_, out.data(), sizeof(uint32_t) * out.size(), hipMemcpyHostToDevice, stream_));
+			launch_lookahead(single(), n_, storage_, selected_, m, scores_, stream_);
+			BGT_HIP_CHECK(
+				hipMemcpyAsync(host_scores.data(), scores_, sizeof(double) * states, hipMemcpyDeviceToHost, stream_));
 			sync();
 			double best = INFINITY;
-			for (double v : hs)
-				best = std::min(best, v);
-			const double limit = best + 2.0 * lookahead_score_error(n_, st.m);
-			launch_lookahead_pick(sc, bsc, n_, limit, bex, blocks, stream_);
-			BGT_HIP_CHECK(hipMemcpyAsync(he.data(), bex, sizeof(uint32_t) * blocks, hipMemcpyDeviceToHost, stream_));
-			sync();
-			uint32_t best_c = 0xFFFFFFFFu;
-			for (uint32_t v : he)
-				best_c = std::min(best_c, v);
-			if (best_c == 0xFFFFFFFFu || best_c == 0u)
+			for (uint32_t c = 1; c < states; ++c)
+				best = std::min(best, host_scores[c]);
+			const double limit = best + 2.0 * lookahead_score_error(n_, m);
+			uint32_t best_c = 1;
+			while (best_c < states && !(host_scores[best_c] <= limit))
+				++best_c;
+			if (best_c == states)
 				throw Error(BGT_HIP_INTERNAL, "lookahead found no candidate");
 			out.push_back(best_c);
 		}
-		device_free(bex, stream_);
-		device_free(bsc, stream_);
-		device_free(sc, stream_);
-		device_free(D, stream_);
-		return out;
-	}
-	std::vector<double> scores(1ull << n_);
-	for (int step = 1; step < steps; ++step) {
-		const int m = (int)out.size();
-		BGT_HIP_CHECK(hipMemcpyAsync(selected_, out.data(), sizeof(uint32_t) * out.size(), hipMemcpyHostToDevice, stream_));
-		launch_lookahead(single(), n_, storage_, selected_, m, scores_, stream_);
-		BGT_HIP_CHECK(hipMemcpyAsync(scores.data(), scores_, sizeof(double) * scores.size(), hipMemcpyDeviceToHost, stream_));
-		sync();
-		double best = INFINITY;
-		for (uint32_t c = 1; c < (1u << n_); ++c)
-			best = std::min(best, scores[c]);
-		const double limit = best + 2.0 * lookahead_score_error(n_, m);
-		uint32_t best_c = 1;
-		while (best_c < (1u << n_) && !(scores[best_c] <= limit))
-			++best_c;
-		if (best_c == (1u << n_))
-			throw Error(BGT_HIP_INTERNAL, "lookahead found no candidate");
-		out.push_back(best_c);
 	}
 	return out;
 }

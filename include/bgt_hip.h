@@ -209,12 +209,17 @@ int bgt_hip_lattice_project(bgt_hip_lattice *source, const bgt_hip_shrink_plan *
 							bgt_hip_lattice **out);
 
 /* k-step Op-BHA lookahead (lattice_api.cpp:116-168) in the reference's PUBLIC coordinates
- * (binary lattices only; steps <= 5, status 3 beyond). The first pick is the Op-BHA
- * selection; each later one scores every candidate by the L1 distance of its 2^(m+1) group
- * masses from uniform, from down-set sums on the device (two 2^n FP64 work arrays for the
- * call). Candidates whose scores lie within twice the score's FP64 rounding bound of the
- * lowest go to the lowest public index (the reference decides such ties, frequent and often
- * exact, by its summation rounding). */
+ * (binary lattices only; steps <= 16, status 3 beyond: 2^16 groups at the last step). The
+ * first pick is the Op-BHA selection; each later one, after m earlier picks, scores every
+ * candidate by the L1 distance of its 2^(m+1) group masses from uniform. Up to m = 4 the
+ * masses of a candidate come from down-set sums built once (two 2^n FP64 work arrays for
+ * the call; below 13 subjects a pass over all states per candidate); from m = 5 on, from
+ * the down-set sums of the posterior masked to each of the 2^m classes of the earlier picks
+ * (2^m transforms per step, up to 64 classes of 2^n FP64 entries, at most 256 MiB, held at
+ * once). Departure from the reference's tie rule: candidates whose scores lie within twice
+ * the score's FP64 rounding bound, 2^(m+1) (2^(m+1) (n+1) + 3) 2^-53, of the lowest go to
+ * the lowest public index (the reference decides such ties, frequent and often exact, by
+ * its summation rounding). */
 int bgt_hip_lattice_select_experiments(bgt_hip_lattice *lattice, int steps, uint32_t *out);
 
 /* One fused round of the adaptive loop: update(experiment, response), atom marginals of the

@@ -1,7 +1,9 @@
 """GPU parity of the k-step Op-BHA lookahead (lattice_api.cpp:116-168).
 
-Below 13 subjects the library scores every candidate by a pass over all states; from 13 on
-it builds down-set sums once and scores each candidate from 2^(m+1) of them. At every step,
+Up to four earlier picks, below 13 subjects the library scores every candidate by a pass over
+all states; from 13 on it builds down-set sums once and scores each candidate from 2^(m+1) of
+them. From five earlier picks on (6..16 steps) it sums, per class of states under the earlier
+picks, the down-set sums of the posterior masked to that class. At every step,
 given the device's own earlier picks, the pick must be the lowest public candidate whose
 oracle score lies within 2 * tau of the oracle's lowest score (tau = G(G(n+1)+3) 2^-53 with
 G = 2^(m+1) groups: the FP64 rounding bound of one score). Where the oracle's scores single
@@ -51,6 +53,22 @@ def test_lookahead_against_oracle_scores(n, updates, steps):
     check_steps(lat, n, steps, orc.lookahead_scores)
 
 
+@pytest.mark.parametrize("n,updates,steps", [(6, 1, 8), (8, 2, 8), (10, 3, 7), (11, 0, 6), (12, 2, 6), (13, 1, 7)])
+def test_lookahead_many_steps_against_oracle_scores(n, updates, steps):
+    # from five earlier picks on, the class-decomposed scores: in LDS up to 12 subjects,
+    # through global down-set transforms from 13
+    lat = session(n, 400 + n + updates, updates)
+    print(f"lookahead n={n} updates={updates} steps={steps}", flush=True)
+    check_steps(lat, n, steps, orc.lookahead_scores)
+
+
+def test_lookahead_sixteen_steps():
+    # the largest lookahead: 2^16 groups at the last step
+    lat = session(4, 41, 1)
+    print("lookahead n=4 steps=16", flush=True)
+    check_steps(lat, 4, 16, orc.lookahead_scores)
+
+
 @pytest.mark.parametrize("n", [12, 14])
 def test_lookahead_fp32c_against_its_own_posterior(n):
     lat = session(n, 900 + n, 3, precision=bgt.Precision.float32_compensated)
@@ -62,7 +80,10 @@ def test_lookahead_fp32c_against_its_own_posterior(n):
                                                        (20, 6, 4, bgt.Precision.float64),
                                                        (20, 6, 4, bgt.Precision.float32_compensated),
                                                        (22, 5, 3, bgt.Precision.float64),
-                                                       (24, 4, 2, bgt.Precision.float32_compensated)])
+                                                       (24, 4, 2, bgt.Precision.float32_compensated),
+                                                       (16, 2, 7, bgt.Precision.float64),
+                                                       (16, 3, 6, bgt.Precision.float32_compensated),
+                                                       (18, 2, 6, bgt.Precision.float64)])
 def test_lookahead_larger_lattices(n, updates, steps, precision):
     lat = session(n, 700 + n, updates, precision=precision)
     print(f"lookahead n={n} steps={steps} {precision}", flush=True)
@@ -87,7 +108,7 @@ def test_lookahead_edges():
     assert lat.select_experiments(0) == []
     assert lat.select_experiments(1) == [lat.select_experiment(bgt.SelectorType.op_bha)]
     with pytest.raises(bgt.Unsupported):
-        lat.select_experiments(6)
+        lat.select_experiments(17)
     with pytest.raises(bgt.InvalidArgument):
         lat.select_experiments(-1)
     m = bgt.Lattice(ND, 4, seeded_prior(3, 8, 0.05, 0.2), variants=2)

@@ -35,11 +35,11 @@ def test_scores_reproduce_the_restated_greedy(n):
             assert 1 + int(np.argmin(sc[1:])) == picks[s]
 
 
-@pytest.mark.parametrize("n", [4, 7, 9, 11])
-def test_scores_match_the_transform_formulation(n):
+@pytest.mark.parametrize("n,steps", [(4, 8), (7, 8), (9, 7), (11, 5)])
+def test_scores_match_the_transform_formulation(n, steps):
     for post in posts(n, 4, 30 + n):
         sel = [orc.select_experiments(post, n, 1)[0]]
-        for m in range(1, 5):
+        for m in range(1, steps):
             ref = orc.lookahead_scores(post, n, sel)
             alt = transform_scores(post, n, sel)
             assert np.max(np.abs(ref[1:] - alt[1:])) <= tau(n, m)
