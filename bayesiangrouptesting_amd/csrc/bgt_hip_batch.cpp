@@ -219,7 +219,7 @@ void batch_step(std::vector<BatchItem> &items, int selector, double up, double l
 		} else if (atoms == kTileBits) {
 			entries = launch_zeta_final(dg, cnt, atoms, k, lat.storage(), objective, true, false, target, s);
 		} else {
-			launch_zeta_pass1(dg, cnt, atoms, lat.storage(), false, k, lat.dilution(), s);
+			launch_zeta_pass1(dg, cnt, atoms, lat.storage(), false, k, lat.dilution(), objective, s);
 			entries = launch_zeta_final(dg, cnt, atoms, k, lat.storage(), objective, false, false, target, s);
 		}
 		if (objective == kObjGather)

@@ -25,7 +25,19 @@ constexpr int kTileBits = 15;
 // the memory channels instead of landing on the same one. (A layout holding each pass-2
 // tile contiguously turns those reads into runs of 64 entries but pass 1's stores into
 // 16-byte pieces; at 2^28 states it made pass 1 slower by more than it saved in pass 2.)
+//
+// Blocked form (FP32, 28..30 atoms, the split pass 2): z is cut into column blocks of
+// 2^kZBlockBits columns (128 bytes) by all 2^(A-15) rows, each block contiguous and blocks
+// 2^(A-15+kZBlockBits) + kZBlockSkew entries apart: entry (row r, column c) sits at
+// (c >> 5) * pitch_b + (r << 5) + (c & 31). Pass 1 stores whole 128-byte lines (a wave's
+// 64 columns are two of them); the 8-column units of pass 2 read 32 bytes of each line of
+// one block, the block's four units side by side on one XCD. The skew keeps the 1024
+// blocks of a row off one memory channel (without it, pass 1's stores at a power-of-two
+// stride ran at a third of their speed). Measured at 2^28 states: pass 2 648 -> 574 us,
+// pass 1 +8 us.
 constexpr int kZSkewBytes = 128;
+constexpr int kZBlockBits = 5;
+constexpr uint32_t kZBlockSkew = 32; // entries (128 bytes of FP32)
 template <typename T>
 constexpr uint32_t z_pitch() { return (1u << kTileBits) + kZSkewBytes / sizeof(T); }
 inline uint64_t z_bytes(int atoms, size_t elem)
@@ -184,8 +196,11 @@ void launch_update(const Descs &descs, int count, uint64_t states, int k, int di
 				   Storage st, hipStream_t stream);
 // exact path: transform of x = y * inv (n >= kTileBits + 1): pass 1 over the low 15 bits.
 // defer_sum: leave the sum of its per-tile partials to a persistent pass 2 (fold below)
+// `objective` is the one of the final pass that reads this z (it decides z's layout)
 void launch_zeta_pass1(const Descs &descs, int count, int n, Storage st, bool fused_update,
-					   int k, int dilution, hipStream_t stream, bool defer_sum = false);
+					   int k, int dilution, int objective, hipStream_t stream, bool defer_sum = false);
+// z of an n-atom lattice read by a final pass with this objective is in the blocked form
+bool blocked_z(int n, Storage st, int objective);
 // pass 2 over bits 15..n-1 (or the whole transform when n == 15), with an objective.
 // fold > 0 (persistent pass 2 after a fused pass 1 with defer_sum): every workgroup sums
 // the `fold` per-tile partials itself, as the separate finalize launch would, and the first

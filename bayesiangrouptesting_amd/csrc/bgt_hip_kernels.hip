@@ -580,7 +580,7 @@ constexpr bool pass1_persistent()
 
 // One pass-1 tile from its loaded entries: the fused update (stored in place), the sweep
 // of bits 0..14, the intermediate z and the tile total.
-template <typename TP, typename TZ, bool FUSED, int LIK>
+template <typename TP, typename TZ, bool FUSED, int LIK, bool ZB>
 __device__ __forceinline__ void pass1_tile(const LatDesc &d, unsigned tile, const TP (&raw)[kTileRegs], int lane,
 										   int wave, double inv, double f_in, double f_out, const double *s_lik,
 										   double *lds)
@@ -607,16 +607,32 @@ __device__ __forceinline__ void pass1_tile(const LatDesc &d, unsigned tile, cons
 		v[j] = x;
 	}
 	zeta_tile(v, 0, lane, wave, lds);
-	TZ *zt = uniform_ptr(static_cast<TZ *>(d.z)) + (uint64_t)tile * z_pitch<TZ>();
-	const uint32_t loz = (lane_bits_f(lane) + ((uint32_t)wave << 6)) * (uint32_t)sizeof(TZ);
 	// FP64 z streams past the caches (measured faster in both passes); FP32 z stays cached,
 	// as pass 2 reads it next (streaming it moved time from pass 2 to pass 1)
+	if constexpr (ZB) {
+		// blocked form (FP32): row `tile`, column c = lane_bits_f + 64 wave + 1024 jlo + 16384 jhi
+		// at (c >> 5) * pitch_b + (tile << 5) + (c & 31); the lane offsets stay below 32
+		// pitches, 2^27 bytes at 30 atoms
+		static_assert(sizeof(TZ) == 4, "the blocked form is FP32");
+		const uint32_t lbf = lane_bits_f(lane);
+		const int atoms = __builtin_amdgcn_readfirstlane(d.atoms);
+		const uint64_t pitch_b = ((uint64_t)1 << (atoms - kTileBits + kZBlockBits)) + kZBlockSkew;
+		TZ *zt = uniform_ptr(static_cast<TZ *>(d.z)) + ((uint64_t)tile << kZBlockBits);
+		const uint32_t pitch_bytes = (uint32_t)pitch_b * (uint32_t)sizeof(TZ);
+		const uint32_t loz = (lbf & 31u) * (uint32_t)sizeof(TZ) + ((lbf >> 5) + ((uint32_t)wave << 1)) * pitch_bytes;
 #pragma unroll
-	for (int j = 0; j < kTileRegs; ++j) {
-		if constexpr (sizeof(TZ) == 8)
-			st_u_nt(zt + (((j & 15) << 10) | ((j >> 4) << 14)), loz, static_cast<TZ>(v[j]));
-		else
-			st_u(zt + (((j & 15) << 10) | ((j >> 4) << 14)), loz, static_cast<TZ>(v[j]));
+		for (int j = 0; j < kTileRegs; ++j)
+			st_u(uniform_ptr(zt + (uint64_t)(((j & 15) << 5) | ((j >> 4) << 9)) * pitch_b), loz, static_cast<TZ>(v[j]));
+	} else {
+		TZ *zt = uniform_ptr(static_cast<TZ *>(d.z)) + (uint64_t)tile * z_pitch<TZ>();
+		const uint32_t loz = (lane_bits_f(lane) + ((uint32_t)wave << 6)) * (uint32_t)sizeof(TZ);
+#pragma unroll
+		for (int j = 0; j < kTileRegs; ++j) {
+			if constexpr (sizeof(TZ) == 8)
+				st_u_nt(zt + (((j & 15) << 10) | ((j >> 4) << 14)), loz, static_cast<TZ>(v[j]));
+			else
+				st_u(zt + (((j & 15) << 10) | ((j >> 4) << 14)), loz, static_cast<TZ>(v[j]));
+		}
 	}
 	// after the full 15-bit sweep, tile index 0 (thread 0, register 0) holds the tile's total
 	if (threadIdx.x == 0)
@@ -629,7 +645,8 @@ __device__ __forceinline__ void pass1_tile(const LatDesc &d, unsigned tile, cons
 // b serves the contiguous tile range of XCD b % 8 in turn and loads its next tile while it
 // transforms the current one; the workgroup barriers of the transform wait for LDS only,
 // so those loads stay in flight across them.
-template <typename TP, typename TZ, bool FUSED, int LIK>
+// ZB: z in the blocked form (FP32 only).
+template <typename TP, typename TZ, bool FUSED, int LIK, bool ZB = false>
 __global__ __launch_bounds__(kTileThreads) void k_zeta_pass1(Descs descs)
 {
 	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
@@ -658,7 +675,7 @@ __global__ __launch_bounds__(kTileThreads) void k_zeta_pass1(Descs descs)
 		const unsigned tile = xcd_tile(blockIdx.x, tiles);
 		TP raw[kTileRegs];
 		pass1_load(d, tile, lane, wave, raw);
-		pass1_tile<TP, TZ, FUSED, LIK>(d, tile, raw, lane, wave, inv, f_in, f_out, s_lik, lds);
+		pass1_tile<TP, TZ, FUSED, LIK, ZB>(d, tile, raw, lane, wave, inv, f_in, f_out, s_lik, lds);
 		return;
 	}
 	if constexpr (pass1_persistent<TP, FUSED, LIK>()) {
@@ -676,7 +693,7 @@ __global__ __launch_bounds__(kTileThreads) void k_zeta_pass1(Descs descs)
 				cur[j] = next[j];
 			if (T + wg_per_xcd < last)
 				pass1_load(d, T + wg_per_xcd, lane, wave, next);
-			pass1_tile<TP, TZ, FUSED, LIK>(d, T, cur, lane, wave, inv, f_in, f_out, s_lik, lds);
+			pass1_tile<TP, TZ, FUSED, LIK, ZB>(d, T, cur, lane, wave, inv, f_in, f_out, s_lik, lds);
 		}
 	}
 }
@@ -1005,9 +1022,14 @@ __global__ __launch_bounds__(kTileThreads) void k_zeta_pass2(Descs descs, int ob
 	const unsigned per_xcd = units >> 3;
 	const unsigned xcd = blockIdx.x & 7u;
 	const unsigned first = xcd * per_xcd + (blockIdx.x >> 3), last = (xcd + 1) * per_xcd;
-	constexpr uint32_t pitch = z_pitch<TIN>();
+	// SPLIT reads z in the blocked form: rows of a column block are 2^kZBlockBits entries
+	// apart, blocks pitch_b apart; tile T is columns (T & (2^(5-LC) - 1)) << LC.. of block
+	// T >> (5 - LC)
+	static_assert(!SPLIT || (sizeof(TIN) == 4 && LC >= 1 && LC <= 3), "the split form is FP32, 28..30 atoms");
+	constexpr uint32_t pitch = SPLIT ? (1u << kZBlockBits) : z_pitch<TIN>();
 	// SPLIT: the upper half's rows start this many entries after the lower half's
 	constexpr size_t half_rows = (size_t)pitch << (kTileBits - LC);
+	constexpr uint64_t pitch_b = 2 * half_rows + kZBlockSkew;
 	const uint64_t lo = p2::thread_z_offset<LC, pitch>(lane, wave);
 	const uint32_t lf = p2::thread_state_offset<LC>(lane, wave);
 	const TIN *in = static_cast<const TIN *>(d.z);
@@ -1061,7 +1083,10 @@ __global__ __launch_bounds__(kTileThreads) void k_zeta_pass2(Descs descs, int ob
 		const bool upper = SPLIT && (((U & 1u) ^ ((T / (wg_per_xcd >> 1)) & 1u)) != 0u);
 		double v[kTileRegs];
 		{
-			const TIN *tb = inw + ((uint64_t)T << LC) + (upper ? half_rows : 0);
+			const uint64_t tile_at = SPLIT ? ((uint64_t)(T >> (kZBlockBits - LC)) * pitch_b +
+											  ((uint64_t)(T & ((1u << (kZBlockBits - LC)) - 1u)) << LC))
+										   : ((uint64_t)T << LC);
+			const TIN *tb = inw + tile_at + (upper ? half_rows : 0);
 			if (SPLIT && !upper) {
 #pragma unroll
 				for (int j = 0; j < kTileRegs; ++j)
@@ -2787,8 +2812,9 @@ void launch_update(const Descs &descs, int count, uint64_t states, int, int, Sto
 }
 
 void launch_zeta_pass1(const Descs &descs, int count, int n, Storage st, bool fused, int k, int dilution,
-					   hipStream_t stream, bool defer_sum)
+					   int objective, hipStream_t stream, bool defer_sum)
 {
+	const bool zblk = blocked_z(n, st, objective);
 	// FP32 storage: persistent workgroups (one per CU) that prefetch their next tile, once
 	// there are more tiles than CUs; otherwise one workgroup per tile
 	const unsigned tiles = 1u << (n - kTileBits);
@@ -2804,6 +2830,13 @@ void launch_zeta_pass1(const Descs &descs, int count, int n, Storage st, bool fu
 				hipLaunchKernelGGL((k_zeta_pass1<double, double, true, 0>), grid, dim3(kTileThreads), 0, stream, descs);
 			else
 				hipLaunchKernelGGL((k_zeta_pass1<double, double, false, 0>), grid, dim3(kTileThreads), 0, stream, descs);
+		} else if (zblk) {
+			if (fused && binary_nd)
+				hipLaunchKernelGGL((k_zeta_pass1<float, float, true, 1, true>), grid, dim3(kTileThreads), 0, stream, descs);
+			else if (fused)
+				hipLaunchKernelGGL((k_zeta_pass1<float, float, true, 0, true>), grid, dim3(kTileThreads), 0, stream, descs);
+			else
+				hipLaunchKernelGGL((k_zeta_pass1<float, float, false, 0, true>), grid, dim3(kTileThreads), 0, stream, descs);
 		} else {
 			if (fused && binary_nd)
 				hipLaunchKernelGGL((k_zeta_pass1<float, float, true, 1>), grid, dim3(kTileThreads), 0, stream, descs);
@@ -2845,6 +2878,12 @@ bool persistent_pass2(int n, int objective, bool from_posterior)
 	return !from_posterior && n >= 21 && n <= 30 && (objective == kObjHalving || objective == kObjBinaryBbpa);
 }
 
+// the split pass 2 (FP32, 28..30 atoms, two-pass form) reads z in the blocked form
+bool blocked_z(int n, Storage st, int objective)
+{
+	return st == Storage::fp32c && n >= 28 && persistent_pass2(n, objective, false) && !three_pass_form(n, st);
+}
+
 // The final transform kernel of a selection (pass 2, or the single pass of 15..20 atoms),
 // without the merge of the workgroups' top-2 lists; returns its workgroup count. `collect`
 // (single pass of 15..20 atoms only) rescans the workgroups flagged by k_band_collect.
@@ -2875,8 +2914,9 @@ unsigned launch_final_kernel(const Descs &descs, int count, int n, Storage st, i
 	if (persistent_pass2(n, objective, from_posterior)) {
 		// persistent pass 2: one workgroup per CU (a multiple of 8, one share per XCD)
 		const dim3 grid(zeta_final_groups(n), count);
-		if (st == Storage::fp32c && n >= 28) {
-			// FP32 runs of 16 bytes or less: the split form, one atom smaller and twice as wide
+		if (blocked_z(n, st, objective)) {
+			// FP32 runs of 16 bytes or less: the split form, one atom smaller and twice as wide,
+			// on z in the blocked form
 			launch_checked("k_zeta_pass2", [&] {
 				const int sc = scaled ? 1 : 0;
 				if (n == 28)

@@ -1223,7 +1223,7 @@ bgt_hip_selection DeviceLattice::select(int selector)
 			collect_band(dz, entries, objective, true, false, target);
 	} else {
 		clock_.begin(stream_, true);
-		launch_zeta_pass1(dz, 1, atoms_, storage_, false, k_, dilution_, stream_);
+		launch_zeta_pass1(dz, 1, atoms_, storage_, false, k_, dilution_, objective, stream_);
 		clock_.end(stream_, "zeta_pass1");
 		clock_.begin(stream_);
 		entries = launch_zeta_final(dz, 1, atoms_, k_, storage_, objective, false, false, target, stream_);
@@ -1496,7 +1496,7 @@ bgt_hip_selection DeviceLattice::round(uint32_t experiment, uint32_t response, c
 	// the light clock brackets one transform pass per round, pass 1 and pass 2 in turn
 	const int light_pass = 1 + (int)(rounds_++ & 1u);
 	clock_.begin(stream_, light_pass == 1);
-	launch_zeta_pass1(ds, 1, atoms_, storage_, true, k_, dilution_, stream_, fold);
+	launch_zeta_pass1(ds, 1, atoms_, storage_, true, k_, dilution_, kObjHalving, stream_, fold);
 	clock_.end(stream_, "round_pass1");
 	commit_update();
 	ds.one.band_width = band_width(kSelBha); // error band of the updated posterior

@@ -148,7 +148,7 @@ void select_noalloc(const void *posterior, int subjects, int precision, int sele
 	} else if (atoms == kTileBits) {
 		launch_zeta_final(ds, 1, atoms, 1, st, objective, true, false, target, stream);
 	} else {
-		launch_zeta_pass1(ds, 1, atoms, st, false, 1, 0, stream);
+		launch_zeta_pass1(ds, 1, atoms, st, false, 1, 0, objective, stream);
 		launch_zeta_final(ds, 1, atoms, 1, st, objective, false, false, target, stream);
 	}
 	// the transform's own rounding (FP64 sums along chains of at most `atoms` additions) on

@@ -1,6 +1,7 @@
 """The three-pass transform of FP32C lattices of 26..30 atoms (pass M over state bits 15..24
 in place, pass T over the rest with the objective), against the oracle and against the
-two-pass form, and the autotuner that chooses between them.
+two-pass form (from 28 atoms the split pass 2 on the blocked layout of z), and the autotuner
+that chooses between them.
 
 Bars: as the FP32C bars elsewhere -- masses within 1e-6 of the oracle's (accurately summed),
 objectives within the FP32C error band, picks the oracle's unless the oracle ranks the two
@@ -35,9 +36,10 @@ def form():
     force(-1)
 
 
-@pytest.mark.parametrize("n,dilution", [(26, False), (27, True), (28, False)])
-def test_three_pass_selection_against_the_oracle(form, n, dilution):
-    form(1)
+@pytest.mark.parametrize("n,dilution,f", [(26, False, 1), (27, True, 1), (28, False, 1), (28, False, 0), (28, True, 0)])
+def test_three_pass_selection_against_the_oracle(form, n, dilution, f):
+    # f = 0 at 28 atoms: the two-pass split form on z in the blocked layout
+    form(f)
     pi = seeded_prior(700 + n, n, 0.005, 0.05)
     lat = bgt.Lattice(DL if dilution else ND, n, pi, precision=FP32C)
     t = bgt.DilutionTable(n, 0.05, 0.9) if dilution else None
