@@ -166,11 +166,15 @@ def _closed_form_best(q):
     return best
 
 
-@pytest.mark.parametrize("n", [29, 30])
-def test_largest_lattices_closed_form_fp32c(n):
+@pytest.mark.parametrize("n,precision", [(29, bgt.Precision.float32_compensated),
+                                         (30, bgt.Precision.float32_compensated),
+                                         (29, bgt.Precision.float64), (30, bgt.Precision.float64)])
+def test_largest_lattices_closed_form(n, precision):
+    # FP64 at 29 and 30 atoms: pass 2 on tiles of 2 and 1 columns, z offsets beyond 32 bits
     pi = seeded_prior(500 + n, n, 0.005, 0.05)
     q = 1.0 - pi  # probability that atom a is negative (bit set)
-    lat = bgt.Lattice(ND, n, pi, precision=bgt.Precision.float32_compensated)
+    print(f"closed form n={n} {precision}", flush=True)
+    lat = bgt.Lattice(ND, n, pi, precision=precision)
     sel = lat.select(bgt.SelectorType.op_bha)
     e = orc.reverse_bits(sel.experiment, n)
     me = float(np.prod([q[a] for a in range(n) if (e >> a) & 1]))
