@@ -514,6 +514,21 @@ def batch_round(lattices: Sequence["Lattice"], experiments: Sequence[Optional[in
     return out, masks
 
 
+def set_batch_graphs(enable: bool) -> bool:
+    """Replay batched steps from captured hipGraphs (one per launch structure, see
+    bgt_hip_batch_graphs); returns the former setting."""
+    prev = ctypes.c_int(0)
+    nat.check(nat.lib().bgt_hip_batch_graphs(1 if enable else 0, ctypes.byref(prev)))
+    return bool(prev.value)
+
+
+def batch_graph_counts() -> tuple[int, int]:
+    """(steps captured into a new graph, steps that replayed one) since the process started."""
+    c, r = ctypes.c_uint64(0), ctypes.c_uint64(0)
+    nat.check(nat.lib().bgt_hip_batch_graph_counts(ctypes.byref(c), ctypes.byref(r)))
+    return int(c.value), int(r.value)
+
+
 def exact_upset_mass(subjects: int, prior: Sequence[float], history: Sequence[tuple[int, int]],
                      experiments, dilution: Optional[DilutionTable] = None) -> "float | np.ndarray":
     """Exact FP64 upset mass(es) of a binary lattice after `history` updates (internal

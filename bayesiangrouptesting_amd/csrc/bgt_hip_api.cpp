@@ -24,9 +24,25 @@ namespace {
 
 thread_local std::string g_last_error;
 
+// Every entry point runs with the calling thread in relaxed stream-capture interaction: a
+// batched step that another thread captures into a graph (bgt_hip_batch_graphs) must not
+// make this thread's allocations and copies on its own streams fail. Nothing the library
+// does on one thread touches a stream another thread captures.
+struct RelaxedCaptureInteraction {
+	hipStreamCaptureMode prev = hipStreamCaptureModeRelaxed;
+	bool swapped = false;
+	RelaxedCaptureInteraction() { swapped = hipThreadExchangeStreamCaptureMode(&prev) == hipSuccess; }
+	~RelaxedCaptureInteraction()
+	{
+		if (swapped)
+			(void)hipThreadExchangeStreamCaptureMode(&prev);
+	}
+};
+
 template <typename F>
 int guarded(F &&f)
 {
+	RelaxedCaptureInteraction relaxed;
 	try {
 		f();
 		g_last_error.clear();
@@ -495,6 +511,20 @@ int bgt_hip_batch_round(bgt_hip_lattice **lattices, const uint32_t *experiments,
 		run_batch(lattices, experiments, responses, dilutions, count, selector, true, threshold_up, threshold_lo,
 				  out, positive, negative);
 	});
+}
+
+int bgt_hip_batch_graphs(int enable, int *previous)
+{
+	return guarded([&] {
+		const bool was = bgt_hip::set_batch_graphs(enable != 0);
+		if (previous)
+			*previous = was ? 1 : 0;
+	});
+}
+
+int bgt_hip_batch_graph_counts(unsigned long long *captures, unsigned long long *replays)
+{
+	return guarded([&] { bgt_hip::batch_graph_counts(captures, replays); });
 }
 
 int bgt_hip_tune(int use_dilution, int subjects, int variants, int precision, int selector, int force, int *form,

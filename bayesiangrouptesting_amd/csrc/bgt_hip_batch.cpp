@@ -1,16 +1,59 @@
 #include "bgt_hip_batch.hpp"
+#include "bgt_hip_resources.hpp"
 
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
+#include <list>
 #include <map>
 #include <tuple>
+#include <utility>
 #include <vector>
 
 namespace bgt_hip {
 
 namespace {
+
+std::atomic<int> g_graphs{0};               // replay captured steps (set_batch_graphs)
+std::atomic<unsigned long long> g_captures{0}, g_replays{0};
+
+// A step's launch structure: everything its kernel arguments and copies depend on (the
+// staging buffers, the upload and download sizes, and per launch group its kind, shape,
+// size, descriptor offset and transform form). Steps of equal signature launch the same
+// kernels with the same arguments; what differs between them travels in the uploaded block.
+using Signature = std::vector<long long>;
+
+// Instantiated graphs of recent step signatures, least recently used first out.
+struct GraphCache {
+	static constexpr size_t kMax = 16;
+	std::list<std::pair<Signature, hipGraphExec_t>> entries; // front: most recent
+
+	hipGraphExec_t find(const Signature &sig)
+	{
+		for (auto it = entries.begin(); it != entries.end(); ++it)
+			if (it->first == sig) {
+				entries.splice(entries.begin(), entries, it);
+				return entries.front().second;
+			}
+		return nullptr;
+	}
+	void insert(Signature sig, hipGraphExec_t ex)
+	{
+		entries.emplace_front(std::move(sig), ex);
+		while (entries.size() > kMax) {
+			(void)hipGraphExecDestroy(entries.back().second);
+			entries.pop_back();
+		}
+	}
+	void clear()
+	{
+		for (auto &e : entries)
+			(void)hipGraphExecDestroy(e.second);
+		entries.clear();
+	}
+};
 
 // Per-thread staging: one stream, device buffers for descriptors and likelihood tables,
 // and the gathered result heads (device and pinned host). Grown on demand, reused.
@@ -20,9 +63,11 @@ struct Staging {
 	size_t dev_bytes = 0;
 	char *host = nullptr; // pinned: descriptors + tables on the way in, heads on the way out
 	size_t host_bytes = 0;
+	GraphCache graphs; // graphs name the buffers above: cleared whenever they move
 
 	~Staging()
 	{
+		graphs.clear();
 		if (dev)
 			(void)hipFree(dev);
 		if (host)
@@ -32,8 +77,13 @@ struct Staging {
 	}
 	void reserve(size_t dbytes, size_t hbytes)
 	{
+		if (stream && dbytes <= dev_bytes && hbytes <= host_bytes)
+			return;
+		std::lock_guard<std::mutex> gate(capture_gate());
 		if (!stream)
 			BGT_HIP_CHECK(hipStreamCreate(&stream));
+		if (dbytes > dev_bytes || hbytes > host_bytes)
+			graphs.clear();
 		if (dbytes > dev_bytes) {
 			if (dev)
 				BGT_HIP_CHECK(hipFree(dev));
@@ -68,6 +118,16 @@ ShapeKey shape_of(const DeviceLattice &l)
 }
 
 } // namespace
+
+bool set_batch_graphs(bool on) { return g_graphs.exchange(on ? 1 : 0) != 0; }
+
+void batch_graph_counts(unsigned long long *captures, unsigned long long *replays)
+{
+	if (captures)
+		*captures = g_captures.load();
+	if (replays)
+		*replays = g_replays.load();
+}
 
 void batch_step(std::vector<BatchItem> &items, int selector, double up, double lo, std::vector<BatchResult> &out)
 {
@@ -175,63 +235,111 @@ void batch_step(std::vector<BatchItem> &items, int selector, double up, double l
 	fill(gs, ds);
 	std::memcpy(h + o_all, ds.data(), sizeof(LatDesc) * count);
 	hipStream_t s = st.stream;
-	BGT_HIP_CHECK(hipMemcpyAsync(d, h, o_in, hipMemcpyHostToDevice, s));
-	auto descs_at = [&](size_t offset) {
-		Descs x;
-		std::memset(&x, 0, sizeof(x));
-		x.many = reinterpret_cast<const LatDesc *>(d + offset);
-		return x;
+	// the step's copies and launches, on the stream or into a graph being captured from it
+	auto enqueue = [&] {
+		BGT_HIP_CHECK(hipMemcpyAsync(d, h, o_in, hipMemcpyHostToDevice, s));
+		auto descs_at = [&](size_t offset) {
+			Descs x;
+			std::memset(&x, 0, sizeof(x));
+			x.many = reinterpret_cast<const LatDesc *>(d + offset);
+			return x;
+		};
+		// 1. updates (x L and the denominator)
+		for (auto &g : gu) {
+			const DeviceLattice &lat = *items[g.second.idx[0]].lat;
+			launch_update(descs_at(g.second.offset), (int)g.second.idx.size(), lat.states(), lat.variants(),
+						  lat.dilution(), lat.storage(), s);
+		}
+		// 2. atom marginals of the updated posteriors
+		for (auto &g : gc) {
+			const DeviceLattice &lat = *items[g.second.idx[0]].lat;
+			launch_atom_masses(descs_at(g.second.offset), (int)g.second.idx.size(), lat.atoms(), lat.storage(), s);
+		}
+		// 3. selection, as DeviceLattice::select does for one lattice
+		for (auto &g : gs) {
+			const DeviceLattice &lat = *items[g.second.idx[0]].lat;
+			const int cnt = (int)g.second.idx.size();
+			const Descs dg = descs_at(g.second.offset);
+			const int sel = DeviceLattice::resolve_selector(selector, lat.variants());
+			if (sel == kSelBrute) {
+				launch_brute_force(dg, cnt, lat.subjects(), lat.variants(), lat.storage(), s);
+				continue;
+			}
+			const int k = lat.variants();
+			const int objective = sel == kSelBha ? kObjHalving : (k == 1 ? kObjBinaryBbpa : kObjGather);
+			const double target = sel == kSelBha ? 0.5 : 1.0 / double(1 << k);
+			const int atoms = lat.atoms();
+			if (objective == kObjGather && bbpa_hist_path(lat.subjects(), k)) {
+				// as DeviceLattice::select: the feature tables from one histogram pass
+				launch_bbpa_hist(dg, cnt, lat.subjects(), k, lat.storage(), false, s);
+				launch_bbpa_score(dg, cnt, lat.subjects(), k, target, s);
+				continue;
+			}
+			unsigned entries = 0;
+			if (atoms <= kSmallBits) {
+				entries = launch_zeta_small(dg, cnt, atoms, k, lat.storage(), objective, target, s);
+			} else if (atoms == kTileBits) {
+				entries = launch_zeta_final(dg, cnt, atoms, k, lat.storage(), objective, true, false, target, s);
+			} else {
+				launch_zeta_pass1(dg, cnt, atoms, lat.storage(), false, k, lat.dilution(), objective, s);
+				entries = launch_zeta_final(dg, cnt, atoms, k, lat.storage(), objective, false, false, target, s);
+			}
+			if (objective == kObjGather)
+				launch_bbpa_score(dg, cnt, lat.subjects(), k, target, s);
+			else if (k == 1 && atoms > kSmallBits)
+				launch_band_collect(dg, cnt, entries, atoms, k, lat.storage(), objective, atoms == kTileBits, false,
+									target, s);
+		}
+		// 4. one copy brings every head back
+		launch_gather_heads(reinterpret_cast<const LatDesc *>(d + o_all), (int)count,
+							reinterpret_cast<unsigned char *>(d + o_heads), s);
+		BGT_HIP_CHECK(hipMemcpyAsync(h + o_heads, d + o_heads, kHeadBytes * count, hipMemcpyDeviceToHost, s));
 	};
-	// 1. updates (x L and the denominator)
-	for (auto &g : gu) {
-		const DeviceLattice &lat = *items[g.second.idx[0]].lat;
-		launch_update(descs_at(g.second.offset), (int)g.second.idx.size(), lat.states(), lat.variants(),
-					  lat.dilution(), lat.storage(), s);
-	}
-	// 2. atom marginals of the updated posteriors
-	for (auto &g : gc) {
-		const DeviceLattice &lat = *items[g.second.idx[0]].lat;
-		launch_atom_masses(descs_at(g.second.offset), (int)g.second.idx.size(), lat.atoms(), lat.storage(), s);
-	}
-	// 3. selection, as DeviceLattice::select does for one lattice
-	for (auto &g : gs) {
-		const DeviceLattice &lat = *items[g.second.idx[0]].lat;
-		const int cnt = (int)g.second.idx.size();
-		const Descs dg = descs_at(g.second.offset);
-		const int sel = DeviceLattice::resolve_selector(selector, lat.variants());
-		if (sel == kSelBrute) {
-			launch_brute_force(dg, cnt, lat.subjects(), lat.variants(), lat.storage(), s);
-			continue;
+	if (g_graphs.load() != 0) {
+		Signature sig{(long long)count,
+					  selector,
+					  (long long)o_in,
+					  (long long)o_heads,
+					  (long long)reinterpret_cast<intptr_t>(d),
+					  (long long)reinterpret_cast<intptr_t>(h)};
+		int kind = 0;
+		for (auto *gm : {&gu, &gc, &gs}) {
+			for (auto &g : *gm) {
+				const DeviceLattice &lat = *items[g.second.idx[0]].lat;
+				sig.insert(sig.end(), {kind, std::get<0>(g.first), std::get<1>(g.first), std::get<2>(g.first),
+									   std::get<3>(g.first), (long long)g.second.idx.size(), (long long)g.second.offset,
+									   three_pass_form(lat.atoms(), lat.storage()) ? 1 : 0});
+			}
+			++kind;
 		}
-		const int k = lat.variants();
-		const int objective = sel == kSelBha ? kObjHalving : (k == 1 ? kObjBinaryBbpa : kObjGather);
-		const double target = sel == kSelBha ? 0.5 : 1.0 / double(1 << k);
-		const int atoms = lat.atoms();
-		if (objective == kObjGather && bbpa_hist_path(lat.subjects(), k)) {
-			// as DeviceLattice::select: the feature tables from one histogram pass
-			launch_bbpa_hist(dg, cnt, lat.subjects(), k, lat.storage(), false, s);
-			launch_bbpa_score(dg, cnt, lat.subjects(), k, target, s);
-			continue;
-		}
-		unsigned entries = 0;
-		if (atoms <= kSmallBits) {
-			entries = launch_zeta_small(dg, cnt, atoms, k, lat.storage(), objective, target, s);
-		} else if (atoms == kTileBits) {
-			entries = launch_zeta_final(dg, cnt, atoms, k, lat.storage(), objective, true, false, target, s);
+		hipGraphExec_t ex = st.graphs.find(sig);
+		if (ex == nullptr) {
+			// no allocation on any thread while the capture is open (see capture_gate)
+			std::lock_guard<std::mutex> gate(capture_gate());
+			// relaxed capture: other threads (lattice construction, say) keep allocating; the
+			// captured region itself makes stream operations on `s` only
+			BGT_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+			hipGraph_t graph = nullptr;
+			try {
+				enqueue();
+			} catch (...) {
+				if (hipStreamEndCapture(s, &graph) == hipSuccess && graph != nullptr)
+					(void)hipGraphDestroy(graph);
+				throw;
+			}
+			BGT_HIP_CHECK(hipStreamEndCapture(s, &graph));
+			const hipError_t e = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+			(void)hipGraphDestroy(graph);
+			BGT_HIP_CHECK(e);
+			st.graphs.insert(std::move(sig), ex);
+			g_captures.fetch_add(1);
 		} else {
-			launch_zeta_pass1(dg, cnt, atoms, lat.storage(), false, k, lat.dilution(), objective, s);
-			entries = launch_zeta_final(dg, cnt, atoms, k, lat.storage(), objective, false, false, target, s);
+			g_replays.fetch_add(1);
 		}
-		if (objective == kObjGather)
-			launch_bbpa_score(dg, cnt, lat.subjects(), k, target, s);
-		else if (k == 1 && atoms > kSmallBits)
-			launch_band_collect(dg, cnt, entries, atoms, k, lat.storage(), objective, atoms == kTileBits, false,
-								target, s);
+		BGT_HIP_CHECK(hipGraphLaunch(ex, s));
+	} else {
+		enqueue();
 	}
-	// 4. one copy brings every head back
-	launch_gather_heads(reinterpret_cast<const LatDesc *>(d + o_all), (int)count,
-						reinterpret_cast<unsigned char *>(d + o_heads), s);
-	BGT_HIP_CHECK(hipMemcpyAsync(h + o_heads, d + o_heads, kHeadBytes * count, hipMemcpyDeviceToHost, s));
 	for (auto &it : items)
 		if (it.select)
 			it.lat->prepare_exact(); // on the host while the device works

@@ -35,4 +35,12 @@ struct BatchResult {
 void batch_step(std::vector<BatchItem> &items, int selector, double threshold_up, double threshold_lo,
 				std::vector<BatchResult> &out);
 
+// Graph replay of batched steps: with it on, a step's copies and launches are captured
+// into a hipGraph once per launch structure (the shape groups, their sizes and the staging
+// buffers) and replayed by steps of the same structure, with the step's own descriptors
+// uploaded by the graph's first copy. Returns the previous setting.
+bool set_batch_graphs(bool on);
+// steps captured into a new graph, and steps that replayed one, since the process started
+void batch_graph_counts(unsigned long long *captures, unsigned long long *replays);
+
 } // namespace bgt_hip

@@ -56,8 +56,15 @@ bool async_supported()
 
 } // namespace
 
+std::mutex &capture_gate()
+{
+	static std::mutex *m = new std::mutex();
+	return *m;
+}
+
 void *device_alloc(size_t bytes, hipStream_t stream)
 {
+	std::lock_guard<std::mutex> gate(capture_gate());
 	void *p = nullptr;
 	if (async_supported())
 		BGT_HIP_CHECK(hipMallocAsync(&p, bytes, stream));
@@ -70,6 +77,7 @@ void device_free(void *p, hipStream_t stream)
 {
 	if (p == nullptr)
 		return;
+	std::lock_guard<std::mutex> gate(capture_gate());
 	if (async_supported())
 		(void)hipFreeAsync(p, stream);
 	else
@@ -93,7 +101,10 @@ void *pinned_alloc(size_t bytes)
 	const size_t block = size_t(1) << c;
 	const size_t chunk = block < (size_t(1) << 16) ? (size_t(1) << 16) : block;
 	void *base = nullptr;
-	BGT_HIP_CHECK(hipHostMalloc(&base, chunk, hipHostMallocDefault));
+	{
+		std::lock_guard<std::mutex> gate(capture_gate());
+		BGT_HIP_CHECK(hipHostMalloc(&base, chunk, hipHostMallocDefault));
+	}
 	std::lock_guard<std::mutex> lk(r.m);
 	for (size_t off = block; off < chunk; off += block)
 		r.pinned_free[c].push_back(static_cast<char *>(base) + off);
@@ -121,6 +132,7 @@ hipStream_t stream_acquire()
 		}
 	}
 	hipStream_t s = nullptr;
+	std::lock_guard<std::mutex> gate(capture_gate());
 	BGT_HIP_CHECK(hipStreamCreate(&s));
 	return s;
 }

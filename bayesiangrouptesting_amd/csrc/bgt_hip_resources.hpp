@@ -5,6 +5,8 @@
 // tree creates and retires lattices continually, so they come from here.
 #pragma once
 
+#include <mutex>
+
 #include <hip/hip_runtime_api.h>
 
 #include <cstddef>
@@ -14,6 +16,11 @@ namespace bgt_hip {
 // Device memory ordered on `stream` (hipMallocAsync on the device's default pool, which
 // keeps freed memory for re-use; plain hipMalloc where stream-ordered allocation is not
 // supported). The allocation is usable by work on `stream`; other streams must wait for it.
+// Held while a thread captures a batched step into a graph, and by the allocation
+// helpers below: the runtime refuses stream-ordered allocation on any stream while a
+// capture is in progress, so allocations on other threads wait for the capture to end.
+std::mutex &capture_gate();
+
 void *device_alloc(size_t bytes, hipStream_t stream);
 // Release after the work already queued on `stream`. Whoever frees must make sure no
 // other stream still uses the memory.

@@ -278,6 +278,20 @@ int bgt_hip_batch_round(bgt_hip_lattice **lattices, const uint32_t *experiments,
 						int selector, double threshold_up, double threshold_lo,
 						bgt_hip_selection *out, uint32_t *positive, uint32_t *negative);
 
+/* Graph replay of batched steps (bgt_hip_batch_step, bgt_hip_batch_round), off by default.
+ * With enable != 0 a step's copies and launches are captured into a hipGraph once per
+ * launch structure (the shape groups and their sizes, the selector and the staging
+ * buffers) and replayed by later steps of the same structure; the step's own descriptors,
+ * experiments and likelihood tables reach the device through the graph's first copy, so the
+ * results equal the launched step's. Up to 16 graphs per calling thread are kept (least
+ * recently used out). previous (may be NULL) receives the former setting. No reference
+ * counterpart (the reference launches every kernel per call). */
+int bgt_hip_batch_graphs(int enable, int *previous);
+
+/* Batched steps that were captured into a new graph, and steps that replayed one, since the
+ * process started (either may be NULL). */
+int bgt_hip_batch_graph_counts(unsigned long long *captures, unsigned long long *replays);
+
 /* ---- autotuning ---- */
 
 /* Measure once per KernelSpec key (include/bgt/types.hpp:132-181) which launch form of the
