@@ -109,12 +109,34 @@ Staging &staging()
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
-// Launch groups: lattices of one shape share grids and templates.
-using ShapeKey = std::tuple<int, int, int, int>; // storage, dilution, subjects, variants
+// Launch groups. A shape group holds lattices of one shape, which share grids and kernel
+// templates. A ragged group (Descs::ragged) holds lattices of different shapes whose kernels
+// take their sizes and workgroup counts from their descriptors; its grid is sized for its
+// largest lattice. Updates and marginals form one ragged group per storage; selections one
+// per transform regime (whole transform in LDS, one tile, one to 32 tiles), storage,
+// objective and variant count, and a shape group beyond that (one-pass BBPA tables, brute
+// force, the persistent pass 2), where templates depend on the shape.
+enum Regime : int { kShape = 0, kSmall = 1, kOneTile = 2, kTiles = 3, kAll = 4 };
+using GroupKey = std::tuple<int, int, int, int, int>; // regime, storage, dilution | objective, subjects, variants
 
-ShapeKey shape_of(const DeviceLattice &l)
+GroupKey shape_key(const DeviceLattice &l)
 {
-	return {static_cast<int>(l.storage()), l.dilution() ? 1 : 0, l.subjects(), l.variants()};
+	return {kShape, static_cast<int>(l.storage()), l.dilution() ? 1 : 0, l.subjects(), l.variants()};
+}
+
+GroupKey all_key(const DeviceLattice &l) { return {kAll, static_cast<int>(l.storage()), 0, 0, 0}; }
+
+int objective_of(int sel, int k) { return sel == kSelBha ? kObjHalving : (k == 1 ? kObjBinaryBbpa : kObjGather); }
+
+GroupKey select_key(const DeviceLattice &l, int selector)
+{
+	const int sel = DeviceLattice::resolve_selector(selector, l.variants());
+	const int k = l.variants(), atoms = l.atoms();
+	const int objective = objective_of(sel, k);
+	if (sel == kSelBrute || (objective == kObjGather && bbpa_hist_path(l.subjects(), k)) || atoms > kRaggedMaxAtoms)
+		return shape_key(l);
+	const int regime = atoms <= kSmallBits ? kSmall : (atoms == kTileBits ? kOneTile : kTiles);
+	return {regime, static_cast<int>(l.storage()), objective, 0, k};
 }
 
 } // namespace
@@ -185,21 +207,29 @@ void batch_step(std::vector<BatchItem> &items, int selector, double up, double l
 		ds[i].band_width =
 			it.select ? it.lat->band_width(DeviceLattice::resolve_selector(selector, it.lat->variants())) : 0.0;
 	}
-	// launch groups: identically shaped lattices share grids and templates; each group gets
-	// its own contiguous block of device descriptors
+	// launch groups (see GroupKey); each group gets its own contiguous block of device
+	// descriptors
 	struct Group {
 		std::vector<size_t> idx;
-		size_t offset = 0; // byte offset of its descriptor block in the staging buffer
+		size_t offset = 0;  // byte offset of its descriptor block in the staging buffer
+		int max_atoms = 0;  // its largest lattice (ragged groups: sizes the grid)
+		int max_subjects = 0;
 	};
-	std::map<ShapeKey, Group> gu, gc, gs;
+	std::map<GroupKey, Group> gu, gc, gs;
+	auto join = [&](std::map<GroupKey, Group> &gm, const GroupKey &key, size_t i) {
+		Group &g = gm[key];
+		g.idx.push_back(i);
+		g.max_atoms = std::max(g.max_atoms, items[i].lat->atoms());
+		g.max_subjects = std::max(g.max_subjects, items[i].lat->subjects());
+	};
 	for (size_t i = 0; i < count; ++i) {
-		const ShapeKey key = shape_of(*items[i].lat);
+		const DeviceLattice &lat = *items[i].lat;
 		if (items[i].update)
-			gu[key].idx.push_back(i);
+			join(gu, all_key(lat), i);
 		if (items[i].classify)
-			gc[key].idx.push_back(i);
+			join(gc, all_key(lat), i);
 		if (items[i].select)
-			gs[key].idx.push_back(i);
+			join(gs, select_key(lat, selector), i);
 	}
 	size_t off = 0;
 	const size_t o_lik = off;
@@ -223,7 +253,7 @@ void batch_step(std::vector<BatchItem> &items, int selector, double up, double l
 			std::memcpy(h + o_lik + sizeof(double) * kLikCapacity * i, liks[i], sizeof(double) * kLikCapacity);
 			du[i].lik = reinterpret_cast<double *>(d + o_lik) + i * kLikCapacity;
 		}
-	auto fill = [&](std::map<ShapeKey, Group> &gm, const std::vector<LatDesc> &src) {
+	auto fill = [&](std::map<GroupKey, Group> &gm, const std::vector<LatDesc> &src) {
 		for (auto &g : gm) {
 			LatDesc *dst = reinterpret_cast<LatDesc *>(h + g.second.offset);
 			for (size_t q = 0; q < g.second.idx.size(); ++q)
@@ -238,37 +268,42 @@ void batch_step(std::vector<BatchItem> &items, int selector, double up, double l
 	// the step's copies and launches, on the stream or into a graph being captured from it
 	auto enqueue = [&] {
 		BGT_HIP_CHECK(hipMemcpyAsync(d, h, o_in, hipMemcpyHostToDevice, s));
-		auto descs_at = [&](size_t offset) {
+		auto descs_at = [&](size_t offset, bool ragged) {
 			Descs x;
 			std::memset(&x, 0, sizeof(x));
 			x.many = reinterpret_cast<const LatDesc *>(d + offset);
+			x.ragged = ragged ? 1 : 0;
 			return x;
 		};
 		// 1. updates (x L and the denominator)
 		for (auto &g : gu) {
 			const DeviceLattice &lat = *items[g.second.idx[0]].lat;
-			launch_update(descs_at(g.second.offset), (int)g.second.idx.size(), lat.states(), lat.variants(),
-						  lat.dilution(), lat.storage(), s);
+			launch_update(descs_at(g.second.offset, true), (int)g.second.idx.size(), 1ull << g.second.max_atoms,
+						  lat.variants(), lat.dilution(), lat.storage(), s);
 		}
 		// 2. atom marginals of the updated posteriors
 		for (auto &g : gc) {
 			const DeviceLattice &lat = *items[g.second.idx[0]].lat;
-			launch_atom_masses(descs_at(g.second.offset), (int)g.second.idx.size(), lat.atoms(), lat.storage(), s);
+			launch_atom_masses(descs_at(g.second.offset, true), (int)g.second.idx.size(), g.second.max_atoms,
+							   lat.storage(), s);
 		}
-		// 3. selection, as DeviceLattice::select does for one lattice
+		// 3. selection, as DeviceLattice::select does for one lattice (a ragged group's sizes
+		// are those of its largest lattice; each lattice takes its own from its descriptor)
 		for (auto &g : gs) {
 			const DeviceLattice &lat = *items[g.second.idx[0]].lat;
 			const int cnt = (int)g.second.idx.size();
-			const Descs dg = descs_at(g.second.offset);
+			const bool ragged = std::get<0>(g.first) != kShape;
+			const Descs dg = descs_at(g.second.offset, ragged);
 			const int sel = DeviceLattice::resolve_selector(selector, lat.variants());
 			if (sel == kSelBrute) {
 				launch_brute_force(dg, cnt, lat.subjects(), lat.variants(), lat.storage(), s);
 				continue;
 			}
 			const int k = lat.variants();
-			const int objective = sel == kSelBha ? kObjHalving : (k == 1 ? kObjBinaryBbpa : kObjGather);
+			const int objective = objective_of(sel, k);
 			const double target = sel == kSelBha ? 0.5 : 1.0 / double(1 << k);
-			const int atoms = lat.atoms();
+			const int atoms = g.second.max_atoms;
+			const int subjects = g.second.max_subjects;
 			if (objective == kObjGather && bbpa_hist_path(lat.subjects(), k)) {
 				// as DeviceLattice::select: the feature tables from one histogram pass
 				launch_bbpa_hist(dg, cnt, lat.subjects(), k, lat.storage(), false, s);
@@ -285,7 +320,7 @@ void batch_step(std::vector<BatchItem> &items, int selector, double up, double l
 				entries = launch_zeta_final(dg, cnt, atoms, k, lat.storage(), objective, false, false, target, s);
 			}
 			if (objective == kObjGather)
-				launch_bbpa_score(dg, cnt, lat.subjects(), k, target, s);
+				launch_bbpa_score(dg, cnt, subjects, k, target, s);
 			else if (k == 1 && atoms > kSmallBits)
 				launch_band_collect(dg, cnt, entries, atoms, k, lat.storage(), objective, atoms == kTileBits, false,
 									target, s);
@@ -307,7 +342,8 @@ void batch_step(std::vector<BatchItem> &items, int selector, double up, double l
 			for (auto &g : *gm) {
 				const DeviceLattice &lat = *items[g.second.idx[0]].lat;
 				sig.insert(sig.end(), {kind, std::get<0>(g.first), std::get<1>(g.first), std::get<2>(g.first),
-									   std::get<3>(g.first), (long long)g.second.idx.size(), (long long)g.second.offset,
+									   std::get<3>(g.first), std::get<4>(g.first), (long long)g.second.idx.size(),
+									   (long long)g.second.offset, g.second.max_atoms, g.second.max_subjects,
 									   three_pass_form(lat.atoms(), lat.storage()) ? 1 : 0});
 			}
 			++kind;

@@ -1,10 +1,11 @@
 // Batched steps over many device-resident lattices: the continuous-batching core of the
 // serving loop and of the level-batched tree. One step applies an (experiment, response)
 // update to each lattice that has one, takes atom marginals and classifies, and selects the
-// next experiment, with one launch per kernel per group of identically shaped lattices,
-// one upload of descriptors and one download of all results. Each lattice goes through
-// the same kernels, with the same launch shapes, as the per-lattice calls (update,
-// classify, select), so its results are identical whether it steps alone or in a batch.
+// next experiment, with one upload of descriptors and one download of all results.
+// Lattices of uneven (n, k) share launches wherever the kernels take their sizes from the
+// descriptors (updates, marginals, and the selections of at most 20 atoms); each lattice's
+// work is split into the same workgroups as in the per-lattice calls (update, classify,
+// select), so its results are identical whether it steps alone or in a batch.
 #pragma once
 
 #include "bgt_hip_lattice.hpp"
@@ -36,7 +37,7 @@ void batch_step(std::vector<BatchItem> &items, int selector, double threshold_up
 				std::vector<BatchResult> &out);
 
 // Graph replay of batched steps: with it on, a step's copies and launches are captured
-// into a hipGraph once per launch structure (the shape groups, their sizes and the staging
+// into a hipGraph once per launch structure (the launch groups, their sizes and the staging
 // buffers) and replayed by steps of the same structure, with the step's own descriptors
 // uploaded by the graph's first copy. Returns the previous setting.
 bool set_batch_graphs(bool on);

@@ -173,10 +173,46 @@ constexpr int kLikCapacity = kMaxVariants * (kMaxSubjects + 1);
 
 // Kernel argument: one lattice by value (single calls need no host-to-device copy) or a
 // device array of `count` lattices (batched launches, blockIdx.y selects the lattice).
+// ragged (with `many`): the lattices may differ in shape. The grid is sized for the largest;
+// each lattice's kernels take its own workgroup count (the functions below, the same counts
+// its per-lattice launches use, so its work is split identically) and their sizes from its
+// descriptor, and the workgroups past that count return at once.
 struct Descs {
 	LatDesc one;
 	double one_lik[kLikCapacity];
 	const LatDesc *many;
+	int ragged;
+};
+
+// Workgroup counts per lattice of the launches a ragged batch merges.
+// k_update: 256 threads x 16 states per workgroup, at most 2048 workgroups
+constexpr unsigned update_groups(int atoms)
+{
+	const uint64_t g = ((1ull << atoms) + 4095ull) / 4096ull;
+	return g < 1ull ? 1u : (g > 2048ull ? 2048u : (unsigned)g);
+}
+// k_atom_masses: chunks of 2^mass_chunk_bits states
+constexpr int mass_chunk_bits(int atoms) { return atoms < 14 ? atoms : 14; }
+constexpr unsigned mass_groups(int atoms) { return 1u << (atoms - mass_chunk_bits(atoms)); }
+// one workgroup per 2^15-state tile (k_zeta_pass1 and k_zeta_final up to 20 atoms)
+constexpr unsigned tile_groups(int atoms) { return atoms <= kTileBits ? 1u : 1u << (atoms - kTileBits); }
+// k_bbpa_score: one thread per experiment of n subjects, at most 2^20 workgroups of 256
+constexpr unsigned score_groups(int n)
+{
+	const uint64_t g = ((1ull << n) + 255ull) / 256ull;
+	return g > (1ull << 20) ? (1u << 20) : (unsigned)g;
+}
+// Largest lattice of a ragged transform group (pass 1, the final pass, the band rescan):
+// above it the pass-1 grid depends on the compute units and pass 2 on the atom count.
+constexpr int kRaggedMaxAtoms = 20;
+// What a finalize kernel's `count` (the producing launch's workgroups) is for each lattice
+// of a ragged launch.
+enum CountRule : int {
+	kCountGiven = 0, // the argument (shape-uniform launches)
+	kCountUpdate = 1, // update_groups(atoms)
+	kCountTiles = 2,  // tile_groups(atoms)
+	kCountScore = 3,  // score_groups(n)
+	kCountMasses = 4, // mass_groups(atoms)
 };
 
 // Host-side error type carrying a provider status code.
@@ -189,7 +225,8 @@ void check_hip(hipError_t e, const char *what);
 #define BGT_HIP_CHECK(expr) ::bgt_hip::check_hip((expr), #expr)
 
 // ---- kernel launchers (bgt_hip_kernels.hip). `descs` is a device array of `count` LatDesc,
-// all of the same (n, k). Every launcher is asynchronous on `stream`. ----
+// all of the same (n, k), or with descs.ragged, of uneven shapes whose largest sets the sizes
+// passed here. Every launcher is asynchronous on `stream`. ----
 void launch_prior(const Descs &descs, int count, int atoms, Storage st, const double *pi_pairs,
 				  uint64_t states, hipStream_t stream);
 void launch_update(const Descs &descs, int count, uint64_t states, int k, int dilution,

@@ -534,6 +534,11 @@ __global__ __launch_bounds__(256) void k_update(Descs descs, uint64_t states)
 	// lattice's posterior (update_probs, the out-of-place form the tree uses,
 	// lattice_base.cpp:210-228).
 	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
+	const unsigned groups = descs.ragged ? update_groups(d.atoms) : gridDim.x;
+	if (blockIdx.x >= groups)
+		return;
+	if (descs.ragged)
+		states = 1ull << d.atoms;
 	__shared__ double s_lik[kMaxVariants * (kMaxSubjects + 1)];
 	__shared__ double red[16];
 	load_lik_table(descs, d, s_lik);
@@ -541,7 +546,7 @@ __global__ __launch_bounds__(256) void k_update(Descs descs, uint64_t states)
 	gptr<const TP> src = G(static_cast<const TP *>(d.src_post ? d.src_post : d.post));
 	const double inv = d.src_post ? d.src_scal[d.src_inv] : d.scal[d.inv_in];
 	double acc = 0.0;
-	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+	const uint64_t stride = (uint64_t)groups * blockDim.x;
 	for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < states; s += stride) {
 		const double x = static_cast<double>(src[s]) * inv;
 		const TP y = static_cast<TP>(x * likelihood((uint32_t)s, d.e, d.n, d.k, d.len, d.dilution, s_lik));
@@ -646,10 +651,13 @@ __device__ __forceinline__ void pass1_tile(const LatDesc &d, unsigned tile, cons
 // transforms the current one; the workgroup barriers of the transform wait for LDS only,
 // so those loads stay in flight across them.
 // ZB: z in the blocked form (FP32 only).
+// ragged (lattices of at most kRaggedMaxAtoms atoms): one workgroup per tile of each lattice.
 template <typename TP, typename TZ, bool FUSED, int LIK, bool ZB = false>
 __global__ __launch_bounds__(kTileThreads) void k_zeta_pass1(Descs descs)
 {
 	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
+	if (descs.ragged && blockIdx.x >= tile_groups(d.atoms))
+		return;
 	__shared__ double lds[kTileLds];
 	__shared__ double s_lik[kMaxVariants * (kMaxSubjects + 1)];
 	if (FUSED)
@@ -709,12 +717,19 @@ __global__ __launch_bounds__(kTileThreads) void k_zeta_final(Descs descs, int lo
 															 int objective, int scaled, double target, int collect)
 {
 	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
+	// ragged: each lattice's own tiles and column split (15..20 atoms)
+	const unsigned tiles = descs.ragged ? (FROM_POST ? 1u : tile_groups(d.atoms)) : gridDim.x;
+	if (blockIdx.x >= tiles)
+		return;
+	if (descs.ragged && !FROM_POST) {
+		logc = 2 * kTileBits - d.atoms;
+		q0 = logc;
+	}
 	__shared__ double lds[kTileLds];
 	__shared__ Top2 red2[16];
 	__shared__ double red[16];
 	__shared__ double s_total;
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-	const unsigned tiles = gridDim.x;
 	const unsigned tile = xcd_tile(blockIdx.x, tiles);
 	if (collect && rescan_skip(d, tile))
 		return;
@@ -1470,6 +1485,8 @@ template <typename TP>
 __global__ __launch_bounds__(1024) void k_zeta_small(Descs descs, int atoms, int objective, double target)
 {
 	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
+	if (descs.ragged)
+		atoms = d.atoms;
 	__shared__ double a[1 << kSmallBits];
 	__shared__ Top2 red2[16];
 	__shared__ double red[16];
@@ -1737,6 +1754,12 @@ __global__ __launch_bounds__(1024) void k_bbpa_tables(Descs descs, int fused)
 __global__ __launch_bounds__(256) void k_bbpa_score(Descs descs, int ns, int k, double target)
 {
 	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
+	// ragged: lattices of one variant count (one target), each with its own subjects
+	if (descs.ragged) {
+		ns = d.n;
+		if (blockIdx.x >= score_groups(ns))
+			return;
+	}
 	__shared__ Top2 red2[16];
 	const uint32_t ec = 1u << ns;
 	const double total = d.scal[kScalTotal];
@@ -1826,8 +1849,13 @@ template <typename TP>
 __global__ __launch_bounds__(256) void k_atom_masses(Descs descs, int atoms)
 {
 	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
+	if (descs.ragged) {
+		atoms = d.atoms;
+		if (blockIdx.x >= mass_groups(atoms))
+			return;
+	}
 	__shared__ double red[16];
-	const int cb = atoms < 14 ? atoms : 14; // chunk bits
+	const int cb = mass_chunk_bits(atoms); // chunk bits
 	const int jb = cb > 8 ? cb - 8 : 0;     // register bits
 	const uint32_t chunk = blockIdx.x;
 	const uint64_t base = (uint64_t)chunk << cb;
@@ -2381,9 +2409,24 @@ __global__ __launch_bounds__(256) void k_tree_stats(const FlatNode *nodes, int n
 
 enum SumMode : int { kSumTotal = 0, kSumDenominator = 1, kSumUpset = 2 };
 
-__global__ __launch_bounds__(1024) void k_finalize_sum(Descs descs, unsigned count, int mode)
+// `count` of a ragged launch: each lattice's workgroups of the producing launch
+__device__ __forceinline__ unsigned ragged_count(const Descs &descs, const LatDesc &d, unsigned count, int rule)
+{
+	if (!descs.ragged)
+		return count;
+	switch (rule) {
+	case kCountUpdate: return update_groups(d.atoms);
+	case kCountTiles: return tile_groups(d.atoms);
+	case kCountScore: return score_groups(d.n);
+	case kCountMasses: return mass_groups(d.atoms);
+	default: return count;
+	}
+}
+
+__global__ __launch_bounds__(1024) void k_finalize_sum(Descs descs, unsigned count, int mode, int rule)
 {
 	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
+	count = ragged_count(descs, d, count, rule);
 	__shared__ double red[16];
 	double t = 0.0;
 	for (unsigned i = threadIdx.x; i < count; i += blockDim.x)
@@ -2402,9 +2445,10 @@ __global__ __launch_bounds__(1024) void k_finalize_sum(Descs descs, unsigned cou
 	}
 }
 
-__global__ __launch_bounds__(1024) void k_finalize_top2(Descs descs, unsigned count)
+__global__ __launch_bounds__(1024) void k_finalize_top2(Descs descs, unsigned count, int rule)
 {
 	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
+	count = ragged_count(descs, d, count, rule);
 	__shared__ Top2 red2[16];
 	Top2 r;
 	top2_init(r);
@@ -2435,6 +2479,8 @@ __global__ __launch_bounds__(1024) void k_finalize_top2(Descs descs, unsigned co
 __global__ __launch_bounds__(256) void k_band_collect(Descs descs, unsigned entries)
 {
 	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
+	if (descs.ragged)
+		entries = 2u * tile_groups(d.atoms) * kTileThreads; // two per thread of the final pass
 	const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
 	Cand c;
 	c.d = INFINITY;
@@ -2730,6 +2776,10 @@ __global__ __launch_bounds__(256) void k_gather_heads(const LatDesc *many, unsig
 __global__ __launch_bounds__(1024) void k_finalize_masses(Descs descs, unsigned count, int atoms)
 {
 	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
+	if (descs.ragged) {
+		atoms = d.atoms;
+		count = mass_groups(atoms);
+	}
 	__shared__ double red[16];
 	for (int a = 0; a < atoms; ++a) {
 		double s = 0.0;
@@ -2807,7 +2857,8 @@ void launch_update(const Descs &descs, int count, uint64_t states, int, int, Sto
 			hipLaunchKernelGGL(k_update<double>, grid, dim3(256), 0, stream, descs, states);
 		else
 			hipLaunchKernelGGL(k_update<float>, grid, dim3(256), 0, stream, descs, states);
-		hipLaunchKernelGGL(k_finalize_sum, dim3(1, count), dim3(1024), 0, stream, descs, grid.x, (int)kSumDenominator);
+		hipLaunchKernelGGL(k_finalize_sum, dim3(1, count), dim3(1024), 0, stream, descs, grid.x, (int)kSumDenominator,
+						   (int)kCountUpdate);
 	});
 }
 
@@ -2847,7 +2898,7 @@ void launch_zeta_pass1(const Descs &descs, int count, int n, Storage st, bool fu
 		}
 		if (!(defer_sum && fused))
 			hipLaunchKernelGGL(k_finalize_sum, dim3(1, count), dim3(1024), 0, stream, descs, tiles,
-							   (int)(fused ? kSumDenominator : kSumTotal));
+							   (int)(fused ? kSumDenominator : kSumTotal), (int)kCountTiles);
 	});
 }
 
@@ -3003,7 +3054,7 @@ unsigned launch_zeta_final(const Descs &descs, int count, int n, int, Storage st
 		return raw_cap;
 	}
 	launch_checked("k_finalize_top2", [&] {
-		hipLaunchKernelGGL(k_finalize_top2, dim3(1, count), dim3(1024), 0, stream, descs, groups);
+		hipLaunchKernelGGL(k_finalize_top2, dim3(1, count), dim3(1024), 0, stream, descs, groups, (int)kCountTiles);
 	});
 	return entries;
 }
@@ -3059,7 +3110,7 @@ void launch_bbpa_score(const Descs &descs, int count, int n, int k, double targe
 	const dim3 grid(grid_for(1ull << n, 256, 1u << 20), count);
 	launch_checked("k_bbpa_score", [&] {
 		hipLaunchKernelGGL(k_bbpa_score, grid, dim3(256), 0, stream, descs, n, k, target);
-		hipLaunchKernelGGL(k_finalize_top2, dim3(1, count), dim3(1024), 0, stream, descs, grid.x);
+		hipLaunchKernelGGL(k_finalize_top2, dim3(1, count), dim3(1024), 0, stream, descs, grid.x, (int)kCountScore);
 	});
 }
 
@@ -3124,7 +3175,7 @@ void launch_brute_force(const Descs &descs, int count, int n, int k, Storage st,
 			hipLaunchKernelGGL((k_brute_force<double, KK>), grid, dim3(256), 0, stream, descs, n); \
 		else                                                                                        \
 			hipLaunchKernelGGL((k_brute_force<float, KK>), grid, dim3(256), 0, stream, descs, n);  \
-		hipLaunchKernelGGL(k_finalize_top2, dim3(1, count), dim3(1024), 0, stream, descs, grid.x); \
+		hipLaunchKernelGGL(k_finalize_top2, dim3(1, count), dim3(1024), 0, stream, descs, grid.x, 0); \
 		return;                                                                                     \
 	}
 		BGT_BF(1)
@@ -3158,7 +3209,7 @@ void launch_upset_mass(const Descs &descs, int count, int atoms, uint32_t e, Sto
 			hipLaunchKernelGGL(k_upset_mass<double>, grid, dim3(256), 0, stream, descs, atoms, e);
 		else
 			hipLaunchKernelGGL(k_upset_mass<float>, grid, dim3(256), 0, stream, descs, atoms, e);
-		hipLaunchKernelGGL(k_finalize_sum, dim3(1, count), dim3(1024), 0, stream, descs, grid.x, (int)kSumUpset);
+		hipLaunchKernelGGL(k_finalize_sum, dim3(1, count), dim3(1024), 0, stream, descs, grid.x, (int)kSumUpset, 0);
 	});
 }
 

@@ -5,8 +5,8 @@ experiment, receive its response, update, classify, until every atom is classifi
 test budget is used up. Sessions arrive over time (a Poisson process in the benchmark
 workload). The engine admits arrived sessions while their device footprint fits the HBM
 budget, and every engine step advances every active session by one round with a single
-batched call (bgt_hip_batch_round: one launch per kernel per group of identically shaped
-lattices, one descriptor upload, one result download). An admitted session's lattice is
+batched call (bgt_hip_batch_round: sessions of uneven (n, k) share each kernel's launch,
+one descriptor upload, one result download). An admitted session's lattice is
 built (allocation and prior kernel on its own stream) by a host worker thread while the
 current step runs on the GPU, and the session joins the first step after it is ready;
 finished sessions free their memory at once, so admitted work keeps flowing rather than

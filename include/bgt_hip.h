@@ -259,9 +259,12 @@ int bgt_hip_lattice_footprint(int use_dilution, int subjects, int variants, int 
 #define BGT_HIP_NO_UPDATE 0xFFFFFFFFu
 
 /* Apply one (experiment, response) update to each lattice (none where responses[i] is
- * BGT_HIP_NO_UPDATE), then select with `selector` on each: one launch per kernel per group
- * of identically shaped lattices, one descriptor upload and one result download for the
- * whole batch. Each lattice runs the same kernels with the same launch shapes as
+ * BGT_HIP_NO_UPDATE), then select with `selector` on each, one descriptor upload and one
+ * result download for the whole batch. Lattices of uneven (n, k) share launches: one per
+ * kernel for all updates and all marginals of a storage form, and one per kernel per
+ * selection regime (transform in LDS up to 14 atoms, one tile at 15, tiles at 16..20) and
+ * objective; only the one-pass BBPA tables, brute force and lattices above 20 atoms launch
+ * per shape. Each lattice's work is split into the same workgroups as in
  * bgt_hip_lattice_update + bgt_hip_lattice_select, so its results are identical to those
  * per-lattice calls. A lattice may appear at most once per batch. No reference
  * counterpart (the reference steps one lattice at a time). */
@@ -280,7 +283,7 @@ int bgt_hip_batch_round(bgt_hip_lattice **lattices, const uint32_t *experiments,
 
 /* Graph replay of batched steps (bgt_hip_batch_step, bgt_hip_batch_round), off by default.
  * With enable != 0 a step's copies and launches are captured into a hipGraph once per
- * launch structure (the shape groups and their sizes, the selector and the staging
+ * launch structure (the launch groups and their sizes, the selector and the staging
  * buffers) and replayed by later steps of the same structure; the step's own descriptors,
  * experiments and likelihood tables reach the device through the graph's first copy, so the
  * results equal the launched step's. Up to 16 graphs per calling thread are kept (least
