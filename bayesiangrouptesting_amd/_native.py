@@ -110,6 +110,7 @@ def _sig(L):
         "bgt_hip_lattice_posterior": ([_H, c_u32, _dptr], c_int),
         "bgt_hip_lattice_read_posterior": ([_H, _dptr, c_u64], c_int),
         "bgt_hip_lattice_upset_mass": ([_H, c_u32, _dptr], c_int),
+        "bgt_hip_lattice_exact_masses": ([_H, c_int, _u32p, _dptr, _dptr], c_int),
         "bgt_hip_lattice_atom_masses": ([_H, _dptr], c_int),
         "bgt_hip_lattice_classify": ([_H, c_double, c_double, _u32p, _u32p], c_int),
         "bgt_hip_lattice_masks": ([_H, _u32p, _u32p], c_int),

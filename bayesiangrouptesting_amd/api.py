@@ -229,6 +229,18 @@ class Lattice:
         nat.check(nat.lib().bgt_hip_lattice_upset_mass(self._h, self._flip(experiment), ctypes.byref(out)))
         return out.value
 
+    def exact_masses(self, experiments) -> tuple[np.ndarray, float]:
+        """Exact FP64 upset masses of public-coordinate experiments, recomputed from the prior
+        and the updates applied so far (the values near ties are re-decided with), and their
+        relative rounding bound."""
+        qs = np.ascontiguousarray(np.array([self._flip(int(e)) for e in np.atleast_1d(experiments)], dtype=np.uint32))
+        out = np.empty(qs.size, dtype=np.float64)
+        bound = ctypes.c_double()
+        nat.check(nat.lib().bgt_hip_lattice_exact_masses(
+            self._h, qs.size, qs.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+            out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(bound)))
+        return out, bound.value
+
     def atom_masses(self) -> np.ndarray:
         """Atom marginals in INTERNAL atom order."""
         out = np.empty(self._n * self._k, dtype=np.float64)

@@ -213,6 +213,25 @@ void ExactHistory::push(uint32_t e, uint32_t r, const double *dil)
 	z_ok_ = false;
 }
 
+void ExactHistory::enum_inputs(double *pairs, uint32_t *pools, double *fac, int stride) const
+{
+	for (int a = 0; a < atoms_; ++a) {
+		pairs[2 * a] = pi_[a];
+		pairs[2 * a + 1] = q_[a];
+	}
+	for (size_t t = 0; t < hist_.size(); ++t) {
+		const Upd &u = hist_[t];
+		pools[t] = u.e;
+		const int len = __builtin_popcount(u.e);
+		for (int m = 0; m < stride; ++m) {
+			double v = 0.0;
+			if (m <= len)
+				v = dilution_ ? u.f[m] : (m == 0 ? u.f_in : u.f_out);
+			fac[t * (size_t)stride + m] = v;
+		}
+	}
+}
+
 double ExactHistory::rel_bound() const
 {
 	// products of up to `atoms` prior factors and `updates` likelihood factors and sums of

@@ -65,6 +65,10 @@ public:
 		f_in = hist_[t].f_in;
 		f_out = hist_[t].f_out;
 	}
+	// inputs of the device enumeration (launch_exact_enum): pairs[2a + bit] (bit 1: atom a
+	// is negative), each update's pool and its factor by the pool's count m of positive atoms,
+	// fac[t * stride + m] for m <= |pool| < stride (zero beyond)
+	void enum_inputs(double *pairs, uint32_t *pools, double *fac, int stride) const;
 	// U(E) / U({}) for an unnormalised mass computed elsewhere; false without a normaliser
 	bool normalise(double u, double &out)
 	{

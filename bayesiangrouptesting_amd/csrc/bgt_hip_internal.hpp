@@ -289,6 +289,41 @@ void launch_bbpa_hist(const Descs &descs, int count, int n, int k, Storage st, b
 // lattice (d.exact_u[i], -1 where the device program does not apply), from d.pairs and
 // d.hist: the quantity ExactHistory computes on the host, as a dense dynamic program.
 void launch_exact_band(const Descs &descs, int count, int atoms, hipStream_t stream);
+
+// Exact unnormalised upset masses by enumeration, for binary lattices of either likelihood
+// (replicated_lattice.cpp:8-28): U(E) = sum over the states S >= E of
+// P(S) prod_t f_t(m_t(S)), with P(S) the prior product and m_t(S) = |E_t \ S| the number of
+// positive atoms in pool E_t. Each term is formed in FP64 from the prior pairs and the
+// factor tables, and the terms are summed in a fixed order: per thread, the 2^kEnumLowBits
+// states that differ in the query's lowest free atoms in a row, rows in turn; then a
+// workgroup tree and a tree over the query's workgroups (enum_groups of its free atoms).
+// A query's value therefore does not depend on the other queries of the launch.
+constexpr int kEnumLowBits = 6;
+constexpr int kEnumMaxGroups = 256;
+constexpr int kEnumMaxHist = 64;
+constexpr unsigned enum_groups(int free_bits)
+{
+	const int g = free_bits - kEnumLowBits - 10;
+	return 1u << (g < 0 ? 0 : (g > 8 ? 8 : g));
+}
+struct EnumJob {
+	const double *pairs;     // 2 atoms: (pi_a, 1 - pi_a), indexed by the state bit
+	const uint32_t *pools;   // updates: the pools
+	const double *fac;       // updates x stride: factor by the pool's positive count
+	const uint32_t *queries; // count experiments
+	double *partial;         // count x kEnumMaxGroups
+	double *out;             // count
+	int atoms, updates, stride, count;
+};
+void launch_exact_enum(const EnumJob &job, hipStream_t stream);
+// Relative rounding bound of one value of launch_exact_enum: the products (prior and
+// likelihood factors), the in-thread sums and the two trees, all terms non-negative.
+inline double enum_rel_bound(int atoms, int updates)
+{
+	const double rows = (double)(1u << kEnumLowBits); // terms per row
+	const double per_thread = 256.0;                  // rows per thread at most (30 free atoms)
+	return ((double)atoms + (double)updates + 3.0 + rows + per_thread + 16.0) * 0x1p-53;
+}
 void launch_brute_force(const Descs &descs, int count, int n, int k, Storage st,
 						hipStream_t stream);
 void launch_atom_masses(const Descs &descs, int count, int atoms, Storage st, hipStream_t stream);

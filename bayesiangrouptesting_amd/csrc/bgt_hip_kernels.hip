@@ -2760,6 +2760,117 @@ __global__ __launch_bounds__(64) void k_exact_band(Descs descs, int atoms)
 }
 
 // Result heads of a batch of lattices into one staging buffer (one copy to the host).
+// Exact unnormalised upset masses by enumeration (see EnumJob): workgroup (x, y) serves query
+// y's share x of the rows. A row is the candidate's atoms, one assignment of the free atoms
+// above the lowest kEnumLowBits, and every assignment of those lowest ones; its prior
+// product and the factors of the pools without a low free atom are formed once per row.
+__global__ __launch_bounds__(256) void k_exact_enum(EnumJob job)
+{
+	const uint32_t e = job.queries[blockIdx.y];
+	const int atoms = job.atoms;
+	const uint32_t full = (1u << atoms) - 1u; // atoms <= 30
+	const uint32_t freeb = full & ~e;
+	const int f = __popc(freeb);
+	const unsigned groups = enum_groups(f);
+	if (blockIdx.x >= groups)
+		return;
+	__shared__ double s_pairs[2 * kMaxAtoms];
+	__shared__ double s_fac[kEnumMaxHist * (kMaxSubjects + 1)];
+	__shared__ uint32_t s_pool[kEnumMaxHist];
+	__shared__ double s_plow[1 << kEnumLowBits];
+	__shared__ uint32_t s_slow[1 << kEnumLowBits];
+	__shared__ double red[16];
+	const int T = job.updates, stride = job.stride;
+	for (int i = threadIdx.x; i < 2 * atoms; i += blockDim.x)
+		s_pairs[i] = job.pairs[i];
+	for (int i = threadIdx.x; i < T * stride; i += blockDim.x)
+		s_fac[i] = job.fac[i];
+	for (int i = threadIdx.x; i < T; i += blockDim.x)
+		s_pool[i] = job.pools[i];
+	const int L = f < kEnumLowBits ? f : kEnumLowBits;
+	uint32_t lowmask = 0u;
+	{
+		uint32_t r = freeb;
+		for (int i = 0; i < L; ++i) {
+			lowmask |= r & (0u - r);
+			r &= r - 1u;
+		}
+	}
+	const uint32_t highmask = freeb & ~lowmask;
+	__syncthreads();
+	if (threadIdx.x < (1u << L)) {
+		// bit i of j assigns the i-th lowest free atom; factors in increasing atom order
+		const unsigned j = threadIdx.x;
+		uint32_t sl = 0u, r = lowmask;
+		double p = 1.0;
+		for (int i = 0; i < L; ++i) {
+			const uint32_t b = r & (0u - r);
+			r &= r - 1u;
+			const unsigned bit = (j >> i) & 1u;
+			sl |= bit ? b : 0u;
+			p *= s_pairs[2 * (__ffs(b) - 1) + bit];
+		}
+		s_slow[j] = sl;
+		s_plow[j] = p;
+	}
+	// pools holding a low free atom change within a row; the others once per row
+	uint64_t inner = 0ull;
+	for (int t = 0; t < T; ++t)
+		if (s_pool[t] & lowmask)
+			inner |= 1ull << t;
+	// the candidate's atoms are negative in every state counted: their factors lead
+	double lead = 1.0;
+	for (uint32_t r = e & full; r != 0u; r &= r - 1u)
+		lead *= s_pairs[2 * (__ffs(r) - 1) + 1];
+	__syncthreads();
+	const int fh = f - L;
+	const uint64_t rows = 1ull << fh;
+	const uint64_t step = (uint64_t)groups * blockDim.x;
+	const unsigned lows = 1u << L;
+	double acc = 0.0;
+	for (uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; h < rows; h += step) {
+		uint32_t sh = 0u, r = highmask;
+		double p = lead;
+		for (int i = 0; i < fh; ++i) {
+			const uint32_t b = r & (0u - r);
+			r &= r - 1u;
+			const unsigned bit = (unsigned)(h >> i) & 1u;
+			sh |= bit ? b : 0u;
+			p *= s_pairs[2 * (__ffs(b) - 1) + bit];
+		}
+		const uint32_t srow = e | sh;
+		for (int t = 0; t < T; ++t)
+			if (!((inner >> t) & 1ull))
+				p *= s_fac[t * stride + __popc(s_pool[t] & ~srow)];
+		double row = 0.0;
+		for (unsigned j = 0; j < lows; ++j) {
+			const uint32_t st = srow | s_slow[j];
+			double term = p * s_plow[j];
+			for (uint64_t w = inner; w != 0ull; w &= w - 1ull) {
+				const int t = __ffsll((long long)w) - 1;
+				term *= s_fac[t * stride + __popc(s_pool[t] & ~st)];
+			}
+			row += term;
+		}
+		acc += row;
+	}
+	const double bs = block_sum(acc, red);
+	if (threadIdx.x == 0)
+		job.partial[(size_t)blockIdx.y * kEnumMaxGroups + blockIdx.x] = bs;
+}
+
+// the query's workgroup sums, in one fixed tree
+__global__ __launch_bounds__(kEnumMaxGroups) void k_exact_enum_final(EnumJob job)
+{
+	const uint32_t full = (1u << job.atoms) - 1u;
+	const unsigned groups = enum_groups(__popc(full & ~job.queries[blockIdx.x]));
+	__shared__ double red[16];
+	const double v = threadIdx.x < groups ? job.partial[(size_t)blockIdx.x * kEnumMaxGroups + threadIdx.x] : 0.0;
+	const double sum = block_sum(v, red);
+	if (threadIdx.x == 0)
+		job.out[blockIdx.x] = sum;
+}
+
 __global__ __launch_bounds__(256) void k_gather_heads(const LatDesc *many, unsigned char *out)
 {
 	// the 1 KB block from the arena's start, then the first kBandHead band candidates from
@@ -3102,6 +3213,19 @@ void launch_exact_band(const Descs &descs, int count, int atoms, hipStream_t str
 {
 	launch_checked("k_exact_band", [&] {
 		hipLaunchKernelGGL(k_exact_band, dim3(kExactBand, count), dim3(64), 0, stream, descs, atoms);
+	});
+}
+
+void launch_exact_enum(const EnumJob &job, hipStream_t stream)
+{
+	if (job.atoms < 1 || job.atoms > kMaxAtoms || job.updates < 0 || job.updates > kEnumMaxHist || job.count < 1 ||
+		job.stride < 1 || job.stride > kMaxSubjects + 1)
+		throw Error(4, "exact enumeration: shape out of range");
+	// sized for the most free atoms a query can have (the empty one); each query uses its own
+	const unsigned groups = enum_groups(job.atoms);
+	launch_checked("k_exact_enum", [&] {
+		hipLaunchKernelGGL(k_exact_enum, dim3(groups, job.count), dim3(256), 0, stream, job);
+		hipLaunchKernelGGL(k_exact_enum_final, dim3(job.count), dim3(kEnumMaxGroups), 0, stream, job);
 	});
 }
 

@@ -811,8 +811,81 @@ bool DeviceLattice::prepare_rescore(int selector)
 	const bool found = r.e1 != 0xFFFFFFFFu && r.d1 < 2.0;
 	if (!found || !((r.d2 - r.d1) <= band_width(sel)))
 		return false;
-	exact_.prepare(true);
+	if (!uses_enum())
+		exact_.prepare(true); // the device enumeration forms its own normaliser
 	return true;
+}
+
+bool DeviceLattice::exact_masses(const uint32_t *es, size_t count, double *out, bool *ok)
+{
+	if (!exact_.usable())
+		return false;
+	if (!uses_enum())
+		return exact_.masses(es, count, out, ok);
+	if (count > 0)
+		enum_masses(es, count, out);
+	for (size_t i = 0; i < count; ++i)
+		ok[i] = true;
+	return true;
+}
+
+double DeviceLattice::exact_rel_bound() const
+{
+	// M = U(E) / U({}): both values' bounds and the division
+	if (uses_enum())
+		return 2.0 * enum_rel_bound(atoms_, exact_.updates()) + 0x1p-53;
+	return exact_.rel_bound();
+}
+
+void DeviceLattice::enum_masses(const uint32_t *es, size_t count, double *out)
+{
+	const int T = exact_.updates();
+	const int stride = n_ + 1;
+	const size_t q = count + 1; // query 0 is the empty experiment: the normaliser U({})
+	auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
+	const size_t o_fac = up(sizeof(double) * 2 * atoms_);
+	const size_t o_pool = o_fac + up(sizeof(double) * (size_t)T * stride + 8);
+	const size_t o_q = o_pool + up(sizeof(uint32_t) * (size_t)T + 4);
+	const size_t in_bytes = o_q + up(sizeof(uint32_t) * q);
+	const size_t o_part = in_bytes;
+	const size_t o_out = o_part + up(sizeof(double) * q * kEnumMaxGroups);
+	const size_t dev_bytes = o_out + up(sizeof(double) * q);
+	const size_t host_bytes = in_bytes + sizeof(double) * q;
+	char *h = static_cast<char *>(pinned_alloc(host_bytes));
+	char *d = static_cast<char *>(device_alloc(dev_bytes, stream_));
+	exact_.enum_inputs(reinterpret_cast<double *>(h), reinterpret_cast<uint32_t *>(h + o_pool),
+					   reinterpret_cast<double *>(h + o_fac), stride);
+	uint32_t *qh = reinterpret_cast<uint32_t *>(h + o_q);
+	qh[0] = 0u;
+	std::memcpy(qh + 1, es, sizeof(uint32_t) * count);
+	double *u = reinterpret_cast<double *>(h + in_bytes);
+	try {
+		BGT_HIP_CHECK(hipMemcpyAsync(d, h, in_bytes, hipMemcpyHostToDevice, stream_));
+		EnumJob job;
+		job.pairs = reinterpret_cast<const double *>(d);
+		job.pools = reinterpret_cast<const uint32_t *>(d + o_pool);
+		job.fac = reinterpret_cast<const double *>(d + o_fac);
+		job.queries = reinterpret_cast<const uint32_t *>(d + o_q);
+		job.partial = reinterpret_cast<double *>(d + o_part);
+		job.out = reinterpret_cast<double *>(d + o_out);
+		job.atoms = atoms_;
+		job.updates = T;
+		job.stride = stride;
+		job.count = (int)q;
+		clock_.begin(stream_);
+		launch_exact_enum(job, stream_);
+		clock_.end(stream_, "exact_enum");
+		BGT_HIP_CHECK(hipMemcpyAsync(u, d + o_out, sizeof(double) * q, hipMemcpyDeviceToHost, stream_));
+		sync();
+	} catch (...) {
+		device_free(d, stream_);
+		pinned_release(h, host_bytes);
+		throw;
+	}
+	device_free(d, stream_);
+	for (size_t i = 0; i < count; ++i)
+		out[i] = u[i + 1] / u[0];
+	pinned_release(h, host_bytes);
 }
 
 double DeviceLattice::band_width(int sel) const
@@ -1145,7 +1218,7 @@ void DeviceLattice::rescore_band(bgt_hip_selection &out, int sel, const SelResul
 				if (!host.empty()) {
 					std::vector<double> hm(host.size());
 					std::unique_ptr<bool[]> hok(new bool[host.size()]);
-					if (!exact_.masses(ex.data(), host.size(), hm.data(), hok.get()))
+					if (!exact_masses(ex.data(), host.size(), hm.data(), hok.get()))
 						return; // the normaliser is beyond the exact program's budget: keep the device result
 					for (size_t h = 0; h < host.size(); ++h) {
 						mass[host[h]] = hm[h];
@@ -1171,7 +1244,7 @@ void DeviceLattice::rescore_band(bgt_hip_selection &out, int sel, const SelResul
 	out.runner_up = e2;
 	out.runner_up_objective = b2;
 	if (complete) {
-		out.gap_bound = 2.0 * (sel == kSelBha ? 1.0 : 2.0) * exact_.rel_bound();
+		out.gap_bound = 2.0 * (sel == kSelBha ? 1.0 : 2.0) * exact_rel_bound();
 		out.near_tie = (b2 - b1) <= out.gap_bound ? 1 : 0;
 	} else {
 		// the band held more candidates than were collected: the exact order of the top

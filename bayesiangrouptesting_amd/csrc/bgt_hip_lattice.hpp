@@ -138,6 +138,15 @@ public:
 	// tie on a lattice with an exact program); if so, its normaliser is computed now, so that
 	// the normalisers of many lattices can run side by side on the host threads.
 	bool prepare_rescore(int selector);
+	// Exact FP64 M(E) of `count` experiments recomputed from the prior and the update
+	// history (internal coordinates): by enumeration on the device for binary dilution
+	// lattices (at most kEnumMaxHist updates), by the host program for binary non-dilution
+	// ones. ok[i] is false where the host program's budget was exceeded; returns false when
+	// the lattice has no exact program.
+	bool exact_masses(const uint32_t *experiments, size_t count, double *out, bool *ok);
+	// relative rounding bound of exact_masses
+	double exact_rel_bound() const;
+	bool uses_exact_enum() const { return uses_enum(); }
 	// Take over `parent`'s host-side state (masks, candidate restriction, update history,
 	// pending normalisation slot) ahead of an out-of-place update from it; the posterior
 	// itself is produced by that update. Both must have the same shape and storage.
@@ -158,6 +167,13 @@ private:
 	void rescore_band(bgt_hip_selection &out, int selector, const SelResult &r);
 	// exact masses of the band on the device (binary non-dilution), after the band kernels
 	void launch_exact(const Descs &ds);
+	// exact masses come from the device enumeration (binary dilution, short enough history)
+	bool uses_enum() const
+	{
+		return dilution_ && k_ == 1 && exact_.usable() && exact_.updates() <= kEnumMaxHist;
+	}
+	// M(E) = U(E) / U({}) by launch_exact_enum, U({}) in the same launch; synchronises
+	void enum_masses(const uint32_t *experiments, size_t count, double *out);
 	// the fields of the head now in the host mailbox, with up to band_entries band candidates
 	void unpack_head(int band_entries, size_t band_offset);
 	// complete band of candidates for exact rescoring (binary non-dilution lattices); the

@@ -153,6 +153,18 @@ int bgt_hip_lattice_read_posterior(bgt_hip_lattice *lattice, double *out, uint64
  * lattice_base.cpp:318-351) */
 int bgt_hip_lattice_upset_mass(bgt_hip_lattice *lattice, uint32_t experiment, double *out);
 
+/* Exact FP64 upset masses M(E) of `count` experiments (internal coordinates), recomputed from
+ * the prior and the updates applied so far rather than read from the stored posterior: the
+ * values near ties are re-decided with. Binary dilution lattices enumerate the states
+ * S >= E on the device (histories of at most 64 updates); binary non-dilution lattices run
+ * the host's pool program. rel_bound (may be NULL) receives the relative rounding bound.
+ * Returns BGT_HIP_UNSUPPORTED for lattices without an exact program (k >= 2, projected
+ * lattices, longer dilution histories) or past the program's budget. The reference has no
+ * counterpart: its masses are read from its FP64 posterior (get_prob_mass,
+ * lattice_base.cpp:318-351), which these equal within their bounds. */
+int bgt_hip_lattice_exact_masses(bgt_hip_lattice *lattice, int count, const uint32_t *experiments, double *out,
+								 double *rel_bound);
+
 /* atom marginals m_a = sum_{S : bit a of S} x_S (compute_atom_masses, lattice_base.cpp:25-41);
  * out holds subjects*variants doubles */
 int bgt_hip_lattice_atom_masses(bgt_hip_lattice *lattice, double *out);

@@ -1,0 +1,142 @@
+"""Exact upset masses of binary dilution lattices by enumeration on the device, and the
+near-tie re-decisions they drive.
+
+A dilution lattice's exact masses U(E) = sum over S >= E of P(S) prod_t f_t(|E_t \\ S|)
+(replicated_lattice.cpp:17-28, dilution.cpp:44-67) are summed on the device from the prior
+and the update history, in FP64 and a fixed order; M(E) = U(E) / U({}). Pinned by:
+  * the closed form with no update, M(E) = prod_{a in E} (1 - pi_a);
+  * the oracle's FP64 posterior after the same updates (orc.update_in_place), summed
+    accurately over the supersets (1e-12 relative, the oracle's own rounding);
+  * the host's pool-count program on the same history (within both stated bounds);
+  * the same query alone and among others: bit-identical.
+Re-decisions: along FP32C dilution sessions every pick is the oracle's FP64 pick, or the
+oracle's own objectives of the two are within the FP64 tie bound.
+"""
+import numpy as np
+import pytest
+
+import bayesiangrouptesting_amd as bgt
+from bayesiangrouptesting_amd import workload
+from oracle import pyoracle as orc
+from tests.gpu_helpers import accurate_scores, seeded_prior, tie_bound
+
+pytestmark = pytest.mark.gpu
+DL = bgt.LatticeType.replicated_dilution
+ND = bgt.LatticeType.replicated_non_dilution
+F64 = bgt.Precision.float64
+F32 = bgt.Precision.float32_compensated
+
+
+def _superset_masses(post, n, queries):
+    """M(E) from an FP64 posterior (internal order), summed accurately (math.fsum per query)."""
+    import math
+    states = np.arange(1 << n, dtype=np.uint64)
+    total = math.fsum(post)
+    out = []
+    for e in queries:
+        out.append(math.fsum(post[(states & np.uint64(e)) == np.uint64(e)]) / total)
+    return np.array(out)
+
+
+def _history(seed, n, steps):
+    g = workload.SplitMix64(seed)
+    hist = []
+    for _ in range(steps):
+        e = (g.next_u64() & g.next_u64()) & ((1 << n) - 1)
+        if e == 0:
+            e = 1 << g.below(n)
+        hist.append((int(e), g.below(2)))
+    return hist
+
+
+def _queries(seed, n, count):
+    g = workload.SplitMix64(seed + 99)
+    qs = [0, (1 << n) - 1, 1, 1 << (n - 1)]
+    while len(qs) < count:
+        qs.append(int(g.next_u64() & g.next_u64() & g.next_u64() & ((1 << n) - 1)))
+    return qs
+
+
+@pytest.mark.parametrize("n", [6, 13, 17])
+def test_closed_form_without_updates(n):
+    pi = seeded_prior(70 + n, n, 0.01, 0.2)
+    lat = bgt.Lattice(DL, n, pi, precision=F32)
+    qs = _queries(n, n, 24)
+    got, bound = lat.exact_masses([bgt.reverse_bits(q, n) for q in qs])  # public coordinates
+    want = np.array([np.prod([1.0 - pi[a] for a in range(n) if (q >> a) & 1]) for q in qs])
+    assert np.all(np.abs(got - want) <= (bound + n * 2.0 ** -52) * want), (got, want)
+    lat.close()
+
+
+@pytest.mark.parametrize("n,steps,prec", [(10, 6, F64), (14, 9, F32), (18, 12, F32), (20, 8, F64)])
+def test_enumeration_matches_the_oracle_and_the_host_program(n, steps, prec):
+    seed = 80 + n
+    pi = seeded_prior(seed, n, 0.01, 0.15)
+    tab = bgt.DilutionTable(n, 0.05, 0.9)
+    dil = orc.dilution_table(n, 0.05, 0.9)
+    lat = bgt.Lattice(DL, n, pi, precision=prec)
+    ref = orc.prior(pi)
+    hist = _history(seed, n, steps)
+    for e, r in hist:
+        print(f"exact enum: n={n} update e={e:#x} r={r}", flush=True)
+        lat.update(bgt.reverse_bits(e, n), r, tab)
+        orc.update_in_place(ref, True, n, 1, e, r, dil)
+    qs = _queries(seed, n, 40)
+    got, bound = lat.exact_masses([bgt.reverse_bits(q, n) for q in qs])
+    assert 0.0 < bound < 1e-13
+    want = _superset_masses(ref, n, qs)
+    rel = np.abs(got - want) / want
+    assert np.max(rel) <= 1e-12, (np.max(rel), qs[int(np.argmax(rel))])
+    host = bgt.exact_upset_mass(n, pi, hist, np.array(qs, dtype=np.uint32), tab)
+    assert np.all(np.abs(got - host) <= (bound + 1e-14) * host), np.max(np.abs(got - host) / host)
+    # a query's value does not depend on the others of its launch
+    one, _ = lat.exact_masses([bgt.reverse_bits(qs[5], n)])
+    assert one[0] == got[5]
+    lat.close()
+
+
+def test_non_dilution_lattices_keep_the_host_program_and_k2_has_none():
+    n = 12
+    pi = seeded_prior(5, n, 0.01, 0.15)
+    lat = bgt.Lattice(ND, n, pi, precision=F32)
+    hist = _history(5, n, 5)
+    for e, r in hist:
+        lat.update(bgt.reverse_bits(e, n), r)
+    qs = _queries(5, n, 16)
+    got, _ = lat.exact_masses([bgt.reverse_bits(q, n) for q in qs])
+    np.testing.assert_array_equal(got, bgt.exact_upset_mass(n, pi, hist, np.array(qs, dtype=np.uint32)))
+    lat.close()
+    k2 = bgt.Lattice(DL, 5, seeded_prior(6, 10, 0.01, 0.1), variants=2)
+    with pytest.raises(bgt.Unsupported):
+        k2.exact_masses([1])
+    k2.close()
+
+
+@pytest.mark.parametrize("n,seed", [(14, 3), (18, 4), (20, 5)])
+def test_fp32c_dilution_sessions_pick_the_oracles_experiments(n, seed):
+    pi = seeded_prior(seed, n, 0.005, 0.1)
+    tab = bgt.DilutionTable(n, 0.05, 0.9)
+    dil = orc.dilution_table(n, 0.05, 0.9)
+    lat = bgt.Lattice(DL, n, pi, precision=F32)
+    ref = orc.prior(pi)
+    truth = workload.true_state(seed, list(pi))
+    sampler = workload.ResponseSampler(seed, n, 1, bgt.reverse_bits(truth, n), list(dil))
+    redecided = 0
+    for step in range(10):
+        s = lat.select(bgt.SelectorType.op_bha)
+        e = bgt.reverse_bits(s.experiment, n)
+        scores = accurate_scores(ref, n, 1, 1)
+        e_ref = int(np.argmin(scores))
+        bound = tie_bound(n, 1, 1, updates=step)
+        print(f"fp32c dilution session n={n} step {step}: pick {e:#x} oracle {e_ref:#x} "
+              f"near_tie={s.near_tie} redecided={s.redecided}", flush=True)
+        if e != e_ref:
+            # the oracle's own objectives of the two within the FP64 tie bound, or a band that
+            # overflowed (only the top two re-decided, reported as a near tie)
+            gap = float(scores[e] - scores[e_ref])
+            assert gap <= bound or (s.redecided == 2 and s.near_tie), (step, e, e_ref, gap, bound, s)
+        redecided += s.redecided > 0
+        r = sampler.draw(s.experiment)
+        lat.update(s.experiment, r, tab)
+        orc.update_in_place(ref, True, n, 1, e, r, dil)
+    lat.close()
