@@ -261,11 +261,12 @@ int bgt_hip_lattice_exact_masses(bgt_hip_lattice *h, int count, const uint32_t *
 		if (count < 0 || (count > 0 && (experiments == nullptr || out == nullptr)))
 			throw bgt_hip::Error(BGT_HIP_BAD_INPUT, "null experiments or output");
 		auto &lat = L(h);
-		const uint32_t full = (1u << (lat.subjects() * lat.variants())) - 1u;
+		const int atoms = lat.subjects() * lat.variants();
+		const uint32_t full = atoms >= 32 ? 0xFFFFFFFFu : ((1u << atoms) - 1u);
 		for (int i = 0; i < count; ++i)
 			if (experiments[i] & ~full)
 				throw bgt_hip::Error(BGT_HIP_BAD_INPUT, "experiment outside the lattice");
-		if (lat.dilution() && lat.variants() == 1 && !lat.uses_exact_enum())
+		if ((lat.dilution() || lat.variants() >= 2) && !lat.uses_exact_enum())
 			throw bgt_hip::Error(BGT_HIP_UNSUPPORTED, "no exact program for this lattice's history");
 		std::unique_ptr<bool[]> ok(new bool[count > 0 ? count : 1]);
 		if (!lat.exact_masses(experiments, (size_t)count, out, ok.get()))

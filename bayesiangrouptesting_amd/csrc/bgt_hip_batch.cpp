@@ -308,6 +308,7 @@ void batch_step(std::vector<BatchItem> &items, int selector, double up, double l
 				// as DeviceLattice::select: the feature tables from one histogram pass
 				launch_bbpa_hist(dg, cnt, lat.subjects(), k, lat.storage(), false, s);
 				launch_bbpa_score(dg, cnt, lat.subjects(), k, target, s);
+				launch_bbpa_band(dg, cnt, lat.subjects(), k, target, s);
 				continue;
 			}
 			unsigned entries = 0;
@@ -319,9 +320,10 @@ void batch_step(std::vector<BatchItem> &items, int selector, double up, double l
 				launch_zeta_pass1(dg, cnt, atoms, lat.storage(), false, k, lat.dilution(), objective, s);
 				entries = launch_zeta_final(dg, cnt, atoms, k, lat.storage(), objective, false, false, target, s);
 			}
-			if (objective == kObjGather)
+			if (objective == kObjGather) {
 				launch_bbpa_score(dg, cnt, subjects, k, target, s);
-			else if (k == 1 && atoms > kSmallBits)
+				launch_bbpa_band(dg, cnt, subjects, k, target, s);
+			} else if (k == 1 && atoms > kSmallBits)
 				launch_band_collect(dg, cnt, entries, atoms, k, lat.storage(), objective, atoms == kTileBits, false,
 									target, s);
 		}

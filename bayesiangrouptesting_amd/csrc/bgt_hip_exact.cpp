@@ -168,12 +168,14 @@ void ExactHistory::Table::add(uint64_t k, double v)
 	}
 }
 
-void ExactHistory::reset(bool usable, int atoms, const double *prior, bool dilution)
+void ExactHistory::reset(bool usable, int atoms, const double *prior, bool dilution, int variants)
 {
-	usable_ = usable && atoms <= 32;
+	usable_ = usable && atoms <= 32 && variants >= 1 && atoms % variants == 0;
 	broken_ = false;
 	dilution_ = dilution;
 	atoms_ = atoms;
+	variants_ = variants;
+	subjects_ = variants >= 1 ? atoms / variants : atoms;
 	pi_.assign(prior, prior + atoms);
 	q_.resize(atoms);
 	for (int a = 0; a < atoms; ++a)
@@ -187,6 +189,37 @@ void ExactHistory::push(uint32_t e, uint32_t r, const double *dil)
 {
 	if (!usable_)
 		return;
+	if (variants_ > 1) {
+		// per variant the factors of the device likelihood tables (replicated_lattice.cpp:8-28,
+		// same FP64 expressions): dilution P(r_v = 1) = D[s-1][m] (the empty pool 0),
+		// non-dilution P(r_v = 1) = 0.99 when no pooled atom is positive, 0.01 otherwise
+		if (dilution_ && dil == nullptr) {
+			broken_ = true;
+			return;
+		}
+		const int len = __builtin_popcount(e);
+		const int stride = subjects_ + 1;
+		Upd u{e, 0.0, 0.0, {}, {}};
+		u.fv.assign((size_t)variants_ * stride, 0.0);
+		for (int v = 0; v < variants_; ++v) {
+			const bool rv = (r >> v) & 1u;
+			for (int m = 0; m <= len; ++m) {
+				double f;
+				if (dilution_) {
+					const double p1 = len > 0 ? dil[(size_t)(len - 1) * stride + m] : 0.0;
+					f = rv ? p1 : 1.0 - p1;
+				} else if (m == 0) {
+					f = rv ? kNegative : (1.0 - kNegative);
+				} else {
+					f = rv ? (1.0 - kNegative) : kNegative;
+				}
+				u.fv[(size_t)v * stride + m] = f;
+			}
+		}
+		hist_.push_back(std::move(u));
+		z_ok_ = false;
+		return;
+	}
 	if (dilution_) {
 		if (dil == nullptr) {
 			broken_ = true;
@@ -195,7 +228,7 @@ void ExactHistory::push(uint32_t e, uint32_t r, const double *dil)
 		// the factors of the device likelihood tables (same FP64 expressions): pool size s,
 		// m positive atoms, P(r = 1) = D[s-1][m]; the empty pool has P(r = 1) = 0
 		const int len = __builtin_popcount(e);
-		Upd u{e, 0.0, 0.0, {}};
+		Upd u{e, 0.0, 0.0, {}, {}};
 		u.f.resize(len + 1);
 		for (int m = 0; m <= len; ++m) {
 			const double p1 = len > 0 ? dil[(size_t)(len - 1) * (atoms_ + 1) + m] : 0.0;
@@ -209,15 +242,24 @@ void ExactHistory::push(uint32_t e, uint32_t r, const double *dil)
 	// the factors of the device likelihood tables (same FP64 expressions)
 	const double contained = r ? kNegative : (1.0 - kNegative);
 	const double missing = r ? (1.0 - kNegative) : kNegative;
-	hist_.push_back({e, contained, missing, {}});
+	hist_.push_back({e, contained, missing, {}, {}});
 	z_ok_ = false;
 }
 
-void ExactHistory::enum_inputs(double *pairs, uint32_t *pools, double *fac, int stride) const
+void ExactHistory::enum_inputs(int variant, double *pairs, uint32_t *pools, double *fac, int stride) const
 {
-	for (int a = 0; a < atoms_; ++a) {
-		pairs[2 * a] = pi_[a];
-		pairs[2 * a + 1] = q_[a];
+	for (int i = 0; i < subjects_; ++i) {
+		pairs[2 * i] = pi_[variant * subjects_ + i];
+		pairs[2 * i + 1] = q_[variant * subjects_ + i];
+	}
+	if (variants_ > 1) {
+		for (size_t t = 0; t < hist_.size(); ++t) {
+			pools[t] = hist_[t].e;
+			const int len = __builtin_popcount(hist_[t].e);
+			for (int m = 0; m < stride; ++m)
+				fac[t * stride + m] = m <= len ? hist_[t].fv[(size_t)variant * (subjects_ + 1) + m] : 0.0;
+		}
+		return;
 	}
 	for (size_t t = 0; t < hist_.size(); ++t) {
 		const Upd &u = hist_[t];
@@ -528,6 +570,8 @@ void ExactHistory::plan_order()
 
 bool ExactHistory::normaliser()
 {
+	if (variants_ != 1)
+		return false; // the pool program is binary
 	if (z_ok_)
 		return true;
 	plan_order();

@@ -43,11 +43,15 @@ int parallel_width();
 
 class ExactHistory {
 public:
-	// binary lattices only; anything else leaves the history unusable. dilution: the pushes
-	// carry dilution tables (subjects x (subjects + 1), row s-1 for pool size s)
-	void reset(bool usable, int atoms, const double *prior, bool dilution = false);
+	// dilution: the pushes carry dilution tables (subjects x (subjects + 1), row s-1 for pool
+	// size s). The host's pool program covers binary lattices (variants == 1); lattices of
+	// several variants keep their history for the device enumeration only (enum_inputs).
+	void reset(bool usable, int atoms, const double *prior, bool dilution = false, int variants = 1);
 	bool usable() const { return usable_ && !broken_; }
 	bool dilution() const { return dilution_; }
+	int variants() const { return variants_; }
+	// pools of the device enumeration: one per update (each variant block separately)
+	int enum_pools() const { return (int)hist_.size(); }
 	void push(uint32_t experiment, uint32_t response, const double *dilution_table = nullptr);
 	int updates() const { return (int)hist_.size(); }
 	// M(E) after all pushed updates; false if the state budget of the program is exceeded
@@ -65,10 +69,15 @@ public:
 		f_in = hist_[t].f_in;
 		f_out = hist_[t].f_out;
 	}
-	// inputs of the device enumeration (launch_exact_enum): pairs[2a + bit] (bit 1: atom a
-	// is negative), each update's pool and its factor by the pool's count m of positive atoms,
-	// fac[t * stride + m] for m <= |pool| < stride (zero beyond)
-	void enum_inputs(double *pairs, uint32_t *pools, double *fac, int stride) const;
+	// inputs of the device enumeration (launch_exact_enum) over variant v's block of subjects
+	// atoms: pairs[2i + bit] of its atoms (bit 1: atom v * subjects + i is negative), each
+	// update's pool and variant v's factor by the pool's count m of positive atoms in the
+	// block, fac[t * stride + m] for m <= |pool| < stride (zero beyond). The prior is a
+	// product over atoms and every likelihood factor depends on one block
+	// (replicated_lattice.cpp:8-28), so the posterior is a product over the variant blocks
+	// and the mass of an atom set is the product of its blocks' masses.
+	void enum_inputs(int variant, double *pairs, uint32_t *pools, double *fac, int stride) const;
+	int subjects() const { return subjects_; }
 	// U(E) / U({}) for an unnormalised mass computed elsewhere; false without a normaliser
 	bool normalise(double u, double &out)
 	{
@@ -92,6 +101,9 @@ private:
 		uint32_t e;
 		double f_in, f_out;   // factor when the pool is / is not contained in S
 		std::vector<double> f; // dilution: factor by the pool's count of positive atoms
+		// several variants: variant v's factor by its count m of positive atoms in the pool,
+		// fv[v * (subjects + 1) + m]
+		std::vector<double> fv;
 	};
 	// Pool-set states of the program in insertion order with their weights. Up to
 	// kDenseBits pools, states index a direct map (reset entry by entry); beyond that an
@@ -122,6 +134,7 @@ private:
 
 	bool usable_ = false, broken_ = false, dilution_ = false;
 	int atoms_ = 0;
+	int variants_ = 1, subjects_ = 0;
 	std::vector<double> pi_, q_;
 	std::vector<Upd> hist_;
 	std::vector<int> order_; // atom order of the program

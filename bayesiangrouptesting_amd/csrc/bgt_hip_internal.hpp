@@ -149,6 +149,8 @@ struct LatDesc {
 	// Op-BBPA histogram tables (k >= 2): fixed-point bins, three 64-bit limbs each, zero
 	// between uses
 	unsigned long long *fixtab;
+	// Op-BBPA error band of lattices of several variants (k_bbpa_band; null: not collected)
+	Cand *gband;
 };
 
 // Device exact rescoring covers the first kExactBand band candidates, histories of at
@@ -277,6 +279,9 @@ void launch_band_collect(const Descs &descs, int count, unsigned entries, int n,
 void launch_gather_heads(const LatDesc *many, int count, unsigned char *out, hipStream_t stream);
 void launch_bbpa_score(const Descs &descs, int count, int n, int k, double target,
 					   hipStream_t stream);
+// after launch_bbpa_score: the candidates within d.band_width of the best into d.gband
+// (lattices of several variants that collect a band; the others return at once)
+void launch_bbpa_band(const Descs &descs, int count, int n, int k, double target, hipStream_t stream);
 // Op-BBPA feature tables of k >= 2 lattices in one pass over the posterior (bbpa.cpp:237-313)
 // where each lane can hold its entries of every table (n >= 6, (2^k - 1) 2^(n-6) <= 64, more
 // than 14 atoms): d.ftab, the total and the atom masses, ready for launch_bbpa_score. fused:

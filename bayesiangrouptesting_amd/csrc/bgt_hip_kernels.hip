@@ -1751,6 +1751,41 @@ __global__ __launch_bounds__(1024) void k_bbpa_tables(Descs descs, int fused)
 
 // BBPA k >= 2 scoring from the feature tables (bbpa.cpp:332-365 for k = 2 and 416-460
 // otherwise); same operation order per experiment.
+// Op-BBPA objective of experiment e from the feature tables (bbpa.cpp:416-460): each response
+// mask's mass by inclusion-exclusion over the tables F_B (F_{} = the total), and the sum of
+// their distances to the target 1/2^k.
+__device__ __forceinline__ double bbpa_objective(const double *fm, uint32_t ec, uint32_t e, int k, double total,
+												 double target)
+{
+	if (k == 2) {
+		const double p0 = fm[ec + e], p1 = fm[2 * ec + e], pb = fm[3 * ec + e];
+		const double r0 = pb;
+		const double r1 = p1 - pb;
+		const double r2 = p0 - pb;
+		const double r3 = total - p0 - p1 + pb;
+		return fabs(r0 - target) + fabs(r1 - target) + fabs(r2 - target) + fabs(r3 - target);
+	}
+	const uint32_t rc = 1u << k, all = rc - 1u;
+	double score = 0.0;
+	for (uint32_t resp = 0; resp < rc; ++resp) {
+		const uint32_t positive = (~resp) & all;
+		const uint32_t missing = resp;
+		double exact = 0.0;
+		for (uint32_t extra = missing;; extra = (extra - 1u) & missing) {
+			const uint32_t fs = positive | extra;
+			const double f = fs == 0 ? total : fm[(size_t)fs * ec + e];
+			if ((__popc(extra) & 1) == 0)
+				exact += f;
+			else
+				exact -= f;
+			if (extra == 0)
+				break;
+		}
+		score += fabs(exact - target);
+	}
+	return score;
+}
+
 __global__ __launch_bounds__(256) void k_bbpa_score(Descs descs, int ns, int k, double target)
 {
 	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
@@ -1766,39 +1801,51 @@ __global__ __launch_bounds__(256) void k_bbpa_score(Descs descs, int ns, int k, 
 	Top2 t;
 	top2_init(t);
 	const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-	if (e < ec && (e & ~d.active) == 0) {
-		const double *fm = d.ftab;
-		double score;
-		if (k == 2) {
-			const double p0 = fm[ec + e], p1 = fm[2 * ec + e], pb = fm[3 * ec + e];
-			const double r0 = pb;
-			const double r1 = p1 - pb;
-			const double r2 = p0 - pb;
-			const double r3 = total - p0 - p1 + pb;
-			score = fabs(r0 - target) + fabs(r1 - target) + fabs(r2 - target) + fabs(r3 - target);
-		} else {
-			const uint32_t rc = 1u << k, all = rc - 1u;
-			score = 0.0;
-			for (uint32_t resp = 0; resp < rc; ++resp) {
-				const uint32_t positive = (~resp) & all;
-				const uint32_t missing = resp;
-				double exact = 0.0;
-				for (uint32_t extra = missing;; extra = (extra - 1u) & missing) {
-					const uint32_t fs = positive | extra;
-					const double f = fs == 0 ? total : fm[(size_t)fs * ec + e];
-					if ((__popc(extra) & 1) == 0)
-						exact += f;
-					else
-						exact -= f;
-					if (extra == 0)
-						break;
-				}
-				score += fabs(exact - target);
-			}
-		}
-		top2_insert(t, score, e);
-	}
+	if (e < ec && (e & ~d.active) == 0)
+		top2_insert(t, bbpa_objective(d.ftab, ec, e, k, total, target), e);
 	publish_top2(d, blockIdx.x, t, red2);
+}
+
+// The Op-BBPA error band of lattices of several variants: after k_bbpa_score and the merge of
+// its top-2 lists, every selectable experiment whose objective (recomputed from the same
+// tables, so the same value) lies within band_width of the best is appended to d.gband,
+// one atomic per wave on counter[0] (cleared by the merge). Appends past band_cap are
+// counted, not stored: counter[0] > band_cap reports an incomplete band.
+__global__ __launch_bounds__(256) void k_bbpa_band(Descs descs, int ns, int k, double target)
+{
+	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
+	if (descs.ragged) {
+		ns = d.n;
+		if (blockIdx.x >= score_groups(ns))
+			return;
+	}
+	if (d.gband == nullptr)
+		return;
+	const uint32_t ec = 1u << ns;
+	const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+	const double lim = d.result->d1 + d.band_width;
+	double s = INFINITY;
+	if (e < ec && (e & ~d.active) == 0)
+		s = bbpa_objective(d.ftab, ec, e, k, d.scal[kScalTotal], target);
+	const bool in = s <= lim;
+	const unsigned long long mask = __ballot(in);
+	if (mask == 0ull)
+		return;
+	const int lane = threadIdx.x & 63;
+	unsigned base = 0u;
+	if (lane == __ffsll((long long)mask) - 1)
+		base = atomicAdd(&d.counter[0], (unsigned)__popcll(mask));
+	base = __shfl(base, __ffsll((long long)mask) - 1);
+	if (in) {
+		const unsigned at = base + (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
+		if (at < d.band_cap) {
+			Cand c;
+			c.d = s;
+			c.e = e;
+			c.flags = 0u;
+			d.gband[at] = c;
+		}
+	}
 }
 
 // Explicit (experiment, state) sweep (benchmarks/experiment_selection_benchmark.cpp:214-247):
@@ -3235,6 +3282,14 @@ void launch_bbpa_score(const Descs &descs, int count, int n, int k, double targe
 	launch_checked("k_bbpa_score", [&] {
 		hipLaunchKernelGGL(k_bbpa_score, grid, dim3(256), 0, stream, descs, n, k, target);
 		hipLaunchKernelGGL(k_finalize_top2, dim3(1, count), dim3(1024), 0, stream, descs, grid.x, (int)kCountScore);
+	});
+}
+
+void launch_bbpa_band(const Descs &descs, int count, int n, int k, double target, hipStream_t stream)
+{
+	const dim3 grid(score_groups(n), count);
+	launch_checked("k_bbpa_band", [&] {
+		hipLaunchKernelGGL(k_bbpa_band, grid, dim3(256), 0, stream, descs, n, k, target);
 	});
 }
 

@@ -214,7 +214,7 @@ void DeviceLattice::init_prior(const double *prior)
 	len_ = 0;
 	clock_.reset();
 	clock_.enabled = false;
-	exact_.reset(k_ == 1, atoms_, prior, dilution_);
+	exact_.reset(true, atoms_, prior, dilution_, k_);
 	pairs_own_ = true;
 	std::vector<double> pairs(2 * kMaxAtoms, 0.0);
 	for (int a = 0; a < atoms_; ++a) {
@@ -426,14 +426,16 @@ ArenaLayout arena_layout(bool dilution, int n, int k)
 		return o;
 	};
 	// binary lattices collect their error band for exact re-decision; the device's dense
-	// program (k_exact_band) covers the non-dilution ones
+	// program (k_exact_band) covers the non-dilution ones. Lattices of several variants
+	// collect their Op-BBPA band (k_bbpa_band) in the same place, right after the 1 KB block.
 	a.exact = k == 1;
+	const size_t gband = k >= 2 ? sizeof(Cand) * kBandSmall : 0;
 	a.scal = take(sizeof(double) * kScalars); // 0
 	a.res = take(sizeof(SelResult));         // 256
 	a.cnt = take(sizeof(unsigned) * 4);      // 512
 	a.mass = take(sizeof(double) * 32);      // 768
 	a.exact_u = take(a.exact ? sizeof(double) * kExactBand : 0);        // kHeadExactWide
-	a.band = take(a.exact ? sizeof(Cand) * band_capacity(atoms) : 0); // kHeadBandWide (exact)
+	a.band = take(a.exact ? sizeof(Cand) * band_capacity(atoms) : gband); // kHeadBandWide (exact), else kHeadBand
 	a.cand = take(a.exact ? sizeof(Cand) * 2 * kTileThreads * zeta_final_groups(atoms) : 0);
 	a.flags = take(a.exact ? sizeof(unsigned) * kMaxFinalGroups : 0);
 	a.hist = take(a.exact ? sizeof(ExactRec) * kExactMaxHist : 0);
@@ -491,6 +493,10 @@ void DeviceLattice::allocate()
 		hist_dev_ = reinterpret_cast<ExactRec *>(arena_ + a.hist);
 		exact_dev_ = reinterpret_cast<double *>(arena_ + a.exact_u);
 		band_cap_ = (unsigned)band_capacity(atoms_);
+		band_host_ = static_cast<Cand *>(pinned_alloc(sizeof(Cand) * band_cap_));
+	} else if (k_ >= 2) {
+		band_ = reinterpret_cast<Cand *>(arena_ + a.band);
+		band_cap_ = (unsigned)kBandSmall;
 		band_host_ = static_cast<Cand *>(pinned_alloc(sizeof(Cand) * band_cap_));
 	}
 	double init[kScalars] = {1.0, 1.0, 1.0, 1.0, 0.0, 0.0, 0.0, 0.0};
@@ -573,7 +579,8 @@ LatDesc DeviceLattice::desc() const
 	d.inv_in = inv_cur_;
 	d.inv_out = inv_cur_ == kScalInv0 ? kScalInv1 : kScalInv0;
 	d.cand = cand_;
-	d.band = band_;
+	d.band = k_ == 1 ? band_ : nullptr;
+	d.gband = k_ >= 2 ? band_ : nullptr;
 	d.band_width = 0.0;
 	d.wgflag = wgflag_;
 	d.band_cap = band_cap_;
@@ -690,7 +697,7 @@ void DeviceLattice::fetch_head()
 	// lattices with an error band (their arena holds at least kHeadBytesWide) bring back
 	// the wide head, and the band's exact masses when they were computed
 	// in one copy (a second one, issued on another engine, waited about 30 us for the first)
-	const bool wide = band_host_ != nullptr;
+	const bool wide = exact_dev_ != nullptr;
 	BGT_HIP_CHECK(hipMemcpyAsync(box->head, arena_, wide ? kHeadBytesWide : kHeadBytes, hipMemcpyDeviceToHost,
 								 stream_));
 	if (clock_.enabled) {
@@ -719,7 +726,8 @@ void DeviceLattice::load_head(const unsigned char *head)
 void DeviceLattice::launch_exact(const Descs &ds_in)
 {
 	exact_pending_ = false;
-	if (exact_dev_ == nullptr || dilution_ || !pairs_own_ || !exact_.usable() || exact_.updates() > kExactMaxHist)
+	if (exact_dev_ == nullptr || dilution_ || k_ != 1 || !pairs_own_ || !exact_.usable() ||
+		exact_.updates() > kExactMaxHist)
 		return;
 	HostBox *box = static_cast<HostBox *>(host_box_);
 	const int n = exact_.updates();
@@ -797,8 +805,12 @@ bgt_hip_selection DeviceLattice::finish_selection(int selector, const SelResult 
 	out.near_tie = (r.d2 - r.d1) <= out.gap_bound ? 1 : 0;
 	if (cand_ != nullptr)
 		out.collected = (int)static_cast<const HostBox *>(host_box_)->counter[2];
-	if (out.near_tie && found && (sel == kSelBha || sel == kSelBbpa) && exact_.usable())
-		rescore(out, sel, r);
+	if (out.near_tie && found && exact_.usable()) {
+		if (k_ == 1 && (sel == kSelBha || sel == kSelBbpa))
+			rescore(out, sel, r);
+		else if (k_ >= 2 && sel == kSelBbpa && uses_enum())
+			rescore_gather(out, r);
+	}
 	return out;
 }
 
@@ -832,50 +844,78 @@ bool DeviceLattice::exact_masses(const uint32_t *es, size_t count, double *out, 
 double DeviceLattice::exact_rel_bound() const
 {
 	// M = U(E) / U({}): both values' bounds and the division
+	// per block M_v = U_v(E) / U_v({}) (two enumerations and a division); the product of
+	// the k blocks' masses adds k - 1 roundings
 	if (uses_enum())
-		return 2.0 * enum_rel_bound(atoms_, exact_.updates()) + 0x1p-53;
+		return (double)k_ * (2.0 * enum_rel_bound(n_, exact_.enum_pools()) + 0x1p-53) + (double)(k_ - 1) * 0x1p-53;
 	return exact_.rel_bound();
 }
 
-void DeviceLattice::enum_masses(const uint32_t *es, size_t count, double *out)
+void DeviceLattice::enum_run(const std::vector<std::vector<uint32_t>> &queries, std::vector<std::vector<double>> &masses)
 {
-	const int T = exact_.updates();
+	// one launch_exact_enum per variant block, all on the stream, one copy of the values back
+	// and one synchronisation; query 0 of each block is the empty set, its normaliser U({})
+	const int T = exact_.enum_pools();
 	const int stride = n_ + 1;
-	const size_t q = count + 1; // query 0 is the empty experiment: the normaliser U({})
 	auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
-	const size_t o_fac = up(sizeof(double) * 2 * atoms_);
-	const size_t o_pool = o_fac + up(sizeof(double) * (size_t)T * stride + 8);
-	const size_t o_q = o_pool + up(sizeof(uint32_t) * (size_t)T + 4);
-	const size_t in_bytes = o_q + up(sizeof(uint32_t) * q);
-	const size_t o_part = in_bytes;
-	const size_t o_out = o_part + up(sizeof(double) * q * kEnumMaxGroups);
-	const size_t dev_bytes = o_out + up(sizeof(double) * q);
-	const size_t host_bytes = in_bytes + sizeof(double) * q;
+	struct Part {
+		size_t q, o_pairs, o_fac, o_pool, o_q, o_part, o_out, h_out;
+	};
+	std::vector<Part> parts(queries.size());
+	size_t in_bytes = 0;
+	for (size_t v = 0; v < queries.size(); ++v) {
+		Part &p = parts[v];
+		p.q = queries[v].size() + 1;
+		if (p.q > 65535)
+			throw Error(BGT_HIP_INTERNAL, "too many exact masses for one launch");
+		p.o_pairs = in_bytes;
+		p.o_fac = p.o_pairs + up(sizeof(double) * 2 * n_);
+		p.o_pool = p.o_fac + up(sizeof(double) * (size_t)T * stride + 8);
+		p.o_q = p.o_pool + up(sizeof(uint32_t) * (size_t)T + 4);
+		in_bytes = p.o_q + up(sizeof(uint32_t) * p.q);
+	}
+	size_t dev_bytes = in_bytes, out_bytes = 0;
+	for (Part &p : parts) {
+		p.o_part = dev_bytes;
+		dev_bytes += up(sizeof(double) * p.q * kEnumMaxGroups);
+		p.o_out = dev_bytes;
+		dev_bytes += up(sizeof(double) * p.q);
+		p.h_out = in_bytes + out_bytes;
+		out_bytes += up(sizeof(double) * p.q);
+	}
+	const size_t host_bytes = in_bytes + out_bytes;
 	char *h = static_cast<char *>(pinned_alloc(host_bytes));
 	char *d = static_cast<char *>(device_alloc(dev_bytes, stream_));
-	exact_.enum_inputs(reinterpret_cast<double *>(h), reinterpret_cast<uint32_t *>(h + o_pool),
-					   reinterpret_cast<double *>(h + o_fac), stride);
-	uint32_t *qh = reinterpret_cast<uint32_t *>(h + o_q);
-	qh[0] = 0u;
-	std::memcpy(qh + 1, es, sizeof(uint32_t) * count);
-	double *u = reinterpret_cast<double *>(h + in_bytes);
+	for (size_t v = 0; v < queries.size(); ++v) {
+		const Part &p = parts[v];
+		exact_.enum_inputs((int)v, reinterpret_cast<double *>(h + p.o_pairs), reinterpret_cast<uint32_t *>(h + p.o_pool),
+						   reinterpret_cast<double *>(h + p.o_fac), stride);
+		uint32_t *qh = reinterpret_cast<uint32_t *>(h + p.o_q);
+		qh[0] = 0u;
+		if (!queries[v].empty())
+			std::memcpy(qh + 1, queries[v].data(), sizeof(uint32_t) * queries[v].size());
+	}
 	try {
 		BGT_HIP_CHECK(hipMemcpyAsync(d, h, in_bytes, hipMemcpyHostToDevice, stream_));
-		EnumJob job;
-		job.pairs = reinterpret_cast<const double *>(d);
-		job.pools = reinterpret_cast<const uint32_t *>(d + o_pool);
-		job.fac = reinterpret_cast<const double *>(d + o_fac);
-		job.queries = reinterpret_cast<const uint32_t *>(d + o_q);
-		job.partial = reinterpret_cast<double *>(d + o_part);
-		job.out = reinterpret_cast<double *>(d + o_out);
-		job.atoms = atoms_;
-		job.updates = T;
-		job.stride = stride;
-		job.count = (int)q;
 		clock_.begin(stream_);
-		launch_exact_enum(job, stream_);
+		for (size_t v = 0; v < queries.size(); ++v) {
+			const Part &p = parts[v];
+			EnumJob job;
+			job.pairs = reinterpret_cast<const double *>(d + p.o_pairs);
+			job.pools = reinterpret_cast<const uint32_t *>(d + p.o_pool);
+			job.fac = reinterpret_cast<const double *>(d + p.o_fac);
+			job.queries = reinterpret_cast<const uint32_t *>(d + p.o_q);
+			job.partial = reinterpret_cast<double *>(d + p.o_part);
+			job.out = reinterpret_cast<double *>(d + p.o_out);
+			job.atoms = n_;
+			job.updates = T;
+			job.stride = stride;
+			job.count = (int)p.q;
+			launch_exact_enum(job, stream_);
+		}
 		clock_.end(stream_, "exact_enum");
-		BGT_HIP_CHECK(hipMemcpyAsync(u, d + o_out, sizeof(double) * q, hipMemcpyDeviceToHost, stream_));
+		for (const Part &p : parts)
+			BGT_HIP_CHECK(hipMemcpyAsync(h + p.h_out, d + p.o_out, sizeof(double) * p.q, hipMemcpyDeviceToHost, stream_));
 		sync();
 	} catch (...) {
 		device_free(d, stream_);
@@ -883,9 +923,33 @@ void DeviceLattice::enum_masses(const uint32_t *es, size_t count, double *out)
 		throw;
 	}
 	device_free(d, stream_);
-	for (size_t i = 0; i < count; ++i)
-		out[i] = u[i + 1] / u[0];
+	masses.resize(queries.size());
+	for (size_t v = 0; v < queries.size(); ++v) {
+		const double *u = reinterpret_cast<const double *>(h + parts[v].h_out);
+		masses[v].resize(queries[v].size());
+		for (size_t i = 0; i < queries[v].size(); ++i)
+			masses[v][i] = u[i + 1] / u[0];
+	}
 	pinned_release(h, host_bytes);
+}
+
+void DeviceLattice::enum_masses(const uint32_t *es, size_t count, double *out)
+{
+	// the mass of an atom set is the product of its variant blocks' masses (the posterior is
+	// a product over the blocks), in variant order
+	const uint32_t smask = (n_ >= 32) ? 0xFFFFFFFFu : ((1u << n_) - 1u);
+	std::vector<std::vector<uint32_t>> q(k_, std::vector<uint32_t>(count));
+	for (int v = 0; v < k_; ++v)
+		for (size_t i = 0; i < count; ++i)
+			q[v][i] = (es[i] >> (v * n_)) & smask;
+	std::vector<std::vector<double>> m;
+	enum_run(q, m);
+	for (size_t i = 0; i < count; ++i) {
+		double x = m[0][i];
+		for (int v = 1; v < k_; ++v)
+			x *= m[v][i];
+		out[i] = x;
+	}
 }
 
 double DeviceLattice::band_width(int sel) const
@@ -928,6 +992,178 @@ void DeviceLattice::collect_band(const Descs &ds, unsigned entries, int objectiv
 	clock_.begin(stream_);
 	launch_band_collect(ds, 1, entries, atoms_, k_, storage_, objective, from_posterior, scaled, target, stream_);
 	clock_.end(stream_, "band_collect");
+}
+
+void DeviceLattice::collect_gather_band(double target)
+{
+	// lattices of several variants whose near ties can be re-decided (a short enough history)
+	if (k_ < 2 || band_ == nullptr || !uses_enum())
+		return;
+	Descs ds = single();
+	ds.one.band_width = band_width(kSelBbpa);
+	clock_.begin(stream_);
+	launch_bbpa_band(ds, 1, n_, k_, target, stream_);
+	clock_.end(stream_, "bbpa_band");
+}
+
+void DeviceLattice::gather_objectives(const uint32_t *es, size_t count, double *out)
+{
+	// exact feature masses F_B(E): the mass of the states whose blocks v in B all contain E,
+	// i.e. the upset mass of E placed in each of those blocks; then the objective of
+	// bbpa.cpp:416-460 over them (F_{} = 1, the normalised total)
+	// F_B(E) is the product over the blocks v in B of M_v(E), the binary mass of E in block
+	// v (the blocks are independent under the posterior), in variant order
+	const uint32_t nb = (1u << k_) - 1u;
+	std::vector<std::vector<uint32_t>> q(k_, std::vector<uint32_t>(es, es + count));
+	std::vector<std::vector<double>> mv;
+	enum_run(q, mv);
+	std::vector<double> f((size_t)count * nb);
+	for (size_t i = 0; i < count; ++i)
+		for (uint32_t b = 1; b <= nb; ++b) {
+			double x = 1.0;
+			bool first = true;
+			for (int v = 0; v < k_; ++v)
+				if ((b >> v) & 1u) {
+					x = first ? mv[v][i] : x * mv[v][i];
+					first = false;
+				}
+			f[i * nb + (b - 1)] = x;
+		}
+	const double target = 1.0 / double(1u << k_);
+	const uint32_t all = nb;
+	for (size_t i = 0; i < count; ++i) {
+		const double *fm = f.data() + i * nb;
+		double score = 0.0;
+		for (uint32_t resp = 0; resp <= all; ++resp) {
+			const uint32_t positive = (~resp) & all;
+			double exact = 0.0;
+			for (uint32_t extra = resp;; extra = (extra - 1u) & resp) {
+				const uint32_t fs = positive | extra;
+				const double fv = fs == 0 ? 1.0 : fm[fs - 1];
+				if ((__builtin_popcount(extra) & 1) == 0)
+					exact += fv;
+				else
+					exact -= fv;
+				if (extra == 0)
+					break;
+			}
+			score += std::fabs(exact - target);
+		}
+		out[i] = score;
+	}
+}
+
+double DeviceLattice::gather_objective_bound() const
+{
+	// each exact feature mass (at most 1) within exact_rel_bound; an objective sums 3^k of
+	// them with signs, in sums of at most 2^k terms each, and adds 2^k distances
+	const double terms = std::pow(3.0, k_), resp = double(1u << k_);
+	return terms * exact_rel_bound() + (terms * resp + resp) * 0x1p-53;
+}
+
+constexpr int kGatherRescoreMax = kBandSmall;
+constexpr int kGatherRescoreChunk = 512;
+
+void DeviceLattice::rescore_gather(bgt_hip_selection &out, const SelResult &r)
+{
+	// Exact Op-BBPA objectives for the candidates of the error band (lattices of several
+	// variants). The device best's exact objective is at most d1 + eps, so the exact best
+	// has a device objective of at most d1 + 2 eps: the band, collected completely unless it
+	// overflows. Every candidate that may still beat the exact runner-up has a device
+	// objective of at most |d_2| + 2 eps (d_2: the second candidate's); those are rescored
+	// together, in one enumeration on the device.
+	HostBox *box = static_cast<HostBox *>(host_box_);
+	const unsigned count = box->counter[0];
+	const double eps = 0.5 * band_width(kSelBbpa);
+	const bool complete = band_ != nullptr && count <= band_cap_;
+	std::vector<Cand> cands;
+	if (complete) {
+		const unsigned inhead = std::min<unsigned>(count, head_band_);
+		if (count > inhead) {
+			BGT_HIP_CHECK(hipMemcpyAsync(band_host_ + inhead, band_ + inhead, sizeof(Cand) * (count - inhead),
+										 hipMemcpyDeviceToHost, stream_));
+			sync();
+		}
+		cands.assign(band_host_, band_host_ + count);
+	}
+	for (const Cand c : {Cand{r.d1, r.e1, 0}, Cand{r.d2, r.e2, 0}}) {
+		if (c.e == 0xFFFFFFFFu)
+			continue;
+		bool seen = false;
+		for (const Cand &x : cands)
+			seen = seen || x.e == c.e;
+		if (!seen)
+			cands.push_back(c);
+	}
+	std::sort(cands.begin(), cands.end(), [](const Cand &a, const Cand &b) {
+		const double x = std::fabs(a.d), y = std::fabs(b.d);
+		return x < y || (x == y && a.e < b.e);
+	});
+	size_t need = std::min<size_t>(cands.size(), 2);
+	bool whole = complete;
+	if (complete && cands.size() >= 2) {
+		const double reach = std::fabs(cands[1].d) + 2.0 * eps;
+		need = 0;
+		while (need < cands.size() && !(std::fabs(cands[need].d) > reach))
+			++need;
+		need = std::max<size_t>(need, 2);
+	}
+	if (need == 0)
+		return;
+	// Rescored in chunks in order of device objective, until the next candidate's lower bound
+	// |d| - eps exceeds the exact runner-up. Late rounds of sessions of several variants
+	// reach plateaus where hundreds of experiments' objectives lie within the band (all near
+	// their maximum, apart by far less than eps); a candidate costs one binary enumeration
+	// over the n subjects per variant block. At most kGatherRescoreMax are rescored: beyond,
+	// the pick is the exact best of those (which include the device's top two), reported as
+	// a near tie.
+	double b1 = INFINITY, b2 = INFINITY;
+	uint32_t e1 = 0xFFFFFFFFu, e2 = 0xFFFFFFFFu;
+	size_t done = 0;
+	std::vector<uint32_t> es;
+	std::vector<double> obj;
+	while (done < need) {
+		if (done >= 2 && std::fabs(cands[done].d) - eps > b2)
+			break; // this and every later candidate lose to the exact runner-up
+		if (done >= (size_t)kGatherRescoreMax) {
+			whole = false;
+			break;
+		}
+		const size_t take = std::min<size_t>({need - done, (size_t)kGatherRescoreChunk,
+											  (size_t)kGatherRescoreMax - done});
+		es.resize(take);
+		obj.resize(take);
+		for (size_t i = 0; i < take; ++i)
+			es[i] = cands[done + i].e;
+		gather_objectives(es.data(), take, obj.data());
+		for (size_t i = 0; i < take; ++i) {
+			const double d = obj[i];
+			const uint32_t e = es[i];
+			if (d < b1 || (d == b1 && e < e1)) {
+				b2 = b1;
+				e2 = e1;
+				b1 = d;
+				e1 = e;
+			} else if (d < b2 || (d == b2 && e < e2)) {
+				b2 = d;
+				e2 = e;
+			}
+		}
+		done += take;
+	}
+	out.experiment = e1;
+	out.objective = b1;
+	out.runner_up = e2;
+	out.runner_up_objective = b2;
+	if (whole) {
+		out.gap_bound = 2.0 * gather_objective_bound();
+		out.near_tie = (b2 - b1) <= out.gap_bound ? 1 : 0;
+	} else {
+		out.near_tie = 1; // the exact order of the candidates rescored, not of the whole band
+	}
+	out.redecided = whole ? 1 : 2;
+	out.candidates = (int)done;
+	out.band = (int)count;
 }
 
 namespace {
@@ -1277,6 +1513,7 @@ bgt_hip_selection DeviceLattice::select(int selector)
 		clock_.begin(stream_);
 		launch_bbpa_score(ds, 1, n_, k_, target, stream_);
 		clock_.end(stream_, "bbpa_score");
+		collect_gather_band(target);
 		fetch_head();
 		return finish_selection(selector, static_cast<HostBox *>(host_box_)->result, false);
 	}
@@ -1308,6 +1545,7 @@ bgt_hip_selection DeviceLattice::select(int selector)
 		clock_.begin(stream_);
 		launch_bbpa_score(dz, 1, n_, k_, target, stream_);
 		clock_.end(stream_, "bbpa_score");
+		collect_gather_band(target);
 	} else {
 		launch_exact(dz);
 	}
@@ -1550,6 +1788,7 @@ bgt_hip_selection DeviceLattice::round(uint32_t experiment, uint32_t response, c
 		clock_.begin(stream_);
 		launch_bbpa_score(ds, 1, n_, k_, 1.0 / double(1 << k_), stream_);
 		clock_.end(stream_, "bbpa_score");
+		collect_gather_band(1.0 / double(1 << k_));
 		fetch_head();
 		classify_from_head(up, lo, masses_out);
 		return finish_selection(kSelBbpa, static_cast<HostBox *>(host_box_)->result, true);

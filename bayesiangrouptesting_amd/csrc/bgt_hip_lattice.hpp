@@ -165,15 +165,24 @@ private:
 	void fill_lik(uint32_t experiment, uint32_t response, const double *dilution_flat);
 	void rescore(bgt_hip_selection &out, int selector, const SelResult &r);
 	void rescore_band(bgt_hip_selection &out, int selector, const SelResult &r);
+	// lattices of several variants: the Op-BBPA band after k_bbpa_score (k_bbpa_band), and its
+	// exact re-decision from exact feature masses by the device enumeration
+	void collect_gather_band(double target);
+	void rescore_gather(bgt_hip_selection &out, const SelResult &r);
+	void gather_objectives(const uint32_t *experiments, size_t count, double *out);
+	double gather_objective_bound() const;
 	// exact masses of the band on the device (binary non-dilution), after the band kernels
 	void launch_exact(const Descs &ds);
 	// exact masses come from the device enumeration (binary dilution, short enough history)
 	bool uses_enum() const
 	{
-		return dilution_ && k_ == 1 && exact_.usable() && exact_.updates() <= kEnumMaxHist;
+		return (dilution_ || k_ >= 2) && exact_.usable() && exact_.enum_pools() <= kEnumMaxHist;
 	}
-	// M(E) = U(E) / U({}) by launch_exact_enum, U({}) in the same launch; synchronises
+	// M(E) = U(E) / U({}) of atom sets by launch_exact_enum, per variant block (the posterior
+	// is a product over the blocks), U({}) in the same launches; synchronises
 	void enum_masses(const uint32_t *experiments, size_t count, double *out);
+	// masses[v][i] = M_v(queries[v][i]), the binary mass of a set of subjects in block v
+	void enum_run(const std::vector<std::vector<uint32_t>> &queries, std::vector<std::vector<double>> &masses);
 	// the fields of the head now in the host mailbox, with up to band_entries band candidates
 	void unpack_head(int band_entries, size_t band_offset);
 	// complete band of candidates for exact rescoring (binary non-dilution lattices); the

@@ -153,15 +153,16 @@ int bgt_hip_lattice_read_posterior(bgt_hip_lattice *lattice, double *out, uint64
  * lattice_base.cpp:318-351) */
 int bgt_hip_lattice_upset_mass(bgt_hip_lattice *lattice, uint32_t experiment, double *out);
 
-/* Exact FP64 upset masses M(E) of `count` experiments (internal coordinates), recomputed from
- * the prior and the updates applied so far rather than read from the stored posterior: the
- * values near ties are re-decided with. Binary dilution lattices enumerate the states
- * S >= E on the device (histories of at most 64 updates); binary non-dilution lattices run
- * the host's pool program. rel_bound (may be NULL) receives the relative rounding bound.
- * Returns BGT_HIP_UNSUPPORTED for lattices without an exact program (k >= 2, projected
- * lattices, longer dilution histories) or past the program's budget. The reference has no
- * counterpart: its masses are read from its FP64 posterior (get_prob_mass,
- * lattice_base.cpp:318-351), which these equal within their bounds. */
+/* Exact FP64 upset masses M(E) of `count` atom sets E (internal coordinates: subjects*variants
+ * bits), recomputed from the prior and the updates applied so far rather than read from the
+ * stored posterior: the values near ties are re-decided with. Dilution lattices and lattices
+ * of several variants enumerate the states S >= E on the device (at most 64 pools over the
+ * history: updates times variants); binary non-dilution lattices run the host's pool
+ * program. rel_bound (may be NULL) receives the relative rounding bound. Returns
+ * BGT_HIP_UNSUPPORTED for lattices without an exact program (projected lattices, longer
+ * histories) or past the program's budget. The reference has no counterpart: its masses are
+ * read from its FP64 posterior (get_prob_mass, lattice_base.cpp:318-351), which these equal
+ * within their bounds. */
 int bgt_hip_lattice_exact_masses(bgt_hip_lattice *lattice, int count, const uint32_t *experiments, double *out,
 								 double *rel_bound);
 

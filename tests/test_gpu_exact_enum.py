@@ -95,7 +95,7 @@ def test_enumeration_matches_the_oracle_and_the_host_program(n, steps, prec):
     lat.close()
 
 
-def test_non_dilution_lattices_keep_the_host_program_and_k2_has_none():
+def test_non_dilution_lattices_keep_the_host_program():
     n = 12
     pi = seeded_prior(5, n, 0.01, 0.15)
     lat = bgt.Lattice(ND, n, pi, precision=F32)
@@ -106,10 +106,68 @@ def test_non_dilution_lattices_keep_the_host_program_and_k2_has_none():
     got, _ = lat.exact_masses([bgt.reverse_bits(q, n) for q in qs])
     np.testing.assert_array_equal(got, bgt.exact_upset_mass(n, pi, hist, np.array(qs, dtype=np.uint32)))
     lat.close()
-    k2 = bgt.Lattice(DL, 5, seeded_prior(6, 10, 0.01, 0.1), variants=2)
-    with pytest.raises(bgt.Unsupported):
-        k2.exact_masses([1])
-    k2.close()
+
+
+@pytest.mark.parametrize("n,k,dil,steps", [(6, 2, False, 7), (7, 2, True, 9), (5, 3, True, 6), (9, 2, False, 12)])
+def test_several_variants_match_the_oracle(n, k, dil, steps):
+    # upset masses of atom sets over every variant block (the feature masses of Op-BBPA are
+    # these with E placed in the blocks of B), after updates with k-bit responses
+    seed = 120 + n * k
+    atoms = n * k
+    pi = seeded_prior(seed, atoms, 0.01, 0.15)
+    tab = bgt.DilutionTable(n, 0.05, 0.9) if dil else None
+    dtab = orc.dilution_table(n, 0.05, 0.9) if dil else None
+    lat = bgt.Lattice(DL if dil else ND, n, pi, variants=k, precision=F32)
+    ref = orc.prior(pi)
+    g = workload.SplitMix64(seed)
+    for _ in range(steps):
+        e = int(g.next_u64() & g.next_u64() & ((1 << n) - 1)) or 1
+        r = g.below(1 << k)
+        print(f"exact enum k={k}: update e={e:#x} r={r}", flush=True)
+        lat.update(e, r, tab)
+        orc.update_in_place(ref, dil, n, k, e, r, dtab)
+    qs = [0] + [int(g.next_u64() & g.next_u64() & ((1 << atoms) - 1)) for _ in range(30)]
+    got, bound = lat.exact_masses(qs)  # several variants: internal coordinates are public ones
+    want = _superset_masses(ref, atoms, qs)
+    rel = np.abs(got - want) / want
+    assert np.max(rel) <= 1e-12, (np.max(rel), qs[int(np.argmax(rel))])
+    assert 0.0 < bound < 1e-12  # k blocks' masses, each within its enumeration's bound
+    lat.close()
+
+
+@pytest.mark.parametrize("n,k,dil,seed", [(8, 2, False, 3), (10, 2, True, 4), (12, 2, False, 5), (6, 3, True, 6),
+                                          (13, 2, True, 7)])
+def test_fp32c_bbpa_several_variants_picks_the_oracles_experiments(n, k, dil, seed):
+    # FP32C Op-BBPA on k >= 2 lattices: near ties within the error band are re-decided with
+    # exact feature masses, so every pick is the oracle's unless the oracle's own objectives
+    # of the two are within the FP64 tie bound (or the band overflowed: top two only, reported)
+    atoms = n * k
+    pi = seeded_prior(seed, atoms, 0.005, 0.1)
+    tab = bgt.DilutionTable(n, 0.05, 0.9) if dil else None
+    dtab = orc.dilution_table(n, 0.05, 0.9) if dil else None
+    lat = bgt.Lattice(DL if dil else ND, n, pi, variants=k, precision=F32)
+    ref = orc.prior(pi)
+    truth = workload.true_state(seed, list(pi))
+    sampler = workload.ResponseSampler(seed, n, k, truth, list(dtab) if dil else None)
+    redecided = 0
+    for step in range(8):
+        s = lat.select(bgt.SelectorType.op_bbpa)
+        scores = accurate_scores(ref, n, k, 2)
+        e_ref = int(np.argmin(scores))
+        bound = tie_bound(atoms, k, 2, updates=step)
+        print(f"fp32c bbpa n={n} k={k} step {step}: pick {s.experiment:#x} oracle {e_ref:#x} "
+              f"near_tie={s.near_tie} redecided={s.redecided} band={s.band} rescored={s.candidates}", flush=True)
+        if s.experiment != e_ref:
+            gap = float(scores[s.experiment] - scores[e_ref])
+            assert gap <= bound or (s.redecided == 2 and s.near_tie), (step, s.experiment, e_ref, gap, bound, s)
+        if s.redecided == 1:
+            # the reported objective is the exact one
+            assert abs(s.objective - float(scores[s.experiment])) <= 1e-12, (step, s)
+        redecided += s.redecided > 0
+        r = sampler.draw(s.experiment)
+        lat.update(s.experiment, r, tab)
+        orc.update_in_place(ref, dil, n, k, s.experiment, r, dtab)
+    lat.close()
 
 
 @pytest.mark.parametrize("n,seed", [(14, 3), (18, 4), (20, 5)])
