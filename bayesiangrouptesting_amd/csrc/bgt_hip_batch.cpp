@@ -201,8 +201,10 @@ void batch_step(std::vector<BatchItem> &items, int selector, double up, double l
 			liks[i] = it.lat->lik_host();
 			it.lat->commit_update();
 		}
-		if (it.select)
+		if (it.select) {
 			it.lat->ensure_z();
+			it.lat->prepare_gather();
+		}
 		ds[i] = it.lat->desc();
 		ds[i].band_width =
 			it.select ? it.lat->band_width(DeviceLattice::resolve_selector(selector, it.lat->variants())) : 0.0;

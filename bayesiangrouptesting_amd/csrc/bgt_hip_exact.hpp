@@ -78,6 +78,14 @@ public:
 	// and the mass of an atom set is the product of its blocks' masses.
 	void enum_inputs(int variant, double *pairs, uint32_t *pools, double *fac, int stride) const;
 	int subjects() const { return subjects_; }
+	// the prior's (pi_a, 1 - pi_a) for every atom, pairs[2a + bit]
+	void prior_pairs(double *pairs) const
+	{
+		for (int a = 0; a < atoms_; ++a) {
+			pairs[2 * a] = pi_[a];
+			pairs[2 * a + 1] = q_[a];
+		}
+	}
 	// U(E) / U({}) for an unnormalised mass computed elsewhere; false without a normaliser
 	bool normalise(double u, double &out)
 	{

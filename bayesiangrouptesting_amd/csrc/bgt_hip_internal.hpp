@@ -107,6 +107,13 @@ struct ExactRec {
 	double fin, fout;
 };
 
+// Updates a lattice of several variants keeps for the exact Op-BBPA re-decision on the device.
+constexpr int kGatherMaxHist = 32;
+// P(r_v = 1) of a non-dilution test whose pool holds no positive subject (replicated_lattice.cpp:6)
+constexpr double kNegativeResponse = 0.99;
+// counter slot: the selection was re-decided exactly on the device (k_gather_pick)
+constexpr int kGatherFlag = 3;
+
 // Everything a kernel needs to know about one lattice. Arrays of these live in device
 // memory so that one launch can serve a batch of lattices (blockIdx.y selects).
 struct LatDesc {
@@ -151,6 +158,16 @@ struct LatDesc {
 	unsigned long long *fixtab;
 	// Op-BBPA error band of lattices of several variants (k_bbpa_band; null: not collected)
 	Cand *gband;
+	// their exact re-decision on the device (k_gather_exact, k_gather_pick): the update
+	// history (pools and responses), the dilution table it was applied with (null:
+	// non-dilution), and the exact objectives of the band entries; gh_len < 0: no exact
+	// program (a longer history, or updates with different tables)
+	uint32_t gh_e[kGatherMaxHist];
+	uint8_t gh_r[kGatherMaxHist];
+	int gh_len;
+	const double *gh_pairs; // the prior's (pi_a, 1 - pi_a), indexed 2a + bit
+	const double *gh_dil;
+	double *gexact;
 };
 
 // Device exact rescoring covers the first kExactBand band candidates, histories of at
@@ -280,7 +297,13 @@ void launch_gather_heads(const LatDesc *many, int count, unsigned char *out, hip
 void launch_bbpa_score(const Descs &descs, int count, int n, int k, double target,
 					   hipStream_t stream);
 // after launch_bbpa_score: the candidates within d.band_width of the best into d.gband
-// (lattices of several variants that collect a band; the others return at once)
+// (lattices of several variants that collect a band; the others return at once), then, where
+// the top two are within the band (a near tie) and the band is complete, their exact FP64
+// objectives from the prior and the update history and the exact best two into d.result
+// (counter[kGatherFlag] = 1). The posterior is a product over the variant blocks (the prior
+// is a product over atoms and each likelihood factor depends on one block,
+// replicated_lattice.cpp:8-28), so a feature mass F_B(E) is the product over v in B of the
+// binary mass M_v(E) = U_v(E) / U_v({}), U_v summed over the 2^(n - |E|) states of block v.
 void launch_bbpa_band(const Descs &descs, int count, int n, int k, double target, hipStream_t stream);
 // Op-BBPA feature tables of k >= 2 lattices in one pass over the posterior (bbpa.cpp:237-313)
 // where each lane can hold its entries of every table (n >= 6, (2^k - 1) 2^(n-6) <= 64, more

@@ -1848,6 +1848,141 @@ __global__ __launch_bounds__(256) void k_bbpa_band(Descs descs, int ns, int k, d
 	}
 }
 
+// Exact re-decision of an Op-BBPA near tie on a lattice of several variants (see
+// launch_bbpa_band): each workgroup takes band entries blockIdx.x, + gridDim.x, ... and writes
+// their exact objectives to d.gexact. Factors as the update kernels' likelihood tables
+// (fill_lik: the same FP64 expressions).
+constexpr unsigned kGatherGroups = 256;
+
+__device__ __forceinline__ bool gather_redecides(const LatDesc &d)
+{
+	if (d.gband == nullptr || d.gh_len < 0)
+		return false;
+	const double d1 = d.result->d1, d2 = d.result->d2;
+	return (d2 - d1) <= d.band_width && d.counter[0] <= d.band_cap;
+}
+
+// U_v(E) of block v: the sum over the states S >= E of the n-subject block of the prior
+// factors of its atoms times the likelihood factors of the history's updates for variant v,
+// every thread a strided share of the states in increasing order, then the workgroup tree
+__device__ double gather_block_mass(const LatDesc &d, const double *s_pairs, int v, uint32_t e, double *red)
+{
+	const int n = d.n;
+	const uint32_t full = (1u << n) - 1u;
+	const uint32_t freeb = full & ~e;
+	const int f = __popc(freeb);
+	double acc = 0.0;
+	for (uint32_t j = threadIdx.x; j < (1u << f); j += blockDim.x) {
+		// deposit j's bits at the free subjects, in increasing order
+		uint32_t s = e, r = freeb;
+		for (int i = 0; i < f; ++i) {
+			const uint32_t b = r & (0u - r);
+			r &= r - 1u;
+			s |= ((j >> i) & 1u) ? b : 0u;
+		}
+		double p = 1.0;
+		for (int i = 0; i < n; ++i)
+			p *= s_pairs[2 * (v * n + i) + ((s >> i) & 1u)];
+		for (int t = 0; t < d.gh_len; ++t) {
+			const uint32_t pool = d.gh_e[t];
+			const int len = __popc(pool);
+			const int m = __popc(pool & ~s); // positive pooled subjects of block v
+			const bool rb = (d.gh_r[t] >> v) & 1u;
+			double fv;
+			if (d.gh_dil != nullptr) {
+				const double p1 = len > 0 ? d.gh_dil[(size_t)(len - 1) * (n + 1) + m] : 0.0;
+				fv = rb ? p1 : 1.0 - p1;
+			} else if (m == 0) {
+				fv = rb ? kNegativeResponse : (1.0 - kNegativeResponse);
+			} else {
+				fv = rb ? (1.0 - kNegativeResponse) : kNegativeResponse;
+			}
+			p *= fv;
+		}
+		acc += p;
+	}
+	const double sum = block_sum(acc, red);
+	__shared__ double s_sum;
+	if (threadIdx.x == 0)
+		s_sum = sum;
+	__syncthreads();
+	const double out = s_sum;
+	__syncthreads();
+	return out;
+}
+
+__global__ __launch_bounds__(256) void k_gather_exact(Descs descs, int k, double target)
+{
+	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
+	if (!gather_redecides(d))
+		return;
+	const unsigned count = d.counter[0];
+	if (blockIdx.x >= count)
+		return;
+	__shared__ double s_pairs[2 * kMaxAtoms];
+	__shared__ double red[16];
+	for (int i = threadIdx.x; i < 2 * d.atoms; i += blockDim.x)
+		s_pairs[i] = d.gh_pairs[i];
+	__syncthreads();
+	double u0[kMaxVariants];
+	for (int v = 0; v < k; ++v)
+		u0[v] = gather_block_mass(d, s_pairs, v, 0u, red);
+	for (unsigned i = blockIdx.x; i < count; i += gridDim.x) {
+		const uint32_t e = d.gband[i].e;
+		double mv[kMaxVariants];
+		for (int v = 0; v < k; ++v)
+			mv[v] = gather_block_mass(d, s_pairs, v, e, red) / u0[v];
+		// bbpa.cpp:416-460 over F_B = prod_{v in B} M_v (F_{} = 1), in variant order
+		const uint32_t all = (1u << k) - 1u;
+		double score = 0.0;
+		for (uint32_t resp = 0; resp <= all; ++resp) {
+			const uint32_t positive = (~resp) & all;
+			double exact = 0.0;
+			for (uint32_t extra = resp;; extra = (extra - 1u) & resp) {
+				const uint32_t fs = positive | extra;
+				double fv = 1.0;
+				bool first = true;
+				for (int v = 0; v < k; ++v)
+					if ((fs >> v) & 1u) {
+						fv = first ? mv[v] : fv * mv[v];
+						first = false;
+					}
+				if ((__popc(extra) & 1) == 0)
+					exact += fv;
+				else
+					exact -= fv;
+				if (extra == 0)
+					break;
+			}
+			score += fabs(exact - target);
+		}
+		if (threadIdx.x == 0)
+			d.gexact[i] = score;
+	}
+}
+
+// the exact best two of the band by (objective, experiment) into d.result
+__global__ __launch_bounds__(1024) void k_gather_pick(Descs descs)
+{
+	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
+	if (!gather_redecides(d))
+		return;
+	__shared__ Top2 red2[16];
+	const unsigned count = d.counter[0];
+	Top2 t;
+	top2_init(t);
+	for (unsigned i = threadIdx.x; i < count; i += blockDim.x)
+		top2_insert(t, d.gexact[i], d.gband[i].e);
+	t = top2_block(t, red2);
+	if (threadIdx.x == 0) {
+		d.result->d1 = t.d1;
+		d.result->d2 = t.d2;
+		d.result->e1 = t.e1;
+		d.result->e2 = t.e2;
+		d.counter[kGatherFlag] = 1u;
+	}
+}
+
 // Explicit (experiment, state) sweep (benchmarks/experiment_selection_benchmark.cpp:214-247):
 // one workgroup per experiment, buckets per thread, fixed-order reductions.
 template <typename TP, int K>
@@ -2512,6 +2647,7 @@ __global__ __launch_bounds__(1024) void k_finalize_top2(Descs descs, unsigned co
 		d.result->e2 = r.e2;
 		d.counter[0] = 0; // band collection (runs next, in stream order)
 		d.counter[1] = 0;
+		d.counter[kGatherFlag] = 0;
 	}
 	// workgroup flags of the band rescan, one per workgroup of the launch just finished
 	if (d.wgflag != nullptr)
@@ -3290,6 +3426,8 @@ void launch_bbpa_band(const Descs &descs, int count, int n, int k, double target
 	const dim3 grid(score_groups(n), count);
 	launch_checked("k_bbpa_band", [&] {
 		hipLaunchKernelGGL(k_bbpa_band, grid, dim3(256), 0, stream, descs, n, k, target);
+		hipLaunchKernelGGL(k_gather_exact, dim3(kGatherGroups, count), dim3(256), 0, stream, descs, k, target);
+		hipLaunchKernelGGL(k_gather_pick, dim3(1, count), dim3(1024), 0, stream, descs);
 	});
 }
 

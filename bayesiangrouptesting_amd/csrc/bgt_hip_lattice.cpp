@@ -141,7 +141,6 @@ void KernelClock::reset()
 
 namespace {
 
-constexpr double kNegativeResponse = 0.99; // replicated_lattice.cpp:6
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
@@ -215,6 +214,11 @@ void DeviceLattice::init_prior(const double *prior)
 	clock_.reset();
 	clock_.enabled = false;
 	exact_.reset(true, atoms_, prior, dilution_, k_);
+	gh_e_.clear();
+	gh_r_.clear();
+	gh_dil_.clear();
+	gh_broken_ = false;
+	gh_uploaded_ = false;
 	pairs_own_ = true;
 	std::vector<double> pairs(2 * kMaxAtoms, 0.0);
 	for (int a = 0; a < atoms_; ++a) {
@@ -388,6 +392,7 @@ DeviceLattice::DeviceLattice(const DeviceLattice &parent, bool copy_posterior)
 	updates_ = parent.updates_;
 	inv_cur_ = parent.inv_cur_;
 	exact_ = parent.exact_;
+	copy_gather_history(parent);
 	if (copy_posterior) {
 		// order the copy after the parent's pending work
 		BGT_HIP_CHECK(hipStreamSynchronize(parent.stream_));
@@ -402,7 +407,7 @@ namespace {
 
 struct ArenaLayout {
 	size_t scal, res, cnt, mass, band, cand, flags, hist, exact_u, pairs, part, tops, ftab, fix, fix_bytes, scores, sel,
-		total;
+		gexact, ghin, total;
 	bool exact;
 };
 
@@ -427,9 +432,10 @@ ArenaLayout arena_layout(bool dilution, int n, int k)
 	};
 	// binary lattices collect their error band for exact re-decision; the device's dense
 	// program (k_exact_band) covers the non-dilution ones. Lattices of several variants
-	// collect their Op-BBPA band (k_bbpa_band) in the same place, right after the 1 KB block.
+	// collect their Op-BBPA band (k_bbpa_band) in the same place, right after the 1 KB block,
+	// with room for every experiment (2^n, n <= 15), so it never overflows.
 	a.exact = k == 1;
-	const size_t gband = k >= 2 ? sizeof(Cand) * kBandSmall : 0;
+	const size_t gband = k >= 2 ? sizeof(Cand) * ec : 0;
 	a.scal = take(sizeof(double) * kScalars); // 0
 	a.res = take(sizeof(SelResult));         // 256
 	a.cnt = take(sizeof(unsigned) * 4);      // 512
@@ -447,6 +453,10 @@ ArenaLayout arena_layout(bool dilution, int n, int k)
 	a.fix = take(a.fix_bytes);
 	a.scores = take(sizeof(double) * scores);
 	a.sel = take(sizeof(uint32_t) * 16);
+	// several variants: exact objectives of the band, and the exact program's inputs (the
+	// prior pairs and the dilution table)
+	a.gexact = take(k >= 2 ? sizeof(double) * ec : 0);
+	a.ghin = take(k >= 2 ? sizeof(double) * (2 * atoms + n * (n + 1)) : 0);
 	a.total = std::max<size_t>(off, a.exact ? kHeadBytesWide : kHeadBytes);
 	return a;
 }
@@ -496,8 +506,10 @@ void DeviceLattice::allocate()
 		band_host_ = static_cast<Cand *>(pinned_alloc(sizeof(Cand) * band_cap_));
 	} else if (k_ >= 2) {
 		band_ = reinterpret_cast<Cand *>(arena_ + a.band);
-		band_cap_ = (unsigned)kBandSmall;
+		band_cap_ = 1u << n_; // every experiment
 		band_host_ = static_cast<Cand *>(pinned_alloc(sizeof(Cand) * band_cap_));
+		gexact_ = reinterpret_cast<double *>(arena_ + a.gexact);
+		ghin_ = reinterpret_cast<double *>(arena_ + a.ghin);
 	}
 	double init[kScalars] = {1.0, 1.0, 1.0, 1.0, 0.0, 0.0, 0.0, 0.0};
 	BGT_HIP_CHECK(hipMemcpyAsync(scal_, init, sizeof(init), hipMemcpyHostToDevice, stream_));
@@ -541,6 +553,7 @@ void DeviceLattice::adopt(const DeviceLattice &parent)
 	updates_ = parent.updates_;
 	inv_cur_ = parent.inv_cur_;
 	exact_ = parent.exact_;
+	copy_gather_history(parent);
 	hist_uploaded_ = 0;
 	exact_pending_ = exact_valid_ = false;
 	pairs_own_ = false; // the device prior pairs are those this lattice was created with
@@ -581,6 +594,17 @@ LatDesc DeviceLattice::desc() const
 	d.cand = cand_;
 	d.band = k_ == 1 ? band_ : nullptr;
 	d.gband = k_ >= 2 ? band_ : nullptr;
+	d.gh_len = -1;
+	if (k_ >= 2 && ghin_ != nullptr && gh_uploaded_ && !gh_broken_ && exact_.usable()) {
+		d.gh_len = (int)gh_e_.size();
+		for (size_t t = 0; t < gh_e_.size(); ++t) {
+			d.gh_e[t] = gh_e_[t];
+			d.gh_r[t] = gh_r_[t];
+		}
+		d.gh_pairs = ghin_;
+		d.gh_dil = dilution_ ? ghin_ + 2 * atoms_ : nullptr;
+		d.gexact = gexact_;
+	}
 	d.band_width = 0.0;
 	d.wgflag = wgflag_;
 	d.band_cap = band_cap_;
@@ -658,7 +682,52 @@ void DeviceLattice::commit_update()
 	inv_cur_ = inv_cur_ == kScalInv0 ? kScalInv1 : kScalInv0;
 	++updates_;
 	exact_.push(e_, r_, pending_dil_);
+	if (k_ >= 2 && !gh_broken_) {
+		// the history of the exact Op-BBPA re-decision on the device (one dilution table only)
+		if ((int)gh_e_.size() >= kGatherMaxHist) {
+			gh_broken_ = true;
+		} else {
+			gh_e_.push_back(e_);
+			gh_r_.push_back((uint8_t)r_);
+			if (dilution_) {
+				const size_t cells = (size_t)n_ * (n_ + 1);
+				if (gh_dil_.empty()) {
+					gh_dil_.assign(pending_dil_, pending_dil_ + cells);
+					gh_uploaded_ = false;
+				} else if (std::memcmp(gh_dil_.data(), pending_dil_, sizeof(double) * cells) != 0) {
+					gh_broken_ = true;
+				}
+			}
+		}
+	}
 	pending_dil_ = nullptr;
+}
+
+void DeviceLattice::copy_gather_history(const DeviceLattice &parent)
+{
+	gh_e_ = parent.gh_e_;
+	gh_r_ = parent.gh_r_;
+	gh_dil_ = parent.gh_dil_;
+	gh_broken_ = parent.gh_broken_;
+	gh_uploaded_ = false;
+}
+
+void DeviceLattice::prepare_gather()
+{
+	// the exact program's inputs into the arena (the prior pairs, the dilution table), once
+	// per lattice and table; synchronous, so that any stream's kernels may read them
+	if (k_ < 2 || ghin_ == nullptr || gh_broken_ || gh_uploaded_ || !exact_.usable())
+		return;
+	const size_t cells = 2 * (size_t)atoms_ + (size_t)n_ * (n_ + 1);
+	double *h = static_cast<double *>(pinned_alloc(sizeof(double) * cells));
+	exact_.prior_pairs(h);
+	for (size_t i = 0; i < (size_t)n_ * (n_ + 1); ++i)
+		h[2 * atoms_ + i] = i < gh_dil_.size() ? gh_dil_[i] : 0.0;
+	const hipError_t e1 = hipMemcpyAsync(ghin_, h, sizeof(double) * cells, hipMemcpyHostToDevice, stream_);
+	const hipError_t e2 = e1 == hipSuccess ? hipStreamSynchronize(stream_) : e1;
+	pinned_release(h, sizeof(double) * cells);
+	BGT_HIP_CHECK(e2);
+	gh_uploaded_ = true;
 }
 
 void DeviceLattice::update(uint32_t experiment, uint32_t response, const double *dil)
@@ -805,12 +874,20 @@ bgt_hip_selection DeviceLattice::finish_selection(int selector, const SelResult 
 	out.near_tie = (r.d2 - r.d1) <= out.gap_bound ? 1 : 0;
 	if (cand_ != nullptr)
 		out.collected = (int)static_cast<const HostBox *>(host_box_)->counter[2];
-	if (out.near_tie && found && exact_.usable()) {
-		if (k_ == 1 && (sel == kSelBha || sel == kSelBbpa))
-			rescore(out, sel, r);
-		else if (k_ >= 2 && sel == kSelBbpa && uses_enum())
-			rescore_gather(out, r);
+	if (k_ >= 2 && sel == kSelBbpa && found) {
+		const HostBox *box = static_cast<const HostBox *>(host_box_);
+		if (box->counter[kGatherFlag] == 1u) {
+			// re-decided on the device (k_gather_pick): the result holds the exact best two
+			out.gap_bound = 2.0 * gather_objective_bound();
+			out.near_tie = (r.d2 - r.d1) <= out.gap_bound ? 1 : 0;
+			out.redecided = 1;
+			out.candidates = (int)box->counter[0];
+			out.band = (int)box->counter[0];
+		}
+		return out;
 	}
+	if (out.near_tie && found && exact_.usable() && k_ == 1 && (sel == kSelBha || sel == kSelBbpa))
+		rescore(out, sel, r);
 	return out;
 }
 
@@ -996,61 +1073,15 @@ void DeviceLattice::collect_band(const Descs &ds, unsigned entries, int objectiv
 
 void DeviceLattice::collect_gather_band(double target)
 {
-	// lattices of several variants whose near ties can be re-decided (a short enough history)
-	if (k_ < 2 || band_ == nullptr || !uses_enum())
+	// lattices of several variants: the band, and its exact re-decision where it applies
+	if (k_ < 2 || band_ == nullptr)
 		return;
+	prepare_gather();
 	Descs ds = single();
 	ds.one.band_width = band_width(kSelBbpa);
 	clock_.begin(stream_);
 	launch_bbpa_band(ds, 1, n_, k_, target, stream_);
 	clock_.end(stream_, "bbpa_band");
-}
-
-void DeviceLattice::gather_objectives(const uint32_t *es, size_t count, double *out)
-{
-	// exact feature masses F_B(E): the mass of the states whose blocks v in B all contain E,
-	// i.e. the upset mass of E placed in each of those blocks; then the objective of
-	// bbpa.cpp:416-460 over them (F_{} = 1, the normalised total)
-	// F_B(E) is the product over the blocks v in B of M_v(E), the binary mass of E in block
-	// v (the blocks are independent under the posterior), in variant order
-	const uint32_t nb = (1u << k_) - 1u;
-	std::vector<std::vector<uint32_t>> q(k_, std::vector<uint32_t>(es, es + count));
-	std::vector<std::vector<double>> mv;
-	enum_run(q, mv);
-	std::vector<double> f((size_t)count * nb);
-	for (size_t i = 0; i < count; ++i)
-		for (uint32_t b = 1; b <= nb; ++b) {
-			double x = 1.0;
-			bool first = true;
-			for (int v = 0; v < k_; ++v)
-				if ((b >> v) & 1u) {
-					x = first ? mv[v][i] : x * mv[v][i];
-					first = false;
-				}
-			f[i * nb + (b - 1)] = x;
-		}
-	const double target = 1.0 / double(1u << k_);
-	const uint32_t all = nb;
-	for (size_t i = 0; i < count; ++i) {
-		const double *fm = f.data() + i * nb;
-		double score = 0.0;
-		for (uint32_t resp = 0; resp <= all; ++resp) {
-			const uint32_t positive = (~resp) & all;
-			double exact = 0.0;
-			for (uint32_t extra = resp;; extra = (extra - 1u) & resp) {
-				const uint32_t fs = positive | extra;
-				const double fv = fs == 0 ? 1.0 : fm[fs - 1];
-				if ((__builtin_popcount(extra) & 1) == 0)
-					exact += fv;
-				else
-					exact -= fv;
-				if (extra == 0)
-					break;
-			}
-			score += std::fabs(exact - target);
-		}
-		out[i] = score;
-	}
 }
 
 double DeviceLattice::gather_objective_bound() const
@@ -1059,111 +1090,6 @@ double DeviceLattice::gather_objective_bound() const
 	// them with signs, in sums of at most 2^k terms each, and adds 2^k distances
 	const double terms = std::pow(3.0, k_), resp = double(1u << k_);
 	return terms * exact_rel_bound() + (terms * resp + resp) * 0x1p-53;
-}
-
-constexpr int kGatherRescoreMax = kBandSmall;
-constexpr int kGatherRescoreChunk = 512;
-
-void DeviceLattice::rescore_gather(bgt_hip_selection &out, const SelResult &r)
-{
-	// Exact Op-BBPA objectives for the candidates of the error band (lattices of several
-	// variants). The device best's exact objective is at most d1 + eps, so the exact best
-	// has a device objective of at most d1 + 2 eps: the band, collected completely unless it
-	// overflows. Every candidate that may still beat the exact runner-up has a device
-	// objective of at most |d_2| + 2 eps (d_2: the second candidate's); those are rescored
-	// together, in one enumeration on the device.
-	HostBox *box = static_cast<HostBox *>(host_box_);
-	const unsigned count = box->counter[0];
-	const double eps = 0.5 * band_width(kSelBbpa);
-	const bool complete = band_ != nullptr && count <= band_cap_;
-	std::vector<Cand> cands;
-	if (complete) {
-		const unsigned inhead = std::min<unsigned>(count, head_band_);
-		if (count > inhead) {
-			BGT_HIP_CHECK(hipMemcpyAsync(band_host_ + inhead, band_ + inhead, sizeof(Cand) * (count - inhead),
-										 hipMemcpyDeviceToHost, stream_));
-			sync();
-		}
-		cands.assign(band_host_, band_host_ + count);
-	}
-	for (const Cand c : {Cand{r.d1, r.e1, 0}, Cand{r.d2, r.e2, 0}}) {
-		if (c.e == 0xFFFFFFFFu)
-			continue;
-		bool seen = false;
-		for (const Cand &x : cands)
-			seen = seen || x.e == c.e;
-		if (!seen)
-			cands.push_back(c);
-	}
-	std::sort(cands.begin(), cands.end(), [](const Cand &a, const Cand &b) {
-		const double x = std::fabs(a.d), y = std::fabs(b.d);
-		return x < y || (x == y && a.e < b.e);
-	});
-	size_t need = std::min<size_t>(cands.size(), 2);
-	bool whole = complete;
-	if (complete && cands.size() >= 2) {
-		const double reach = std::fabs(cands[1].d) + 2.0 * eps;
-		need = 0;
-		while (need < cands.size() && !(std::fabs(cands[need].d) > reach))
-			++need;
-		need = std::max<size_t>(need, 2);
-	}
-	if (need == 0)
-		return;
-	// Rescored in chunks in order of device objective, until the next candidate's lower bound
-	// |d| - eps exceeds the exact runner-up. Late rounds of sessions of several variants
-	// reach plateaus where hundreds of experiments' objectives lie within the band (all near
-	// their maximum, apart by far less than eps); a candidate costs one binary enumeration
-	// over the n subjects per variant block. At most kGatherRescoreMax are rescored: beyond,
-	// the pick is the exact best of those (which include the device's top two), reported as
-	// a near tie.
-	double b1 = INFINITY, b2 = INFINITY;
-	uint32_t e1 = 0xFFFFFFFFu, e2 = 0xFFFFFFFFu;
-	size_t done = 0;
-	std::vector<uint32_t> es;
-	std::vector<double> obj;
-	while (done < need) {
-		if (done >= 2 && std::fabs(cands[done].d) - eps > b2)
-			break; // this and every later candidate lose to the exact runner-up
-		if (done >= (size_t)kGatherRescoreMax) {
-			whole = false;
-			break;
-		}
-		const size_t take = std::min<size_t>({need - done, (size_t)kGatherRescoreChunk,
-											  (size_t)kGatherRescoreMax - done});
-		es.resize(take);
-		obj.resize(take);
-		for (size_t i = 0; i < take; ++i)
-			es[i] = cands[done + i].e;
-		gather_objectives(es.data(), take, obj.data());
-		for (size_t i = 0; i < take; ++i) {
-			const double d = obj[i];
-			const uint32_t e = es[i];
-			if (d < b1 || (d == b1 && e < e1)) {
-				b2 = b1;
-				e2 = e1;
-				b1 = d;
-				e1 = e;
-			} else if (d < b2 || (d == b2 && e < e2)) {
-				b2 = d;
-				e2 = e;
-			}
-		}
-		done += take;
-	}
-	out.experiment = e1;
-	out.objective = b1;
-	out.runner_up = e2;
-	out.runner_up_objective = b2;
-	if (whole) {
-		out.gap_bound = 2.0 * gather_objective_bound();
-		out.near_tie = (b2 - b1) <= out.gap_bound ? 1 : 0;
-	} else {
-		out.near_tie = 1; // the exact order of the candidates rescored, not of the whole band
-	}
-	out.redecided = whole ? 1 : 2;
-	out.candidates = (int)done;
-	out.band = (int)count;
 }
 
 namespace {

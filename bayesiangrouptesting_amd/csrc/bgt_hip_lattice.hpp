@@ -147,6 +147,9 @@ public:
 	// relative rounding bound of exact_masses
 	double exact_rel_bound() const;
 	bool uses_exact_enum() const { return uses_enum(); }
+	// lattices of several variants: the device's exact program inputs uploaded (batched steps
+	// call it while preparing, before their launches)
+	void prepare_gather();
 	// Take over `parent`'s host-side state (masks, candidate restriction, update history,
 	// pending normalisation slot) ahead of an out-of-place update from it; the posterior
 	// itself is produced by that update. Both must have the same shape and storage.
@@ -168,9 +171,8 @@ private:
 	// lattices of several variants: the Op-BBPA band after k_bbpa_score (k_bbpa_band), and its
 	// exact re-decision from exact feature masses by the device enumeration
 	void collect_gather_band(double target);
-	void rescore_gather(bgt_hip_selection &out, const SelResult &r);
-	void gather_objectives(const uint32_t *experiments, size_t count, double *out);
 	double gather_objective_bound() const;
+	void copy_gather_history(const DeviceLattice &parent);
 	// exact masses of the band on the device (binary non-dilution), after the band kernels
 	void launch_exact(const Descs &ds);
 	// exact masses come from the device enumeration (binary dilution, short enough history)
@@ -219,6 +221,14 @@ private:
 	unsigned *wgflag_ = nullptr; // workgroups of the final transform to rescan
 	unsigned band_cap_ = 0;
 	Cand *band_host_ = nullptr; // pinned host copy of the band
+	// several variants: the device re-decision's history (pools, responses, the dilution
+	// table), exact objectives of the band and the arena copy of the program's inputs
+	std::vector<uint32_t> gh_e_;
+	std::vector<uint8_t> gh_r_;
+	std::vector<double> gh_dil_;
+	bool gh_broken_ = false, gh_uploaded_ = false;
+	double *gexact_ = nullptr;
+	double *ghin_ = nullptr;
 	unsigned head_band_ = 0;    // band candidates that came back with the last head
 	ExactRec *hist_dev_ = nullptr; // update history for the device exact program
 	double *exact_dev_ = nullptr;  // its exact masses of the first kExactBand band candidates
