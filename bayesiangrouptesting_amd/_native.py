@@ -101,6 +101,9 @@ def _sig(L):
         "bgt_hip_tune": ([c_int, c_int, c_int, c_int, c_int, c_int, ctypes.POINTER(c_int), _dptr, _dptr,
                           ctypes.c_char_p, c_int], c_int),
         "bgt_hip_tune_force": ([c_int], c_int),
+        "bgt_hip_tune_force_pass1": ([c_int], c_int),
+        "bgt_hip_tune_forms": ([c_int, c_int, c_int, c_int, c_int, c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int),
+                                _dptr, ctypes.c_char_p, c_int], c_int),
         "bgt_hip_select_noalloc": ([ctypes.c_void_p, c_int, c_int, c_int, c_double, ctypes.c_void_p, ctypes.c_void_p,
                                     ctypes.c_void_p], c_int),
         "bgt_hip_lattice_create": ([c_int, c_int, c_int, _dptr, c_int, ctypes.POINTER(_H)], c_int),

@@ -552,40 +552,83 @@ int bgt_hip_batch_graph_counts(unsigned long long *captures, unsigned long long 
 	return guarded([&] { bgt_hip::batch_graph_counts(captures, replays); });
 }
 
+namespace {
+
+bgt_hip::TuneResult run_tune(int use_dilution, int subjects, int variants, int precision, int selector, int force,
+							 char *spec_id, int spec_len)
+{
+	if (subjects <= 0 || variants <= 0 || subjects * variants > bgt_hip::kMaxAtoms)
+		throw bgt_hip::Error(BGT_HIP_BAD_INPUT, "tune: shape out of range");
+	bgt_hip::require_device();
+	bgt_hip::TuneKey key;
+	key.dilution = use_dilution != 0;
+	key.subjects = subjects;
+	key.variants = variants;
+	key.selector = selector == BGT_HIP_SELECT_AUTO ? (variants == 1 ? BGT_HIP_SELECT_OP_BHA : BGT_HIP_SELECT_OP_BBPA)
+												   : selector;
+	key.storage = storage_of(precision);
+	const bgt_hip::TuneResult r = bgt_hip::tune(key, force != 0);
+	if (spec_id && spec_len > 0) {
+		std::strncpy(spec_id, r.id.c_str(), (size_t)spec_len - 1);
+		spec_id[spec_len - 1] = '\0';
+	}
+	return r;
+}
+
+} // namespace
+
 int bgt_hip_tune(int use_dilution, int subjects, int variants, int precision, int selector, int force, int *form,
 				 double *ms_two_pass, double *ms_three_pass, char *spec_id, int spec_len)
 {
 	return guarded([&] {
-		if (subjects <= 0 || variants <= 0 || subjects * variants > bgt_hip::kMaxAtoms)
-			throw bgt_hip::Error(BGT_HIP_BAD_INPUT, "tune: shape out of range");
-		bgt_hip::require_device();
-		bgt_hip::TuneKey key;
-		key.dilution = use_dilution != 0;
-		key.subjects = subjects;
-		key.variants = variants;
-		key.selector = selector == BGT_HIP_SELECT_AUTO ? (variants == 1 ? BGT_HIP_SELECT_OP_BHA : BGT_HIP_SELECT_OP_BBPA)
-													   : selector;
-		key.storage = storage_of(precision);
-		const bgt_hip::TuneResult r = bgt_hip::tune(key, force != 0);
+		const bgt_hip::TuneResult r = run_tune(use_dilution, subjects, variants, precision, selector, force, spec_id,
+											   spec_len);
 		if (form)
 			*form = r.form;
+		// each form with its faster pass-1 grid
+		auto best = [&](int f2) {
+			const double a = r.ms[f2][0], b = r.ms[f2][1];
+			return (a > 0.0 && (b <= 0.0 || a <= b)) ? a : b;
+		};
 		if (ms_two_pass)
-			*ms_two_pass = r.ms[0];
+			*ms_two_pass = best(0);
 		if (ms_three_pass)
-			*ms_three_pass = r.ms[1];
-		if (spec_id && spec_len > 0) {
-			std::strncpy(spec_id, r.id.c_str(), (size_t)spec_len - 1);
-			spec_id[spec_len - 1] = '\0';
-		}
+			*ms_three_pass = best(1);
+	});
+}
+
+int bgt_hip_tune_forms(int use_dilution, int subjects, int variants, int precision, int selector, int force,
+					   int *pass2_form, int *pass1_form, double *ms, char *spec_id, int spec_len)
+{
+	return guarded([&] {
+		const bgt_hip::TuneResult r = run_tune(use_dilution, subjects, variants, precision, selector, force, spec_id,
+											   spec_len);
+		if (pass2_form)
+			*pass2_form = r.form;
+		if (pass1_form)
+			*pass1_form = r.pass1;
+		if (ms)
+			for (int f2 = 0; f2 < bgt_hip::kPass2Forms; ++f2)
+				for (int f1 = 0; f1 < bgt_hip::kPass1Forms; ++f1)
+					ms[f2 * bgt_hip::kPass1Forms + f1] = r.ms[f2][f1];
 	});
 }
 
 int bgt_hip_tune_force(int form)
 {
 	return guarded([&] {
-		if (form < -1 || form > 1)
-			throw bgt_hip::Error(BGT_HIP_BAD_INPUT, "tune_force: form is -1, 0 or 1");
+		if (form < -1 || form >= bgt_hip::kPass2Forms)
+			throw bgt_hip::Error(BGT_HIP_BAD_INPUT, "tune_force: form is -1, 0, 1 or 2");
 		bgt_hip::set_pass2_form(form);
+	});
+}
+
+int bgt_hip_tune_force_pass1(int form)
+{
+	return guarded([&] {
+		if (form < -1 || form >= bgt_hip::kPass1Forms)
+			throw bgt_hip::Error(BGT_HIP_BAD_INPUT, "tune_force_pass1: form is -1, 0 or 1");
+		bgt_hip::set_pass1_form(form);
 	});
 }
 

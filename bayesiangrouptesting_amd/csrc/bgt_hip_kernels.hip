@@ -3163,7 +3163,7 @@ void launch_zeta_pass1(const Descs &descs, int count, int n, Storage st, bool fu
 	// FP32 storage: persistent workgroups (one per CU) that prefetch their next tile, once
 	// there are more tiles than CUs; otherwise one workgroup per tile
 	const unsigned tiles = 1u << (n - kTileBits);
-	const bool persistent = st == Storage::fp32c && (!fused || (k == 1 && !dilution));
+	const bool persistent = st == Storage::fp32c && (!fused || (k == 1 && !dilution)) && tuned_pass1_form(n, st) != 1;
 	const unsigned groups = persistent ? zeta_final_groups(n) : tiles;
 	const dim3 grid(groups, count);
 	const bool binary_nd = (k == 1 && !dilution);
@@ -3223,10 +3223,12 @@ bool persistent_pass2(int n, int objective, bool from_posterior)
 	return !from_posterior && n >= 21 && n <= 30 && (objective == kObjHalving || objective == kObjBinaryBbpa);
 }
 
-// the split pass 2 (FP32, 28..30 atoms, two-pass form) reads z in the blocked form
+// the split pass 2 (FP32, 28..30 atoms, two-pass form 0) reads z in the blocked form; the
+// plain pass 2 (two-pass form 2, chosen by the tuner or pinned) reads the row form
 bool blocked_z(int n, Storage st, int objective)
 {
-	return st == Storage::fp32c && n >= 28 && persistent_pass2(n, objective, false) && !three_pass_form(n, st);
+	return st == Storage::fp32c && n >= 28 && persistent_pass2(n, objective, false) && !three_pass_form(n, st) &&
+		   tuned_pass2_form(n, st) != 2;
 }
 
 // The final transform kernel of a selection (pass 2, or the single pass of 15..20 atoms),

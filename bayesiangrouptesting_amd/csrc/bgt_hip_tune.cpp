@@ -8,9 +8,11 @@
 // measured once per key on the device and kept in memory and in a cache file of the same
 // naming scheme, so a later process starts from the measured choice.
 //
-// Tuned: the form of the high-bit transform for FP32 lattices of 26..30 atoms (two passes,
-// the second reading 2^(30-A)- or 2^(31-A)-entry pieces, or three passes reading whole
-// 128-byte lines; see the kernels). Everything else is fixed by the shape.
+// Tuned, for FP32 lattices of 26..30 atoms: the form of the high-bit transform (two passes
+// with the split pass 2 on the blocked z, reading 32-byte pieces; three passes reading whole
+// 128-byte lines; two passes with the plain pass 2 on the row-form z, reading 2^(30-A)-entry
+// pieces) and the grid of pass 1 (persistent with a prefetch of the next tile, or one
+// workgroup per tile); see the kernels. Everything else is fixed by the shape.
 #include "bgt_hip_tune.hpp"
 
 #include "bgt_hip_lattice.hpp"
@@ -33,8 +35,12 @@ namespace bgt_hip {
 namespace {
 
 std::mutex g_mu;
-int g_forced = -1;                    // set_pass2_form: -1 none
-std::map<std::string, int> g_choices; // spec id -> pass-2 form
+int g_forced = -1;  // set_pass2_form: -1 none
+int g_forced1 = -1; // set_pass1_form: -1 none
+struct Choice {
+	int pass2 = 0, pass1 = 0;
+};
+std::map<std::string, Choice> g_choices; // spec id -> forms
 
 // Provider value of this provider in the reference's enum once a `hip` value is added after
 // auto_select, cpu, cuda (types.hpp:96-101); precision values of Precision (float32 = 0,
@@ -80,14 +86,33 @@ size_t spec_hash(const TuneKey &k)
 
 bool applies(int atoms, Storage st) { return st == Storage::fp32c && atoms >= 26 && atoms <= 30; }
 
-int read_choice(const std::filesystem::path &f)
+bool read_choice(const std::filesystem::path &f, Choice &c)
 {
 	std::ifstream in(f);
 	std::string line;
-	while (std::getline(in, line))
+	int p2 = -1, p1 = -1;
+	while (std::getline(in, line)) {
 		if (line.rfind("pass2_form=", 0) == 0)
-			return std::atoi(line.c_str() + 11);
-	return -1;
+			p2 = std::atoi(line.c_str() + 11);
+		else if (line.rfind("pass1_form=", 0) == 0)
+			p1 = std::atoi(line.c_str() + 11);
+	}
+	if (p2 < 0 || p2 >= kPass2Forms || p1 < 0 || p1 >= kPass1Forms)
+		return false;
+	c.pass2 = p2;
+	c.pass1 = p1;
+	return true;
+}
+
+// the remembered choice of any tuned key of this atom count and storage (the transform does
+// not depend on dilution or selector)
+const Choice *choice_for(int atoms, Storage st)
+{
+	for (const auto &kv : g_choices)
+		if (kv.first.find("-n" + std::to_string(atoms) + "-v1-") != std::string::npos &&
+			kv.first.find("-r" + std::to_string(precision_value(st)) + "-") != std::string::npos)
+			return &kv.second;
+	return nullptr;
 }
 
 } // namespace
@@ -107,21 +132,35 @@ int tuned_pass2_form(int atoms, Storage st)
 	if (!applies(atoms, st))
 		return 0;
 	std::lock_guard<std::mutex> lock(g_mu);
-	if (g_forced >= 0)
-		return g_forced;
-	// the transform does not depend on dilution or selector: any tuned key of this atom
-	// count and storage decides
-	for (const auto &kv : g_choices)
-		if (kv.first.find("-n" + std::to_string(atoms) + "-v1-") != std::string::npos &&
-			kv.first.find("-r" + std::to_string(precision_value(st)) + "-") != std::string::npos)
-			return kv.second;
-	return -1;
+	int f = g_forced;
+	if (f < 0) {
+		const Choice *c = choice_for(atoms, st);
+		f = c ? c->pass2 : -1;
+	}
+	return (f == 2 && atoms < 28) ? 0 : f; // below 28 atoms the plain pass 2 is form 0
+}
+
+int tuned_pass1_form(int atoms, Storage st)
+{
+	if (!applies(atoms, st))
+		return 0;
+	std::lock_guard<std::mutex> lock(g_mu);
+	if (g_forced1 >= 0)
+		return g_forced1;
+	const Choice *c = choice_for(atoms, st);
+	return c ? c->pass1 : -1;
 }
 
 void set_pass2_form(int form)
 {
 	std::lock_guard<std::mutex> lock(g_mu);
 	g_forced = form;
+}
+
+void set_pass1_form(int form)
+{
+	std::lock_guard<std::mutex> lock(g_mu);
+	g_forced1 = form;
 }
 
 TuneResult tune(const TuneKey &key_in, bool force)
@@ -132,6 +171,7 @@ TuneResult tune(const TuneKey &key_in, bool force)
 	r.id = spec_id(key);
 	if (key.variants != 1 || !applies(key.subjects, key.storage)) {
 		r.form = 0;
+		r.pass1 = 0;
 		r.from_cache = false;
 		return r;
 	}
@@ -140,55 +180,65 @@ TuneResult tune(const TuneKey &key_in, bool force)
 		std::lock_guard<std::mutex> lock(g_mu);
 		auto it = g_choices.find(r.id);
 		if (it != g_choices.end()) {
-			r.form = it->second;
+			r.form = it->second.pass2;
+			r.pass1 = it->second.pass1;
 			r.from_cache = true;
 			return r;
 		}
 		std::error_code ec;
-		if (std::filesystem::exists(file, ec)) {
-			const int c = read_choice(file);
-			if (c == 0 || c == 1) {
-				g_choices[r.id] = c;
-				r.form = c;
-				r.from_cache = true;
-				return r;
-			}
+		Choice c;
+		if (std::filesystem::exists(file, ec) && read_choice(file, c)) {
+			g_choices[r.id] = c;
+			r.form = c.pass2;
+			r.pass1 = c.pass1;
+			r.from_cache = true;
+			return r;
 		}
 	}
-	// measure: the selection of a prior lattice of this shape under each form (pass 1 and
-	// the band work are common to both; only the high-bit transform differs)
+	// measure: the selection of a prior lattice of this shape under each pair of forms (the
+	// band work is common to all; pass 1's grid and the high-bit transform differ)
 	const int n = key.subjects;
 	std::vector<double> prior(n);
 	for (int a = 0; a < n; ++a)
 		prior[a] = 0.005 + 0.045 * (a + 0.5) / n;
 	auto lat = DeviceLattice::make(key.dilution, n, 1, prior.data(), key.storage);
-	int saved;
+	int saved2, saved1;
 	{
 		std::lock_guard<std::mutex> lock(g_mu);
-		saved = g_forced;
+		saved2 = g_forced;
+		saved1 = g_forced1;
 	}
-	double best[2] = {0.0, 0.0};
-	for (int form = 0; form < 2; ++form) {
-		set_pass2_form(form);
-		double mn = 1e30;
-		for (int i = 0; i < 7; ++i) {
-			const auto t0 = std::chrono::steady_clock::now();
-			lat->select(key.selector);
-			const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-			if (i >= 2)
-				mn = std::min(mn, ms);
+	double best = 1e300;
+	for (int f2 = 0; f2 < kPass2Forms; ++f2) {
+		if (f2 == 2 && n < 28)
+			continue; // the plain pass 2 is form 0 there
+		for (int f1 = 0; f1 < kPass1Forms; ++f1) {
+			set_pass2_form(f2);
+			set_pass1_form(f1);
+			double mn = 1e30;
+			for (int i = 0; i < 7; ++i) {
+				const auto t0 = std::chrono::steady_clock::now();
+				lat->select(key.selector);
+				const double ms =
+					std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+				if (i >= 2)
+					mn = std::min(mn, ms);
+			}
+			r.ms[f2][f1] = mn;
+			if (mn < best) {
+				best = mn;
+				r.form = f2;
+				r.pass1 = f1;
+			}
 		}
-		best[form] = mn;
 	}
-	set_pass2_form(saved);
+	set_pass2_form(saved2);
+	set_pass1_form(saved1);
 	DeviceLattice::recycle(std::move(lat));
-	r.form = best[1] < best[0] ? 1 : 0;
-	r.ms[0] = best[0];
-	r.ms[1] = best[1];
 	r.from_cache = false;
 	{
 		std::lock_guard<std::mutex> lock(g_mu);
-		g_choices[r.id] = r.form;
+		g_choices[r.id] = Choice{r.form, r.pass1};
 	}
 	std::error_code ec;
 	std::filesystem::create_directories(cache_dir(), ec);
@@ -203,8 +253,10 @@ TuneResult tune(const TuneKey &key_in, bool force)
 			<< "precision=" << precision_value(key.storage) << '\n'
 			<< "gpu_arch=" << key.arch << '\n'
 			<< "pass2_form=" << r.form << '\n'
-			<< "select_ms_two_pass=" << best[0] << '\n'
-			<< "select_ms_three_pass=" << best[1] << '\n';
+			<< "pass1_form=" << r.pass1 << '\n';
+		for (int f2 = 0; f2 < kPass2Forms; ++f2)
+			for (int f1 = 0; f1 < kPass1Forms; ++f1)
+				out << "select_ms_" << f2 << "_" << f1 << "=" << r.ms[f2][f1] << '\n';
 	}
 	return r;
 }

@@ -179,16 +179,21 @@ def run_sessions(precision, n, pi, steps, warmup, seed, drive=None, clock=2):
             "classified": classified, "sessions": sess + 1, "lattice": lat}
 
 
+PASS2_FORMS = ["two-pass split on blocked z", "three-pass", "two-pass plain on row z"]
+PASS1_FORMS = ["persistent with prefetch", "one workgroup per tile"]
+
+
 def tune_forms(n) -> dict:
-    """The autotuner's choice of the FP32C transform form at this size (measured once, untimed)."""
+    """The autotuner's choice of the FP32C transform's launch forms at this size (measured once,
+    untimed): the high-bit form and the pass-1 grid, with the select time of every pair."""
     from bayesiangrouptesting_amd import _native as nat
-    f = ctypes.c_int(-1)
-    two, three = ctypes.c_double(), ctypes.c_double()
+    p2, p1 = ctypes.c_int(-1), ctypes.c_int(-1)
+    ms = (ctypes.c_double * 6)()
     sid = ctypes.create_string_buffer(256)
-    nat.check(nat.lib().bgt_hip_tune(0, n, 1, nat.FP32C, 1, 1, ctypes.byref(f), ctypes.byref(two),
-                                     ctypes.byref(three), sid, 256))
-    return {"spec": sid.value.decode(), "form": "three-pass" if f.value == 1 else "two-pass",
-            "select_ms_two_pass": two.value, "select_ms_three_pass": three.value}
+    nat.check(nat.lib().bgt_hip_tune_forms(0, n, 1, nat.FP32C, 1, 1, ctypes.byref(p2), ctypes.byref(p1), ms,
+                                           sid, 256))
+    return {"spec": sid.value.decode(), "form": PASS2_FORMS[p2.value], "pass1": PASS1_FORMS[p1.value],
+            "select_ms": {f"{PASS2_FORMS[a]} / {PASS1_FORMS[b]}": ms[a * 2 + b] for a in range(3) for b in range(2)}}
 
 
 def cpu_baseline(n, pi, steps, all_cores: bool = False):

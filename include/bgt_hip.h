@@ -310,22 +310,32 @@ int bgt_hip_batch_graph_counts(unsigned long long *captures, unsigned long long 
 
 /* ---- autotuning ---- */
 
-/* Measure once per KernelSpec key (include/bgt/types.hpp:132-181) which launch form of the
- * transform's high bits is fastest on this device for lattices of this shape -- the two-pass
- * form or the three-pass form for FP32C lattices of 26..30 atoms -- and use it from then on.
- * Results are cached in memory and in a text file per key named as the reference's JIT
+/* Measure once per KernelSpec key (include/bgt/types.hpp:132-181) which launch forms of the
+ * transform are fastest on this device for lattices of this shape, and use them from then on.
+ * For FP32C lattices of 26..30 atoms: the form of the high bits (0: two passes, from 28 atoms
+ * the split pass 2 on the blocked intermediate; 1: three passes; 2: two passes with the plain
+ * pass 2 on the row-form intermediate, 28..30 atoms) and the grid of pass 1 (0: persistent
+ * with a prefetch of the next tile; 1: one workgroup per tile); every other shape has one
+ * form. Results are cached in memory and in a text file per key named as the reference's JIT
  * artefacts (kernel_spec_id, src/runtime/jit_compiler.cpp:13-38) in BGT_HIP_TUNE_DIR
  * (default: the temporary directory's bayesian_group_testing_hip_tune); force = 1 measures
- * again. form receives 0 (two passes) or 1 (three passes); ms_* the measured selection times
- * (0 when taken from a cache or when the shape has one form only); spec_id (may be NULL) the
- * key. Run it before sessions of that shape start: the choice applies process-wide. The
- * reference's KernelRegistry / JitCompiler have no device measurement. */
+ * again. form receives the high-bit form; ms_two_pass / ms_three_pass the measured selection
+ * time of forms 0 and 1, each with its faster pass-1 grid (0 when taken from a cache or when
+ * the shape has one form only); spec_id (may be NULL) the key. Run it before sessions of that
+ * shape start: the choice applies process-wide. The reference's KernelRegistry /
+ * JitCompiler have no device measurement. */
 int bgt_hip_tune(int use_dilution, int subjects, int variants, int precision, int selector, int force, int *form,
 				 double *ms_two_pass, double *ms_three_pass, char *spec_id, int spec_len);
 
-/* Pin the transform form process-wide (0: two passes, 1: three passes where it applies);
- * -1 returns to the tuned choice, or the built-in default (two passes). */
+/* As bgt_hip_tune, with both choices and every measurement: ms (may be NULL) receives 6
+ * selection times, ms[pass2_form * 2 + pass1_form]. No reference counterpart. */
+int bgt_hip_tune_forms(int use_dilution, int subjects, int variants, int precision, int selector, int force,
+					   int *pass2_form, int *pass1_form, double *ms, char *spec_id, int spec_len);
+
+/* Pin the high-bit form (0, 1, 2 as above) or the pass-1 grid (0, 1) process-wide; -1
+ * returns to the tuned choice, or the built-in default (form 0, persistent pass 1). */
 int bgt_hip_tune_force(int form);
+int bgt_hip_tune_force_pass1(int form);
 
 /* ---- selection without device allocation ---- */
 

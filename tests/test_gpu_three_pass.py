@@ -30,16 +30,19 @@ FP32C = bgt.Precision.float32_compensated
 
 @pytest.fixture
 def form():
-    def force(f):
+    def force(f, p1=-1):
         assert nat.lib().bgt_hip_tune_force(f) == nat.OK
+        assert nat.lib().bgt_hip_tune_force_pass1(p1) == nat.OK
     yield force
     force(-1)
 
 
-@pytest.mark.parametrize("n,dilution,f", [(26, False, 1), (27, True, 1), (28, False, 1), (28, False, 0), (28, True, 0)])
-def test_three_pass_selection_against_the_oracle(form, n, dilution, f):
-    # f = 0 at 28 atoms: the two-pass split form on z in the blocked layout
-    form(f)
+@pytest.mark.parametrize("n,dilution,f,p1", [(26, False, 1, -1), (27, True, 1, -1), (28, False, 1, -1), (28, False, 0, -1),
+                                             (28, True, 0, -1), (28, False, 2, -1), (28, True, 2, 1), (28, False, 0, 1)])
+def test_three_pass_selection_against_the_oracle(form, n, dilution, f, p1):
+    # f = 0 at 28 atoms: the two-pass split form on z in the blocked layout; f = 2: the plain
+    # pass 2 on the row layout; p1 = 1: pass 1 with one workgroup per tile
+    form(f, p1)
     pi = seeded_prior(700 + n, n, 0.005, 0.05)
     lat = bgt.Lattice(DL if dilution else ND, n, pi, precision=FP32C)
     t = bgt.DilutionTable(n, 0.05, 0.9) if dilution else None
@@ -106,22 +109,30 @@ def test_tuner_measures_caches_and_applies(tmp_path, monkeypatch, form):
     f = ctypes.c_int(-1)
     two, three = ctypes.c_double(), ctypes.c_double()
     sid = ctypes.create_string_buffer(256)
-    assert L.bgt_hip_tune(0, 28, 1, nat.FP32C, 1, 1, ctypes.byref(f), ctypes.byref(two), ctypes.byref(three),
-                          sid, 256) == nat.OK, L.bgt_hip_last_error()
-    assert f.value in (0, 1)
-    assert two.value > 0 and three.value > 0
-    assert f.value == (1 if three.value < two.value else 0)
+    p2, p1 = ctypes.c_int(-1), ctypes.c_int(-1)
+    ms = (ctypes.c_double * 6)()
+    assert L.bgt_hip_tune_forms(0, 28, 1, nat.FP32C, 1, 1, ctypes.byref(p2), ctypes.byref(p1), ms, sid, 256) == nat.OK, \
+        L.bgt_hip_last_error()
+    # every pair of forms measured; the choice is the fastest
+    table = list(ms)
+    assert all(t > 0 for t in table)
+    assert table[p2.value * 2 + p1.value] == min(table)
     name = sid.value.decode()
     assert name.startswith("kernel-") and "-n28-v1-s32-d0-q1-r2-" in name and name.endswith("gfx950")
     path = os.path.join(str(tmp_path), name + ".bgthip")
     assert os.path.exists(path)
     with open(path) as fh:
-        assert f"pass2_form={f.value}" in fh.read()
-    # a second call is served from the cache without measuring
+        text = fh.read()
+    assert f"pass2_form={p2.value}" in text and f"pass1_form={p1.value}" in text
+    # a second call is served from the cache without measuring, through either entry point
     g = ctypes.c_int(-1)
     assert L.bgt_hip_tune(0, 28, 1, nat.FP32C, 1, 0, ctypes.byref(g), ctypes.byref(two), ctypes.byref(three),
                           None, 0) == nat.OK
-    assert g.value == f.value and two.value == 0.0 and three.value == 0.0
+    assert g.value == p2.value and two.value == 0.0 and three.value == 0.0
+    # and the cached pair through the entry point that reports both
+    q2, q1 = ctypes.c_int(-1), ctypes.c_int(-1)
+    assert L.bgt_hip_tune_forms(0, 28, 1, nat.FP32C, 1, 0, ctypes.byref(q2), ctypes.byref(q1), None, None, 0) == nat.OK
+    assert (q2.value, q1.value) == (p2.value, p1.value)
     # FP64 and small shapes have one form only
     assert L.bgt_hip_tune(0, 28, 1, nat.FP64, 1, 0, ctypes.byref(g), None, None, None, 0) == nat.OK
     assert g.value == 0
