@@ -321,6 +321,52 @@ def shrink_workload(repeats: int = 10) -> dict:
     return out
 
 
+def graphs_workload(steps: int = 24, lattices: int = 48) -> dict:
+    """Batched steps of a fixed set of lattices of uneven shape (binary, n = 10..16 with and
+    without dilution, FP32C), every lattice updated and re-selected each step, with the steps
+    launched and with the steps replayed from captured hipGraphs (one capture per launch
+    structure, then replays). Per-step host-call-to-results time, mean over the steps after the
+    first two; the two runs follow the same responses and must select the same experiments."""
+    import bayesiangrouptesting_amd as bgt
+    from bayesiangrouptesting_amd import workload
+
+    shapes = [(10 + (i % 7), (i // 7) % 2 == 1) for i in range(lattices)]
+    ND, DL = bgt.LatticeType.replicated_non_dilution, bgt.LatticeType.replicated_dilution
+    tables = [bgt.DilutionTable(n, 0.05, 0.9) if d else None for n, d in shapes]
+    out = {"workload": f"{lattices} binary lattices, n = 10..16, dilution and not, FP32C; {steps} batched rounds "
+                       "(update, classification, Op-BHA select) of every lattice, launched and replayed from hipGraphs"}
+    picks = {}
+    prev = bgt.set_batch_graphs(False)
+    try:
+        for use in (False, True):
+            lats = [bgt.Lattice(DL if d else ND, n, workload.priors(300 + i, n, 0.005, 0.1),
+                                precision=bgt.Precision.float32_compensated) for i, (n, d) in enumerate(shapes)]
+            g = workload.SplitMix64(17)
+            pending = [None] * lattices
+            times, seq = [], []
+            c0, r0 = bgt.batch_graph_counts()
+            bgt.set_batch_graphs(use)
+            for step in range(steps):
+                resps = [None if e is None else g.below(2) for e in pending]
+                t0 = time.perf_counter()
+                sels, _ = bgt.batch_round(lats, pending, resps, bgt.SelectorType.op_bha, THRESHOLD, THRESHOLD, tables)
+                times.append(time.perf_counter() - t0)
+                pending = [s.experiment for s in sels]
+                seq.append(tuple(pending))
+            bgt.set_batch_graphs(False)
+            c1, r1 = bgt.batch_graph_counts()
+            key = "graphs" if use else "launched"
+            out[key] = {"step_ms": float(np.mean(times[2:])) * 1e3, "captures": c1 - c0, "replays": r1 - r0}
+            picks[key] = seq
+            for lat in lats:
+                lat.close()
+    finally:
+        bgt.set_batch_graphs(prev)
+    out["identical_selections"] = picks["graphs"] == picks["launched"]
+    out["speedup"] = out["launched"]["step_ms"] / out["graphs"]["step_ms"]
+    return out
+
+
 def bbpa_workload(rounds: int = 8, repeats: int = 3) -> dict:
     """Multinomial dilution lattice n = 8, k = 3 (2^24 states), DilutionTable(8, 0.05, 0.9),
     priors U[0.01, 0.10]: `rounds` rounds of Op-BBPA select -> update with responses sampled
@@ -615,6 +661,8 @@ def main() -> int:
         line["bbpa"] = bbpa_workload()
         log("shrink (n=26 -> 24)")
         line["shrink"] = shrink_workload()
+        log("batched steps: launched and graph replay")
+        line["batch_graphs"] = graphs_workload()
         log(f"serving ({args.serving_sessions} sessions at {args.serving_rate:g}/s)")
         line["serving"] = serving_workload(args.serving_sessions, args.serving_rate)
         log("serving n=28 sessions")
