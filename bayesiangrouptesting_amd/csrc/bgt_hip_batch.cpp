@@ -308,9 +308,8 @@ void batch_step(std::vector<BatchItem> &items, int selector, double up, double l
 			const int subjects = g.second.max_subjects;
 			if (objective == kObjGather && bbpa_hist_path(lat.subjects(), k)) {
 				// as DeviceLattice::select: the feature tables from one histogram pass
-				launch_bbpa_hist(dg, cnt, lat.subjects(), k, lat.storage(), false, s);
-				launch_bbpa_score(dg, cnt, lat.subjects(), k, target, s);
-				launch_bbpa_band(dg, cnt, lat.subjects(), k, target, s);
+				launch_bbpa_hist(dg, cnt, lat.subjects(), k, lat.storage(), false, s, false);
+				launch_bbpa_finish(dg, cnt, k, false, target, s);
 				continue;
 			}
 			unsigned entries = 0;

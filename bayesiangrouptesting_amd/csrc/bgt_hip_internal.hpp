@@ -312,7 +312,15 @@ void launch_bbpa_band(const Descs &descs, int count, int n, int k, double target
 // denominator (slot inv_out).
 bool bbpa_hist_path(int n, int k);
 size_t bbpa_fix_bytes(int n, int k);
-void launch_bbpa_hist(const Descs &descs, int count, int n, int k, Storage st, bool fused, hipStream_t stream);
+void launch_bbpa_hist(const Descs &descs, int count, int n, int k, Storage st, bool fused, hipStream_t stream,
+					  bool tables = true);
+// After the one-pass table kernel alone (launch_bbpa_hist with tables = false): the tables,
+// the scores, the top two, the band and the exact re-decision in one workgroup per lattice
+// (the one-pass path has at most 10 subjects, 1,024 experiments); the results of
+// launch_bbpa_tables + launch_bbpa_score + launch_bbpa_band, the exact objectives summed per
+// band entry in state order (bit-identical between per-lattice and batched calls, which both
+// take this path).
+void launch_bbpa_finish(const Descs &descs, int count, int k, bool fused, double target, hipStream_t stream);
 // Unnormalised exact upset masses U(E) of the first kExactBand band candidates of each
 // lattice (d.exact_u[i], -1 where the device program does not apply), from d.pairs and
 // d.hist: the quantity ExactHistory computes on the host, as a dense dynamic program.

@@ -1983,6 +1983,186 @@ __global__ __launch_bounds__(1024) void k_gather_pick(Descs descs)
 	}
 }
 
+// The rest of an Op-BBPA selection on the one-pass tables, in one workgroup per lattice
+// (every experiment of at most 10 subjects one thread): the tables as k_bbpa_tables, each
+// experiment's objective as k_bbpa_score (the same operations), the top two as
+// k_finalize_top2 (the order (|objective|, experiment) is total, so the same two), the band as
+// k_bbpa_band, and at a near tie the exact re-decision: every state S of every variant block v
+// gets its weight w_v(S) (the prior factors of the block's atoms times the history's factors
+// for variant v) in LDS, thread i sums band entry i's per-variant binary masses
+// U_v(E) = sum of w_v(S) over S >= E in state order, against the block normalisers U_v({})
+// summed by the workgroup, and forms its exact objective; the exact best two go into
+// d.result (counter[kGatherFlag] = 1). Six launches become one. The one-pass path holds at
+// most 2^10 states per block and 2048 weights in all ((2^k - 1) 2^(n-6) <= 64).
+constexpr int kFinishWeights = 2048;
+__global__ __launch_bounds__(1024) void k_bbpa_finish(Descs descs, int fused, int k, double target)
+{
+	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
+	__shared__ Top2 red2[16];
+	__shared__ double red[16];
+	__shared__ double s_pairs[2 * kMaxAtoms];
+	__shared__ double s_w[kFinishWeights];
+	__shared__ double s_u0[kMaxVariants];
+	__shared__ unsigned s_count;
+	__shared__ double s_lim;
+	__shared__ int s_redecide;
+	const int n = d.n;
+	const uint32_t ec = 1u << n;
+	const unsigned cells = ((1u << k) - 1u) * ec;
+	unsigned long long *g = d.fixtab;
+	// 1. the tables (k_bbpa_tables)
+	const double t = fix_value(&g[3u * cells]);
+	const double scale = fused ? 1.0 / t : 1.0;
+	for (unsigned i = threadIdx.x; i < cells; i += blockDim.x)
+		d.ftab[(size_t)ec + i] = fix_value(&g[3u * i]) * scale;
+	for (unsigned i = threadIdx.x; i < (unsigned)(k * n); i += blockDim.x) {
+		const unsigned v = i / n, a = i % n;
+		d.masses[i] = fix_value(&g[3u * (((1u << v) - 1u) * ec + (1u << a))]) * scale;
+	}
+	const double total = fused ? t * scale : t;
+	if (threadIdx.x == 0) {
+		if (fused) {
+			d.scal[kScalDen] = t;
+			d.scal[d.inv_out] = scale;
+			d.scal[kScalTotal] = total;
+		} else {
+			d.scal[kScalTotal] = t;
+		}
+		s_count = 0u;
+	}
+	__syncthreads(); // every thread has read the bins; the table writes are visible
+	for (unsigned i = threadIdx.x; i < 3u * (cells + 1u); i += blockDim.x)
+		g[i] = 0ull;
+	// 2. objectives and the top two (k_bbpa_score, k_finalize_top2)
+	const uint32_t e = threadIdx.x;
+	double score = INFINITY;
+	const bool mine = e < ec && (e & ~d.active) == 0u;
+	if (mine)
+		score = bbpa_objective(d.ftab, ec, e, k, total, target);
+	Top2 tt;
+	top2_init(tt);
+	if (mine)
+		top2_insert(tt, score, e);
+	tt = top2_block(tt, red2);
+	if (threadIdx.x == 0) {
+		d.result->d1 = fabs(tt.d1);
+		d.result->d2 = fabs(tt.d2);
+		d.result->e1 = tt.e1;
+		d.result->e2 = tt.e2;
+		d.counter[0] = 0;
+		d.counter[1] = 0;
+		d.counter[kGatherFlag] = 0;
+		s_lim = fabs(tt.d1) + d.band_width;
+		s_redecide = d.gband != nullptr && d.gh_len >= 0 && (fabs(tt.d2) - fabs(tt.d1)) <= d.band_width;
+	}
+	__syncthreads();
+	if (d.gband == nullptr)
+		return;
+	// 3. the band (k_bbpa_band): every selectable experiment within band_width of the best
+	const bool in = mine && score <= s_lim;
+	unsigned at = 0u;
+	if (in)
+		at = atomicAdd(&s_count, 1u);
+	if (in)
+		d.gband[at] = Cand{score, e, 0u};
+	__syncthreads();
+	const unsigned count = s_count;
+	if (threadIdx.x == 0)
+		d.counter[0] = count;
+	if (!s_redecide)
+		return;
+	// 4. the exact re-decision: the block weights, their normalisers, then one band entry per
+	// thread
+	for (int i = threadIdx.x; i < 2 * d.atoms; i += blockDim.x)
+		s_pairs[i] = d.gh_pairs[i];
+	__syncthreads();
+	for (uint32_t idx = threadIdx.x; idx < (uint32_t)k * ec; idx += blockDim.x) {
+		const int v = (int)(idx >> n);
+		const uint32_t st = idx & (ec - 1u);
+		double p = 1.0;
+		for (int i = 0; i < n; ++i)
+			p *= s_pairs[2 * (v * n + i) + ((st >> i) & 1u)];
+		for (int q = 0; q < d.gh_len; ++q) {
+			const uint32_t pool = d.gh_e[q];
+			const int len = __popc(pool);
+			const int m = __popc(pool & ~st); // positive pooled subjects of block v
+			const bool rb = (d.gh_r[q] >> v) & 1u;
+			double fv;
+			if (d.gh_dil != nullptr) {
+				const double p1 = len > 0 ? d.gh_dil[(size_t)(len - 1) * (n + 1) + m] : 0.0;
+				fv = rb ? p1 : 1.0 - p1;
+			} else if (m == 0) {
+				fv = rb ? kNegativeResponse : (1.0 - kNegativeResponse);
+			} else {
+				fv = rb ? (1.0 - kNegativeResponse) : kNegativeResponse;
+			}
+			p *= fv;
+		}
+		s_w[idx] = p;
+	}
+	__syncthreads();
+	for (int v = 0; v < k; ++v) {
+		double acc = 0.0;
+		for (uint32_t st = threadIdx.x; st < ec; st += blockDim.x)
+			acc += s_w[((uint32_t)v << n) + st];
+		const double u = block_sum(acc, red);
+		if (threadIdx.x == 0)
+			s_u0[v] = u;
+		__syncthreads();
+	}
+	Top2 ex;
+	top2_init(ex);
+	if (threadIdx.x < count) {
+		const uint32_t ce = d.gband[threadIdx.x].e;
+		const uint32_t freeb = (ec - 1u) & ~ce;
+		double mv[kMaxVariants];
+		for (int v = 0; v < k; ++v) {
+			// the supersets of ce in increasing order: st = ce | sub, sub over the subsets of
+			// the free subjects by the (sub - freeb) & freeb step
+			const double *w = s_w + ((uint32_t)v << n);
+			double acc = 0.0;
+			uint32_t sub = 0u;
+			do {
+				acc += w[ce | sub];
+				sub = (sub - freeb) & freeb;
+			} while (sub != 0u);
+			mv[v] = acc / s_u0[v];
+		}
+		const uint32_t all = (1u << k) - 1u;
+		double sc = 0.0;
+		for (uint32_t resp = 0; resp <= all; ++resp) {
+			const uint32_t positive = (~resp) & all;
+			double exact = 0.0;
+			for (uint32_t extra = resp;; extra = (extra - 1u) & resp) {
+				const uint32_t fs = positive | extra;
+				double fv = 1.0;
+				bool first = true;
+				for (int v = 0; v < k; ++v)
+					if ((fs >> v) & 1u) {
+						fv = first ? mv[v] : fv * mv[v];
+						first = false;
+					}
+				if ((__popc(extra) & 1) == 0)
+					exact += fv;
+				else
+					exact -= fv;
+				if (extra == 0)
+					break;
+			}
+			sc += fabs(exact - target);
+		}
+		top2_insert(ex, sc, ce);
+	}
+	ex = top2_block(ex, red2);
+	if (threadIdx.x == 0) {
+		d.result->d1 = ex.d1;
+		d.result->d2 = ex.d2;
+		d.result->e1 = ex.e1;
+		d.result->e2 = ex.e2;
+		d.counter[kGatherFlag] = 1u;
+	}
+}
+
 // Explicit (experiment, state) sweep (benchmarks/experiment_selection_benchmark.cpp:214-247):
 // one workgroup per experiment, buckets per thread, fixed-order reductions.
 template <typename TP, int K>
@@ -3423,6 +3603,16 @@ void launch_bbpa_score(const Descs &descs, int count, int n, int k, double targe
 	});
 }
 
+void launch_bbpa_finish(const Descs &descs, int count, int k, bool fused, double target, hipStream_t stream)
+{
+	const int n = descs.many ? 0 : descs.one.n; // batched groups hold one shape of the one-pass path
+	if (k < 2 || k > kMaxVariants || (n > 0 && ((unsigned)k << n) > (unsigned)kFinishWeights))
+		throw Error(4, "launch_bbpa_finish: the one-pass path's shapes only");
+	launch_checked("k_bbpa_finish", [&] {
+		hipLaunchKernelGGL(k_bbpa_finish, dim3(1, count), dim3(1024), 0, stream, descs, fused ? 1 : 0, k, target);
+	});
+}
+
 void launch_bbpa_band(const Descs &descs, int count, int n, int k, double target, hipStream_t stream)
 {
 	const dim3 grid(score_groups(n), count);
@@ -3446,7 +3636,8 @@ size_t bbpa_fix_bytes(int n, int k)
 	return bbpa_hist_path(n, k) ? sizeof(unsigned long long) * 3u * (((1u << k) - 1u) * (1u << n) + 1u) : 0;
 }
 
-void launch_bbpa_hist(const Descs &descs, int count, int n, int k, Storage st, bool fused, hipStream_t stream)
+void launch_bbpa_hist(const Descs &descs, int count, int n, int k, Storage st, bool fused, hipStream_t stream,
+					  bool tables)
 {
 	if (!bbpa_hist_path(n, k))
 		throw Error(4, "one-pass feature tables need k >= 2, n >= 6 and (2^k - 1) 2^(n-6) <= 64");
@@ -3479,6 +3670,8 @@ void launch_bbpa_hist(const Descs &descs, int count, int n, int k, Storage st, b
 #undef BGT_HIST
 		throw Error(4, "no one-pass table kernel for this shape");
 	});
+	if (!tables)
+		return; // launch_bbpa_finish converts them
 	launch_checked("k_bbpa_tables", [&] {
 		hipLaunchKernelGGL(k_bbpa_tables, dim3(1, count), dim3(1024), 0, stream, descs, fused ? 1 : 0);
 	});

@@ -1433,13 +1433,15 @@ bgt_hip_selection DeviceLattice::select(int selector)
 	const double target = sel == kSelBha ? 0.5 : 1.0 / double(1 << k_);
 	if (objective == kObjGather && bbpa_hist_path(n_, k_)) {
 		// the feature tables from one histogram pass (bbpa.cpp:237-313), no transform
+		prepare_gather();
+		Descs dh = single();
+		dh.one.band_width = band_width(sel);
 		clock_.begin(stream_, true);
-		launch_bbpa_hist(ds, 1, n_, k_, storage_, false, stream_);
+		launch_bbpa_hist(dh, 1, n_, k_, storage_, false, stream_, false);
 		clock_.end(stream_, "bbpa_hist");
 		clock_.begin(stream_);
-		launch_bbpa_score(ds, 1, n_, k_, target, stream_);
-		clock_.end(stream_, "bbpa_score");
-		collect_gather_band(target);
+		launch_bbpa_finish(dh, 1, k_, false, target, stream_);
+		clock_.end(stream_, "bbpa_finish");
 		fetch_head();
 		return finish_selection(selector, static_cast<HostBox *>(host_box_)->result, false);
 	}
@@ -1708,13 +1710,26 @@ bgt_hip_selection DeviceLattice::round(uint32_t experiment, uint32_t response, c
 		prepare_update(experiment, response, dil);
 		const Descs ds = single(); // the new experiment's factors, the pre-update 1/D
 		clock_.begin(stream_, true);
-		launch_bbpa_hist(ds, 1, n_, k_, storage_, true, stream_);
+		launch_bbpa_hist(ds, 1, n_, k_, storage_, true, stream_, false);
 		clock_.end(stream_, "round_hist");
 		commit_update();
+		// the tables of the pre-update descriptors (their inv_out slot receives 1/D), the band
+		// and the exact program of the updated lattice
+		prepare_gather();
+		Descs df = ds;
+		{
+			const LatDesc post = desc();
+			df.one.band_width = band_width(kSelBbpa);
+			df.one.gh_len = post.gh_len;
+			std::memcpy(df.one.gh_e, post.gh_e, sizeof(post.gh_e));
+			std::memcpy(df.one.gh_r, post.gh_r, sizeof(post.gh_r));
+			df.one.gh_pairs = post.gh_pairs;
+			df.one.gh_dil = post.gh_dil;
+			df.one.gexact = post.gexact;
+		}
 		clock_.begin(stream_);
-		launch_bbpa_score(ds, 1, n_, k_, 1.0 / double(1 << k_), stream_);
-		clock_.end(stream_, "bbpa_score");
-		collect_gather_band(1.0 / double(1 << k_));
+		launch_bbpa_finish(df, 1, k_, true, 1.0 / double(1 << k_), stream_);
+		clock_.end(stream_, "bbpa_finish");
 		fetch_head();
 		classify_from_head(up, lo, masses_out);
 		return finish_selection(kSelBbpa, static_cast<HostBox *>(host_box_)->result, true);
