@@ -1584,8 +1584,15 @@ __global__ __launch_bounds__(1024) void k_zeta_small(Descs descs, int atoms, int
 // the device table in fixed point, 96 fractional bits as three 32-bit digits in 64-bit
 // integer limbs: integer additions commute, so the tables do not depend on the order in
 // which workgroups finish (the same bits alone or batched, run after run).
-constexpr int kHistThreads = 256;
-constexpr int kHistWaves = kHistThreads / 64;
+// Workgroup size: the largest whose waves fit 4 per SIMD beside their tables (one workgroup
+// of 16 waves per CU where each lane holds at most 32 table entries, 4 waves otherwise):
+// more waves per workgroup hide the row loads' latency and share one merge into the device
+// table (the merge's atomics are per workgroup, not per wave). Four variants over two row
+// bits (30 entries) would spill at the 128-register limit of 16 waves, so they keep 4.
+constexpr int hist_threads(int k, int nh)
+{
+	return ((((1 << k) - 1) << nh) <= 32 && !(k == 4 && nh == 1)) ? 1024 : 256;
+}
 
 __device__ __forceinline__ void fix_add(unsigned long long *bin3, double x)
 {
@@ -1619,10 +1626,11 @@ __device__ __forceinline__ double fix_value(const unsigned long long *bin3)
 // x = y * inv. K variants, NH = n - 6 (2^NH registers per lane). The device table d.fixtab
 // holds (2^K - 1) tables of 2^n bins (table B at (B - 1) 2^n) and the total, three limbs each.
 template <typename TP, bool FUSED, int K, int NH>
-__global__ __launch_bounds__(kHistThreads) void k_bbpa_hist(Descs descs)
+__global__ __launch_bounds__(hist_threads(K, NH)) void k_bbpa_hist(Descs descs)
 {
 	constexpr int R = 1 << NH;
 	constexpr int T = (1 << K) - 1;
+	constexpr int kHistWaves = hist_threads(K, NH) / 64;
 	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
 	extern __shared__ double s_acc[]; // T * 2^n + 1 doubles: the workgroup's tables
 	__shared__ double s_lik[kLikCapacity];
@@ -3644,22 +3652,25 @@ void launch_bbpa_hist(const Descs &descs, int count, int n, int k, Storage st, b
 	const unsigned cells = ((1u << k) - 1u) * (1u << n);
 	const size_t lds = sizeof(double) * (cells + 1u);
 	const uint64_t rows = 1ull << (n * k - n);
-	const unsigned groups =
-		(unsigned)std::max<uint64_t>(1, std::min<uint64_t>((rows + kHistWaves - 1) / kHistWaves, 2ull * compute_units()));
+	const int threads = hist_threads(k, n - 6);
+	const unsigned waves = (unsigned)threads / 64u;
+	// one 1024-thread workgroup per CU, or two of 256 threads
+	const uint64_t cap = (threads == 1024 ? 1ull : 2ull) * compute_units();
+	const unsigned groups = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((rows + waves - 1) / waves, cap));
 	const dim3 grid(groups, count);
 	launch_checked("k_bbpa_hist", [&] {
 #define BGT_HIST(KV, NHV)                                                                                       \
 	if (k == KV && n - 6 == NHV) {                                                                              \
 		if (st == Storage::fp64) {                                                                              \
 			if (fused)                                                                                          \
-				hipLaunchKernelGGL((k_bbpa_hist<double, true, KV, NHV>), grid, dim3(kHistThreads), lds, stream, descs); \
+				hipLaunchKernelGGL((k_bbpa_hist<double, true, KV, NHV>), grid, dim3(threads), lds, stream, descs); \
 			else                                                                                                \
-				hipLaunchKernelGGL((k_bbpa_hist<double, false, KV, NHV>), grid, dim3(kHistThreads), lds, stream, descs); \
+				hipLaunchKernelGGL((k_bbpa_hist<double, false, KV, NHV>), grid, dim3(threads), lds, stream, descs); \
 		} else {                                                                                                \
 			if (fused)                                                                                          \
-				hipLaunchKernelGGL((k_bbpa_hist<float, true, KV, NHV>), grid, dim3(kHistThreads), lds, stream, descs); \
+				hipLaunchKernelGGL((k_bbpa_hist<float, true, KV, NHV>), grid, dim3(threads), lds, stream, descs); \
 			else                                                                                                \
-				hipLaunchKernelGGL((k_bbpa_hist<float, false, KV, NHV>), grid, dim3(kHistThreads), lds, stream, descs); \
+				hipLaunchKernelGGL((k_bbpa_hist<float, false, KV, NHV>), grid, dim3(threads), lds, stream, descs); \
 		}                                                                                                       \
 		return;                                                                                                 \
 	}
