@@ -134,6 +134,7 @@ def _sig(L):
         "bgt_hip_batch_round": ([ctypes.POINTER(_H), _u32p, _u32p, ctypes.POINTER(_dptr), c_int, c_int, c_double,
                                  c_double, ctypes.POINTER(SelectionC), _u32p, _u32p], c_int),
         "bgt_hip_batch_graphs": ([c_int, ctypes.POINTER(c_int)], c_int),
+        "bgt_hip_tree_shrink": ([c_int, ctypes.POINTER(c_int)], c_int),
         "bgt_hip_batch_graph_counts": ([ctypes.POINTER(c_u64), ctypes.POINTER(c_u64)], c_int),
         "bgt_hip_exact_upset_mass": ([c_int, _dptr, c_int, _u32p, _u32p, c_int, _u32p, _dptr], c_int),
         "bgt_hip_exact_upset_mass_dilution": ([c_int, _dptr, _dptr, c_int, _u32p, _u32p, c_int, _u32p, _dptr], c_int),

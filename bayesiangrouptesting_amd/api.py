@@ -534,6 +534,15 @@ def set_batch_graphs(enable: bool) -> bool:
     return bool(prev.value)
 
 
+def set_tree_shrink(enable: bool) -> bool:
+    """Shrink the children of run_simulation's tree, as the reference's LocalTree does, or
+    keep them at full size with a candidate restriction (the default, equal in exact
+    arithmetic; bgt_hip_tree_shrink); returns the former setting."""
+    prev = ctypes.c_int(0)
+    nat.check(nat.lib().bgt_hip_tree_shrink(1 if enable else 0, ctypes.byref(prev)))
+    return bool(prev.value)
+
+
 def batch_graph_counts() -> tuple[int, int]:
     """(steps captured into a new graph, steps that replayed one) since the process started."""
     c, r = ctypes.c_uint64(0), ctypes.c_uint64(0)

@@ -552,6 +552,15 @@ int bgt_hip_batch_graph_counts(unsigned long long *captures, unsigned long long 
 	return guarded([&] { bgt_hip::batch_graph_counts(captures, replays); });
 }
 
+int bgt_hip_tree_shrink(int enable, int *previous)
+{
+	return guarded([&] {
+		const bool was = bgt_hip::set_tree_shrink(enable != 0);
+		if (previous)
+			*previous = was ? 1 : 0;
+	});
+}
+
 namespace {
 
 bgt_hip::TuneResult run_tune(int use_dilution, int subjects, int variants, int precision, int selector, int force,

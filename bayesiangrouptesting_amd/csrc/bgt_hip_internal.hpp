@@ -165,8 +165,13 @@ struct LatDesc {
 	uint32_t gh_e[kGatherMaxHist];
 	uint8_t gh_r[kGatherMaxHist];
 	int gh_len;
-	const double *gh_pairs; // the prior's (pi_a, 1 - pi_a), indexed 2a + bit
-	const double *gh_dil;
+	// the program's frame: gx_n subjects, of which those in gx_keep, in order, are this
+	// lattice's (a projected lattice keeps the program of the lattice it came from; pools,
+	// pairs and the table are in the frame, band entries are placed in it)
+	int gx_n;
+	uint32_t gx_keep;
+	const double *gh_pairs; // the prior's (pi_a, 1 - pi_a), indexed 2a + bit (frame atoms)
+	const double *gh_dil;   // rows of gx_n + 1 entries
 	double *gexact;
 };
 
@@ -367,8 +372,11 @@ void launch_upset_mass(const Descs &descs, int count, int atoms, uint32_t experi
 					   Storage st, hipStream_t stream);
 // destination entry b = sum over the subsets `red` of `removed` (increasing order) of
 // source[base(b) | red] * (1/D), base(b) = b's bits placed at the atoms outside `removed`
+// project_posterior: removals of at most 6 atoms sum each kept entry's 2^r source entries in
+// the reference's serial order (bit-identical); larger ones in chunks (k_project_wide), whose
+// sums go through `partial` (at least partial_cap doubles, up to 2048 used)
 void launch_project(const void *src, const double *src_scal, int inv_slot, void *dst, int dst_atoms, uint32_t removed,
-					Storage st, hipStream_t stream);
+					Storage st, hipStream_t stream, double *partial, size_t partial_cap);
 void launch_read_posterior(const Descs &descs, int atoms, Storage st, double *out,
 						   hipStream_t stream);
 void launch_lookahead(const Descs &descs, int n, Storage st, const uint32_t *selected,

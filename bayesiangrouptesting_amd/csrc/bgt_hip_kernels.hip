@@ -1870,12 +1870,53 @@ __device__ __forceinline__ bool gather_redecides(const LatDesc &d)
 	return (d2 - d1) <= d.band_width && d.counter[0] <= d.band_cap;
 }
 
-// U_v(E) of block v: the sum over the states S >= E of the n-subject block of the prior
-// factors of its atoms times the likelihood factors of the history's updates for variant v,
-// every thread a strided share of the states in increasing order, then the workgroup tree
+// A set of a projected lattice's subjects placed in the frame of its exact program: bit i
+// of e goes to the i-th set bit of keep (keep = every subject: e itself).
+__device__ __forceinline__ uint32_t frame_deposit(uint32_t e, uint32_t keep)
+{
+	uint32_t out = 0u;
+	for (uint32_t r = keep; e != 0u && r != 0u; r &= r - 1u, e >>= 1)
+		out |= (e & 1u) ? (r & (0u - r)) : 0u;
+	return out;
+}
+
+// The Op-BBPA objective from the per-variant binary masses M_v(E): bbpa.cpp:416-460 over
+// F_B = prod_{v in B} M_v (F_{} = 1), in variant order
+__device__ double gather_objective(const double *mv, int k, double target)
+{
+	const uint32_t all = (1u << k) - 1u;
+	double score = 0.0;
+	for (uint32_t resp = 0; resp <= all; ++resp) {
+		const uint32_t positive = (~resp) & all;
+		double exact = 0.0;
+		for (uint32_t extra = resp;; extra = (extra - 1u) & resp) {
+			const uint32_t fs = positive | extra;
+			double fv = 1.0;
+			bool first = true;
+			for (int v = 0; v < k; ++v)
+				if ((fs >> v) & 1u) {
+					fv = first ? mv[v] : fv * mv[v];
+					first = false;
+				}
+			if ((__popc(extra) & 1) == 0)
+				exact += fv;
+			else
+				exact -= fv;
+			if (extra == 0)
+				break;
+		}
+		score += fabs(exact - target);
+	}
+	return score;
+}
+
+// U_v(E) of block v (E in the program's frame of d.gx_n subjects): the sum over the states
+// S >= E of the block of the prior factors of its atoms times the likelihood factors of the
+// history's updates for variant v, every thread a strided share of the states in increasing
+// order, then the workgroup tree
 __device__ double gather_block_mass(const LatDesc &d, const double *s_pairs, int v, uint32_t e, double *red)
 {
-	const int n = d.n;
+	const int n = d.gx_n;
 	const uint32_t full = (1u << n) - 1u;
 	const uint32_t freeb = full & ~e;
 	const int f = __popc(freeb);
@@ -1929,41 +1970,18 @@ __global__ __launch_bounds__(256) void k_gather_exact(Descs descs, int k, double
 		return;
 	__shared__ double s_pairs[2 * kMaxAtoms];
 	__shared__ double red[16];
-	for (int i = threadIdx.x; i < 2 * d.atoms; i += blockDim.x)
+	for (int i = threadIdx.x; i < 2 * k * d.gx_n; i += blockDim.x)
 		s_pairs[i] = d.gh_pairs[i];
 	__syncthreads();
 	double u0[kMaxVariants];
 	for (int v = 0; v < k; ++v)
 		u0[v] = gather_block_mass(d, s_pairs, v, 0u, red);
 	for (unsigned i = blockIdx.x; i < count; i += gridDim.x) {
-		const uint32_t e = d.gband[i].e;
+		const uint32_t e = frame_deposit(d.gband[i].e, d.gx_keep);
 		double mv[kMaxVariants];
 		for (int v = 0; v < k; ++v)
 			mv[v] = gather_block_mass(d, s_pairs, v, e, red) / u0[v];
-		// bbpa.cpp:416-460 over F_B = prod_{v in B} M_v (F_{} = 1), in variant order
-		const uint32_t all = (1u << k) - 1u;
-		double score = 0.0;
-		for (uint32_t resp = 0; resp <= all; ++resp) {
-			const uint32_t positive = (~resp) & all;
-			double exact = 0.0;
-			for (uint32_t extra = resp;; extra = (extra - 1u) & resp) {
-				const uint32_t fs = positive | extra;
-				double fv = 1.0;
-				bool first = true;
-				for (int v = 0; v < k; ++v)
-					if ((fs >> v) & 1u) {
-						fv = first ? mv[v] : fv * mv[v];
-						first = false;
-					}
-				if ((__popc(extra) & 1) == 0)
-					exact += fv;
-				else
-					exact -= fv;
-				if (extra == 0)
-					break;
-			}
-			score += fabs(exact - target);
-		}
+		const double score = gather_objective(mv, k, target);
 		if (threadIdx.x == 0)
 			d.gexact[i] = score;
 	}
@@ -2079,87 +2097,84 @@ __global__ __launch_bounds__(1024) void k_bbpa_finish(Descs descs, int fused, in
 		d.counter[0] = count;
 	if (!s_redecide)
 		return;
-	// 4. the exact re-decision: the block weights, their normalisers, then one band entry per
-	// thread
-	for (int i = threadIdx.x; i < 2 * d.atoms; i += blockDim.x)
+	// 4. the exact re-decision in the program's frame (xn subjects; a projected lattice's band
+	// entries are placed in it): the block weights, their normalisers, then one band entry per
+	// thread; a frame whose blocks exceed the weights' room sums each entry's block masses
+	// with the whole workgroup instead (gather_block_mass, as k_gather_exact)
+	const int xn = d.gx_n;
+	for (int i = threadIdx.x; i < 2 * k * xn; i += blockDim.x)
 		s_pairs[i] = d.gh_pairs[i];
 	__syncthreads();
-	for (uint32_t idx = threadIdx.x; idx < (uint32_t)k * ec; idx += blockDim.x) {
-		const int v = (int)(idx >> n);
-		const uint32_t st = idx & (ec - 1u);
-		double p = 1.0;
-		for (int i = 0; i < n; ++i)
-			p *= s_pairs[2 * (v * n + i) + ((st >> i) & 1u)];
-		for (int q = 0; q < d.gh_len; ++q) {
-			const uint32_t pool = d.gh_e[q];
-			const int len = __popc(pool);
-			const int m = __popc(pool & ~st); // positive pooled subjects of block v
-			const bool rb = (d.gh_r[q] >> v) & 1u;
-			double fv;
-			if (d.gh_dil != nullptr) {
-				const double p1 = len > 0 ? d.gh_dil[(size_t)(len - 1) * (n + 1) + m] : 0.0;
-				fv = rb ? p1 : 1.0 - p1;
-			} else if (m == 0) {
-				fv = rb ? kNegativeResponse : (1.0 - kNegativeResponse);
-			} else {
-				fv = rb ? (1.0 - kNegativeResponse) : kNegativeResponse;
-			}
-			p *= fv;
-		}
-		s_w[idx] = p;
-	}
-	__syncthreads();
-	for (int v = 0; v < k; ++v) {
-		double acc = 0.0;
-		for (uint32_t st = threadIdx.x; st < ec; st += blockDim.x)
-			acc += s_w[((uint32_t)v << n) + st];
-		const double u = block_sum(acc, red);
-		if (threadIdx.x == 0)
-			s_u0[v] = u;
-		__syncthreads();
-	}
 	Top2 ex;
 	top2_init(ex);
-	if (threadIdx.x < count) {
-		const uint32_t ce = d.gband[threadIdx.x].e;
-		const uint32_t freeb = (ec - 1u) & ~ce;
-		double mv[kMaxVariants];
-		for (int v = 0; v < k; ++v) {
-			// the supersets of ce in increasing order: st = ce | sub, sub over the subsets of
-			// the free subjects by the (sub - freeb) & freeb step
-			const double *w = s_w + ((uint32_t)v << n);
-			double acc = 0.0;
-			uint32_t sub = 0u;
-			do {
-				acc += w[ce | sub];
-				sub = (sub - freeb) & freeb;
-			} while (sub != 0u);
-			mv[v] = acc / s_u0[v];
-		}
-		const uint32_t all = (1u << k) - 1u;
-		double sc = 0.0;
-		for (uint32_t resp = 0; resp <= all; ++resp) {
-			const uint32_t positive = (~resp) & all;
-			double exact = 0.0;
-			for (uint32_t extra = resp;; extra = (extra - 1u) & resp) {
-				const uint32_t fs = positive | extra;
-				double fv = 1.0;
-				bool first = true;
-				for (int v = 0; v < k; ++v)
-					if ((fs >> v) & 1u) {
-						fv = first ? mv[v] : fv * mv[v];
-						first = false;
-					}
-				if ((__popc(extra) & 1) == 0)
-					exact += fv;
-				else
-					exact -= fv;
-				if (extra == 0)
-					break;
+	if ((k << xn) <= kFinishWeights) {
+		const uint32_t xc = 1u << xn;
+		for (uint32_t idx = threadIdx.x; idx < (uint32_t)k * xc; idx += blockDim.x) {
+			const int v = (int)(idx >> xn);
+			const uint32_t st = idx & (xc - 1u);
+			double p = 1.0;
+			for (int i = 0; i < xn; ++i)
+				p *= s_pairs[2 * (v * xn + i) + ((st >> i) & 1u)];
+			for (int q = 0; q < d.gh_len; ++q) {
+				const uint32_t pool = d.gh_e[q];
+				const int len = __popc(pool);
+				const int m = __popc(pool & ~st); // positive pooled subjects of block v
+				const bool rb = (d.gh_r[q] >> v) & 1u;
+				double fv;
+				if (d.gh_dil != nullptr) {
+					const double p1 = len > 0 ? d.gh_dil[(size_t)(len - 1) * (xn + 1) + m] : 0.0;
+					fv = rb ? p1 : 1.0 - p1;
+				} else if (m == 0) {
+					fv = rb ? kNegativeResponse : (1.0 - kNegativeResponse);
+				} else {
+					fv = rb ? (1.0 - kNegativeResponse) : kNegativeResponse;
+				}
+				p *= fv;
 			}
-			sc += fabs(exact - target);
+			s_w[idx] = p;
 		}
-		top2_insert(ex, sc, ce);
+		__syncthreads();
+		for (int v = 0; v < k; ++v) {
+			double acc = 0.0;
+			for (uint32_t st = threadIdx.x; st < xc; st += blockDim.x)
+				acc += s_w[((uint32_t)v << xn) + st];
+			const double u = block_sum(acc, red);
+			if (threadIdx.x == 0)
+				s_u0[v] = u;
+			__syncthreads();
+		}
+		if (threadIdx.x < count) {
+			const uint32_t ce = d.gband[threadIdx.x].e;
+			const uint32_t xe = frame_deposit(ce, d.gx_keep);
+			const uint32_t freeb = (xc - 1u) & ~xe;
+			double mv[kMaxVariants];
+			for (int v = 0; v < k; ++v) {
+				// the supersets of xe in increasing order: st = xe | sub, sub over the subsets
+				// of the free subjects by the (sub - freeb) & freeb step
+				const double *w = s_w + ((uint32_t)v << xn);
+				double acc = 0.0;
+				uint32_t sub = 0u;
+				do {
+					acc += w[xe | sub];
+					sub = (sub - freeb) & freeb;
+				} while (sub != 0u);
+				mv[v] = acc / s_u0[v];
+			}
+			top2_insert(ex, gather_objective(mv, k, target), ce);
+		}
+	} else {
+		double u0[kMaxVariants];
+		for (int v = 0; v < k; ++v)
+			u0[v] = gather_block_mass(d, s_pairs, v, 0u, red);
+		for (unsigned i = 0; i < count; ++i) {
+			const uint32_t ce = d.gband[i].e;
+			const uint32_t xe = frame_deposit(ce, d.gx_keep);
+			double mv[kMaxVariants];
+			for (int v = 0; v < k; ++v)
+				mv[v] = gather_block_mass(d, s_pairs, v, xe, red) / u0[v];
+			if (threadIdx.x == 0)
+				top2_insert(ex, gather_objective(mv, k, target), ce);
+		}
 	}
 	ex = top2_block(ex, red2);
 	if (threadIdx.x == 0) {
@@ -2335,6 +2350,62 @@ __global__ __launch_bounds__(256) void k_project(const TS *src, const double *sr
 			sum += static_cast<double>(src[base | red]) * inv;
 			red = (red - removed) & removed;
 		} while (red != 0u);
+		dst[b] = static_cast<TD>(sum);
+	}
+}
+
+// Projections that sum many source entries into each kept entry (2^r terms, r > kProjectSerialBits:
+// typically most subjects classified at once, leaving few kept entries whose serial sums
+// above would each run 2^r dependent additions in a handful of threads): a workgroup sums a
+// contiguous chunk of one kept entry's terms (the subsets of the removed atoms by index),
+// each thread its strided share in index order, then the workgroup tree; k_project_join adds
+// an entry's chunk sums in chunk order. Every source entry is read once, in a fixed order of
+// additions, so the result does not depend on timing.
+constexpr int kProjectSerialBits = 6;
+template <typename TS, typename TD>
+__global__ __launch_bounds__(256) void k_project_wide(const TS *src, const double *src_scal, int inv_slot, TD *dst,
+													  double *partial, uint64_t dst_states, uint32_t removed, int r,
+													  uint32_t chunks)
+{
+	__shared__ double red[4];
+	const double inv = src_scal[inv_slot];
+	const uint64_t per = (1ull << r) / chunks;
+	for (uint64_t w = blockIdx.x; w < dst_states * chunks; w += gridDim.x) {
+		const uint64_t b = w / chunks;
+		const uint64_t g = w % chunks;
+		uint32_t base = (uint32_t)b;
+		for (uint32_t m = removed; m; m &= m - 1u) {
+			const uint32_t low = (m & (0u - m)) - 1u;
+			base = ((base & ~low) << 1) | (base & low);
+		}
+		double sum = 0.0;
+		for (uint64_t j = g * per + threadIdx.x; j < (g + 1) * per; j += blockDim.x) {
+			// j's bits deposited at the removed atoms, lowest first
+			uint32_t sub = 0u;
+			uint64_t q = j;
+			for (uint32_t m = removed; q != 0u; m &= m - 1u, q >>= 1)
+				sub |= (q & 1u) ? (m & (0u - m)) : 0u;
+			sum += static_cast<double>(src[base | sub]) * inv;
+		}
+		const double s = block_sum(sum, red);
+		if (threadIdx.x == 0) {
+			if (chunks == 1)
+				dst[b] = static_cast<TD>(s);
+			else
+				partial[w] = s;
+		}
+	}
+}
+
+template <typename TD>
+__global__ __launch_bounds__(256) void k_project_join(const double *partial, TD *dst, uint64_t dst_states,
+													  uint32_t chunks)
+{
+	for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < dst_states;
+		 b += (uint64_t)gridDim.x * blockDim.x) {
+		double sum = 0.0;
+		for (uint32_t g = 0; g < chunks; ++g)
+			sum += partial[b * chunks + g];
 		dst[b] = static_cast<TD>(sum);
 	}
 }
@@ -3737,9 +3808,39 @@ void launch_upset_mass(const Descs &descs, int count, int atoms, uint32_t e, Sto
 }
 
 void launch_project(const void *src, const double *src_scal, int inv_slot, void *dst, int dst_atoms, uint32_t removed,
-					Storage st, hipStream_t stream)
+					Storage st, hipStream_t stream, double *partial, size_t partial_cap)
 {
 	const uint64_t states = 1ull << dst_atoms;
+	const int r = __builtin_popcount(removed);
+	if (r > kProjectSerialBits) {
+		// about 2048 workgroups: kept entries times chunks of at least 256 terms each, the
+		// chunk sums within the partial buffer
+		uint64_t chunks = 1;
+		while (states * chunks * 2 <= 2048 && (1ull << r) / (chunks * 2) >= 256 && states * chunks * 2 <= partial_cap)
+			chunks *= 2;
+		if (chunks > 1 && partial == nullptr)
+			throw Error(4, "wide projection without a partial buffer");
+		const dim3 grid((unsigned)std::min<uint64_t>(states * chunks, 8192));
+		const dim3 join(grid_for(states, 256, 1024));
+		launch_checked("k_project_wide", [&] {
+			if (st == Storage::fp64) {
+				hipLaunchKernelGGL((k_project_wide<double, double>), grid, dim3(256), 0, stream,
+								   static_cast<const double *>(src), src_scal, inv_slot, static_cast<double *>(dst),
+								   partial, states, removed, r, (uint32_t)chunks);
+				if (chunks > 1)
+					hipLaunchKernelGGL(k_project_join<double>, join, dim3(256), 0, stream, partial,
+									   static_cast<double *>(dst), states, (uint32_t)chunks);
+			} else {
+				hipLaunchKernelGGL((k_project_wide<float, float>), grid, dim3(256), 0, stream,
+								   static_cast<const float *>(src), src_scal, inv_slot, static_cast<float *>(dst),
+								   partial, states, removed, r, (uint32_t)chunks);
+				if (chunks > 1)
+					hipLaunchKernelGGL(k_project_join<float>, join, dim3(256), 0, stream, partial,
+									   static_cast<float *>(dst), states, (uint32_t)chunks);
+			}
+		});
+		return;
+	}
 	const dim3 grid(grid_for(states, 256, 8192));
 	launch_checked("k_project", [&] {
 		if (st == Storage::fp64)

@@ -214,6 +214,9 @@ void DeviceLattice::init_prior(const double *prior)
 	clock_.reset();
 	clock_.enabled = false;
 	exact_.reset(true, atoms_, prior, dilution_, k_);
+	proj_rel_ = 0.0;
+	xn_ = n_;
+	xkeep_ = (n_ >= 32) ? 0xFFFFFFFFu : ((1u << n_) - 1u);
 	gh_e_.clear();
 	gh_r_.clear();
 	gh_dil_.clear();
@@ -262,6 +265,7 @@ public:
 			return nullptr;
 		auto l = std::move(it->second.back());
 		it->second.pop_back();
+		--count_;
 		bytes_ -= l->device_bytes();
 		return l;
 	}
@@ -276,8 +280,9 @@ public:
 					budget_ = std::strtoull(env, nullptr, 10);
 				budget_set_ = true;
 			}
-			if (bytes_ + l->device_bytes() <= budget_) {
+			if (bytes_ + l->device_bytes() <= budget_ && count_ < kMaxCached) {
 				bytes_ += l->device_bytes();
+				++count_;
 				free_[Key{l->dilution(), l->subjects(), l->variants(), (int)l->storage()}].push_back(std::move(l));
 				return;
 			}
@@ -287,8 +292,12 @@ public:
 
 private:
 	using Key = std::tuple<bool, int, int, int>;
+	// small lattices cost little device memory but each holds pinned host buffers (about
+	// 64 KiB): the count is bounded too
+	static constexpr size_t kMaxCached = 4096;
 	std::mutex m_;
 	std::map<Key, std::vector<std::unique_ptr<DeviceLattice>>> free_;
+	size_t count_ = 0;
 	uint64_t bytes_ = 0, budget_ = 0;
 	bool budget_set_ = false;
 };
@@ -349,7 +358,8 @@ std::unique_ptr<DeviceLattice> DeviceLattice::project(const DeviceLattice &src, 
 	const uint32_t keep = all & ~removed;
 	// the source's pending work (its last update) comes first
 	BGT_HIP_CHECK(hipStreamSynchronize(src.stream_));
-	launch_project(src.post_, src.scal_, src.inv_cur_, l->post_, l->atoms_, removed, src.storage_, l->stream_);
+	launch_project(src.post_, src.scal_, src.inv_cur_, l->post_, l->atoms_, removed, src.storage_, l->stream_, l->part_,
+				   2048);
 	HostBox *box = static_cast<HostBox *>(l->host_box_);
 	for (int s = 0; s < kScalars; ++s)
 		box->scal[s] = s < 4 ? 1.0 : 0.0; // pending normalisation 1: the sums are the values
@@ -372,8 +382,19 @@ std::unique_ptr<DeviceLattice> DeviceLattice::project(const DeviceLattice &src, 
 	l->hist_uploaded_ = 0;
 	l->exact_pending_ = l->exact_valid_ = false;
 	l->pairs_own_ = false;
-	const std::vector<double> none((size_t)l->atoms_, 0.0);
-	l->exact_.reset(false, l->atoms_, none.data());
+	// the exact program stays the source's, in its frame: the kept subjects of this
+	// projection, placed among the source's own kept subjects
+	uint32_t kept_subjects = 0;
+	for (int i = 0; i < src.n_; ++i)
+		if (!((removed >> i) & 1u))
+			kept_subjects |= 1u << i;
+	l->exact_ = src.exact_;
+	l->xn_ = src.xn_;
+	l->xkeep_ = src.to_frame(kept_subjects);
+	// each kept entry sums 2^r source entries in increasing order, each times 1/D: at most
+	// 2^r roundings relative (the FP32C form's one rounding to FP32 counts as the extra update)
+	l->proj_rel_ = src.proj_rel_ + std::ldexp(1.0, __builtin_popcount(removed));
+	l->copy_gather_history(src);
 	l->clock_.reset();
 	l->clock_.enabled = false;
 	l->busy_ = true;
@@ -392,6 +413,9 @@ DeviceLattice::DeviceLattice(const DeviceLattice &parent, bool copy_posterior)
 	updates_ = parent.updates_;
 	inv_cur_ = parent.inv_cur_;
 	exact_ = parent.exact_;
+	xn_ = parent.xn_;
+	xkeep_ = parent.xkeep_;
+	proj_rel_ = parent.proj_rel_;
 	copy_gather_history(parent);
 	if (copy_posterior) {
 		// order the copy after the parent's pending work
@@ -456,7 +480,8 @@ ArenaLayout arena_layout(bool dilution, int n, int k)
 	// several variants: exact objectives of the band, and the exact program's inputs (the
 	// prior pairs and the dilution table)
 	a.gexact = take(k >= 2 ? sizeof(double) * ec : 0);
-	a.ghin = take(k >= 2 ? sizeof(double) * (2 * atoms + n * (n + 1)) : 0);
+	// (sized for the largest frame a projection can keep: kMaxAtoms atoms, kMaxAtoms / k subjects)
+	a.ghin = take(k >= 2 ? sizeof(double) * (2 * kMaxAtoms + (kMaxAtoms / k) * (kMaxAtoms / k + 1)) : 0);
 	a.total = std::max<size_t>(off, a.exact ? kHeadBytesWide : kHeadBytes);
 	return a;
 }
@@ -553,6 +578,9 @@ void DeviceLattice::adopt(const DeviceLattice &parent)
 	updates_ = parent.updates_;
 	inv_cur_ = parent.inv_cur_;
 	exact_ = parent.exact_;
+	xn_ = parent.xn_;
+	xkeep_ = parent.xkeep_;
+	proj_rel_ = parent.proj_rel_;
 	copy_gather_history(parent);
 	hist_uploaded_ = 0;
 	exact_pending_ = exact_valid_ = false;
@@ -595,6 +623,8 @@ LatDesc DeviceLattice::desc() const
 	d.band = k_ == 1 ? band_ : nullptr;
 	d.gband = k_ >= 2 ? band_ : nullptr;
 	d.gh_len = -1;
+	d.gx_n = xn_;
+	d.gx_keep = xkeep_;
 	if (k_ >= 2 && ghin_ != nullptr && gh_uploaded_ && !gh_broken_ && exact_.usable()) {
 		d.gh_len = (int)gh_e_.size();
 		for (size_t t = 0; t < gh_e_.size(); ++t) {
@@ -602,7 +632,7 @@ LatDesc DeviceLattice::desc() const
 			d.gh_r[t] = gh_r_[t];
 		}
 		d.gh_pairs = ghin_;
-		d.gh_dil = dilution_ ? ghin_ + 2 * atoms_ : nullptr;
+		d.gh_dil = dilution_ ? ghin_ + 2 * (size_t)xn_ * k_ : nullptr;
 		d.gexact = gexact_;
 	}
 	d.band_width = 0.0;
@@ -681,21 +711,33 @@ void DeviceLattice::commit_update()
 {
 	inv_cur_ = inv_cur_ == kScalInv0 ? kScalInv1 : kScalInv0;
 	++updates_;
-	exact_.push(e_, r_, pending_dil_);
+	// the exact program works in its frame (a projected lattice's pools placed among the
+	// subjects of the lattice it came from, its table at the frame's row stride)
+	const uint32_t xe = to_frame(e_);
+	std::vector<double> ftab;
+	const double *xdil = pending_dil_;
+	if (dilution_ && pending_dil_ != nullptr && xn_ != n_) {
+		frame_table(pending_dil_, ftab);
+		xdil = ftab.data();
+	}
+	exact_.push(xe, r_, xdil);
 	if (k_ >= 2 && !gh_broken_) {
-		// the history of the exact Op-BBPA re-decision on the device (one dilution table only)
+		// the history of the exact Op-BBPA re-decision on the device (one dilution table only:
+		// the rows this lattice can use must equal those recorded)
 		if ((int)gh_e_.size() >= kGatherMaxHist) {
 			gh_broken_ = true;
 		} else {
-			gh_e_.push_back(e_);
+			gh_e_.push_back(xe);
 			gh_r_.push_back((uint8_t)r_);
 			if (dilution_) {
-				const size_t cells = (size_t)n_ * (n_ + 1);
+				const size_t cells = (size_t)xn_ * (xn_ + 1);
 				if (gh_dil_.empty()) {
-					gh_dil_.assign(pending_dil_, pending_dil_ + cells);
+					gh_dil_.assign(xdil, xdil + cells);
 					gh_uploaded_ = false;
-				} else if (std::memcmp(gh_dil_.data(), pending_dil_, sizeof(double) * cells) != 0) {
-					gh_broken_ = true;
+				} else {
+					for (int row = 0; row < n_ && !gh_broken_; ++row)
+						gh_broken_ = std::memcmp(gh_dil_.data() + (size_t)row * (xn_ + 1), xdil + (size_t)row * (xn_ + 1),
+												 sizeof(double) * (n_ + 1)) != 0;
 				}
 			}
 		}
@@ -712,17 +754,47 @@ void DeviceLattice::copy_gather_history(const DeviceLattice &parent)
 	gh_uploaded_ = false;
 }
 
+uint32_t DeviceLattice::to_frame(uint32_t subjects) const
+{
+	if (xn_ == n_)
+		return subjects;
+	uint32_t out = 0;
+	for (uint32_t r = xkeep_; subjects != 0u && r != 0u; r &= r - 1u, subjects >>= 1)
+		if (subjects & 1u)
+			out |= r & (0u - r);
+	return out;
+}
+
+uint32_t DeviceLattice::atoms_to_frame(uint32_t atoms) const
+{
+	if (xn_ == n_)
+		return atoms;
+	const uint32_t smask = (1u << n_) - 1u;
+	uint32_t out = 0;
+	for (int v = 0; v < k_; ++v)
+		out |= to_frame((atoms >> (v * n_)) & smask) << (v * xn_);
+	return out;
+}
+
+void DeviceLattice::frame_table(const double *dil, std::vector<double> &out) const
+{
+	out.assign((size_t)xn_ * (xn_ + 1), 0.0);
+	for (int row = 0; row < n_; ++row)
+		std::memcpy(out.data() + (size_t)row * (xn_ + 1), dil + (size_t)row * (n_ + 1), sizeof(double) * (n_ + 1));
+}
+
 void DeviceLattice::prepare_gather()
 {
 	// the exact program's inputs into the arena (the prior pairs, the dilution table), once
 	// per lattice and table; synchronous, so that any stream's kernels may read them
 	if (k_ < 2 || ghin_ == nullptr || gh_broken_ || gh_uploaded_ || !exact_.usable())
 		return;
-	const size_t cells = 2 * (size_t)atoms_ + (size_t)n_ * (n_ + 1);
+	const size_t xatoms = (size_t)xn_ * k_;
+	const size_t cells = 2 * xatoms + (size_t)xn_ * (xn_ + 1);
 	double *h = static_cast<double *>(pinned_alloc(sizeof(double) * cells));
 	exact_.prior_pairs(h);
-	for (size_t i = 0; i < (size_t)n_ * (n_ + 1); ++i)
-		h[2 * atoms_ + i] = i < gh_dil_.size() ? gh_dil_[i] : 0.0;
+	for (size_t i = 0; i < (size_t)xn_ * (xn_ + 1); ++i)
+		h[2 * xatoms + i] = i < gh_dil_.size() ? gh_dil_[i] : 0.0;
 	const hipError_t e1 = hipMemcpyAsync(ghin_, h, sizeof(double) * cells, hipMemcpyHostToDevice, stream_);
 	const hipError_t e2 = e1 == hipSuccess ? hipStreamSynchronize(stream_) : e1;
 	pinned_release(h, sizeof(double) * cells);
@@ -795,7 +867,7 @@ void DeviceLattice::load_head(const unsigned char *head)
 void DeviceLattice::launch_exact(const Descs &ds_in)
 {
 	exact_pending_ = false;
-	if (exact_dev_ == nullptr || dilution_ || k_ != 1 || !pairs_own_ || !exact_.usable() ||
+	if (exact_dev_ == nullptr || dilution_ || k_ != 1 || !pairs_own_ || xn_ != n_ || !exact_.usable() ||
 		exact_.updates() > kExactMaxHist)
 		return;
 	HostBox *box = static_cast<HostBox *>(host_box_);
@@ -909,6 +981,13 @@ bool DeviceLattice::exact_masses(const uint32_t *es, size_t count, double *out, 
 {
 	if (!exact_.usable())
 		return false;
+	std::vector<uint32_t> xs;
+	if (xn_ != n_) { // a projected lattice: the sets in the program's frame
+		xs.resize(count);
+		for (size_t i = 0; i < count; ++i)
+			xs[i] = atoms_to_frame(es[i]);
+		es = xs.data();
+	}
 	if (!uses_enum())
 		return exact_.masses(es, count, out, ok);
 	if (count > 0)
@@ -924,7 +1003,7 @@ double DeviceLattice::exact_rel_bound() const
 	// per block M_v = U_v(E) / U_v({}) (two enumerations and a division); the product of
 	// the k blocks' masses adds k - 1 roundings
 	if (uses_enum())
-		return (double)k_ * (2.0 * enum_rel_bound(n_, exact_.enum_pools()) + 0x1p-53) + (double)(k_ - 1) * 0x1p-53;
+		return (double)k_ * (2.0 * enum_rel_bound(xn_, exact_.enum_pools()) + 0x1p-53) + (double)(k_ - 1) * 0x1p-53;
 	return exact_.rel_bound();
 }
 
@@ -933,7 +1012,8 @@ void DeviceLattice::enum_run(const std::vector<std::vector<uint32_t>> &queries, 
 	// one launch_exact_enum per variant block, all on the stream, one copy of the values back
 	// and one synchronisation; query 0 of each block is the empty set, its normaliser U({})
 	const int T = exact_.enum_pools();
-	const int stride = n_ + 1;
+	const int xn = xn_; // the program's frame
+	const int stride = xn + 1;
 	auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
 	struct Part {
 		size_t q, o_pairs, o_fac, o_pool, o_q, o_part, o_out, h_out;
@@ -946,7 +1026,7 @@ void DeviceLattice::enum_run(const std::vector<std::vector<uint32_t>> &queries, 
 		if (p.q > 65535)
 			throw Error(BGT_HIP_INTERNAL, "too many exact masses for one launch");
 		p.o_pairs = in_bytes;
-		p.o_fac = p.o_pairs + up(sizeof(double) * 2 * n_);
+		p.o_fac = p.o_pairs + up(sizeof(double) * 2 * xn);
 		p.o_pool = p.o_fac + up(sizeof(double) * (size_t)T * stride + 8);
 		p.o_q = p.o_pool + up(sizeof(uint32_t) * (size_t)T + 4);
 		in_bytes = p.o_q + up(sizeof(uint32_t) * p.q);
@@ -984,7 +1064,7 @@ void DeviceLattice::enum_run(const std::vector<std::vector<uint32_t>> &queries, 
 			job.queries = reinterpret_cast<const uint32_t *>(d + p.o_q);
 			job.partial = reinterpret_cast<double *>(d + p.o_part);
 			job.out = reinterpret_cast<double *>(d + p.o_out);
-			job.atoms = n_;
+			job.atoms = xn;
 			job.updates = T;
 			job.stride = stride;
 			job.count = (int)p.q;
@@ -1012,13 +1092,13 @@ void DeviceLattice::enum_run(const std::vector<std::vector<uint32_t>> &queries, 
 
 void DeviceLattice::enum_masses(const uint32_t *es, size_t count, double *out)
 {
-	// the mass of an atom set is the product of its variant blocks' masses (the posterior is
-	// a product over the blocks), in variant order
-	const uint32_t smask = (n_ >= 32) ? 0xFFFFFFFFu : ((1u << n_) - 1u);
+	// the mass of an atom set (in the program's frame) is the product of its variant blocks'
+	// masses (the posterior is a product over the blocks), in variant order
+	const uint32_t smask = (xn_ >= 32) ? 0xFFFFFFFFu : ((1u << xn_) - 1u);
 	std::vector<std::vector<uint32_t>> q(k_, std::vector<uint32_t>(count));
 	for (int v = 0; v < k_; ++v)
 		for (size_t i = 0; i < count; ++i)
-			q[v][i] = (es[i] >> (v * n_)) & smask;
+			q[v][i] = (es[i] >> (v * xn_)) & smask;
 	std::vector<std::vector<double>> m;
 	enum_run(q, m);
 	for (size_t i = 0; i < count; ++i) {
@@ -1721,6 +1801,8 @@ bgt_hip_selection DeviceLattice::round(uint32_t experiment, uint32_t response, c
 			const LatDesc post = desc();
 			df.one.band_width = band_width(kSelBbpa);
 			df.one.gh_len = post.gh_len;
+			df.one.gx_n = post.gx_n;
+			df.one.gx_keep = post.gx_keep;
 			std::memcpy(df.one.gh_e, post.gh_e, sizeof(post.gh_e));
 			std::memcpy(df.one.gh_r, post.gh_r, sizeof(post.gh_r));
 			df.one.gh_pairs = post.gh_pairs;
@@ -1794,7 +1876,7 @@ void DeviceLattice::error_bound(double *posterior_rel, double *mass_rel) const
 	// common-scale error of the summed denominator. Masses add the transform's roundings.
 	const double u64 = std::ldexp(1.0, -53);
 	if (storage_ == Storage::fp64) {
-		*posterior_rel = (2.0 * updates_ + 1.0 + atoms_) * u64;
+		*posterior_rel = (2.0 * updates_ + 1.0 + atoms_ + proj_rel_) * u64;
 		*mass_rel = *posterior_rel + 2.0 * atoms_ * u64;
 	} else {
 		const double u32 = std::ldexp(1.0, -24);

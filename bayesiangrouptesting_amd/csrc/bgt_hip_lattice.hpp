@@ -229,6 +229,20 @@ private:
 	bool gh_broken_ = false, gh_uploaded_ = false;
 	double *gexact_ = nullptr;
 	double *ghin_ = nullptr;
+	// The exact program's frame: a projected lattice (project) keeps the program of the
+	// lattice it came from, in that lattice's numbering of xn_ subjects, of which the kept
+	// ones, in order, are this lattice's subjects (xkeep_). The marginal of the original
+	// posterior over the kept atoms is the projected posterior, and it stays so under updates
+	// whose pools hold only kept subjects, so M(E) of this lattice is the program's M of E
+	// placed in the frame. Unprojected: xn_ = n_ and every subject kept.
+	int xn_ = 0;
+	uint32_t xkeep_ = 0;
+	double proj_rel_ = 0.0; // FP64: the projections' summation roundings, in units of 2^-53
+	uint32_t to_frame(uint32_t subjects) const; // a set of this lattice's subjects in the frame
+	uint32_t atoms_to_frame(uint32_t atoms) const; // per variant block
+	// the dilution table of the current update (this lattice's rows) at the frame's row
+	// stride, rows beyond n_ zero (pools never hold more than n_ subjects)
+	void frame_table(const double *dil, std::vector<double> &out) const;
 	unsigned head_band_ = 0;    // band candidates that came back with the last head
 	ExactRec *hist_dev_ = nullptr; // update history for the device exact program
 	double *exact_dev_ = nullptr;  // its exact masses of the first kExactBand band candidates

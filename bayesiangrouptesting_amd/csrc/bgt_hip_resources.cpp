@@ -16,8 +16,16 @@ struct Resources {
 	std::mutex m;
 	int async_alloc = -1; // -1 unknown, 0 unsupported, 1 stream-ordered allocation in use
 	std::vector<void *> pinned_free[24]; // by log2 of the block size
-	std::vector<hipStream_t> streams;
+	std::vector<hipStream_t> streams; // the shared lattice streams, created on demand
+	size_t next_stream = 0;
 };
+
+// Lattices share a bounded set of streams. Every stream holds runtime queue state and
+// completion signals, of which the kernel driver grants a process a fixed number (4096 signal
+// events), while a process can hold thousands of lattices at once (a tree level's children,
+// lattices kept for re-use). Work that lattices put on one stream runs in the order it was
+// enqueued, so sharing removes overlap between them, never ordering within one.
+constexpr size_t kLatticeStreams = 32;
 
 Resources &res()
 {
@@ -125,25 +133,27 @@ hipStream_t stream_acquire()
 	Resources &r = res();
 	{
 		std::lock_guard<std::mutex> lk(r.m);
-		if (!r.streams.empty()) {
-			hipStream_t s = r.streams.back();
-			r.streams.pop_back();
-			return s;
-		}
+		if (r.streams.size() >= kLatticeStreams)
+			return r.streams[r.next_stream++ % r.streams.size()];
 	}
 	hipStream_t s = nullptr;
-	std::lock_guard<std::mutex> gate(capture_gate());
-	BGT_HIP_CHECK(hipStreamCreate(&s));
+	{
+		std::lock_guard<std::mutex> gate(capture_gate());
+		BGT_HIP_CHECK(hipStreamCreate(&s));
+	}
+	std::lock_guard<std::mutex> lk(r.m);
+	if (r.streams.size() >= kLatticeStreams) {
+		// another thread filled the set meanwhile: use one of it, destroy the spare
+		(void)hipStreamDestroy(s);
+		return r.streams[r.next_stream++ % r.streams.size()];
+	}
+	r.streams.push_back(s);
 	return s;
 }
 
-void stream_release(hipStream_t s)
+void stream_release(hipStream_t)
 {
-	if (s == nullptr)
-		return;
-	Resources &r = res();
-	std::lock_guard<std::mutex> lk(r.m);
-	r.streams.push_back(s);
+	// the streams are shared and live until the process ends
 }
 
 } // namespace bgt_hip

@@ -31,8 +31,9 @@ void device_free(void *p, hipStream_t stream);
 void *pinned_alloc(size_t bytes);
 void pinned_release(void *p, size_t bytes);
 
-// A stream of default priority from a pool; stream_release returns it (the caller has
-// synchronised it).
+// A stream of default priority for a lattice, one of a bounded set that lattices share (a
+// lattice's work stays ordered; other lattices' work may queue on the same stream);
+// stream_release gives up the caller's use of it.
 hipStream_t stream_acquire();
 void stream_release(hipStream_t s);
 

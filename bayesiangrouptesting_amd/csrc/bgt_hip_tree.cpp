@@ -3,10 +3,15 @@
 // src/runtime/simulation_runner.cpp:92-132).
 //
 // The reference shrinks each child lattice by marginalising subjects whose atoms are all
-// classified (lattice_base.cpp:266-316). The device tree keeps every lattice at full size
-// and restricts experiments to the still-active subjects instead; the two are equal in
-// exact arithmetic (the reference's own no_shrink_tree_reference_test pins this at 1e-12),
-// and index order is preserved by Tree::true_ex, so ties resolve the same way.
+// classified (lattice_base.cpp:266-316). The device tree does the same when shrinking is on
+// (set_tree_shrink): a child whose classification covers every variant of some subject is
+// replaced by its projection (DeviceLattice::project), selects in its own numbering, and
+// the experiment is mapped back by Tree::true_ex (tree.cpp:55-68). By default it keeps
+// every lattice at full size and restricts experiments to the still-active subjects
+// instead: the two are equal in exact arithmetic (the reference's own
+// no_shrink_tree_reference_test pins this at 1e-12), index order is preserved by true_ex,
+// so ties resolve the same way, and at tree sizes whose levels are bound by launches (2^16
+// states, depth 6) the projections cost more than the smaller children save.
 #include "bgt_hip_tree.hpp"
 
 #include "bgt_hip_batch.hpp"
@@ -14,7 +19,9 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <atomic>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <vector>
 
@@ -32,6 +39,27 @@ struct TreeCtx {
 	int n, k, depth, selector;
 	double up, lo;
 	const double *dil;
+	bool shrink;
+};
+
+std::atomic<bool> g_tree_shrink{false};
+
+// A node's lattice numbering: the original subjects still in it, in order (the removed ones
+// are `removed`); atom v * cur + j of the lattice is original atom v * n + kept[j].
+struct Frame {
+	std::vector<int> kept;
+	uint32_t removed = 0;
+
+	uint32_t to_original(uint32_t x, int n, int k) const
+	{
+		const int cur = (int)kept.size();
+		uint32_t out = 0;
+		for (int v = 0; v < k; ++v)
+			for (int j = 0; j < cur; ++j)
+				if ((x >> (v * cur + j)) & 1u)
+					out |= 1u << (v * n + kept[j]);
+		return out;
+	}
 };
 
 uint32_t active_subjects(uint32_t classified, int n, int k)
@@ -63,23 +91,48 @@ std::unique_ptr<HostNode> build_levels(const TreeCtx &c, std::unique_ptr<DeviceL
 	struct Open {
 		HostNode *node;
 		std::unique_ptr<DeviceLattice> lat;
+		Frame frame;
 	};
 	const int atoms = c.n * c.k;
 	const int rc = 1 << c.k;
-	std::vector<std::unique_ptr<DeviceLattice>> pool;
+	// lattices to reuse, by subject count (projected children come in every size)
+	std::map<int, std::vector<std::unique_ptr<DeviceLattice>>> pool;
 	auto take = [&](const DeviceLattice &like) {
-		if (pool.empty())
+		auto &free = pool[like.subjects()];
+		if (free.empty())
 			return DeviceLattice::make_like(like);
-		auto l = std::move(pool.back());
-		pool.pop_back();
+		auto l = std::move(free.back());
+		free.pop_back();
 		return l;
+	};
+	auto give = [&](std::unique_ptr<DeviceLattice> l) {
+		const int n = l->subjects();
+		pool[n].push_back(std::move(l));
+	};
+	// the dilution table at each subject count: the original's leading rows (pool sizes
+	// at most the count), at the row stride of that count
+	std::map<int, std::vector<double>> tables;
+	auto table_for = [&](int cur) -> const double * {
+		if (c.dil == nullptr || cur == c.n)
+			return c.dil;
+		auto &t = tables[cur];
+		if (t.empty()) {
+			t.resize((size_t)cur * (cur + 1));
+			for (int row = 0; row < cur; ++row)
+				std::memcpy(t.data() + (size_t)row * (cur + 1), c.dil + (size_t)row * (c.n + 1),
+							sizeof(double) * (cur + 1));
+		}
+		return t.data();
 	};
 	auto root = std::make_unique<HostNode>();
 	root->pos = root_lat->positive_mask();
 	root->neg = root_lat->negative_mask();
+	Frame root_frame;
+	for (int i = 0; i < c.n; ++i)
+		root_frame.kept.push_back(i);
 	std::vector<Open> frontier;
 	if (!is_classified(root->pos, root->neg, atoms) && c.depth > 0)
-		frontier.push_back({root.get(), std::move(root_lat)});
+		frontier.push_back({root.get(), std::move(root_lat), std::move(root_frame)});
 	else
 		DeviceLattice::recycle(std::move(root_lat));
 	std::vector<BatchItem> items;
@@ -89,7 +142,9 @@ std::unique_ptr<HostNode> build_levels(const TreeCtx &c, std::unique_ptr<DeviceL
 		items.assign(frontier.size(), BatchItem{});
 		for (size_t i = 0; i < frontier.size(); ++i) {
 			const HostNode &nd = *frontier[i].node;
-			frontier[i].lat->set_active(active_subjects(nd.pos | nd.neg, c.n, c.k));
+			// a shrunk lattice holds only subjects with an unclassified atom
+			if (!c.shrink)
+				frontier[i].lat->set_active(active_subjects(nd.pos | nd.neg, c.n, c.k));
 			items[i].lat = frontier[i].lat.get();
 			items[i].classify = false;
 		}
@@ -105,9 +160,12 @@ std::unique_ptr<HostNode> build_levels(const TreeCtx &c, std::unique_ptr<DeviceL
 			bs.near_ties += res[i].selection.near_tie;
 			HostNode &nd = *frontier[i].node;
 			const DeviceLattice &parent = *frontier[i].lat;
+			const Frame &fr = frontier[i].frame;
+			// the experiment in the original numbering (true_ex, tree.cpp:55-68)
+			const uint32_t full = (e == 0xFFFFFFFFu || !c.shrink) ? e : fr.to_original(e, c.n, 1);
 			for (int r = 0; r < rc; ++r) {
 				auto child = std::make_unique<HostNode>();
-				child->ex = e;
+				child->ex = full;
 				child->res = (uint32_t)r;
 				child->stage = nd.stage + 1;
 				HostNode *cp = child.get();
@@ -120,28 +178,60 @@ std::unique_ptr<HostNode> build_levels(const TreeCtx &c, std::unique_ptr<DeviceL
 				it.experiment = e;
 				it.response = (uint32_t)r;
 				it.source = &parent;
-				it.dilution = c.dil;
+				it.dilution = table_for(parent.subjects());
 				it.select = false;
 				items.push_back(it);
-				kids.push_back({cp, std::move(lat)});
+				kids.push_back({cp, std::move(lat), fr});
 			}
 		}
 		batch_step(items, c.selector, c.up, c.lo, res);
+		std::vector<HostNode *> parents(frontier.size());
+		for (size_t i = 0; i < frontier.size(); ++i)
+			parents[i] = frontier[i].node;
 		for (auto &o : frontier)
-			pool.push_back(std::move(o.lat));
+			give(std::move(o.lat));
 		frontier.clear();
 		for (size_t i = 0; i < kids.size(); ++i) {
-			HostNode &nd = *kids[i].node;
-			nd.pos = res[i].positive;
-			nd.neg = res[i].negative;
-			if (is_classified(nd.pos, nd.neg, atoms) || nd.stage >= c.depth)
-				pool.push_back(std::move(kids[i].lat)); // a leaf
-			else
-				frontier.push_back(std::move(kids[i]));
+			Open &kid = kids[i];
+			HostNode &nd = *kid.node;
+			if (c.shrink) {
+				// classification in the original numbering: the parent's (removed subjects'
+				// atoms included) and this child's sticky masks over its own atoms
+				const HostNode &pn = *parents[i / rc];
+				nd.pos = pn.pos | kid.frame.to_original(res[i].positive, c.n, c.k);
+				nd.neg = pn.neg | kid.frame.to_original(res[i].negative, c.n, c.k);
+			} else {
+				nd.pos = res[i].positive;
+				nd.neg = res[i].negative;
+			}
+			if (is_classified(nd.pos, nd.neg, atoms) || nd.stage >= c.depth) {
+				give(std::move(kid.lat)); // a leaf
+				continue;
+			}
+			if (c.shrink) {
+				// update_metadata_with_shrinking's plan (lattice_base.cpp:266-305) and its
+				// projection (apply_shrink_plan, 307-316)
+				bgt_hip_shrink_plan plan;
+				make_shrink_plan(c.n, kid.lat->subjects(), c.k, nd.pos | nd.neg, kid.frame.removed, plan);
+				if (plan.removable_current_atoms != 0u && plan.new_current_subjects > 0) {
+					auto projected = DeviceLattice::project(*kid.lat, plan.removable_current_atoms);
+					give(std::move(kid.lat));
+					kid.lat = std::move(projected);
+					std::vector<int> kept;
+					for (size_t j = 0; j < kid.frame.kept.size(); ++j)
+						if (!((plan.removable_current_atoms >> j) & 1u))
+							kept.push_back(kid.frame.kept[j]);
+					kid.frame.kept = std::move(kept);
+					kid.frame.removed = plan.new_classified_subjects;
+					bs.projections += 1;
+				}
+			}
+			frontier.push_back(std::move(kid));
 		}
 	}
-	for (auto &l : pool)
-		DeviceLattice::recycle(std::move(l));
+	for (auto &entry : pool)
+		for (auto &l : entry.second)
+			DeviceLattice::recycle(std::move(l));
 	return root;
 }
 
@@ -175,6 +265,8 @@ void flatten(const HostNode &n, std::vector<FlatNode> &out)
 
 } // namespace
 
+bool set_tree_shrink(bool enable) { return g_tree_shrink.exchange(enable); }
+
 bgt_hip_tree_stats run_tree(bool dilution, int n, int k, const double *prior, int depth, double up, double lo,
 							double branch, int selector, const double *dil, Storage storage, TreeBuildStats *bs_out)
 {
@@ -189,7 +281,7 @@ bgt_hip_tree_stats run_tree(bool dilution, int n, int k, const double *prior, in
 		throw Error(BGT_HIP_BAD_INPUT, "Op-BHA tree selection is only supported for binary lattices");
 	if (dilution && dil == nullptr)
 		throw Error(BGT_HIP_BAD_INPUT, "dilution table is required for dilution models");
-	TreeCtx c{n, k, depth, DeviceLattice::resolve_selector(selector, k), up, lo, dil};
+	TreeCtx c{n, k, depth, DeviceLattice::resolve_selector(selector, k), up, lo, dil, g_tree_shrink.load()};
 	auto root_lat = DeviceLattice::make(dilution, n, k, prior, storage);
 	hipStream_t s = nullptr;
 	TreeBuildStats bs;

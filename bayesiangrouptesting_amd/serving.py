@@ -12,6 +12,13 @@ current step runs on the GPU, and the session joins the first step after it is r
 finished sessions free their memory at once, so admitted work keeps flowing rather than
 waiting for a batch to drain.
 
+With shrinking (ServingEngine(shrink=True)), a session whose classification covers every
+variant of some subject replaces its lattice by the projection onto the remaining subjects
+(update_metadata_with_shrinking, lattice_base.cpp:266-316, as the reference's trees do for
+every child, global_tree_intra.cpp:8-40), frees the difference of the footprints, and
+selects again on the smaller lattice in the next step before its next test. Its experiments
+and masks are reported in its original numbering.
+
 Responses come from a seeded simulated lab: each session draws its true state from its
 prior and samples responses with the reference's likelihood (the client side of the
 loop). A session's selections are identical whether it is served alone or in any mix,
@@ -21,6 +28,7 @@ from __future__ import annotations
 
 import concurrent.futures
 import dataclasses
+import functools
 import math
 import time
 from typing import Optional, Sequence
@@ -58,6 +66,7 @@ class SessionResult:
     finished: float
     near_ties: list = dataclasses.field(default_factory=list)  # per experiment: reported near tie
     masks: list = dataclasses.field(default_factory=list)  # per experiment: (positive, negative) before it
+    shrinks: int = 0  # projections of the session's lattice (shrinking engines)
 
     @property
     def latency(self) -> float:
@@ -84,36 +93,106 @@ def serving_mix(count: int, rate: float, seed: int = 5, lo: float = 0.005, hi: f
     return out
 
 
+@functools.lru_cache(maxsize=None)
+def _table(subjects: int) -> api.DilutionTable:
+    # DilutionTable(s, a, h) holds D[p-1][m] for pool sizes p <= s from (p, m) alone, so the
+    # table of a projected lattice is the leading rows of its original's
+    return api.DilutionTable(subjects, 0.05, 0.9)
+
+
+def _to_original(x: int, kept: Sequence[int], current: int, original: int, variants: int) -> int:
+    """A set of a projected lattice's atoms (internal numbering, variant-major) in the
+    numbering of the lattice it came from; kept[j] is current subject j's original index."""
+    out = 0
+    for v in range(variants):
+        for j, sbj in enumerate(kept):
+            if (x >> (v * current + j)) & 1:
+                out |= 1 << (v * original + sbj)
+    return out
+
+
 class _Live:
     """An admitted session: its lattice, simulated lab and progress."""
 
-    def __init__(self, spec: SessionSpec, precision: api.Precision, admitted: float):
+    def __init__(self, spec: SessionSpec, precision: api.Precision, admitted: float, shrink: bool = False):
         self.spec = spec
         t = api.LatticeType.replicated_dilution if spec.dilution else api.LatticeType.replicated_non_dilution
         self.lattice = api.Lattice(t, spec.subjects, spec.prior, variants=spec.variants, precision=precision)
-        self.table = api.DilutionTable(spec.subjects, 0.05, 0.9) if spec.dilution else None
+        self.table = _table(spec.subjects) if spec.dilution else None
         truth = workload.true_state(spec.seed, spec.prior)
         if spec.variants == 1:
             truth = api.reverse_bits(truth, spec.subjects)  # public coordinates
         self.sampler = workload.ResponseSampler(spec.seed, spec.subjects, spec.variants, truth,
                                                 list(self.table.values) if self.table is not None else None)
         self.admitted = admitted
-        self.pending: Optional[int] = None  # experiment selected and not yet applied
+        self.pending: Optional[int] = None  # experiment selected and not yet applied (lattice's coordinates)
+        self.pending_original = 0  # the same in the session's original public coordinates
+        self.shrink = shrink
+        self.kept = list(range(spec.subjects))  # original (internal) index of each current subject
+        self.removed = 0  # subjects summed out (original internal numbering)
+        self.pos_internal = self.neg_internal = 0  # classification, original internal atoms
+        self.shape = (spec.dilution, spec.subjects, spec.variants)  # the current lattice's
         self.tests = 0
         self.experiments: list[int] = []
         self.near_ties: list[bool] = []
         self.masks: list[tuple[int, int]] = []
         self.pos = self.neg = 0
+        self.shrinks = 0
 
     def classified(self) -> bool:
         return bin(self.pos | self.neg).count("1") == self.spec.subjects * self.spec.variants
+
+    def _public(self, x: int) -> int:
+        # original internal numbering -> original public coordinates (bit-reversed for k = 1)
+        return api.reverse_bits(x, self.spec.subjects) if self.spec.variants == 1 else x
+
+    def absorb(self, p: int, q: int) -> None:
+        """A step's classification masks (public coordinates of the current lattice)."""
+        if not self.shrink:
+            self.pos, self.neg = p, q
+            return
+        lat, n0, k = self.lattice, self.spec.subjects, self.spec.variants
+        nc = len(self.kept)
+        self.pos_internal |= _to_original(lat._flip(p), self.kept, nc, n0, k)
+        self.neg_internal |= _to_original(lat._flip(q), self.kept, nc, n0, k)
+        self.pos, self.neg = self._public(self.pos_internal), self._public(self.neg_internal)
+
+    def original(self, e: int) -> int:
+        """A selected experiment (current lattice's public coordinates) in the original ones."""
+        if not self.shrink or e == 0xFFFFFFFF:
+            return e
+        nc = len(self.kept)
+        return self._public(_to_original(self.lattice._flip(e), self.kept, nc, self.spec.subjects, 1))
+
+    def shrink_plan(self) -> Optional[api.ShrinkPlan]:
+        """make_shrink_plan over the classification so far, when it removes some subject."""
+        if not self.shrink:
+            return None
+        plan = api.make_shrink_plan(self.spec.subjects, len(self.kept), self.spec.variants,
+                                    self.pos_internal | self.neg_internal, self.removed)
+        return plan if plan.has_shrink() and plan.new_current_subjects > 0 else None
+
+    def apply(self, plan: api.ShrinkPlan) -> None:
+        """The lattice replaced by its projection (project_posterior, shrink.hpp:70-86)."""
+        projected = self.lattice.project(plan)
+        self.lattice.close()
+        self.lattice = projected
+        self.kept = [sbj for j, sbj in enumerate(self.kept) if not (plan.removable_current_atoms >> j) & 1]
+        assert len(self.kept) == plan.new_current_subjects
+        self.removed = plan.new_classified_subjects
+        self.shape = (self.spec.dilution, len(self.kept), self.spec.variants)
+        if self.spec.dilution:
+            self.table = _table(len(self.kept))
+        self.pending = None
+        self.shrinks += 1
 
 
 class ServingEngine:
     def __init__(self, precision: api.Precision = api.Precision.float64, hbm_budget: int = 64 << 30,
                  max_tests: int = 20, threshold_up: float = 0.01, threshold_lo: float = 0.01,
-                 max_active: int = 1024):
+                 max_active: int = 1024, shrink: bool = False):
         self.precision = precision
+        self.shrink = shrink
         self.hbm_budget = hbm_budget
         self.max_tests = max_tests
         self.up = threshold_up
@@ -121,8 +200,12 @@ class ServingEngine:
         self.max_active = max_active
 
     def footprint(self, spec: SessionSpec) -> int:
-        t = api.LatticeType.replicated_dilution if spec.dilution else api.LatticeType.replicated_non_dilution
-        return api.lattice_footprint(t, spec.subjects, spec.variants, self.precision)
+        return self._footprint((spec.dilution, spec.subjects, spec.variants))
+
+    def _footprint(self, shape: tuple) -> int:
+        dil, n, k = shape
+        t = api.LatticeType.replicated_dilution if dil else api.LatticeType.replicated_non_dilution
+        return api.lattice_footprint(t, n, k, self.precision)
 
     def run(self, specs: Sequence[SessionSpec], realtime: bool = True) -> dict:
         """Serve `specs` (arrival times relative to the start). With realtime, sessions
@@ -146,6 +229,12 @@ class ServingEngine:
         batch_sizes: list[int] = []
         admit_wait_s = 0.0  # time the serving thread waited for admissions with nothing to step
         fp = {}
+
+        def footprint(shape) -> int:
+            if shape not in fp:
+                fp[shape] = self._footprint(shape)
+            return fp[shape]
+
         t0 = time.perf_counter()
         builder = concurrent.futures.ThreadPoolExecutor(max_workers=1)
 
@@ -160,15 +249,13 @@ class ServingEngine:
                     spec = queue[qi]
                     if realtime and spec.arrival > tnow:
                         break
-                    key = (spec.dilution, spec.subjects, spec.variants)
-                    if key not in fp:
-                        fp[key] = self.footprint(spec)
-                    if used + fp[key] > self.hbm_budget:
+                    need = footprint((spec.dilution, spec.subjects, spec.variants))
+                    if used + need > self.hbm_budget:
                         if not live and not building:
                             raise RuntimeError(f"session {spec.sid} alone exceeds the HBM budget")
                         break
-                    building.append((builder.submit(_Live, spec, self.precision, tnow), spec))
-                    used += fp[key]
+                    building.append((builder.submit(_Live, spec, self.precision, tnow, self.shrink), spec))
+                    used += need
                     peak_used = max(peak_used, used)
                     qi += 1
                 # sessions whose lattices are ready join this step
@@ -200,7 +287,7 @@ class ServingEngine:
                         resps.append(None)
                     else:
                         exps.append(s.pending)
-                        resps.append(s.sampler.draw(s.pending))
+                        resps.append(s.sampler.draw(s.pending_original))
                 ts = time.perf_counter()
                 sels, masks = api.batch_round([s.lattice for s in live], exps, resps, api.SelectorType.auto_select,
                                               self.up, self.lo, [s.table for s in live])
@@ -211,21 +298,31 @@ class ServingEngine:
                 for s, sel, (p, q), r in zip(live, sels, masks, resps):
                     if r is not None:
                         s.tests += 1
-                    s.pos, s.neg = p, q
+                    s.absorb(p, q)
                     finished = s.classified() or s.tests >= self.max_tests
                     if not finished:
+                        plan = s.shrink_plan()
+                        if plan is not None:
+                            # projected: this step's selection was on the larger lattice, so the
+                            # next step selects again before the next test
+                            before = s.shape
+                            s.apply(plan)
+                            used += footprint(s.shape) - footprint(before)
+                            keep.append(s)
+                            continue
                         s.pending = sel.experiment
-                        s.experiments.append(sel.experiment)
+                        s.pending_original = s.original(sel.experiment)
+                        s.experiments.append(s.pending_original)
                         s.near_ties.append(bool(sel.near_tie))
-                        s.masks.append((p, q))
+                        s.masks.append((s.pos, s.neg))
                         keep.append(s)
                         continue
                     spec = s.spec
                     done.append(SessionResult(spec.sid, spec.subjects, spec.variants, spec.dilution, s.tests,
                                               s.classified(), s.pos, s.neg, s.experiments,
                                               spec.arrival if realtime else 0.0, s.admitted,
-                                              tdone, s.near_ties, s.masks))
-                    used -= fp[(spec.dilution, spec.subjects, spec.variants)]
+                                              tdone, s.near_ties, s.masks, s.shrinks))
+                    used -= footprint(s.shape)
                     s.lattice.close()
                 live = keep
         finally:
@@ -252,24 +349,28 @@ class ServingEngine:
 
 
 def serve_alone(spec: SessionSpec, precision: api.Precision = api.Precision.float64, max_tests: int = 20,
-                up: float = 0.01, lo: float = 0.01) -> SessionResult:
-    """The same session through per-lattice calls (update, classify, select), one at a time."""
-    s = _Live(spec, precision, 0.0)
-    sel = s.lattice.select(api.SelectorType.auto_select)
-    s.lattice.update_classification(up, lo)
-    s.pos, s.neg = s.lattice.positive_classification_mask(), s.lattice.negative_classification_mask()
-    while not (s.classified() or s.tests >= max_tests):
-        e = sel.experiment
-        s.experiments.append(e)
-        r = s.sampler.draw(e)
-        s.lattice.update(e, r, s.table)
-        s.tests += 1
+                up: float = 0.01, lo: float = 0.01, shrink: bool = False) -> SessionResult:
+    """The same session through per-lattice calls (update, classify, select), one at a time,
+    in the order of the engine's steps (a projection, then a selection on the new lattice)."""
+    s = _Live(spec, precision, 0.0, shrink)
+    while True:
+        if s.pending is not None:
+            r = s.sampler.draw(s.pending_original)
+            s.lattice.update(s.pending, r, s.table)
+            s.tests += 1
         s.lattice.update_classification(up, lo)
-        s.pos, s.neg = s.lattice.positive_classification_mask(), s.lattice.negative_classification_mask()
+        s.absorb(s.lattice.positive_classification_mask(), s.lattice.negative_classification_mask())
         if s.classified() or s.tests >= max_tests:
             break
+        plan = s.shrink_plan()
+        if plan is not None:
+            s.apply(plan)
+            continue
         sel = s.lattice.select(api.SelectorType.auto_select)
+        s.pending = sel.experiment
+        s.pending_original = s.original(sel.experiment)
+        s.experiments.append(s.pending_original)
     out = SessionResult(spec.sid, spec.subjects, spec.variants, spec.dilution, s.tests, s.classified(), s.pos, s.neg,
-                        s.experiments, 0.0, 0.0, 0.0)
+                        s.experiments, 0.0, 0.0, 0.0, shrinks=s.shrinks)
     s.lattice.close()
     return out

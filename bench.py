@@ -24,8 +24,9 @@ timing, the autotuner measures the transform forms for this KernelSpec (untimed)
 names the form it chose.
 
 Also reported, each for both storage forms: the batched tree simulation (n = 16 dilution,
-depth 6) to TreeStats, Op-BBPA rounds of the n = 8, k = 3 dilution lattice, and the serving
-loop over a mix of adaptive sessions with Poisson arrivals (p50/p99 latency, sessions/s).
+depth 6) to TreeStats, with full-size and with shrinking children, Op-BBPA rounds of the
+n = 8, k = 3 dilution lattice, and the serving loop over a mix of adaptive sessions with
+Poisson arrivals (p50/p99 latency, sessions/s), with and without shrinking sessions.
 
 CPU baseline: the C restatement of the reference CPU provider (oracle/, built with the
 reference's default flags) runs one full n = 28 round on one host thread.
@@ -278,6 +279,25 @@ def tree_workload(repeats: int = 3) -> dict:
             best = min(best, time.perf_counter() - t0)
         stats[prec] = st
         out["fp64" if prec == bgt.Precision.float64 else "fp32c"] = {"ms": best * 1e3, "leaves": st.total_leaves}
+    # the same trees with shrinking children (projections, as the reference's LocalTree)
+    prev = bgt.set_tree_shrink(True)
+    try:
+        for prec in (bgt.Precision.float64, bgt.Precision.float32_compensated):
+            cfg = bgt.SimulationConfig(lattice_type=bgt.LatticeType.replicated_dilution, subjects=c.subjects,
+                                       variants=1, prior=pi)
+            cfg.options.search_depth = 6
+            cfg.options.precision = prec
+            cfg.dilution = bgt.DilutionParameters(c.alpha, c.h)
+            best = float("inf")
+            for _ in range(repeats):
+                t0 = time.perf_counter()
+                st = bgt.run_simulation(cfg).stats
+                best = min(best, time.perf_counter() - t0)
+            key = "fp64" if prec == bgt.Precision.float64 else "fp32c"
+            out[key]["shrinking_ms"] = best * 1e3
+            out[key]["shrinking_leaves_equal"] = st.total_leaves == stats[prec].total_leaves
+    finally:
+        bgt.set_tree_shrink(prev)
     t0 = time.perf_counter()
     ref = orc.run_simulation(True, c.subjects, 1, pi, 6, alpha=c.alpha, h=c.h, variant="fast")
     out["cpu_port_ms"] = (time.perf_counter() - t0) * 1e3
@@ -463,6 +483,16 @@ def serving_workload(sessions: int, rate: float, seed: int = 5) -> dict:
                                                             or d["classification_first"])][:20]
     out["fp32c_vs_fp64_capacity"] = (out["fp32c"]["saturated"]["sessions_per_s"] /
                                      out["fp64"]["saturated"]["sessions_per_s"])
+    # shrinking sessions (a session's lattice projected whenever a subject is fully
+    # classified, as the reference's trees do): capacity at saturation, both forms
+    for prec, key in ((bgt.Precision.float64, "fp64"), (bgt.Precision.float32_compensated, "fp32c")):
+        eng = serving.ServingEngine(precision=prec, hbm_budget=96 << 30, shrink=True)
+        rep = eng.run(specs, realtime=False)
+        results = rep.pop("results")
+        rep["sessions_shrunk"] = sum(r.shrinks > 0 for r in results)
+        out[key]["saturated_shrinking"] = rep
+        log(f"serving {key} saturated, shrinking: {rep['sessions_per_s']:.1f} sessions/s "
+            f"p50 {rep['latency_ms']['p50']:.1f} ms p99 {rep['latency_ms']['p99']:.1f} ms")
     return out
 
 

@@ -308,6 +308,16 @@ int bgt_hip_batch_graphs(int enable, int *previous);
  * process started (either may be NULL). */
 int bgt_hip_batch_graph_counts(unsigned long long *captures, unsigned long long *replays);
 
+/* Shrinking in bgt_hip_provider_run_simulation's tree, off by default: on, a child whose
+ * classification covers every variant of some subject is replaced by its projection onto the
+ * other subjects and selects in that numbering (global_tree_intra.cpp:8-40 with
+ * update_metadata_with_shrinking, lattice_base.cpp:266-316); off keeps full-size children
+ * with experiments restricted to the subjects still in play (equal in exact arithmetic, as
+ * the reference's no_shrink_tree_reference_test states, and faster where the tree's levels
+ * are bound by launches). previous (may be NULL) receives the former setting. No reference
+ * counterpart (the reference always shrinks). */
+int bgt_hip_tree_shrink(int enable, int *previous);
+
 /* ---- autotuning ---- */
 
 /* Measure once per KernelSpec key (include/bgt/types.hpp:132-181) which launch forms of the

@@ -3,8 +3,12 @@ model::Lattice (update_metadata_with_shrinking, lattice_base.cpp:266-316) agains
 
 Bars:
 * The projection of a device posterior is bit-identical to the oracle's projection of the
-  same values. FP64 projects the device's FP64 values; FP32C projects its FP32 values and
-  stores each FP64 sum rounded to FP32, which the oracle's sums rounded the same way give.
+  same values when it removes at most 6 atoms (2^6 terms per kept entry, summed in the
+  reference's serial order). FP64 projects the device's FP64 values; FP32C projects its
+  FP32 values and stores each FP64 sum rounded to FP32, which the oracle's sums rounded the
+  same way give. Larger removals sum each kept entry's terms in chunks (a fixed order, not
+  the serial one): within the serial sum's own rounding bound (2^r roundings) of the
+  oracle's FP64 sums, FP32C within one FP32 rounding of them.
 * A shrinking session (update, classification, projection whenever a subject becomes fully
   classified) follows the oracle's own update_metadata_with_shrinking step for step: equal
   classification masks, removed subjects and subject counts, and posteriors within 1e-12
@@ -58,6 +62,44 @@ def test_projection_is_bit_identical(n, k, dil, prec, removed_subjects):
     np.testing.assert_array_equal(got, want)
     # the source is unchanged and still usable
     np.testing.assert_array_equal(sl.lattice.posterior(), x)
+    proj.close()
+    sl.close()
+
+
+@pytest.mark.parametrize("n,k,dil,prec,removed_subjects", [
+    (14, 1, False, F64, list(range(2, 12))), (12, 1, True, F32, [0, 1, 2, 4, 5, 7, 8, 9, 11]),
+    (9, 2, False, F64, [0, 1, 2, 3, 5, 6, 7]), (16, 1, False, F64, list(range(13))), (7, 3, True, F32, [1, 2, 5]),
+])
+def test_wide_projection_matches_the_oracle(n, k, dil, prec, removed_subjects):
+    pi = seeded_prior(760 + n * k, n * k, 0.02, 0.2)
+    table = bgt.DilutionTable(n, 0.05, 0.9) if dil else None
+    sl = bgt.ShrinkingLattice(DL if dil else ND, n, pi, variants=k, precision=prec, dilution=table)
+    g = workload.SplitMix64(n * 37 + k)
+    for r in _responses(n + 2 * k, k, 3):
+        sl.update(g.below(1 << n) | 1, r)
+    x = sl.lattice.posterior()
+    classified = 0
+    for i in removed_subjects:
+        for v in range(k):
+            classified |= 1 << (v * n + i)
+    plan = bgt.make_shrink_plan(n, n, k, classified, 0)
+    r = plan.reduced_atom_count
+    assert r > 6
+    print(f"wide projection n={n} k={k} removes {r} atoms, {prec.name}", flush=True)
+    proj = sl.lattice.project(plan)
+    got = proj.posterior()
+    want = orc.project_posterior(x, orc.make_shrink_plan(n, n, k, classified, 0))
+    if prec == F64:
+        # both are sums of 2^r positive terms: the oracle's serial order carries up to 2^r
+        # roundings of the sum, the device's chunked order fewer
+        np.testing.assert_array_less(np.abs(got - want), (2.0 ** r + 64) * 2.0 ** -53 * want + 1e-300)
+    else:
+        # one FP32 rounding of sums that differ far below FP32's resolution
+        w32 = want.astype(np.float32).astype(np.float64)
+        assert np.all(np.abs(got - w32) <= np.spacing(np.abs(w32).astype(np.float32)).astype(np.float64)), \
+            np.max(np.abs(got - w32) / w32)
+    # the mass is kept (FP32C: each stored entry one FP32 rounding)
+    assert abs(got.sum() - x.sum()) <= (1e-12 if prec == F64 else 2.0 ** -23) * x.sum()
     proj.close()
     sl.close()
 

@@ -1,0 +1,232 @@
+"""Shrinking on the hot path: exact masses of projected lattices, FP32C re-decisions after a
+projection, and the serving engine with shrinking, against the oracle.
+
+A projected lattice keeps the exact program of the lattice it came from, in that lattice's
+numbering: its M(E) is the program's M of E placed among the kept subjects (the projected
+posterior is the marginal of the original, project_posterior, shrink.hpp:70-86, and stays
+so under updates that pool kept subjects only). Pinned by:
+  * the oracle's own projection (orc.project_posterior) followed by the same updates, its
+    superset sums taken accurately (math.fsum): 1e-12 relative, for projections of
+    projections too, k = 1 and k = 2, with and without dilution;
+  * FP32C shrinking sessions: every pick is the oracle's, or the oracle's own objectives of
+    the two are within the FP64 tie bound (or the band overflowed and was reported);
+  * served shrinking sessions (ServingEngine(shrink=True), FP64) follow the oracle's
+    update_metadata_with_shrinking (orc.shrink_step) step for step: the same experiments in
+    the original numbering, the same test counts and classification masks;
+  * a shrinking session served alone (per-lattice calls) equals the same session in a mix.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import bayesiangrouptesting_amd as bgt
+from bayesiangrouptesting_amd import serving, workload
+from oracle import pyoracle as orc
+from tests.gpu_helpers import (accurate_scores, assert_pick, oracle_scores, pick_bound, seeded_prior,
+                               tie_bound)
+
+pytestmark = pytest.mark.gpu
+ND = bgt.LatticeType.replicated_non_dilution
+DL = bgt.LatticeType.replicated_dilution
+F64 = bgt.Precision.float64
+F32 = bgt.Precision.float32_compensated
+
+
+def _superset_masses(post, atoms, queries):
+    states = np.arange(1 << atoms, dtype=np.uint64)
+    total = math.fsum(post)
+    return np.array([math.fsum(post[(states & np.uint64(q)) == np.uint64(q)]) / total for q in queries])
+
+
+def _subject_atoms(subjects, n, k):
+    out = 0
+    for i in subjects:
+        for v in range(k):
+            out |= 1 << (v * n + i)
+    return out
+
+
+def _kept_after(kept, plan):
+    return [s for j, s in enumerate(kept) if not (plan.removable_current_atoms >> j) & 1]
+
+
+@pytest.mark.parametrize("n,k,dil,first,second", [
+    (12, 1, False, [2, 7], [0]), (11, 1, True, [10], [3, 4]), (7, 2, False, [1], [5]), (6, 2, True, [0, 3], [1]),
+])
+def test_projected_exact_masses_match_the_oracle(n, k, dil, first, second):
+    seed = 40 + n * k
+    atoms = n * k
+    pi = seeded_prior(seed, atoms, 0.02, 0.2)
+    dtab = orc.dilution_table(n, 0.05, 0.9) if dil else None
+    lat = bgt.Lattice(DL if dil else ND, n, pi, variants=k, precision=F32)
+    post = orc.prior(pi)
+    g = workload.SplitMix64(seed)
+    cur, kept, removed = n, list(range(n)), 0
+
+    def updates(count):
+        nonlocal post
+        for _ in range(count):
+            e = int(g.next_u64() & g.next_u64() & ((1 << cur) - 1)) or 1
+            r = g.below(1 << k)
+            tab = bgt.DilutionTable(cur, 0.05, 0.9) if dil else None
+            lat.update(bgt.reverse_bits(e, cur) if k == 1 else e, r, tab)
+            orc.update_in_place(post, dil, cur, k, e, r, dtab)
+
+    def project(subjects_original):
+        nonlocal lat, post, cur, kept, removed
+        classified = _subject_atoms(subjects_original, n, k) | _subject_atoms(
+            [i for i in range(n) if (removed >> i) & 1], n, k)
+        plan = bgt.make_shrink_plan(n, cur, k, classified, removed)
+        post = orc.project_posterior(post, orc.make_shrink_plan(n, cur, k, classified, removed))
+        nxt = lat.project(plan)
+        lat.close()
+        lat = nxt
+        kept = _kept_after(kept, plan)
+        cur, removed = plan.new_current_subjects, plan.new_classified_subjects
+
+    updates(4)
+    project(first)
+    updates(3)
+    project(second)
+    updates(3)
+    assert len(kept) == cur and len(post) == 1 << (cur * k)
+    ca = cur * k
+    qs = [0, (1 << ca) - 1] + [int(g.next_u64() & g.next_u64() & ((1 << ca) - 1)) for _ in range(24)]
+    print(f"projected exact masses n={n} k={k} dil={dil}: {cur} subjects left", flush=True)
+    got, bound = lat.exact_masses([bgt.reverse_bits(q, cur) if k == 1 else q for q in qs])
+    want = _superset_masses(post, ca, qs)
+    rel = np.abs(got - want) / want
+    assert np.max(rel) <= 1e-12, (np.max(rel), qs[int(np.argmax(rel))])
+    assert 0.0 < bound < 1e-12
+    lat.close()
+
+
+@pytest.mark.parametrize("n,k,dil,seed", [(14, 1, False, 3), (13, 1, True, 4), (8, 2, False, 5), (7, 2, True, 6)])
+def test_fp32c_shrinking_sessions_pick_the_oracles_experiments(n, k, dil, seed):
+    # classification thresholds wide enough that subjects leave early; every selection after a
+    # projection runs on the smaller lattice, its near ties re-decided in the original frame
+    up = lo = 0.05
+    atoms = n * k
+    pi = seeded_prior(seed, atoms, 0.005, 0.1)
+    table = bgt.DilutionTable(n, 0.05, 0.9) if dil else None
+    dtab = orc.dilution_table(n, 0.05, 0.9) if dil else None
+    sl = bgt.ShrinkingLattice(DL if dil else ND, n, pi, variants=k, precision=F32, dilution=table)
+    post = orc.prior(pi)
+    cur, pos, neg, clas = n, 0, 0, 0
+    truth = workload.true_state(seed, list(pi))
+    sampler = workload.ResponseSampler(seed, n, k, truth, list(dtab) if dil else None)
+    selector = 1 if k == 1 else 2
+    shrinks = 0
+    for step in range(10):
+        s = sl.lattice.select(bgt.SelectorType.auto_select)
+        e = bgt.reverse_bits(s.experiment, cur) if k == 1 else s.experiment
+        scores = accurate_scores(post, cur, k, selector)
+        e_ref = int(np.argmin(scores))
+        bound = tie_bound(cur * k, k, selector, updates=step + shrinks)
+        print(f"fp32c shrinking n={n} k={k} step {step}: {cur} subjects, pick {e:#x} oracle {e_ref:#x} "
+              f"near_tie={s.near_tie} redecided={s.redecided}", flush=True)
+        if e != e_ref:
+            gap = float(scores[e] - scores[e_ref])
+            assert gap <= bound or (s.redecided == 2 and s.near_tie), (step, e, e_ref, gap, bound, s)
+        kept = [i for i in range(n) if not (clas >> i) & 1]
+        full = sum(1 << kept[j] for j in range(cur) if (e_ref >> j) & 1)
+        r = sampler.draw(full)
+        sl.update(e_ref, r)
+        orc.update_in_place(post, dil, cur, k, e_ref, r, dtab)
+        nxt = orc.shrink_step(n, k, up, lo, post, cur, pos, neg, clas)
+        if nxt[1] == 0:
+            break
+        sl.update_metadata_with_shrinking(up, lo)
+        shrinks += nxt[1] < cur
+        post, cur, pos, neg, clas = nxt
+        assert (sl.positive, sl.negative, sl.current_subjects) == (pos, neg, cur), step
+    assert shrinks >= 1, "the session never shrank: pick a case that classifies"
+    sl.close()
+
+
+def _shrink_slice(specs):
+    # up to three sessions of each (variants, dilution) kind with at most 20 atoms, one of
+    # them with 18 or more
+    picks, seen, big = [], {}, set()
+    for s in specs:
+        atoms = s.subjects * s.variants
+        if atoms > 20:
+            continue
+        key = (s.variants, s.dilution)
+        want_big = key not in big and atoms >= 18
+        if seen.get(key, 0) < 2 or (want_big and seen.get(key, 0) < 3):
+            seen[key] = seen.get(key, 0) + 1
+            big.update([key] if atoms >= 18 else [])
+            picks.append(s)
+    return picks
+
+
+SPECS = _shrink_slice(serving.serving_mix(512, rate=300.0, seed=5))
+
+
+@pytest.fixture(scope="module")
+def served():
+    eng = serving.ServingEngine(precision=F64, hbm_budget=16 << 30, shrink=True)
+    rep = eng.run(SPECS, realtime=False)
+    assert rep["sessions"] == len(SPECS)
+    return {r.sid: r for r in rep["results"]}
+
+
+def test_shrinking_slice_shrinks(served):
+    assert {(s.variants, s.dilution) for s in SPECS} == {(1, False), (1, True), (2, False), (2, True)}
+    assert sum(r.shrinks > 0 for r in served.values()) >= len(SPECS) // 2
+
+
+@pytest.mark.parametrize("spec", SPECS, ids=lambda s: f"sid{s.sid}-n{s.subjects}k{s.variants}{'d' if s.dilution else ''}")
+def test_served_shrinking_session_follows_the_oracle(served, spec):
+    got = served[spec.sid]
+    n, k = spec.subjects, spec.variants
+    selector = 1 if k == 1 else 2
+    flip = (lambda v: orc.reverse_bits(v, n)) if k == 1 else (lambda v: v)
+    dtab = orc.dilution_table(n, 0.05, 0.9) if spec.dilution else None
+    truth = workload.true_state(spec.seed, spec.prior)
+    sampler = workload.ResponseSampler(spec.seed, n, k, flip(truth), list(dtab) if dtab is not None else None)
+    post = orc.prior(spec.prior)
+    cur, pos, neg, clas = n, 0, 0, 0
+    up = lo = 0.01
+    tests = 0
+    while True:
+        # classification, then the shrink, as update_metadata_with_shrinking does
+        nxt = orc.shrink_step(n, k, up, lo, post, cur, pos, neg, clas)
+        pos, neg = nxt[2], nxt[3]
+        if bin(pos | neg).count("1") == n * k or tests >= 20:
+            break
+        post, cur, clas = nxt[0], nxt[1], nxt[4]
+        kept = [i for i in range(n) if not (clas >> i) & 1]
+        e_ref, objs = oracle_scores(post, cur, k, selector)
+        acc = accurate_scores(post, cur, k, selector)
+        assert tests < len(got.experiments), (tests, got.experiments)
+        e_full = flip(got.experiments[tests])  # original internal numbering
+        assert e_full & clas == 0, (tests, e_full, clas)
+        e_dev = sum(1 << j for j in range(cur) if (e_full >> kept[j]) & 1)
+        scale_err = abs(post.sum() - 1.0) + 2.0 ** -45
+        bound = tie_bound(cur * k, k, selector, scale_err, updates=tests + bin(clas).count("1"))
+        assert_pick(e_dev, objs, e_ref, pick_bound(objs, acc, e_dev, e_ref, bound), f"test {tests}")
+        if e_dev != e_ref:
+            return  # a tie within FP64 rounding: the trajectories part here
+        r = sampler.draw(flip(e_full))
+        orc.update_in_place(post, spec.dilution, cur, k, e_ref, r, dtab)
+        tests += 1
+    assert got.tests == tests
+    assert (flip(got.positive), flip(got.negative)) == (pos, neg)
+    assert len(got.experiments) == tests
+
+
+@pytest.mark.parametrize("prec", [F64, F32])
+def test_shrinking_session_alone_equals_the_mix(prec):
+    specs = SPECS[:6]
+    eng = serving.ServingEngine(precision=prec, hbm_budget=16 << 30, shrink=True)
+    rep = eng.run(specs, realtime=False)
+    mixed = {r.sid: r for r in rep["results"]}
+    for spec in specs:
+        print(f"shrinking alone vs mix: sid {spec.sid} {prec.name}", flush=True)
+        alone = serving.serve_alone(spec, prec, shrink=True)
+        m = mixed[spec.sid]
+        assert (alone.experiments, alone.tests, alone.positive, alone.negative, alone.shrinks) == \
+            (m.experiments, m.tests, m.positive, m.negative, m.shrinks), spec.sid

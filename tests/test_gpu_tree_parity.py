@@ -1,6 +1,8 @@
 """run_simulation -> TreeStats on the GPU against the CPU restatement's shrinking tree
 (simulation_runner.cpp:92-132) and the reference's own known answers. Bar: 1e-12
-absolute (tests/cpp/support/assertions.hpp:14-31), identical leaf counts."""
+absolute (tests/cpp/support/assertions.hpp:14-31), identical leaf counts. Every case runs
+twice: with the device tree's shrinking children and with full-size children restricted to
+the subjects still in play (the default)."""
 import numpy as np
 import pytest
 
@@ -13,6 +15,13 @@ pytestmark = pytest.mark.gpu
 FIELDS = ("correct_probability", "incorrect_probability", "false_positive_probability",
           "false_negative_probability", "unclassified_probability", "expected_stages", "expected_tests")
 SEL = {"auto_select": 0, "op_bha": 1, "op_bbpa": 2, "brute_force": 3}
+
+
+@pytest.fixture(autouse=True, params=[True, False], ids=["shrink", "full"])
+def tree_mode(request):
+    prev = bgt.set_tree_shrink(request.param)
+    yield request.param
+    bgt.set_tree_shrink(prev)
 
 
 def gpu_stats(dilution, n, k, pi, depth, up=0.01, lo=0.01, branch=0.0, selector=0, alpha=0.99, h=0.005,
