@@ -192,6 +192,7 @@ void batch_step(std::vector<BatchItem> &items, int selector, double up, double l
 	// selection descriptors the state after it
 	std::vector<LatDesc> du(count), ds(count);
 	std::vector<const double *> liks(count, nullptr);
+	std::vector<char> staged(count, 0); // k_exact_band runs for these lattices' bands
 	for (size_t i = 0; i < count; ++i) {
 		auto &it = items[i];
 		if (it.update) {
@@ -208,6 +209,10 @@ void batch_step(std::vector<BatchItem> &items, int selector, double up, double l
 		ds[i] = it.lat->desc();
 		ds[i].band_width =
 			it.select ? it.lat->band_width(DeviceLattice::resolve_selector(selector, it.lat->variants())) : 0.0;
+		// FP32C binary non-dilution selections: their bands' exact masses on the device, as
+		// the per-lattice round computes them (near ties are common in this form)
+		if (it.select && it.lat->storage() == Storage::fp32c)
+			staged[i] = it.lat->stage_exact(ds[i], true) ? 1 : 0;
 	}
 	// launch groups (see GroupKey); each group gets its own contiguous block of device
 	// descriptors
@@ -217,7 +222,7 @@ void batch_step(std::vector<BatchItem> &items, int selector, double up, double l
 		int max_atoms = 0;  // its largest lattice (ragged groups: sizes the grid)
 		int max_subjects = 0;
 	};
-	std::map<GroupKey, Group> gu, gc, gs;
+	std::map<GroupKey, Group> gu, gc, gs, gx;
 	auto join = [&](std::map<GroupKey, Group> &gm, const GroupKey &key, size_t i) {
 		Group &g = gm[key];
 		g.idx.push_back(i);
@@ -232,11 +237,13 @@ void batch_step(std::vector<BatchItem> &items, int selector, double up, double l
 			join(gc, all_key(lat), i);
 		if (items[i].select)
 			join(gs, select_key(lat, selector), i);
+		if (staged[i])
+			join(gx, all_key(lat), i);
 	}
 	size_t off = 0;
 	const size_t o_lik = off;
 	off = align256(off + sizeof(double) * kLikCapacity * count);
-	for (auto *gm : {&gu, &gc, &gs})
+	for (auto *gm : {&gu, &gc, &gs, &gx})
 		for (auto &g : *gm) {
 			g.second.offset = off;
 			off = align256(off + sizeof(LatDesc) * g.second.idx.size());
@@ -265,6 +272,7 @@ void batch_step(std::vector<BatchItem> &items, int selector, double up, double l
 	fill(gu, du);
 	fill(gc, ds);
 	fill(gs, ds);
+	fill(gx, ds);
 	std::memcpy(h + o_all, ds.data(), sizeof(LatDesc) * count);
 	hipStream_t s = st.stream;
 	// the step's copies and launches, on the stream or into a graph being captured from it
@@ -328,7 +336,11 @@ void batch_step(std::vector<BatchItem> &items, int selector, double up, double l
 				launch_band_collect(dg, cnt, entries, atoms, k, lat.storage(), objective, atoms == kTileBits, false,
 									target, s);
 		}
-		// 4. one copy brings every head back
+		// 4. the exact masses of the staged bands (lattices whose top two are no near tie return
+		// at once; a few workgroups per lattice share its band entries)
+		for (auto &g : gx)
+			launch_exact_band(descs_at(g.second.offset, true), (int)g.second.idx.size(), g.second.max_atoms, s, 16);
+		// 5. one copy brings every head back
 		launch_gather_heads(reinterpret_cast<const LatDesc *>(d + o_all), (int)count,
 							reinterpret_cast<unsigned char *>(d + o_heads), s);
 		BGT_HIP_CHECK(hipMemcpyAsync(h + o_heads, d + o_heads, kHeadBytes * count, hipMemcpyDeviceToHost, s));
@@ -341,7 +353,7 @@ void batch_step(std::vector<BatchItem> &items, int selector, double up, double l
 					  (long long)reinterpret_cast<intptr_t>(d),
 					  (long long)reinterpret_cast<intptr_t>(h)};
 		int kind = 0;
-		for (auto *gm : {&gu, &gc, &gs}) {
+		for (auto *gm : {&gu, &gc, &gs, &gx}) {
 			for (auto &g : *gm) {
 				const DeviceLattice &lat = *items[g.second.idx[0]].lat;
 				sig.insert(sig.end(), {kind, std::get<0>(g.first), std::get<1>(g.first), std::get<2>(g.first),
@@ -384,7 +396,7 @@ void batch_step(std::vector<BatchItem> &items, int selector, double up, double l
 			it.lat->prepare_exact(); // on the host while the device works
 	BGT_HIP_CHECK(hipStreamSynchronize(s));
 	for (size_t i = 0; i < count; ++i)
-		items[i].lat->load_head(reinterpret_cast<const unsigned char *>(h + o_heads + kHeadBytes * i));
+		items[i].lat->load_head(reinterpret_cast<const unsigned char *>(h + o_heads + kHeadBytes * i), staged[i] != 0);
 	// the exact normalisers of the lattices whose selections will be re-decided, side by side
 	// on the host threads (each lattice's program is its own; the results do not depend on
 	// which thread computes them)

@@ -327,9 +327,12 @@ void launch_bbpa_hist(const Descs &descs, int count, int n, int k, Storage st, b
 // take this path).
 void launch_bbpa_finish(const Descs &descs, int count, int k, bool fused, double target, hipStream_t stream);
 // Unnormalised exact upset masses U(E) of the first kExactBand band candidates of each
-// lattice (d.exact_u[i], -1 where the device program does not apply), from d.pairs and
-// d.hist: the quantity ExactHistory computes on the host, as a dense dynamic program.
-void launch_exact_band(const Descs &descs, int count, int atoms, hipStream_t stream);
+// lattice whose top two are a near tie (d.exact_u[i], -1 where the device program does not
+// apply, -2 for dominated entries), from d.pairs and d.hist: the quantity ExactHistory
+// computes on the host, as a dense dynamic program. per_lattice workgroups (one wavefront
+// each) share a lattice's band entries; ragged descriptors carry their own atom counts.
+void launch_exact_band(const Descs &descs, int count, int atoms, hipStream_t stream,
+					   unsigned per_lattice = kExactBand);
 
 // Exact unnormalised upset masses by enumeration, for binary lattices of either likelihood
 // (replicated_lattice.cpp:8-28): U(E) = sum over the states S >= E of

@@ -3081,124 +3081,131 @@ __global__ __launch_bounds__(64) void k_exact_band(Descs descs, int atoms)
 	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
 	if (d.exact_u == nullptr || d.band == nullptr)
 		return;
+	if (descs.ragged)
+		atoms = d.atoms;
 	// the newest record arrived as an argument: keep it on the device for later rounds (no
 	// block reads that slot in this launch; every lane takes the argument instead)
 	if (d.hist_new_at >= 0 && blockIdx.x == 0 && threadIdx.x == 0)
 		d.hist[d.hist_new_at] = d.hist_new;
-	const unsigned count = min(d.counter[0], d.band_cap);
-	const unsigned i = blockIdx.x;
-	if (i >= count || i >= (unsigned)kExactBand)
+	// only a near tie is re-decided: the host's own test on the same two values
+	if (!(d.result->d2 - d.result->d1 <= d.band_width))
 		return;
+	const unsigned count = min(d.counter[0], d.band_cap);
 	__shared__ uint32_t s_pool[kExactMaxPools];
 	__shared__ double s_fin[kExactMaxPools], s_fout[kExactMaxPools];
 	const int lane = threadIdx.x; // one wavefront
 	const int h = d.hist_len;
-	const Cand me = d.band[i];
-	const uint32_t e = me.e;
-	// Dominance (every state has positive mass, so M strictly decreases under inclusion):
-	// a candidate surely above 1/2 loses to a one-atom extension in the band that is surely
-	// above 1/2, one surely below to a one-atom reduction surely below. Such candidates
-	// need no mass: -2. Sure side: signed and |d| beyond half the band width. (The host
-	// uses these marks only for bands of at most kExactBand entries.)
-	const double half_band = 0.5 * d.band_width;
-	const int side = ((me.flags & kCandSided) && fabs(me.d) > half_band) ? (me.d > 0.0 ? 1 : -1) : 0;
-	if (side != 0 && count <= (unsigned)kExactBand) {
-		bool dom = false;
-		for (unsigned base = 0; base < count; base += 64 * 8) {
-			Cand o[8]; // eight loads in flight per lane
-#pragma unroll
-			for (int u = 0; u < 8; ++u) {
-				const unsigned j = base + (unsigned)lane + 64u * u;
-				o[u] = j < count ? d.band[j] : me;
+	// band entries blockIdx.x, + gridDim.x, ...: every decision below is uniform over the
+	// wavefront, so the early continues keep the barriers matched
+	for (unsigned i = blockIdx.x; i < count && i < (unsigned)kExactBand; i += gridDim.x) {
+		const Cand me = d.band[i];
+		const uint32_t e = me.e;
+		// Dominance (every state has positive mass, so M strictly decreases under inclusion):
+		// a candidate surely above 1/2 loses to a one-atom extension in the band that is surely
+		// above 1/2, one surely below to a one-atom reduction surely below. Such candidates
+		// need no mass: -2. Sure side: signed and |d| beyond half the band width. (The host
+		// uses these marks only for bands of at most kExactBand entries.)
+		const double half_band = 0.5 * d.band_width;
+		const int side = ((me.flags & kCandSided) && fabs(me.d) > half_band) ? (me.d > 0.0 ? 1 : -1) : 0;
+		if (side != 0 && count <= (unsigned)kExactBand) {
+			bool dom = false;
+			for (unsigned base = 0; base < count; base += 64 * 8) {
+				Cand o[8]; // eight loads in flight per lane
+	#pragma unroll
+				for (int u = 0; u < 8; ++u) {
+					const unsigned j = base + (unsigned)lane + 64u * u;
+					o[u] = j < count ? d.band[j] : me;
+				}
+	#pragma unroll
+				for (int u = 0; u < 8; ++u) {
+					const uint32_t x = o[u].e ^ e;
+					if (x == 0u || (x & (x - 1u)) != 0u)
+						continue;
+					const int os =
+						((o[u].flags & kCandSided) && fabs(o[u].d) > half_band) ? (o[u].d > 0.0 ? 1 : -1) : 0;
+					dom = dom || (side > 0 && os > 0 && (o[u].e & x) != 0u) ||
+						  (side < 0 && os < 0 && (o[u].e & x) == 0u);
+				}
 			}
-#pragma unroll
-			for (int u = 0; u < 8; ++u) {
-				const uint32_t x = o[u].e ^ e;
-				if (x == 0u || (x & (x - 1u)) != 0u)
-					continue;
-				const int os =
-					((o[u].flags & kCandSided) && fabs(o[u].d) > half_band) ? (o[u].d > 0.0 ? 1 : -1) : 0;
-				dom = dom || (side > 0 && os > 0 && (o[u].e & x) != 0u) ||
-					  (side < 0 && os < 0 && (o[u].e & x) == 0u);
+			if (__ballot(dom) != 0ull) {
+				if (lane == 0)
+					d.exact_u[i] = -2.0;
+				continue;
 			}
 		}
-		if (__ballot(dom) != 0ull) {
+		if (h > kExactMaxHist) {
 			if (lane == 0)
-				d.exact_u[i] = -2.0;
-			return;
+				d.exact_u[i] = -1.0;
+			continue;
 		}
-	}
-	if (h > kExactMaxHist) {
-		if (lane == 0)
-			d.exact_u[i] = -1.0;
-		return;
-	}
-	// lane t stages update t (its remainder outside E and factors), lane a atom a's pair
-	uint32_t rest = 0;
-	double fin0 = 1.0, fout0 = 1.0;
-	if (lane < h) {
-		const ExactRec r = lane == d.hist_new_at ? d.hist_new : d.hist[lane];
-		rest = r.e & ~e;
-		fin0 = r.fin;
-		fout0 = r.fout;
-	}
-	const double pv = lane < atoms ? d.pairs[2 * lane] : 1.0;
-	const double qv = lane < atoms ? d.pairs[2 * lane + 1] : 1.0;
-	// Q(E) times the always-contained factors (updates whose pool lies inside E see it
-	// contained), in the host's order
-	double lead = 1.0;
-	for (int at = 0; at < atoms; ++at) {
-		const double x = lane_d(qv, at);
-		if ((e >> at) & 1u)
-			lead *= x;
-	}
-	// The other updates group by their remaining atoms, one pool per distinct remainder in
-	// order of first occurrence, with the factors of the group multiplied in update order.
-	bool first = lane < h && rest != 0u;
-	double fin = fin0, fout = fout0;
-	for (int t = 0; t < h; ++t) {
-		const uint32_t rt = lane_u(rest, t);
-		const double fi = lane_d(fin0, t), fo = lane_d(fout0, t);
-		if (rt == 0u)
-			lead *= fi;
-		if (rt == rest) {
-			if (t < lane)
-				first = false;
-			else if (t > lane) {
-				fin *= fi;
-				fout *= fo;
+		// lane t stages update t (its remainder outside E and factors), lane a atom a's pair
+		uint32_t rest = 0;
+		double fin0 = 1.0, fout0 = 1.0;
+		if (lane < h) {
+			const ExactRec r = lane == d.hist_new_at ? d.hist_new : d.hist[lane];
+			rest = r.e & ~e;
+			fin0 = r.fin;
+			fout0 = r.fout;
+		}
+		const double pv = lane < atoms ? d.pairs[2 * lane] : 1.0;
+		const double qv = lane < atoms ? d.pairs[2 * lane + 1] : 1.0;
+		// Q(E) times the always-contained factors (updates whose pool lies inside E see it
+		// contained), in the host's order
+		double lead = 1.0;
+		for (int at = 0; at < atoms; ++at) {
+			const double x = lane_d(qv, at);
+			if ((e >> at) & 1u)
+				lead *= x;
+		}
+		// The other updates group by their remaining atoms, one pool per distinct remainder in
+		// order of first occurrence, with the factors of the group multiplied in update order.
+		bool first = lane < h && rest != 0u;
+		double fin = fin0, fout = fout0;
+		for (int t = 0; t < h; ++t) {
+			const uint32_t rt = lane_u(rest, t);
+			const double fi = lane_d(fin0, t), fo = lane_d(fout0, t);
+			if (rt == 0u)
+				lead *= fi;
+			if (rt == rest) {
+				if (t < lane)
+					first = false;
+				else if (t > lane) {
+					fin *= fi;
+					fout *= fo;
+				}
 			}
 		}
-	}
-	const unsigned long long firsts = __ballot(first);
-	const int c = __popcll(firsts);
-	if (c > kExactMaxPools) {
+		const unsigned long long firsts = __ballot(first);
+		const int c = __popcll(firsts);
+		if (c > kExactMaxPools) {
+			if (lane == 0)
+				d.exact_u[i] = -1.0;
+			continue;
+		}
+		if (first) {
+			const int j = __popcll(firsts & ((1ull << lane) - 1ull));
+			s_pool[j] = rest;
+			s_fin[j] = fin;
+			s_fout[j] = fout;
+		}
+		__syncthreads();
+		const uint32_t pool = lane < c ? s_pool[lane] : 0u;
+		const double pfin = lane < c ? s_fin[lane] : 1.0, pfout = lane < c ? s_fout[lane] : 1.0;
+		uint32_t relevant = 0;
+		for (int j = 0; j < c; ++j)
+			relevant |= lane_u(pool, j);
+		double w;
+		switch (c <= 6 ? 0 : c - 6) {
+		case 0: w = exact_dense_regs<0>(c, relevant, pool, pfin, pfout, pv, qv); break;
+		case 1: w = exact_dense_regs<1>(c, relevant, pool, pfin, pfout, pv, qv); break;
+		case 2: w = exact_dense_regs<2>(c, relevant, pool, pfin, pfout, pv, qv); break;
+		case 3: w = exact_dense_regs<3>(c, relevant, pool, pfin, pfout, pv, qv); break;
+		default: w = exact_dense_regs<4>(c, relevant, pool, pfin, pfout, pv, qv); break;
+		}
 		if (lane == 0)
-			d.exact_u[i] = -1.0;
-		return;
+			d.exact_u[i] = lead * w;
+		__syncthreads(); // the next entry restages the pools
 	}
-	if (first) {
-		const int j = __popcll(firsts & ((1ull << lane) - 1ull));
-		s_pool[j] = rest;
-		s_fin[j] = fin;
-		s_fout[j] = fout;
-	}
-	__syncthreads();
-	const uint32_t pool = lane < c ? s_pool[lane] : 0u;
-	const double pfin = lane < c ? s_fin[lane] : 1.0, pfout = lane < c ? s_fout[lane] : 1.0;
-	uint32_t relevant = 0;
-	for (int j = 0; j < c; ++j)
-		relevant |= lane_u(pool, j);
-	double w;
-	switch (c <= 6 ? 0 : c - 6) {
-	case 0: w = exact_dense_regs<0>(c, relevant, pool, pfin, pfout, pv, qv); break;
-	case 1: w = exact_dense_regs<1>(c, relevant, pool, pfin, pfout, pv, qv); break;
-	case 2: w = exact_dense_regs<2>(c, relevant, pool, pfin, pfout, pv, qv); break;
-	case 3: w = exact_dense_regs<3>(c, relevant, pool, pfin, pfout, pv, qv); break;
-	default: w = exact_dense_regs<4>(c, relevant, pool, pfin, pfout, pv, qv); break;
-	}
-	if (lane == 0)
-		d.exact_u[i] = lead * w;
 }
 
 // Result heads of a batch of lattices into one staging buffer (one copy to the host).
@@ -3653,10 +3660,11 @@ void launch_band_collect(const Descs &descs, int count, unsigned entries, int n,
 
 
 
-void launch_exact_band(const Descs &descs, int count, int atoms, hipStream_t stream)
+void launch_exact_band(const Descs &descs, int count, int atoms, hipStream_t stream, unsigned per_lattice)
 {
+	per_lattice = std::max(1u, std::min(per_lattice, (unsigned)kExactBand));
 	launch_checked("k_exact_band", [&] {
-		hipLaunchKernelGGL(k_exact_band, dim3(kExactBand, count), dim3(64), 0, stream, descs, atoms);
+		hipLaunchKernelGGL(k_exact_band, dim3(per_lattice, count), dim3(64), 0, stream, descs, atoms);
 	});
 }
 

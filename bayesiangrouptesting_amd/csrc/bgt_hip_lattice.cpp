@@ -856,20 +856,21 @@ void DeviceLattice::fetch_head()
 	unpack_head(wide ? kBandHeadWide : kBandHead, wide ? kHeadBandWide : kHeadBand);
 }
 
-void DeviceLattice::load_head(const unsigned char *head)
+void DeviceLattice::load_head(const unsigned char *head, bool exact_staged)
 {
 	HostBox *box = static_cast<HostBox *>(host_box_);
 	std::memcpy(box->head, head, kHeadBytes);
-	exact_valid_ = exact_pending_ = false;
+	exact_valid_ = false;
+	exact_pending_ = exact_staged; // the band's exact masses stay on the device until a near tie wants them
 	unpack_head(kBandHead, kHeadBand);
 }
 
-void DeviceLattice::launch_exact(const Descs &ds_in)
+bool DeviceLattice::stage_exact(LatDesc &d, bool sync_upload)
 {
 	exact_pending_ = false;
 	if (exact_dev_ == nullptr || dilution_ || k_ != 1 || !pairs_own_ || xn_ != n_ || !exact_.usable() ||
 		exact_.updates() > kExactMaxHist)
-		return;
+		return false;
 	HostBox *box = static_cast<HostBox *>(host_box_);
 	const int n = exact_.updates();
 	int new_at = -1;
@@ -879,25 +880,35 @@ void DeviceLattice::launch_exact(const Descs &ds_in)
 			exact_.record(t, rec.e, rec.fin, rec.fout);
 			rec.pad = 0;
 		}
-		if (n - hist_uploaded_ == 1)
+		if (n - hist_uploaded_ == 1) {
 			new_at = n - 1; // one new record: a kernel argument, no copy ahead of the kernel
-		else
+		} else {
 			BGT_HIP_CHECK(hipMemcpyAsync(hist_dev_ + hist_uploaded_, box->hist + hist_uploaded_,
 										 sizeof(ExactRec) * (n - hist_uploaded_), hipMemcpyHostToDevice, stream_));
+			if (sync_upload)
+				sync(); // the kernel runs on another stream
+		}
 		hist_uploaded_ = n;
 	}
-	Descs ds = ds_in;
-	ds.one.hist = hist_dev_;
-	ds.one.hist_len = n;
-	ds.one.hist_new_at = new_at;
+	d.hist = hist_dev_;
+	d.hist_len = n;
+	d.hist_new_at = new_at;
 	if (new_at >= 0)
-		ds.one.hist_new = box->hist[new_at];
-	ds.one.pairs = pairs_;
-	ds.one.exact_u = exact_dev_;
+		d.hist_new = box->hist[new_at];
+	d.pairs = pairs_;
+	d.exact_u = exact_dev_;
+	exact_pending_ = true;
+	return true;
+}
+
+void DeviceLattice::launch_exact(const Descs &ds_in)
+{
+	Descs ds = ds_in;
+	if (!stage_exact(ds.one, false))
+		return;
 	clock_.begin(stream_);
 	launch_exact_band(ds, 1, atoms_, stream_);
 	clock_.end(stream_, "exact_band");
-	exact_pending_ = true;
 }
 
 void DeviceLattice::unpack_head(int band_entries, size_t band_offset)
@@ -1253,11 +1264,23 @@ void DeviceLattice::rescore_band(bgt_hip_selection &out, int sel, const SelResul
 	std::vector<Cand> cands;
 	if (complete) {
 		const unsigned inhead = std::min<unsigned>(count, head_band_);
+		bool wait = false;
 		if (count > inhead) {
 			BGT_HIP_CHECK(hipMemcpyAsync(band_host_ + inhead, band_ + inhead, sizeof(Cand) * (count - inhead),
 										 hipMemcpyDeviceToHost, stream_));
-			sync();
+			wait = true;
 		}
+		if (exact_pending_ && !exact_valid_ && count > 0) {
+			// a batched step left the band's exact masses on the device (k_exact_band)
+			BGT_HIP_CHECK(hipMemcpyAsync(box->exact_u, exact_dev_,
+										 sizeof(double) * std::min<unsigned>(count, kExactBand), hipMemcpyDeviceToHost,
+										 stream_));
+			exact_valid_ = true;
+			wait = true;
+		}
+		exact_pending_ = false;
+		if (wait)
+			sync();
 		cands.assign(band_host_, band_host_ + count);
 		if (exact_valid_) {
 			// band entry i carries i + 1 in the upper flag bits: its exact mass came back

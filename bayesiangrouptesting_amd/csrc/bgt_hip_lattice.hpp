@@ -126,8 +126,15 @@ public:
 	// error band of the selection objective (device objectives closer than this may be
 	// misordered)
 	double band_width(int selector) const;
-	// take a result head gathered by a batched step (kHeadBytes of the arena's start)
-	void load_head(const unsigned char *head);
+	// take a result head gathered by a batched step (kHeadBytes of the arena's start);
+	// exact_staged: the step ran k_exact_band for this lattice (stage_exact)
+	void load_head(const unsigned char *head, bool exact_staged = false);
+	// Binary non-dilution lattices with a dense exact program: the descriptor fields of
+	// k_exact_band (history, prior pairs, output), the history's new records uploaded (a
+	// single new record travels in the descriptor; more are copied on this lattice's stream,
+	// then synchronised when sync_upload, for kernels on another stream). False when the
+	// device program does not apply.
+	bool stage_exact(LatDesc &d, bool sync_upload);
 	// sticky classification from the masses of the last head (lattice_base.cpp:246-264)
 	void classify_from_head(double threshold_up, double threshold_lo, double *masses_out);
 	// selection result of the last head
