@@ -117,26 +117,51 @@ size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 // objective and variant count, and a shape group beyond that (one-pass BBPA tables, brute
 // force, the persistent pass 2), where templates depend on the shape.
 enum Regime : int { kShape = 0, kSmall = 1, kOneTile = 2, kTiles = 3, kAll = 4 };
-using GroupKey = std::tuple<int, int, int, int, int>; // regime, storage, dilution | objective, subjects, variants
+// regime, storage, dilution | objective, subjects | size class, variants, fused (0: no; 1 + dilution:
+// the update fused into the selection's first pass, whose likelihood form must be uniform)
+using GroupKey = std::tuple<int, int, int, int, int, int>;
 
 GroupKey shape_key(const DeviceLattice &l)
 {
-	return {kShape, static_cast<int>(l.storage()), l.dilution() ? 1 : 0, l.subjects(), l.variants()};
+	return {kShape, static_cast<int>(l.storage()), l.dilution() ? 1 : 0, l.subjects(), l.variants(), 0};
 }
 
-GroupKey all_key(const DeviceLattice &l) { return {kAll, static_cast<int>(l.storage()), 0, 0, 0}; }
+// Updates and marginals: one ragged group per storage and size class of three atoms. A
+// ragged grid is sized for its largest lattice and the workgroups beyond a lattice's own
+// count exit at once; a single group for all sizes launched hundreds of thousands of those
+// per step (a 2^26-state session beside 2^12-state ones).
+GroupKey all_key(const DeviceLattice &l, bool by_size = true)
+{
+	const int a = l.atoms();
+	return {kAll, static_cast<int>(l.storage()), 0, by_size ? (a <= 14 ? 0 : (a - 12) / 3) : 0, 0, 0};
+}
 
 int objective_of(int sel, int k) { return sel == kSelBha ? kObjHalving : (k == 1 ? kObjBinaryBbpa : kObjGather); }
 
-GroupKey select_key(const DeviceLattice &l, int selector)
+GroupKey select_key(const DeviceLattice &l, int selector, bool fused)
 {
 	const int sel = DeviceLattice::resolve_selector(selector, l.variants());
 	const int k = l.variants(), atoms = l.atoms();
 	const int objective = objective_of(sel, k);
-	if (sel == kSelBrute || (objective == kObjGather && bbpa_hist_path(l.subjects(), k)) || atoms > kRaggedMaxAtoms)
-		return shape_key(l);
+	const int f = fused ? 1 + (l.dilution() ? 1 : 0) : 0;
+	if (sel == kSelBrute || (objective == kObjGather && bbpa_hist_path(l.subjects(), k)) || atoms > kRaggedMaxAtoms) {
+		GroupKey key = shape_key(l);
+		std::get<5>(key) = f;
+		return key;
+	}
 	const int regime = atoms <= kSmallBits ? kSmall : (atoms == kTileBits ? kOneTile : kTiles);
-	return {regime, static_cast<int>(l.storage()), objective, 0, k};
+	// the tiled regime (16..20 atoms) in two size classes, for the same reason as all_key
+	return {regime, static_cast<int>(l.storage()), objective, regime == kTiles && atoms >= 18 ? 1 : 0, k, f};
+}
+
+// A step that updates, classifies and selects a lattice by auto_select runs as its fused
+// round does (DeviceLattice::round) wherever that round fuses, so that the two give the same
+// bits: more than one tile (the update in the first pass of the transform, or in the one-pass
+// Op-BBPA histogram; the atom masses from the selection's own outputs).
+bool fuses(const BatchItem &it, int selector)
+{
+	return it.update && it.select && it.classify && it.source == nullptr && selector == kSelAuto &&
+		   it.lat->atoms() > kTileBits;
 }
 
 } // namespace
@@ -193,8 +218,34 @@ void batch_step(std::vector<BatchItem> &items, int selector, double up, double l
 	std::vector<LatDesc> du(count), ds(count);
 	std::vector<const double *> liks(count, nullptr);
 	std::vector<char> staged(count, 0); // k_exact_band runs for these lattices' bands
+	std::vector<char> fused(count, 0);
 	for (size_t i = 0; i < count; ++i) {
 		auto &it = items[i];
+		if (fuses(it, selector)) {
+			// the round's descriptors: the pre-update state with the new experiment's factors
+			// (the selection's first pass applies the update), the band width and the exact
+			// re-decision's history after it
+			fused[i] = 1;
+			du[i] = it.lat->desc();
+			liks[i] = it.lat->lik_host();
+			it.lat->commit_update();
+			it.lat->ensure_z();
+			it.lat->prepare_gather();
+			const LatDesc post = it.lat->desc();
+			ds[i] = du[i];
+			ds[i].band_width = it.lat->band_width(DeviceLattice::resolve_selector(selector, it.lat->variants()));
+			ds[i].gh_len = post.gh_len;
+			ds[i].gx_n = post.gx_n;
+			ds[i].gx_keep = post.gx_keep;
+			std::memcpy(ds[i].gh_e, post.gh_e, sizeof(post.gh_e));
+			std::memcpy(ds[i].gh_r, post.gh_r, sizeof(post.gh_r));
+			ds[i].gh_pairs = post.gh_pairs;
+			ds[i].gh_dil = post.gh_dil;
+			ds[i].gexact = post.gexact;
+			if (it.lat->storage() == Storage::fp32c)
+				staged[i] = it.lat->stage_exact(ds[i], true) ? 1 : 0;
+			continue;
+		}
 		if (it.update) {
 			du[i] = it.lat->desc();
 			if (it.source)
@@ -231,14 +282,14 @@ void batch_step(std::vector<BatchItem> &items, int selector, double up, double l
 	};
 	for (size_t i = 0; i < count; ++i) {
 		const DeviceLattice &lat = *items[i].lat;
-		if (items[i].update)
+		if (items[i].update && !fused[i])
 			join(gu, all_key(lat), i);
-		if (items[i].classify)
+		if (items[i].classify && !fused[i])
 			join(gc, all_key(lat), i);
 		if (items[i].select)
-			join(gs, select_key(lat, selector), i);
+			join(gs, select_key(lat, selector, fused[i] != 0), i);
 		if (staged[i])
-			join(gx, all_key(lat), i);
+			join(gx, all_key(lat, false), i);
 	}
 	size_t off = 0;
 	const size_t o_lik = off;
@@ -261,6 +312,8 @@ void batch_step(std::vector<BatchItem> &items, int selector, double up, double l
 		if (items[i].update) {
 			std::memcpy(h + o_lik + sizeof(double) * kLikCapacity * i, liks[i], sizeof(double) * kLikCapacity);
 			du[i].lik = reinterpret_cast<double *>(d + o_lik) + i * kLikCapacity;
+			if (fused[i])
+				ds[i].lik = du[i].lik;
 		}
 	auto fill = [&](std::map<GroupKey, Group> &gm, const std::vector<LatDesc> &src) {
 		for (auto &g : gm) {
@@ -314,10 +367,16 @@ void batch_step(std::vector<BatchItem> &items, int selector, double up, double l
 			const double target = sel == kSelBha ? 0.5 : 1.0 / double(1 << k);
 			const int atoms = g.second.max_atoms;
 			const int subjects = g.second.max_subjects;
+			// fused (as DeviceLattice::round): the group's update runs in the first pass, the
+			// outputs are scaled by the new 1/D; its likelihood form is the group's (dilution
+			// is part of the key)
+			const int fkey = std::get<5>(g.first);
+			const bool fused = fkey != 0;
+			const int dil = fused ? fkey - 1 : (lat.dilution() ? 1 : 0);
 			if (objective == kObjGather && bbpa_hist_path(lat.subjects(), k)) {
 				// as DeviceLattice::select: the feature tables from one histogram pass
-				launch_bbpa_hist(dg, cnt, lat.subjects(), k, lat.storage(), false, s, false);
-				launch_bbpa_finish(dg, cnt, k, false, target, s);
+				launch_bbpa_hist(dg, cnt, lat.subjects(), k, lat.storage(), fused, s, false);
+				launch_bbpa_finish(dg, cnt, k, fused, target, s);
 				continue;
 			}
 			unsigned entries = 0;
@@ -326,14 +385,17 @@ void batch_step(std::vector<BatchItem> &items, int selector, double up, double l
 			} else if (atoms == kTileBits) {
 				entries = launch_zeta_final(dg, cnt, atoms, k, lat.storage(), objective, true, false, target, s);
 			} else {
-				launch_zeta_pass1(dg, cnt, atoms, lat.storage(), false, k, lat.dilution(), objective, s);
-				entries = launch_zeta_final(dg, cnt, atoms, k, lat.storage(), objective, false, false, target, s);
+				// a persistent pass 2 sums the fused update's pass-1 partials itself (shape groups)
+				const bool fold = fused && !ragged && persistent_pass2(atoms, objective, false);
+				launch_zeta_pass1(dg, cnt, atoms, lat.storage(), fused, k, dil, objective, s, fold);
+				entries = launch_zeta_final(dg, cnt, atoms, k, lat.storage(), objective, false, fused, target, s,
+											fold ? (1u << (atoms - kTileBits)) : 0u);
 			}
 			if (objective == kObjGather) {
-				launch_bbpa_score(dg, cnt, subjects, k, target, s);
-				launch_bbpa_band(dg, cnt, subjects, k, target, s);
+				launch_bbpa_score(dg, cnt, subjects, k, target, s, fused);
+				launch_bbpa_band(dg, cnt, subjects, k, target, s, fused);
 			} else if (k == 1 && atoms > kSmallBits)
-				launch_band_collect(dg, cnt, entries, atoms, k, lat.storage(), objective, atoms == kTileBits, false,
+				launch_band_collect(dg, cnt, entries, atoms, k, lat.storage(), objective, atoms == kTileBits, fused,
 									target, s);
 		}
 		// 4. the exact masses of the staged bands (lattices whose top two are no near tie return
@@ -357,7 +419,8 @@ void batch_step(std::vector<BatchItem> &items, int selector, double up, double l
 			for (auto &g : *gm) {
 				const DeviceLattice &lat = *items[g.second.idx[0]].lat;
 				sig.insert(sig.end(), {kind, std::get<0>(g.first), std::get<1>(g.first), std::get<2>(g.first),
-									   std::get<3>(g.first), std::get<4>(g.first), (long long)g.second.idx.size(),
+									   std::get<3>(g.first), std::get<4>(g.first), std::get<5>(g.first),
+									   (long long)g.second.idx.size(),
 									   (long long)g.second.offset, g.second.max_atoms, g.second.max_subjects,
 									   three_pass_form(lat.atoms(), lat.storage()) ? 1 : 0});
 			}

@@ -299,8 +299,8 @@ void launch_band_collect(const Descs &descs, int count, unsigned entries, int n,
 // copy the kHeadBytes head of each of `count` lattices (arena base = LatDesc::scal) into
 // out[i * kHeadBytes]
 void launch_gather_heads(const LatDesc *many, int count, unsigned char *out, hipStream_t stream);
-void launch_bbpa_score(const Descs &descs, int count, int n, int k, double target,
-					   hipStream_t stream);
+void launch_bbpa_score(const Descs &descs, int count, int n, int k, double target, hipStream_t stream,
+					   bool scaled = false);
 // after launch_bbpa_score: the candidates within d.band_width of the best into d.gband
 // (lattices of several variants that collect a band; the others return at once), then, where
 // the top two are within the band (a near tie) and the band is complete, their exact FP64
@@ -309,7 +309,9 @@ void launch_bbpa_score(const Descs &descs, int count, int n, int k, double targe
 // is a product over atoms and each likelihood factor depends on one block,
 // replicated_lattice.cpp:8-28), so a feature mass F_B(E) is the product over v in B of the
 // binary mass M_v(E) = U_v(E) / U_v({}), U_v summed over the 2^(n - |E|) states of block v.
-void launch_bbpa_band(const Descs &descs, int count, int n, int k, double target, hipStream_t stream);
+// scaled (score and band): tables from a final pass scaled by the pending 1/D (fused update)
+void launch_bbpa_band(const Descs &descs, int count, int n, int k, double target, hipStream_t stream,
+					  bool scaled = false);
 // Op-BBPA feature tables of k >= 2 lattices in one pass over the posterior (bbpa.cpp:237-313)
 // where each lane can hold its entries of every table (n >= 6, (2^k - 1) 2^(n-6) <= 64, more
 // than 14 atoms): d.ftab, the total and the atom masses, ready for launch_bbpa_score. fused:

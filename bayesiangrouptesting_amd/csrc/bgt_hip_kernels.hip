@@ -1794,7 +1794,9 @@ __device__ __forceinline__ double bbpa_objective(const double *fm, uint32_t ec, 
 	return score;
 }
 
-__global__ __launch_bounds__(256) void k_bbpa_score(Descs descs, int ns, int k, double target)
+// scaled: the tables come from a final pass that scaled the transform by the pending 1/D (a
+// fused update), so the total is scaled the same way (the final pass's own total)
+__global__ __launch_bounds__(256) void k_bbpa_score(Descs descs, int ns, int k, double target, int scaled)
 {
 	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
 	// ragged: lattices of one variant count (one target), each with its own subjects
@@ -1805,7 +1807,7 @@ __global__ __launch_bounds__(256) void k_bbpa_score(Descs descs, int ns, int k, 
 	}
 	__shared__ Top2 red2[16];
 	const uint32_t ec = 1u << ns;
-	const double total = d.scal[kScalTotal];
+	const double total = scaled ? d.scal[kScalTotal] * d.scal[d.inv_out] : d.scal[kScalTotal];
 	Top2 t;
 	top2_init(t);
 	const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1819,7 +1821,7 @@ __global__ __launch_bounds__(256) void k_bbpa_score(Descs descs, int ns, int k, 
 // tables, so the same value) lies within band_width of the best is appended to d.gband,
 // one atomic per wave on counter[0] (cleared by the merge). Appends past band_cap are
 // counted, not stored: counter[0] > band_cap reports an incomplete band.
-__global__ __launch_bounds__(256) void k_bbpa_band(Descs descs, int ns, int k, double target)
+__global__ __launch_bounds__(256) void k_bbpa_band(Descs descs, int ns, int k, double target, int scaled)
 {
 	const LatDesc &d = descs.many ? descs.many[blockIdx.y] : descs.one;
 	if (descs.ragged) {
@@ -1833,8 +1835,9 @@ __global__ __launch_bounds__(256) void k_bbpa_band(Descs descs, int ns, int k, d
 	const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
 	const double lim = d.result->d1 + d.band_width;
 	double s = INFINITY;
+	const double total = scaled ? d.scal[kScalTotal] * d.scal[d.inv_out] : d.scal[kScalTotal];
 	if (e < ec && (e & ~d.active) == 0)
-		s = bbpa_objective(d.ftab, ec, e, k, d.scal[kScalTotal], target);
+		s = bbpa_objective(d.ftab, ec, e, k, total, target);
 	const bool in = s <= lim;
 	const unsigned long long mask = __ballot(in);
 	if (mask == 0ull)
@@ -3681,11 +3684,11 @@ void launch_exact_enum(const EnumJob &job, hipStream_t stream)
 	});
 }
 
-void launch_bbpa_score(const Descs &descs, int count, int n, int k, double target, hipStream_t stream)
+void launch_bbpa_score(const Descs &descs, int count, int n, int k, double target, hipStream_t stream, bool scaled)
 {
 	const dim3 grid(grid_for(1ull << n, 256, 1u << 20), count);
 	launch_checked("k_bbpa_score", [&] {
-		hipLaunchKernelGGL(k_bbpa_score, grid, dim3(256), 0, stream, descs, n, k, target);
+		hipLaunchKernelGGL(k_bbpa_score, grid, dim3(256), 0, stream, descs, n, k, target, scaled ? 1 : 0);
 		hipLaunchKernelGGL(k_finalize_top2, dim3(1, count), dim3(1024), 0, stream, descs, grid.x, (int)kCountScore);
 	});
 }
@@ -3700,11 +3703,11 @@ void launch_bbpa_finish(const Descs &descs, int count, int k, bool fused, double
 	});
 }
 
-void launch_bbpa_band(const Descs &descs, int count, int n, int k, double target, hipStream_t stream)
+void launch_bbpa_band(const Descs &descs, int count, int n, int k, double target, hipStream_t stream, bool scaled)
 {
 	const dim3 grid(score_groups(n), count);
 	launch_checked("k_bbpa_band", [&] {
-		hipLaunchKernelGGL(k_bbpa_band, grid, dim3(256), 0, stream, descs, n, k, target);
+		hipLaunchKernelGGL(k_bbpa_band, grid, dim3(256), 0, stream, descs, n, k, target, scaled ? 1 : 0);
 		hipLaunchKernelGGL(k_gather_exact, dim3(kGatherGroups, count), dim3(256), 0, stream, descs, k, target);
 		hipLaunchKernelGGL(k_gather_pick, dim3(1, count), dim3(1024), 0, stream, descs);
 	});

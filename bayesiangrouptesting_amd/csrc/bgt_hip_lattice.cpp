@@ -1162,16 +1162,17 @@ void DeviceLattice::collect_band(const Descs &ds, unsigned entries, int objectiv
 	clock_.end(stream_, "band_collect");
 }
 
-void DeviceLattice::collect_gather_band(double target)
+void DeviceLattice::collect_gather_band(double target, const Descs *fused)
 {
 	// lattices of several variants: the band, and its exact re-decision where it applies
+	// (fused: the descriptors of a fused round, whose tables are scaled by the pending 1/D)
 	if (k_ < 2 || band_ == nullptr)
 		return;
 	prepare_gather();
-	Descs ds = single();
+	Descs ds = fused ? *fused : single();
 	ds.one.band_width = band_width(kSelBbpa);
 	clock_.begin(stream_);
-	launch_bbpa_band(ds, 1, n_, k_, target, stream_);
+	launch_bbpa_band(ds, 1, n_, k_, target, stream_, fused != nullptr);
 	clock_.end(stream_, "bbpa_band");
 }
 
@@ -1803,12 +1804,48 @@ bgt_hip_selection DeviceLattice::round(uint32_t experiment, uint32_t response, c
 		// multinomial lattices: Op-BBPA (auto_select's choice, simulation_helpers.cpp:40-45);
 		// with tables on chip, the update and the histogram of the updated posterior are one
 		// pass, and the atom masses come with the single-variant tables
-		if (!bbpa_hist_path(n_, k_)) {
+		if (!bbpa_hist_path(n_, k_) && atoms_ <= kTileBits) {
 			update(experiment, response, dil);
 			classify(up, lo);
 			if (masses_out)
 				fetch_masses(masses_out);
 			return select(kSelBbpa);
+		}
+		if (!bbpa_hist_path(n_, k_)) {
+			// the transform path: the update fused into pass 1 (the general likelihood), then
+			// the feature tables and the atom masses of the updated posterior from the final
+			// pass, scaled by the new 1/D (as the binary round below)
+			const double target = 1.0 / double(1 << k_);
+			prepare_update(experiment, response, dil);
+			ensure_z();
+			Descs ds = single(); // pre-update descriptors (the final pass applies inv_out)
+			clock_.begin(stream_, true);
+			launch_zeta_pass1(ds, 1, atoms_, storage_, true, k_, dilution_, kObjGather, stream_, false);
+			clock_.end(stream_, "round_pass1");
+			commit_update();
+			prepare_gather();
+			{
+				const LatDesc post = desc();
+				ds.one.band_width = band_width(kSelBbpa);
+				ds.one.gh_len = post.gh_len;
+				ds.one.gx_n = post.gx_n;
+				ds.one.gx_keep = post.gx_keep;
+				std::memcpy(ds.one.gh_e, post.gh_e, sizeof(post.gh_e));
+				std::memcpy(ds.one.gh_r, post.gh_r, sizeof(post.gh_r));
+				ds.one.gh_pairs = post.gh_pairs;
+				ds.one.gh_dil = post.gh_dil;
+				ds.one.gexact = post.gexact;
+			}
+			clock_.begin(stream_);
+			launch_zeta_final(ds, 1, atoms_, k_, storage_, kObjGather, false, true, target, stream_);
+			clock_.end(stream_, "round_final");
+			clock_.begin(stream_);
+			launch_bbpa_score(ds, 1, n_, k_, target, stream_, true);
+			clock_.end(stream_, "bbpa_score");
+			collect_gather_band(target, &ds);
+			fetch_head();
+			classify_from_head(up, lo, masses_out);
+			return finish_selection(kSelBbpa, static_cast<HostBox *>(host_box_)->result, true);
 		}
 		prepare_update(experiment, response, dil);
 		const Descs ds = single(); // the new experiment's factors, the pre-update 1/D

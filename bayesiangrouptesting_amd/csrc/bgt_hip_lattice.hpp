@@ -177,7 +177,7 @@ private:
 	void rescore_band(bgt_hip_selection &out, int selector, const SelResult &r);
 	// lattices of several variants: the Op-BBPA band after k_bbpa_score (k_bbpa_band), and its
 	// exact re-decision from exact feature masses by the device enumeration
-	void collect_gather_band(double target);
+	void collect_gather_band(double target, const Descs *fused = nullptr);
 	double gather_objective_bound() const;
 	void copy_gather_history(const DeviceLattice &parent);
 	// exact masses of the band on the device (binary non-dilution), after the band kernels

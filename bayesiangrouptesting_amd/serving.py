@@ -350,15 +350,18 @@ class ServingEngine:
 
 def serve_alone(spec: SessionSpec, precision: api.Precision = api.Precision.float64, max_tests: int = 20,
                 up: float = 0.01, lo: float = 0.01, shrink: bool = False) -> SessionResult:
-    """The same session through per-lattice calls (update, classify, select), one at a time,
-    in the order of the engine's steps (a projection, then a selection on the new lattice)."""
+    """The same session through per-lattice calls, one at a time, in the order of the
+    engine's steps: the fused round (update, classification and the next selection) after a
+    test, classification and selection otherwise (the first step, and after a projection)."""
     s = _Live(spec, precision, 0.0, shrink)
     while True:
         if s.pending is not None:
             r = s.sampler.draw(s.pending_original)
-            s.lattice.update(s.pending, r, s.table)
+            sel = s.lattice.round(s.pending, r, up, lo, s.table)
             s.tests += 1
-        s.lattice.update_classification(up, lo)
+        else:
+            s.lattice.update_classification(up, lo)
+            sel = None
         s.absorb(s.lattice.positive_classification_mask(), s.lattice.negative_classification_mask())
         if s.classified() or s.tests >= max_tests:
             break
@@ -366,7 +369,8 @@ def serve_alone(spec: SessionSpec, precision: api.Precision = api.Precision.floa
         if plan is not None:
             s.apply(plan)
             continue
-        sel = s.lattice.select(api.SelectorType.auto_select)
+        if sel is None:
+            sel = s.lattice.select(api.SelectorType.auto_select)
         s.pending = sel.experiment
         s.pending_original = s.original(sel.experiment)
         s.experiments.append(s.pending_original)

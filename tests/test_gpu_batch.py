@@ -1,8 +1,10 @@
 """Batched steps (bgt_hip_batch_step / bgt_hip_batch_round) against per-lattice calls.
 
 Bar: bit-identical. Each lattice in a batch runs the same kernels with the same launch
-shapes as the per-lattice update / classify / select calls, so selections, objectives,
-classification masks and the stored posteriors must match exactly, whatever the mix."""
+shapes as the per-lattice calls: the fused round (update + classification + auto_select
+selection, DeviceLattice::round) for a step that updates, and update / classify / select
+otherwise. Selections, objectives, classification masks and the stored posteriors must match
+exactly, whatever the mix."""
 import numpy as np
 import pytest
 
@@ -40,9 +42,10 @@ def test_batch_round_equals_per_lattice_calls():
         sels, masks = bgt.batch_round(b, pending, resps, bgt.SelectorType.auto_select, 0.01, 0.01, tables)
         for i, (lat, e, r, t) in enumerate(zip(a, pending, resps, tables)):
             if r is not None:
-                lat.update(e, r, t)
-            lat.update_classification(0.01, 0.01)
-            s = lat.select(bgt.SelectorType.auto_select)
+                s = lat.round(e, r, 0.01, 0.01, t)
+            else:
+                lat.update_classification(0.01, 0.01)
+                s = lat.select(bgt.SelectorType.auto_select)
             assert (s.experiment, s.objective, s.runner_up, s.near_tie, s.redecided) == \
                 (sels[i].experiment, sels[i].objective, sels[i].runner_up, sels[i].near_tie, sels[i].redecided), (step, i)
             assert masks[i] == (lat.positive_classification_mask(), lat.negative_classification_mask()), (step, i)
