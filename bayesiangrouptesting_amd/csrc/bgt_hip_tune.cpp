@@ -195,20 +195,34 @@ TuneResult tune(const TuneKey &key_in, bool force)
 			return r;
 		}
 	}
-	// measure: the selection of a prior lattice of this shape under each pair of forms (the
-	// band work is common to all; pass 1's grid and the high-bit transform differ)
+	// measure: fused rounds (update, classification, next selection: the work of a session's
+	// test) on a lattice of this shape under each pair of forms; the band work is common to
+	// all, pass 1's grid and the high-bit transform differ. The thresholds classify nothing,
+	// the responses alternate, each round tests the previous round's pick.
 	const int n = key.subjects;
 	std::vector<double> prior(n);
 	for (int a = 0; a < n; ++a)
 		prior[a] = 0.005 + 0.045 * (a + 0.5) / n;
+	std::vector<double> dil;
+	if (key.dilution) {
+		// DilutionTable(n, 0.05, 0.9) (dilution.cpp:44-67): any valid table times the same
+		dil.assign((size_t)n * (n + 1), 0.0);
+		for (int s = 1; s <= n; ++s) {
+			dil[(size_t)(s - 1) * (n + 1)] = 0.05;
+			for (int m = 1; m <= s; ++m)
+				dil[(size_t)(s - 1) * (n + 1) + m] = 1.0 - 0.05 * m / ((s - m) * 0.9 + m);
+		}
+	}
 	auto lat = DeviceLattice::make(key.dilution, n, 1, prior.data(), key.storage);
+	uint32_t e = lat->select(key.selector).experiment;
 	int saved2, saved1;
 	{
 		std::lock_guard<std::mutex> lock(g_mu);
 		saved2 = g_forced;
 		saved1 = g_forced1;
 	}
-	double best = 1e300;
+	double best = 1e300, best0 = 1e300;
+	int pass1_0 = 0;
 	for (int f2 = 0; f2 < kPass2Forms; ++f2) {
 		if (f2 == 2 && n < 28)
 			continue; // the plain pass 2 is form 0 there
@@ -216,11 +230,13 @@ TuneResult tune(const TuneKey &key_in, bool force)
 			set_pass2_form(f2);
 			set_pass1_form(f1);
 			double mn = 1e30;
-			for (int i = 0; i < 7; ++i) {
+			for (int i = 0; i < 9; ++i) {
 				const auto t0 = std::chrono::steady_clock::now();
-				lat->select(key.selector);
+				const bgt_hip_selection s = lat->round(e, (uint32_t)(i & 1), dil.empty() ? nullptr : dil.data(), 0.0,
+													   0.0, nullptr);
 				const double ms =
 					std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+				e = (s.experiment == 0u || s.experiment == 0xFFFFFFFFu) ? 1u : s.experiment;
 				if (i >= 2)
 					mn = std::min(mn, ms);
 			}
@@ -230,7 +246,17 @@ TuneResult tune(const TuneKey &key_in, bool force)
 				r.form = f2;
 				r.pass1 = f1;
 			}
+			if (f2 == 0 && mn < best0) {
+				best0 = mn;
+				pass1_0 = f1;
+			}
 		}
+	}
+	// form 0 (the split pass 2 where it applies) stays unless another is faster by more than
+	// the measurement's spread (3 %): near-equal forms would otherwise be picked by noise
+	if (r.form != 0 && best0 <= 1.03 * best) {
+		r.form = 0;
+		r.pass1 = pass1_0;
 	}
 	set_pass2_form(saved2);
 	set_pass1_form(saved1);

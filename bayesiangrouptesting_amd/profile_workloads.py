@@ -13,6 +13,10 @@
     python -m bayesiangrouptesting_amd.profile_workloads lookahead28 [fp32c|fp64] [calls]
         n = 28 binary lattice (the benchmark's priors) after 4 seeded updates: 5-step
         select_experiments calls
+    python -m bayesiangrouptesting_amd.profile_workloads serving [fp32c|fp64] [sessions] [full|shrink]
+        the serving mix (seed 5) with every session queued at once: batched steps until each
+        session is classified or has had 20 tests (shrink: sessions project their lattice when a
+        subject is fully classified)
 """
 from __future__ import annotations
 
@@ -101,6 +105,19 @@ def lookahead28(form: str = "fp64", calls: int = 2) -> None:
     lat.close()
 
 
+def serving_mix(form: str = "fp32c", sessions: int = 512, mode: str = "full") -> None:
+    import bayesiangrouptesting_amd as bgt
+    from bayesiangrouptesting_amd import serving
+
+    prec = bgt.Precision.float32_compensated if form == "fp32c" else bgt.Precision.float64
+    specs = serving.serving_mix(sessions, rate=300.0, seed=5)
+    eng = serving.ServingEngine(precision=prec, hbm_budget=96 << 30, shrink=(mode == "shrink"))
+    rep = eng.run(specs, realtime=False)
+    print(f"serving {form} {mode}: {rep['sessions']} sessions, {rep['sessions_per_s']:.1f} sessions/s, "
+          f"{rep['step_ms']['steps']} steps ({rep['step_ms']['total_s'] * 1e3:.1f} ms in steps), "
+          f"mean batch {rep['batch']['mean']:.1f}", flush=True)
+
+
 def copy(mib: int = 1024) -> None:
     hip = ctypes.CDLL("libamdhip64.so")
     nbytes = mib << 20
@@ -130,6 +147,9 @@ def main(argv) -> int:
         lookahead28(argv[1] if len(argv) > 1 else "fp64", int(argv[2]) if len(argv) > 2 else 2)
     elif what == "copy":
         copy(int(argv[1]) if len(argv) > 1 else 1024)
+    elif what == "serving":
+        serving_mix(argv[1] if len(argv) > 1 else "fp32c", int(argv[2]) if len(argv) > 2 else 512,
+                    argv[3] if len(argv) > 3 else "full")
     else:
         print(__doc__)
         return 2

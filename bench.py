@@ -186,7 +186,7 @@ PASS1_FORMS = ["persistent with prefetch", "one workgroup per tile"]
 
 def tune_forms(n) -> dict:
     """The autotuner's choice of the FP32C transform's launch forms at this size (measured once,
-    untimed): the high-bit form and the pass-1 grid, with the select time of every pair."""
+    untimed): the high-bit form and the pass-1 grid, with the round time of every pair."""
     from bayesiangrouptesting_amd import _native as nat
     p2, p1 = ctypes.c_int(-1), ctypes.c_int(-1)
     ms = (ctypes.c_double * 6)()
@@ -194,7 +194,7 @@ def tune_forms(n) -> dict:
     nat.check(nat.lib().bgt_hip_tune_forms(0, n, 1, nat.FP32C, 1, 1, ctypes.byref(p2), ctypes.byref(p1), ms,
                                            sid, 256))
     return {"spec": sid.value.decode(), "form": PASS2_FORMS[p2.value], "pass1": PASS1_FORMS[p1.value],
-            "select_ms": {f"{PASS2_FORMS[a]} / {PASS1_FORMS[b]}": ms[a * 2 + b] for a in range(3) for b in range(2)}}
+            "round_ms": {f"{PASS2_FORMS[a]} / {PASS1_FORMS[b]}": ms[a * 2 + b] for a in range(3) for b in range(2)}}
 
 
 def cpu_baseline(n, pi, steps, all_cores: bool = False):

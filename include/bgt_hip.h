@@ -326,19 +326,22 @@ int bgt_hip_tree_shrink(int enable, int *previous);
  * the split pass 2 on the blocked intermediate; 1: three passes; 2: two passes with the plain
  * pass 2 on the row-form intermediate, 28..30 atoms) and the grid of pass 1 (0: persistent
  * with a prefetch of the next tile; 1: one workgroup per tile); every other shape has one
- * form. Results are cached in memory and in a text file per key named as the reference's JIT
- * artefacts (kernel_spec_id, src/runtime/jit_compiler.cpp:13-38) in BGT_HIP_TUNE_DIR
- * (default: the temporary directory's bayesian_group_testing_hip_tune); force = 1 measures
- * again. form receives the high-bit form; ms_two_pass / ms_three_pass the measured selection
- * time of forms 0 and 1, each with its faster pass-1 grid (0 when taken from a cache or when
- * the shape has one form only); spec_id (may be NULL) the key. Run it before sessions of that
+ * form. Each pair is timed over fused rounds (update, classification and the next selection,
+ * the work of one test); form 0 is kept unless another is faster by more than 3 %, so that
+ * forms within the measurement's spread are not chosen by noise. Results are cached in memory
+ * and in a text file per key named as the reference's JIT artefacts (kernel_spec_id,
+ * src/runtime/jit_compiler.cpp:13-38) in BGT_HIP_TUNE_DIR (default: the temporary
+ * directory's bayesian_group_testing_hip_tune); force = 1 measures again. form receives the
+ * high-bit form; ms_two_pass / ms_three_pass the measured round time of forms 0 and 1, each
+ * with its faster pass-1 grid (0 when taken from a cache or when the shape has one form
+ * only); spec_id (may be NULL) the key. Run it before sessions of that
  * shape start: the choice applies process-wide. The reference's KernelRegistry /
  * JitCompiler have no device measurement. */
 int bgt_hip_tune(int use_dilution, int subjects, int variants, int precision, int selector, int force, int *form,
 				 double *ms_two_pass, double *ms_three_pass, char *spec_id, int spec_len);
 
 /* As bgt_hip_tune, with both choices and every measurement: ms (may be NULL) receives 6
- * selection times, ms[pass2_form * 2 + pass1_form]. No reference counterpart. */
+ * round times, ms[pass2_form * 2 + pass1_form]. No reference counterpart. */
 int bgt_hip_tune_forms(int use_dilution, int subjects, int variants, int precision, int selector, int force,
 					   int *pass2_form, int *pass1_form, double *ms, char *spec_id, int spec_len);
 

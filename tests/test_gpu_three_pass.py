@@ -113,10 +113,12 @@ def test_tuner_measures_caches_and_applies(tmp_path, monkeypatch, form):
     ms = (ctypes.c_double * 6)()
     assert L.bgt_hip_tune_forms(0, 28, 1, nat.FP32C, 1, 1, ctypes.byref(p2), ctypes.byref(p1), ms, sid, 256) == nat.OK, \
         L.bgt_hip_last_error()
-    # every pair of forms measured; the choice is the fastest
+    # every pair of forms measured; the choice is the fastest, or form 0's fastest pass-1 grid
+    # when that is within 3 % of the fastest (near-equal forms are not decided by noise)
     table = list(ms)
     assert all(t > 0 for t in table)
-    assert table[p2.value * 2 + p1.value] == min(table)
+    chosen = table[p2.value * 2 + p1.value]
+    assert chosen == min(table) or (p2.value == 0 and chosen == min(table[0:2]) and chosen <= 1.03 * min(table))
     name = sid.value.decode()
     assert name.startswith("kernel-") and "-n28-v1-s32-d0-q1-r2-" in name and name.endswith("gfx950")
     path = os.path.join(str(tmp_path), name + ".bgthip")
