@@ -43,6 +43,17 @@ int parallel_width();
 
 class ExactHistory {
 public:
+	ExactHistory() = default;
+	// A copy carries the program (prior, history, atom order, normaliser), not the working
+	// memory of its evaluations, which grows to megabytes per worker after rescoring and
+	// would otherwise be copied with every child lattice and projection.
+	ExactHistory(const ExactHistory &o) { copy_program(o); }
+	ExactHistory &operator=(const ExactHistory &o)
+	{
+		if (this != &o)
+			copy_program(o);
+		return *this;
+	}
 	// dilution: the pushes carry dilution tables (subjects x (subjects + 1), row s-1 for pool
 	// size s). The host's pool program covers binary lattices (variants == 1); lattices of
 	// several variants keep their history for the device enumeration only (enum_inputs).
@@ -133,6 +144,21 @@ private:
 		Table a, b;
 		std::vector<double> da, db; // dense program
 	};
+	void copy_program(const ExactHistory &o)
+	{
+		usable_ = o.usable_;
+		broken_ = o.broken_;
+		dilution_ = o.dilution_;
+		atoms_ = o.atoms_;
+		variants_ = o.variants_;
+		subjects_ = o.subjects_;
+		pi_ = o.pi_;
+		q_ = o.q_;
+		hist_ = o.hist_;
+		order_ = o.order_;
+		z_ = o.z_;
+		z_ok_ = o.z_ok_;
+	}
 	bool unnormalised(uint32_t e, double &out, Scratch &sc) const;
 	bool counts(uint32_t e, double &out, Scratch &sc) const; // the dilution program
 	double dense(uint32_t e, Scratch &sc) const; // sc.pool/fin/fout built by unnormalised

@@ -575,12 +575,13 @@ __device__ __forceinline__ void pass1_load(const LatDesc &d, unsigned tile, int 
 }
 
 // Pass-1 variants that run persistent with a prefetch of the next tile (FP32 storage, 32
-// registers per thread; not the general likelihood, whose table lookups leave too few).
-// The launcher picks the grid by the same rule.
-template <typename TP, bool FUSED, int LIK>
+// registers per thread): all but the fused general likelihood on the blocked intermediate,
+// which would spill (its table lookups beside the blocked store's offsets). The launcher
+// picks the grid by the same rule.
+template <typename TP, bool FUSED, int LIK, bool ZB>
 constexpr bool pass1_persistent()
 {
-	return sizeof(TP) == 4 && (!FUSED || LIK == 1);
+	return sizeof(TP) == 4 && (!FUSED || LIK == 1 || !ZB);
 }
 
 // One pass-1 tile from its loaded entries: the fused update (stored in place), the sweep
@@ -679,14 +680,14 @@ __global__ __launch_bounds__(kTileThreads) void k_zeta_pass1(Descs descs)
 	}
 	const double inv = uniform(d.scal[d.inv_in]);
 	const double f_in = FUSED ? uniform(s_lik[0]) : 0.0, f_out = FUSED ? uniform(s_lik[1]) : 0.0;
-	if (!pass1_persistent<TP, FUSED, LIK>() || gridDim.x >= tiles) {
+	if (!pass1_persistent<TP, FUSED, LIK, ZB>() || gridDim.x >= tiles) {
 		const unsigned tile = xcd_tile(blockIdx.x, tiles);
 		TP raw[kTileRegs];
 		pass1_load(d, tile, lane, wave, raw);
 		pass1_tile<TP, TZ, FUSED, LIK, ZB>(d, tile, raw, lane, wave, inv, f_in, f_out, s_lik, lds);
 		return;
 	}
-	if constexpr (pass1_persistent<TP, FUSED, LIK>()) {
+	if constexpr (pass1_persistent<TP, FUSED, LIK, ZB>()) {
 		const unsigned wg_per_xcd = gridDim.x >> 3;
 		const unsigned per_xcd = tiles >> 3;
 		const unsigned xcd = blockIdx.x & 7u;
@@ -3432,7 +3433,8 @@ void launch_zeta_pass1(const Descs &descs, int count, int n, Storage st, bool fu
 	// FP32 storage: persistent workgroups (one per CU) that prefetch their next tile, once
 	// there are more tiles than CUs; otherwise one workgroup per tile
 	const unsigned tiles = 1u << (n - kTileBits);
-	const bool persistent = st == Storage::fp32c && (!fused || (k == 1 && !dilution)) && tuned_pass1_form(n, st) != 1;
+	const bool persistent =
+		st == Storage::fp32c && (!fused || (k == 1 && !dilution) || !zblk) && tuned_pass1_form(n, st) != 1;
 	const unsigned groups = persistent ? zeta_final_groups(n) : tiles;
 	const dim3 grid(groups, count);
 	const bool binary_nd = (k == 1 && !dilution);

@@ -356,8 +356,10 @@ std::unique_ptr<DeviceLattice> DeviceLattice::project(const DeviceLattice &src, 
 	else
 		l.reset(new DeviceLattice(src.dilution_, n2, src.k_, src.storage_, Blank{}));
 	const uint32_t keep = all & ~removed;
-	// the source's pending work (its last update) comes first
-	BGT_HIP_CHECK(hipStreamSynchronize(src.stream_));
+	// the source's pending work on its own stream (an update) comes first; work of batched
+	// steps has finished when they return. (The stream is shared with other lattices: a wait
+	// for the whole stream would also wait for theirs.)
+	const_cast<DeviceLattice &>(src).settle();
 	launch_project(src.post_, src.scal_, src.inv_cur_, l->post_, l->atoms_, removed, src.storage_, l->stream_, l->part_,
 				   2048);
 	HostBox *box = static_cast<HostBox *>(l->host_box_);

@@ -131,6 +131,7 @@ class _Live:
         self.kept = list(range(spec.subjects))  # original (internal) index of each current subject
         self.removed = 0  # subjects summed out (original internal numbering)
         self.pos_internal = self.neg_internal = 0  # classification, original internal atoms
+        self._planned = None  # the classification the last shrink plan was made for
         self.shape = (spec.dilution, spec.subjects, spec.variants)  # the current lattice's
         self.tests = 0
         self.experiments: list[int] = []
@@ -168,8 +169,11 @@ class _Live:
         """make_shrink_plan over the classification so far, when it removes some subject."""
         if not self.shrink:
             return None
-        plan = api.make_shrink_plan(self.spec.subjects, len(self.kept), self.spec.variants,
-                                    self.pos_internal | self.neg_internal, self.removed)
+        classified = self.pos_internal | self.neg_internal
+        if classified == self._planned:
+            return None  # nothing classified since the last plan, which removed what it could
+        self._planned = classified
+        plan = api.make_shrink_plan(self.spec.subjects, len(self.kept), self.spec.variants, classified, self.removed)
         return plan if plan.has_shrink() and plan.new_current_subjects > 0 else None
 
     def apply(self, plan: api.ShrinkPlan) -> None:
