@@ -46,7 +46,10 @@ SHAPES = [(3, 1, False), (10, 1, True), (14, 1, False), (15, 1, True), (16, 1, F
           (20, 1, False), (22, 1, True), (3, 2, True), (6, 2, False), (4, 3, True), (5, 3, False), (3, 4, False),
           # k >= 2 above 15 atoms: the transform over pass 1 and the multi-tile final pass,
           # feature tables gathered from it, scored by inclusion-exclusion
-          (8, 2, True), (10, 2, False), (11, 2, True), (12, 2, False), (13, 2, True), (6, 3, True), (4, 4, True)]
+          (8, 2, True), (10, 2, False), (11, 2, True), (12, 2, False), (13, 2, True), (6, 3, True), (4, 4, True),
+          # five variants (the most the provider takes): whole transform in LDS, one tile, and the
+          # multi-tile gather path
+          (2, 5, True), (3, 5, False), (4, 5, True)]
 
 
 @pytest.mark.parametrize("n,k,dilution", SHAPES)
@@ -57,8 +60,8 @@ def test_update_select_masses_sequence(n, k, dilution):
     ref = orc.prior(pi)
     atoms = n * k
     selectors = [1, 2] if k == 1 else [2]
-    if n <= 10 and n + atoms <= 30:
-        selectors.append(3)
+    if n <= 10 and n + atoms <= 30 and k <= 4:
+        selectors.append(3)  # brute force takes at most 4 variants
     for step, (e, r) in enumerate(op_sequence(seed, n, k, 4)):
         # public coordinates for k=1: flip once for the oracle (internal)
         e_int = orc.reverse_bits(e, n) if k == 1 else e
@@ -94,7 +97,8 @@ def test_update_select_masses_sequence(n, k, dilution):
         assert den > 0
 
 
-@pytest.mark.parametrize("n,k,dilution", [(6, 1, False), (12, 1, True), (16, 1, False), (4, 2, True), (4, 3, False)])
+@pytest.mark.parametrize("n,k,dilution", [(6, 1, False), (12, 1, True), (16, 1, False), (4, 2, True), (4, 3, False),
+                                          (3, 5, True)])
 def test_classification_masks_match(n, k, dilution):
     seed = 77 + n
     lat, pi = make(n, k, dilution, seed, lo=0.005, hi=0.05)
