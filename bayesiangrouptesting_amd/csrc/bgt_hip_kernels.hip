@@ -2019,11 +2019,12 @@ __global__ __launch_bounds__(1024) void k_gather_pick(Descs descs)
 // k_finalize_top2 (the order (|objective|, experiment) is total, so the same two), the band as
 // k_bbpa_band, and at a near tie the exact re-decision: every state S of every variant block v
 // gets its weight w_v(S) (the prior factors of the block's atoms times the history's factors
-// for variant v) in LDS, thread i sums band entry i's per-variant binary masses
-// U_v(E) = sum of w_v(S) over S >= E in state order, against the block normalisers U_v({})
-// summed by the workgroup, and forms its exact objective; the exact best two go into
-// d.result (counter[kGatherFlag] = 1). Six launches become one. The one-pass path holds at
-// most 2^10 states per block and 2048 weights in all ((2^k - 1) 2^(n-6) <= 64).
+// for variant v) in LDS, the superset sums U_v(E) of every block are formed there in place
+// (one subject bit at a time, so each is a sum tree of depth n), and thread i takes band
+// entry i's per-variant binary masses U_v(E) / U_v({}) and forms its exact objective; the
+// exact best two go into d.result (counter[kGatherFlag] = 1). Six launches become one. The
+// one-pass path holds at most 2^10 states per block and 2048 weights in all
+// ((2^k - 1) 2^(n-6) <= 64).
 constexpr int kFinishWeights = 2048;
 __global__ __launch_bounds__(1024) void k_bbpa_finish(Descs descs, int fused, int k, double target)
 {
@@ -2032,7 +2033,6 @@ __global__ __launch_bounds__(1024) void k_bbpa_finish(Descs descs, int fused, in
 	__shared__ double red[16];
 	__shared__ double s_pairs[2 * kMaxAtoms];
 	__shared__ double s_w[kFinishWeights];
-	__shared__ double s_u0[kMaxVariants];
 	__shared__ unsigned s_count;
 	__shared__ double s_lim;
 	__shared__ int s_redecide;
@@ -2138,32 +2138,25 @@ __global__ __launch_bounds__(1024) void k_bbpa_finish(Descs descs, int fused, in
 			s_w[idx] = p;
 		}
 		__syncthreads();
-		for (int v = 0; v < k; ++v) {
-			double acc = 0.0;
-			for (uint32_t st = threadIdx.x; st < xc; st += blockDim.x)
-				acc += s_w[((uint32_t)v << xn) + st];
-			const double u = block_sum(acc, red);
-			if (threadIdx.x == 0)
-				s_u0[v] = u;
+		// every block's superset sums in place, one subject bit at a time (each sum a tree of
+		// depth xn): afterwards s_w[v][S] = U_v(S) and s_w[v][0] = U_v({}), the normaliser
+		const uint32_t half = xc >> 1;
+		for (int b = 0; b < xn; ++b) {
+			const uint32_t low = (1u << b) - 1u;
+			for (uint32_t idx = threadIdx.x; idx < (uint32_t)k * half; idx += blockDim.x) {
+				const uint32_t v = idx / half, r = idx % half;
+				const uint32_t st = ((r & ~low) << 1) | (r & low); // bit b clear
+				double *w = s_w + (v << xn);
+				w[st] += w[st | (1u << b)];
+			}
 			__syncthreads();
 		}
 		if (threadIdx.x < count) {
 			const uint32_t ce = d.gband[threadIdx.x].e;
 			const uint32_t xe = frame_deposit(ce, d.gx_keep);
-			const uint32_t freeb = (xc - 1u) & ~xe;
 			double mv[kMaxVariants];
-			for (int v = 0; v < k; ++v) {
-				// the supersets of xe in increasing order: st = xe | sub, sub over the subsets
-				// of the free subjects by the (sub - freeb) & freeb step
-				const double *w = s_w + ((uint32_t)v << xn);
-				double acc = 0.0;
-				uint32_t sub = 0u;
-				do {
-					acc += w[xe | sub];
-					sub = (sub - freeb) & freeb;
-				} while (sub != 0u);
-				mv[v] = acc / s_u0[v];
-			}
+			for (int v = 0; v < k; ++v)
+				mv[v] = s_w[((uint32_t)v << xn) + xe] / s_w[(uint32_t)v << xn];
 			top2_insert(ex, gather_objective(mv, k, target), ce);
 		}
 	} else {
