@@ -111,6 +111,12 @@ def _to_original(x: int, kept: Sequence[int], current: int, original: int, varia
     return out
 
 
+def _kept_after(kept: Sequence[int], plan: api.ShrinkPlan) -> list[int]:
+    """The original index of each subject a projection keeps: current subject j leaves when
+    its atoms are removable (they are whole subjects, so bit j of variant 0 tells)."""
+    return [sbj for j, sbj in enumerate(kept) if not (plan.removable_current_atoms >> j) & 1]
+
+
 class _Live:
     """An admitted session: its lattice, simulated lab and progress."""
 
@@ -181,7 +187,7 @@ class _Live:
         projected = self.lattice.project(plan)
         self.lattice.close()
         self.lattice = projected
-        self.kept = [sbj for j, sbj in enumerate(self.kept) if not (plan.removable_current_atoms >> j) & 1]
+        self.kept = _kept_after(self.kept, plan)
         assert len(self.kept) == plan.new_current_subjects
         self.removed = plan.new_classified_subjects
         self.shape = (self.spec.dilution, len(self.kept), self.spec.variants)
