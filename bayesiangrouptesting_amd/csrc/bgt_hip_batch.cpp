@@ -469,6 +469,23 @@ void batch_step(std::vector<BatchItem> &items, int selector, double up, double l
 			tied.push_back(i);
 	if (tied.size() > 1)
 		parallel_for(tied.size(), [&](size_t j, int) { items[tied[j]].lat->prepare_rescore(selector); });
+	// the binary dilution lattices' first exact masses by one enumeration launch and one
+	// synchronisation for the step, where each lattice's re-decision would make its own
+	{
+		std::vector<DeviceLattice *> enum_lats;
+		std::vector<std::vector<uint32_t>> enum_sets;
+		for (size_t i : tied) {
+			if (!items[i].lat->uses_exact_enum())
+				continue;
+			std::vector<uint32_t> sets = items[i].lat->enum_ahead_sets(selector);
+			if (sets.empty())
+				continue;
+			enum_lats.push_back(items[i].lat);
+			enum_sets.push_back(std::move(sets));
+		}
+		if (!enum_lats.empty())
+			DeviceLattice::enum_ahead_run(enum_lats, enum_sets, s);
+	}
 	for (size_t i = 0; i < count; ++i) {
 		auto &it = items[i];
 		if (it.classify)

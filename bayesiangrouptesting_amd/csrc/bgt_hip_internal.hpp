@@ -362,6 +362,9 @@ struct EnumJob {
 	int atoms, updates, stride, count;
 };
 void launch_exact_enum(const EnumJob &job, hipStream_t stream);
+// The jobs of several lattices in one launch: jobs_dev is a device copy of `jobs`. Every
+// query's value is the one launch_exact_enum gives it.
+void launch_exact_enum_many(const EnumJob *jobs_dev, const std::vector<EnumJob> &jobs, hipStream_t stream);
 // Relative rounding bound of one value of launch_exact_enum: the products (prior and
 // likelihood factors), the in-thread sums and the two trees, all terms non-negative.
 inline double enum_rel_bound(int atoms, int updates)

@@ -3205,13 +3205,18 @@ __global__ __launch_bounds__(64) void k_exact_band(Descs descs, int atoms)
 	}
 }
 
-// Result heads of a batch of lattices into one staging buffer (one copy to the host).
 // Exact unnormalised upset masses by enumeration (see EnumJob): workgroup (x, y) serves query
 // y's share x of the rows. A row is the candidate's atoms, one assignment of the free atoms
 // above the lowest kEnumLowBits, and every assignment of those lowest ones; its prior
 // product and the factors of the pools without a low free atom are formed once per row.
-__global__ __launch_bounds__(256) void k_exact_enum(EnumJob job)
+// MANY: job blockIdx.z of a device array (several lattices' jobs in one launch, the grid
+// sized for the largest; a query's workgroups do the same work as in a launch of its own).
+template <bool MANY>
+__global__ __launch_bounds__(256) void k_exact_enum(EnumJob one, const EnumJob *many)
 {
+	const EnumJob job = MANY ? many[blockIdx.z] : one;
+	if (MANY && blockIdx.y >= (unsigned)job.count)
+		return;
 	const uint32_t e = job.queries[blockIdx.y];
 	const int atoms = job.atoms;
 	const uint32_t full = (1u << atoms) - 1u; // atoms <= 30
@@ -3305,9 +3310,13 @@ __global__ __launch_bounds__(256) void k_exact_enum(EnumJob job)
 		job.partial[(size_t)blockIdx.y * kEnumMaxGroups + blockIdx.x] = bs;
 }
 
-// the query's workgroup sums, in one fixed tree
-__global__ __launch_bounds__(kEnumMaxGroups) void k_exact_enum_final(EnumJob job)
+// the query's workgroup sums, in one fixed tree (MANY: query x of job y)
+template <bool MANY>
+__global__ __launch_bounds__(kEnumMaxGroups) void k_exact_enum_final(EnumJob one, const EnumJob *many)
 {
+	const EnumJob job = MANY ? many[blockIdx.y] : one;
+	if (MANY && blockIdx.x >= (unsigned)job.count)
+		return;
 	const uint32_t full = (1u << job.atoms) - 1u;
 	const unsigned groups = enum_groups(__popc(full & ~job.queries[blockIdx.x]));
 	__shared__ double red[16];
@@ -3317,6 +3326,7 @@ __global__ __launch_bounds__(kEnumMaxGroups) void k_exact_enum_final(EnumJob job
 		job.out[blockIdx.x] = sum;
 }
 
+// Result heads of a batch of lattices into one staging buffer (one copy to the host).
 __global__ __launch_bounds__(256) void k_gather_heads(const LatDesc *many, unsigned char *out)
 {
 	// the 1 KB block from the arena's start, then the first kBandHead band candidates from
@@ -3666,16 +3676,43 @@ void launch_exact_band(const Descs &descs, int count, int atoms, hipStream_t str
 	});
 }
 
-void launch_exact_enum(const EnumJob &job, hipStream_t stream)
+static void check_enum_job(const EnumJob &job)
 {
 	if (job.atoms < 1 || job.atoms > kMaxAtoms || job.updates < 0 || job.updates > kEnumMaxHist || job.count < 1 ||
-		job.stride < 1 || job.stride > kMaxSubjects + 1)
+		job.count > 65535 || job.stride < 1 || job.stride > kMaxSubjects + 1)
 		throw Error(4, "exact enumeration: shape out of range");
+}
+
+void launch_exact_enum(const EnumJob &job, hipStream_t stream)
+{
+	check_enum_job(job);
 	// sized for the most free atoms a query can have (the empty one); each query uses its own
 	const unsigned groups = enum_groups(job.atoms);
 	launch_checked("k_exact_enum", [&] {
-		hipLaunchKernelGGL(k_exact_enum, dim3(groups, job.count), dim3(256), 0, stream, job);
-		hipLaunchKernelGGL(k_exact_enum_final, dim3(job.count), dim3(kEnumMaxGroups), 0, stream, job);
+		hipLaunchKernelGGL(k_exact_enum<false>, dim3(groups, job.count), dim3(256), 0, stream, job, nullptr);
+		hipLaunchKernelGGL(k_exact_enum_final<false>, dim3(job.count), dim3(kEnumMaxGroups), 0, stream, job,
+						   nullptr);
+	});
+}
+
+void launch_exact_enum_many(const EnumJob *jobs_dev, const std::vector<EnumJob> &jobs, hipStream_t stream)
+{
+	if (jobs.empty())
+		return;
+	if (jobs.size() > 65535)
+		throw Error(4, "exact enumeration: too many jobs for one launch");
+	unsigned groups = 1u, count = 1u;
+	for (const EnumJob &j : jobs) {
+		check_enum_job(j);
+		groups = std::max(groups, enum_groups(j.atoms));
+		count = std::max(count, (unsigned)j.count);
+	}
+	const EnumJob none{};
+	launch_checked("k_exact_enum_many", [&] {
+		hipLaunchKernelGGL(k_exact_enum<true>, dim3(groups, count, (unsigned)jobs.size()), dim3(256), 0, stream,
+						   none, jobs_dev);
+		hipLaunchKernelGGL(k_exact_enum_final<true>, dim3(count, (unsigned)jobs.size()), dim3(kEnumMaxGroups), 0,
+						   stream, none, jobs_dev);
 	});
 }
 

@@ -915,6 +915,7 @@ void DeviceLattice::launch_exact(const Descs &ds_in)
 
 void DeviceLattice::unpack_head(int band_entries, size_t band_offset)
 {
+	ahead_ = Ahead{}; // formed for the previous head
 	HostBox *box = static_cast<HostBox *>(host_box_);
 	std::memcpy(box->scal, box->head, sizeof(double) * kScalars);
 	std::memcpy(&box->result, box->head + kHeadResult, sizeof(SelResult));
@@ -1003,7 +1004,7 @@ bool DeviceLattice::exact_masses(const uint32_t *es, size_t count, double *out, 
 	}
 	if (!uses_enum())
 		return exact_.masses(es, count, out, ok);
-	if (count > 0)
+	if (count > 0 && !(ahead_.valid && ahead_masses(es, count, out)))
 		enum_masses(es, count, out);
 	for (size_t i = 0; i < count; ++i)
 		ok[i] = true;
@@ -1239,19 +1240,10 @@ void DeviceLattice::rescore(bgt_hip_selection &out, int sel, const SelResult &r)
 	clock_.host("host_rescore", std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
 }
 
-void DeviceLattice::rescore_band(bgt_hip_selection &out, int sel, const SelResult &r)
+std::vector<Cand> DeviceLattice::live_band(int sel, const SelResult &r, bool &complete_out, unsigned &count_out)
 {
-	// Exact FP64 objectives for the candidates within the error band. The device best's
-	// true objective is at most d1 + eps, so the true best has a device objective of at most
-	// d1 + 2 eps: the band, which the device collects completely (the per-thread top-2 lists
-	// plus a rescan of every workgroup whose lists may have hidden more) unless it overflows
-	// its capacity.
-	// Dominance: M(E) is strictly decreasing under inclusion (every state has positive
-	// mass), and both objectives are increasing in |M - 1/2| (binary BBPA's is 2|M - 1/2|).
-	// A candidate whose mass is surely above 1/2 therefore loses to any one-subject extension
-	// whose mass is surely above 1/2, and one surely below 1/2 to any one-subject reduction
-	// surely below 1/2. Such candidates are not rescored; the rest are, in order of device
-	// objective, until the next one's lower bound exceeds the exact runner-up.
+	// The band's candidates, one entry per experiment, less those the dominance test
+	// removes (see rescore_band); complete_out: the band was collected whole.
 	HostBox *box = static_cast<HostBox *>(host_box_);
 	const double eps = 0.5 * band_width(sel);
 	const unsigned count = box->counter[0];
@@ -1375,6 +1367,69 @@ void DeviceLattice::rescore_band(bgt_hip_selection &out, int sel, const SelResul
 	phase("t_dominance");
 	clock_.host("n_band_unique", (double)cands.size());
 	clock_.host("n_band_live", (double)live.size());
+	complete_out = complete;
+	count_out = count;
+	return live;
+}
+
+// most candidates one rescoring computes exact masses for
+constexpr int kMaxRescored = 4096;
+
+// the live band sorted by device objective; returns how many lead it that may be needed
+// before the exact runner-up is known (the first chunk of rescore_band's scan)
+static size_t first_chunk(std::vector<Cand> &live, double eps, size_t cap)
+{
+	std::sort(live.begin(), live.end(), [](const Cand &a, const Cand &b) {
+		const double x = std::fabs(a.d), y = std::fabs(b.d);
+		return x < y || (x == y && a.e < b.e);
+	});
+	if (live.size() < 2)
+		return 1;
+	const double reach = std::fabs(live[1].d) + 2.0 * eps;
+	size_t need = 0;
+	while (need < live.size() && need < cap && !(std::fabs(live[need].d) > reach))
+		++need;
+	return std::max<size_t>(need, 1);
+}
+
+void DeviceLattice::rescore_band(bgt_hip_selection &out, int sel, const SelResult &r)
+{
+	// Exact FP64 objectives for the candidates within the error band. The device best's
+	// true objective is at most d1 + eps, so the true best has a device objective of at most
+	// d1 + 2 eps: the band, which the device collects completely (the per-thread top-2 lists
+	// plus a rescan of every workgroup whose lists may have hidden more) unless it overflows
+	// its capacity.
+	// Dominance: M(E) is strictly decreasing under inclusion (every state has positive
+	// mass), and both objectives are increasing in |M - 1/2| (binary BBPA's is 2|M - 1/2|).
+	// A candidate whose mass is surely above 1/2 therefore loses to any one-subject extension
+	// whose mass is surely above 1/2, and one surely below 1/2 to any one-subject reduction
+	// surely below 1/2. Such candidates are not rescored; the rest are, in order of device
+	// objective, until the next one's lower bound exceeds the exact runner-up.
+	HostBox *box = static_cast<HostBox *>(host_box_);
+	const double eps = 0.5 * band_width(sel);
+	// masses enumerated ahead by a batched step serve this rescoring only
+	struct Spent {
+		Ahead &a;
+		~Spent() { a = Ahead{}; }
+	} spent{ahead_};
+	bool complete = false;
+	unsigned count = 0;
+	std::vector<Cand> live;
+	if (ahead_.valid) {
+		live = std::move(ahead_.live);
+		complete = ahead_.complete;
+		count = ahead_.count;
+	} else {
+		live = live_band(sel, r, complete, count);
+	}
+	auto tp = std::chrono::steady_clock::now();
+	auto phase = [&](const char *name) {
+		if (!clock_.enabled)
+			return;
+		const auto now = std::chrono::steady_clock::now();
+		clock_.host(name, std::chrono::duration<double, std::milli>(now - tp).count());
+		tp = now;
+	};
 	if (complete && live.size() == 1) {
 		// every other candidate loses to one that beats it strictly: the survivor is the
 		// exact best, decided without its exact mass (objective: the device's value)
@@ -1394,7 +1449,6 @@ void DeviceLattice::rescore_band(bgt_hip_selection &out, int sel, const SelResul
 	double b1 = INFINITY, b2 = INFINITY;
 	uint32_t e1 = 0xFFFFFFFFu, e2 = 0xFFFFFFFFu;
 	int rescored = 0;
-	constexpr int kMaxRescored = 4096;
 	auto objective_of = [&](double m) {
 		return sel == kSelBha ? std::fabs(m - 0.5) : std::fabs(m - t) + std::fabs((1.0 - m) - t);
 	};
@@ -1432,27 +1486,17 @@ void DeviceLattice::rescore_band(bgt_hip_selection &out, int sel, const SelResul
 		clock_.host("n_exact_device", (double)live.size());
 		clock_.host("n_exact_host", 0.0);
 	} else {
-		std::sort(live.begin(), live.end(), [](const Cand &a, const Cand &b) {
-			const double x = std::fabs(a.d), y = std::fabs(b.d);
-			return x < y || (x == y && a.e < b.e);
-		});
 		// The scan below is sequential; the exact masses it needs are computed ahead, in
 		// parallel, so the result is that of the sequential scan. It never goes past a device
 		// objective of |d_2| + 2 eps (d_2: the second candidate's): the exact objectives of the
 		// first two candidates are at most |d_2| + eps, so the exact runner-up is too, and the
 		// scan stops at the first candidate whose lower bound |d| - eps exceeds it. Those masses
-		// are computed in one parallel run; later chunks are only a guard.
+		// are computed in one parallel run (or were, by a batched step's enumeration); later
+		// chunks are only a guard.
 		std::vector<uint32_t> ex;
 		std::vector<double> mass;
 		std::unique_ptr<bool[]> okv;
-		size_t ready = 0, chunk = 64;
-		if (live.size() >= 2) {
-			const double reach = std::fabs(live[1].d) + 2.0 * eps;
-			size_t need = 0;
-			while (need < live.size() && need < (size_t)kMaxRescored && !(std::fabs(live[need].d) > reach))
-				++need;
-			chunk = std::max<size_t>(need, 1);
-		}
+		size_t ready = 0, chunk = first_chunk(live, eps, (size_t)kMaxRescored);
 		for (size_t i = 0; i < live.size(); ++i) {
 			const Cand &c = live[i];
 			if (std::fabs(c.d) - eps > b2)
@@ -1522,6 +1566,141 @@ void DeviceLattice::rescore_band(bgt_hip_selection &out, int sel, const SelResul
 	out.redecided = complete ? 1 : 2;
 	out.candidates = rescored;
 	out.band = (int)count;
+}
+
+std::vector<uint32_t> DeviceLattice::enum_ahead_sets(int selector)
+{
+	// the conditions under which finish_selection calls rescore_band, and the sets of that
+	// scan's first request for exact masses (no candidate has a device mass here)
+	ahead_ = Ahead{};
+	std::vector<uint32_t> sets;
+	const int sel = resolve_selector(selector, k_);
+	if (k_ != 1 || !uses_enum() || !(sel == kSelBha || sel == kSelBbpa))
+		return sets;
+	const SelResult r = static_cast<const HostBox *>(host_box_)->result;
+	const bool found = r.e1 != 0xFFFFFFFFu && r.d1 < 2.0;
+	if (!found || !((r.d2 - r.d1) <= band_width(sel)))
+		return sets;
+	bool complete = false;
+	unsigned count = 0;
+	std::vector<Cand> live = live_band(sel, r, complete, count);
+	if (!(complete && live.size() == 1) && !live.empty()) {
+		const size_t need = std::min(first_chunk(live, 0.5 * band_width(sel), (size_t)kMaxRescored), live.size());
+		sets.reserve(need);
+		for (size_t i = 0; i < need; ++i)
+			sets.push_back(xn_ != n_ ? atoms_to_frame(live[i].e) : live[i].e);
+	}
+	ahead_.live = std::move(live);
+	ahead_.complete = complete;
+	ahead_.count = count;
+	ahead_.valid = true;
+	return sets;
+}
+
+bool DeviceLattice::ahead_masses(const uint32_t *frame_sets, size_t count, double *out) const
+{
+	for (size_t i = 0; i < count; ++i) {
+		const auto it = std::lower_bound(ahead_.sets.begin(), ahead_.sets.end(), frame_sets[i]);
+		if (it == ahead_.sets.end() || *it != frame_sets[i])
+			return false;
+		out[i] = ahead_.masses[(size_t)(it - ahead_.sets.begin())];
+	}
+	return true;
+}
+
+void DeviceLattice::enum_ahead_run(const std::vector<DeviceLattice *> &lats,
+								   const std::vector<std::vector<uint32_t>> &sets, hipStream_t stream)
+{
+	// enum_run's inputs and outputs for every lattice with sets, one job each (one variant
+	// block), the job table itself uploaded with them; query 0 of a job is U({})
+	auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
+	struct Part {
+		DeviceLattice *lat;
+		const std::vector<uint32_t> *q;
+		size_t o_pairs, o_fac, o_pool, o_q, o_part, o_out;
+	};
+	std::vector<Part> parts;
+	for (size_t i = 0; i < lats.size(); ++i)
+		if (!sets[i].empty() && lats[i]->ahead_.valid)
+			parts.push_back(Part{lats[i], &sets[i], 0, 0, 0, 0, 0, 0});
+	if (parts.empty())
+		return;
+	const size_t o_jobs = 0;
+	size_t in_bytes = up(sizeof(EnumJob) * parts.size());
+	for (Part &p : parts) {
+		const int xn = p.lat->xn_, T = p.lat->exact_.enum_pools();
+		if (p.q->size() + 1 > 65535)
+			throw Error(BGT_HIP_INTERNAL, "too many exact masses for one launch");
+		p.o_pairs = in_bytes;
+		p.o_fac = p.o_pairs + up(sizeof(double) * 2 * xn);
+		p.o_pool = p.o_fac + up(sizeof(double) * (size_t)T * (xn + 1) + 8);
+		p.o_q = p.o_pool + up(sizeof(uint32_t) * (size_t)T + 4);
+		in_bytes = p.o_q + up(sizeof(uint32_t) * (p.q->size() + 1));
+	}
+	size_t dev_bytes = in_bytes;
+	for (Part &p : parts) {
+		p.o_part = dev_bytes;
+		dev_bytes += up(sizeof(double) * (p.q->size() + 1) * kEnumMaxGroups);
+	}
+	const size_t o_outs = dev_bytes; // every job's values, contiguous: one copy back
+	for (Part &p : parts) {
+		p.o_out = dev_bytes;
+		dev_bytes += sizeof(double) * (p.q->size() + 1);
+	}
+	const size_t out_bytes = dev_bytes - o_outs;
+	const size_t host_bytes = in_bytes + out_bytes;
+	char *h = static_cast<char *>(pinned_alloc(host_bytes));
+	char *d = nullptr;
+	try {
+		d = static_cast<char *>(device_alloc(dev_bytes, stream));
+		std::vector<EnumJob> jobs(parts.size());
+		for (size_t j = 0; j < parts.size(); ++j) {
+			const Part &p = parts[j];
+			const int xn = p.lat->xn_;
+			p.lat->exact_.enum_inputs(0, reinterpret_cast<double *>(h + p.o_pairs),
+									  reinterpret_cast<uint32_t *>(h + p.o_pool), reinterpret_cast<double *>(h + p.o_fac),
+									  xn + 1);
+			uint32_t *qh = reinterpret_cast<uint32_t *>(h + p.o_q);
+			qh[0] = 0u;
+			std::memcpy(qh + 1, p.q->data(), sizeof(uint32_t) * p.q->size());
+			EnumJob &job = jobs[j];
+			job.pairs = reinterpret_cast<const double *>(d + p.o_pairs);
+			job.pools = reinterpret_cast<const uint32_t *>(d + p.o_pool);
+			job.fac = reinterpret_cast<const double *>(d + p.o_fac);
+			job.queries = reinterpret_cast<const uint32_t *>(d + p.o_q);
+			job.partial = reinterpret_cast<double *>(d + p.o_part);
+			job.out = reinterpret_cast<double *>(d + p.o_out);
+			job.atoms = xn;
+			job.updates = p.lat->exact_.enum_pools();
+			job.stride = xn + 1;
+			job.count = (int)(p.q->size() + 1);
+		}
+		std::memcpy(h + o_jobs, jobs.data(), sizeof(EnumJob) * jobs.size());
+		BGT_HIP_CHECK(hipMemcpyAsync(d, h, in_bytes, hipMemcpyHostToDevice, stream));
+		launch_exact_enum_many(reinterpret_cast<const EnumJob *>(d + o_jobs), jobs, stream);
+		BGT_HIP_CHECK(hipMemcpyAsync(h + in_bytes, d + o_outs, out_bytes, hipMemcpyDeviceToHost, stream));
+		BGT_HIP_CHECK(hipStreamSynchronize(stream));
+	} catch (...) {
+		device_free(d, stream);
+		pinned_release(h, host_bytes);
+		throw;
+	}
+	device_free(d, stream);
+	for (const Part &p : parts) {
+		const double *u = reinterpret_cast<const double *>(h + in_bytes + (p.o_out - o_outs));
+		std::vector<size_t> order(p.q->size());
+		for (size_t i = 0; i < order.size(); ++i)
+			order[i] = i;
+		std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return (*p.q)[a] < (*p.q)[b]; });
+		Ahead &a = p.lat->ahead_;
+		a.sets.resize(order.size());
+		a.masses.resize(order.size());
+		for (size_t i = 0; i < order.size(); ++i) {
+			a.sets[i] = (*p.q)[order[i]];
+			a.masses[i] = u[order[i] + 1] / u[0]; // as enum_run forms M
+		}
+	}
+	pinned_release(h, host_bytes);
 }
 
 bgt_hip_selection DeviceLattice::select(int selector)

@@ -145,6 +145,14 @@ public:
 	// tie on a lattice with an exact program); if so, its normaliser is computed now, so that
 	// the normalisers of many lattices can run side by side on the host threads.
 	bool prepare_rescore(int selector);
+	// After load_head, binary dilution lattices: the sets (the exact program's frame) whose
+	// masses finish_selection's re-decision will first ask the device enumeration for; empty
+	// when it will ask for none. The band it worked out is kept for that re-decision.
+	std::vector<uint32_t> enum_ahead_sets(int selector);
+	// The enumerations of several lattices' sets (enum_ahead_sets) in one launch on `stream`
+	// and one synchronisation; each lattice's next re-decision takes its masses from there.
+	static void enum_ahead_run(const std::vector<DeviceLattice *> &lats,
+							   const std::vector<std::vector<uint32_t>> &sets, hipStream_t stream);
 	// Exact FP64 M(E) of `count` experiments recomputed from the prior and the update
 	// history (internal coordinates): by enumeration on the device for binary dilution
 	// lattices (at most kEnumMaxHist updates), by the host program for binary non-dilution
@@ -175,6 +183,19 @@ private:
 	void fill_lik(uint32_t experiment, uint32_t response, const double *dilution_flat);
 	void rescore(bgt_hip_selection &out, int selector, const SelResult &r);
 	void rescore_band(bgt_hip_selection &out, int selector, const SelResult &r);
+	// the band of the head now loaded, one entry per experiment, less the dominated entries
+	std::vector<Cand> live_band(int selector, const SelResult &r, bool &complete, unsigned &count);
+	// masses of the head's first re-decision chunk, enumerated ahead (enum_ahead_run)
+	struct Ahead {
+		bool valid = false;         // formed for the head now loaded
+		std::vector<Cand> live;     // live_band's result
+		bool complete = false;      // and its other outputs
+		unsigned count = 0;
+		std::vector<uint32_t> sets; // frame sets, increasing
+		std::vector<double> masses; // their M(E)
+	};
+	Ahead ahead_;
+	bool ahead_masses(const uint32_t *frame_sets, size_t count, double *out) const;
 	// lattices of several variants: the Op-BBPA band after k_bbpa_score (k_bbpa_band), and its
 	// exact re-decision from exact feature masses by the device enumeration
 	void collect_gather_band(double target, const Descs *fused = nullptr);

@@ -54,6 +54,56 @@ def test_batch_round_equals_per_lattice_calls():
         np.testing.assert_array_equal(x.posterior(), y.posterior())
 
 
+def _paired_dilution(i, n, shrink):
+    # subjects 2j and 2j+1 share a prior, so an experiment holding one of a pair ties exactly
+    # with its mirror image: the top two tie and FP32C re-decides by enumeration every step
+    base = seeded_prior(900 + i, (n + 1) // 2, 0.02, 0.2)
+    pi = np.array([base[j // 2] for j in range(n)])
+    lat = bgt.Lattice(DL, n, pi, precision=F32)
+    if not shrink:
+        return lat
+    lat.update(bgt.reverse_bits(0b111, n), 0, bgt.DilutionTable(n, 0.05, 0.9))
+    plan = bgt.make_shrink_plan(n, n, 1, 0b101, 0)  # subjects 0 and 2 leave
+    out = lat.project(plan)
+    lat.close()
+    return out
+
+
+def test_batched_dilution_redecisions_equal_per_lattice_calls():
+    # the batched step enumerates the first exact masses of every tied binary dilution
+    # lattice in one launch; per-lattice calls enumerate their own
+    shapes = [(10, False), (12, False), (13, True), (14, False), (16, False), (17, True), (18, False), (20, False)]
+    a = [_paired_dilution(i, n, s) for i, (n, s) in enumerate(shapes)]
+    b = [_paired_dilution(i, n, s) for i, (n, s) in enumerate(shapes)]
+    subjects = [lat.subjects() for lat in a]
+    tables = [bgt.DilutionTable(m, 0.05, 0.9) for m in subjects]
+    g = workload.SplitMix64(11)
+    pending = [None] * len(a)
+    redecided = 0
+    for step in range(4):
+        resps = [None if e is None else g.below(2) for e in pending]
+        print(f"batched dilution re-decisions: step {step}", flush=True)
+        sels, masks = bgt.batch_round(b, pending, resps, bgt.SelectorType.auto_select, 0.01, 0.01, tables)
+        for i, (lat, e, r, t) in enumerate(zip(a, pending, resps, tables)):
+            if r is not None:
+                s = lat.round(e, r, 0.01, 0.01, t)
+            else:
+                lat.update_classification(0.01, 0.01)
+                s = lat.select(bgt.SelectorType.auto_select)
+            got = sels[i]
+            assert (s.experiment, s.objective, s.runner_up, s.runner_up_objective, s.near_tie, s.redecided,
+                    s.candidates, s.band) == (got.experiment, got.objective, got.runner_up, got.runner_up_objective,
+                                              got.near_tie, got.redecided, got.candidates, got.band), (step, i)
+            assert masks[i] == (lat.positive_classification_mask(), lat.negative_classification_mask()), (step, i)
+            redecided += s.redecided == 1 and s.candidates > 0
+        pending = [s.experiment for s in sels]
+    assert redecided >= len(a), redecided
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x.posterior(), y.posterior())
+        x.close()
+        y.close()
+
+
 def test_batch_step_without_classification_and_errors():
     n = 12
     a = [bgt.Lattice(ND, n, seeded_prior(3 + i, n, 0.01, 0.1)) for i in range(3)]
