@@ -1914,46 +1914,55 @@ __device__ double gather_objective(const double *mv, int k, double target)
 	return score;
 }
 
-// U_v(E) of block v (E in the program's frame of d.gx_n subjects): the sum over the states
-// S >= E of the block of the prior factors of its atoms times the likelihood factors of the
-// history's updates for variant v, every thread a strided share of the states in increasing
-// order, then the workgroup tree
-__device__ double gather_block_mass(const LatDesc &d, const double *s_pairs, int v, uint32_t e, double *red)
+// The weight of state S of block v (in the program's frame of d.gx_n subjects): the prior
+// factors of its atoms times the likelihood factors of the history's updates for variant v
+__device__ __forceinline__ double gather_weight(const LatDesc &d, const double *s_pairs, int v, uint32_t s)
 {
 	const int n = d.gx_n;
-	const uint32_t full = (1u << n) - 1u;
-	const uint32_t freeb = full & ~e;
+	double p = 1.0;
+	for (int i = 0; i < n; ++i)
+		p *= s_pairs[2 * (v * n + i) + ((s >> i) & 1u)];
+	for (int t = 0; t < d.gh_len; ++t) {
+		const uint32_t pool = d.gh_e[t];
+		const int len = __popc(pool);
+		const int m = __popc(pool & ~s); // positive pooled subjects of block v
+		const bool rb = (d.gh_r[t] >> v) & 1u;
+		double fv;
+		if (d.gh_dil != nullptr) {
+			const double p1 = len > 0 ? d.gh_dil[(size_t)(len - 1) * (n + 1) + m] : 0.0;
+			fv = rb ? p1 : 1.0 - p1;
+		} else if (m == 0) {
+			fv = rb ? kNegativeResponse : (1.0 - kNegativeResponse);
+		} else {
+			fv = rb ? (1.0 - kNegativeResponse) : kNegativeResponse;
+		}
+		p *= fv;
+	}
+	return p;
+}
+
+// the superset j of E: j's bits deposited at the subjects outside E, in increasing order
+__device__ __forceinline__ uint32_t gather_superset(uint32_t e, uint32_t freeb, int f, uint32_t j)
+{
+	uint32_t s = e, r = freeb;
+	for (int i = 0; i < f; ++i) {
+		const uint32_t b = r & (0u - r);
+		r &= r - 1u;
+		s |= ((j >> i) & 1u) ? b : 0u;
+	}
+	return s;
+}
+
+// U_v(E) of block v (E in the program's frame): the sum of gather_weight over the states
+// S >= E, every thread a strided share of the states in increasing order, then the
+// workgroup tree
+__device__ double gather_block_mass(const LatDesc &d, const double *s_pairs, int v, uint32_t e, double *red)
+{
+	const uint32_t freeb = ((1u << d.gx_n) - 1u) & ~e;
 	const int f = __popc(freeb);
 	double acc = 0.0;
-	for (uint32_t j = threadIdx.x; j < (1u << f); j += blockDim.x) {
-		// deposit j's bits at the free subjects, in increasing order
-		uint32_t s = e, r = freeb;
-		for (int i = 0; i < f; ++i) {
-			const uint32_t b = r & (0u - r);
-			r &= r - 1u;
-			s |= ((j >> i) & 1u) ? b : 0u;
-		}
-		double p = 1.0;
-		for (int i = 0; i < n; ++i)
-			p *= s_pairs[2 * (v * n + i) + ((s >> i) & 1u)];
-		for (int t = 0; t < d.gh_len; ++t) {
-			const uint32_t pool = d.gh_e[t];
-			const int len = __popc(pool);
-			const int m = __popc(pool & ~s); // positive pooled subjects of block v
-			const bool rb = (d.gh_r[t] >> v) & 1u;
-			double fv;
-			if (d.gh_dil != nullptr) {
-				const double p1 = len > 0 ? d.gh_dil[(size_t)(len - 1) * (n + 1) + m] : 0.0;
-				fv = rb ? p1 : 1.0 - p1;
-			} else if (m == 0) {
-				fv = rb ? kNegativeResponse : (1.0 - kNegativeResponse);
-			} else {
-				fv = rb ? (1.0 - kNegativeResponse) : kNegativeResponse;
-			}
-			p *= fv;
-		}
-		acc += p;
-	}
+	for (uint32_t j = threadIdx.x; j < (1u << f); j += blockDim.x)
+		acc += gather_weight(d, s_pairs, v, gather_superset(e, freeb, f, j));
 	const double sum = block_sum(acc, red);
 	__shared__ double s_sum;
 	if (threadIdx.x == 0)
@@ -1962,6 +1971,23 @@ __device__ double gather_block_mass(const LatDesc &d, const double *s_pairs, int
 	const double out = s_sum;
 	__syncthreads();
 	return out;
+}
+
+// The same sum by one wavefront: lane l the states l, l + 64, ... in increasing order, then
+// the wavefront's butterfly (each step's two partners add the same two values, so every lane
+// ends with the same sum). Frames of at most kWaveMassBits subjects: a lane then adds at
+// most 256 terms in sequence, the share the exact masses' rounding bound allows a thread.
+constexpr int kWaveMassBits = 14;
+__device__ double gather_block_mass_wave(const LatDesc &d, const double *s_pairs, int v, uint32_t e, int lane)
+{
+	const uint32_t freeb = ((1u << d.gx_n) - 1u) & ~e;
+	const int f = __popc(freeb);
+	double acc = 0.0;
+	for (uint32_t j = (uint32_t)lane; j < (1u << f); j += 64u)
+		acc += gather_weight(d, s_pairs, v, gather_superset(e, freeb, f, j));
+	for (int o = 32; o > 0; o >>= 1)
+		acc += __shfl_xor(acc, o);
+	return acc;
 }
 
 __global__ __launch_bounds__(256) void k_gather_exact(Descs descs, int k, double target)
@@ -2033,6 +2059,7 @@ __global__ __launch_bounds__(1024) void k_bbpa_finish(Descs descs, int fused, in
 	__shared__ double red[16];
 	__shared__ double s_pairs[2 * kMaxAtoms];
 	__shared__ double s_w[kFinishWeights];
+	__shared__ double s_u0[kMaxVariants];
 	__shared__ unsigned s_count;
 	__shared__ double s_lim;
 	__shared__ int s_redecide;
@@ -2103,8 +2130,9 @@ __global__ __launch_bounds__(1024) void k_bbpa_finish(Descs descs, int fused, in
 		return;
 	// 4. the exact re-decision in the program's frame (xn subjects; a projected lattice's band
 	// entries are placed in it): the block weights, their normalisers, then one band entry per
-	// thread; a frame whose blocks exceed the weights' room sums each entry's block masses
-	// with the whole workgroup instead (gather_block_mass, as k_gather_exact)
+	// thread; a frame whose blocks exceed the weights' room (a projected lattice keeps its
+	// original frame) sums each entry's block masses with one wavefront instead, or with the
+	// whole workgroup beyond kWaveMassBits subjects
 	const int xn = d.gx_n;
 	for (int i = threadIdx.x; i < 2 * k * xn; i += blockDim.x)
 		s_pairs[i] = d.gh_pairs[i];
@@ -2158,6 +2186,24 @@ __global__ __launch_bounds__(1024) void k_bbpa_finish(Descs descs, int fused, in
 			for (int v = 0; v < k; ++v)
 				mv[v] = s_w[((uint32_t)v << xn) + xe] / s_w[(uint32_t)v << xn];
 			top2_insert(ex, gather_objective(mv, k, target), ce);
+		}
+	} else if (xn <= kWaveMassBits) {
+		// one wavefront per normaliser, then per band entry (no workgroup barrier per sum)
+		const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
+		if (wave < k) {
+			const double u = gather_block_mass_wave(d, s_pairs, wave, 0u, lane);
+			if (lane == 0)
+				s_u0[wave] = u;
+		}
+		__syncthreads();
+		for (unsigned i = (unsigned)wave; i < count; i += (unsigned)waves) {
+			const uint32_t ce = d.gband[i].e;
+			const uint32_t xe = frame_deposit(ce, d.gx_keep);
+			double mv[kMaxVariants];
+			for (int v = 0; v < k; ++v)
+				mv[v] = gather_block_mass_wave(d, s_pairs, v, xe, lane) / s_u0[v];
+			if (lane == 0)
+				top2_insert(ex, gather_objective(mv, k, target), ce);
 		}
 	} else {
 		double u0[kMaxVariants];

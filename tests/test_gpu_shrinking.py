@@ -145,6 +145,50 @@ def test_fp32c_shrinking_sessions_pick_the_oracles_experiments(n, k, dil, seed):
     sl.close()
 
 
+@pytest.mark.parametrize("dil", [False, True])
+def test_projected_several_variants_redecide_in_the_original_frame(dil):
+    # 12 subjects x 2 variants with one prior per variant: experiments that the history does
+    # not tell apart tie, so every FP32C selection is a near tie and is re-decided exactly. Two
+    # subjects are summed out first: the 10-subject lattice selects in one pass over its
+    # posterior and re-decides in the original 12-subject frame, whose block weights do not
+    # fit the finish kernel's LDS (one wavefront sums each band entry's block masses).
+    n, k = 12, 2
+    pi = np.array([0.05] * n + [0.08] * n)
+    dtab = orc.dilution_table(n, 0.05, 0.9) if dil else None
+    lat = bgt.Lattice(DL if dil else ND, n, pi, variants=k, precision=F32)
+    post = orc.prior(pi)
+    e0, r0 = 0b000111000111, 1
+    lat.update(e0, r0, bgt.DilutionTable(n, 0.05, 0.9) if dil else None)
+    orc.update_in_place(post, dil, n, k, e0, r0, dtab)
+    classified = _subject_atoms([3, 8], n, k)
+    plan = bgt.make_shrink_plan(n, n, k, classified, 0)
+    post = orc.project_posterior(post, orc.make_shrink_plan(n, n, k, classified, 0))
+    nxt = lat.project(plan)
+    lat.close()
+    lat = nxt
+    cur = n - 2
+    g = workload.SplitMix64(21 + dil)
+    redecided = 0
+    for step in range(5):
+        s = lat.select(bgt.SelectorType.auto_select)
+        e = s.experiment
+        e_ref, objs = oracle_scores(post, cur, k, 2)
+        acc = accurate_scores(post, cur, k, 2)
+        print(f"projected k=2 dil={dil} step {step}: pick {e:#x} oracle {e_ref:#x} near_tie={s.near_tie} "
+              f"redecided={s.redecided} band={s.band}", flush=True)
+        bound = tie_bound(cur * k, k, 2, updates=step + 2)
+        assert_pick(e, objs, e_ref, pick_bound(objs, acc, e, e_ref, bound), f"step {step}")
+        if s.redecided == 1:
+            # the exact objective the device re-decided with, against the oracle's posterior
+            redecided += 1
+            assert abs(s.objective - acc[e]) <= 1e-10, (step, s.objective, acc[e])
+        r = g.below(1 << k)
+        lat.update(e_ref, r, bgt.DilutionTable(cur, 0.05, 0.9) if dil else None)
+        orc.update_in_place(post, dil, cur, k, e_ref, r, dtab)
+    assert redecided >= 3, redecided
+    lat.close()
+
+
 def _shrink_slice(specs):
     # up to three sessions of each (variants, dilution) kind with at most 20 atoms, one of
     # them with 18 or more
