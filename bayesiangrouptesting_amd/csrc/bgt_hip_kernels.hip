@@ -1914,22 +1914,52 @@ __device__ double gather_objective(const double *mv, int k, double target)
 	return score;
 }
 
-// The weight of state S of block v (in the program's frame of d.gx_n subjects): the prior
-// factors of its atoms times the likelihood factors of the history's updates for variant v
-__device__ __forceinline__ double gather_weight(const LatDesc &d, const double *s_pairs, int v, uint32_t s)
+// The exact program's update history for the block-weight sums, staged in LDS by
+// stage_gather (several variants: frames of at most kMaxAtoms / 2 subjects)
+constexpr int kGatherDil = (kMaxAtoms / 2) * (kMaxAtoms / 2 + 1);
+struct GatherHist {
+	uint32_t e[kGatherMaxHist]; // pools (frame subjects)
+	uint32_t r[kGatherMaxHist]; // responses, bit v for variant v
+	double dil[kGatherDil];     // dilution table rows of n + 1 entries
+	int len, n, dilution;
+};
+
+// every thread of the workgroup; ends with a barrier
+__device__ void stage_gather(const LatDesc &d, GatherHist &h)
 {
-	const int n = d.gx_n;
+	const int n = d.gx_n, len = d.gh_len < 0 ? 0 : d.gh_len;
+	for (int i = threadIdx.x; i < len; i += blockDim.x) {
+		h.e[i] = d.gh_e[i];
+		h.r[i] = d.gh_r[i];
+	}
+	const int cells = min(n * (n + 1), kGatherDil);
+	if (d.gh_dil != nullptr)
+		for (int i = threadIdx.x; i < cells; i += blockDim.x)
+			h.dil[i] = d.gh_dil[i];
+	if (threadIdx.x == 0) {
+		h.len = len;
+		h.n = n;
+		h.dilution = d.gh_dil != nullptr ? 1 : 0;
+	}
+	__syncthreads();
+}
+
+// The weight of state S of block v (in the program's frame of h.n subjects): the prior
+// factors of its atoms times the likelihood factors of the history's updates for variant v
+__device__ __forceinline__ double gather_weight(const GatherHist &h, const double *s_pairs, int v, uint32_t s)
+{
+	const int n = h.n;
 	double p = 1.0;
 	for (int i = 0; i < n; ++i)
 		p *= s_pairs[2 * (v * n + i) + ((s >> i) & 1u)];
-	for (int t = 0; t < d.gh_len; ++t) {
-		const uint32_t pool = d.gh_e[t];
+	for (int t = 0; t < h.len; ++t) {
+		const uint32_t pool = h.e[t];
 		const int len = __popc(pool);
 		const int m = __popc(pool & ~s); // positive pooled subjects of block v
-		const bool rb = (d.gh_r[t] >> v) & 1u;
+		const bool rb = (h.r[t] >> v) & 1u;
 		double fv;
-		if (d.gh_dil != nullptr) {
-			const double p1 = len > 0 ? d.gh_dil[(size_t)(len - 1) * (n + 1) + m] : 0.0;
+		if (h.dilution) {
+			const double p1 = len > 0 ? h.dil[(len - 1) * (n + 1) + m] : 0.0;
 			fv = rb ? p1 : 1.0 - p1;
 		} else if (m == 0) {
 			fv = rb ? kNegativeResponse : (1.0 - kNegativeResponse);
@@ -1956,13 +1986,13 @@ __device__ __forceinline__ uint32_t gather_superset(uint32_t e, uint32_t freeb, 
 // U_v(E) of block v (E in the program's frame): the sum of gather_weight over the states
 // S >= E, every thread a strided share of the states in increasing order, then the
 // workgroup tree
-__device__ double gather_block_mass(const LatDesc &d, const double *s_pairs, int v, uint32_t e, double *red)
+__device__ double gather_block_mass(const GatherHist &h, const double *s_pairs, int v, uint32_t e, double *red)
 {
-	const uint32_t freeb = ((1u << d.gx_n) - 1u) & ~e;
+	const uint32_t freeb = ((1u << h.n) - 1u) & ~e;
 	const int f = __popc(freeb);
 	double acc = 0.0;
 	for (uint32_t j = threadIdx.x; j < (1u << f); j += blockDim.x)
-		acc += gather_weight(d, s_pairs, v, gather_superset(e, freeb, f, j));
+		acc += gather_weight(h, s_pairs, v, gather_superset(e, freeb, f, j));
 	const double sum = block_sum(acc, red);
 	__shared__ double s_sum;
 	if (threadIdx.x == 0)
@@ -1978,13 +2008,13 @@ __device__ double gather_block_mass(const LatDesc &d, const double *s_pairs, int
 // ends with the same sum). Frames of at most kWaveMassBits subjects: a lane then adds at
 // most 256 terms in sequence, the share the exact masses' rounding bound allows a thread.
 constexpr int kWaveMassBits = 14;
-__device__ double gather_block_mass_wave(const LatDesc &d, const double *s_pairs, int v, uint32_t e, int lane)
+__device__ double gather_block_mass_wave(const GatherHist &h, const double *s_pairs, int v, uint32_t e, int lane)
 {
-	const uint32_t freeb = ((1u << d.gx_n) - 1u) & ~e;
+	const uint32_t freeb = ((1u << h.n) - 1u) & ~e;
 	const int f = __popc(freeb);
 	double acc = 0.0;
 	for (uint32_t j = (uint32_t)lane; j < (1u << f); j += 64u)
-		acc += gather_weight(d, s_pairs, v, gather_superset(e, freeb, f, j));
+		acc += gather_weight(h, s_pairs, v, gather_superset(e, freeb, f, j));
 	for (int o = 32; o > 0; o >>= 1)
 		acc += __shfl_xor(acc, o);
 	return acc;
@@ -2000,17 +2030,18 @@ __global__ __launch_bounds__(256) void k_gather_exact(Descs descs, int k, double
 		return;
 	__shared__ double s_pairs[2 * kMaxAtoms];
 	__shared__ double red[16];
+	__shared__ GatherHist s_hist;
 	for (int i = threadIdx.x; i < 2 * k * d.gx_n; i += blockDim.x)
 		s_pairs[i] = d.gh_pairs[i];
-	__syncthreads();
+	stage_gather(d, s_hist);
 	double u0[kMaxVariants];
 	for (int v = 0; v < k; ++v)
-		u0[v] = gather_block_mass(d, s_pairs, v, 0u, red);
+		u0[v] = gather_block_mass(s_hist, s_pairs, v, 0u, red);
 	for (unsigned i = blockIdx.x; i < count; i += gridDim.x) {
 		const uint32_t e = frame_deposit(d.gband[i].e, d.gx_keep);
 		double mv[kMaxVariants];
 		for (int v = 0; v < k; ++v)
-			mv[v] = gather_block_mass(d, s_pairs, v, e, red) / u0[v];
+			mv[v] = gather_block_mass(s_hist, s_pairs, v, e, red) / u0[v];
 		const double score = gather_objective(mv, k, target);
 		if (threadIdx.x == 0)
 			d.gexact[i] = score;
@@ -2059,7 +2090,7 @@ __global__ __launch_bounds__(1024) void k_bbpa_finish(Descs descs, int fused, in
 	__shared__ double red[16];
 	__shared__ double s_pairs[2 * kMaxAtoms];
 	__shared__ double s_w[kFinishWeights];
-	__shared__ double s_u0[kMaxVariants];
+	__shared__ GatherHist s_hist;
 	__shared__ unsigned s_count;
 	__shared__ double s_lim;
 	__shared__ int s_redecide;
@@ -2131,12 +2162,13 @@ __global__ __launch_bounds__(1024) void k_bbpa_finish(Descs descs, int fused, in
 	// 4. the exact re-decision in the program's frame (xn subjects; a projected lattice's band
 	// entries are placed in it): the block weights, their normalisers, then one band entry per
 	// thread; a frame whose blocks exceed the weights' room (a projected lattice keeps its
-	// original frame) sums each entry's block masses with one wavefront instead, or with the
-	// whole workgroup beyond kWaveMassBits subjects
+	// original frame) sums the normalisers with the whole workgroup and each entry's block
+	// masses with one wavefront instead (with the whole workgroup beyond kWaveMassBits
+	// subjects)
 	const int xn = d.gx_n;
 	for (int i = threadIdx.x; i < 2 * k * xn; i += blockDim.x)
 		s_pairs[i] = d.gh_pairs[i];
-	__syncthreads();
+	stage_gather(d, s_hist);
 	Top2 ex;
 	top2_init(ex);
 	if ((k << xn) <= kFinishWeights) {
@@ -2144,26 +2176,7 @@ __global__ __launch_bounds__(1024) void k_bbpa_finish(Descs descs, int fused, in
 		for (uint32_t idx = threadIdx.x; idx < (uint32_t)k * xc; idx += blockDim.x) {
 			const int v = (int)(idx >> xn);
 			const uint32_t st = idx & (xc - 1u);
-			double p = 1.0;
-			for (int i = 0; i < xn; ++i)
-				p *= s_pairs[2 * (v * xn + i) + ((st >> i) & 1u)];
-			for (int q = 0; q < d.gh_len; ++q) {
-				const uint32_t pool = d.gh_e[q];
-				const int len = __popc(pool);
-				const int m = __popc(pool & ~st); // positive pooled subjects of block v
-				const bool rb = (d.gh_r[q] >> v) & 1u;
-				double fv;
-				if (d.gh_dil != nullptr) {
-					const double p1 = len > 0 ? d.gh_dil[(size_t)(len - 1) * (xn + 1) + m] : 0.0;
-					fv = rb ? p1 : 1.0 - p1;
-				} else if (m == 0) {
-					fv = rb ? kNegativeResponse : (1.0 - kNegativeResponse);
-				} else {
-					fv = rb ? (1.0 - kNegativeResponse) : kNegativeResponse;
-				}
-				p *= fv;
-			}
-			s_w[idx] = p;
+			s_w[idx] = gather_weight(s_hist, s_pairs, v, st);
 		}
 		__syncthreads();
 		// every block's superset sums in place, one subject bit at a time (each sum a tree of
@@ -2188,33 +2201,31 @@ __global__ __launch_bounds__(1024) void k_bbpa_finish(Descs descs, int fused, in
 			top2_insert(ex, gather_objective(mv, k, target), ce);
 		}
 	} else if (xn <= kWaveMassBits) {
-		// one wavefront per normaliser, then per band entry (no workgroup barrier per sum)
+		// the normalisers by the whole workgroup, then one wavefront per band entry (no
+		// workgroup barrier per entry)
+		double u0[kMaxVariants];
+		for (int v = 0; v < k; ++v)
+			u0[v] = gather_block_mass(s_hist, s_pairs, v, 0u, red);
 		const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
-		if (wave < k) {
-			const double u = gather_block_mass_wave(d, s_pairs, wave, 0u, lane);
-			if (lane == 0)
-				s_u0[wave] = u;
-		}
-		__syncthreads();
 		for (unsigned i = (unsigned)wave; i < count; i += (unsigned)waves) {
 			const uint32_t ce = d.gband[i].e;
 			const uint32_t xe = frame_deposit(ce, d.gx_keep);
 			double mv[kMaxVariants];
 			for (int v = 0; v < k; ++v)
-				mv[v] = gather_block_mass_wave(d, s_pairs, v, xe, lane) / s_u0[v];
+				mv[v] = gather_block_mass_wave(s_hist, s_pairs, v, xe, lane) / u0[v];
 			if (lane == 0)
 				top2_insert(ex, gather_objective(mv, k, target), ce);
 		}
 	} else {
 		double u0[kMaxVariants];
 		for (int v = 0; v < k; ++v)
-			u0[v] = gather_block_mass(d, s_pairs, v, 0u, red);
+			u0[v] = gather_block_mass(s_hist, s_pairs, v, 0u, red);
 		for (unsigned i = 0; i < count; ++i) {
 			const uint32_t ce = d.gband[i].e;
 			const uint32_t xe = frame_deposit(ce, d.gx_keep);
 			double mv[kMaxVariants];
 			for (int v = 0; v < k; ++v)
-				mv[v] = gather_block_mass(d, s_pairs, v, xe, red) / u0[v];
+				mv[v] = gather_block_mass(s_hist, s_pairs, v, xe, red) / u0[v];
 			if (threadIdx.x == 0)
 				top2_insert(ex, gather_objective(mv, k, target), ce);
 		}
